@@ -1,0 +1,74 @@
+/*
+ * rivet_gen.h — seeded synthetic work lists for the reconstruction path.
+ *
+ * The reference has no stored vectors for its kernels: its tests drive
+ * every kernel with a seeded LCG (src/dsp/hevc.rs:1444-1449,
+ * src/dsp/u16_sweep.rs:39-48). This generator uses the same LCG to build
+ * whole pictures of the shapes the benchmark configurations name: a CU
+ * quadtree with symmetric / AMP partitions, quarter-sample vectors, a TU
+ * quadtree with sparse Laplacian coefficients, per-CU QP, per-CTB SAO
+ * (HEVC); macroblocks with 16x16..4x4 partitions, 4x4 / 8x8 transforms and
+ * intra macroblocks (H.264). The generated descriptions are what a parser
+ * would hand the device: they feed the product path and the oracle alike.
+ * No libm: Laplacian magnitudes use fixed-point thresholds.
+ */
+#ifndef RIVET_GEN_H
+#define RIVET_GEN_H
+
+#include "rivet_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rv_gen_hevc_cfg {
+    int32_t width, height, bit_depth;
+    int32_t b_picture;      /* 0: P (list 0 only); 1: B                    */
+    int32_t bi_percent;     /* B: % of PUs bi-predicted (never 8x4 / 4x8)  */
+    int32_t explicit_wp;    /* 1: explicit WP, denominators 6, weights
+                               32..96, offsets +-32 (syntax units)          */
+    int32_t intra_percent;  /* % of 8x8 CUs with 4x4 luma TUs made intra   */
+    int32_t nz_permille;    /* per-coefficient nonzero probability x 1000  */
+    int32_t lap_b;          /* Laplacian scale (8, 12, 16)                 */
+    int32_t qp_min, qp_max;
+    int32_t mv_range;       /* |mv| <= mv_range quarter samples            */
+    int32_t deblock, sao;
+    int32_t poc;
+    int32_t ref_poc[2];     /* POC of the list-0 / list-1 reference        */
+    int32_t ref_slot[2];    /* DPB slot of the list-0 / list-1 reference   */
+} rv_gen_hevc_cfg;
+
+typedef struct rv_gen_h264_cfg {
+    int32_t width, height;  /* coded, multiples of 16                     */
+    int32_t b_picture;
+    int32_t bi_percent;     /* B: % of inter partitions bi-predicted       */
+    int32_t weighting;      /* RV_H264_DEFAULT / RV_H264_IMPLICIT          */
+    int32_t intra_percent;  /* % of macroblocks intra                      */
+    int32_t nz_permille;
+    int32_t lap_b;
+    int32_t qp_min, qp_max;
+    int32_t mv_range;
+    int32_t deblock;
+    int32_t poc;
+    int32_t ref_poc[2];
+    int32_t ref_slot[2];
+} rv_gen_h264_cfg;
+
+typedef struct rv_gen rv_gen;
+
+int rv_gen_hevc(const rv_gen_hevc_cfg* cfg, uint64_t seed, rv_gen** out);
+int rv_gen_h264(const rv_gen_h264_cfg* cfg, uint64_t seed, rv_gen** out);
+/* The generated description (owned by the generator object). */
+const rv_hevc_pic* rv_gen_hevc_pic(const rv_gen* g);
+const rv_h264_pic* rv_gen_h264_pic(const rv_gen* g);
+void rv_gen_free(rv_gen* g);
+/* Uniform random planes, tightly packed, 4:2:0 (1 byte per sample at 8
+ * bits, uint16 above). */
+int rv_gen_planes(int width, int height, int bit_depth, uint64_t seed, void* y, void* cb, void* cr);
+/* Derived sub-seed for (seed, a, b, c): stream / picture / purpose. */
+uint64_t rv_gen_subseed(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

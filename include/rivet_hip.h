@@ -1,0 +1,312 @@
+/*
+ * rivet_hip.h — C ABI of the MI355X (gfx950) picture-reconstruction tier.
+ *
+ * Two levels, both plain C (no C++ or torch types cross this boundary):
+ *
+ *  (1) Work-list formats and the batched picture API. A host decoder (the
+ *      parser, which stays on the CPU) describes one inter picture as
+ *      host-packed lists — coding units, prediction units, transform units
+ *      with their dequantised coefficients, SAO parameters — and the
+ *      library reconstructs it on the device: motion compensation, inverse
+ *      transform + residual add, deblocking, SAO. Reference pictures stay
+ *      resident in device memory (a per-session decoded-picture buffer).
+ *
+ *  (2) The per-entry parity API (rv_hevc_u8_*, rv_hevc_u16_*, rv_h264_u8_*):
+ *      one function per entry of the reference's kernel tables
+ *      (HevcDsp<S>, src/dsp/hevc.rs:165-229; H264Dsp<S>, src/dsp/h264.rs:
+ *      123-170), same argument meaning, host buffers copied to the device
+ *      and back per call. It exists so a table-for-table drop-in and the
+ *      kernel-level parity tests can call single kernels.
+ *
+ * Every entry returns int: 0 (RV_OK) or a negative RV_E_* code. Nothing
+ * aborts or throws across the boundary; rv_last_error() gives a
+ * thread-local message for the last failure on the calling thread. The
+ * reference's kernels are infallible Rust functions that panic on
+ * out-of-range slices; here the same preconditions are checked and refused
+ * with RV_E_RANGE.
+ *
+ * Threading: every entry is re-entrant. Sessions are independent and may
+ * be driven from different host threads; rv_jobs_launch serialises on an
+ * internal lock. Completion is observed with rv_jobs_wait (HIP events).
+ */
+#ifndef RIVET_HIP_H
+#define RIVET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Errors                                                              */
+/* ------------------------------------------------------------------ */
+#define RV_OK 0
+#define RV_E_INVALID (-1)  /* bad argument (null pointer, bad enum)     */
+#define RV_E_NODEVICE (-2) /* no HIP device / rv_init not called        */
+#define RV_E_HIP (-3)      /* a HIP runtime call failed (message kept)  */
+#define RV_E_NOMEM (-4)    /* device or pinned allocation failed        */
+#define RV_E_RANGE (-5)    /* size / depth / fraction outside the range */
+#define RV_E_STATE (-6)    /* call out of order (e.g. job still running) */
+
+/* Limits of the work-list formats. */
+#define RV_MAX_REFS 4 /* reference indices per list (DPB of up to 4 refs) */
+
+/* ------------------------------------------------------------------ */
+/* Shared types                                                        */
+/* ------------------------------------------------------------------ */
+
+/* A motion vector in quarter luma samples (src/hevc/frame.rs:20-26,
+ * src/h264/frame.rs:18-24). */
+typedef struct rv_mv {
+    int16_t x, y;
+} rv_mv;
+
+/* SAO parameters of one CTB for one component (SaoParams,
+ * src/hevc/pic.rs:10-17). offsets are SaoOffsetVal[1..=4], already scaled
+ * by log2OffsetScale. */
+typedef struct rv_sao {
+    uint8_t type_idx;      /* 0 off, 1 band, 2 edge                     */
+    uint8_t band_or_class; /* sao_band_position (band) / SaoEoClass     */
+    int16_t offsets[4];
+} rv_sao;
+
+/* ------------------------------------------------------------------ */
+/* HEVC work lists (Main / Main10, 4:2:0)                               */
+/* ------------------------------------------------------------------ */
+
+/* One coding unit: what the deblocking side data needs (QpY, prediction
+ * mode, coding-block edges; src/hevc/ctu.rs:751-800) plus, for an intra
+ * CU, where its host-supplied prediction block lives. */
+typedef struct rv_hevc_cu {
+    uint16_t x, y;      /* luma top-left                               */
+    uint8_t log2_size;  /* 3..6                                        */
+    int8_t qp_y;        /* QpY of the CU                               */
+    uint8_t intra;      /* 1 = intra CU: prediction supplied by host   */
+    uint8_t part_mode;  /* PartMode (informational)                    */
+    uint32_t pred_off;  /* intra: sample offset into intra_pred: Y (s*s)
+                           then Cb and Cr ((s/2)^2 each), raster          */
+} rv_hevc_cu;
+
+/* One prediction unit (the arguments of predict_block,
+ * src/hevc/inter.rs:555-567). */
+typedef struct rv_hevc_pu {
+    uint16_t x, y;      /* luma top-left                               */
+    uint8_t w, h;       /* luma size, 4..64                            */
+    uint8_t dir;        /* bit 0: list 0 used, bit 1: list 1 used      */
+    uint8_t pad0;
+    int8_t ref_idx[2];  /* per list, -1 = unused                       */
+    uint16_t pad1;
+    rv_mv mv[2];        /* quarter-sample vectors per list             */
+} rv_hevc_pu;
+
+/* One transform unit. Luma TUs are listed whether coded or not (their
+ * edges feed the deblocking filter, src/hevc/ctu.rs:1493-1510); chroma TUs
+ * only when coded. */
+typedef struct rv_hevc_tu {
+    uint16_t x, y;      /* top-left in component samples              */
+    uint8_t comp;       /* 0 Y, 1 Cb, 2 Cr                             */
+    uint8_t log2;       /* 2..5                                        */
+    uint8_t flags;      /* RV_TU_DST | RV_TU_CODED                     */
+    uint8_t max_x;      /* nonzero bounds, inclusive (IdctFn,          */
+    uint8_t max_y;      /*   src/dsp/hevc.rs:18-21)                    */
+    uint8_t pad[3];
+    uint32_t coef_off;  /* coded: offset of the n*n raster in coefs    */
+} rv_hevc_tu;
+
+#define RV_TU_DST 1u   /* 4x4 DST (intra luma 4x4, src/hevc/ctu.rs:2041) */
+#define RV_TU_CODED 2u /* cbf: coefficients present                      */
+
+/* One HEVC P/B picture to reconstruct. Planes are the session's DPB
+ * slots; ref_slot names them per list and reference index. */
+typedef struct rv_hevc_pic {
+    int32_t width, height;  /* coded luma size, multiples of 8         */
+    int32_t bit_depth;      /* 8..12                                   */
+    int32_t log2_ctb;       /* 4..6                                    */
+    int32_t poc;
+    int32_t num_ref[2];
+    int32_t ref_slot[2][RV_MAX_REFS];
+    int32_t ref_poc[2][RV_MAX_REFS];
+    int32_t ref_long_term[2][RV_MAX_REFS];
+    /* Explicit weighted prediction (8.5.3.3.4.3; Weighting::Explicit,
+     * src/hevc/inter.rs:71-84, derived as in src/hevc/ctu.rs:1254-1292). */
+    int32_t weighted;                        /* 0 default, 1 explicit  */
+    int32_t log2_wd[3];                      /* shift1 folded in        */
+    int32_t wp_w[2][RV_MAX_REFS][3];
+    int32_t wp_o[2][RV_MAX_REFS][3];         /* in sample units         */
+    /* Loop filters (SliceFilterParams, src/hevc/pic.rs:19-35). */
+    int32_t deblock;                          /* 1 = on                  */
+    int32_t beta_offset, tc_offset;           /* *_div2 * 2              */
+    int32_t cb_qp_offset, cr_qp_offset;       /* PPS offsets             */
+    int32_t sao_luma, sao_chroma;             /* slice_sao_*_flag        */
+    /* Work lists. */
+    int32_t n_cus, n_pus, n_tus;
+    const rv_hevc_cu* cus;
+    const rv_hevc_pu* pus;
+    const rv_hevc_tu* tus;
+    const int16_t* coefs;
+    uint32_t n_coefs;
+    uint32_t n_intra_pred;
+    const uint16_t* intra_pred;  /* samples, any depth, as uint16       */
+    const rv_sao* sao;           /* 3 per CTB (Y, Cb, Cr), raster CTBs   */
+} rv_hevc_pic;
+
+/* ------------------------------------------------------------------ */
+/* H.264 work lists (High, 8-bit, 4:2:0, frame pictures)                */
+/* ------------------------------------------------------------------ */
+
+/* One macroblock (raster order, every MB of the picture): the facts the
+ * deblocking filter reads (MbInfo, src/h264/mb.rs:322-369). */
+typedef struct rv_h264_mb {
+    uint8_t intra;          /* 1 = intra MB: prediction supplied by host */
+    uint8_t transform_8x8;
+    int8_t qp;              /* QP_Y                                   */
+    int8_t qpc[2];          /* QP_C for Cb, Cr                        */
+    uint8_t pad;
+    uint16_t nz_mask;       /* 4x4 luma blocks with coefficients (raster;
+                               all four of an 8x8 when any)             */
+    uint32_t pred_off;      /* intra: offset into intra_pred (Y 256,
+                               Cb 64, Cr 64)                            */
+} rv_h264_mb;
+
+/* One inter partition (the arguments of predict_partition,
+ * src/h264/inter.rs:172-185). */
+typedef struct rv_h264_part {
+    uint16_t x, y;       /* luma top-left                              */
+    uint8_t w, h;        /* 4, 8 or 16                                 */
+    uint8_t dir;         /* bit 0: list 0, bit 1: list 1               */
+    uint8_t pad0;
+    int8_t ref_idx[2];
+    uint16_t pad1;
+    rv_mv mv[2];
+} rv_h264_part;
+
+/* One coded residual block (the arguments of Residual4Fn / Residual8Fn,
+ * src/dsp/h264.rs:109-116). */
+typedef struct rv_h264_blk {
+    uint16_t x, y;      /* top-left in component samples              */
+    uint8_t comp;       /* 0 Y, 1 Cb, 2 Cr                            */
+    uint8_t size;       /* 4 or 8                                     */
+    uint8_t has_dc;     /* 1: dc replaces position 0 (chroma DC)      */
+    uint8_t pad;
+    int32_t dc;
+    uint32_t coef_off;  /* offset of size*size int32 in coefs         */
+} rv_h264_blk;
+
+#define RV_H264_DEFAULT 0
+#define RV_H264_EXPLICIT 1
+#define RV_H264_IMPLICIT 2
+
+typedef struct rv_h264_pic {
+    int32_t width, height;   /* coded luma size, multiples of 16        */
+    int32_t bit_depth;       /* 8                                       */
+    int32_t poc;
+    int32_t num_ref[2];
+    int32_t ref_slot[2][RV_MAX_REFS];
+    int32_t ref_poc[2][RV_MAX_REFS];
+    int32_t ref_long_term[2][RV_MAX_REFS];
+    int32_t weighting;       /* RV_H264_DEFAULT / EXPLICIT / IMPLICIT     */
+    int32_t log_wd[3];       /* explicit only                            */
+    int32_t wp_w[2][RV_MAX_REFS][3];
+    int32_t wp_o[2][RV_MAX_REFS][3];
+    int32_t deblock;         /* 1 = on (disable_deblocking_filter_idc 0) */
+    int32_t offset_a, offset_b;
+    int32_t n_mbs, n_parts, n_blks;
+    const rv_h264_mb* mbs;
+    const rv_h264_part* parts;
+    const rv_h264_blk* blks;
+    const int32_t* coefs;
+    uint32_t n_coefs;
+    uint32_t n_intra_pred;
+    const uint16_t* intra_pred;
+} rv_h264_pic;
+
+/* ------------------------------------------------------------------ */
+/* Device, sessions, jobs                                              */
+/* ------------------------------------------------------------------ */
+
+/* Select the device (mirrors Cpu::detect_honouring_env, src/dsp/mod.rs:
+ * 200,304: chosen once). Must precede every device call. */
+int rv_init(int device_ordinal);
+/* "hip-gfx950" once initialised (mirrors Cpu::rung, src/dsp/mod.rs:271). */
+const char* rv_tier_name(void);
+/* Thread-local message of the last failure on this thread. */
+const char* rv_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int rv_device_count(void);
+
+#define RV_CODEC_HEVC 0
+#define RV_CODEC_H264 1
+
+typedef struct rv_session rv_session;
+
+/* A decode session: one stream's DPB planes resident on the device plus
+ * one reconstruction scratch picture. dpb_slots reference/output slots of
+ * width x height (luma, 4:2:0) at bit_depth. */
+int rv_session_create(int codec, int width, int height, int bit_depth,
+                      int dpb_slots, rv_session** out);
+int rv_session_destroy(rv_session* s);
+/* Device bytes a session of this shape holds (DPB + scratch), for
+ * admission by footprint. */
+int64_t rv_session_footprint(int codec, int width, int height, int bit_depth,
+                             int dpb_slots);
+/* Host planes are tightly packed (row stride = plane width), 1 byte per
+ * sample at 8 bits, 2 (little-endian uint16) above. */
+int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb,
+                  const void* cr);
+int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr);
+
+/* A prepared picture: the work lists copied into device memory (they are
+ * not referenced after the call returns). The job writes its picture into
+ * DPB slot out_slot of its session when launched; it may be launched many
+ * times (it reads only its references). */
+typedef struct rv_job rv_job;
+int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* pic, int out_slot,
+                    rv_job** out);
+int rv_h264_prepare(rv_session* s, const rv_h264_pic* pic, int out_slot,
+                    rv_job** out);
+int rv_job_free(rv_job* j);
+
+/* A batch: a set of prepared jobs launched together. Creating it uploads
+ * the small per-batch tables (picture descriptors, per-stage item
+ * offsets); it can then be launched many times. */
+typedef struct rv_batch rv_batch;
+int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out);
+int rv_batch_free(rv_batch* b);
+
+#define RV_LAUNCH_PLAIN 0   /* one kernel sequence per picture            */
+#define RV_LAUNCH_BATCHED 1 /* one launch per stage over all pictures     */
+#define RV_LAUNCH_GRAPH 2   /* BATCHED, replayed from a captured hipGraph */
+
+/* Launch a batch asynchronously on the library's stream. */
+int rv_batch_launch(rv_batch* b, int mode);
+/* Block until everything launched so far has completed. */
+int rv_sync(void);
+/* Record a completion marker after everything launched so far; poll or
+ * wait for it (0 = complete, 1 = pending). Markers are recycled by id. */
+int rv_mark(int* id_out);
+int rv_mark_query(int id);
+int rv_mark_wait(int id);
+/* Milliseconds between two markers (both complete). */
+int rv_mark_elapsed(int id_start, int id_end, float* ms);
+
+/* Stage switches (mirror H26X_NO_DEBLOCK / H26X_NO_SAO,
+ * src/hevc/decoder.rs:1247-1248): bit 0 skip deblocking, bit 1 skip SAO.
+ * Also read once from RIVET_HIP_NO_DEBLOCK / RIVET_HIP_NO_SAO. */
+int rv_set_stage_mask(int mask);
+
+/* Algorithmic bytes of a prepared job per kernel class (for roofline
+ * reporting): mc, tx, dbk, sao. */
+int rv_job_bytes(const rv_job* j, int64_t out[4]);
+
+/* sizeof of the work-list structs, for bindings to check their layout:
+ * 0 rv_hevc_cu, 1 rv_hevc_pu, 2 rv_hevc_tu, 3 rv_sao, 4 rv_hevc_pic,
+ * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic;
+ * -1 for an unknown index. */
+int rv_abi_size(int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RIVET_HIP_H */

@@ -1,0 +1,110 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes binding of the CPU oracle.
+
+Loads oracle/liboracle.so (built by oracle/Makefile). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only as the checker / CPU baseline; the product path never
+does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def _load():
+    if not LIB.exists():
+        raise ImportError(f"{LIB} missing: run `make -C oracle`")
+    return C.CDLL(str(LIB))
+
+
+lib = _load()
+P = C.c_void_p
+i16p = C.POINTER(C.c_int16)
+i32p = C.POINTER(C.c_int32)
+u8p = C.POINTER(C.c_uint8)
+_sig = {
+    "orc_hevc_idct": (None, [i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_idst4": (None, [i16p, C.c_int]),
+    "orc_hevc_idct_matrix": (None, [i16p, C.c_int, C.c_int, i16p]),
+    "orc_hevc_add_residual": (None, [C.c_int, P, C.c_int, i16p, C.c_int, C.c_int]),
+    "orc_hevc_copy": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_qpel_h": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_qpel_v": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_qpel_v2": (None, [i16p, i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_epel_h": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_epel_v": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_epel_v2": (None, [i16p, i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_uni": (None, [C.c_int, P, C.c_int, i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_bi": (None, [C.c_int, P, C.c_int, i16p, i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_weighted_uni": (None, [C.c_int, P, C.c_int, i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int]),
+    "orc_hevc_weighted_bi": (None, [C.c_int, P, C.c_int, i16p, i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_mc_fused": (None, [C.c_int, C.c_int, P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i16p,
+                                 C.c_int]),
+    "orc_hevc_sao_band": (None, [C.c_int, P, C.c_int, P, C.c_int, C.c_int, C.c_int, i16p, C.c_int, C.c_int]),
+    "orc_hevc_sao_edge": (None, [C.c_int, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i16p,
+                                 C.c_int]),
+    "orc_hevc_deblock_luma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, i32p, u8p, u8p, C.c_int]),
+    "orc_hevc_deblock_chroma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, u8p, u8p, C.c_int]),
+    "orc_hevc_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+    "orc_h264_qpel": (None, [C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_chroma": (None, [u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_avg": (None, [u8p, C.c_int, u8p, u8p, C.c_int, C.c_int]),
+    "orc_h264_weighted_uni": (None, [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_weighted_bi": (None, [u8p, C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int]),
+    "orc_h264_deblock": (None, [C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int, i16p, C.c_int]),
+    "orc_h264_residual4": (None, [u8p, C.c_int, i32p, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_residual8": (None, [u8p, C.c_int, i32p, C.c_int]),
+    "orc_h264_implicit_pair": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p]),
+    "orc_h264_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+}
+for _n, (_r, _a) in _sig.items():
+    _f = getattr(lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def ptr(a: np.ndarray, t=None):
+    a = np.ascontiguousarray(a)
+    return a.ctypes.data_as(t) if t is not None else C.c_void_p(a.ctypes.data)
+
+
+def reconstruct(codec: int, pic, slots: list, stop_after: int = 0):
+    """Run the oracle on a picture description (ctypes pointer).
+
+    slots[k] = (y, cb, cr) numpy planes of DPB slot k (or None).
+    Returns the three output planes.
+    """
+    c = pic.contents
+    w, h = c.width, c.height
+    dt = np.uint8 if c.bit_depth <= 8 else np.uint16
+    flat = (C.c_void_p * (3 * len(slots)))()
+    keep = []
+    for k, s in enumerate(slots):
+        for j in range(3):
+            if s is None:
+                flat[3 * k + j] = None
+            else:
+                a = np.ascontiguousarray(s[j], dtype=dt)
+                keep.append(a)
+                flat[3 * k + j] = a.ctypes.data
+    outs = [np.zeros((h, w), dt), np.zeros((h // 2, w // 2), dt), np.zeros((h // 2, w // 2), dt)]
+    optr = (C.c_void_p * 3)(*(o.ctypes.data for o in outs))
+    f = lib.orc_hevc_reconstruct if codec == 0 else lib.orc_h264_reconstruct
+    rc = f(C.cast(pic, C.c_void_p), flat, len(slots), optr, stop_after)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {rc}")
+    return outs

@@ -1,0 +1,799 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle for the H.264 reconstruction
+// path (8-bit, 4:2:0, frame pictures). A C++ restatement of the reference's
+// scalar kernel table (src/dsp/h264.rs) and of its drivers for one inter
+// picture (src/h264/inter.rs, the weighting / residual lines of
+// src/h264/recon.rs, the non-MBAFF part of src/h264/deblock.rs). Only
+// tests, smoke() and the CPU baseline of bench.py load it.
+//
+// Pinned by (tests/test_oracle_h264.py): full-sample position == copy,
+// half-sample positions reproduce linear ramps and constants, the
+// quarter positions are the rounded means of their neighbours, the
+// 2-D centre j equals a direct 6x6 sum, the 4x4 / 8x8 inverse transforms
+// against the standard's matrix form (8.5.12.2 / 8.5.13.2) and the DC
+// closed form (a DC of 64 adds 1 everywhere, src/h264/transform.rs:263-276),
+// implicit weights against their closed form, deblocking identity at
+// bS 0 / alpha 0, flat planes, and hand-computed step edges.
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "oracle.h"
+
+namespace {
+
+const int PRED_STRIDE = 16;  // src/dsp/h264.rs:22-24
+
+// Tables 8-15, 8-16, 8-17 of H.264 (src/h264/tables_gen.rs:331-392,
+// src/h264/tables.rs:83).
+const int ALPHA[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,   0,   0,   0,   0,   0,   0,   4,   4,
+                       5,  6,  7,  8,  9,  10, 12, 13, 15, 17,  20,  22,  25,  28,  32,  36,  40,  45,
+                       50, 56, 63, 71, 80, 90, 101, 113, 127, 144, 162, 182, 203, 226, 255, 255};
+const int BETA[52] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  0,  0,  0,  0,  2,  2,  2,  3,  3,  3,  3,  4,  4,  4,
+                      6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13, 14, 14, 15, 15, 16, 16, 17, 17, 18, 18};
+const int TC0[52][3] = {
+    {0, 0, 0},  {0, 0, 0},  {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},    {0, 0, 0},
+    {0, 0, 0},  {0, 0, 0},  {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},    {0, 0, 0},
+    {0, 0, 0},  {0, 0, 1},  {0, 0, 1},   {0, 0, 1},   {0, 0, 1},   {0, 1, 1},   {0, 1, 1},    {1, 1, 1},
+    {1, 1, 1},  {1, 1, 1},  {1, 1, 1},   {1, 1, 2},   {1, 1, 2},   {1, 1, 2},   {1, 1, 2},    {1, 2, 3},
+    {1, 2, 3},  {2, 2, 3},  {2, 2, 4},   {2, 3, 4},   {2, 3, 4},   {3, 3, 5},   {3, 4, 6},    {3, 4, 6},
+    {4, 5, 7},  {4, 5, 8},  {4, 6, 9},   {5, 7, 10},  {6, 8, 11},  {6, 8, 13},  {7, 10, 14},  {8, 11, 16},
+    {9, 12, 18}, {10, 13, 20}, {11, 15, 23}, {13, 17, 25}};
+
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---------------------------------------------------------------------
+// Luma interpolation (src/dsp/h264.rs:395-493)
+// ---------------------------------------------------------------------
+inline int tap6(int a, int b, int c, int d, int e, int f) { return a - 5 * b + 20 * c + 20 * d - 5 * e + f; }
+
+struct QpelSrc {
+    const uint8_t* s;
+    long st;
+    int max;
+    int G(int x, int y) const { return s[(y + 2) * st + x + 2]; }
+    int b1(int x, int yy) const {
+        const uint8_t* r = s + yy * st + x;
+        return tap6(r[0], r[1], r[2], r[3], r[4], r[5]);
+    }
+    int h1(int xx, int y) const {
+        const uint8_t* c = s + y * st + xx;
+        return tap6(c[0], c[st], c[2 * st], c[3 * st], c[4 * st], c[5 * st]);
+    }
+    int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, max); }
+    int hh(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, max); }
+    int j(int x, int y) const {
+        int j1 = tap6(b1(x, y), b1(x, y + 1), b1(x, y + 2), b1(x, y + 3), b1(x, y + 4), b1(x, y + 5));
+        return clampi((j1 + 512) >> 10, 0, max);
+    }
+};
+
+void qpel(int pos, uint8_t* dst, const uint8_t* src, long stride, int w, int h, int max) {
+    QpelSrc q{src, stride, max};
+    int XF = pos & 3, YF = pos >> 2;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int v;
+            switch (YF * 4 + XF) {
+                case 0: v = q.G(x, y); break;
+                case 1: v = (q.G(x, y) + q.b(x, y) + 1) >> 1; break;
+                case 2: v = q.b(x, y); break;
+                case 3: v = (q.G(x + 1, y) + q.b(x, y) + 1) >> 1; break;
+                case 4: v = (q.G(x, y) + q.hh(x, y) + 1) >> 1; break;
+                case 8: v = q.hh(x, y); break;
+                case 12: v = (q.G(x, y + 1) + q.hh(x, y) + 1) >> 1; break;
+                case 10: v = q.j(x, y); break;
+                case 5: v = (q.b(x, y) + q.hh(x, y) + 1) >> 1; break;
+                case 7: v = (q.b(x, y) + q.hh(x + 1, y) + 1) >> 1; break;
+                case 13: v = (q.hh(x, y) + q.b(x, y + 1) + 1) >> 1; break;
+                case 15: v = (q.hh(x + 1, y) + q.b(x, y + 1) + 1) >> 1; break;
+                case 6: v = (q.b(x, y) + q.j(x, y) + 1) >> 1; break;
+                case 14: v = (q.j(x, y) + q.b(x, y + 1) + 1) >> 1; break;
+                case 9: v = (q.hh(x, y) + q.j(x, y) + 1) >> 1; break;
+                default: v = (q.j(x, y) + q.hh(x + 1, y) + 1) >> 1; break;  // 11: (3, 2)
+            }
+            dst[y * PRED_STRIDE + x] = (uint8_t)v;
+        }
+}
+
+// chroma_scalar (src/dsp/h264.rs:499-520).
+void chroma(uint8_t* dst, const uint8_t* src, long stride, int w, int h, int xf, int yf) {
+    int a = (8 - xf) * (8 - yf), b = xf * (8 - yf), c = (8 - xf) * yf, d = xf * yf;
+    for (int y = 0; y < h; ++y) {
+        const uint8_t* r0 = src + y * stride;
+        const uint8_t* r1 = src + (y + 1) * stride;
+        for (int x = 0; x < w; ++x) {
+            int v = a * r0[x] + b * r0[x + 1] + c * r1[x] + d * r1[x + 1];
+            dst[y * PRED_STRIDE + x] = (uint8_t)((v + 32) >> 6);
+        }
+    }
+}
+
+// copy / avg / weighted_uni / weighted_bi (src/dsp/h264.rs:522-597).
+void copy_k(uint8_t* dst, long stride, const uint8_t* src, int w, int h) {
+    for (int y = 0; y < h; ++y) std::memcpy(dst + y * stride, src + y * PRED_STRIDE, w);
+}
+void avg_k(uint8_t* dst, long stride, const uint8_t* a, const uint8_t* b, int w, int h) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            dst[y * stride + x] = (uint8_t)((a[y * PRED_STRIDE + x] + b[y * PRED_STRIDE + x] + 1) >> 1);
+}
+void wuni_k(uint8_t* dst, long stride, const uint8_t* src, int w, int h, int log_wd, int wt, int o, int max) {
+    if (log_wd >= 1) {
+        int round = 1 << (log_wd - 1);
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                dst[y * stride + x] = (uint8_t)clampi(((src[y * PRED_STRIDE + x] * wt + round) >> log_wd) + o, 0, max);
+    } else {
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                dst[y * stride + x] = (uint8_t)clampi(src[y * PRED_STRIDE + x] * wt + o, 0, max);
+    }
+}
+void wbi_k(uint8_t* dst, long stride, const uint8_t* a, const uint8_t* b, int w, int h, int log_wd, int w0, int w1,
+           int o0, int o1, int max) {
+    int off = (o0 + o1 + 1) >> 1, round = 1 << log_wd;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int v = ((a[y * PRED_STRIDE + x] * w0 + b[y * PRED_STRIDE + x] * w1 + round) >> (log_wd + 1)) + off;
+            dst[y * stride + x] = (uint8_t)clampi(v, 0, max);
+        }
+}
+
+// ---------------------------------------------------------------------
+// Deblocking kernels (src/dsp/h264.rs:603-763)
+// ---------------------------------------------------------------------
+
+// deblock_line (:603-656). tc0 < 0 on entry means "bS 4".
+void deblock_line(int* p, int* q, bool strong_bs, int tc0, int alpha, int beta, bool chroma, int max) {
+    int p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+    int q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    if (!(std::abs(p0 - q0) < alpha && std::abs(p1 - p0) < beta && std::abs(q1 - q0) < beta)) return;
+    int ap = std::abs(p2 - p0), aq = std::abs(q2 - q0);
+    if (!strong_bs) {
+        int tc = chroma ? tc0 + 1 : tc0 + (ap < beta) + (aq < beta);
+        int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        p[0] = clampi(p0 + delta, 0, max);
+        q[0] = clampi(q0 - delta, 0, max);
+        if (!chroma) {
+            if (ap < beta) p[1] = p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0);
+            if (aq < beta) q[1] = q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0);
+        }
+    } else {
+        bool strong = std::abs(p0 - q0) < ((alpha >> 2) + 2);
+        if (!chroma && ap < beta && strong) {
+            p[0] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+            p[1] = (p2 + p1 + p0 + q0 + 2) >> 2;
+            p[2] = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
+        } else {
+            p[0] = (2 * p1 + p0 + q1 + 2) >> 2;
+        }
+        if (!chroma && aq < beta && strong) {
+            q[0] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+            q[1] = (p0 + q0 + q1 + q2 + 2) >> 2;
+            q[2] = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
+        } else {
+            q[0] = (2 * q1 + q0 + p1 + 2) >> 2;
+        }
+    }
+}
+
+// deblock_luma_lines<16> (:658-700) / deblock_chroma_scalar (:731-763).
+// step moves along the edge, across across it.
+void deblock_lines(bool chroma_k, uint8_t* d, long off, long step, long across, int alpha, int beta,
+                   const int16_t* tc0, int max) {
+    int lines = chroma_k ? 8 : 16, per = chroma_k ? 2 : 4;
+    for (int i = 0; i < lines; ++i) {
+        int t = 0;
+        if (tc0) {
+            if (tc0[i / per] < 0) continue;
+            t = tc0[i / per];
+        }
+        long base = off + i * step;
+        int p[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+        int n = chroma_k ? 2 : 4;
+        for (int k = 0; k < n; ++k) {
+            p[k] = d[base - (k + 1) * across];
+            q[k] = d[base + k * across];
+        }
+        deblock_line(p, q, tc0 == nullptr, t, alpha, beta, chroma_k, max);
+        int wn = chroma_k ? 1 : 3;
+        for (int k = 0; k < wn; ++k) {
+            d[base - (k + 1) * across] = (uint8_t)p[k];
+            d[base + k * across] = (uint8_t)q[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Inverse transforms, 8-bit: wrapping i16 (src/dsp/h264.rs:766-1066)
+// ---------------------------------------------------------------------
+inline int16_t wa(int16_t a, int16_t b) { return (int16_t)(uint16_t)((uint16_t)a + (uint16_t)b); }
+inline int16_t ws(int16_t a, int16_t b) { return (int16_t)(uint16_t)((uint16_t)a - (uint16_t)b); }
+
+// add_rows_scalar (:803-816).
+void add_rows(uint8_t* dst, long stride, const int16_t* v, int n, int max) {
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            int16_t r = (int16_t)(wa(v[y * n + x], 32) >> 6);
+            uint8_t* p = dst + y * stride + x;
+            *p = (uint8_t)clampi((int)wa((int16_t)*p, r), 0, max);
+        }
+}
+
+// idct4_add_scalar (:774-801).
+void idct4_add(uint8_t* dst, long stride, const int16_t* coeffs, int max) {
+    int16_t d[16], tmp[16];
+    std::memcpy(d, coeffs, sizeof(d));
+    for (int i = 0; i < 4; ++i) {
+        int16_t d0 = d[i * 4], d1 = d[i * 4 + 1], d2 = d[i * 4 + 2], d3 = d[i * 4 + 3];
+        int16_t e0 = wa(d0, d2), e1 = ws(d0, d2), e2 = ws((int16_t)(d1 >> 1), d3), e3 = wa(d1, (int16_t)(d3 >> 1));
+        tmp[i * 4] = wa(e0, e3);
+        tmp[i * 4 + 1] = wa(e1, e2);
+        tmp[i * 4 + 2] = ws(e1, e2);
+        tmp[i * 4 + 3] = ws(e0, e3);
+    }
+    for (int j = 0; j < 4; ++j) {
+        int16_t f0 = tmp[j], f1 = tmp[4 + j], f2 = tmp[8 + j], f3 = tmp[12 + j];
+        int16_t g0 = wa(f0, f2), g1 = ws(f0, f2), g2 = ws((int16_t)(f1 >> 1), f3), g3 = wa(f1, (int16_t)(f3 >> 1));
+        d[j] = wa(g0, g3);
+        d[4 + j] = wa(g1, g2);
+        d[8 + j] = ws(g1, g2);
+        d[12 + j] = ws(g0, g3);
+    }
+    add_rows(dst, stride, d, 4, max);
+}
+
+// idct8_1d_i16 (:913-958).
+void idct8_1d(const int16_t* d, int16_t* o) {
+    int16_t a0 = wa(d[0], d[4]), a4 = ws(d[0], d[4]);
+    int16_t a2 = ws((int16_t)(d[2] >> 1), d[6]), a6 = wa(d[2], (int16_t)(d[6] >> 1));
+    int16_t b0 = wa(a0, a6), b2 = wa(a4, a2), b4 = ws(a4, a2), b6 = ws(a0, a6);
+    int16_t a1 = ws(ws(ws(d[5], d[3]), d[7]), (int16_t)(d[7] >> 1));
+    int16_t a3 = ws(ws(wa(d[1], d[7]), d[3]), (int16_t)(d[3] >> 1));
+    int16_t a5 = wa(wa(ws(d[7], d[1]), d[5]), (int16_t)(d[5] >> 1));
+    int16_t a7 = wa(wa(wa(d[3], d[5]), d[1]), (int16_t)(d[1] >> 1));
+    int16_t b1 = wa(a1, (int16_t)(a7 >> 2)), b7 = ws(a7, (int16_t)(a1 >> 2));
+    int16_t b3 = wa(a3, (int16_t)(a5 >> 2)), b5 = ws((int16_t)(a3 >> 2), a5);
+    o[0] = wa(b0, b7);
+    o[1] = wa(b2, b5);
+    o[2] = wa(b4, b3);
+    o[3] = wa(b6, b1);
+    o[4] = ws(b6, b1);
+    o[5] = ws(b4, b3);
+    o[6] = ws(b2, b5);
+    o[7] = ws(b0, b7);
+}
+
+// idct8_add_scalar (:960-982).
+void idct8_add(uint8_t* dst, long stride, const int16_t* coeffs, int max) {
+    int16_t tmp[64], out[64];
+    for (int i = 0; i < 8; ++i) idct8_1d(coeffs + i * 8, tmp + i * 8);
+    for (int j = 0; j < 8; ++j) {
+        int16_t col[8], o[8];
+        for (int k = 0; k < 8; ++k) col[k] = tmp[k * 8 + j];
+        idct8_1d(col, o);
+        for (int i = 0; i < 8; ++i) out[i * 8 + j] = o[i];
+    }
+    add_rows(dst, stride, out, 8, max);
+}
+
+// dc_add_scalar (:984-996), 8-bit.
+void dc_add(uint8_t* dst, long stride, int dc, int n, int max) {
+    int v = (int)(int16_t)(wa((int16_t)dc, 32) >> 6);
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            uint8_t* p = dst + y * stride + x;
+            *p = (uint8_t)clampi(*p + v, 0, max);
+        }
+}
+
+// residual4_scalar / residual8_scalar with narrow4 / narrow8 (:998-1066).
+void residual4(uint8_t* dst, long stride, const int32_t* coefs, int dc, bool has_dc, int max) {
+    int16_t c[16];
+    int ac = 0;
+    for (int i = 0; i < 16; ++i) {
+        c[i] = (int16_t)clampi(coefs[i], -32768, 32767);
+        if (i != 0) ac |= coefs[i];
+    }
+    if (has_dc) c[0] = (int16_t)dc;
+    if (ac != 0) idct4_add(dst, stride, c, max);
+    else if (c[0] != 0) dc_add(dst, stride, c[0], 4, max);
+}
+void residual8(uint8_t* dst, long stride, const int32_t* coefs, int max) {
+    int16_t c[64];
+    int ac = 0;
+    for (int i = 0; i < 64; ++i) {
+        c[i] = (int16_t)clampi(coefs[i], -32768, 32767);
+        if (i != 0) ac |= coefs[i];
+    }
+    if (ac != 0) idct8_add(dst, stride, c, max);
+    else if (c[0] != 0) dc_add(dst, stride, c[0], 8, max);
+}
+
+// implicit_pair (src/h264/recon.rs:214-233).
+void implicit_pair(int cur_poc, int poc0, int poc1, bool long0, bool long1, int* w0, int* w1) {
+    int tb = clampi(cur_poc - poc0, -128, 127);
+    int td = clampi(poc1 - poc0, -128, 127);
+    if (td == 0 || long0 || long1) {
+        *w0 = 32;
+        *w1 = 32;
+        return;
+    }
+    int tx = (16384 + std::abs(td / 2)) / td;
+    int dsf = clampi((tb * tx + 32) >> 6, -1024, 1023);
+    int ww1 = dsf >> 2;
+    if (ww1 < -64 || ww1 > 128) {
+        *w0 = 32;
+        *w1 = 32;
+        return;
+    }
+    *w0 = 64 - ww1;
+    *w1 = ww1;
+}
+
+// ---------------------------------------------------------------------
+// Frames (src/h264/frame.rs:13-15: luma border 32, chroma 16)
+// ---------------------------------------------------------------------
+struct Plane {
+    std::vector<uint8_t> data;
+    int width = 0, height = 0, pad = 0;
+    long stride = 0;
+    void init(int w, int h, int p) {
+        width = w;
+        height = h;
+        pad = p;
+        stride = w + 2 * p;
+        data.assign((size_t)stride * (h + 2 * p), 0);
+    }
+    long offset(long x, long y) const { return (long)pad * stride + pad + y * stride + x; }
+    void load(const uint8_t* s) {
+        for (int y = 0; y < height; ++y) std::memcpy(&data[offset(0, y)], s + (long)y * width, width);
+    }
+    void store(uint8_t* d) const {
+        for (int y = 0; y < height; ++y) std::memcpy(d + (long)y * width, &data[offset(0, y)], width);
+    }
+    void extend() {
+        for (int y = 0; y < height; ++y) {
+            long row = offset(0, y);
+            for (int i = 1; i <= pad; ++i) {
+                data[row - i] = data[row];
+                data[row + width - 1 + i] = data[row + width - 1];
+            }
+        }
+        long first = offset(-pad, 0), last = offset(-pad, height - 1);
+        for (int i = 1; i <= pad; ++i) {
+            std::memcpy(&data[first - i * stride], &data[first], stride);
+            std::memcpy(&data[last + i * stride], &data[last], stride);
+        }
+    }
+};
+struct Frame {
+    Plane p[3];
+    void init(int w, int h) {
+        p[0].init(w, h, 32);
+        p[1].init(w / 2, h / 2, 16);
+        p[2].init(w / 2, h / 2, 16);
+    }
+};
+
+// interp (src/h264/inter.rs:107-140), frame parity: the window from the
+// padded plane when it fits, else a clamped gather with stride 32.
+template <typename K>
+void interp(const Plane& src, int x0, int y0, int ww, int hh, std::vector<uint8_t>& window, K kernel) {
+    int pad = src.pad;
+    if (x0 >= -pad && y0 >= -pad && x0 + ww <= src.width + pad && y0 + hh <= src.height + pad) {
+        kernel(&src.data[src.offset(x0, y0)], src.stride);
+    } else {
+        for (int y = 0; y < hh; ++y) {
+            int yy = clampi(y0 + y, 0, src.height - 1);
+            for (int x = 0; x < ww; ++x) {
+                int xx = clampi(x0 + x, 0, src.width - 1);
+                window[y * 32 + x] = src.data[src.offset(xx, yy)];
+            }
+        }
+        kernel(window.data(), 32);
+    }
+}
+
+struct BlockMotion {
+    int mv[2];
+    int ref_idx;
+    int ref_id;  // the referenced picture (its DPB slot); -1 none
+};
+
+// motion_bs (src/h264/deblock.rs:97-146), frame macroblocks (mvy_limit 4).
+int motion_bs(const BlockMotion* m0, const BlockMotion* m1, int pa, int qa) {
+    const BlockMotion &p0 = m0[pa], &p1 = m1[pa], &q0 = m0[qa], &q1 = m1[qa];
+    int pn = (p0.ref_idx >= 0) + (p1.ref_idx >= 0), qn = (q0.ref_idx >= 0) + (q1.ref_idx >= 0);
+    if (pn != qn) return 1;
+    auto far = [](const int* a, const int* b) { return std::abs(a[0] - b[0]) >= 4 || std::abs(a[1] - b[1]) >= 4; };
+    auto same = [](const BlockMotion& a, const BlockMotion& b) { return a.ref_id == b.ref_id; };
+    if (pn == 1) {
+        const BlockMotion& pp = p0.ref_idx >= 0 ? p0 : p1;
+        const BlockMotion& qq = q0.ref_idx >= 0 ? q0 : q1;
+        if (!same(pp, qq)) return 1;
+        return far(pp.mv, qq.mv);
+    }
+    bool straight = same(p0, q0) && same(p1, q1);
+    bool crossed = same(p0, q1) && same(p1, q0);
+    if (!straight && !crossed) return 1;
+    if (same(p0, p1)) {
+        bool pa_ = far(p0.mv, q0.mv) || far(p1.mv, q1.mv);
+        bool pb_ = far(p0.mv, q1.mv) || far(p1.mv, q0.mv);
+        return pa_ && pb_;
+    }
+    if (straight) return far(p0.mv, q0.mv) || far(p1.mv, q1.mv);
+    return far(p0.mv, q1.mv) || far(p1.mv, q0.mv);
+}
+
+struct Thr {
+    int alpha, beta;
+    int16_t lut[4];
+};
+
+const uint32_t BS2_SPREAD[16] = {0x00000000, 0x00000002, 0x00000200, 0x00000202, 0x00020000, 0x00020002,
+                                 0x00020200, 0x00020202, 0x02000000, 0x02000002, 0x02000200, 0x02000202,
+                                 0x02020000, 0x02020002, 0x02020200, 0x02020202};
+inline uint32_t gather_col(uint32_t x, int s) {
+    x = (x >> s) & 0x1111;
+    return (x | (x >> 3) | (x >> 6) | (x >> 9)) & 0xF;
+}
+
+// filter_edge (src/h264/deblock.rs:192-213): byte 0 decides bS 4.
+void filter_edge(bool chroma_k, bool vertical, uint8_t* data, long off, long stride, uint32_t bs, const Thr& t,
+                 int max) {
+    long step = vertical ? stride : 1, across = vertical ? 1 : stride;
+    if ((bs & 0xFF) == 4) {
+        deblock_lines(chroma_k, data, off, step, across, t.alpha, t.beta, nullptr, max);
+    } else {
+        int16_t tc[4];
+        for (int k = 0; k < 4; ++k) tc[k] = t.lut[(bs >> (8 * k)) & 3];
+        deblock_lines(chroma_k, data, off, step, across, t.alpha, t.beta, tc, max);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_h264_qpel(int pos, uint8_t* dst, const uint8_t* src, int stride, int w, int h, int max) {
+    qpel(pos, dst, src, stride, w, h, max);
+}
+void orc_h264_chroma(uint8_t* dst, const uint8_t* src, int stride, int w, int h, int xf, int yf) {
+    chroma(dst, src, stride, w, h, xf, yf);
+}
+void orc_h264_avg(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w, int h) {
+    avg_k(dst, stride, a, b, w, h);
+}
+void orc_h264_weighted_uni(uint8_t* dst, int stride, const uint8_t* src, int w, int h, int log_wd, int wt, int o,
+                           int max) {
+    wuni_k(dst, stride, src, w, h, log_wd, wt, o, max);
+}
+void orc_h264_weighted_bi(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w, int h, int log_wd,
+                          int w0, int w1, int o0, int o1, int max) {
+    wbi_k(dst, stride, a, b, w, h, log_wd, w0, w1, o0, o1, max);
+}
+void orc_h264_deblock(int chroma_k, int vertical, uint8_t* data, int off, int stride, int alpha, int beta,
+                      const int16_t* tc0, int max) {
+    long step = vertical ? stride : 1, across = vertical ? 1 : stride;
+    deblock_lines(chroma_k != 0, data, off, step, across, alpha, beta, tc0, max);
+}
+void orc_h264_residual4(uint8_t* dst, int stride, const int32_t* coefs, int dc, int has_dc, int max) {
+    residual4(dst, stride, coefs, dc, has_dc != 0, max);
+}
+void orc_h264_residual8(uint8_t* dst, int stride, const int32_t* coefs, int max) {
+    residual8(dst, stride, coefs, max);
+}
+void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long1, int32_t out[2]) {
+    int w0, w1;
+    implicit_pair(cur_poc, poc0, poc1, long0 != 0, long1 != 0, &w0, &w1);
+    out[0] = w0;
+    out[1] = w1;
+}
+
+int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
+                         int stop_after) {
+    if (!pic || !slot_planes || !out) return RV_E_INVALID;
+    if (pic->bit_depth != 8) return RV_E_RANGE;
+    int W = pic->width, H = pic->height;
+    if (W <= 0 || H <= 0 || (W & 15) || (H & 15)) return RV_E_RANGE;
+    int mbw = W / 16, mbh = H / 16;
+    if (pic->n_mbs != mbw * mbh) return RV_E_RANGE;
+    int max = 255;
+    std::vector<Frame> refs(n_slots);
+    std::vector<bool> loaded(n_slots, false);
+    auto ref_frame = [&](int slot) -> const Frame* {
+        if (slot < 0 || slot >= n_slots || !slot_planes[3 * slot]) return nullptr;
+        if (!loaded[slot]) {
+            refs[slot].init(W, H);
+            for (int c = 0; c < 3; ++c) {
+                refs[slot].p[c].load((const uint8_t*)slot_planes[3 * slot + c]);
+                refs[slot].p[c].extend();
+            }
+            loaded[slot] = true;
+        }
+        return &refs[slot];
+    };
+    Frame cur;
+    cur.init(W, H);
+    int w4 = W / 4, h4 = H / 4;
+    // Per-4x4 motion per list (BlockMotion, src/h264/frame.rs:42-74), raster
+    // per macroblock: index mb * 16 + (by * 4 + bx).
+    BlockMotion none{{0, 0}, -1, -1};
+    std::vector<BlockMotion> motion[2] = {std::vector<BlockMotion>((size_t)mbw * mbh * 16, none),
+                                          std::vector<BlockMotion>((size_t)mbw * mbh * 16, none)};
+    std::vector<int> part_of((size_t)w4 * h4, -1);
+    (void)h4;
+    // Intra macroblocks: host-supplied prediction.
+    for (int a = 0; a < pic->n_mbs; ++a) {
+        const rv_h264_mb& m = pic->mbs[a];
+        if (!m.intra) continue;
+        int mx = (a % mbw) * 16, my = (a / mbw) * 16;
+        const uint16_t* pr = pic->intra_pred + m.pred_off;
+        for (int c = 0; c < 3; ++c) {
+            int n = c ? 8 : 16, px = c ? mx / 2 : mx, py = c ? my / 2 : my;
+            Plane& pl = cur.p[c];
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) pl.data[pl.offset(px + x, py + y)] = (uint8_t)pr[y * n + x];
+            pr += n * n;
+        }
+    }
+    // Inter partitions (predict_partition, src/h264/inter.rs:172-332).
+    std::vector<uint8_t> pred[2][3];
+    for (auto& l : pred)
+        for (auto& c : l) c.assign(16 * PRED_STRIDE, 0);
+    std::vector<uint8_t> window(32 * 32);
+    for (int i = 0; i < pic->n_parts; ++i) {
+        const rv_h264_part& pt = pic->parts[i];
+        int x = pt.x, y = pt.y, w = pt.w, h = pt.h, cw = w / 2, ch = h / 2;
+        const Frame* rf[2] = {nullptr, nullptr};
+        for (int l = 0; l < 2; ++l) {
+            if (!((pt.dir >> l) & 1)) continue;
+            int ri = pt.ref_idx[l];
+            if (ri < 0 || ri >= RV_MAX_REFS || ri >= pic->num_ref[l]) return RV_E_RANGE;
+            rf[l] = ref_frame(pic->ref_slot[l][ri]);
+            if (!rf[l]) return RV_E_RANGE;
+            int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
+            int xi = x + (mvx >> 2), yi = y + (mvy >> 2);
+            int pos = (mvy & 3) * 4 + (mvx & 3);
+            interp(rf[l]->p[0], xi - 2, yi - 2, w + 5, h + 5, window,
+                   [&](const uint8_t* s, long st) { qpel(pos, pred[l][0].data(), s, st, w, h, max); });
+            int xci = x / 2 + (mvx >> 3), yci = y / 2 + (mvy >> 3), xf = mvx & 7, yf = mvy & 7;
+            for (int c = 1; c < 3; ++c)
+                interp(rf[l]->p[c], xci, yci, cw + 1, ch + 1, window,
+                       [&](const uint8_t* s, long st) { chroma(pred[l][c].data(), s, st, cw, ch, xf, yf); });
+            // Motion for the deblocking filter.
+            for (int by = y / 4; by < (y + h) / 4; ++by)
+                for (int bx = x / 4; bx < (x + w) / 4; ++bx) {
+                    int mb = (by / 4) * mbw + bx / 4, blk = (by % 4) * 4 + (bx % 4);
+                    BlockMotion& bm = motion[l][(size_t)mb * 16 + blk];
+                    bm.mv[0] = mvx;
+                    bm.mv[1] = mvy;
+                    bm.ref_idx = ri;
+                    bm.ref_id = pic->ref_slot[l][ri];
+                }
+        }
+        if (!rf[0] && !rf[1]) return RV_E_RANGE;
+        for (int by = y / 4; by < (y + h) / 4; ++by)
+            for (int bx = x / 4; bx < (x + w) / 4; ++bx) part_of[(size_t)by * w4 + bx] = i;
+        // Weighting (src/h264/recon.rs:142-170).
+        bool both = rf[0] && rf[1];
+        int single = rf[0] ? 0 : 1;
+        bool weighted = false;
+        int log_wd[3] = {0, 0, 0}, wt[3][2] = {{1, 1}, {1, 1}, {1, 1}}, o[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+        if (pic->weighting == RV_H264_EXPLICIT) {
+            weighted = true;
+            for (int c = 0; c < 3; ++c) {
+                log_wd[c] = pic->log_wd[c];
+                for (int l = 0; l < 2; ++l)
+                    if (rf[l]) {
+                        wt[c][l] = pic->wp_w[l][pt.ref_idx[l]][c];
+                        o[c][l] = pic->wp_o[l][pt.ref_idx[l]][c];
+                    }
+            }
+        } else if (pic->weighting == RV_H264_IMPLICIT && both) {
+            weighted = true;
+            int w0, w1;
+            implicit_pair(pic->poc, pic->ref_poc[0][pt.ref_idx[0]], pic->ref_poc[1][pt.ref_idx[1]],
+                          pic->ref_long_term[0][pt.ref_idx[0]] != 0, pic->ref_long_term[1][pt.ref_idx[1]] != 0, &w0,
+                          &w1);
+            for (int c = 0; c < 3; ++c) {
+                log_wd[c] = 5;
+                wt[c][0] = w0;
+                wt[c][1] = w1;
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            Plane& pl = cur.p[c];
+            int px = c ? x / 2 : x, py = c ? y / 2 : y, pw = c ? cw : w, ph = c ? ch : h;
+            uint8_t* dst = &pl.data[pl.offset(px, py)];
+            const uint8_t* a = pred[0][c].data();
+            const uint8_t* b = pred[1][c].data();
+            if (!weighted) {
+                if (!both) copy_k(dst, pl.stride, single == 0 ? a : b, pw, ph);
+                else avg_k(dst, pl.stride, a, b, pw, ph);
+            } else if (!both) {
+                wuni_k(dst, pl.stride, single == 0 ? a : b, pw, ph, log_wd[c], wt[c][single], o[c][single], max);
+            } else {
+                wbi_k(dst, pl.stride, a, b, pw, ph, log_wd[c], wt[c][0], wt[c][1], o[c][0], o[c][1], max);
+            }
+        }
+    }
+    // Residual (src/h264/recon.rs:870-926, 1200-1226).
+    if (stop_after != 1) {
+        for (int i = 0; i < pic->n_blks; ++i) {
+            const rv_h264_blk& bk = pic->blks[i];
+            int n = bk.size;
+            if (bk.coef_off + (uint64_t)n * n > pic->n_coefs) return RV_E_RANGE;
+            Plane& pl = cur.p[bk.comp];
+            uint8_t* dst = &pl.data[pl.offset(bk.x, bk.y)];
+            if (n == 8) residual8(dst, pl.stride, pic->coefs + bk.coef_off, max);
+            else residual4(dst, pl.stride, pic->coefs + bk.coef_off, bk.dc, bk.has_dc != 0, max);
+        }
+    }
+    // Deblocking (deblock_mb_rows, src/h264/deblock.rs:567-815), frame
+    // picture, one slice, disable_deblocking_filter_idc 0.
+    if ((stop_after == 0 || stop_after >= 3) && pic->deblock) {
+        Thr thr[52];
+        for (int qa = 0; qa < 52; ++qa) {
+            int ia = clampi(qa + pic->offset_a, 0, 51), ib = clampi(qa + pic->offset_b, 0, 51);
+            thr[qa].alpha = ALPHA[ia];
+            thr[qa].beta = BETA[ib];
+            thr[qa].lut[0] = -1;
+            for (int k = 0; k < 3; ++k) thr[qa].lut[k + 1] = (int16_t)TC0[ia][k];
+        }
+        auto get = [&](int qp_p, int qp_q) -> const Thr& { return thr[clampi((qp_p + qp_q + 1) >> 1, 0, 51)]; };
+        // part_edges (MbInfo, src/h264/mb.rs:350-365) from the partitions.
+        auto part_edges = [&](int mbx, int mby, uint32_t* pe_v, uint32_t* pe_h) {
+            *pe_v = *pe_h = 0;
+            for (int e = 1; e < 4; ++e)
+                for (int k = 0; k < 4; ++k) {
+                    int bx = mbx * 4, by = mby * 4;
+                    if (part_of[(size_t)(by + k) * w4 + bx + e - 1] != part_of[(size_t)(by + k) * w4 + bx + e])
+                        *pe_v |= 1u << (e * 4 + k);
+                    if (part_of[(size_t)(by + e - 1) * w4 + bx + k] != part_of[(size_t)(by + e) * w4 + bx + k])
+                        *pe_h |= 1u << (e * 4 + k);
+                }
+        };
+        const BlockMotion* m0 = motion[0].data();
+        const BlockMotion* m1 = motion[1].data();
+        auto edge_motion = [&](uint32_t rest, uint32_t chg, auto bs_of) -> uint32_t {
+            if (rest == 0) return 0;
+            uint32_t packed = 0, held = 0;
+            bool known = false;
+            for (int k = 0; k < 4; ++k) {
+                if ((chg >> k) & 1) known = false;
+                if (!((rest >> k) & 1)) continue;
+                if (!known) {
+                    held = (uint32_t)bs_of(k);
+                    known = true;
+                }
+                packed |= held << (k * 8);
+            }
+            return packed;
+        };
+        for (int mby = 0; mby < mbh; ++mby)
+            for (int mbx = 0; mbx < mbw; ++mbx) {
+                int addr = mby * mbw + mbx;
+                const rv_h264_mb& m = pic->mbs[addr];
+                int left = mbx > 0 ? addr - 1 : addr, above = mby > 0 ? addr - mbw : addr;
+                bool filter_left = mbx > 0, filter_top = mby > 0;
+                uint32_t bs_v[4] = {0, 0, 0, 0}, bs_h[4] = {0, 0, 0, 0};
+                bool internal_odd = !m.transform_8x8;
+                uint32_t pe_v, pe_h;
+                part_edges(mbx, mby, &pe_v, &pe_h);
+                if (m.intra) {
+                    if (filter_left) bs_v[0] = 0x04040404;
+                    if (filter_top) bs_h[0] = 0x04040404;
+                    for (int e = 1; e < 4; ++e) bs_v[e] = bs_h[e] = 0x03030303;
+                } else {
+                    uint32_t nz = m.nz_mask;
+                    int step = internal_odd ? 1 : 2;
+                    // internal_edge (src/h264/deblock.rs:255-282).
+                    auto internal = [&](uint32_t coef, uint32_t part, int e, bool vertical) -> uint32_t {
+                        uint32_t c = vertical ? gather_col(coef, e - 1) : (coef >> ((e - 1) * 4)) & 0xF;
+                        uint32_t bs = BS2_SPREAD[c];
+                        uint32_t rest = ((part >> (e * 4)) & 0xF) & ~c;
+                        for (int k = 0; k < 4; ++k) {
+                            if (!((rest >> k) & 1)) continue;
+                            int pb = vertical ? k * 4 + e - 1 : (e - 1) * 4 + k;
+                            int qb = vertical ? k * 4 + e : e * 4 + k;
+                            bs |= (uint32_t)motion_bs(m0, m1, addr * 16 + pb, addr * 16 + qb) << (k * 8);
+                        }
+                        return bs;
+                    };
+                    if (nz != 0 || pe_v != 0) {
+                        uint32_t coef_v = nz | (nz >> 1);
+                        for (int e = step; e < 4; e += step) bs_v[e] = internal(coef_v, pe_v, e, true);
+                    }
+                    if (nz != 0 || pe_h != 0) {
+                        uint32_t coef_h = nz | (nz >> 4);
+                        for (int e = step; e < 4; e += step) bs_h[e] = internal(coef_h, pe_h, e, false);
+                    }
+                    if (filter_left) {
+                        const rv_h264_mb& ml = pic->mbs[left];
+                        if (ml.intra) {
+                            bs_v[0] = 0x04040404;
+                        } else {
+                            uint32_t lpv, lph;
+                            part_edges(mbx - 1, mby, &lpv, &lph);
+                            uint32_t c = gather_col(ml.nz_mask, 3) | gather_col(nz, 0);
+                            uint32_t rest = ~c & 0xF;
+                            uint32_t chg = gather_col(lph, 3) | gather_col(pe_h, 0);
+                            bs_v[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
+                                          return motion_bs(m0, m1, left * 16 + k * 4 + 3, addr * 16 + k * 4);
+                                      });
+                        }
+                    }
+                    if (filter_top) {
+                        const rv_h264_mb& ma = pic->mbs[above];
+                        if (ma.intra) {
+                            bs_h[0] = 0x04040404;
+                        } else {
+                            uint32_t apv, aph;
+                            part_edges(mbx, mby - 1, &apv, &aph);
+                            uint32_t c = ((ma.nz_mask >> 12) | nz) & 0xF;
+                            uint32_t rest = ~c & 0xF;
+                            uint32_t chg = gather_col(apv, 3) | gather_col(pe_v, 0);
+                            bs_h[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
+                                          return motion_bs(m0, m1, above * 16 + 12 + k, addr * 16 + k);
+                                      });
+                        }
+                    }
+                    if ((bs_v[0] | bs_v[1] | bs_v[2] | bs_v[3] | bs_h[0] | bs_h[1] | bs_h[2] | bs_h[3]) == 0) continue;
+                }
+                const rv_h264_mb& ml = pic->mbs[left];
+                const rv_h264_mb& ma = pic->mbs[above];
+                // filter_luma_style (src/h264/deblock.rs:352-451).
+                {
+                    int step = internal_odd ? 1 : 2;
+                    uint32_t iv = internal_odd ? (bs_v[1] | bs_v[2] | bs_v[3]) : bs_v[2];
+                    uint32_t ih = internal_odd ? (bs_h[1] | bs_h[2] | bs_h[3]) : bs_h[2];
+                    if (bs_v[0] | bs_h[0] | iv | ih) {
+                        Plane& Y = cur.p[0];
+                        long base = Y.offset(mbx * 16, mby * 16);
+                        const Thr& ti = get(m.qp, m.qp);
+                        if (bs_v[0]) filter_edge(false, true, Y.data.data(), base, Y.stride, bs_v[0], get(ml.qp, m.qp), max);
+                        for (int e = step; e < 4; e += step)
+                            if (bs_v[e]) filter_edge(false, true, Y.data.data(), base + e * 4, Y.stride, bs_v[e], ti, max);
+                        if (bs_h[0]) filter_edge(false, false, Y.data.data(), base, Y.stride, bs_h[0], get(ma.qp, m.qp), max);
+                        for (int e = step; e < 4; e += step)
+                            if (bs_h[e])
+                                filter_edge(false, false, Y.data.data(), base + e * 4 * Y.stride, Y.stride, bs_h[e], ti,
+                                            max);
+                    }
+                }
+                // filter_chroma_style, 4:2:0 (src/h264/deblock.rs:453-560).
+                if (bs_v[0] | bs_h[0] | bs_v[2] | bs_h[2]) {
+                    Plane& Cb = cur.p[1];
+                    Plane& Cr = cur.p[2];
+                    long base = Cb.offset(mbx * 8, mby * 8);
+                    long st = Cb.stride;
+                    const Thr& tib = get(m.qpc[0], m.qpc[0]);
+                    const Thr& tir = get(m.qpc[1], m.qpc[1]);
+                    if (bs_v[0]) {
+                        filter_edge(true, true, Cb.data.data(), base, st, bs_v[0], get(ml.qpc[0], m.qpc[0]), max);
+                        filter_edge(true, true, Cr.data.data(), base, st, bs_v[0], get(ml.qpc[1], m.qpc[1]), max);
+                    }
+                    if (bs_v[2]) {
+                        filter_edge(true, true, Cb.data.data(), base + 4, st, bs_v[2], tib, max);
+                        filter_edge(true, true, Cr.data.data(), base + 4, st, bs_v[2], tir, max);
+                    }
+                    if (bs_h[0]) {
+                        filter_edge(true, false, Cb.data.data(), base, st, bs_h[0], get(ma.qpc[0], m.qpc[0]), max);
+                        filter_edge(true, false, Cr.data.data(), base, st, bs_h[0], get(ma.qpc[1], m.qpc[1]), max);
+                    }
+                    if (bs_h[2]) {
+                        filter_edge(true, false, Cb.data.data(), base + 4 * st, st, bs_h[2], tib, max);
+                        filter_edge(true, false, Cr.data.data(), base + 4 * st, st, bs_h[2], tir, max);
+                    }
+                }
+            }
+    }
+    for (int c = 0; c < 3; ++c) cur.p[c].store((uint8_t*)out[c]);
+    return RV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1128 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle for the HEVC reconstruction
+// path. A restatement, in C++, of the reference's scalar kernel table
+// (src/dsp/hevc.rs) and of the drivers that call it for one inter picture
+// (src/hevc/inter.rs, src/hevc/deblock.rs, src/hevc/sao.rs, the residual
+// and side-data lines of src/hevc/ctu.rs). Only tests, smoke() and the CPU
+// baseline of bench.py load it. Each function cites the lines it follows.
+//
+// Pinned by (tests/test_oracle_hevc.py): the reference's own closed-form
+// tests (butterfly == matrix product, src/dsp/hevc.rs:1511-1536; the
+// add_residual known answers :1495-1507), DC-only closed forms, filter
+// invariants (constant planes, symmetric half-sample filter on ramps,
+// fraction 0 == copy, two-pass == direct 2-D sum at 8 bits), combine
+// identities (bi(a,a) == uni(a), explicit weights 2^d / 0 == default),
+// deblocking (bS 0 / tc 0 identity, flat planes stay flat, hand-computed
+// step edges), SAO (zero offsets identity, band closed form, edge classes
+// on peaks/valleys), and border equivalence (padded plane + gather ==
+// coordinate clamping).
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "oracle.h"
+
+namespace {
+
+// Table 8-12 / 8-13 of H.265 (src/hevc/tables_gen.rs:173-191).
+const int QPEL[4][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {-1, 4, -10, 58, 17, -5, 1, 0},
+    {-1, 4, -11, 40, 40, -11, 4, -1},
+    {0, 1, -5, 17, 58, -10, 4, -1},
+};
+const int EPEL[8][4] = {
+    {0, 0, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
+    {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2},
+};
+// tC' and beta' by Q (Table 8-12 of H.265; src/hevc/tables_gen.rs:195-199).
+const int TC_TABLE[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                          1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4,
+                          4, 4, 5, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 22, 24};
+const int BETA_TABLE[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
+                            0,  0,  0,  6,  7,  8,  9,  10, 11, 12, 13, 14, 15,
+                            16, 17, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38,
+                            40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64};
+
+// The 32-point inverse transform matrix of 8.6.4.2. The standard lists it
+// as a table; every entry is +-c[m] for the 32 distinct values c[m] of its
+// first column, with m = ((2n + 1) k) mod 128 folded onto 0..32 by the
+// cosine symmetries. Built from that rule here (the reference transcribes
+// the table, src/hevc/tables_gen.rs:136-169; tests check both agree on the
+// properties the standard gives the table).
+struct Transform32 {
+    int m[32][32];
+    Transform32() {
+        static const int c[33] = {64, 90, 90, 90, 89, 88, 87, 85, 83, 82, 80,
+                                  78, 75, 73, 70, 67, 64, 61, 57, 54, 50, 46,
+                                  43, 38, 36, 31, 25, 22, 18, 13, 9,  4,  0};
+        for (int k = 0; k < 32; ++k)
+            for (int n = 0; n < 32; ++n) {
+                int a = ((2 * n + 1) * k) % 128;
+                int v;
+                if (a <= 32) v = c[a];
+                else if (a <= 64) v = -c[64 - a];
+                else if (a <= 96) v = -c[a - 64];
+                else v = c[128 - a];
+                m[k][n] = v;
+            }
+    }
+};
+const Transform32 T32;
+
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+inline int sgn(int v) { return (v > 0) - (v < 0); }
+
+template <typename S> inline int rd(const S* p, long i) { return (int)p[i]; }
+
+// ---------------------------------------------------------------------
+// Inverse transforms (src/dsp/hevc.rs:474-566)
+// ---------------------------------------------------------------------
+
+// idct1: the even/odd partial butterfly (src/dsp/hevc.rs:474-497).
+void idct1(const int16_t* src, int stride, int n, int nz, int* out) {
+    if (n == 1) {
+        out[0] = 64 * (int)src[0];
+        return;
+    }
+    int half = n / 2;
+    int e[16];
+    idct1(src, stride * 2, half, (nz + 1) / 2, e);
+    int step = 32 / n;
+    for (int k = 0; k < half; ++k) {
+        int o = 0;
+        for (int j = 1; j < nz; j += 2) o += T32.m[j * step][k] * (int)src[j * stride];
+        out[k] = e[k] + o;
+        out[n - 1 - k] = e[k] - o;
+    }
+}
+
+// idct_scalar (src/dsp/hevc.rs:499-526).
+void idct(int16_t* c, int N, int bd_shift, int max_x, int max_y) {
+    int round2 = 1 << (bd_shift - 1);
+    if (max_x == 0 && max_y == 0) {
+        int v = clampi((c[0] * 64 + 64) >> 7, -32768, 32767);
+        int16_t r = (int16_t)clampi((v * 64 + round2) >> bd_shift, -32768, 32767);
+        for (int i = 0; i < N * N; ++i) c[i] = r;
+        return;
+    }
+    int16_t tmp[32 * 32];
+    std::memset(tmp, 0, sizeof(tmp));
+    int out[32];
+    for (int x = 0; x <= max_x; ++x) {
+        idct1(c + x, N, N, max_y + 1, out);
+        for (int y = 0; y < N; ++y) tmp[y * N + x] = (int16_t)clampi((out[y] + 64) >> 7, -32768, 32767);
+    }
+    for (int y = 0; y < N; ++y) {
+        idct1(tmp + y * N, 1, N, max_x + 1, out);
+        for (int x = 0; x < N; ++x)
+            c[y * N + x] = (int16_t)clampi((out[x] + round2) >> bd_shift, -32768, 32767);
+    }
+}
+
+// idst4_scalar (src/dsp/hevc.rs:528-556).
+void idst4(int16_t* c, int bd_shift) {
+    static const int M[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
+    int round2 = 1 << (bd_shift - 1);
+    int16_t tmp[16];
+    for (int x = 0; x < 4; ++x)
+        for (int i = 0; i < 4; ++i) {
+            int s = 0;
+            for (int j = 0; j < 4; ++j) s += M[j][i] * c[j * 4 + x];
+            tmp[i * 4 + x] = (int16_t)clampi((s + 64) >> 7, -32768, 32767);
+        }
+    for (int y = 0; y < 4; ++y)
+        for (int i = 0; i < 4; ++i) {
+            int s = 0;
+            for (int j = 0; j < 4; ++j) s += M[j][i] * tmp[y * 4 + j];
+            c[y * 4 + i] = (int16_t)clampi((s + round2) >> bd_shift, -32768, 32767);
+        }
+}
+
+// add_residual_scalar (src/dsp/hevc.rs:558-566).
+template <typename S> void add_residual(S* dst, long stride, const int16_t* res, int n, int max) {
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            S* p = dst + y * stride + x;
+            *p = (S)clampi((int)*p + res[y * n + x], 0, max);
+        }
+}
+
+// ---------------------------------------------------------------------
+// Interpolation (src/dsp/hevc.rs:677-882)
+// ---------------------------------------------------------------------
+
+template <typename S> void copy_k(int16_t* dst, const S* src, long ss, int w, int h, int shift) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) dst[y * w + x] = (int16_t)(rd(src, y * ss + x) << shift);
+}
+template <typename S>
+void filt_h(const int* f, int taps, int16_t* dst, const S* src, long ss, int w, int h, int shift) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int acc = 0;
+            for (int k = 0; k < taps; ++k) acc += f[k] * rd(src, y * ss + x + k);
+            dst[y * w + x] = (int16_t)(acc >> shift);
+        }
+}
+template <typename S>
+void filt_v(const int* f, int taps, int16_t* dst, const S* src, long ss, int w, int h, int shift) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int acc = 0;
+            for (int k = 0; k < taps; ++k) acc += f[k] * rd(src, (y + k) * ss + x);
+            dst[y * w + x] = (int16_t)(acc >> shift);
+        }
+}
+// qpel_v2 / epel_v2: vertical taps over 14-bit rows, >> 6 (:737-755, :800-818).
+void filt_v2(const int* f, int taps, int16_t* dst, const int16_t* src, long ss, int w, int h) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int acc = 0;
+            for (int k = 0; k < taps; ++k) acc += f[k] * (int)src[(y + k) * ss + x];
+            dst[y * w + x] = (int16_t)(acc >> 6);
+        }
+}
+
+// interp_scalar (src/dsp/hevc.rs:823-882): src at the window origin.
+template <typename S>
+void interp_k(bool luma, int16_t* pred, int16_t* tmp, const S* src, long ss, int w, int h, int fx, int fy,
+              int bd) {
+    int reach = luma ? 3 : 1, taps = luma ? 8 : 4;
+    int shift1 = std::min(bd, 12) - 8;
+    int shift3 = 14 - bd;
+    const int* fxt = luma ? QPEL[fx] : EPEL[fx];
+    const int* fyt = luma ? QPEL[fy] : EPEL[fy];
+    if (fx == 0 && fy == 0) {
+        copy_k(pred, src + reach * ss + reach, ss, w, h, shift3);
+    } else if (fy == 0) {
+        filt_h(fxt, taps, pred, src + reach * ss, ss, w, h, shift1);
+    } else if (fx == 0) {
+        filt_v(fyt, taps, pred, src + reach, ss, w, h, shift1);
+    } else {
+        filt_h(fxt, taps, tmp, src, ss, w, h + taps - 1, shift1);
+        filt_v2(fyt, taps, pred, tmp, w, w, h);
+    }
+}
+
+// uni_scalar / bi_scalar / weighted_* (src/dsp/hevc.rs:997-1087).
+template <typename S> void uni_k(S* dst, long stride, const int16_t* src, int w, int h, int shift, int max) {
+    int round = shift > 0 ? 1 << (shift - 1) : 0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) dst[y * stride + x] = (S)clampi((src[y * w + x] + round) >> shift, 0, max);
+}
+template <typename S>
+void bi_k(S* dst, long stride, const int16_t* a, const int16_t* b, int w, int h, int shift, int max) {
+    int round = 1 << (shift - 1);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            dst[y * stride + x] = (S)clampi((a[y * w + x] + b[y * w + x] + round) >> shift, 0, max);
+}
+template <typename S>
+void wuni_k(S* dst, long stride, const int16_t* src, int w, int h, int log2_wd, int wt, int o, int max) {
+    if (log2_wd >= 1) {
+        int round = 1 << (log2_wd - 1);
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                dst[y * stride + x] = (S)clampi(((src[y * w + x] * wt + round) >> log2_wd) + o, 0, max);
+    } else {
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) dst[y * stride + x] = (S)clampi(src[y * w + x] * wt + o, 0, max);
+    }
+}
+template <typename S>
+void wbi_k(S* dst, long stride, const int16_t* a, const int16_t* b, int w, int h, int log2_wd, int w0, int w1,
+           int o0, int o1, int max) {
+    int round = (o0 + o1 + 1) << log2_wd;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int v = (a[y * w + x] * w0 + b[y * w + x] * w1 + round) >> (log2_wd + 1);
+            dst[y * stride + x] = (S)clampi(v, 0, max);
+        }
+}
+
+// ---------------------------------------------------------------------
+// SAO kernels (src/dsp/hevc.rs:1181-1224)
+// ---------------------------------------------------------------------
+template <typename S>
+void sao_band_k(S* dst, long ds, const S* src, long ss, int w, int h, const int16_t* table, int shift, int max) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int v = src[y * ss + x];
+            dst[y * ds + x] = (S)clampi(v + table[v >> shift], 0, max);
+        }
+}
+template <typename S>
+void sao_edge_k(S* dst, const S* src, long origin, long stride, int w, int h, long na, long nb, const int16_t* off,
+                int max) {
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            long i = origin + y * stride + x;
+            int v = src[i], a = src[i + na], b = src[i + nb];
+            int e = 2 + sgn(v - a) + sgn(v - b);
+            dst[i] = (S)clampi(v + off[e], 0, max);
+        }
+}
+
+// ---------------------------------------------------------------------
+// Deblocking kernels (src/dsp/hevc.rs:1230-1411)
+// ---------------------------------------------------------------------
+
+// luma_edge_scalar (:1234-1324): one 4-line segment.
+template <typename S>
+void luma_edge(S* d, long pos, long step, long along, int beta, int tc, bool no_p, bool no_q, int max) {
+    auto s = [&](int line, int k) -> int { return d[pos + line * along + k * step]; };
+    int dp0 = std::abs(s(0, -3) - 2 * s(0, -2) + s(0, -1));
+    int dp3 = std::abs(s(3, -3) - 2 * s(3, -2) + s(3, -1));
+    int dq0 = std::abs(s(0, 2) - 2 * s(0, 1) + s(0, 0));
+    int dq3 = std::abs(s(3, 2) - 2 * s(3, 1) + s(3, 0));
+    int dpq0 = dp0 + dq0, dpq3 = dp3 + dq3, dp = dp0 + dp3, dq = dq0 + dq3;
+    int dd = dpq0 + dpq3;
+    if (dd >= beta) return;
+    auto dsam = [&](int line, int dpq) -> bool {
+        return dpq < (beta >> 2) &&
+               std::abs(s(line, -4) - s(line, -1)) + std::abs(s(line, 0) - s(line, 3)) < (beta >> 3) &&
+               std::abs(s(line, -1) - s(line, 0)) < ((5 * tc + 1) >> 1);
+    };
+    bool strong = dsam(0, 2 * dpq0) && dsam(3, 2 * dpq3);
+    bool dep = dp < ((beta + (beta >> 1)) >> 3);
+    bool deq = dq < ((beta + (beta >> 1)) >> 3);
+    for (int line = 0; line < 4; ++line) {
+        long base = pos + line * along;
+        auto at = [&](int k) -> S& { return d[base + k * step]; };
+        int p0 = at(-1), p1 = at(-2), p2 = at(-3), p3 = at(-4);
+        int q0 = at(0), q1 = at(1), q2 = at(2), q3 = at(3);
+        if (strong) {
+            if (!no_p) {
+                at(-1) = (S)clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - 2 * tc, p0 + 2 * tc);
+                at(-2) = (S)clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - 2 * tc, p1 + 2 * tc);
+                at(-3) = (S)clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - 2 * tc, p2 + 2 * tc);
+            }
+            if (!no_q) {
+                at(0) = (S)clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - 2 * tc, q0 + 2 * tc);
+                at(1) = (S)clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - 2 * tc, q1 + 2 * tc);
+                at(2) = (S)clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - 2 * tc, q2 + 2 * tc);
+            }
+        } else {
+            int delta = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
+            if (std::abs(delta) < tc * 10) {
+                delta = clampi(delta, -tc, tc);
+                if (!no_p) at(-1) = (S)clampi(p0 + delta, 0, max);
+                if (!no_q) at(0) = (S)clampi(q0 - delta, 0, max);
+                if (dep && !no_p) {
+                    int dp_ = clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -(tc >> 1), tc >> 1);
+                    at(-2) = (S)clampi(p1 + dp_, 0, max);
+                }
+                if (deq && !no_q) {
+                    int dq_ = clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -(tc >> 1), tc >> 1);
+                    at(1) = (S)clampi(q1 + dq_, 0, max);
+                }
+            }
+        }
+    }
+}
+
+// chroma_edge_scalar (:1326-1353).
+template <typename S>
+void chroma_edge(S* d, long pos, long step, long along, int n, int tc, bool no_p, bool no_q, int max) {
+    for (int line = 0; line < n; ++line) {
+        long base = pos + line * along;
+        int p0 = d[base - step], p1 = d[base - 2 * step], q0 = d[base], q1 = d[base + step];
+        int delta = clampi((((q0 - p0) << 2) + p1 - q1 + 4) >> 3, -tc, tc);
+        if (!no_p) d[base - step] = (S)clampi(p0 + delta, 0, max);
+        if (!no_q) d[base] = (S)clampi(q0 - delta, 0, max);
+    }
+}
+
+// deblock_luma_scalar (:1355-1382): two 4-line segments.
+template <typename S>
+void deblock_luma(S* d, long off, long step, long along, const int* beta, const int* tc, const bool* np,
+                  const bool* nq, int max) {
+    for (int seg = 0; seg < 2; ++seg) {
+        if (beta[seg] == 0 && tc[seg] == 0) continue;
+        luma_edge(d, off + 4 * seg * along, step, along, beta[seg], tc[seg], np[seg], nq[seg], max);
+    }
+}
+// deblock_chroma_scalar (:1384-1411): four 2-line segments.
+template <typename S>
+void deblock_chroma(S* d, long off, long step, long along, const int* tc, const bool* np, const bool* nq,
+                    int max) {
+    for (int seg = 0; seg < 4; ++seg) {
+        if (tc[seg] == 0) continue;
+        chroma_edge(d, off + 2 * seg * along, step, along, 2, tc[seg], np[seg], nq[seg], max);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Picture state (src/hevc/frame.rs:12-17, 133-214; src/hevc/pic.rs)
+// ---------------------------------------------------------------------
+const int LUMA_PAD = 80, CHROMA_PAD = 40;
+
+template <typename S> struct Plane {
+    std::vector<S> data;
+    int width = 0, height = 0, pad = 0;
+    long stride = 0;
+    void init(int w, int h, int p) {
+        width = w;
+        height = h;
+        pad = p;
+        stride = w + 2 * p;
+        data.assign((size_t)stride * (h + 2 * p), 0);
+    }
+    long origin() const { return (long)pad * stride + pad; }
+    long offset(long x, long y) const { return origin() + y * stride + x; }
+    S at_clamped(int x, int y) const {
+        x = clampi(x, 0, width - 1);
+        y = clampi(y, 0, height - 1);
+        return data[offset(x, y)];
+    }
+    void load(const S* src) {
+        for (int y = 0; y < height; ++y) std::memcpy(&data[offset(0, y)], src + (long)y * width, width * sizeof(S));
+    }
+    void store(S* dst) const {
+        for (int y = 0; y < height; ++y) std::memcpy(dst + (long)y * width, &data[offset(0, y)], width * sizeof(S));
+    }
+    // extend_rows / extend_top / extend_bottom (src/hevc/frame.rs:169-214).
+    void extend() {
+        for (int y = 0; y < height; ++y) {
+            long row = origin() + y * stride;
+            S l = data[row], r = data[row + width - 1];
+            for (int i = 1; i <= pad; ++i) {
+                data[row - i] = l;
+                data[row + width - 1 + i] = r;
+            }
+        }
+        long first = origin() - pad;
+        for (int i = 1; i <= pad; ++i) std::memcpy(&data[first - i * stride], &data[first], stride * sizeof(S));
+        long last = origin() + (long)(height - 1) * stride - pad;
+        for (int i = 1; i <= pad; ++i) std::memcpy(&data[last + i * stride], &data[last], stride * sizeof(S));
+    }
+};
+
+template <typename S> struct Frame {
+    Plane<S> p[3];
+    void init(int w, int h) {
+        p[0].init(w, h, LUMA_PAD);
+        p[1].init(w / 2, h / 2, CHROMA_PAD);
+        p[2].init(w / 2, h / 2, CHROMA_PAD);
+    }
+};
+
+// MotionInfo (src/hevc/frame.rs:44-58).
+struct MotionInfo {
+    int mv[2][2];
+    int ref_delta[2];
+    int ref_idx[2];
+    int flags;  // bit 0/1 long-term per list, bit 2 intra
+    bool uses(int l) const { return ref_idx[l] >= 0; }
+    bool long_term(int l) const { return (flags >> l) & 1; }
+};
+
+struct PicInfo {
+    int w4, h4, wc, hc, log2_ctb;
+    std::vector<uint8_t> pred_mode, cbf_luma, edges, filter_exempt;
+    std::vector<int8_t> qp_y;
+    std::vector<MotionInfo> motion;
+    int idx4(int x, int y) const { return (y >> 2) * w4 + (x >> 2); }
+};
+
+// PicInfo::fill4 (src/hevc/pic.rs:374-406).
+template <typename T> void fill4(std::vector<T>& a, int w4, int x, int y, int w, int h, T v) {
+    for (int by = y >> 2; by < (y + h) >> 2; ++by)
+        for (int bx = x >> 2; bx < (x + w) >> 2; ++bx) a[by * w4 + bx] = v;
+}
+
+// chroma_qp (src/hevc/ctu.rs:3072-3084), 4:2:0 (ChromaArrayType 1).
+int chroma_qp420(int qpi) {
+    if (qpi < 30) return qpi;
+    if (qpi >= 43) return qpi - 6;
+    static const int T[14] = {29, 30, 31, 32, 33, 33, 34, 34, 35, 35, 36, 36, 37, 37};
+    return T[qpi - 30];
+}
+
+// ---------------------------------------------------------------------
+// Inter prediction driver (src/hevc/inter.rs:86-195, 553-742)
+// ---------------------------------------------------------------------
+template <typename S> struct Mc {
+    std::vector<int16_t> pred[2][3];
+    std::vector<int16_t> tmp;
+    std::vector<S> window;
+    Mc() {
+        for (auto& l : pred)
+            for (auto& c : l) c.assign(64 * 64, 0);
+        tmp.assign(64 * (64 + 7) + 64 * 64, 0);
+        window.assign(71 * 71, 0);
+    }
+};
+
+// source (src/hevc/inter.rs:92-125): the filter window, from the padded
+// plane when it fits, else gathered with clamping.
+template <typename S>
+const S* source(std::vector<S>& window, const Plane<S>& pl, int xi, int yi, int w, int h, bool luma, long* stride) {
+    int reach = luma ? 3 : 1, taps = luma ? 8 : 4;
+    int x0 = xi - reach, y0 = yi - reach, ww = w + taps - 1, hh = h + taps - 1;
+    int pad = pl.pad;
+    bool inside = x0 >= -pad && y0 >= -pad && x0 + ww <= pl.width + pad && y0 + hh <= pl.height + pad;
+    if (inside) {
+        *stride = pl.stride;
+        return &pl.data[pl.offset(x0, y0)];
+    }
+    for (int yy = 0; yy < hh; ++yy)
+        for (int xx = 0; xx < ww; ++xx) window[yy * ww + xx] = pl.at_clamped(x0 + xx, y0 + yy);
+    *stride = ww;
+    return window.data();
+}
+
+// interp (src/hevc/inter.rs:131-195), through the scalar table's two-pass
+// entries.
+template <typename S>
+void interp_drv(Mc<S>& mc, const Plane<S>& pl, int xi, int yi, int xf, int yf, int w, int h, bool luma, int bd,
+                int16_t* out) {
+    long ss;
+    const S* src = source(mc.window, pl, xi, yi, w, h, luma, &ss);
+    interp_k(luma, out, mc.tmp.data(), src, ss, w, h, xf, yf, bd);
+}
+
+// copy_rows (src/hevc/inter.rs:257-286).
+template <typename S>
+bool copy_rows(const Plane<S>& src, Plane<S>& dst, int sx, int sy, int dx, int dy, int bw, int bh) {
+    int pad = src.pad;
+    if (sx < -pad || sy < -pad || sx + bw > src.width + pad || sy + bh > src.height + pad) return false;
+    for (int r = 0; r < bh; ++r)
+        std::memcpy(&dst.data[dst.offset(dx, dy + r)], &src.data[src.offset(sx, sy + r)], bw * sizeof(S));
+    return true;
+}
+
+struct Weight {
+    bool explicit_;
+    int log2_wd;
+    int w[2], o[2];
+};
+
+// predict_block (src/hevc/inter.rs:555-742) with the scalar table
+// (fused_mc = false, src/dsp/hevc.rs:286): direct copies for whole-sample
+// default uni-prediction, else two-pass interpolation and the combiners.
+template <typename S>
+void predict_block(Mc<S>& mc, Frame<S>& cur, int bd, int x, int y, int w, int h, const Frame<S>* rf[2],
+                   const int mv[2][2], const Weight wt[3]) {
+    int cw = w / 2, ch = h / 2;
+    bool both = rf[0] && rf[1];
+    bool direct[3] = {false, false, false};
+    if (!both) {
+        int l = rf[0] ? 0 : 1;
+        const Frame<S>& r = *rf[l];
+        int mvx = mv[l][0], mvy = mv[l][1];
+        auto plain = [&](int c) { return !wt[c].explicit_; };
+        if ((mvx & 3) == 0 && (mvy & 3) == 0 && plain(0)) {
+            direct[0] = copy_rows(r.p[0], cur.p[0], x + (mvx >> 2), y + (mvy >> 2), x, y, w, h);
+        }
+        // 4:2:0: chroma vector = mv in eighth chroma samples (:581-594).
+        if ((mvx & 7) == 0 && (mvy & 7) == 0 && plain(1) && plain(2)) {
+            int xci = x / 2 + (mvx >> 3), yci = y / 2 + (mvy >> 3);
+            direct[1] = copy_rows(r.p[1], cur.p[1], xci, yci, x / 2, y / 2, cw, ch);
+            direct[2] = copy_rows(r.p[2], cur.p[2], xci, yci, x / 2, y / 2, cw, ch);
+        }
+    }
+    for (int list = 0; list < 2; ++list) {
+        if (!rf[list]) continue;
+        int mvx = mv[list][0], mvy = mv[list][1];
+        for (int c = 0; c < 3; ++c) {
+            if (direct[c]) continue;
+            bool luma = c == 0;
+            if (luma)
+                interp_drv(mc, rf[list]->p[0], x + (mvx >> 2), y + (mvy >> 2), mvx & 3, mvy & 3, w, h, true, bd,
+                           mc.pred[list][0].data());
+            else
+                interp_drv(mc, rf[list]->p[c], x / 2 + (mvx >> 3), y / 2 + (mvy >> 3), mvx & 7, mvy & 7, cw, ch,
+                           false, bd, mc.pred[list][c].data());
+        }
+    }
+    int max = (1 << bd) - 1;
+    for (int c = 0; c < 3; ++c) {
+        if (direct[c]) continue;
+        Plane<S>& pl = cur.p[c];
+        int px = c ? x / 2 : x, py = c ? y / 2 : y, pw = c ? cw : w, ph = c ? ch : h;
+        S* dst = &pl.data[pl.offset(px, py)];
+        const int16_t* a = mc.pred[0][c].data();
+        const int16_t* b = mc.pred[1][c].data();
+        if (!wt[c].explicit_) {
+            if (!both) uni_k(dst, pl.stride, rf[0] ? a : b, pw, ph, 14 - bd, max);
+            else bi_k(dst, pl.stride, a, b, pw, ph, 15 - bd, max);
+        } else {
+            if (!both) {
+                int l = rf[0] ? 0 : 1;
+                wuni_k(dst, pl.stride, l == 0 ? a : b, pw, ph, wt[c].log2_wd, wt[c].w[l], wt[c].o[l], max);
+            } else {
+                wbi_k(dst, pl.stride, a, b, pw, ph, wt[c].log2_wd, wt[c].w[0], wt[c].w[1], wt[c].o[0], wt[c].o[1],
+                      max);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Deblocking driver (src/hevc/deblock.rs), one slice and one tile.
+// ---------------------------------------------------------------------
+
+// motion_bs (src/hevc/deblock.rs:15-60).
+int motion_bs(const MotionInfo& a, const MotionInfo& b) {
+    int na = a.uses(0) + a.uses(1), nb = b.uses(0) + b.uses(1);
+    if (na != nb) return 1;
+    auto far = [](const int* p, const int* q) { return std::abs(p[0] - q[0]) >= 4 || std::abs(p[1] - q[1]) >= 4; };
+    if (na == 1) {
+        int la = a.uses(0) ? 0 : 1, lb = b.uses(0) ? 0 : 1;
+        if (a.ref_delta[la] != b.ref_delta[lb] || a.long_term(la) != b.long_term(lb)) return 1;
+        return far(a.mv[la], b.mv[lb]);
+    }
+    auto key = [](const MotionInfo& m, int l) { return std::make_pair(m.ref_delta[l], m.long_term(l)); };
+    auto pa0 = key(a, 0), pa1 = key(a, 1), pb0 = key(b, 0), pb1 = key(b, 1);
+    bool same_set = (pa0 == pb0 && pa1 == pb1) || (pa0 == pb1 && pa1 == pb0);
+    if (!same_set) return 1;
+    if (pa0 != pa1) {
+        if (pa0 == pb0) return far(a.mv[0], b.mv[0]) || far(a.mv[1], b.mv[1]);
+        return far(a.mv[0], b.mv[1]) || far(a.mv[1], b.mv[0]);
+    }
+    bool c1 = far(a.mv[0], b.mv[0]) || far(a.mv[1], b.mv[1]);
+    bool c2 = far(a.mv[0], b.mv[1]) || far(a.mv[1], b.mv[0]);
+    return c1 && c2;
+}
+
+// boundary_strengths (src/hevc/deblock.rs:62-180) over the whole picture,
+// with every CTB in one slice and one tile (the neighbour checks of
+// :125-137 and :153-165 always pass).
+void boundary_strengths(const PicInfo& info, std::vector<uint8_t>& ver, std::vector<uint8_t>& hor) {
+    int w4 = info.w4;
+    ver.assign((size_t)w4 * info.h4, 0);
+    hor.assign((size_t)w4 * info.h4, 0);
+    for (int by = 0; by < info.h4; ++by) {
+        int y = by * 4;
+        bool hor_row = y > 0 && y % 8 == 0;
+        int bx_start = hor_row ? 0 : 2, bx_step = hor_row ? 1 : 2;
+        for (int bx = bx_start; bx < w4; bx += bx_step) {
+            int x = bx * 4, q = by * w4 + bx;
+            int edges = info.edges[q];
+            bool want_v = x > 0 && x % 8 == 0 && (edges & 3) != 0;
+            bool want_h = hor_row && (edges & 12) != 0;
+            if (!want_v && !want_h) continue;
+            bool intra_q = info.pred_mode[q] == 1;
+            if (want_v) {
+                int p = q - 1;
+                int b = (intra_q || info.pred_mode[p] == 1)                              ? 2
+                        : ((edges & 1) && (info.cbf_luma[p] || info.cbf_luma[q])) ? 1
+                                                                                  : motion_bs(info.motion[p], info.motion[q]);
+                ver[q] = (uint8_t)b;
+            }
+            if (want_h) {
+                int p = q - w4;
+                int b = (intra_q || info.pred_mode[p] == 1)                              ? 2
+                        : ((edges & 4) && (info.cbf_luma[p] || info.cbf_luma[q])) ? 1
+                                                                                  : motion_bs(info.motion[p], info.motion[q]);
+                hor[q] = (uint8_t)b;
+            }
+        }
+    }
+}
+
+struct SliceParams {
+    int beta_offset, tc_offset, cb_qp_offset, cr_qp_offset;
+};
+
+// deblock_rows (src/hevc/deblock.rs:210-515) for rows 0..h4 (the whole
+// picture: the standard's order, which row-by-row application equals).
+template <typename S>
+void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, int bd) {
+    std::vector<uint8_t> ver, hor;
+    boundary_strengths(info, ver, hor);
+    int w4 = info.w4, h4 = info.h4;
+    int max = (1 << bd) - 1, sh_l = bd - 8, sh_c = bd - 8;
+    auto luma_params = [&](int b, int p, int q, int* beta, int* tc, bool* np, bool* nq) {
+        int qp = (info.qp_y[p] + info.qp_y[q] + 1) >> 1;
+        *beta = BETA_TABLE[clampi(qp + sl.beta_offset, 0, 51)] * (1 << sh_l);
+        *tc = TC_TABLE[clampi(qp + 2 * (b - 1) + sl.tc_offset, 0, 53)] * (1 << sh_l);
+        *np = (info.filter_exempt[p] & 5) != 0;
+        *nq = (info.filter_exempt[q] & 5) != 0;
+    };
+    auto chroma_tc = [&](int p, int q, int* out) {
+        int qp_avg = (info.qp_y[p] + info.qp_y[q] + 1) >> 1;
+        for (int c = 0; c < 2; ++c) {
+            int qpi = qp_avg + (c == 0 ? sl.cb_qp_offset : sl.cr_qp_offset);
+            int qpc = qpi < 0 ? qpi : chroma_qp420(qpi);
+            out[c] = TC_TABLE[clampi(qpc + 2 + sl.tc_offset, 0, 53)] * (1 << sh_c);
+        }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<uint8_t>& bs = pass == 0 ? ver : hor;
+        Plane<S>& Y = f.p[0];
+        if (pass == 0) {
+            for (int by = 0; by < h4; by += 2) {
+                bool two = by + 1 < h4;
+                for (int bx = 2; bx < w4; bx += 2) {
+                    int b0 = bs[by * w4 + bx], b1 = two ? bs[(by + 1) * w4 + bx] : 0;
+                    if ((b0 | b1) == 0) continue;
+                    int beta[2] = {0, 0}, tc[2] = {0, 0};
+                    bool np[2] = {false, false}, nq[2] = {false, false};
+                    int bb[2] = {b0, b1};
+                    for (int seg = 0; seg < 2; ++seg)
+                        if (bb[seg]) {
+                            int q = (by + seg) * w4 + bx;
+                            luma_params(bb[seg], q - 1, q, &beta[seg], &tc[seg], &np[seg], &nq[seg]);
+                        }
+                    deblock_luma(Y.data.data(), Y.offset(bx * 4, by * 4), 1, Y.stride, beta, tc, np, nq, max);
+                }
+            }
+        } else {
+            int by0 = 2;
+            for (int by = by0; by < h4; by += 2) {
+                for (int bx = 0; bx < w4; bx += 2) {
+                    int b0 = bs[by * w4 + bx], b1 = bx + 1 < w4 ? bs[by * w4 + bx + 1] : 0;
+                    if ((b0 | b1) == 0) continue;
+                    int beta[2] = {0, 0}, tc[2] = {0, 0};
+                    bool np[2] = {false, false}, nq[2] = {false, false};
+                    int bb[2] = {b0, b1};
+                    for (int seg = 0; seg < 2; ++seg)
+                        if (bb[seg]) {
+                            int q = by * w4 + bx + seg;
+                            luma_params(bb[seg], q - w4, q, &beta[seg], &tc[seg], &np[seg], &nq[seg]);
+                        }
+                    deblock_luma(Y.data.data(), Y.offset(bx * 4, by * 4), Y.stride, 1, beta, tc, np, nq, max);
+                }
+            }
+        }
+        // Chroma, 4:2:0: bS == 2 edges on the 16-luma grid (:409-513).
+        Plane<S>& Cb = f.p[1];
+        Plane<S>& Cr = f.p[2];
+        if (pass == 0) {
+            for (int by = 0; by < h4; by += 4) {
+                int cnt = std::min(h4 - by, 4);
+                for (int bx = 0; bx < w4; bx += 4) {
+                    bool any = false;
+                    int tcs[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+                    bool np[4] = {false, false, false, false}, nq[4] = {false, false, false, false};
+                    for (int seg = 0; seg < cnt; ++seg) {
+                        int q = (by + seg) * w4 + bx;
+                        if (bs[q] != 2) continue;
+                        any = true;
+                        int t[2];
+                        chroma_tc(q - 1, q, t);
+                        tcs[0][seg] = t[0];
+                        tcs[1][seg] = t[1];
+                        np[seg] = (info.filter_exempt[q - 1] & 5) != 0;
+                        nq[seg] = (info.filter_exempt[q] & 5) != 0;
+                    }
+                    if (!any) continue;
+                    int x = bx * 4, y = by * 4;
+                    deblock_chroma(Cb.data.data(), Cb.offset(x / 2, y / 2), 1, Cb.stride, tcs[0], np, nq, max);
+                    deblock_chroma(Cr.data.data(), Cr.offset(x / 2, y / 2), 1, Cr.stride, tcs[1], np, nq, max);
+                }
+            }
+        } else {
+            for (int by = 0; by < h4; ++by) {
+                if ((by * 4) % 16 != 0) continue;
+                for (int bx = 0; bx < w4; bx += 4) {
+                    int cnt = std::min(w4 - bx, 4);
+                    bool any = false;
+                    int tcs[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+                    bool np[4] = {false, false, false, false}, nq[4] = {false, false, false, false};
+                    for (int seg = 0; seg < cnt; ++seg) {
+                        int q = by * w4 + bx + seg;
+                        if (bs[q] != 2) continue;
+                        any = true;
+                        int t[2];
+                        chroma_tc(q - w4, q, t);
+                        tcs[0][seg] = t[0];
+                        tcs[1][seg] = t[1];
+                        np[seg] = (info.filter_exempt[q - w4] & 5) != 0;
+                        nq[seg] = (info.filter_exempt[q] & 5) != 0;
+                    }
+                    if (!any) continue;
+                    int x = bx * 4, y = by * 4;
+                    deblock_chroma(Cb.data.data(), Cb.offset(x / 2, y / 2), Cb.stride, 1, tcs[0], np, nq, max);
+                    deblock_chroma(Cr.data.data(), Cr.offset(x / 2, y / 2), Cr.stride, 1, tcs[1], np, nq, max);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// SAO driver (src/hevc/sao.rs:110-431), one slice and one tile: a
+// neighbouring CTB is usable exactly when it lies inside the picture.
+// ---------------------------------------------------------------------
+template <typename S>
+void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, int y0, int w, int h, int sw,
+             int sh, int bd, const rv_sao& p, const bool nb[8]) {
+    // nb: l, r, t, b, tl, tr, bl, br.
+    int max = (1 << bd) - 1;
+    long stride = src.stride;
+    int pic_w = dst.width, pic_h = dst.height;
+    auto at = [&](int x, int y) { return src.offset(x, y); };
+    if (p.type_idx == 1) {
+        int shift = bd - 5;
+        int16_t table[32] = {0};
+        for (int k = 0; k < 4; ++k) table[(k + p.band_or_class) & 31] = p.offsets[k];
+        long off = at(x0, y0);
+        sao_band_k(&dst.data[off], stride, &src.data[off], stride, w, h, table, shift, max);
+        return;
+    }
+    if (p.type_idx != 2) return;
+    static const int HP[4][2] = {{-1, 1}, {0, 0}, {-1, 1}, {1, -1}};
+    static const int VP[4][2] = {{0, 0}, {-1, 1}, {-1, 1}, {-1, 1}};
+    const int* hp = HP[p.band_or_class & 3];
+    const int* vp = VP[p.band_or_class & 3];
+    int16_t off_tab[5] = {p.offsets[0], p.offsets[1], 0, p.offsets[2], p.offsets[3]};
+    bool L = nb[0], R = nb[1], Tt = nb[2], B = nb[3], TL = nb[4], TR = nb[5], BL = nb[6], BR = nb[7];
+    int xs = x0, xe = x0 + w, ys = y0, ye = y0 + h;
+    bool uses_x = hp[0] != 0, uses_y = vp[0] != 0, diag = uses_x && uses_y;
+    if ((uses_x && !L) || (diag && (!TL || !BL))) xs += 1;
+    if ((uses_x && !R) || (diag && (!TR || !BR))) xe -= 1;
+    if ((uses_y && !Tt) || (diag && (!TL || !TR))) ys += 1;
+    if ((uses_y && !B) || (diag && (!BL || !BR))) ye -= 1;
+    long na = vp[0] * stride + hp[0], nbb = vp[1] * stride + hp[1];
+    bool interior_ok = xs < xe && ys < ye;
+    if (interior_ok) sao_edge_k(dst.data.data(), src.data.data(), at(xs, ys), stride, xe - xs, ye - ys, na, nbb, off_tab, max);
+    auto usable = [&](int x, int y, int xn, int yn) -> bool {
+        if (xn < 0 || yn < 0 || xn >= pic_w || yn >= pic_h) return false;
+        int cx = (xn * sw) >> info.log2_ctb, cy = (yn * sh) >> info.log2_ctb;
+        int cx0 = (x * sw) >> info.log2_ctb, cy0 = (y * sh) >> info.log2_ctb;
+        int dx = cx - cx0, dy = cy - cy0;
+        if (dx == 0 && dy == 0) return true;
+        if (dx == -1 && dy == 0) return L;
+        if (dx == 1 && dy == 0) return R;
+        if (dx == 0 && dy == -1) return Tt;
+        if (dx == 0 && dy == 1) return B;
+        if (dx == -1 && dy == -1) return TL;
+        if (dx == 1 && dy == -1) return TR;
+        if (dx == -1 && dy == 1) return BL;
+        return BR;
+    };
+    auto sample = [&](int x, int y) {
+        int xa = x + hp[0], ya = y + vp[0], xb = x + hp[1], yb = y + vp[1];
+        if (!usable(x, y, xa, ya) || !usable(x, y, xb, yb)) return;
+        long i = at(x, y);
+        int v = src.data[i], a = src.data[i + na], b = src.data[i + nbb];
+        int e = 2 + sgn(v - a) + sgn(v - b);
+        dst.data[i] = (S)clampi(v + off_tab[e], 0, max);
+    };
+    if (interior_ok) {
+        for (int y = y0; y < ys; ++y)
+            for (int x = x0; x < x0 + w; ++x) sample(x, y);
+        for (int y = ys; y < ye; ++y) {
+            for (int x = x0; x < xs; ++x) sample(x, y);
+            for (int x = xe; x < x0 + w; ++x) sample(x, y);
+        }
+        for (int y = ye; y < y0 + h; ++y)
+            for (int x = x0; x < x0 + w; ++x) sample(x, y);
+    } else {
+        for (int y = y0; y < y0 + h; ++y)
+            for (int x = x0; x < x0 + w; ++x) sample(x, y);
+    }
+}
+
+template <typename S>
+void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv_hevc_pic* pic) {
+    int ctb = 1 << info.log2_ctb;
+    int pw = f.p[0].width, ph = f.p[0].height;
+    for (int ry = 0; ry < info.hc; ++ry)
+        for (int rx = 0; rx < info.wc; ++rx) {
+            const rv_sao* params = &pic->sao[3 * (ry * info.wc + rx)];
+            rv_sao eff[3];
+            for (int c = 0; c < 3; ++c) {
+                eff[c] = params[c];
+                bool on = c == 0 ? pic->sao_luma : pic->sao_chroma;
+                if (!on) eff[c].type_idx = 0;
+            }
+            if (eff[0].type_idx == 0 && eff[1].type_idx == 0 && eff[2].type_idx == 0) continue;
+            auto in = [&](int dx, int dy) {
+                int nx = rx + dx, ny = ry + dy;
+                return nx >= 0 && ny >= 0 && nx < info.wc && ny < info.hc;
+            };
+            bool nb[8] = {in(-1, 0), in(1, 0), in(0, -1), in(0, 1), in(-1, -1), in(1, -1), in(-1, 1), in(1, 1)};
+            for (int c = 0; c < 3; ++c) {
+                if (eff[c].type_idx == 0) continue;
+                int sw = c ? 2 : 1, sh = c ? 2 : 1;
+                int x0 = rx * ctb / sw, y0 = ry * ctb / sh;
+                int w = std::min(ctb / sw, pw / sw - x0), h = std::min(ctb / sh, ph / sh - y0);
+                sao_ctb(deb.p[c], f.p[c], info, x0, y0, w, h, sw, sh, pic->bit_depth, eff[c], nb);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------
+// Whole picture
+// ---------------------------------------------------------------------
+template <typename S>
+int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
+                int stop_after) {
+    int W = pic->width, H = pic->height, bd = pic->bit_depth;
+    if (W <= 0 || H <= 0 || (W & 7) || (H & 7)) return RV_E_RANGE;
+    // Reference frames (padded, border extended, as a decoded picture is).
+    std::vector<Frame<S>> refs(n_slots);
+    std::vector<bool> loaded(n_slots, false);
+    auto ref_frame = [&](int slot) -> const Frame<S>* {
+        if (slot < 0 || slot >= n_slots || !slot_planes[3 * slot]) return nullptr;
+        if (!loaded[slot]) {
+            refs[slot].init(W, H);
+            for (int c = 0; c < 3; ++c) {
+                refs[slot].p[c].load((const S*)slot_planes[3 * slot + c]);
+                refs[slot].p[c].extend();
+            }
+            loaded[slot] = true;
+        }
+        return &refs[slot];
+    };
+    Frame<S> cur;
+    cur.init(W, H);
+    PicInfo info;
+    info.log2_ctb = pic->log2_ctb;
+    info.w4 = W / 4;
+    info.h4 = H / 4;
+    int ctb = 1 << pic->log2_ctb;
+    info.wc = (W + ctb - 1) / ctb;
+    info.hc = (H + ctb - 1) / ctb;
+    size_t n4 = (size_t)info.w4 * info.h4;
+    info.pred_mode.assign(n4, 0);
+    info.cbf_luma.assign(n4, 0);
+    info.edges.assign(n4, 0);
+    info.filter_exempt.assign(n4, 0);
+    info.qp_y.assign(n4, 0);
+    MotionInfo none{};
+    none.ref_idx[0] = none.ref_idx[1] = -1;
+    info.motion.assign(n4, none);
+    int max = (1 << bd) - 1;
+
+    // Coding units: QP, prediction mode, CU edges (src/hevc/ctu.rs:751-800);
+    // intra CUs get their host-supplied prediction.
+    for (int i = 0; i < pic->n_cus; ++i) {
+        const rv_hevc_cu& cu = pic->cus[i];
+        int s = 1 << cu.log2_size;
+        fill4(info.qp_y, info.w4, cu.x, cu.y, s, s, (int8_t)cu.qp_y);
+        for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 1;
+        for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 4;
+        if (cu.intra) {
+            fill4(info.pred_mode, info.w4, cu.x, cu.y, s, s, (uint8_t)1);
+            MotionInfo mi = none;
+            mi.flags = 4;
+            fill4(info.motion, info.w4, cu.x, cu.y, s, s, mi);
+            // PU edges of the single intra PB.
+            for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 2;
+            for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 8;
+            const uint16_t* pr = pic->intra_pred + cu.pred_off;
+            for (int c = 0; c < 3; ++c) {
+                int n = c ? s / 2 : s, px = c ? cu.x / 2 : cu.x, py = c ? cu.y / 2 : cu.y;
+                Plane<S>& pl = cur.p[c];
+                for (int yy = 0; yy < n; ++yy)
+                    for (int xx = 0; xx < n; ++xx) pl.data[pl.offset(px + xx, py + yy)] = (S)pr[yy * n + xx];
+                pr += n * n;
+            }
+        }
+    }
+    // Prediction units: motion side data, PU edges, and the prediction.
+    Mc<S> mc;
+    for (int i = 0; i < pic->n_pus; ++i) {
+        const rv_hevc_pu& pu = pic->pus[i];
+        MotionInfo mi = none;
+        const Frame<S>* rf[2] = {nullptr, nullptr};
+        int mv[2][2] = {{0, 0}, {0, 0}};
+        for (int l = 0; l < 2; ++l) {
+            if (!((pu.dir >> l) & 1)) continue;
+            int ri = pu.ref_idx[l];
+            if (ri < 0 || ri >= RV_MAX_REFS || ri >= pic->num_ref[l]) return RV_E_RANGE;
+            rf[l] = ref_frame(pic->ref_slot[l][ri]);
+            if (!rf[l]) return RV_E_RANGE;
+            mv[l][0] = pu.mv[l].x;
+            mv[l][1] = pu.mv[l].y;
+            mi.ref_idx[l] = ri;
+            mi.mv[l][0] = mv[l][0];
+            mi.mv[l][1] = mv[l][1];
+            mi.ref_delta[l] = pic->poc - pic->ref_poc[l][ri];
+            if (pic->ref_long_term[l][ri]) mi.flags |= 1 << l;
+        }
+        if (!rf[0] && !rf[1]) return RV_E_RANGE;
+        fill4(info.motion, info.w4, pu.x, pu.y, pu.w, pu.h, mi);
+        for (int yy = pu.y; yy < std::min(pu.y + pu.h, H); yy += 4) info.edges[info.idx4(pu.x, yy)] |= 2;
+        for (int xx = pu.x; xx < std::min(pu.x + pu.w, W); xx += 4) info.edges[info.idx4(xx, pu.y)] |= 8;
+        // Weighting (src/hevc/ctu.rs:1254-1292).
+        Weight wt[3];
+        for (int c = 0; c < 3; ++c) {
+            wt[c].explicit_ = pic->weighted != 0;
+            wt[c].log2_wd = pic->log2_wd[c];
+            for (int l = 0; l < 2; ++l) {
+                wt[c].w[l] = 1;
+                wt[c].o[l] = 0;
+                if (rf[l]) {
+                    wt[c].w[l] = pic->wp_w[l][pu.ref_idx[l]][c];
+                    wt[c].o[l] = pic->wp_o[l][pu.ref_idx[l]][c];
+                }
+            }
+        }
+        predict_block(mc, cur, bd, pu.x, pu.y, pu.w, pu.h, rf, mv, wt);
+    }
+    // Transform units: edges and cbf (src/hevc/ctu.rs:1493-1510), then the
+    // residual (src/hevc/ctu.rs:2031-2088).
+    std::vector<int16_t> coeffs(32 * 32);
+    for (int i = 0; i < pic->n_tus; ++i) {
+        const rv_hevc_tu& tu = pic->tus[i];
+        int n = 1 << tu.log2;
+        if (tu.comp == 0) {
+            for (int yy = tu.y; yy < std::min(tu.y + n, H); yy += 4) info.edges[info.idx4(tu.x, yy)] |= 1;
+            for (int xx = tu.x; xx < std::min(tu.x + n, W); xx += 4) info.edges[info.idx4(xx, tu.y)] |= 4;
+            if (tu.flags & RV_TU_CODED) fill4(info.cbf_luma, info.w4, tu.x, tu.y, n, n, (uint8_t)1);
+        }
+    }
+    if (stop_after != 1) {
+        for (int i = 0; i < pic->n_tus; ++i) {
+            const rv_hevc_tu& tu = pic->tus[i];
+            if (!(tu.flags & RV_TU_CODED)) continue;
+            int n = 1 << tu.log2;
+            if (tu.coef_off + (uint64_t)n * n > pic->n_coefs) return RV_E_RANGE;
+            std::memcpy(coeffs.data(), pic->coefs + tu.coef_off, n * n * sizeof(int16_t));
+            int bd_shift = 20 - bd;
+            if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
+            else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);
+            Plane<S>& pl = cur.p[tu.comp];
+            add_residual(&pl.data[pl.offset(tu.x, tu.y)], pl.stride, coeffs.data(), n, max);
+        }
+    }
+    if (stop_after == 0 || stop_after >= 3) {
+        if (pic->deblock) {
+            SliceParams sl{pic->beta_offset, pic->tc_offset, pic->cb_qp_offset, pic->cr_qp_offset};
+            deblock_picture(cur, info, sl, bd);
+        }
+    }
+    if ((stop_after == 0 || stop_after >= 4) && (pic->sao_luma || pic->sao_chroma) && pic->sao) {
+        Frame<S> deb = cur;  // the deblocked copy SAO reads (src/hevc/sao.rs:1-4)
+        sao_picture(cur, deb, info, pic);
+    }
+    for (int c = 0; c < 3; ++c) cur.p[c].store((S*)out[c]);
+    return RV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_hevc_idct(int16_t* coeffs, int log2, int bd_shift, int max_x, int max_y) {
+    idct(coeffs, 1 << log2, bd_shift, max_x, max_y);
+}
+void orc_hevc_idst4(int16_t* coeffs, int bd_shift) { idst4(coeffs, bd_shift); }
+
+void orc_hevc_idct_matrix(const int16_t* c, int n, int bd_shift, int16_t* out) {
+    int step = 32 / n;
+    std::vector<int> tmp(n * n);
+    for (int x = 0; x < n; ++x)
+        for (int i = 0; i < n; ++i) {
+            long long s = 0;
+            for (int j = 0; j < n; ++j) s += (long long)T32.m[j * step][i] * c[j * n + x];
+            long long v = (s + 64) >> 7;
+            tmp[i * n + x] = (int)std::min(std::max(v, -32768LL), 32767LL);
+        }
+    long long round = 1LL << (bd_shift - 1);
+    for (int y = 0; y < n; ++y)
+        for (int i = 0; i < n; ++i) {
+            long long s = 0;
+            for (int j = 0; j < n; ++j) s += (long long)T32.m[j * step][i] * tmp[y * n + j];
+            long long v = (s + round) >> bd_shift;
+            out[y * n + i] = (int16_t)std::min(std::max(v, -32768LL), 32767LL);
+        }
+}
+
+#define ORC_DISPATCH(ss, call8, call16) \
+    do {                                \
+        if ((ss) == 1) {                \
+            typedef uint8_t S;          \
+            call8;                      \
+        } else {                        \
+            typedef uint16_t S;         \
+            call16;                     \
+        }                               \
+    } while (0)
+#define ORC_BOTH(ss, call) ORC_DISPATCH(ss, call, call)
+
+void orc_hevc_add_residual(int ss, void* dst, int stride, const int16_t* res, int n, int max) {
+    ORC_BOTH(ss, add_residual((S*)dst, stride, res, n, max));
+}
+void orc_hevc_copy(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int shift) {
+    ORC_BOTH(ss, copy_k(dst, (const S*)src, src_stride, w, h, shift));
+}
+void orc_hevc_qpel_h(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift) {
+    ORC_BOTH(ss, filt_h(QPEL[frac], 8, dst, (const S*)src, src_stride, w, h, shift));
+}
+void orc_hevc_qpel_v(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift) {
+    ORC_BOTH(ss, filt_v(QPEL[frac], 8, dst, (const S*)src, src_stride, w, h, shift));
+}
+void orc_hevc_qpel_v2(int16_t* dst, const int16_t* src, int src_stride, int w, int h, int frac) {
+    filt_v2(QPEL[frac], 8, dst, src, src_stride, w, h);
+}
+void orc_hevc_epel_h(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift) {
+    ORC_BOTH(ss, filt_h(EPEL[frac], 4, dst, (const S*)src, src_stride, w, h, shift));
+}
+void orc_hevc_epel_v(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift) {
+    ORC_BOTH(ss, filt_v(EPEL[frac], 4, dst, (const S*)src, src_stride, w, h, shift));
+}
+void orc_hevc_epel_v2(int16_t* dst, const int16_t* src, int src_stride, int w, int h, int frac) {
+    filt_v2(EPEL[frac], 4, dst, src, src_stride, w, h);
+}
+void orc_hevc_uni(int ss, void* dst, int stride, const int16_t* src, int w, int h, int shift, int max) {
+    ORC_BOTH(ss, uni_k((S*)dst, stride, src, w, h, shift, max));
+}
+void orc_hevc_bi(int ss, void* dst, int stride, const int16_t* a, const int16_t* b, int w, int h, int shift,
+                 int max) {
+    ORC_BOTH(ss, bi_k((S*)dst, stride, a, b, w, h, shift, max));
+}
+void orc_hevc_weighted_uni(int ss, void* dst, int stride, const int16_t* src, int w, int h, int log2_wd, int wt,
+                           int o, int max) {
+    ORC_BOTH(ss, wuni_k((S*)dst, stride, src, w, h, log2_wd, wt, o, max));
+}
+void orc_hevc_weighted_bi(int ss, void* dst, int stride, const int16_t* a, const int16_t* b, int w, int h,
+                          int log2_wd, int w0, int w1, int o0, int o1, int max) {
+    ORC_BOTH(ss, wbi_k((S*)dst, stride, a, b, w, h, log2_wd, w0, w1, o0, o1, max));
+}
+// qpel_uni / epel_uni / qpel_bi / epel_bi (src/dsp/hevc.rs:887-991).
+void orc_hevc_mc_fused(int ss, int luma, void* dst, int dst_stride, const void* src, int src_stride, int w, int h,
+                       int fx, int fy, const int16_t* other, int bit_depth) {
+    std::vector<int16_t> tmp(64 * 71), pred(64 * 64);
+    int max = (1 << bit_depth) - 1;
+    ORC_BOTH(ss, {
+        interp_k(luma != 0, pred.data(), tmp.data(), (const S*)src, src_stride, w, h, fx, fy, bit_depth);
+        if (other) bi_k((S*)dst, dst_stride, other, pred.data(), w, h, 15 - bit_depth, max);
+        else uni_k((S*)dst, dst_stride, pred.data(), w, h, 14 - bit_depth, max);
+    });
+}
+void orc_hevc_sao_band(int ss, void* dst, int dst_stride, const void* src, int src_stride, int w, int h,
+                       const int16_t* table32, int shift, int max) {
+    ORC_BOTH(ss, sao_band_k((S*)dst, dst_stride, (const S*)src, src_stride, w, h, table32, shift, max));
+}
+void orc_hevc_sao_edge(int ss, void* dst, const void* src, int origin, int stride, int w, int h, int na, int nb,
+                       const int16_t* off5, int max) {
+    ORC_BOTH(ss, sao_edge_k((S*)dst, (const S*)src, origin, stride, w, h, na, nb, off5, max));
+}
+void orc_hevc_deblock_luma(int ss, int vertical, void* data, int off, int stride, const int32_t beta[2],
+                           const int32_t tc[2], const uint8_t no_p[2], const uint8_t no_q[2], int max) {
+    int b[2] = {beta[0], beta[1]}, t[2] = {tc[0], tc[1]};
+    bool np[2] = {no_p[0] != 0, no_p[1] != 0}, nq[2] = {no_q[0] != 0, no_q[1] != 0};
+    long step = vertical ? 1 : stride, along = vertical ? stride : 1;
+    ORC_BOTH(ss, deblock_luma((S*)data, off, step, along, b, t, np, nq, max));
+}
+void orc_hevc_deblock_chroma(int ss, int vertical, void* data, int off, int stride, const int32_t tc[4],
+                             const uint8_t no_p[4], const uint8_t no_q[4], int max) {
+    int t[4] = {tc[0], tc[1], tc[2], tc[3]};
+    bool np[4], nq[4];
+    for (int i = 0; i < 4; ++i) {
+        np[i] = no_p[i] != 0;
+        nq[i] = no_q[i] != 0;
+    }
+    long step = vertical ? 1 : stride, along = vertical ? stride : 1;
+    ORC_BOTH(ss, deblock_chroma((S*)data, off, step, along, t, np, nq, max));
+}
+
+int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
+                         int stop_after) {
+    if (!pic || !slot_planes || !out) return RV_E_INVALID;
+    if (pic->bit_depth == 8) return reconstruct<uint8_t>(pic, slot_planes, n_slots, out, stop_after);
+    if (pic->bit_depth > 8 && pic->bit_depth <= 12)
+        return reconstruct<uint16_t>(pic, slot_planes, n_slots, out, stop_after);
+    return RV_E_RANGE;
+}
+
+}  // extern "C"

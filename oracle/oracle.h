@@ -1,0 +1,93 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * The CPU oracle: a C++ restatement of the reference's scalar dsp kernels
+ * (src/dsp/hevc.rs, src/dsp/h264.rs) and of the drivers that decide their
+ * call geometry (src/hevc/inter.rs, deblock.rs, sao.rs; src/h264/inter.rs,
+ * recon.rs, deblock.rs). Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker: the product path
+ * (librivet_hip.so) never links, loads or calls it.
+ *
+ * Sample buffers are passed as void* with a sample size: 1 byte (8-bit
+ * pictures, the reference's u8 tables) or 2 bytes (uint16, deeper).
+ */
+#ifndef RIVET_ORACLE_H
+#define RIVET_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/rivet_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- HEVC dsp entries (HevcDsp<S>, src/dsp/hevc.rs:165-229) ---------- */
+void orc_hevc_idct(int16_t* coeffs, int log2, int bd_shift, int max_x, int max_y);
+void orc_hevc_idst4(int16_t* coeffs, int bd_shift);
+/* The plain matrix product with the two clips (src/dsp/hevc.rs:1417-1442),
+ * an independent statement the butterfly must equal. */
+void orc_hevc_idct_matrix(const int16_t* coeffs, int n, int bd_shift, int16_t* out);
+void orc_hevc_add_residual(int ss, void* dst, int stride, const int16_t* res, int n, int max);
+void orc_hevc_copy(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int shift);
+void orc_hevc_qpel_h(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift);
+void orc_hevc_qpel_v(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift);
+void orc_hevc_qpel_v2(int16_t* dst, const int16_t* src, int src_stride, int w, int h, int frac);
+void orc_hevc_epel_h(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift);
+void orc_hevc_epel_v(int ss, int16_t* dst, const void* src, int src_stride, int w, int h, int frac, int shift);
+void orc_hevc_epel_v2(int16_t* dst, const int16_t* src, int src_stride, int w, int h, int frac);
+void orc_hevc_uni(int ss, void* dst, int stride, const int16_t* src, int w, int h, int shift, int max);
+void orc_hevc_bi(int ss, void* dst, int stride, const int16_t* a, const int16_t* b, int w, int h,
+                 int shift, int max);
+void orc_hevc_weighted_uni(int ss, void* dst, int stride, const int16_t* src, int w, int h,
+                           int log2_wd, int wt, int o, int max);
+void orc_hevc_weighted_bi(int ss, void* dst, int stride, const int16_t* a, const int16_t* b, int w,
+                          int h, int log2_wd, int w0, int w1, int o0, int o1, int max);
+/* Fused interpolation + default combine (src/dsp/hevc.rs:887-991); luma
+ * selects the 8-tap (qpel) or 4-tap (epel) kernel. other == NULL: uni. */
+void orc_hevc_mc_fused(int ss, int luma, void* dst, int dst_stride, const void* src,
+                       int src_stride, int w, int h, int fx, int fy, const int16_t* other,
+                       int bit_depth);
+void orc_hevc_sao_band(int ss, void* dst, int dst_stride, const void* src, int src_stride, int w,
+                       int h, const int16_t* table32, int shift, int max);
+void orc_hevc_sao_edge(int ss, void* dst, const void* src, int origin, int stride, int w, int h,
+                       int na, int nb, const int16_t* off5, int max);
+/* vertical = 1: _v (lines stride apart, p/q 1 apart); 0: _h. */
+void orc_hevc_deblock_luma(int ss, int vertical, void* data, int off, int stride,
+                           const int32_t beta[2], const int32_t tc[2], const uint8_t no_p[2],
+                           const uint8_t no_q[2], int max);
+void orc_hevc_deblock_chroma(int ss, int vertical, void* data, int off, int stride,
+                             const int32_t tc[4], const uint8_t no_p[4], const uint8_t no_q[4],
+                             int max);
+
+/* ---- HEVC whole picture ------------------------------------------------
+ * slot_planes[3 * slot + c] = tightly packed plane c of DPB slot `slot`.
+ * out[3] receives the reconstructed picture (tightly packed). stop_after:
+ * 1 = prediction only, 2 = + residual, 3 = + deblocking, 0/4 = + SAO. */
+int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots,
+                         void* const out[3], int stop_after);
+
+/* ---- H.264 dsp entries (H264Dsp<S>, src/dsp/h264.rs:123-170), 8-bit -- */
+void orc_h264_qpel(int pos, uint8_t* dst, const uint8_t* src, int stride, int w, int h, int max);
+void orc_h264_chroma(uint8_t* dst, const uint8_t* src, int stride, int w, int h, int xf, int yf);
+void orc_h264_avg(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w, int h);
+void orc_h264_weighted_uni(uint8_t* dst, int stride, const uint8_t* src, int w, int h, int log_wd,
+                           int wt, int o, int max);
+void orc_h264_weighted_bi(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w,
+                          int h, int log_wd, int w0, int w1, int o0, int o1, int max);
+/* tc0 == NULL: the bS 4 (intra) kernel. chroma selects the 8-line 4:2:0
+ * chroma kernel, else the 16-line luma one. */
+void orc_h264_deblock(int chroma, int vertical, uint8_t* data, int off, int stride, int alpha,
+                      int beta, const int16_t* tc0, int max);
+void orc_h264_residual4(uint8_t* dst, int stride, const int32_t* coefs, int dc, int has_dc,
+                        int max);
+void orc_h264_residual8(uint8_t* dst, int stride, const int32_t* coefs, int max);
+void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long1, int32_t out[2]);
+
+int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots,
+                         void* const out[3], int stop_after);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,707 @@
+// Host runtime of the HIP tier: device selection, sessions (DPB planes
+// resident in device memory), job preparation (work lists validated and
+// copied to the device), batches (per-stage item tables, hipGraph replay)
+// and completion markers. Every extern "C" entry catches everything and
+// returns an RV_E_* code; messages go to a thread-local buffer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rivet_hip.h"
+#include "rv_device.h"
+#include "rv_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+std::mutex g_mu;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+int g_stage_mask = 0;
+constexpr int kMarks = 4096;
+hipEvent_t g_marks[kMarks];
+int g_next_mark = 0;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+    return fail(RV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define RV_HIP(call)                                  \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+bool ready() { return g_device >= 0 && g_stream; }
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct PlaneGeom {
+    int pw[3], ph[3], pitch[3];
+    size_t bytes[3];
+    int ss;
+};
+
+PlaneGeom plane_geom(int w, int h, int bd) {
+    PlaneGeom g;
+    g.ss = bd > 8 ? 2 : 1;
+    for (int c = 0; c < 3; ++c) {
+        g.pw[c] = c ? w / 2 : w;
+        g.ph[c] = c ? h / 2 : h;
+        size_t row = align_up((size_t)g.pw[c] * g.ss, 256);
+        g.pitch[c] = (int)(row / g.ss);
+        g.bytes[c] = row * g.ph[c];
+    }
+    return g;
+}
+
+}  // namespace
+
+struct rv_session {
+    int codec, w, h, bd, slots;
+    PlaneGeom g;
+    std::vector<void*> planes;  // slots * 3
+};
+
+struct rv_job {
+    rv_session* s = nullptr;
+    int codec = 0;
+    int out_slot = 0;
+    void* arena = nullptr;   // work lists + descriptor
+    void* scratch = nullptr; // reconstruction planes + side maps
+    rv::HevcPicDev hd{};
+    rv::H264PicDev h4{};
+    void* desc_dev = nullptr;  // device copy of the descriptor
+    // Single-picture tables for the plain path.
+    rv::BatchDev single{};
+    int64_t bytes[4] = {0, 0, 0, 0};
+};
+
+struct rv_batch {
+    std::vector<rv_job*> jobs;
+    std::vector<rv_job*> hevc, h264;
+    rv::BatchDev bh{}, b4{};
+    std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
+    void* tables = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+namespace {
+
+template <typename T> T* carve(char*& p, size_t n) {
+    T* r = (T*)p;
+    p += align_up(n * sizeof(T), 256);
+    return r;
+}
+template <typename T> size_t carve_size(size_t n) { return align_up(n * sizeof(T), 256); }
+
+int launch_codec(const rv::BatchDev& b, const std::vector<int32_t>& pre, int n, bool hevc, hipStream_t st) {
+    for (int s = 0; s < rv::kStages; ++s) {
+        int total = pre[(size_t)s * (n + 1) + n];
+        hipError_t e = hevc ? rv::hevc_launch_stage(s, b, total, st) : rv::h264_launch_stage(s, b, total, st);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
+    return RV_OK;
+}
+
+// ---- HEVC job preparation -------------------------------------------
+int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
+    if (p->width != s->w || p->height != s->h || p->bit_depth != s->bd)
+        return fail(RV_E_RANGE, "picture shape differs from the session");
+    if (p->log2_ctb < 4 || p->log2_ctb > 6) return fail(RV_E_RANGE, "log2_ctb outside 4..6");
+    for (int l = 0; l < 2; ++l) {
+        if (p->num_ref[l] < 0 || p->num_ref[l] > RV_MAX_REFS) return fail(RV_E_RANGE, "num_ref outside 0..4");
+        for (int r = 0; r < p->num_ref[l]; ++r)
+            if (p->ref_slot[l][r] < 0 || p->ref_slot[l][r] >= s->slots) return fail(RV_E_RANGE, "ref_slot outside the DPB");
+    }
+    if (p->n_cus < 0 || p->n_pus < 0 || p->n_tus < 0) return fail(RV_E_INVALID, "negative list length");
+    if ((p->n_cus && !p->cus) || (p->n_pus && !p->pus) || (p->n_tus && !p->tus) || (p->n_coefs && !p->coefs) ||
+        (p->n_intra_pred && !p->intra_pred))
+        return fail(RV_E_INVALID, "null work list");
+    if ((p->sao_luma || p->sao_chroma) && !p->sao) return fail(RV_E_INVALID, "SAO enabled without parameters");
+    if (p->weighted)
+        for (int c = 0; c < 3; ++c)
+            if (p->log2_wd[c] < 0 || p->log2_wd[c] > 14) return fail(RV_E_RANGE, "log2_wd outside 0..14");
+    const int W = p->width, H = p->height;
+    int64_t cu_area = 0, pred_area = 0, tu_area = 0;
+    for (int i = 0; i < p->n_cus; ++i) {
+        const rv_hevc_cu& c = p->cus[i];
+        if (c.log2_size < 3 || c.log2_size > 6) return fail(RV_E_RANGE, "CU size outside 8..64");
+        int sz = 1 << c.log2_size;
+        if ((c.x & 7) || (c.y & 7) || c.x + sz > W || c.y + sz > H) return fail(RV_E_RANGE, "CU outside the picture");
+        if (c.qp_y < -6 * (s->bd - 8) || c.qp_y > 51) return fail(RV_E_RANGE, "QpY outside -QpBdOffset..51");
+        cu_area += (int64_t)sz * sz;
+        if (c.intra) {
+            if ((uint64_t)c.pred_off + (uint64_t)sz * sz * 3 / 2 > p->n_intra_pred)
+                return fail(RV_E_RANGE, "intra prediction outside intra_pred");
+            pred_area += (int64_t)sz * sz;
+        }
+    }
+    for (int i = 0; i < p->n_pus; ++i) {
+        const rv_hevc_pu& u = p->pus[i];
+        if (u.w < 4 || u.h < 4 || u.w > 64 || u.h > 64 || (u.w & 3) || (u.h & 3) || (u.x & 3) || (u.y & 3) ||
+            u.x + u.w > W || u.y + u.h > H)
+            return fail(RV_E_RANGE, "PU outside the picture or of bad size");
+        if (u.dir < 1 || u.dir > 3) return fail(RV_E_RANGE, "PU without a list");
+        for (int l = 0; l < 2; ++l)
+            if ((u.dir >> l) & 1)
+                if (u.ref_idx[l] < 0 || u.ref_idx[l] >= p->num_ref[l]) return fail(RV_E_RANGE, "ref_idx outside the list");
+        pred_area += (int64_t)u.w * u.h;
+    }
+    for (int i = 0; i < p->n_tus; ++i) {
+        const rv_hevc_tu& t = p->tus[i];
+        if (t.log2 < 2 || t.log2 > 5 || t.comp > 2) return fail(RV_E_RANGE, "TU size / component");
+        int n = 1 << t.log2, pw = t.comp ? W / 2 : W, ph = t.comp ? H / 2 : H;
+        if ((t.x & 3) || (t.y & 3) || t.x + n > pw || t.y + n > ph) return fail(RV_E_RANGE, "TU outside the plane");
+        if ((t.flags & RV_TU_DST) && (t.log2 != 2 || t.comp != 0)) return fail(RV_E_RANGE, "DST on a non-4x4 luma TU");
+        if (t.flags & RV_TU_CODED) {
+            if (t.max_x >= n || t.max_y >= n) return fail(RV_E_RANGE, "nonzero bounds outside the TU");
+            if ((uint64_t)t.coef_off + (uint64_t)n * n > p->n_coefs) return fail(RV_E_RANGE, "coefficients outside coefs");
+        }
+        if (t.comp == 0) tu_area += (int64_t)n * n;
+    }
+    const int64_t area = (int64_t)W * H;
+    if (cu_area != area || pred_area != area || tu_area != area)
+        return fail(RV_E_RANGE, "CU / PU / luma TU lists do not tile the picture");
+    return RV_OK;
+}
+
+void hevc_bytes(const rv_hevc_pic* p, int64_t out[4]) {
+    const int64_t s = p->bit_depth > 8 ? 2 : 1;
+    int64_t mc = 0;
+    for (int i = 0; i < p->n_pus; ++i) {
+        const rv_hevc_pu& u = p->pus[i];
+        bool uni_default = u.dir != 3 && !p->weighted;
+        for (int l = 0; l < 2; ++l) {
+            if (!((u.dir >> l) & 1)) continue;
+            int mvx = u.mv[l].x, mvy = u.mv[l].y;
+            bool dl = uni_default && (mvx & 3) == 0 && (mvy & 3) == 0;
+            bool dc = uni_default && (mvx & 7) == 0 && (mvy & 7) == 0;
+            mc += dl ? (int64_t)u.w * u.h : (int64_t)(u.w + 7) * (u.h + 7);
+            mc += 2 * (dc ? (int64_t)(u.w / 2) * (u.h / 2) : (int64_t)(u.w / 2 + 3) * (u.h / 2 + 3));
+        }
+    }
+    int64_t planes = (int64_t)p->width * p->height * 3 / 2;
+    for (int i = 0; i < p->n_cus; ++i)
+        if (p->cus[i].intra) mc += (int64_t)(1 << p->cus[i].log2_size) * (1 << p->cus[i].log2_size) * 3 / 2 * 2;
+    out[0] = s * (mc + planes);
+    int64_t coef = 0;
+    for (int i = 0; i < p->n_tus; ++i)
+        if (p->tus[i].flags & RV_TU_CODED) coef += (int64_t)1 << (2 * p->tus[i].log2);
+    out[1] = 2 * coef + 2 * s * coef;
+    bool chroma = false;
+    for (int i = 0; i < p->n_cus && !chroma; ++i) chroma = p->cus[i].intra != 0;
+    int64_t luma = (int64_t)p->width * p->height;
+    out[2] = p->deblock ? 2 * (2 * s * (luma + (chroma ? luma / 2 : 0))) : 0;
+    out[3] = 2 * s * planes;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rv_abi_size(int which) {
+    switch (which) {
+        case 0: return (int)sizeof(rv_hevc_cu);
+        case 1: return (int)sizeof(rv_hevc_pu);
+        case 2: return (int)sizeof(rv_hevc_tu);
+        case 3: return (int)sizeof(rv_sao);
+        case 4: return (int)sizeof(rv_hevc_pic);
+        case 5: return (int)sizeof(rv_h264_mb);
+        case 6: return (int)sizeof(rv_h264_part);
+        case 7: return (int)sizeof(rv_h264_blk);
+        case 8: return (int)sizeof(rv_h264_pic);
+        default: return -1;
+    }
+}
+
+int rv_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rv_init(int device_ordinal) {
+    try {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_device >= 0) {
+            if (g_device == device_ordinal) return RV_OK;
+            return fail(RV_E_STATE, "already initialised on another device");
+        }
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RV_E_NODEVICE, "no HIP device");
+        if (device_ordinal < 0 || device_ordinal >= n) return fail(RV_E_NODEVICE, "device ordinal out of range");
+        RV_HIP(hipSetDevice(device_ordinal));
+        hipDeviceProp_t prop;
+        RV_HIP(hipGetDeviceProperties(&prop, device_ordinal));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(RV_E_NODEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+        RV_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+        for (int i = 0; i < kMarks; ++i) RV_HIP(hipEventCreate(&g_marks[i]));
+        const char* e = std::getenv("RIVET_HIP_NO_DEBLOCK");
+        if (e && *e && *e != '0') g_stage_mask |= 1;
+        e = std::getenv("RIVET_HIP_NO_SAO");
+        if (e && *e && *e != '0') g_stage_mask |= 2;
+        g_device = device_ordinal;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_HIP, "exception in rv_init");
+    }
+}
+
+const char* rv_tier_name(void) { return g_device >= 0 ? "hip-gfx950" : "hip-uninitialised"; }
+const char* rv_last_error(void) { return g_err.c_str(); }
+
+int rv_set_stage_mask(int mask) {
+    if (mask < 0 || mask > 3) return fail(RV_E_INVALID, "stage mask is 0..3");
+    g_stage_mask = mask;
+    return RV_OK;
+}
+
+int64_t rv_session_footprint(int codec, int width, int height, int bit_depth, int dpb_slots) {
+    if (width <= 0 || height <= 0 || dpb_slots <= 0) return -1;
+    PlaneGeom g = plane_geom(width, height, bit_depth);
+    int64_t pic = (int64_t)(g.bytes[0] + g.bytes[1] + g.bytes[2]);
+    int64_t n4 = (int64_t)(width / 4) * (height / 4);
+    // DPB + one job's reconstruction planes and side maps + work lists
+    // (bounded by the coefficients of a fully coded picture).
+    int64_t lists = (int64_t)width * height * 3 / 2 * (codec == RV_CODEC_H264 ? 4 : 2) / 4 + n4 * 32;
+    return (int64_t)dpb_slots * pic + pic + n4 * 16 + lists;
+}
+
+int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_slots, rv_session** out) {
+    try {
+        if (!out) return fail(RV_E_INVALID, "null out");
+        *out = nullptr;
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        if (codec != RV_CODEC_HEVC && codec != RV_CODEC_H264) return fail(RV_E_INVALID, "unknown codec");
+        int m = codec == RV_CODEC_HEVC ? 8 : 16;
+        if (width <= 0 || height <= 0 || width % m || height % m || width > 8192 || height > 8192)
+            return fail(RV_E_RANGE, "picture size");
+        if (bit_depth < 8 || bit_depth > 12 || (codec == RV_CODEC_H264 && bit_depth != 8))
+            return fail(RV_E_RANGE, "bit depth");
+        if (dpb_slots < 1 || dpb_slots > 64) return fail(RV_E_RANGE, "dpb_slots outside 1..64");
+        rv_session* s = new rv_session();
+        s->codec = codec;
+        s->w = width;
+        s->h = height;
+        s->bd = bit_depth;
+        s->slots = dpb_slots;
+        s->g = plane_geom(width, height, bit_depth);
+        s->planes.assign((size_t)dpb_slots * 3, nullptr);
+        for (int i = 0; i < dpb_slots * 3; ++i) {
+            hipError_t e = hipMalloc(&s->planes[i], s->g.bytes[i % 3]);
+            if (e != hipSuccess) {
+                for (void* p : s->planes)
+                    if (p) (void)hipFree(p);
+                delete s;
+                return fail(RV_E_NOMEM, "DPB allocation failed");
+            }
+            (void)hipMemset(s->planes[i], 0, s->g.bytes[i % 3]);
+        }
+        *out = s;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_session_create");
+    }
+}
+
+int rv_session_destroy(rv_session* s) {
+    if (!s) return fail(RV_E_INVALID, "null session");
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    for (void* p : s->planes)
+        if (p) (void)hipFree(p);
+    delete s;
+    return RV_OK;
+}
+
+int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb, const void* cr) {
+    if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
+    if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+    const void* src[3] = {y, cb, cr};
+    for (int c = 0; c < 3; ++c)
+        RV_HIP(hipMemcpy2DAsync(s->planes[(size_t)slot * 3 + c], (size_t)s->g.pitch[c] * s->g.ss, src[c],
+                                (size_t)s->g.pw[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
+                                hipMemcpyHostToDevice, g_stream));
+    RV_HIP(hipStreamSynchronize(g_stream));
+    return RV_OK;
+}
+
+int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr) {
+    if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
+    if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+    void* dst[3] = {y, cb, cr};
+    for (int c = 0; c < 3; ++c)
+        RV_HIP(hipMemcpy2DAsync(dst[c], (size_t)s->g.pw[c] * s->g.ss, s->planes[(size_t)slot * 3 + c],
+                                (size_t)s->g.pitch[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
+                                hipMemcpyDeviceToHost, g_stream));
+    RV_HIP(hipStreamSynchronize(g_stream));
+    return RV_OK;
+}
+
+int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** out) {
+    try {
+        if (!s || !p || !out) return fail(RV_E_INVALID, "null argument");
+        *out = nullptr;
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        if (s->codec != RV_CODEC_HEVC) return fail(RV_E_INVALID, "not an HEVC session");
+        if (out_slot < 0 || out_slot >= s->slots) return fail(RV_E_RANGE, "out_slot outside the DPB");
+        int rc = validate_hevc(s, p);
+        if (rc) return rc;
+        const int W = p->width, H = p->height;
+        // Host-side lists: MC tiles, luma TUs, coded TUs (4x4 first).
+        std::vector<rv::McTile> tiles;
+        tiles.reserve((size_t)p->n_pus * 2);
+        for (int i = 0; i < p->n_pus; ++i) {
+            const rv_hevc_pu& u = p->pus[i];
+            for (int dy = 0; dy < u.h; dy += 16)
+                for (int dx = 0; dx < u.w; dx += 16) {
+                    rv::McTile t;
+                    t.pu = (uint32_t)i;
+                    t.dx = (uint8_t)dx;
+                    t.dy = (uint8_t)dy;
+                    t.tw = (uint8_t)std::min(16, u.w - dx);
+                    t.th = (uint8_t)std::min(16, u.h - dy);
+                    tiles.push_back(t);
+                }
+        }
+        std::vector<rv_hevc_tu> ltus, ctus4, ctusb;
+        for (int i = 0; i < p->n_tus; ++i) {
+            const rv_hevc_tu& t = p->tus[i];
+            if (t.comp == 0) ltus.push_back(t);
+            if (t.flags & RV_TU_CODED) (t.log2 == 2 ? ctus4 : ctusb).push_back(t);
+        }
+        std::stable_sort(ctusb.begin(), ctusb.end(),
+                         [](const rv_hevc_tu& a, const rv_hevc_tu& b) { return a.log2 > b.log2; });
+        std::vector<rv_hevc_tu> ctus(ctus4);
+        ctus.insert(ctus.end(), ctusb.begin(), ctusb.end());
+        int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
+        size_t n_sao = (size_t)wc * hc * 3;
+        std::vector<rv_sao> sao(n_sao, rv_sao{0, 0, {0, 0, 0, 0}});
+        if (p->sao) std::memcpy(sao.data(), p->sao, n_sao * sizeof(rv_sao));
+
+        rv_job* j = new rv_job();
+        j->s = s;
+        j->codec = RV_CODEC_HEVC;
+        j->out_slot = out_slot;
+        hevc_bytes(p, j->bytes);
+        // Device arena.
+        size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
+                    carve_size<rv_hevc_tu>(ltus.size()) + carve_size<rv_hevc_tu>(ctus.size()) +
+                    carve_size<rv::McTile>(tiles.size()) + carve_size<int16_t>(p->n_coefs) +
+                    carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
+                    carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
+        size_t n4 = (size_t)(W / 4) * (H / 4);
+        size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
+                     carve_size<int8_t>(n4) + carve_size<uint32_t>(n4) + 1024;
+        if (hipMalloc(&j->arena, sz) != hipSuccess || hipMalloc(&j->scratch, scr) != hipSuccess) {
+            if (j->arena) (void)hipFree(j->arena);
+            delete j;
+            return fail(RV_E_NOMEM, "job allocation failed");
+        }
+        std::vector<char> host(sz, 0);
+        char* hp = host.data();
+        char* base = (char*)j->arena;
+        auto dev = [&](char* hptr) { return base + (hptr - host.data()); };
+        rv::HevcPicDev* hdesc = carve<rv::HevcPicDev>(hp, 1);
+        rv_hevc_pu* h_pus = carve<rv_hevc_pu>(hp, p->n_pus);
+        rv_hevc_cu* h_cus = carve<rv_hevc_cu>(hp, p->n_cus);
+        rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, ltus.size());
+        rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, ctus.size());
+        rv::McTile* h_tiles = carve<rv::McTile>(hp, tiles.size());
+        int16_t* h_coefs = carve<int16_t>(hp, p->n_coefs);
+        uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
+        rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
+        const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
+        int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
+        if (p->n_pus) std::memcpy(h_pus, p->pus, p->n_pus * sizeof(rv_hevc_pu));
+        if (p->n_cus) std::memcpy(h_cus, p->cus, p->n_cus * sizeof(rv_hevc_cu));
+        if (!ltus.empty()) std::memcpy(h_ltus, ltus.data(), ltus.size() * sizeof(rv_hevc_tu));
+        if (!ctus.empty()) std::memcpy(h_ctus, ctus.data(), ctus.size() * sizeof(rv_hevc_tu));
+        if (!tiles.empty()) std::memcpy(h_tiles, tiles.data(), tiles.size() * sizeof(rv::McTile));
+        if (p->n_coefs) std::memcpy(h_coefs, p->coefs, p->n_coefs * sizeof(int16_t));
+        if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
+        std::memcpy(h_sao, sao.data(), n_sao * sizeof(rv_sao));
+        // Scratch: reconstruction planes and side maps.
+        char* sp = (char*)j->scratch;
+        rv::HevcPicDev& d = j->hd;
+        std::memset(&d, 0, sizeof(d));
+        d.w = W;
+        d.h = H;
+        d.bd = p->bit_depth;
+        d.log2_ctb = p->log2_ctb;
+        d.wc = wc;
+        d.hc = hc;
+        d.w4 = W / 4;
+        d.h4 = H / 4;
+        for (int c = 0; c < 3; ++c) {
+            d.pitch[c] = s->g.pitch[c];
+            d.pw[c] = s->g.pw[c];
+            d.ph[c] = s->g.ph[c];
+            d.cur[c] = sp;
+            sp += s->g.bytes[c];
+            d.out[c] = s->planes[(size_t)out_slot * 3 + c];
+        }
+        for (int l = 0; l < 2; ++l)
+            for (int r = 0; r < RV_MAX_REFS; ++r) {
+                int slot = r < p->num_ref[l] ? p->ref_slot[l][r] : 0;
+                for (int c = 0; c < 3; ++c) d.ref[l][r][c] = s->planes[(size_t)slot * 3 + c];
+                d.ref_delta[l][r] = r < p->num_ref[l] ? p->poc - p->ref_poc[l][r] : 0;
+                d.ref_lt[l][r] = r < p->num_ref[l] ? (p->ref_long_term[l][r] != 0) : 0;
+                for (int c = 0; c < 3; ++c) {
+                    d.wp_w[l][r][c] = p->wp_w[l][r][c];
+                    d.wp_o[l][r][c] = p->wp_o[l][r][c];
+                }
+            }
+        d.weighted = p->weighted;
+        for (int c = 0; c < 3; ++c) d.log2_wd[c] = p->log2_wd[c];
+        d.deblock = (g_stage_mask & 1) ? 0 : p->deblock;
+        d.beta_offset = p->beta_offset;
+        d.tc_offset = p->tc_offset;
+        d.cb_qp_offset = p->cb_qp_offset;
+        d.cr_qp_offset = p->cr_qp_offset;
+        d.sao_luma = (g_stage_mask & 2) ? 0 : p->sao_luma;
+        d.sao_chroma = (g_stage_mask & 2) ? 0 : p->sao_chroma;
+        d.n_cus = p->n_cus;
+        d.n_pus = p->n_pus;
+        d.n_ltus = (int)ltus.size();
+        d.n_tiles = (int)tiles.size();
+        d.n_tx4 = (int)ctus4.size();
+        d.n_txbig = (int)ctusb.size();
+        d.cus = (const rv_hevc_cu*)dev((char*)h_cus);
+        d.pus = (const rv_hevc_pu*)dev((char*)h_pus);
+        d.ltus = (const rv_hevc_tu*)dev((char*)h_ltus);
+        d.ctus = (const rv_hevc_tu*)dev((char*)h_ctus);
+        d.tiles = (const rv::McTile*)dev((char*)h_tiles);
+        d.coefs = (const int16_t*)dev((char*)h_coefs);
+        d.intra_pred = (const uint16_t*)dev((char*)h_ip);
+        d.sao = (const rv_sao*)dev((char*)h_sao);
+        d.cuinfo = (uint8_t*)sp;
+        sp += carve_size<uint8_t>(n4);
+        d.tuinfo = (uint8_t*)sp;
+        sp += carve_size<uint8_t>(n4);
+        d.bs = (uint8_t*)sp;
+        sp += carve_size<uint8_t>(n4);
+        d.qp = (int8_t*)sp;
+        sp += carve_size<int8_t>(n4);
+        d.pumap = (uint32_t*)sp;
+        rv::hevc_stage_items(d);
+        *hdesc = d;
+        j->desc_dev = dev((char*)hdesc);
+        *h_ptr = (const rv::HevcPicDev*)j->desc_dev;
+        for (int st = 0; st < rv::kStages; ++st) {
+            h_pre[st * 2] = 0;
+            h_pre[st * 2 + 1] = d.items[st];
+        }
+        j->single.n = 1;
+        j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
+        j->single.h264 = nullptr;
+        j->single.prefix = (const int32_t*)dev((char*)h_pre);
+        hipError_t e = hipMemcpy(j->arena, host.data(), sz, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(j->scratch, 0, scr);
+        if (e != hipSuccess) {
+            (void)hipFree(j->arena);
+            (void)hipFree(j->scratch);
+            delete j;
+            return hip_fail(e, "job upload");
+        }
+        *out = j;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_hevc_prepare");
+    }
+}
+
+int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** out) {
+    if (out) *out = nullptr;
+    (void)s;
+    (void)p;
+    (void)out_slot;
+    return fail(RV_E_INVALID, "H.264 jobs are not available in this build");
+}
+
+int rv_job_free(rv_job* j) {
+    if (!j) return fail(RV_E_INVALID, "null job");
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    if (j->arena) (void)hipFree(j->arena);
+    if (j->scratch) (void)hipFree(j->scratch);
+    delete j;
+    return RV_OK;
+}
+
+int rv_job_bytes(const rv_job* j, int64_t out[4]) {
+    if (!j || !out) return fail(RV_E_INVALID, "null argument");
+    for (int i = 0; i < 4; ++i) out[i] = j->bytes[i];
+    return RV_OK;
+}
+
+int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
+    try {
+        if (!jobs || !out || n <= 0) return fail(RV_E_INVALID, "empty batch");
+        *out = nullptr;
+        rv_batch* b = new rv_batch();
+        for (int i = 0; i < n; ++i) {
+            if (!jobs[i]) {
+                delete b;
+                return fail(RV_E_INVALID, "null job in batch");
+            }
+            b->jobs.push_back(jobs[i]);
+            (jobs[i]->codec == RV_CODEC_HEVC ? b->hevc : b->h264).push_back(jobs[i]);
+        }
+        int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
+        b->pre_h.assign((size_t)rv::kStages * (nh + 1), 0);
+        b->pre_4.assign((size_t)rv::kStages * (n4 + 1), 0);
+        for (int s = 0; s < rv::kStages; ++s) {
+            for (int i = 0; i < nh; ++i)
+                b->pre_h[(size_t)s * (nh + 1) + i + 1] = b->pre_h[(size_t)s * (nh + 1) + i] + b->hevc[i]->hd.items[s];
+            for (int i = 0; i < n4; ++i)
+                b->pre_4[(size_t)s * (n4 + 1) + i + 1] = b->pre_4[(size_t)s * (n4 + 1) + i] + b->h264[i]->h4.items[s];
+        }
+        size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
+                    carve_size<int32_t>(b->pre_4.size());
+        std::vector<char> host(sz, 0);
+        if (hipMalloc(&b->tables, sz) != hipSuccess) {
+            delete b;
+            return fail(RV_E_NOMEM, "batch tables");
+        }
+        char* hp = host.data();
+        char* base = (char*)b->tables;
+        auto dev = [&](char* h) { return base + (h - host.data()); };
+        void** ph = carve<void*>(hp, nh);
+        void** p4 = carve<void*>(hp, n4);
+        int32_t* preh = carve<int32_t>(hp, b->pre_h.size());
+        int32_t* pre4 = carve<int32_t>(hp, b->pre_4.size());
+        for (int i = 0; i < nh; ++i) ph[i] = b->hevc[i]->desc_dev;
+        for (int i = 0; i < n4; ++i) p4[i] = b->h264[i]->desc_dev;
+        std::memcpy(preh, b->pre_h.data(), b->pre_h.size() * 4);
+        std::memcpy(pre4, b->pre_4.data(), b->pre_4.size() * 4);
+        b->bh.n = nh;
+        b->bh.hevc = (const rv::HevcPicDev* const*)dev((char*)ph);
+        b->bh.prefix = (const int32_t*)dev((char*)preh);
+        b->b4.n = n4;
+        b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
+        b->b4.prefix = (const int32_t*)dev((char*)pre4);
+        hipError_t e = hipMemcpy(b->tables, host.data(), sz, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(b->tables);
+            delete b;
+            return hip_fail(e, "batch upload");
+        }
+        *out = b;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_batch_create");
+    }
+}
+
+int rv_batch_free(rv_batch* b) {
+    if (!b) return fail(RV_E_INVALID, "null batch");
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->tables) (void)hipFree(b->tables);
+    delete b;
+    return RV_OK;
+}
+
+static int launch_batched(rv_batch* b, hipStream_t st) {
+    int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
+    if (nh) {
+        int rc = launch_codec(b->bh, b->pre_h, nh, true, st);
+        if (rc) return rc;
+    }
+    if (n4) {
+        int rc = launch_codec(b->b4, b->pre_4, n4, false, st);
+        if (rc) return rc;
+    }
+    return RV_OK;
+}
+
+int rv_batch_launch(rv_batch* b, int mode) {
+    try {
+        if (!b) return fail(RV_E_INVALID, "null batch");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (mode == RV_LAUNCH_PLAIN) {
+            for (rv_job* j : b->jobs) {
+                std::vector<int32_t> pre((size_t)rv::kStages * 2);
+                const int32_t* items = j->codec == RV_CODEC_HEVC ? j->hd.items : j->h4.items;
+                for (int s = 0; s < rv::kStages; ++s) {
+                    pre[s * 2] = 0;
+                    pre[s * 2 + 1] = items[s];
+                }
+                int rc = launch_codec(j->single, pre, 1, j->codec == RV_CODEC_HEVC, g_stream);
+                if (rc) return rc;
+            }
+            return RV_OK;
+        }
+        if (mode == RV_LAUNCH_BATCHED) return launch_batched(b, g_stream);
+        if (mode == RV_LAUNCH_GRAPH) {
+            if (!b->exec) {
+                RV_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+                int rc = launch_batched(b, g_stream);
+                hipGraph_t graph = nullptr;
+                hipError_t e = hipStreamEndCapture(g_stream, &graph);
+                if (rc) return rc;
+                if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+                b->graph = graph;
+                RV_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+            }
+            RV_HIP(hipGraphLaunch(b->exec, g_stream));
+            return RV_OK;
+        }
+        return fail(RV_E_INVALID, "unknown launch mode");
+    } catch (...) {
+        return fail(RV_E_HIP, "exception in rv_batch_launch");
+    }
+}
+
+int rv_sync(void) {
+    if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+    RV_HIP(hipStreamSynchronize(g_stream));
+    return RV_OK;
+}
+
+int rv_mark(int* id_out) {
+    if (!id_out) return fail(RV_E_INVALID, "null out");
+    if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+    std::lock_guard<std::mutex> lk(g_mu);
+    int id = g_next_mark;
+    g_next_mark = (g_next_mark + 1) % kMarks;
+    RV_HIP(hipEventRecord(g_marks[id], g_stream));
+    *id_out = id;
+    return RV_OK;
+}
+
+int rv_mark_query(int id) {
+    if (id < 0 || id >= kMarks || !ready()) return fail(RV_E_INVALID, "bad marker");
+    hipError_t e = hipEventQuery(g_marks[id]);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return hip_fail(e, "hipEventQuery");
+}
+
+int rv_mark_wait(int id) {
+    if (id < 0 || id >= kMarks || !ready()) return fail(RV_E_INVALID, "bad marker");
+    RV_HIP(hipEventSynchronize(g_marks[id]));
+    return RV_OK;
+}
+
+int rv_mark_elapsed(int id_start, int id_end, float* ms) {
+    if (!ms || id_start < 0 || id_start >= kMarks || id_end < 0 || id_end >= kMarks || !ready())
+        return fail(RV_E_INVALID, "bad marker");
+    RV_HIP(hipEventElapsedTime(ms, g_marks[id_start], g_marks[id_end]));
+    return RV_OK;
+}
+
+}  // extern "C"

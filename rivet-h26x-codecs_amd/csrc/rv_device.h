@@ -1,0 +1,91 @@
+// Device-side descriptors shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/rivet_hip.h"
+
+namespace rv {
+
+// One motion-compensation work item: a tile of at most 16x16 luma samples
+// (and its two 4:2:0 chroma tiles) inside one prediction unit. Splitting
+// PUs into tiles on the host keeps work items of even size.
+struct McTile {
+    uint32_t pu;
+    uint8_t dx, dy;  // tile offset inside the PU, luma samples
+    uint8_t tw, th;  // tile size, luma samples (multiples of 4, <= 16)
+};
+
+// Per-picture descriptor of an HEVC job, resident in device memory.
+struct HevcPicDev {
+    int32_t w, h, bd, log2_ctb, wc, hc, w4, h4;
+    int32_t pitch[3];  // samples
+    int32_t pw[3], ph[3];
+    void* cur[3];  // reconstruction target (deblocked in place)
+    void* out[3];  // SAO output: the job's DPB slot
+    const void* ref[2][RV_MAX_REFS][3];
+    int32_t weighted, log2_wd[3];
+    int32_t wp_w[2][RV_MAX_REFS][3], wp_o[2][RV_MAX_REFS][3];
+    int32_t ref_delta[2][RV_MAX_REFS], ref_lt[2][RV_MAX_REFS];
+    int32_t deblock, beta_offset, tc_offset, cb_qp_offset, cr_qp_offset;
+    int32_t sao_luma, sao_chroma;
+    // Work lists.
+    int32_t n_cus, n_pus, n_ltus, n_tiles;
+    int32_t n_tx4, n_txbig;  // coded TUs: 4x4 ones first, then the rest
+    const rv_hevc_cu* cus;
+    const rv_hevc_pu* pus;
+    const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
+    const McTile* tiles;
+    const rv_hevc_tu* ctus;  // coded TUs, 4x4 first
+    const int16_t* coefs;
+    const uint16_t* intra_pred;
+    const rv_sao* sao;
+    // Per-4x4 side data (job scratch).
+    uint8_t* cuinfo;   // bit 0 intra, bit 1 CU left edge, bit 2 CU top edge
+    int8_t* qp;        // QpY
+    uint32_t* pumap;   // PU index (bits 0..29), bit 30 PU left edge, bit 31 top
+    uint8_t* tuinfo;   // bit 0 cbf, bit 1 TU left edge, bit 2 TU top edge
+    uint8_t* bs;       // bits 0..1 bS of the left edge, bits 2..3 top edge
+    // Item counts per stage (derived).
+    int32_t items[5];
+};
+
+// Per-picture descriptor of an H.264 job.
+struct H264PicDev {
+    int32_t w, h, mbw, mbh, w4, h4;
+    int32_t pitch[3];
+    int32_t pw[3], ph[3];
+    void* cur[3];
+    void* out[3];
+    const void* ref[2][RV_MAX_REFS][3];
+    int32_t ref_id[2][RV_MAX_REFS];
+    int32_t weighting;  // RV_H264_*
+    int32_t log_wd[3];
+    int32_t wp_w[2][RV_MAX_REFS][3], wp_o[2][RV_MAX_REFS][3];
+    int32_t imp_w[RV_MAX_REFS][RV_MAX_REFS][2];  // implicit (w0, w1)
+    int32_t deblock, offset_a, offset_b;
+    int32_t n_mbs, n_parts, n_blks4, n_blks8;
+    const rv_h264_mb* mbs;
+    const rv_h264_part* parts;
+    const rv_h264_blk* blks;  // 4x4 first, then 8x8
+    const int32_t* coefs;
+    const uint16_t* intra_pred;
+    uint32_t* partmap;   // per 4x4: partition index (0xFFFFFFFF = intra)
+    uint32_t* bsv;       // per MB x 4 edges: packed vertical bS (byte/segment)
+    uint32_t* bsh;       // per MB x 4 edges: packed horizontal bS
+    int32_t items[5];
+};
+
+// Per-batch tables: the descriptors and, per stage, prefix sums of the
+// per-picture work-item counts.
+constexpr int kStages = 5;
+struct BatchDev {
+    int32_t n;
+    const HevcPicDev* const* hevc;  // per picture (null for H.264)
+    const H264PicDev* const* h264;
+    const int32_t* prefix;          // kStages x (n + 1)
+};
+
+}  // namespace rv

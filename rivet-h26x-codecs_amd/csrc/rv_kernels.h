@@ -1,0 +1,18 @@
+// Launch entry points of the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rv_device.h"
+
+namespace rv {
+
+// HEVC (hevc.hip).
+void hevc_stage_items(HevcPicDev& P);
+hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
+
+// H.264 (h264.hip).
+void h264_stage_items(H264PicDev& P);
+hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
+
+}  // namespace rv
