@@ -283,6 +283,11 @@ int rv_batch_free(rv_batch* b);
 int rv_batch_launch(rv_batch* b, int mode);
 /* Block until everything launched so far has completed. */
 int rv_sync(void);
+/* Launch the batch's BATCHED stages `reps` times with a HIP event between
+ * consecutive stages on the library's stream, wait, and write the average
+ * duration of each stage in milliseconds (out_ms[0..4]; for a batch with
+ * both codecs, HEVC stages then H.264 stages are summed per index). */
+int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]);
 /* Record a completion marker after everything launched so far; poll or
  * wait for it (0 = complete, 1 = pending). Markers are recycled by id. */
 int rv_mark(int* id_out);

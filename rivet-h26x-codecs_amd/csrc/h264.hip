@@ -1,17 +1,661 @@
-// H.264 picture reconstruction on gfx950 (stages filled in below).
+// H.264 picture reconstruction on gfx950 (8-bit, 4:2:0, frame pictures).
+//
+//   stage 0  intra prediction blocks, motion-compensated partitions,
+//            per-4x4 partition map
+//   stage 1  4x4 / 8x8 inverse transforms + residual add (one block per lane)
+//   stage 2  boundary strengths per macroblock; wavefront counters reset
+//   stage 3  deblocking as a macroblock wavefront: one workgroup per MB row
+//            (rows taken in dispatch order from a ticket), MB (x, y) after
+//            MB (x + 1, y - 1) through per-row progress counters — the
+//            raster order of the reference (src/h264/deblock.rs:617-813)
+//   stage 4  (none: the picture is reconstructed in its DPB slot)
+//
+// Arithmetic restates src/dsp/h264.rs bit for bit (wrapping i16 transforms
+// at 8 bits, :766-1066).
 #include <hip/hip_runtime.h>
 
 #include "rv_device.h"
 #include "rv_kernels.h"
 
 namespace rv {
+namespace {
 
-void h264_stage_items(H264PicDev& P) {
-    for (int s = 0; s < kStages; ++s) P.items[s] = 0;
+// Tables 8-16 / 8-17 of H.264 (src/h264/tables_gen.rs:331-392).
+__constant__ uint8_t kAlpha[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,   0,   0,   0,   0,   0,   4,   4,
+                                   5,  6,  7,  8,  9,  10, 12, 13, 15, 17, 20,  22,  25,  28,  32,  36,  40,  45,
+                                   50, 56, 63, 71, 80, 90, 101, 113, 127, 144, 162, 182, 203, 226, 255, 255};
+__constant__ uint8_t kBeta[52] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  0,  0,  0,  0,  2,  2,  2,  3,  3,  3,  3,  4,  4,  4,
+                                  6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13, 14, 14, 15, 15, 16, 16, 17, 17, 18, 18};
+__constant__ uint8_t kTc0[52][3] = {
+    {0, 0, 0},  {0, 0, 0},  {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},    {0, 0, 0},
+    {0, 0, 0},  {0, 0, 0},  {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},   {0, 0, 0},    {0, 0, 0},
+    {0, 0, 0},  {0, 0, 1},  {0, 0, 1},   {0, 0, 1},   {0, 0, 1},   {0, 1, 1},   {0, 1, 1},    {1, 1, 1},
+    {1, 1, 1},  {1, 1, 1},  {1, 1, 1},   {1, 1, 2},   {1, 1, 2},   {1, 1, 2},   {1, 1, 2},    {1, 2, 3},
+    {1, 2, 3},  {2, 2, 3},  {2, 2, 4},   {2, 3, 4},   {2, 3, 4},   {3, 3, 5},   {3, 4, 6},    {3, 4, 6},
+    {4, 5, 7},  {4, 5, 8},  {4, 6, 9},   {5, 7, 10},  {6, 8, 11},  {6, 8, 13},  {7, 10, 14},  {8, 11, 16},
+    {9, 12, 18}, {10, 13, 20}, {11, 15, 23}, {13, 17, 25}};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+__device__ __forceinline__ int find_pic(const int32_t* pre, int n, int it) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
-hipError_t h264_launch_stage(int, const BatchDev&, int total, hipStream_t) {
-    return total <= 0 ? hipSuccess : hipErrorInvalidValue;
+// ---------------------------------------------------------------------
+// Stage 0: intra blocks, motion compensation
+// ---------------------------------------------------------------------
+
+__device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
+    return a - 5 * b + 20 * c + 20 * d - 5 * e + f;
+}
+
+// qpel_scalar<XF, YF> (src/dsp/h264.rs:427-493) over a window staged in
+// LDS: W is the window (stride 32), origin two left / two above the block.
+struct Win {
+    const int16_t* w;
+    __device__ int G(int x, int y) const { return w[(y + 2) * 32 + x + 2]; }
+    __device__ int b1(int x, int yy) const {
+        const int16_t* r = w + yy * 32 + x;
+        return tap6(r[0], r[1], r[2], r[3], r[4], r[5]);
+    }
+    __device__ int h1(int xx, int y) const {
+        const int16_t* c = w + y * 32 + xx;
+        return tap6(c[0], c[32], c[64], c[96], c[128], c[160]);
+    }
+    __device__ int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, 255); }
+    __device__ int h(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, 255); }
+    __device__ int j(int x, int y) const {
+        int j1 = tap6(b1(x, y), b1(x, y + 1), b1(x, y + 2), b1(x, y + 3), b1(x, y + 4), b1(x, y + 5));
+        return clampi((j1 + 512) >> 10, 0, 255);
+    }
+};
+
+__device__ int qpel_at(const Win& q, int pos, int x, int y) {
+    switch (pos) {
+        case 0: return q.G(x, y);
+        case 1: return (q.G(x, y) + q.b(x, y) + 1) >> 1;
+        case 2: return q.b(x, y);
+        case 3: return (q.G(x + 1, y) + q.b(x, y) + 1) >> 1;
+        case 4: return (q.G(x, y) + q.h(x, y) + 1) >> 1;
+        case 8: return q.h(x, y);
+        case 12: return (q.G(x, y + 1) + q.h(x, y) + 1) >> 1;
+        case 10: return q.j(x, y);
+        case 5: return (q.b(x, y) + q.h(x, y) + 1) >> 1;
+        case 7: return (q.b(x, y) + q.h(x + 1, y) + 1) >> 1;
+        case 13: return (q.h(x, y) + q.b(x, y + 1) + 1) >> 1;
+        case 15: return (q.h(x + 1, y) + q.b(x, y + 1) + 1) >> 1;
+        case 6: return (q.b(x, y) + q.j(x, y) + 1) >> 1;
+        case 14: return (q.j(x, y) + q.b(x, y + 1) + 1) >> 1;
+        case 9: return (q.h(x, y) + q.j(x, y) + 1) >> 1;
+        default: return (q.j(x, y) + q.h(x + 1, y) + 1) >> 1;  // 11: (3, 2)
+    }
+}
+
+// Stage a ww x hh window at (x0, y0) of a plane, clamped to the picture
+// (the replicated border of src/h264/frame.rs and the gather of
+// src/h264/inter.rs:127-139 read the same samples). Stride 32.
+__device__ void stage_window(const uint8_t* __restrict__ pl, int pitch, int pw, int ph, int x0, int y0, int ww, int hh,
+                             int16_t* win, int tid) {
+    for (int e = tid; e < hh * 32; e += 64) {
+        int r = e >> 5, c = e & 31;
+        if (c < ww) win[r * 32 + c] = pl[(size_t)clampi(y0 + r, 0, ph - 1) * pitch + clampi(x0 + c, 0, pw - 1)];
+    }
+}
+
+// The combine of predict_partition (src/h264/inter.rs:292-331) with
+// copy / avg / weighted_uni / weighted_bi (src/dsp/h264.rs:522-597).
+__device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, const int8_t* ri, int a, int b) {
+    const bool both = dir == 3;
+    const int single = dir == 1 ? 0 : 1;
+    int log_wd, w0 = 1, w1 = 1, o0 = 0, o1 = 0;
+    if (P.weighting == RV_H264_EXPLICIT) {
+        log_wd = P.log_wd[c];
+        if (dir & 1) {
+            w0 = P.wp_w[0][ri[0]][c];
+            o0 = P.wp_o[0][ri[0]][c];
+        }
+        if (dir & 2) {
+            w1 = P.wp_w[1][ri[1]][c];
+            o1 = P.wp_o[1][ri[1]][c];
+        }
+    } else if (P.weighting == RV_H264_IMPLICIT && both) {
+        log_wd = 5;
+        w0 = P.imp_w[ri[0]][ri[1]][0];
+        w1 = P.imp_w[ri[0]][ri[1]][1];
+    } else {
+        return both ? (a + b + 1) >> 1 : (single == 0 ? a : b);
+    }
+    if (!both) {
+        int v = single == 0 ? a : b, wt = single == 0 ? w0 : w1, o = single == 0 ? o0 : o1;
+        if (log_wd >= 1) return clampi(((v * wt + (1 << (log_wd - 1))) >> log_wd) + o, 0, 255);
+        return clampi(v * wt + o, 0, 255);
+    }
+    return clampi(((a * w0 + b * w1 + (1 << log_wd)) >> (log_wd + 1)) + ((o0 + o1 + 1) >> 1), 0, 255);
+}
+
+__device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
+    const rv_h264_part pt = P.parts[idx];
+    const int x = pt.x, y = pt.y, w = pt.w, h = pt.h, dir = pt.dir;
+    // Partition map for the deblocking filter.
+    for (int e = tid; e < (w >> 2) * (h >> 2); e += 64) {
+        int bx = (x >> 2) + e % (w >> 2), by = (y >> 2) + e / (w >> 2);
+        P.partmap[(size_t)by * P.w4 + bx] = (uint32_t)idx;
+    }
+    int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+        const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
+        int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
+        int xi = x + (mvx >> 2), yi = y + (mvy >> 2), pos = (mvy & 3) * 4 + (mvx & 3);
+        stage_window(ref, P.pitch[0], P.pw[0], P.ph[0], xi - 2, yi - 2, w + 5, h + 5, win, tid);
+        __syncthreads();
+        Win q{win};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
+            if (ox < w && oy < h) pred[l][k] = qpel_at(q, pos, ox, oy);
+        }
+        __syncthreads();
+    }
+    uint8_t* dst = (uint8_t*)P.cur[0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
+        if (ox < w && oy < h) dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (uint8_t)combine(P, 0, dir, pt.ref_idx, pred[0][k], pred[1][k]);
+    }
+    // Chroma: eighth-sample bilinear (src/h264/inter.rs:243-282), 4:2:0.
+    const int cw = w >> 1, ch = h >> 1;
+    for (int c = 1; c < 3; ++c) {
+        int cp[2] = {0, 0};
+        for (int l = 0; l < 2; ++l) {
+            if (!((dir >> l) & 1)) continue;
+            const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][c];
+            int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
+            int xci = (x >> 1) + (mvx >> 3), yci = (y >> 1) + (mvy >> 3), xf = mvx & 7, yf = mvy & 7;
+            stage_window(ref, P.pitch[c], P.pw[c], P.ph[c], xci, yci, cw + 1, ch + 1, win, tid);
+            __syncthreads();
+            int ox = tid & 7, oy = tid >> 3;
+            if (ox < cw && oy < ch) {
+                int a = (8 - xf) * (8 - yf), b = xf * (8 - yf), cc = (8 - xf) * yf, d = xf * yf;
+                const int16_t* r0 = win + oy * 32 + ox;
+                cp[l] = (a * r0[0] + b * r0[1] + cc * r0[32] + d * r0[33] + 32) >> 6;
+            }
+            __syncthreads();
+        }
+        int ox = tid & 7, oy = tid >> 3;
+        if (ox < cw && oy < ch) {
+            uint8_t* d = (uint8_t*)P.cur[c];
+            d[(size_t)((y >> 1) + oy) * P.pitch[c] + (x >> 1) + ox] = (uint8_t)combine(P, c, dir, pt.ref_idx, cp[0], cp[1]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) h264_stage0(BatchDev b) {
+    __shared__ int16_t win[21 * 32];
+    const int32_t* pre = b.prefix + 0 * (b.n + 1);
+    const int it = blockIdx.x;
+    const int p = find_pic(pre, b.n, it);
+    const H264PicDev& P = *b.h264[p];
+    int i = it - pre[p];
+    const int tid = threadIdx.x;
+    if (i < P.n_mbs) {
+        // Intra macroblock: host-supplied prediction; no partition.
+        const rv_h264_mb m = P.mbs[i];
+        if (!m.intra) return;
+        int mx = (i % P.mbw) * 16, my = (i / P.mbw) * 16;
+        const uint16_t* pr = P.intra_pred + m.pred_off;
+        for (int e = tid; e < 256; e += 64) {
+            int yy = e >> 4, xx = e & 15;
+            ((uint8_t*)P.cur[0])[(size_t)(my + yy) * P.pitch[0] + mx + xx] = (uint8_t)pr[e];
+        }
+        for (int c = 1; c < 3; ++c) {
+            const uint16_t* pc = pr + 256 + (c - 1) * 64;
+            int yy = tid >> 3, xx = tid & 7;
+            ((uint8_t*)P.cur[c])[(size_t)((my >> 1) + yy) * P.pitch[c] + (mx >> 1) + xx] = (uint8_t)pc[tid];
+        }
+        if (tid < 16) P.partmap[(size_t)((my >> 2) + (tid >> 2)) * P.w4 + (mx >> 2) + (tid & 3)] = 0xFFFFFFFFu;
+        return;
+    }
+    i -= P.n_mbs;
+    mc_part(P, i, tid, win);
+}
+
+// ---------------------------------------------------------------------
+// Stage 1: residual, one block per lane (src/dsp/h264.rs:766-1066)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int16_t wa(int16_t a, int16_t b) { return (int16_t)(uint16_t)((uint16_t)a + (uint16_t)b); }
+__device__ __forceinline__ int16_t ws(int16_t a, int16_t b) { return (int16_t)(uint16_t)((uint16_t)a - (uint16_t)b); }
+__device__ __forceinline__ int16_t sh(int16_t a, int s) { return (int16_t)(a >> s); }
+
+// add_rows_scalar (:803-816).
+__device__ __forceinline__ void add_px(uint8_t* p, int16_t v) {
+    int16_t r = sh(wa(v, 32), 6);
+    *p = (uint8_t)clampi((int)wa((int16_t)*p, r), 0, 255);
+}
+
+// residual4_scalar with narrow4 and idct4_add_scalar / dc_add_scalar.
+__device__ void residual4(uint8_t* dst, int pitch, const int32_t* co, int has_dc, int dc) {
+    int16_t d[16];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int v = co[i];
+        d[i] = (int16_t)clampi(v, -32768, 32767);
+        if (i) ac |= v;
+    }
+    if (has_dc) d[0] = (int16_t)dc;
+    if (ac == 0) {
+        if (d[0] == 0) return;
+        int v = (int)sh(wa(d[0], 32), 6);
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < 4; ++x) {
+                uint8_t* p = dst + y * pitch + x;
+                *p = (uint8_t)clampi(*p + v, 0, 255);
+            }
+        return;
+    }
+    int16_t t[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int16_t d0 = d[i * 4], d1 = d[i * 4 + 1], d2 = d[i * 4 + 2], d3 = d[i * 4 + 3];
+        int16_t e0 = wa(d0, d2), e1 = ws(d0, d2), e2 = ws(sh(d1, 1), d3), e3 = wa(d1, sh(d3, 1));
+        t[i * 4] = wa(e0, e3);
+        t[i * 4 + 1] = wa(e1, e2);
+        t[i * 4 + 2] = ws(e1, e2);
+        t[i * 4 + 3] = ws(e0, e3);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int16_t f0 = t[j], f1 = t[4 + j], f2 = t[8 + j], f3 = t[12 + j];
+        int16_t g0 = wa(f0, f2), g1 = ws(f0, f2), g2 = ws(sh(f1, 1), f3), g3 = wa(f1, sh(f3, 1));
+        add_px(dst + 0 * pitch + j, wa(g0, g3));
+        add_px(dst + 1 * pitch + j, wa(g1, g2));
+        add_px(dst + 2 * pitch + j, ws(g1, g2));
+        add_px(dst + 3 * pitch + j, ws(g0, g3));
+    }
+}
+
+// idct8_1d_i16 (:913-958).
+__device__ __forceinline__ void idct8_1d(const int16_t* d, int16_t* o) {
+    int16_t a0 = wa(d[0], d[4]), a4 = ws(d[0], d[4]);
+    int16_t a2 = ws(sh(d[2], 1), d[6]), a6 = wa(d[2], sh(d[6], 1));
+    int16_t b0 = wa(a0, a6), b2 = wa(a4, a2), b4 = ws(a4, a2), b6 = ws(a0, a6);
+    int16_t a1 = ws(ws(ws(d[5], d[3]), d[7]), sh(d[7], 1));
+    int16_t a3 = ws(ws(wa(d[1], d[7]), d[3]), sh(d[3], 1));
+    int16_t a5 = wa(wa(ws(d[7], d[1]), d[5]), sh(d[5], 1));
+    int16_t a7 = wa(wa(wa(d[3], d[5]), d[1]), sh(d[1], 1));
+    int16_t b1 = wa(a1, sh(a7, 2)), b7 = ws(a7, sh(a1, 2));
+    int16_t b3 = wa(a3, sh(a5, 2)), b5 = ws(sh(a3, 2), a5);
+    o[0] = wa(b0, b7);
+    o[1] = wa(b2, b5);
+    o[2] = wa(b4, b3);
+    o[3] = wa(b6, b1);
+    o[4] = ws(b6, b1);
+    o[5] = ws(b4, b3);
+    o[6] = ws(b2, b5);
+    o[7] = ws(b0, b7);
+}
+
+// residual8_scalar with narrow8 and idct8_add_scalar / dc_add_scalar.
+__device__ void residual8(uint8_t* dst, int pitch, const int32_t* co) {
+    int16_t d[64];
+    int ac = 0;
+    for (int i = 0; i < 64; ++i) {
+        int v = co[i];
+        d[i] = (int16_t)clampi(v, -32768, 32767);
+        if (i) ac |= v;
+    }
+    if (ac == 0) {
+        if (d[0] == 0) return;
+        int v = (int)sh(wa(d[0], 32), 6);
+        for (int y = 0; y < 8; ++y)
+            for (int x = 0; x < 8; ++x) {
+                uint8_t* p = dst + y * pitch + x;
+                *p = (uint8_t)clampi(*p + v, 0, 255);
+            }
+        return;
+    }
+    int16_t t[64];
+    for (int i = 0; i < 8; ++i) idct8_1d(d + i * 8, t + i * 8);
+    for (int j = 0; j < 8; ++j) {
+        int16_t col[8], o[8];
+        for (int k = 0; k < 8; ++k) col[k] = t[k * 8 + j];
+        idct8_1d(col, o);
+        for (int i = 0; i < 8; ++i) add_px(dst + i * pitch + j, o[i]);
+    }
+}
+
+__global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
+    const int32_t* pre = b.prefix + 1 * (b.n + 1);
+    const int it = blockIdx.x;
+    const int p = find_pic(pre, b.n, it);
+    const H264PicDev& P = *b.h264[p];
+    const int i = (it - pre[p]) * 64 + threadIdx.x;
+    const int n4 = P.n_blks4;
+    const int items4 = (n4 + 63) / 64;
+    int k;
+    if (i < items4 * 64) {
+        k = i;
+        if (k >= n4) return;
+    } else {
+        k = n4 + (i - items4 * 64);
+        if (k >= n4 + P.n_blks8) return;
+    }
+    const rv_h264_blk bk = P.blks[k];
+    uint8_t* dst = (uint8_t*)P.cur[bk.comp] + (size_t)bk.y * P.pitch[bk.comp] + bk.x;
+    if (bk.size == 8) residual8(dst, P.pitch[bk.comp], P.coefs + bk.coef_off);
+    else residual4(dst, P.pitch[bk.comp], P.coefs + bk.coef_off, bk.has_dc, bk.dc);
+}
+
+// ---------------------------------------------------------------------
+// Stage 2: boundary strengths (src/h264/deblock.rs:97-146, 255-282, 640-766)
+// ---------------------------------------------------------------------
+struct Mot {
+    int n;
+    int ref[2], mv[2][2];
+};
+
+__device__ __forceinline__ Mot mot_of(const H264PicDev& P, int bx, int by) {
+    uint32_t pi = P.partmap[(size_t)by * P.w4 + bx];
+    Mot m;
+    m.n = 0;
+    m.ref[0] = m.ref[1] = -1;
+    m.mv[0][0] = m.mv[0][1] = m.mv[1][0] = m.mv[1][1] = 0;
+    if (pi >= (uint32_t)P.n_parts) return m;
+    const rv_h264_part pt = P.parts[pi];
+    for (int l = 0; l < 2; ++l)
+        if ((pt.dir >> l) & 1) {
+            m.ref[l] = P.ref_id[l][pt.ref_idx[l]];
+            m.mv[l][0] = pt.mv[l].x;
+            m.mv[l][1] = pt.mv[l].y;
+            ++m.n;
+        }
+    return m;
+}
+
+__device__ __forceinline__ bool far4(const int* a, const int* b) {
+    return abs(a[0] - b[0]) >= 4 || abs(a[1] - b[1]) >= 4;
+}
+
+// motion_bs (:97-146), frame macroblocks.
+__device__ int motion_bs(const Mot& p, const Mot& q) {
+    if (p.n != q.n) return 1;
+    if (p.n == 1) {
+        int lp = p.ref[0] >= 0 ? 0 : 1, lq = q.ref[0] >= 0 ? 0 : 1;
+        if (p.ref[lp] != q.ref[lq]) return 1;
+        return far4(p.mv[lp], q.mv[lq]);
+    }
+    bool straight = p.ref[0] == q.ref[0] && p.ref[1] == q.ref[1];
+    bool crossed = p.ref[0] == q.ref[1] && p.ref[1] == q.ref[0];
+    if (!straight && !crossed) return 1;
+    if (p.ref[0] == p.ref[1]) {
+        bool a = far4(p.mv[0], q.mv[0]) || far4(p.mv[1], q.mv[1]);
+        bool b = far4(p.mv[0], q.mv[1]) || far4(p.mv[1], q.mv[0]);
+        return a && b;
+    }
+    if (straight) return far4(p.mv[0], q.mv[0]) || far4(p.mv[1], q.mv[1]);
+    return far4(p.mv[0], q.mv[1]) || far4(p.mv[1], q.mv[0]);
+}
+
+__device__ __forceinline__ uint32_t partof(const H264PicDev& P, int bx, int by) {
+    return P.partmap[(size_t)by * P.w4 + bx];
+}
+
+__device__ void mb_bs(const H264PicDev& P, int addr) {
+    const int mbx = addr % P.mbw, mby = addr / P.mbw;
+    const rv_h264_mb m = P.mbs[addr];
+    uint32_t bv[4] = {0, 0, 0, 0}, bh[4] = {0, 0, 0, 0};
+    const bool fl = mbx > 0, ft = mby > 0;
+    const int bx0 = mbx * 4, by0 = mby * 4;
+    if (m.intra) {
+        if (fl) bv[0] = 0x04040404u;
+        if (ft) bh[0] = 0x04040404u;
+        for (int e = 1; e < 4; ++e) bv[e] = bh[e] = 0x03030303u;
+    } else {
+        const uint32_t nz = m.nz_mask;
+        const int step = m.transform_8x8 ? 2 : 1;
+        // Internal edges: a coefficient on either side settles bS 2; else
+        // motion, only across a partition boundary.
+        for (int e = step; e < 4; e += step)
+            for (int k = 0; k < 4; ++k) {
+                // Vertical edge e, block row k.
+                int pb = k * 4 + e - 1, qb = k * 4 + e;
+                uint32_t v;
+                if (((nz >> pb) | (nz >> qb)) & 1) v = 2;
+                else if (partof(P, bx0 + e - 1, by0 + k) != partof(P, bx0 + e, by0 + k))
+                    v = motion_bs(mot_of(P, bx0 + e - 1, by0 + k), mot_of(P, bx0 + e, by0 + k));
+                else v = 0;
+                bv[e] |= v << (8 * k);
+                // Horizontal edge e, block column k.
+                pb = (e - 1) * 4 + k;
+                qb = e * 4 + k;
+                if (((nz >> pb) | (nz >> qb)) & 1) v = 2;
+                else if (partof(P, bx0 + k, by0 + e - 1) != partof(P, bx0 + k, by0 + e))
+                    v = motion_bs(mot_of(P, bx0 + k, by0 + e - 1), mot_of(P, bx0 + k, by0 + e));
+                else v = 0;
+                bh[e] |= v << (8 * k);
+            }
+        if (fl) {
+            const rv_h264_mb ml = P.mbs[addr - 1];
+            if (ml.intra) bv[0] = 0x04040404u;
+            else
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t v;
+                    if (((ml.nz_mask >> (k * 4 + 3)) | (nz >> (k * 4))) & 1) v = 2;
+                    else v = motion_bs(mot_of(P, bx0 - 1, by0 + k), mot_of(P, bx0, by0 + k));
+                    bv[0] |= v << (8 * k);
+                }
+        }
+        if (ft) {
+            const rv_h264_mb ma = P.mbs[addr - P.mbw];
+            if (ma.intra) bh[0] = 0x04040404u;
+            else
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t v;
+                    if (((ma.nz_mask >> (12 + k)) | (nz >> k)) & 1) v = 2;
+                    else v = motion_bs(mot_of(P, bx0 + k, by0 - 1), mot_of(P, bx0 + k, by0));
+                    bh[0] |= v << (8 * k);
+                }
+        }
+    }
+    for (int e = 0; e < 4; ++e) {
+        P.bsv[addr * 4 + e] = bv[e];
+        P.bsh[addr * 4 + e] = bh[e];
+    }
+}
+
+__global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
+    const int32_t* pre = b.prefix + 2 * (b.n + 1);
+    const int it = blockIdx.x;
+    const int p = find_pic(pre, b.n, it);
+    const H264PicDev& P = *b.h264[p];
+    const int i = (it - pre[p]) * 64 + threadIdx.x;
+    // Wavefront counters: ticket (entry 0) and per-row progress (1..mbh).
+    if (i <= P.mbh) P.bsv[(size_t)P.n_mbs * 4 + i] = 0;
+    if (i < P.n_mbs) mb_bs(P, i);
+}
+
+// ---------------------------------------------------------------------
+// Stage 3: deblocking wavefront (src/h264/deblock.rs:567-815)
+// ---------------------------------------------------------------------
+
+// deblock_line (src/dsp/h264.rs:603-656) on one line, in place through
+// `d` (q0) with `across` crossing the edge.
+__device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alpha, int beta, bool chroma) {
+    int p0 = d[-across], p1 = d[-2 * across], q0 = d[0], q1 = d[across];
+    if (!(abs(p0 - q0) < alpha && abs(p1 - p0) < beta && abs(q1 - q0) < beta)) return;
+    if (chroma) {
+        if (!bs4) {
+            int tc = tc0 + 1;
+            int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+            d[-across] = (uint8_t)clampi(p0 + delta, 0, 255);
+            d[0] = (uint8_t)clampi(q0 - delta, 0, 255);
+        } else {
+            d[-across] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+            d[0] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+        }
+        return;
+    }
+    int p2 = d[-3 * across], q2 = d[2 * across];
+    int ap = abs(p2 - p0), aq = abs(q2 - q0);
+    if (!bs4) {
+        int tc = tc0 + (ap < beta) + (aq < beta);
+        int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        d[-across] = (uint8_t)clampi(p0 + delta, 0, 255);
+        d[0] = (uint8_t)clampi(q0 - delta, 0, 255);
+        if (ap < beta) d[-2 * across] = (uint8_t)(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0));
+        if (aq < beta) d[across] = (uint8_t)(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0));
+    } else {
+        int p3 = d[-4 * across], q3 = d[3 * across];
+        bool strong = abs(p0 - q0) < ((alpha >> 2) + 2);
+        if (ap < beta && strong) {
+            d[-across] = (uint8_t)((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
+            d[-2 * across] = (uint8_t)((p2 + p1 + p0 + q0 + 2) >> 2);
+            d[-3 * across] = (uint8_t)((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
+        } else {
+            d[-across] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+        }
+        if (aq < beta && strong) {
+            d[0] = (uint8_t)((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
+            d[across] = (uint8_t)((p0 + q0 + q1 + q2 + 2) >> 2);
+            d[2 * across] = (uint8_t)((2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3);
+        } else {
+            d[0] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+        }
+    }
+}
+
+// One line of one edge: `seg` is the line's segment (bS byte), `qa` the
+// QP average of the edge (ThrTable::get, :311-344).
+__device__ __forceinline__ void edge_line(const H264PicDev& P, uint8_t* d, long across, uint32_t bs, int seg, int qa,
+                                          bool chroma) {
+    if (bs == 0) return;
+    const int ia = clampi(qa + P.offset_a, 0, 51), ib = clampi(qa + P.offset_b, 0, 51);
+    const int alpha = kAlpha[ia], beta = kBeta[ib];
+    const bool bs4 = (bs & 0xFF) == 4;  // byte 0 decides (filter_edge, :192-213)
+    int tc0 = 0;
+    if (!bs4) {
+        int s = (bs >> (8 * seg)) & 3;
+        if (s == 0) return;  // tC0 = -1: leave the line alone
+        tc0 = kTc0[ia][s - 1];
+    }
+    line_filter(d, across, bs4, tc0, alpha, beta, chroma);
+}
+
+__device__ void deblock_mb(const H264PicDev& P, int mbx, int mby, int tid) {
+    const int addr = mby * P.mbw + mbx;
+    const rv_h264_mb m = P.mbs[addr];
+    const rv_h264_mb ml = P.mbs[mbx > 0 ? addr - 1 : addr];
+    const rv_h264_mb ma = P.mbs[mby > 0 ? addr - P.mbw : addr];
+    const uint32_t* bv = P.bsv + addr * 4;
+    const uint32_t* bh = P.bsh + addr * 4;
+    const bool odd = !m.transform_8x8;
+    // Luma: lanes 0..15 are the lines; chroma: 16..23 Cb, 24..31 Cr.
+    if (tid < 16) {
+        uint8_t* Y = (uint8_t*)P.cur[0];
+        const int pitch = P.pitch[0];
+        uint8_t* row = Y + (size_t)(mby * 16 + tid) * pitch + mbx * 16;
+        for (int e = 0; e < 4; ++e) {
+            if (e > 0 && (e & 1) && !odd) continue;
+            int qa = e == 0 ? ((ml.qp + m.qp + 1) >> 1) : m.qp;
+            edge_line(P, row + e * 4, 1, bv[e], tid >> 2, qa, false);
+        }
+    } else if (tid < 32) {
+        const int c = tid < 24 ? 1 : 2, line = tid & 7;
+        uint8_t* pl = (uint8_t*)P.cur[c];
+        uint8_t* row = pl + (size_t)(mby * 8 + line) * P.pitch[c] + mbx * 8;
+        int qa0 = (ml.qpc[c - 1] + m.qpc[c - 1] + 1) >> 1;
+        edge_line(P, row, 1, bv[0], line >> 1, qa0, true);
+        edge_line(P, row + 4, 1, bv[2], line >> 1, m.qpc[c - 1], true);
+    }
+    __syncthreads();
+    if (tid < 16) {
+        uint8_t* Y = (uint8_t*)P.cur[0];
+        const long pitch = P.pitch[0];
+        uint8_t* col = Y + (size_t)(mby * 16) * pitch + mbx * 16 + tid;
+        for (int e = 0; e < 4; ++e) {
+            if (e > 0 && (e & 1) && !odd) continue;
+            int qa = e == 0 ? ((ma.qp + m.qp + 1) >> 1) : m.qp;
+            edge_line(P, col + e * 4 * pitch, pitch, bh[e], tid >> 2, qa, false);
+        }
+    } else if (tid < 32) {
+        const int c = tid < 24 ? 1 : 2, line = tid & 7;
+        uint8_t* pl = (uint8_t*)P.cur[c];
+        const long pitch = P.pitch[c];
+        uint8_t* col = pl + (size_t)(mby * 8) * pitch + mbx * 8 + line;
+        int qa0 = (ma.qpc[c - 1] + m.qpc[c - 1] + 1) >> 1;
+        edge_line(P, col, pitch, bh[0], line >> 1, qa0, true);
+        edge_line(P, col + 4 * pitch, pitch, bh[2], line >> 1, m.qpc[c - 1], true);
+    }
+    __syncthreads();
+}
+
+// Spin bound: far beyond any real wait (a row waits for one macroblock of
+// the row above); a run that reaches it abandons the row instead of hanging.
+constexpr long kSpinLimit = 1L << 26;
+
+__global__ void __launch_bounds__(64) h264_stage3(BatchDev b) {
+    const int32_t* pre = b.prefix + 3 * (b.n + 1);
+    const int it = blockIdx.x;
+    const int p = find_pic(pre, b.n, it);
+    const H264PicDev& P = *b.h264[p];
+    if (!P.deblock) return;
+    const int tid = threadIdx.x;
+    uint32_t* ctr = P.bsv + (size_t)P.n_mbs * 4;  // [0] ticket, [1 + y] progress
+    __shared__ int row_s;
+    if (tid == 0) row_s = (int)atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    const int y = row_s;
+    if (y >= P.mbh) return;
+    __shared__ int ok;
+    for (int x = 0; x < P.mbw; ++x) {
+        if (y > 0) {
+            const uint32_t need = (uint32_t)min(x + 2, P.mbw);
+            if (tid == 0) {
+                long spins = 0;
+                while (__hip_atomic_load(&ctr[y], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need &&
+                       ++spins < kSpinLimit)
+                    __builtin_amdgcn_s_sleep(2);
+                ok = spins < kSpinLimit;
+            }
+            __syncthreads();
+            if (!ok) return;
+            __threadfence();
+        }
+        deblock_mb(P, x, y, tid);
+        if (tid == 0) {
+            __threadfence();
+            __hip_atomic_store(&ctr[1 + y], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+}  // namespace
+
+void h264_stage_items(H264PicDev& P) {
+    P.items[0] = P.n_mbs + P.n_parts;
+    P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
+    P.items[2] = (P.n_mbs + 1 + 63) / 64;
+    P.items[3] = P.deblock ? P.mbh : 0;
+    P.items[4] = 0;
+}
+
+hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    switch (stage) {
+        case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
+        case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
+        case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
+        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(64), 0, st, b); break;
+        default: return hipSuccess;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace rv

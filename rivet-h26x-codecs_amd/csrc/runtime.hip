@@ -206,6 +206,95 @@ void hevc_bytes(const rv_hevc_pic* p, int64_t out[4]) {
     out[3] = 2 * s * planes;
 }
 
+// implicit_pair (src/h264/recon.rs:214-233): weights at logWD 5 from POC
+// distances; (32, 32) for long-term references, equal POCs or a ratio out
+// of range.
+void implicit_weights(int cur, int poc0, int poc1, bool lt0, bool lt1, int* w0, int* w1) {
+    int tb = std::min(std::max(cur - poc0, -128), 127);
+    int td = std::min(std::max(poc1 - poc0, -128), 127);
+    *w0 = *w1 = 32;
+    if (td == 0 || lt0 || lt1) return;
+    int tx = (16384 + std::abs(td / 2)) / td;
+    int dsf = std::min(std::max((tb * tx + 32) >> 6, -1024), 1023);
+    int v1 = dsf >> 2;
+    if (v1 < -64 || v1 > 128) return;
+    *w0 = 64 - v1;
+    *w1 = v1;
+}
+
+int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
+    if (p->width != s->w || p->height != s->h || p->bit_depth != 8)
+        return fail(RV_E_RANGE, "picture shape differs from the session");
+    const int W = p->width, H = p->height, mbw = W / 16, mbh = H / 16;
+    if (p->n_mbs != mbw * mbh || !p->mbs) return fail(RV_E_RANGE, "n_mbs must cover the picture");
+    for (int l = 0; l < 2; ++l) {
+        if (p->num_ref[l] < 0 || p->num_ref[l] > RV_MAX_REFS) return fail(RV_E_RANGE, "num_ref outside 0..4");
+        for (int r = 0; r < p->num_ref[l]; ++r) {
+            if (p->ref_slot[l][r] < 0 || p->ref_slot[l][r] >= s->slots) return fail(RV_E_RANGE, "ref_slot outside the DPB");
+            if (p->ref_slot[l][r] == out_slot) return fail(RV_E_RANGE, "out_slot is also a reference");
+        }
+    }
+    if (p->weighting < RV_H264_DEFAULT || p->weighting > RV_H264_IMPLICIT) return fail(RV_E_INVALID, "weighting");
+    if (p->weighting == RV_H264_EXPLICIT)
+        for (int c = 0; c < 3; ++c)
+            if (p->log_wd[c] < 0 || p->log_wd[c] > 7) return fail(RV_E_RANGE, "log_wd outside 0..7");
+    if (p->n_parts < 0 || p->n_blks < 0 || (p->n_parts && !p->parts) || (p->n_blks && !p->blks) ||
+        (p->n_coefs && !p->coefs) || (p->n_intra_pred && !p->intra_pred))
+        return fail(RV_E_INVALID, "null work list");
+    int64_t area = 0;
+    for (int i = 0; i < p->n_mbs; ++i) {
+        const rv_h264_mb& m = p->mbs[i];
+        if (m.qp < 0 || m.qp > 51 || m.qpc[0] < 0 || m.qpc[0] > 51 || m.qpc[1] < 0 || m.qpc[1] > 51)
+            return fail(RV_E_RANGE, "QP outside 0..51");
+        if (m.intra) {
+            if ((uint64_t)m.pred_off + 384 > p->n_intra_pred) return fail(RV_E_RANGE, "intra prediction outside intra_pred");
+            area += 256;
+        }
+    }
+    for (int i = 0; i < p->n_parts; ++i) {
+        const rv_h264_part& t = p->parts[i];
+        if ((t.w != 4 && t.w != 8 && t.w != 16) || (t.h != 4 && t.h != 8 && t.h != 16) || (t.x & 3) || (t.y & 3) ||
+            t.x + t.w > W || t.y + t.h > H)
+            return fail(RV_E_RANGE, "partition outside the picture or of bad size");
+        if ((t.x >> 4) != ((t.x + t.w - 1) >> 4) || (t.y >> 4) != ((t.y + t.h - 1) >> 4))
+            return fail(RV_E_RANGE, "partition crosses a macroblock");
+        if (p->mbs[(t.y >> 4) * mbw + (t.x >> 4)].intra) return fail(RV_E_RANGE, "partition inside an intra macroblock");
+        if (t.dir < 1 || t.dir > 3) return fail(RV_E_RANGE, "partition without a list");
+        for (int l = 0; l < 2; ++l)
+            if ((t.dir >> l) & 1)
+                if (t.ref_idx[l] < 0 || t.ref_idx[l] >= p->num_ref[l]) return fail(RV_E_RANGE, "ref_idx outside the list");
+        area += (int64_t)t.w * t.h;
+    }
+    if (area != (int64_t)W * H) return fail(RV_E_RANGE, "partitions and intra macroblocks do not tile the picture");
+    for (int i = 0; i < p->n_blks; ++i) {
+        const rv_h264_blk& b = p->blks[i];
+        int pw = b.comp ? W / 2 : W, ph = b.comp ? H / 2 : H;
+        if (b.comp > 2 || (b.size != 4 && b.size != 8) || (b.size == 8 && b.comp != 0) || (b.x & 3) || (b.y & 3) ||
+            b.x + b.size > pw || b.y + b.size > ph)
+            return fail(RV_E_RANGE, "residual block outside its plane or of bad size");
+        if ((uint64_t)b.coef_off + (uint64_t)b.size * b.size > p->n_coefs) return fail(RV_E_RANGE, "coefficients outside coefs");
+    }
+    return RV_OK;
+}
+
+void h264_bytes(const rv_h264_pic* p, int64_t out[4]) {
+    int64_t mc = 0;
+    for (int i = 0; i < p->n_parts; ++i) {
+        const rv_h264_part& t = p->parts[i];
+        for (int l = 0; l < 2; ++l)
+            if ((t.dir >> l) & 1) mc += (int64_t)(t.w + 5) * (t.h + 5) + 2 * (int64_t)(t.w / 2 + 1) * (t.h / 2 + 1);
+    }
+    int64_t planes = (int64_t)p->width * p->height * 3 / 2;
+    for (int i = 0; i < p->n_mbs; ++i)
+        if (p->mbs[i].intra) mc += 384 * 2;
+    out[0] = mc + planes;
+    int64_t coef = 0;
+    for (int i = 0; i < p->n_blks; ++i) coef += (int64_t)p->blks[i].size * p->blks[i].size;
+    out[1] = 4 * coef + 2 * coef;
+    out[2] = p->deblock ? 2 * planes : 0;
+    out[3] = 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -523,11 +612,131 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
 }
 
 int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** out) {
-    if (out) *out = nullptr;
-    (void)s;
-    (void)p;
-    (void)out_slot;
-    return fail(RV_E_INVALID, "H.264 jobs are not available in this build");
+    try {
+        if (!s || !p || !out) return fail(RV_E_INVALID, "null argument");
+        *out = nullptr;
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        if (s->codec != RV_CODEC_H264) return fail(RV_E_INVALID, "not an H.264 session");
+        if (out_slot < 0 || out_slot >= s->slots) return fail(RV_E_RANGE, "out_slot outside the DPB");
+        int rc = validate_h264(s, p, out_slot);
+        if (rc) return rc;
+        const int W = p->width, H = p->height, mbw = W / 16, mbh = H / 16;
+        std::vector<rv_h264_blk> b4, b8;
+        for (int i = 0; i < p->n_blks; ++i) (p->blks[i].size == 8 ? b8 : b4).push_back(p->blks[i]);
+        std::vector<rv_h264_blk> blks(b4);
+        blks.insert(blks.end(), b8.begin(), b8.end());
+        rv_job* j = new rv_job();
+        j->s = s;
+        j->codec = RV_CODEC_H264;
+        j->out_slot = out_slot;
+        h264_bytes(p, j->bytes);
+        size_t sz = carve_size<rv::H264PicDev>(1) + carve_size<rv_h264_mb>(p->n_mbs) +
+                    carve_size<rv_h264_part>(p->n_parts) + carve_size<rv_h264_blk>(blks.size()) +
+                    carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
+                    carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
+        size_t n4 = (size_t)(W / 4) * (H / 4), nmb = (size_t)mbw * mbh;
+        size_t scr = carve_size<uint32_t>(n4) + carve_size<uint32_t>(nmb * 4 + mbh + 1) + carve_size<uint32_t>(nmb * 4) + 1024;
+        if (hipMalloc(&j->arena, sz) != hipSuccess || hipMalloc(&j->scratch, scr) != hipSuccess) {
+            if (j->arena) (void)hipFree(j->arena);
+            delete j;
+            return fail(RV_E_NOMEM, "job allocation failed");
+        }
+        std::vector<char> host(sz, 0);
+        char* hp = host.data();
+        char* base = (char*)j->arena;
+        auto dev = [&](char* hptr) { return base + (hptr - host.data()); };
+        rv::H264PicDev* hdesc = carve<rv::H264PicDev>(hp, 1);
+        rv_h264_mb* h_mbs = carve<rv_h264_mb>(hp, p->n_mbs);
+        rv_h264_part* h_parts = carve<rv_h264_part>(hp, p->n_parts);
+        rv_h264_blk* h_blks = carve<rv_h264_blk>(hp, blks.size());
+        int32_t* h_coefs = carve<int32_t>(hp, p->n_coefs);
+        uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
+        const rv::H264PicDev** h_ptr = carve<const rv::H264PicDev*>(hp, 1);
+        int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
+        std::memcpy(h_mbs, p->mbs, p->n_mbs * sizeof(rv_h264_mb));
+        if (p->n_parts) std::memcpy(h_parts, p->parts, p->n_parts * sizeof(rv_h264_part));
+        if (!blks.empty()) std::memcpy(h_blks, blks.data(), blks.size() * sizeof(rv_h264_blk));
+        if (p->n_coefs) std::memcpy(h_coefs, p->coefs, p->n_coefs * sizeof(int32_t));
+        if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
+        rv::H264PicDev& d = j->h4;
+        std::memset(&d, 0, sizeof(d));
+        d.w = W;
+        d.h = H;
+        d.mbw = mbw;
+        d.mbh = mbh;
+        d.w4 = W / 4;
+        d.h4 = H / 4;
+        for (int c = 0; c < 3; ++c) {
+            d.pitch[c] = s->g.pitch[c];
+            d.pw[c] = s->g.pw[c];
+            d.ph[c] = s->g.ph[c];
+            // Reconstructed and deblocked in place in the output slot.
+            d.cur[c] = d.out[c] = s->planes[(size_t)out_slot * 3 + c];
+        }
+        for (int l = 0; l < 2; ++l)
+            for (int r = 0; r < RV_MAX_REFS; ++r) {
+                int slot = r < p->num_ref[l] ? p->ref_slot[l][r] : 0;
+                for (int c = 0; c < 3; ++c) d.ref[l][r][c] = s->planes[(size_t)slot * 3 + c];
+                d.ref_id[l][r] = r < p->num_ref[l] ? slot : -1;
+                for (int c = 0; c < 3; ++c) {
+                    d.wp_w[l][r][c] = p->wp_w[l][r][c];
+                    d.wp_o[l][r][c] = p->wp_o[l][r][c];
+                }
+            }
+        d.weighting = p->weighting;
+        for (int c = 0; c < 3; ++c) d.log_wd[c] = p->log_wd[c];
+        for (int r0 = 0; r0 < RV_MAX_REFS; ++r0)
+            for (int r1 = 0; r1 < RV_MAX_REFS; ++r1) {
+                int w0 = 32, w1 = 32;
+                if (r0 < p->num_ref[0] && r1 < p->num_ref[1])
+                    implicit_weights(p->poc, p->ref_poc[0][r0], p->ref_poc[1][r1], p->ref_long_term[0][r0] != 0,
+                                     p->ref_long_term[1][r1] != 0, &w0, &w1);
+                d.imp_w[r0][r1][0] = w0;
+                d.imp_w[r0][r1][1] = w1;
+            }
+        d.deblock = (g_stage_mask & 1) ? 0 : p->deblock;
+        d.offset_a = p->offset_a;
+        d.offset_b = p->offset_b;
+        d.n_mbs = p->n_mbs;
+        d.n_parts = p->n_parts;
+        d.n_blks4 = (int)b4.size();
+        d.n_blks8 = (int)b8.size();
+        d.mbs = (const rv_h264_mb*)dev((char*)h_mbs);
+        d.parts = (const rv_h264_part*)dev((char*)h_parts);
+        d.blks = (const rv_h264_blk*)dev((char*)h_blks);
+        d.coefs = (const int32_t*)dev((char*)h_coefs);
+        d.intra_pred = (const uint16_t*)dev((char*)h_ip);
+        char* sp = (char*)j->scratch;
+        d.partmap = (uint32_t*)sp;
+        sp += carve_size<uint32_t>(n4);
+        d.bsv = (uint32_t*)sp;  // nmb * 4 strengths, then the wavefront counters
+        sp += carve_size<uint32_t>(nmb * 4 + mbh + 1);
+        d.bsh = (uint32_t*)sp;
+        rv::h264_stage_items(d);
+        *hdesc = d;
+        j->desc_dev = dev((char*)hdesc);
+        *h_ptr = (const rv::H264PicDev*)j->desc_dev;
+        for (int st = 0; st < rv::kStages; ++st) {
+            h_pre[st * 2] = 0;
+            h_pre[st * 2 + 1] = d.items[st];
+        }
+        j->single.n = 1;
+        j->single.hevc = nullptr;
+        j->single.h264 = (const rv::H264PicDev* const*)dev((char*)h_ptr);
+        j->single.prefix = (const int32_t*)dev((char*)h_pre);
+        hipError_t e = hipMemcpy(j->arena, host.data(), sz, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(j->scratch, 0, scr);
+        if (e != hipSuccess) {
+            (void)hipFree(j->arena);
+            (void)hipFree(j->scratch);
+            delete j;
+            return hip_fail(e, "job upload");
+        }
+        *out = j;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_h264_prepare");
+    }
 }
 
 int rv_job_free(rv_job* j) {
@@ -663,6 +872,44 @@ int rv_batch_launch(rv_batch* b, int mode) {
         return fail(RV_E_INVALID, "unknown launch mode");
     } catch (...) {
         return fail(RV_E_HIP, "exception in rv_batch_launch");
+    }
+}
+
+int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
+    try {
+        if (!b || !out_ms || reps <= 0) return fail(RV_E_INVALID, "bad argument");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipEvent_t ev[rv::kStages + 1];
+        for (auto& e : ev) RV_HIP(hipEventCreate(&e));
+        double acc[rv::kStages] = {0, 0, 0, 0, 0};
+        int rc = RV_OK;
+        for (int r = 0; r < reps && rc == RV_OK; ++r) {
+            RV_HIP(hipEventRecord(ev[0], g_stream));
+            for (int s = 0; s < rv::kStages && rc == RV_OK; ++s) {
+                int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
+                if (nh) {
+                    hipError_t e = rv::hevc_launch_stage(s, b->bh, b->pre_h[(size_t)s * (nh + 1) + nh], g_stream);
+                    if (e != hipSuccess) rc = hip_fail(e, "kernel launch");
+                }
+                if (n4 && rc == RV_OK) {
+                    hipError_t e = rv::h264_launch_stage(s, b->b4, b->pre_4[(size_t)s * (n4 + 1) + n4], g_stream);
+                    if (e != hipSuccess) rc = hip_fail(e, "kernel launch");
+                }
+                RV_HIP(hipEventRecord(ev[s + 1], g_stream));
+            }
+            RV_HIP(hipEventSynchronize(ev[rv::kStages]));
+            for (int s = 0; s < rv::kStages; ++s) {
+                float ms = 0;
+                RV_HIP(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+                acc[s] += ms;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int s = 0; s < rv::kStages; ++s) out_ms[s] = (float)(acc[s] / reps);
+        return rc;
+    } catch (...) {
+        return fail(RV_E_HIP, "exception in rv_batch_profile");
     }
 }
 

@@ -201,6 +201,12 @@ class Batch:
     def launch(self, mode: int = RV_LAUNCH_BATCHED) -> None:
         check(lib.rv_batch_launch(self._h, mode))
 
+    def profile(self, reps: int = 10):
+        """Average milliseconds of each of the five stages (HIP events)."""
+        out = (C.c_float * 5)()
+        check(lib.rv_batch_profile(self._h, reps, out))
+        return list(out)
+
     def close(self) -> None:
         if self._h and self._h.value:
             check(lib.rv_batch_free(self._h))

@@ -134,6 +134,7 @@ PROTOTYPES = {
     "rv_batch_free": (i32, [P]),
     "rv_batch_launch": (i32, [P, i32]),
     "rv_sync": (i32, []),
+    "rv_batch_profile": (i32, [P, i32, C.POINTER(C.c_float)]),
     "rv_mark": (i32, [C.POINTER(i32)]),
     "rv_mark_query": (i32, [i32]),
     "rv_mark_wait": (i32, [i32]),
