@@ -88,6 +88,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--streams", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: one per picture, <= 16)")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="run only the 256 MiB copy probe once (counter calibration) and exit")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path; "
@@ -99,6 +103,10 @@ def main() -> int:
     from rivet_hip import configs as cf
 
     rh.init(0)
+    if args.probe_only:
+        gbs = rh.probe_copy_gbs(256 << 20, 1)
+        log(f"[bench] copy probe: 256 MiB read + 256 MiB written, {gbs:.1f} GB/s")
+        return 0
     log(f"[bench] tier {rh.tier_name()}; generating {args.streams} x 3840x2160 pictures")
     pics, refs, sessions, jobs = build_workload(rh, cf, args.streams)
     batch = rh.Batch(jobs)
@@ -126,12 +134,18 @@ def main() -> int:
 
     # Timed region: GRAPH replay of the batched launches (inputs resident).
     t_graph = time_steps(rh, batch, rh.RV_LAUNCH_GRAPH, args.steps, args.warmup)
-    t_batched = time_steps(rh, batch, rh.RV_LAUNCH_BATCHED, max(10, args.steps // 4), 3)
-    t_plain = time_steps(rh, batch, rh.RV_LAUNCH_PLAIN, max(10, args.steps // 4), 3)
-    stage_ms = batch.profile(max(10, args.steps // 4))
+    if args.graph_only:
+        t_batched = t_plain = t_graph
+        stage_ms = batch.profile(2)
+    else:
+        t_batched = time_steps(rh, batch, rh.RV_LAUNCH_BATCHED, max(10, args.steps // 4), 3)
+        t_plain = time_steps(rh, batch, rh.RV_LAUNCH_PLAIN, max(10, args.steps // 4), 3)
+        stage_ms = batch.profile(max(10, args.steps // 4))
     log(f"[bench] step {t_graph * 1e3:.3f} ms (graph), {t_batched * 1e3:.3f} (batched), {t_plain * 1e3:.3f} (plain); "
         f"stages ms {['%.3f' % v for v in stage_ms]}")
 
+    copy_gbs = rh.probe_copy_gbs(1 << 30, 10)
+    log(f"[bench] achievable copy bandwidth: {copy_gbs:.1f} GB/s")
     nb = {"mc": 0, "tx": 0, "dbk": 0, "sao": 0}
     for j in jobs:
         for k, v in j.bytes().items():
@@ -150,6 +164,7 @@ def main() -> int:
         "traffic": None,
         "algorithmic_bytes": int(stage_bytes[dom]),
         "avg_launch_ms": round(stage_ms[dom], 4),
+        "achievable_copy_gbs": round(copy_gbs, 1),
     }
     cpu_value = luma / cpu_wall / 1e6
     out = {

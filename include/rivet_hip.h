@@ -305,6 +305,12 @@ int rv_set_stage_mask(int mask);
  * reporting): mc, tx, dbk, sao. */
 int rv_job_bytes(const rv_job* j, int64_t out[4]);
 
+/* Achievable device-memory bandwidth: a plain 16-byte-vector copy kernel
+ * (rv_copy_probe) over `bytes` bytes, `reps` times; writes (read + write
+ * bytes) / average kernel time in GB/s. Also the known-size kernel that
+ * calibrates hardware traffic counters. */
+int rv_probe_copy(int64_t bytes, int reps, double* gbs);
+
 /* sizeof of the work-list structs, for bindings to check their layout:
  * 0 rv_hevc_cu, 1 rv_hevc_pu, 2 rv_hevc_tu, 3 rv_sao, 4 rv_hevc_pic,
  * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic;

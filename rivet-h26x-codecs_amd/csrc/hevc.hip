@@ -13,6 +13,8 @@
 // function cites what it follows.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rv_device.h"
 #include "rv_kernels.h"
 
@@ -37,23 +39,8 @@ __constant__ uint8_t kTc[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 
 __constant__ uint8_t kBeta[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  6,  7,
                                   8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24, 26, 28, 30, 32,
                                   34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64};
-// The 32 distinct magnitudes of the 8.6.4.2 matrix (its first column).
-__constant__ int16_t kDctC[33] = {64, 90, 90, 90, 89, 88, 87, 85, 83, 82, 80, 78, 75, 73, 70, 67, 64,
-                                  61, 57, 54, 50, 46, 43, 38, 36, 31, 25, 22, 18, 13, 9,  4,  0};
-__constant__ int8_t kDst[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 __device__ __forceinline__ int sgn(int v) { return (v > 0) - (v < 0); }
-
-// Entry (k, n) of the 32-point matrix: +-c[m], m = (2n + 1) k mod 128
-// folded by the cosine symmetries.
-__device__ __forceinline__ int dct_entry(int k, int n) {
-    int a = ((2 * n + 1) * k) & 127;
-    if (a <= 32) return kDctC[a];
-    if (a <= 64) return -kDctC[64 - a];
-    if (a <= 96) return -kDctC[a - 64];
-    return kDctC[128 - a];
-}
 
 __device__ __forceinline__ int find_pic(const int32_t* pre, int n, int it) {
     int lo = 0, hi = n - 1;
@@ -84,32 +71,51 @@ __device__ void intra_copy(const HevcPicDev& P, const rv_hevc_cu& cu, int tid) {
     }
 }
 
-// One component of one list of an MC tile: the 14-bit prediction
-// (interp_scalar, src/dsp/hevc.rs:823-882) of up to NOUT outputs per lane.
-// The (w + taps - 1) x (h + taps - 1) window is staged in LDS from
-// coalesced row loads, coordinates clamped to the picture (the replicated
-// border of src/hevc/frame.rs:169-214 and the gather of
-// src/hevc/inter.rs:116-123 read the same samples).
-template <typename S, int TAPS, int WIN_W, int OUT_W, int NOUT>
-__device__ void mc_component(const S* __restrict__ ref, int pitch, int pw, int ph, int xi, int yi, int fx, int fy,
-                             int bw, int bh, int bd, int16_t* win, int16_t* tmp, int tid, int out[NOUT]) {
-    constexpr int REACH = TAPS / 2 - 1;
-    const int ww = bw + TAPS - 1, hh = bh + TAPS - 1;
-    const int x0 = xi - REACH, y0 = yi - REACH;
-    // Stage the window: lanes cover WIN_W-wide row slots.
-    for (int e = tid; e < hh * WIN_W; e += 64) {
-        int r = e / WIN_W, c = e - r * WIN_W;
-        if (c < ww) {
-            int gx = clampi(x0 + c, 0, pw - 1), gy = clampi(y0 + r, 0, ph - 1);
-            win[r * WIN_W + c] = (int16_t)ref[(size_t)gy * pitch + gx];
+// Staged window of one component and list: (bw + TAPS - 1) x (bh + TAPS - 1)
+// samples at (xi - REACH, yi - REACH), coordinates clamped to the picture
+// (the replicated border of src/hevc/frame.rs:169-214 and the gather of
+// src/hevc/inter.rs:116-123 read the same samples). Lanes cover WIN_W-wide
+// row slots; NLD loads per lane, all issued before any is used.
+template <typename S, int TAPS, int WIN_W, int NLD>
+struct WinLoad {
+    int v[NLD];
+    __device__ __forceinline__ void issue(const S* __restrict__ ref, int pitch, int pw, int ph, int xi, int yi, int bw,
+                                          int bh, int tid) {
+        constexpr int REACH = TAPS / 2 - 1;
+        const int ww = bw + TAPS - 1, hh = bh + TAPS - 1;
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            int e = tid + 64 * k, r = e / WIN_W, c = e % WIN_W;
+            // Clamped, so always a valid address; slots past the window
+            // are staged but never read.
+            int gx = clampi(xi - REACH + c, 0, pw - 1), gy = clampi(yi - REACH + r, 0, ph - 1);
+            v[k] = (int)ref[(size_t)gy * pitch + gx];
+            (void)hh;
+            (void)ww;
         }
     }
-    __syncthreads();
+    __device__ __forceinline__ void store(int16_t* win, int tid) const {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            int e = tid + 64 * k;
+            if (e < (TAPS == 8 ? 23 : 11) * WIN_W) win[e] = (int16_t)v[k];
+        }
+    }
+};
+
+// The 14-bit prediction of one component and list from its staged window
+// (interp_scalar, src/dsp/hevc.rs:823-882), NOUT outputs per lane in
+// OUT_W-wide slots. Intermediates of the two-pass case are truncated to i16
+// as the reference's first stage stores them.
+template <int TAPS, int WIN_W, int OUT_W, int NOUT>
+__device__ void mc_predict(const int16_t* win, int16_t* tmp, int fx, int fy, int bw, int bh, int bd, int tid,
+                           int out[NOUT]) {
+    constexpr int REACH = TAPS / 2 - 1;
+    const int hh = bh + TAPS - 1;
     const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
     const int8_t* fxt = TAPS == 8 ? kQpel[fx] : kEpel[fx];
     const int8_t* fyt = TAPS == 8 ? kQpel[fy] : kEpel[fy];
     if (fx != 0 && fy != 0) {
-        // First stage over every window row, truncated to 14 bits (i16).
         for (int e = tid; e < hh * OUT_W; e += 64) {
             int r = e / OUT_W, c = e - r * OUT_W;
             if (c < bw) {
@@ -179,40 +185,140 @@ __device__ __forceinline__ int combine(const HevcPicDev& P, int c, int dir, cons
     return clampi((a * w0 + b * w1 + ((o0 + o1 + 1) << lwd)) >> (lwd + 1), 0, max);
 }
 
+typedef short short4v __attribute__((ext_vector_type(4)));
+
+// Luma 14-bit prediction of one list from its staged window (interp_scalar,
+// src/dsp/hevc.rs:823-882; 8-tap QPEL_FILTERS). Lane (x = tid & 15,
+// y0 = 4 * (tid >> 4)) produces rows y0..y0+3 of column x. The horizontal
+// pass reads window rows as 8-byte LDS vectors, four outputs per task, and
+// writes them transposed (tmpT[x * 24 + row]) so the vertical pass reads
+// its eleven inputs as three 8-byte vectors too. First-stage values are
+// truncated to i16 as the reference stores them.
+__device__ void luma_predict(const int16_t* win, int16_t* tmpT, int fx, int fy, int tw, int th, int bd, int tid,
+                             int out[4]) {
+    const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
+    if (fx != 0) {
+        int f[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = kQpel[fx][k];
+        const int r0 = fy ? 0 : 3, nr = fy ? th + 7 : th, ng = tw >> 2;
+        for (int e = tid; e < nr * 4; e += 64) {
+            const int r = e >> 2, g = e & 3;
+            if (g < ng) {
+                const int16_t* src = win + (r0 + r) * 32 + 4 * g;
+                const short4v a = *(const short4v*)src, b = *(const short4v*)(src + 4), c = *(const short4v*)(src + 8);
+                const int sv[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    int acc = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) acc += f[k] * sv[i + k];
+                    tmpT[(4 * g + i) * 24 + r] = (int16_t)(acc >> shift1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int x = tid & 15, y0 = (tid >> 4) * 4;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (x < tw && y0 < th) {
+        if (fx == 0 && fy == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[k] = (int16_t)((int)win[(y0 + 3 + k) * 32 + x + 3] << shift3);
+        } else if (fy == 0) {
+            const short4v a = *(const short4v*)(tmpT + x * 24 + y0);
+            out[0] = a.x;
+            out[1] = a.y;
+            out[2] = a.z;
+            out[3] = a.w;
+        } else {
+            int f[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) f[k] = kQpel[fy][k];
+            int sv[11];
+            int sh;
+            if (fx == 0) {
+#pragma unroll
+                for (int k = 0; k < 11; ++k) sv[k] = win[(y0 + k) * 32 + x + 3];
+                sh = shift1;
+            } else {
+                const int16_t* src = tmpT + x * 24 + y0;
+                const short4v a = *(const short4v*)src, b = *(const short4v*)(src + 4), c = *(const short4v*)(src + 8);
+                sv[0] = a.x, sv[1] = a.y, sv[2] = a.z, sv[3] = a.w, sv[4] = b.x, sv[5] = b.y;
+                sv[6] = b.z, sv[7] = b.w, sv[8] = c.x, sv[9] = c.y, sv[10] = c.z;
+                sh = 6;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc += f[k] * sv[i + k];
+                out[i] = (int16_t)(acc >> sh);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// LDS of one MC work item: luma windows per list, chroma windows per list
+// and component, and the first-stage rows of the two-pass case.
+struct McLds {
+    int16_t luma[2][23 * 32];
+    int16_t chroma[2][2][11 * 16];
+    int16_t tmp[16 * 24];
+};
+
 template <typename S>
-__device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, int16_t* win, int16_t* tmp) {
+__device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, McLds& L) {
     const rv_hevc_pu pu = P.pus[t.pu];
     const int x = pu.x + t.dx, y = pu.y + t.dy, tw = t.tw, th = t.th, bd = P.bd;
     const int dir = pu.dir;
-    // Luma: outputs in 16-wide slots, four per lane.
+    const int cw = tw >> 1, ch = th >> 1, cx = x >> 1, cy = y >> 1;
+    // Every window load of the item in flight at once.
+    WinLoad<S, 8, 32, 12> wl[2];
+    WinLoad<S, 4, 16, 3> wc[2][2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+        const int mvx = pu.mv[l].x, mvy = pu.mv[l].y;
+        wl[l].issue((const S*)P.ref[l][pu.ref_idx[l]][0], P.pitch[0], P.pw[0], P.ph[0], x + (mvx >> 2), y + (mvy >> 2),
+                    tw, th, tid);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            wc[l][c].issue((const S*)P.ref[l][pu.ref_idx[l]][c + 1], P.pitch[c + 1], P.pw[c + 1], P.ph[c + 1],
+                           cx + (mvx >> 3), cy + (mvy >> 3), cw, ch, tid);
+    }
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+        wl[l].store(L.luma[l], tid);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) wc[l][c].store(L.chroma[l][c], tid);
+    }
+    __syncthreads();
+    // Luma: lane (x, 4-row group), four outputs.
     {
         int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            const S* ref = (const S*)P.ref[l][pu.ref_idx[l]][0];
-            int mvx = pu.mv[l].x, mvy = pu.mv[l].y;
-            mc_component<S, 8, 32, 16, 4>(ref, P.pitch[0], P.pw[0], P.ph[0], x + (mvx >> 2), y + (mvy >> 2), mvx & 3,
-                                          mvy & 3, tw, th, bd, win, tmp, tid, pred[l]);
+            luma_predict(L.luma[l], L.tmp, pu.mv[l].x & 3, pu.mv[l].y & 3, tw, th, bd, tid, pred[l]);
         }
         S* dst = (S*)P.cur[0];
+        const int ox = tid & 15, oy0 = (tid >> 4) * 4;
+        if (ox < tw && oy0 < th) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
-            if (ox < tw && oy < th)
-                dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (S)combine(P, 0, dir, pu.ref_idx, pred[0][k], pred[1][k], bd);
+            for (int k = 0; k < 4; ++k)
+                dst[(size_t)(y + oy0 + k) * P.pitch[0] + x + ox] =
+                    (S)combine(P, 0, dir, pu.ref_idx, pred[0][k], pred[1][k], bd);
         }
     }
     // Chroma (4:2:0): eighth-sample vectors in chroma units
     // (src/hevc/inter.rs:581-594), outputs in 8-wide slots, one per lane.
-    const int cw = tw >> 1, ch = th >> 1, cx = x >> 1, cy = y >> 1;
     for (int c = 1; c < 3; ++c) {
         int pred[2][1] = {{0}, {0}};
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            const S* ref = (const S*)P.ref[l][pu.ref_idx[l]][c];
-            int mvx = pu.mv[l].x, mvy = pu.mv[l].y;
-            mc_component<S, 4, 16, 8, 1>(ref, P.pitch[c], P.pw[c], P.ph[c], cx + (mvx >> 3), cy + (mvy >> 3), mvx & 7,
-                                         mvy & 7, cw, ch, bd, win, tmp, tid, pred[l]);
+            mc_predict<4, 16, 8, 1>(L.chroma[l][c - 1], L.tmp, pu.mv[l].x & 7, pu.mv[l].y & 7, cw, ch, bd, tid, pred[l]);
         }
         int oy = tid >> 3, ox = tid & 7;
         if (ox < cw && oy < ch) {
@@ -222,111 +328,214 @@ __device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, int16_t* 
     }
 }
 
+// Side data of one CU, PU or luma TU, one thread each
+// (src/hevc/ctu.rs:751-800 for CUs and PUs, :1493-1510 for TUs).
+__device__ void side_entry(const HevcPicDev& P, int k) {
+    if (k < P.n_cus) {
+        const rv_hevc_cu cu = P.cus[k];
+        const int n4 = 1 << (cu.log2_size - 2), bx0 = cu.x >> 2, by0 = cu.y >> 2;
+        for (int by = by0; by < by0 + n4; ++by)
+            for (int bx = bx0; bx < bx0 + n4; ++bx) {
+                const size_t q = (size_t)by * P.w4 + bx;
+                const bool left = bx == bx0, top = by == by0;
+                P.qp[q] = cu.qp_y;
+                P.cuinfo[q] = (uint8_t)(cu.intra | (left << 1) | (top << 2));
+                if (cu.intra) P.pumap[q] = 0x3FFFFFFFu | ((uint32_t)left << 30) | ((uint32_t)top << 31);
+            }
+        return;
+    }
+    k -= P.n_cus;
+    if (k < P.n_pus) {
+        const rv_hevc_pu pu = P.pus[k];
+        const int bx0 = pu.x >> 2, by0 = pu.y >> 2, nw = pu.w >> 2, nh = pu.h >> 2;
+        for (int by = by0; by < by0 + nh; ++by)
+            for (int bx = bx0; bx < bx0 + nw; ++bx)
+                P.pumap[(size_t)by * P.w4 + bx] = (uint32_t)k | ((uint32_t)(bx == bx0) << 30) | ((uint32_t)(by == by0) << 31);
+        return;
+    }
+    k -= P.n_pus;
+    if (k < P.n_ltus) {
+        const rv_hevc_tu tu = P.ltus[k];
+        const int n4 = 1 << (tu.log2 - 2), bx0 = tu.x >> 2, by0 = tu.y >> 2;
+        const uint8_t cbf = (tu.flags & RV_TU_CODED) ? 1 : 0;
+        for (int by = by0; by < by0 + n4; ++by)
+            for (int bx = bx0; bx < bx0 + n4; ++bx)
+                P.tuinfo[(size_t)by * P.w4 + bx] = (uint8_t)(cbf | ((bx == bx0) << 1) | ((by == by0) << 2));
+    }
+}
+
 __global__ void __launch_bounds__(64) hevc_stage0(BatchDev b) {
-    __shared__ int16_t win[23 * 32];
-    __shared__ int16_t tmp[23 * 16];
+    __shared__ McLds lds;
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
-    const int it = blockIdx.x;
+    const int it = xcd_item();
+    if (it >= pre[b.n]) return;
     const int p = find_pic(pre, b.n, it);
     const HevcPicDev& P = *b.hevc[p];
     int i = it - pre[p];
     const int tid = threadIdx.x;
-    if (i < P.n_cus) {
-        // CU: QP, prediction mode, coding-block edges (src/hevc/ctu.rs:751-800).
-        const rv_hevc_cu cu = P.cus[i];
-        int n4 = 1 << (cu.log2_size - 2);
-        int bx0 = cu.x >> 2, by0 = cu.y >> 2;
-        for (int e = tid; e < n4 * n4; e += 64) {
-            int by = by0 + e / n4, bx = bx0 + e % n4;
-            size_t q = (size_t)by * P.w4 + bx;
-            bool left = bx == bx0, top = by == by0;
-            P.qp[q] = cu.qp_y;
-            P.cuinfo[q] = (uint8_t)(cu.intra | (left << 1) | (top << 2));
-            if (cu.intra) P.pumap[q] = 0x3FFFFFFFu | ((uint32_t)left << 30) | ((uint32_t)top << 31);
-        }
-        if (cu.intra) {
-            if (P.bd == 8) intra_copy<uint8_t>(P, cu, tid);
-            else intra_copy<uint16_t>(P, cu, tid);
-        }
+    if (i < P.n_side) {
+        side_entry(P, i * 64 + tid);
         return;
     }
-    i -= P.n_cus;
-    if (i < P.n_pus) {
-        const rv_hevc_pu pu = P.pus[i];
-        int nw = pu.w >> 2, nh = pu.h >> 2, bx0 = pu.x >> 2, by0 = pu.y >> 2;
-        for (int e = tid; e < nw * nh; e += 64) {
-            int by = by0 + e / nw, bx = bx0 + e % nw;
-            P.pumap[(size_t)by * P.w4 + bx] = (uint32_t)i | ((uint32_t)(bx == bx0) << 30) | ((uint32_t)(by == by0) << 31);
-        }
+    i -= P.n_side;
+    if (i < P.n_intra) {
+        const rv_hevc_cu cu = P.cus[P.intra_idx[i]];
+        if (P.bd == 8) intra_copy<uint8_t>(P, cu, tid);
+        else intra_copy<uint16_t>(P, cu, tid);
         return;
     }
-    i -= P.n_pus;
-    if (i < P.n_ltus) {
-        // Luma TU edges and cbf (src/hevc/ctu.rs:1493-1510).
-        const rv_hevc_tu tu = P.ltus[i];
-        int n4 = 1 << (tu.log2 - 2), bx0 = tu.x >> 2, by0 = tu.y >> 2;
-        uint8_t cbf = (tu.flags & RV_TU_CODED) ? 1 : 0;
-        for (int e = tid; e < n4 * n4; e += 64) {
-            int by = by0 + e / n4, bx = bx0 + e % n4;
-            P.tuinfo[(size_t)by * P.w4 + bx] = (uint8_t)(cbf | ((bx == bx0) << 1) | ((by == by0) << 2));
-        }
-        return;
-    }
-    i -= P.n_ltus;
+    i -= P.n_intra;
     const McTile t = P.tiles[i];
-    if (P.bd == 8) mc_tile<uint8_t>(P, t, tid, win, tmp);
-    else mc_tile<uint16_t>(P, t, tid, win, tmp);
+    if (P.bd == 8) mc_tile<uint8_t>(P, t, tid, lds);
+    else mc_tile<uint16_t>(P, t, tid, lds);
 }
 
 // ---------------------------------------------------------------------
 // Stage 1: inverse transform + residual add; boundary strengths
 // ---------------------------------------------------------------------
 
-// idct_scalar / idst4_scalar (src/dsp/hevc.rs:499-556) in matrix form: the
-// partial butterfly is the matrix product regrouped exactly (the
-// reference's butterfly_equals_matrix, :1511-1536), and coefficients
-// outside the nonzero bounds contribute nothing. Stage 1 rounds by 7 and
-// clips to i16; stage 2 rounds by bdShift = 20 - BitDepth and clips; then
-// add_residual_scalar (:558-566). `lane` indexes the TU's outputs.
-template <typename S>
-__device__ void tu_residual(const HevcPicDev& P, const rv_hevc_tu& tu, int lane, int nlanes, int16_t* c,
-                            int16_t* t1, const int8_t* mat, bool active) {
-    const int log2 = tu.log2, n = 1 << log2, nn = n * n;
-    const bool dst = (tu.flags & RV_TU_DST) != 0;
-    const int step = 32 >> log2;
-    // The DST ignores the bounds (idst4_scalar takes all sixteen).
-    const int mx = dst ? 3 : tu.max_x, my = dst ? 3 : tu.max_y;
-    if (active)
-        for (int e = lane; e < nn; e += nlanes) c[e] = P.coefs[tu.coef_off + e];
-    __syncthreads();
-    // Stage 1 (columns): t1[i][x] = clip((sum_j M[j][i] c[j][x] + 64) >> 7).
-    if (active)
-        for (int e = lane; e < nn; e += nlanes) {
-            int i = e >> log2, x = e & (n - 1);
-            int s = 0;
-            if (x <= mx)
-                for (int j = 0; j <= my; ++j) {
-                    int m = dst ? (int)kDst[j][i] : (int)mat[(j * step) * 32 + i];
-                    s += m * (int)c[j * n + x];
-                }
-            t1[e] = (int16_t)clampi((s + 64) >> 7, -32768, 32767);
+// Magnitudes of the 8.6.4.2 matrix (its first column) and entry (k, n):
+// +-mag(m), m = (2n + 1) k mod 128 folded by the cosine symmetries.
+constexpr int mag32(int m) {
+    return m == 0    ? 64
+           : m <= 3  ? 90
+           : m == 4  ? 89
+           : m == 5  ? 88
+           : m == 6  ? 87
+           : m == 7  ? 85
+           : m == 8  ? 83
+           : m == 9  ? 82
+           : m == 10 ? 80
+           : m == 11 ? 78
+           : m == 12 ? 75
+           : m == 13 ? 73
+           : m == 14 ? 70
+           : m == 15 ? 67
+           : m == 16 ? 64
+           : m == 17 ? 61
+           : m == 18 ? 57
+           : m == 19 ? 54
+           : m == 20 ? 50
+           : m == 21 ? 46
+           : m == 22 ? 43
+           : m == 23 ? 38
+           : m == 24 ? 36
+           : m == 25 ? 31
+           : m == 26 ? 25
+           : m == 27 ? 22
+           : m == 28 ? 18
+           : m == 29 ? 13
+           : m == 30 ? 9
+           : m == 31 ? 4
+                     : 0;
+}
+constexpr int tcoef(int k, int n) {
+    return (((2 * n + 1) * k) & 127) <= 32   ? mag32(((2 * n + 1) * k) & 127)
+           : (((2 * n + 1) * k) & 127) <= 64 ? -mag32(64 - (((2 * n + 1) * k) & 127))
+           : (((2 * n + 1) * k) & 127) <= 96 ? -mag32((((2 * n + 1) * k) & 127) - 64)
+                                             : mag32(128 - (((2 * n + 1) * k) & 127));
+}
+
+// idct1 (src/dsp/hevc.rs:474-497): the even/odd partial butterfly over all
+// N inputs. The reference stops at the nonzero bound; zeros beyond it add
+// nothing, so the dense butterfly gives the same sums. Unrolled, every
+// matrix entry is an immediate.
+template <int N>
+__device__ __forceinline__ void idct1d(const int (&s)[N], int (&o)[N]) {
+    if constexpr (N == 1) {
+        o[0] = 64 * s[0];
+    } else {
+        constexpr int H = N / 2, STEP = 32 / N;
+        int ev[H], eo[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) ev[k] = s[2 * k];
+        idct1d<H>(ev, eo);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            int acc = 0;
+#pragma unroll
+            for (int j = 1; j < N; j += 2) acc += tcoef(j * STEP, k) * s[j];
+            o[k] = eo[k] + acc;
+            o[N - 1 - k] = eo[k] - acc;
         }
-    __syncthreads();
-    const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1), max = (1 << P.bd) - 1;
+    }
+}
+
+// idst4_scalar's 1-D step (src/dsp/hevc.rs:529-556): o[i] = sum_j M[j][i] s[j].
+__device__ __forceinline__ void idst1d(const int (&s)[4], int (&o)[4]) {
+    o[0] = 29 * s[0] + 74 * s[1] + 84 * s[2] + 55 * s[3];
+    o[1] = 55 * s[0] + 74 * s[1] - 29 * s[2] - 84 * s[3];
+    o[2] = 74 * s[0] + 0 * s[1] - 74 * s[2] + 74 * s[3];
+    o[3] = 84 * s[0] - 74 * s[1] + 55 * s[2] - 29 * s[3];
+}
+
+// One wave reconstructs 64 / N coded TUs of size N, N lanes each:
+// idct_scalar / idst4_scalar (src/dsp/hevc.rs:499-556) — columns, round by
+// 7 and clip to i16, rows, round by bdShift = 20 - BitDepth and clip — then
+// add_residual_scalar (:558-566).
+template <int N, typename S>
+__device__ void tu_group(const HevcPicDev& P, int first, int count, int tid, int16_t* lds) {
+    constexpr int TPW = 64 / N, LS = N + 1;  // TUs per wave, padded LDS row
+    const int slot = tid / N, lane = tid % N;
+    const bool active = slot < count;
+    const rv_hevc_tu tu = P.ctus[first + (active ? slot : 0)];
+    int16_t* t = lds + slot * N * LS;
+    const bool dst = N == 4 && (tu.flags & RV_TU_DST);
+    (void)TPW;
     if (active) {
-        S* plane = (S*)P.cur[tu.comp];
-        int pitch = P.pitch[tu.comp];
-        for (int e = lane; e < nn; e += nlanes) {
-            int y = e >> log2, i = e & (n - 1);
-            int s = 0;
-            for (int j = 0; j <= mx; ++j) {
-                int m = dst ? (int)kDst[j][i] : (int)mat[(j * step) * 32 + i];
-                s += m * (int)t1[y * n + j];
-            }
-            int r = clampi((s + round2) >> bd_shift, -32768, 32767);
-            S* d = plane + (size_t)(tu.y + y) * pitch + tu.x + i;
-            *d = (S)clampi((int)*d + r, 0, max);
+        // Column `lane`.
+        int s[N], o[N];
+        const int16_t* c = P.coefs + tu.coef_off + lane;
+#pragma unroll
+        for (int j = 0; j < N; ++j) s[j] = c[j * N];
+        if constexpr (N == 4) {
+            if (dst) idst1d(s, o);
+            else idct1d<4>(s, o);
+        } else {
+            idct1d<N>(s, o);
         }
+#pragma unroll
+        for (int y = 0; y < N; ++y) t[y * LS + lane] = (int16_t)clampi((o[y] + 64) >> 7, -32768, 32767);
+    }
+    __syncthreads();
+    const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1);
+    if (active) {
+        // Row `lane`, in place.
+        int s[N], o[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) s[j] = t[lane * LS + j];
+        if constexpr (N == 4) {
+            if (dst) idst1d(s, o);
+            else idct1d<4>(s, o);
+        } else {
+            idct1d<N>(s, o);
+        }
+#pragma unroll
+        for (int x = 0; x < N; ++x) t[lane * LS + x] = (int16_t)clampi((o[x] + round2) >> bd_shift, -32768, 32767);
+    }
+    __syncthreads();
+    if (active) {
+        // Residual add: lane = column, coalesced along each row.
+        const int max = (1 << P.bd) - 1, pitch = P.pitch[tu.comp];
+        S* d = (S*)P.cur[tu.comp] + (size_t)tu.y * pitch + tu.x + lane;
+#pragma unroll 4
+        for (int y = 0; y < N; ++y) d[(size_t)y * pitch] = (S)clampi((int)d[(size_t)y * pitch] + t[y * LS + lane], 0, max);
+    }
+}
+
+template <typename S>
+__device__ void tx_item(const HevcPicDev& P, int i, int tid, int16_t* lds) {
+    int cls = 0;
+    while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
+    const int tpw = 16 >> cls;
+    const int k = i - P.tx_items[cls];
+    const int first = P.tx_first[cls] + k * tpw;
+    const int count = min(tpw, P.n_tx[cls] - k * tpw);
+    switch (cls) {
+        case 0: tu_group<4, S>(P, first, count, tid, lds); break;
+        case 1: tu_group<8, S>(P, first, count, tid, lds); break;
+        case 2: tu_group<16, S>(P, first, count, tid, lds); break;
+        default: tu_group<32, S>(P, first, count, tid, lds); break;
     }
 }
 
@@ -418,36 +627,20 @@ __device__ void block_bs(const HevcPicDev& P, int q) {
 }
 
 __global__ void __launch_bounds__(64) hevc_stage1(BatchDev b) {
-    __shared__ int16_t cbuf[32 * 32];
-    __shared__ int16_t t1[32 * 32];
-    __shared__ int8_t mat[32 * 32];
+    __shared__ int16_t lds[2 * 32 * 33];
     const int32_t* pre = b.prefix + 1 * (b.n + 1);
-    const int it = blockIdx.x;
+    const int it = xcd_item();
+    if (it >= pre[b.n]) return;
     const int p = find_pic(pre, b.n, it);
     const HevcPicDev& P = *b.hevc[p];
     int i = it - pre[p];
     const int tid = threadIdx.x;
-    const int n4items = (P.n_tx4 + 3) >> 2;
-    const int ntx = n4items + P.n_txbig;
-    if (i < ntx) {
-        for (int e = tid; e < 32 * 32; e += 64) mat[e] = (int8_t)dct_entry(e >> 5, e & 31);
-        if (i < n4items) {
-            // Four 4x4 TUs, sixteen lanes each.
-            int k = i * 4 + (tid >> 4);
-            bool active = k < P.n_tx4;
-            rv_hevc_tu tu = P.ctus[active ? k : P.n_tx4 - 1];
-            int16_t* c = cbuf + (tid >> 4) * 16;
-            int16_t* tt = t1 + (tid >> 4) * 16;
-            if (P.bd == 8) tu_residual<uint8_t>(P, tu, tid & 15, 16, c, tt, mat, active);
-            else tu_residual<uint16_t>(P, tu, tid & 15, 16, c, tt, mat, active);
-        } else {
-            const rv_hevc_tu tu = P.ctus[P.n_tx4 + (i - n4items)];
-            if (P.bd == 8) tu_residual<uint8_t>(P, tu, tid, 64, cbuf, t1, mat, true);
-            else tu_residual<uint16_t>(P, tu, tid, 64, cbuf, t1, mat, true);
-        }
+    if (i < P.tx_items[4]) {
+        if (P.bd == 8) tx_item<uint8_t>(P, i, tid, lds);
+        else tx_item<uint16_t>(P, i, tid, lds);
         return;
     }
-    i -= ntx;
+    i -= P.tx_items[4];
     int q = i * 64 + tid;
     if (q < P.w4 * P.h4) block_bs(P, q);
 }
@@ -631,7 +824,8 @@ __device__ void deblock_h_seg(const HevcPicDev& P, int t) {
 
 __global__ void __launch_bounds__(256) hevc_stage2(BatchDev b) {
     const int32_t* pre = b.prefix + 2 * (b.n + 1);
-    const int it = blockIdx.x;
+    const int it = xcd_item();
+    if (it >= pre[b.n]) return;
     const int p = find_pic(pre, b.n, it);
     const HevcPicDev& P = *b.hevc[p];
     if (!P.deblock) return;
@@ -643,7 +837,8 @@ __global__ void __launch_bounds__(256) hevc_stage2(BatchDev b) {
 
 __global__ void __launch_bounds__(256) hevc_stage3(BatchDev b) {
     const int32_t* pre = b.prefix + 3 * (b.n + 1);
-    const int it = blockIdx.x;
+    const int it = xcd_item();
+    if (it >= pre[b.n]) return;
     const int p = find_pic(pre, b.n, it);
     const HevcPicDev& P = *b.hevc[p];
     if (!P.deblock) return;
@@ -660,94 +855,187 @@ __global__ void __launch_bounds__(256) hevc_stage3(BatchDev b) {
 // slice and one tile, a neighbour is usable exactly when it lies inside
 // the picture, so a sample is edge-filtered iff both of its class's
 // neighbours are inside the picture (the interior + ring of sao_ctb).
+// SAO of one group of G = 16 / sizeof(S) samples of one row (one 16-byte
+// vector) inside one CTB with parameters `sp`: rows y-1, y, y+1 and the
+// samples left and right of the group are loaded first; the class decides
+// which of them are the two neighbours (static register indices).
 template <typename S>
-__device__ void sao_group(const HevcPicDev& P, int c, int gx, int y) {
+__device__ void sao_group(const HevcPicDev& P, int c, int x0, int y, const rv_sao& sp, bool on) {
+    constexpr int G = 16 / sizeof(S);
     const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c];
     const S* src = (const S*)P.cur[c];
     S* dst = (S*)P.out[c];
-    const int x0 = gx * 4;
-    const int sh = c ? 1 : 0;
-    const int rx = (x0 << sh) >> P.log2_ctb, ry = (y << sh) >> P.log2_ctb;
-    rv_sao sp = P.sao[3 * (ry * P.wc + rx) + c];
-    const bool on = c == 0 ? P.sao_luma : P.sao_chroma;
-    const size_t row = (size_t)y * pitch;
-    int v[4];
+    int v[3][G + 2];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = src[row + x0 + k];
+    for (int r = 0; r < 3; ++r) {
+        const S* row = src + (size_t)clampi(y + r - 1, 0, ph - 1) * pitch;
+        union {
+            uint4 q;
+            S s[G];
+        } u;
+        u.q = *(const uint4*)(row + x0);  // within the row's 256-byte-aligned pitch
+#pragma unroll
+        for (int k = 0; k < G; ++k) v[r][k + 1] = u.s[k];
+        v[r][0] = row[max(x0 - 1, 0)];
+        v[r][G + 1] = row[min(x0 + G, pw - 1)];
+    }
     const int max = (1 << P.bd) - 1;
-    int o[4] = {v[0], v[1], v[2], v[3]};
+    int o[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) o[k] = v[1][k + 1];
     if (on && sp.type_idx == 1) {
         const int shift = P.bd - 5;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int band = ((v[k] >> shift) - sp.band_or_class) & 31;
-            if (band < 4) o[k] = clampi(v[k] + sp.offsets[band], 0, max);
+        for (int k = 0; k < G; ++k) {
+            int band = ((o[k] >> shift) - sp.band_or_class) & 31;
+            if (band < 4) o[k] = clampi(o[k] + sp.offsets[band], 0, max);
         }
     } else if (on && sp.type_idx == 2) {
         const int cls = sp.band_or_class & 3;
-        const int hx = cls == 1 ? 0 : (cls == 3 ? 1 : -1);  // first neighbour dx
-        const int vy = cls == 0 ? 0 : -1;                   // first neighbour dy
-        const int off5[5] = {sp.offsets[0], sp.offsets[1], 0, sp.offsets[2], sp.offsets[3]};
+        const int vy = cls == 0 ? 0 : -1;
+        const bool rows_ok = y + vy >= 0 && y - vy < ph && y - vy >= 0 && y + vy < ph;
+        const int o0 = sp.offsets[0], o1 = sp.offsets[1], o2 = sp.offsets[2], o3 = sp.offsets[3];
+        auto apply = [&](int k, int a, int bb) {
+            const int e = sgn(o[k] - a) + sgn(o[k] - bb);  // edgeIdx - 2
+            const int off = e == -2 ? o0 : e == -1 ? o1 : e == 0 ? 0 : e == 1 ? o2 : o3;
+            o[k] = clampi(o[k] + off, 0, max);
+        };
+        if (rows_ok) {
+            switch (cls) {
+                case 0:  // (x - 1, y), (x + 1, y)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int x = x0 + k;
-            int xa = x + hx, ya = y + vy, xb = x - hx, yb = y - vy;
-            if (xa < 0 || xa >= pw || xb < 0 || xb >= pw || ya < 0 || ya >= ph || yb < 0 || yb >= ph) continue;
-            int a = src[(size_t)ya * pitch + xa], bb = src[(size_t)yb * pitch + xb];
-            int e = 2 + sgn(v[k] - a) + sgn(v[k] - bb);
-            o[k] = clampi(v[k] + off5[e], 0, max);
+                    for (int k = 0; k < G; ++k)
+                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[1][k], v[1][k + 2]);
+                    break;
+                case 1:  // (x, y - 1), (x, y + 1)
+#pragma unroll
+                    for (int k = 0; k < G; ++k)
+                        if (x0 + k < pw) apply(k, v[0][k + 1], v[2][k + 1]);
+                    break;
+                case 2:  // (x - 1, y - 1), (x + 1, y + 1)
+#pragma unroll
+                    for (int k = 0; k < G; ++k)
+                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k], v[2][k + 2]);
+                    break;
+                default:  // (x + 1, y - 1), (x - 1, y + 1)
+#pragma unroll
+                    for (int k = 0; k < G; ++k)
+                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k + 2], v[2][k]);
+                    break;
+            }
         }
     }
+    S* drow = dst + (size_t)y * pitch + x0;
+    if (x0 + G <= pw) {
+        union {
+            uint4 q;
+            S s[G];
+        } u;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dst[row + x0 + k] = (S)o[k];
+        for (int k = 0; k < G; ++k) u.s[k] = (S)o[k];
+        *(uint4*)drow = u.q;
+    } else {
+        for (int k = 0; k < G && x0 + k < pw; ++k) drow[k] = (S)o[k];
+    }
+}
+
+// One sample, for CTBs narrower than a vector group.
+template <typename S>
+__device__ void sao_sample(const HevcPicDev& P, int c, int x, int y, const rv_sao& sp, bool on) {
+    const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c], max = (1 << P.bd) - 1;
+    const S* src = (const S*)P.cur[c];
+    const int vv = src[(size_t)y * pitch + x];
+    int r = vv;
+    if (on && sp.type_idx == 1) {
+        int band = ((vv >> (P.bd - 5)) - sp.band_or_class) & 31;
+        if (band < 4) r = clampi(vv + sp.offsets[band], 0, max);
+    } else if (on && sp.type_idx == 2) {
+        const int cls = sp.band_or_class & 3;
+        const int hx = cls == 1 ? 0 : (cls == 3 ? 1 : -1), vy = cls == 0 ? 0 : -1;
+        const int xa = x + hx, ya = y + vy, xb = x - hx, yb = y - vy;
+        if (xa >= 0 && xa < pw && xb >= 0 && xb < pw && ya >= 0 && ya < ph && yb >= 0 && yb < ph) {
+            const int a = src[(size_t)ya * pitch + xa], bb = src[(size_t)yb * pitch + xb];
+            const int e = sgn(vv - a) + sgn(vv - bb);
+            const int off = e == -2 ? sp.offsets[0] : e == -1 ? sp.offsets[1] : e == 0 ? 0 : e == 1 ? sp.offsets[2] : sp.offsets[3];
+            r = clampi(vv + off, 0, max);
+        }
+    }
+    ((S*)P.out[c])[(size_t)y * pitch + x] = (S)r;
+}
+
+// One CTB per workgroup (256 lanes), all three components: the parameters
+// are uniform across each wave, so the type and class branches are scalar.
+template <typename S>
+__device__ void sao_ctb(const HevcPicDev& P, int rx, int ry, int tid) {
+    constexpr int G = 16 / sizeof(S);
+    for (int c = 0; c < 3; ++c) {
+        const int sh = c ? 1 : 0;
+        const int cs = (1 << P.log2_ctb) >> sh;
+        const int x0 = rx * cs, y0 = ry * cs;
+        const int w = min(cs, P.pw[c] - x0), h = min(cs, P.ph[c] - y0);
+        const rv_sao sp = P.sao[3 * (ry * P.wc + rx) + c];
+        const bool on = (c == 0 ? P.sao_luma : P.sao_chroma) && sp.type_idx != 0;
+        if (cs % G == 0) {
+            const int gpr = cs / G;
+            for (int t = tid; t < cs * gpr; t += 256) {
+                const int yy = t / gpr, gx = t - yy * gpr;
+                if (yy < h && gx * G < w) sao_group<S>(P, c, x0 + gx * G, y0 + yy, sp, on);
+            }
+        } else {
+            for (int t = tid; t < w * h; t += 256) sao_sample<S>(P, c, x0 + t % w, y0 + t / w, sp, on);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) hevc_stage4(BatchDev b) {
     const int32_t* pre = b.prefix + 4 * (b.n + 1);
-    const int it = blockIdx.x;
+    const int it = xcd_item();
+    if (it >= pre[b.n]) return;
     const int p = find_pic(pre, b.n, it);
     const HevcPicDev& P = *b.hevc[p];
-    int t = (it - pre[p]) * 256 + threadIdx.x;
-    const int gy = P.pw[0] >> 2, gc = P.pw[1] >> 2;
-    const int ny = gy * P.ph[0], nc = gc * P.ph[1];
-    int c, gx, y;
-    if (t < ny) {
-        c = 0;
-        y = t / gy;
-        gx = t - y * gy;
-    } else if (t < ny + 2 * nc) {
-        t -= ny;
-        c = 1 + (t >= nc);
-        if (t >= nc) t -= nc;
-        y = t / gc;
-        gx = t - y * gc;
-    } else {
-        return;
-    }
-    if (P.bd == 8) sao_group<uint8_t>(P, c, gx, y);
-    else sao_group<uint16_t>(P, c, gx, y);
+    const int i = it - pre[p];
+    const int ry = i / P.wc, rx = i - ry * P.wc;
+    if (P.bd == 8) sao_ctb<uint8_t>(P, rx, ry, threadIdx.x);
+    else sao_ctb<uint16_t>(P, rx, ry, threadIdx.x);
 }
 
 }  // namespace
 
+__global__ void __launch_bounds__(256) rv_copy_probe(uint4* __restrict__ dst, const uint4* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStream_t st) {
+    const int64_t n = bytes / 16;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(rv_copy_probe, dim3(grid), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n);
+    return hipGetLastError();
+}
+
 // Work items per stage for one picture (the grid unit of each stage).
 void hevc_stage_items(HevcPicDev& P) {
-    P.items[0] = P.n_cus + P.n_pus + P.n_ltus + P.n_tiles;
-    P.items[1] = ((P.n_tx4 + 3) >> 2) + P.n_txbig + (P.w4 * P.h4 + 63) / 64;
+    P.n_side = (P.n_cus + P.n_pus + P.n_ltus + 63) / 64;
+    P.items[0] = P.n_side + P.n_intra + P.n_tiles;
+    P.tx_items[0] = 0;
+    for (int c = 0; c < 4; ++c) {
+        int tpw = 16 >> c;
+        P.tx_items[c + 1] = P.tx_items[c] + (P.n_tx[c] + tpw - 1) / tpw;
+    }
+    P.items[1] = P.tx_items[4] + (P.w4 * P.h4 + 63) / 64;
     P.items[2] = ((P.w >> 3) * P.h4 + 255) / 256;
     P.items[3] = (P.h >= 16) ? ((((P.h >> 3) - 1) * P.w4) + 255) / 256 : 0;
-    int ny = (P.pw[0] >> 2) * P.ph[0], nc = (P.pw[1] >> 2) * P.ph[1];
-    P.items[4] = (ny + 2 * nc + 255) / 256;
+    P.items[4] = P.wc * P.hc;
 }
 
 hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
     if (total <= 0) return hipSuccess;
+    const dim3 grid(8 * ((total + 7) / 8));  // see xcd_item()
     switch (stage) {
-        case 0: hipLaunchKernelGGL(hevc_stage0, dim3(total), dim3(64), 0, st, b); break;
-        case 1: hipLaunchKernelGGL(hevc_stage1, dim3(total), dim3(64), 0, st, b); break;
-        case 2: hipLaunchKernelGGL(hevc_stage2, dim3(total), dim3(256), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(hevc_stage3, dim3(total), dim3(256), 0, st, b); break;
-        case 4: hipLaunchKernelGGL(hevc_stage4, dim3(total), dim3(256), 0, st, b); break;
+        case 0: hipLaunchKernelGGL(hevc_stage0, grid, dim3(64), 0, st, b); break;
+        case 1: hipLaunchKernelGGL(hevc_stage1, grid, dim3(64), 0, st, b); break;
+        case 2: hipLaunchKernelGGL(hevc_stage2, grid, dim3(256), 0, st, b); break;
+        case 3: hipLaunchKernelGGL(hevc_stage3, grid, dim3(256), 0, st, b); break;
+        case 4: hipLaunchKernelGGL(hevc_stage4, grid, dim3(256), 0, st, b); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -464,16 +464,28 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     tiles.push_back(t);
                 }
         }
-        std::vector<rv_hevc_tu> ltus, ctus4, ctusb;
+        std::vector<rv_hevc_tu> ltus, bysize[4];
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
             if (t.comp == 0) ltus.push_back(t);
-            if (t.flags & RV_TU_CODED) (t.log2 == 2 ? ctus4 : ctusb).push_back(t);
+            if (t.flags & RV_TU_CODED) bysize[t.log2 - 2].push_back(t);
         }
-        std::stable_sort(ctusb.begin(), ctusb.end(),
-                         [](const rv_hevc_tu& a, const rv_hevc_tu& b) { return a.log2 > b.log2; });
-        std::vector<rv_hevc_tu> ctus(ctus4);
-        ctus.insert(ctus.end(), ctusb.begin(), ctusb.end());
+        // Coded TUs in kernel order (size classes, each in picture order),
+        // their coefficients repacked contiguously in that order so a wave's
+        // TUs read one run of memory.
+        std::vector<rv_hevc_tu> ctus;
+        for (auto& v : bysize) ctus.insert(ctus.end(), v.begin(), v.end());
+        std::vector<int16_t> coefs;
+        coefs.reserve(p->n_coefs);
+        for (rv_hevc_tu& t : ctus) {
+            const size_t nn = (size_t)1 << (2 * t.log2);
+            const uint32_t off = (uint32_t)coefs.size();
+            coefs.insert(coefs.end(), p->coefs + t.coef_off, p->coefs + t.coef_off + nn);
+            t.coef_off = off;
+        }
+        std::vector<int32_t> intra_idx;
+        for (int i = 0; i < p->n_cus; ++i)
+            if (p->cus[i].intra) intra_idx.push_back(i);
         int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
         size_t n_sao = (size_t)wc * hc * 3;
         std::vector<rv_sao> sao(n_sao, rv_sao{0, 0, {0, 0, 0, 0}});
@@ -487,8 +499,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         // Device arena.
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
                     carve_size<rv_hevc_tu>(ltus.size()) + carve_size<rv_hevc_tu>(ctus.size()) +
-                    carve_size<rv::McTile>(tiles.size()) + carve_size<int16_t>(p->n_coefs) +
+                    carve_size<rv::McTile>(tiles.size()) + carve_size<int16_t>(coefs.size()) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
+                    carve_size<int32_t>(intra_idx.size()) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4);
         size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
@@ -508,9 +521,11 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, ltus.size());
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, ctus.size());
         rv::McTile* h_tiles = carve<rv::McTile>(hp, tiles.size());
-        int16_t* h_coefs = carve<int16_t>(hp, p->n_coefs);
+        int16_t* h_coefs = carve<int16_t>(hp, coefs.size());
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
+        int32_t* h_intra = carve<int32_t>(hp, intra_idx.size());
+        if (!intra_idx.empty()) std::memcpy(h_intra, intra_idx.data(), intra_idx.size() * sizeof(int32_t));
         const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         if (p->n_pus) std::memcpy(h_pus, p->pus, p->n_pus * sizeof(rv_hevc_pu));
@@ -518,7 +533,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         if (!ltus.empty()) std::memcpy(h_ltus, ltus.data(), ltus.size() * sizeof(rv_hevc_tu));
         if (!ctus.empty()) std::memcpy(h_ctus, ctus.data(), ctus.size() * sizeof(rv_hevc_tu));
         if (!tiles.empty()) std::memcpy(h_tiles, tiles.data(), tiles.size() * sizeof(rv::McTile));
-        if (p->n_coefs) std::memcpy(h_coefs, p->coefs, p->n_coefs * sizeof(int16_t));
+        if (!coefs.empty()) std::memcpy(h_coefs, coefs.data(), coefs.size() * sizeof(int16_t));
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
         std::memcpy(h_sao, sao.data(), n_sao * sizeof(rv_sao));
         // Scratch: reconstruction planes and side maps.
@@ -565,8 +580,13 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.n_pus = p->n_pus;
         d.n_ltus = (int)ltus.size();
         d.n_tiles = (int)tiles.size();
-        d.n_tx4 = (int)ctus4.size();
-        d.n_txbig = (int)ctusb.size();
+        d.n_intra = (int)intra_idx.size();
+        d.intra_idx = (const int32_t*)dev((char*)h_intra);
+        for (int c = 0, acc = 0; c < 4; ++c) {
+            d.n_tx[c] = (int)bysize[c].size();
+            d.tx_first[c] = acc;
+            acc += d.n_tx[c];
+        }
         d.cus = (const rv_hevc_cu*)dev((char*)h_cus);
         d.pus = (const rv_hevc_pu*)dev((char*)h_pus);
         d.ltus = (const rv_hevc_tu*)dev((char*)h_ltus);
@@ -910,6 +930,39 @@ int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
         return rc;
     } catch (...) {
         return fail(RV_E_HIP, "exception in rv_batch_profile");
+    }
+}
+
+int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
+    try {
+        if (!gbs || bytes < 4096 || reps <= 0) return fail(RV_E_INVALID, "bad argument");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        bytes &= ~(int64_t)4095;
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+            if (a) (void)hipFree(a);
+            return fail(RV_E_NOMEM, "probe buffers");
+        }
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipEvent_t e0, e1;
+        RV_HIP(hipEventCreate(&e0));
+        RV_HIP(hipEventCreate(&e1));
+        RV_HIP(hipMemsetAsync(a, 1, bytes, g_stream));
+        RV_HIP(rv::launch_copy_probe(b, a, bytes, g_stream));
+        RV_HIP(hipEventRecord(e0, g_stream));
+        for (int r = 0; r < reps; ++r) RV_HIP(rv::launch_copy_probe(b, a, bytes, g_stream));
+        RV_HIP(hipEventRecord(e1, g_stream));
+        RV_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_HIP, "exception in rv_probe_copy");
     }
 }
 

@@ -33,12 +33,16 @@ struct HevcPicDev {
     int32_t sao_luma, sao_chroma;
     // Work lists.
     int32_t n_cus, n_pus, n_ltus, n_tiles;
-    int32_t n_tx4, n_txbig;  // coded TUs: 4x4 ones first, then the rest
+    int32_t n_side, n_intra;  // stage-0 side-data items (64 list entries each), intra CUs
+    const int32_t* intra_idx; // indices of the intra CUs in cus
+    int32_t n_tx[4];         // coded TUs per size (4, 8, 16, 32), in that order
+    int32_t tx_first[4];     // index in ctus of each size's first TU
+    int32_t tx_items[5];     // stage-1 item offsets per size (64 / N TUs per item)
     const rv_hevc_cu* cus;
     const rv_hevc_pu* pus;
     const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
     const McTile* tiles;
-    const rv_hevc_tu* ctus;  // coded TUs, 4x4 first
+    const rv_hevc_tu* ctus;  // coded TUs grouped by size, smallest first
     const int16_t* coefs;
     const uint16_t* intra_pred;
     const rv_sao* sao;
@@ -77,6 +81,16 @@ struct H264PicDev {
     uint32_t* bsh;       // per MB x 4 edges: packed horizontal bS
     int32_t items[5];
 };
+
+// Work item of this workgroup: consecutive workgroups go round-robin to the
+// eight XCDs, so workgroup b takes item (b % 8) * chunk + b / 8 — each XCD
+// walks one contiguous run of items (in a batch, mostly one picture) and
+// its L2 sees that run's reference windows and output lines. The grid is
+// 8 * chunk; items past `total` exit.
+__device__ __forceinline__ int xcd_item() {
+    const int chunk = gridDim.x >> 3;
+    return (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+}
 
 // Per-batch tables: the descriptors and, per stage, prefix sums of the
 // per-picture work-item counts.

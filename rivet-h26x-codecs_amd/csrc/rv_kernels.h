@@ -15,4 +15,7 @@ hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_
 void h264_stage_items(H264PicDev& P);
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
 
+// Plain copy of `bytes` (a multiple of 4096) with 16-byte vectors.
+hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStream_t st);
+
 }  // namespace rv

@@ -246,5 +246,12 @@ def mark_elapsed_ms(a: int, b: int) -> float:
     return float(v.value)
 
 
+def probe_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Achievable copy bandwidth (read + write GB/s) of a plain vector copy."""
+    v = C.c_double()
+    check(lib.rv_probe_copy(nbytes, reps, C.byref(v)))
+    return float(v.value)
+
+
 def set_stage_mask(mask: int) -> None:
     check(lib.rv_set_stage_mask(mask))

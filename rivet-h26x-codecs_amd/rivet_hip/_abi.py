@@ -142,6 +142,7 @@ PROTOTYPES = {
     "rv_set_stage_mask": (i32, [i32]),
     "rv_job_bytes": (i32, [P, C.POINTER(i64)]),
     "rv_abi_size": (i32, [i32]),
+    "rv_probe_copy": (i32, [i64, i32, C.POINTER(C.c_double)]),
     "rv_gen_hevc": (i32, [C.POINTER(GenHevcCfg), C.c_uint64, C.POINTER(P)]),
     "rv_gen_h264": (i32, [C.POINTER(GenH264Cfg), C.c_uint64, C.POINTER(P)]),
     "rv_gen_hevc_pic": (C.POINTER(HevcPic), [P]),
