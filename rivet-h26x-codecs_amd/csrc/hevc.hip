@@ -71,33 +71,33 @@ __device__ void intra_copy(const HevcPicDev& P, const rv_hevc_cu& cu, int tid) {
     }
 }
 
+typedef short short2v __attribute__((ext_vector_type(2)));
+
 // Staged window of one component and list: (bw + TAPS - 1) x (bh + TAPS - 1)
 // samples at (xi - REACH, yi - REACH), coordinates clamped to the picture
 // (the replicated border of src/hevc/frame.rs:169-214 and the gather of
 // src/hevc/inter.rs:116-123 read the same samples). Lanes cover WIN_W-wide
 // row slots; NLD loads per lane, all issued before any is used.
-template <typename S, int TAPS, int WIN_W, int NLD>
+template <int TAPS, int WIN_W, int NLD>
 struct WinLoad {
     int v[NLD];
-    __device__ __forceinline__ void issue(const S* __restrict__ ref, int pitch, int pw, int ph, int xi, int yi, int bw,
-                                          int bh, int tid) {
+    template <typename S>
+    __device__ __forceinline__ void issue(const S* __restrict__ ref, int pitch, int pw, int ph, int xi, int yi,
+                                          int tid) {
         constexpr int REACH = TAPS / 2 - 1;
-        const int ww = bw + TAPS - 1, hh = bh + TAPS - 1;
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            int e = tid + 64 * k, r = e / WIN_W, c = e % WIN_W;
             // Clamped, so always a valid address; slots past the window
             // are staged but never read.
-            int gx = clampi(xi - REACH + c, 0, pw - 1), gy = clampi(yi - REACH + r, 0, ph - 1);
-            v[k] = (int)ref[(size_t)gy * pitch + gx];
-            (void)hh;
-            (void)ww;
+            const int e = tid + 64 * k, r = e / WIN_W, c = e % WIN_W;
+            const int gx = clampi(xi - REACH + c, 0, pw - 1), gy = clampi(yi - REACH + r, 0, ph - 1);
+            v[k] = (int)ref[gy * pitch + gx];
         }
     }
     __device__ __forceinline__ void store(int16_t* win, int tid) const {
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
-            int e = tid + 64 * k;
+            const int e = tid + 64 * k;
             if (e < (TAPS == 8 ? 23 : 11) * WIN_W) win[e] = (int16_t)v[k];
         }
     }
@@ -185,7 +185,53 @@ __device__ __forceinline__ int combine(const HevcPicDev& P, int c, int dir, cons
     return clampi((a * w0 + b * w1 + ((o0 + o1 + 1) << lwd)) >> (lwd + 1), 0, max);
 }
 
-typedef short short4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short2v pk(int lo, int hi) {
+    short2v r;
+    r.x = (short)lo;
+    r.y = (short)hi;
+    return r;
+}
+__device__ __forceinline__ int dot2(short2v a, short2v b, int c) { return __builtin_amdgcn_sdot2(a, b, c, false); }
+
+// The eight taps of one phase as packed pairs: even[i] = (f[2i], f[2i+1])
+// for outputs whose inputs start on a pair boundary, odd[i] = (f[2i-1],
+// f[2i]) (f[-1] = f[8] = 0) for outputs starting one sample later.
+struct Taps8 {
+    short2v e[4], o[5];
+    __device__ explicit Taps8(int frac) {
+        int f[10];
+        f[0] = 0;
+        f[9] = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k + 1] = kQpel[frac][k];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) e[i] = pk(f[2 * i + 1], f[2 * i + 2]);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) o[i] = pk(f[2 * i], f[2 * i + 1]);
+    }
+};
+
+// Four consecutive 8-tap outputs from twelve inputs held as six packed
+// pairs p[0..5] (inputs 2i, 2i+1): output j = sum_k f[k] in[j + k], exact
+// in i32 (v_dot2_i32_i16).
+__device__ __forceinline__ void filt4(const short2v (&p)[6], const Taps8& t, int (&acc)[4]) {
+    acc[0] = dot2(p[3], t.e[3], dot2(p[2], t.e[2], dot2(p[1], t.e[1], dot2(p[0], t.e[0], 0))));
+    acc[1] = dot2(p[4], t.o[4], dot2(p[3], t.o[3], dot2(p[2], t.o[2], dot2(p[1], t.o[1], dot2(p[0], t.o[0], 0)))));
+    acc[2] = dot2(p[4], t.e[3], dot2(p[3], t.e[2], dot2(p[2], t.e[1], dot2(p[1], t.e[0], 0))));
+    acc[3] = dot2(p[5], t.o[4], dot2(p[4], t.o[3], dot2(p[3], t.o[2], dot2(p[2], t.o[1], dot2(p[1], t.o[0], 0)))));
+}
+
+__device__ __forceinline__ void load_pairs(const int16_t* src, short2v (&p)[6]) {
+    // Three 8-byte LDS reads; src is 8-byte aligned.
+    const uint2 a = *(const uint2*)src, b = *(const uint2*)(src + 4), c = *(const uint2*)(src + 8);
+    p[0] = __builtin_bit_cast(short2v, a.x);
+    p[1] = __builtin_bit_cast(short2v, a.y);
+    p[2] = __builtin_bit_cast(short2v, b.x);
+    p[3] = __builtin_bit_cast(short2v, b.y);
+    p[4] = __builtin_bit_cast(short2v, c.x);
+    p[5] = __builtin_bit_cast(short2v, c.y);
+}
 
 // Luma 14-bit prediction of one list from its staged window (interp_scalar,
 // src/dsp/hevc.rs:823-882; 8-tap QPEL_FILTERS). Lane (x = tid & 15,
@@ -198,23 +244,17 @@ __device__ void luma_predict(const int16_t* win, int16_t* tmpT, int fx, int fy, 
                              int out[4]) {
     const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
     if (fx != 0) {
-        int f[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) f[k] = kQpel[fx][k];
+        const Taps8 t(fx);
         const int r0 = fy ? 0 : 3, nr = fy ? th + 7 : th, ng = tw >> 2;
         for (int e = tid; e < nr * 4; e += 64) {
             const int r = e >> 2, g = e & 3;
             if (g < ng) {
-                const int16_t* src = win + (r0 + r) * 32 + 4 * g;
-                const short4v a = *(const short4v*)src, b = *(const short4v*)(src + 4), c = *(const short4v*)(src + 8);
-                const int sv[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+                short2v p[6];
+                load_pairs(win + (r0 + r) * 32 + 4 * g, p);
+                int acc[4];
+                filt4(p, t, acc);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int acc = 0;
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) acc += f[k] * sv[i + k];
-                    tmpT[(4 * g + i) * 24 + r] = (int16_t)(acc >> shift1);
-                }
+                for (int i = 0; i < 4; ++i) tmpT[(4 * g + i) * 24 + r] = (int16_t)(acc[i] >> shift1);
             }
         }
         __syncthreads();
@@ -226,35 +266,32 @@ __device__ void luma_predict(const int16_t* win, int16_t* tmpT, int fx, int fy, 
 #pragma unroll
             for (int k = 0; k < 4; ++k) out[k] = (int16_t)((int)win[(y0 + 3 + k) * 32 + x + 3] << shift3);
         } else if (fy == 0) {
-            const short4v a = *(const short4v*)(tmpT + x * 24 + y0);
-            out[0] = a.x;
-            out[1] = a.y;
-            out[2] = a.z;
-            out[3] = a.w;
+            const uint2 a = *(const uint2*)(tmpT + x * 24 + y0);
+            const short2v lo = __builtin_bit_cast(short2v, a.x), hi = __builtin_bit_cast(short2v, a.y);
+            out[0] = lo.x;
+            out[1] = lo.y;
+            out[2] = hi.x;
+            out[3] = hi.y;
         } else {
-            int f[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) f[k] = kQpel[fy][k];
-            int sv[11];
+            const Taps8 t(fy);
+            short2v p[6];
             int sh;
             if (fx == 0) {
+                int sv[12];
 #pragma unroll
                 for (int k = 0; k < 11; ++k) sv[k] = win[(y0 + k) * 32 + x + 3];
+                sv[11] = 0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) p[i] = pk(sv[2 * i], sv[2 * i + 1]);
                 sh = shift1;
             } else {
-                const int16_t* src = tmpT + x * 24 + y0;
-                const short4v a = *(const short4v*)src, b = *(const short4v*)(src + 4), c = *(const short4v*)(src + 8);
-                sv[0] = a.x, sv[1] = a.y, sv[2] = a.z, sv[3] = a.w, sv[4] = b.x, sv[5] = b.y;
-                sv[6] = b.z, sv[7] = b.w, sv[8] = c.x, sv[9] = c.y, sv[10] = c.z;
+                load_pairs(tmpT + x * 24 + y0, p);
                 sh = 6;
             }
+            int acc[4];
+            filt4(p, t, acc);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) acc += f[k] * sv[i + k];
-                out[i] = (int16_t)(acc >> sh);
-            }
+            for (int i = 0; i < 4; ++i) out[i] = (int16_t)(acc[i] >> sh);
         }
     }
     __syncthreads();
@@ -268,32 +305,47 @@ struct McLds {
     int16_t tmp[16 * 24];
 };
 
+// All window loads of one MC tile: luma and both chroma components of
+// each list it uses.
+struct TileLoads {
+    WinLoad<8, 32, 12> l[2];
+    WinLoad<4, 16, 3> c[2][2];
+};
+
 template <typename S>
-__device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, McLds& L) {
-    const rv_hevc_pu pu = P.pus[t.pu];
-    const int x = pu.x + t.dx, y = pu.y + t.dy, tw = t.tw, th = t.th, bd = P.bd;
-    const int dir = pu.dir;
-    const int cw = tw >> 1, ch = th >> 1, cx = x >> 1, cy = y >> 1;
-    // Every window load of the item in flight at once.
-    WinLoad<S, 8, 32, 12> wl[2];
-    WinLoad<S, 4, 16, 3> wc[2][2];
+__device__ __forceinline__ void tile_issue_t(const HevcPicDev& P, const McTile& t, TileLoads& w, int tid) {
+    const int cx = t.x >> 1, cy = t.y >> 1;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-        if (!((dir >> l) & 1)) continue;
-        const int mvx = pu.mv[l].x, mvy = pu.mv[l].y;
-        wl[l].issue((const S*)P.ref[l][pu.ref_idx[l]][0], P.pitch[0], P.pw[0], P.ph[0], x + (mvx >> 2), y + (mvy >> 2),
-                    tw, th, tid);
+        if (!((t.dir >> l) & 1)) continue;
+        const int mvx = t.mv[l].x, mvy = t.mv[l].y;
+        const int ri = (t.refs >> (4 * l)) & 15;
+        w.l[l].issue((const S*)P.ref[l][ri][0], P.pitch[0], P.pw[0], P.ph[0], t.x + (mvx >> 2),
+                     t.y + (mvy >> 2), tid);
 #pragma unroll
         for (int c = 0; c < 2; ++c)
-            wc[l][c].issue((const S*)P.ref[l][pu.ref_idx[l]][c + 1], P.pitch[c + 1], P.pw[c + 1], P.ph[c + 1],
-                           cx + (mvx >> 3), cy + (mvy >> 3), cw, ch, tid);
+            w.c[l][c].issue((const S*)P.ref[l][ri][c + 1], P.pitch[c + 1], P.pw[c + 1], P.ph[c + 1],
+                            cx + (mvx >> 3), cy + (mvy >> 3), tid);
     }
+}
+
+__device__ __forceinline__ void tile_issue(const HevcPicDev& P, const McTile& t, TileLoads& w, int tid) {
+    if (P.bd == 8) tile_issue_t<uint8_t>(P, t, w, tid);
+    else tile_issue_t<uint16_t>(P, t, w, tid);
+}
+
+// Stage the loaded windows, predict, combine and write one tile.
+template <typename S>
+__device__ void tile_finish_t(const HevcPicDev& P, const McTile& t, const TileLoads& w, int tid, McLds& L) {
+    const int x = t.x, y = t.y, tw = t.tw, th = t.th, bd = P.bd, dir = t.dir;
+    const int cw = tw >> 1, ch = th >> 1, cx = x >> 1, cy = y >> 1;
+    const int8_t ref_idx[2] = {(int8_t)(t.refs & 15), (int8_t)(t.refs >> 4)};
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
-        wl[l].store(L.luma[l], tid);
+        w.l[l].store(L.luma[l], tid);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) wc[l][c].store(L.chroma[l][c], tid);
+        for (int c = 0; c < 2; ++c) w.c[l][c].store(L.chroma[l][c], tid);
     }
     __syncthreads();
     // Luma: lane (x, 4-row group), four outputs.
@@ -301,15 +353,14 @@ __device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, McLds& L)
         int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            luma_predict(L.luma[l], L.tmp, pu.mv[l].x & 3, pu.mv[l].y & 3, tw, th, bd, tid, pred[l]);
+            luma_predict(L.luma[l], L.tmp, t.mv[l].x & 3, t.mv[l].y & 3, tw, th, bd, tid, pred[l]);
         }
         S* dst = (S*)P.cur[0];
         const int ox = tid & 15, oy0 = (tid >> 4) * 4;
         if (ox < tw && oy0 < th) {
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                dst[(size_t)(y + oy0 + k) * P.pitch[0] + x + ox] =
-                    (S)combine(P, 0, dir, pu.ref_idx, pred[0][k], pred[1][k], bd);
+                dst[(y + oy0 + k) * P.pitch[0] + x + ox] = (S)combine(P, 0, dir, ref_idx, pred[0][k], pred[1][k], bd);
         }
     }
     // Chroma (4:2:0): eighth-sample vectors in chroma units
@@ -318,12 +369,12 @@ __device__ void mc_tile(const HevcPicDev& P, const McTile& t, int tid, McLds& L)
         int pred[2][1] = {{0}, {0}};
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            mc_predict<4, 16, 8, 1>(L.chroma[l][c - 1], L.tmp, pu.mv[l].x & 7, pu.mv[l].y & 7, cw, ch, bd, tid, pred[l]);
+            mc_predict<4, 16, 8, 1>(L.chroma[l][c - 1], L.tmp, t.mv[l].x & 7, t.mv[l].y & 7, cw, ch, bd, tid, pred[l]);
         }
-        int oy = tid >> 3, ox = tid & 7;
+        const int oy = tid >> 3, ox = tid & 7;
         if (ox < cw && oy < ch) {
             S* dst = (S*)P.cur[c];
-            dst[(size_t)(cy + oy) * P.pitch[c] + cx + ox] = (S)combine(P, c, dir, pu.ref_idx, pred[0][0], pred[1][0], bd);
+            dst[(cy + oy) * P.pitch[c] + cx + ox] = (S)combine(P, c, dir, ref_idx, pred[0][0], pred[1][0], bd);
         }
     }
 }
@@ -364,7 +415,7 @@ __device__ void side_entry(const HevcPicDev& P, int k) {
     }
 }
 
-__global__ void __launch_bounds__(64) hevc_stage0(BatchDev b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage0(BatchDev b) {
     __shared__ McLds lds;
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = xcd_item();
@@ -385,9 +436,16 @@ __global__ void __launch_bounds__(64) hevc_stage0(BatchDev b) {
         return;
     }
     i -= P.n_intra;
-    const McTile t = P.tiles[i];
-    if (P.bd == 8) mc_tile<uint8_t>(P, t, tid, lds);
-    else mc_tile<uint16_t>(P, t, tid, lds);
+    McTile t;
+    *(uint4*)&t = *(const uint4*)&P.tiles[i];
+    TileLoads w;
+    if (P.bd == 8) {
+        tile_issue_t<uint8_t>(P, t, w, tid);
+        tile_finish_t<uint8_t>(P, t, w, tid, lds);
+    } else {
+        tile_issue_t<uint16_t>(P, t, w, tid);
+        tile_finish_t<uint16_t>(P, t, w, tid, lds);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -626,7 +684,7 @@ __device__ void block_bs(const HevcPicDev& P, int q) {
     P.bs[q] = (uint8_t)out;
 }
 
-__global__ void __launch_bounds__(64) hevc_stage1(BatchDev b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage1(BatchDev b) {
     __shared__ int16_t lds[2 * 32 * 33];
     const int32_t* pre = b.prefix + 1 * (b.n + 1);
     const int it = xcd_item();

@@ -456,11 +456,17 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             for (int dy = 0; dy < u.h; dy += 16)
                 for (int dx = 0; dx < u.w; dx += 16) {
                     rv::McTile t;
-                    t.pu = (uint32_t)i;
-                    t.dx = (uint8_t)dx;
-                    t.dy = (uint8_t)dy;
+                    std::memset(&t, 0, sizeof(t));
+                    t.x = (uint16_t)(u.x + dx);
+                    t.y = (uint16_t)(u.y + dy);
                     t.tw = (uint8_t)std::min(16, u.w - dx);
                     t.th = (uint8_t)std::min(16, u.h - dy);
+                    t.dir = u.dir;
+                    t.refs = 0;
+                    for (int l = 0; l < 2; ++l) {
+                        if ((u.dir >> l) & 1) t.refs |= (uint8_t)(u.ref_idx[l] << (4 * l));
+                        t.mv[l] = u.mv[l];
+                    }
                     tiles.push_back(t);
                 }
         }

@@ -10,13 +10,17 @@
 namespace rv {
 
 // One motion-compensation work item: a tile of at most 16x16 luma samples
-// (and its two 4:2:0 chroma tiles) inside one prediction unit. Splitting
-// PUs into tiles on the host keeps work items of even size.
-struct McTile {
-    uint32_t pu;
-    uint8_t dx, dy;  // tile offset inside the PU, luma samples
-    uint8_t tw, th;  // tile size, luma samples (multiples of 4, <= 16)
+// (and its two 4:2:0 chroma tiles) of one prediction unit, carrying the
+// PU's motion so a wave reads one record per item. Splitting PUs into
+// tiles on the host keeps work items of even size.
+struct alignas(16) McTile {
+    uint16_t x, y;    // tile top-left, luma samples
+    uint8_t tw, th;   // tile size, luma samples (multiples of 4, <= 16)
+    uint8_t dir;      // bit 0: list 0, bit 1: list 1
+    uint8_t refs;     // ref_idx of list 0 in bits 0..3, list 1 in bits 4..7
+    rv_mv mv[2];
 };
+static_assert(sizeof(McTile) == 16, "one 16-byte load per tile");
 
 // Per-picture descriptor of an HEVC job, resident in device memory.
 struct HevcPicDev {
