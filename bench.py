@@ -90,6 +90,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: one per picture, <= 16)")
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the 256 MiB copy probe once (counter calibration) and exit")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serve-loop measurement (config 5)")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -209,10 +210,26 @@ def main() -> int:
             "stage_bytes": {k: int(v) for k, v in zip(STAGES, stage_bytes)},
         },
     }
+    if not args.graph_only and not args.no_serve:
+        from rivet_hip import serve
+
+        for j in jobs:
+            j.close()
+        jobs = []
+        batch.close()
+        sessions = None
+        log("[bench] serve loop (config 5)")
+        sv = serve.experiment(log=log)
+        log(f"[bench] serve: batched {sv['batched']['luma_msamples_per_s']} Msample/s p50 "
+            f"{sv['batched']['latency_ms_p50']} ms p99 {sv['batched']['latency_ms_p99']} ms; plain "
+            f"{sv['plain']['luma_msamples_per_s']} Msample/s p99 {sv['plain']['latency_ms_p99']} ms")
+        out["serve_p99_ms"] = sv["batched"]["latency_ms_p99"]
+        out["extra"]["serve"] = sv
     print(json.dumps(out), flush=True)
     for j in jobs:
         j.close()
-    batch.close()
+    if jobs:
+        batch.close()
     return 0 if bit_exact else 1
 
 

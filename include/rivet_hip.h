@@ -26,8 +26,10 @@
  * with RV_E_RANGE.
  *
  * Threading: every entry is re-entrant. Sessions are independent and may
- * be driven from different host threads; rv_jobs_launch serialises on an
- * internal lock. Completion is observed with rv_jobs_wait (HIP events).
+ * be driven from different host threads; launches serialise on an internal
+ * lock. Completion is observed with rv_sync or markers (HIP events).
+ * Kernels run on one library stream; plane uploads and downloads use a
+ * second stream ordered after the kernels they depend on.
  */
 #ifndef RIVET_HIP_H
 #define RIVET_HIP_H
@@ -101,6 +103,16 @@ typedef struct rv_hevc_pu {
     rv_mv mv[2];        /* quarter-sample vectors per list             */
 } rv_hevc_pu;
 
+/* One nonzero scaled transform coefficient: pos = y * n + x in its n x n
+ * TU. A coded TU's coefficients are the run nz[coef_off .. coef_off + n_nz)
+ * (positions distinct); every other coefficient of the TU is zero — the
+ * dense n x n raster of IdctFn (src/dsp/hevc.rs:18-21) is these values
+ * scattered into zeros. */
+typedef struct rv_nz {
+    uint16_t pos;
+    int16_t val;
+} rv_nz;
+
 /* One transform unit. Luma TUs are listed whether coded or not (their
  * edges feed the deblocking filter, src/hevc/ctu.rs:1493-1510); chroma TUs
  * only when coded. */
@@ -111,8 +123,9 @@ typedef struct rv_hevc_tu {
     uint8_t flags;      /* RV_TU_DST | RV_TU_CODED                     */
     uint8_t max_x;      /* nonzero bounds, inclusive (IdctFn,          */
     uint8_t max_y;      /*   src/dsp/hevc.rs:18-21)                    */
-    uint8_t pad[3];
-    uint32_t coef_off;  /* coded: offset of the n*n raster in coefs    */
+    uint8_t pad;
+    uint16_t n_nz;      /* coded: number of nonzero coefficients       */
+    uint32_t coef_off;  /* coded: first entry in nz                    */
 } rv_hevc_tu;
 
 #define RV_TU_DST 1u   /* 4x4 DST (intra luma 4x4, src/hevc/ctu.rs:2041) */
@@ -145,8 +158,8 @@ typedef struct rv_hevc_pic {
     const rv_hevc_cu* cus;
     const rv_hevc_pu* pus;
     const rv_hevc_tu* tus;
-    const int16_t* coefs;
-    uint32_t n_coefs;
+    const rv_nz* nz;             /* nonzero coefficients of coded TUs    */
+    uint32_t n_nz;
     uint32_t n_intra_pred;
     const uint16_t* intra_pred;  /* samples, any depth, as uint16       */
     const rv_sao* sao;           /* 3 per CTB (Y, Cb, Cr), raster CTBs   */
@@ -296,6 +309,12 @@ int rv_mark_wait(int id);
 /* Milliseconds between two markers (both complete). */
 int rv_mark_elapsed(int id_start, int id_end, float* ms);
 
+/* Deferred frees: with enable = 1, rv_job_free / rv_batch_free /
+ * rv_session_destroy queue the object instead of synchronizing the compute
+ * stream, and rv_sync() (after synchronizing) releases the queue. For loops
+ * that keep launching while pictures complete. */
+int rv_defer_free(int enable);
+
 /* Stage switches (mirror H26X_NO_DEBLOCK / H26X_NO_SAO,
  * src/hevc/decoder.rs:1247-1248): bit 0 skip deblocking, bit 1 skip SAO.
  * Also read once from RIVET_HIP_NO_DEBLOCK / RIVET_HIP_NO_SAO. */
@@ -313,7 +332,7 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs);
 
 /* sizeof of the work-list structs, for bindings to check their layout:
  * 0 rv_hevc_cu, 1 rv_hevc_pu, 2 rv_hevc_tu, 3 rv_sao, 4 rv_hevc_pic,
- * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic;
+ * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic, 9 rv_nz;
  * -1 for an unknown index. */
 int rv_abi_size(int which);
 

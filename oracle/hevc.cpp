@@ -975,8 +975,15 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
             const rv_hevc_tu& tu = pic->tus[i];
             if (!(tu.flags & RV_TU_CODED)) continue;
             int n = 1 << tu.log2;
-            if (tu.coef_off + (uint64_t)n * n > pic->n_coefs) return RV_E_RANGE;
-            std::memcpy(coeffs.data(), pic->coefs + tu.coef_off, n * n * sizeof(int16_t));
+            if (tu.coef_off + (uint64_t)tu.n_nz > pic->n_nz) return RV_E_RANGE;
+            // The dense raster the reference's IdctFn takes: the nonzero
+            // list scattered into zeros.
+            std::fill(coeffs.begin(), coeffs.begin() + n * n, (int16_t)0);
+            for (uint32_t k = 0; k < tu.n_nz; ++k) {
+                const rv_nz& e = pic->nz[tu.coef_off + k];
+                if (e.pos >= n * n) return RV_E_RANGE;
+                coeffs[e.pos] = e.val;
+            }
             int bd_shift = 20 - bd;
             if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
             else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);

@@ -61,7 +61,7 @@ struct rv_gen {
     std::vector<rv_hevc_cu> cus;
     std::vector<rv_hevc_pu> pus;
     std::vector<rv_hevc_tu> tus;
-    std::vector<int16_t> coefs16;
+    std::vector<rv_nz> nz;
     std::vector<uint16_t> intra_pred;
     std::vector<rv_sao> sao;
     std::vector<rv_h264_mb> mbs;
@@ -82,32 +82,26 @@ struct HevcBuilder {
         : c(cfg), g(gen), rs(rv_gen_subseed(seed, 0, 0, STRUCT)), rc(rv_gen_subseed(seed, 0, 0, COEF)),
           ri(rv_gen_subseed(seed, 0, 0, INTRA)), W(cfg.width), H(cfg.height), max((1 << cfg.bit_depth) - 1) {}
 
-    // Coefficients of one n x n TU: returns whether any is nonzero.
+    // Coefficients of one n x n TU (nonzero list, raster order): returns
+    // whether any is nonzero.
     bool coefs(int n, rv_hevc_tu& tu) {
-        uint32_t off = (uint32_t)g.coefs16.size();
-        g.coefs16.resize(off + n * n, 0);
+        const uint32_t off = (uint32_t)g.nz.size();
         int mx = 0, my = 0;
-        bool any = false;
         for (int y = 0; y < n; ++y)
             for (int x = 0; x < n; ++x) {
                 if (!rc.permille(c.nz_permille)) continue;
                 int m = rc.laplace(c.lap_b);
                 int v = (rc.next() & 1) ? -m : m;
-                g.coefs16[off + y * n + x] = (int16_t)v;
-                any = true;
+                g.nz.push_back(rv_nz{(uint16_t)(y * n + x), (int16_t)v});
                 if (x > mx) mx = x;
                 if (y > my) my = y;
             }
-        if (!any) {
-            g.coefs16.resize(off);
-            tu.coef_off = 0;
-            tu.max_x = tu.max_y = 0;
-            return false;
-        }
-        tu.coef_off = off;
-        tu.max_x = (uint8_t)mx;
-        tu.max_y = (uint8_t)my;
-        return true;
+        const uint32_t cnt = (uint32_t)g.nz.size() - off;
+        tu.coef_off = cnt ? off : 0;
+        tu.n_nz = (uint16_t)cnt;
+        tu.max_x = (uint8_t)(cnt ? mx : 0);
+        tu.max_y = (uint8_t)(cnt ? my : 0);
+        return cnt != 0;
     }
 
     void add_tu(int comp, int x, int y, int log2, bool dst, bool always) {
@@ -283,8 +277,8 @@ struct HevcBuilder {
         p.cus = g.cus.data();
         p.pus = g.pus.data();
         p.tus = g.tus.data();
-        p.coefs = g.coefs16.data();
-        p.n_coefs = (uint32_t)g.coefs16.size();
+        p.nz = g.nz.data();
+        p.n_nz = (uint32_t)g.nz.size();
         p.intra_pred = g.intra_pred.data();
         p.n_intra_pred = (uint32_t)g.intra_pred.size();
         p.sao = g.sao.data();

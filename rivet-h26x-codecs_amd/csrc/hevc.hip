@@ -540,12 +540,20 @@ __device__ void tu_group(const HevcPicDev& P, int first, int count, int tid, int
     int16_t* t = lds + slot * N * LS;
     const bool dst = N == 4 && (tu.flags & RV_TU_DST);
     (void)TPW;
+    // The dense raster in LDS: zeros, then the TU's nonzero list.
+    for (int e = lane; e < N * LS; e += N) t[e] = 0;
+    __syncthreads();
+    if (active)
+        for (int k = lane; k < tu.n_nz; k += N) {
+            const rv_nz z = P.nz[tu.coef_off + k];
+            t[(z.pos >> (31 - __builtin_clz(N))) * LS + (z.pos & (N - 1))] = z.val;
+        }
+    __syncthreads();
     if (active) {
-        // Column `lane`.
+        // Column `lane`, in place.
         int s[N], o[N];
-        const int16_t* c = P.coefs + tu.coef_off + lane;
 #pragma unroll
-        for (int j = 0; j < N; ++j) s[j] = c[j * N];
+        for (int j = 0; j < N; ++j) s[j] = t[j * LS + lane];
         if constexpr (N == 4) {
             if (dst) idst1d(s, o);
             else idct1d<4>(s, o);

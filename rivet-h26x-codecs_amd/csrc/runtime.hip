@@ -23,7 +23,11 @@ thread_local std::string g_err;
 std::mutex g_mu;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+hipStream_t g_io = nullptr;  // plane uploads / downloads
 int g_stage_mask = 0;
+bool g_defer = false;
+std::vector<void*> g_deferred;  // device allocations freed at the next rv_sync
+std::vector<void*> g_deferred_host;
 constexpr int kMarks = 4096;
 hipEvent_t g_marks[kMarks];
 int g_next_mark = 0;
@@ -125,7 +129,7 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
             if (p->ref_slot[l][r] < 0 || p->ref_slot[l][r] >= s->slots) return fail(RV_E_RANGE, "ref_slot outside the DPB");
     }
     if (p->n_cus < 0 || p->n_pus < 0 || p->n_tus < 0) return fail(RV_E_INVALID, "negative list length");
-    if ((p->n_cus && !p->cus) || (p->n_pus && !p->pus) || (p->n_tus && !p->tus) || (p->n_coefs && !p->coefs) ||
+    if ((p->n_cus && !p->cus) || (p->n_pus && !p->pus) || (p->n_tus && !p->tus) || (p->n_nz && !p->nz) ||
         (p->n_intra_pred && !p->intra_pred))
         return fail(RV_E_INVALID, "null work list");
     if ((p->sao_luma || p->sao_chroma) && !p->sao) return fail(RV_E_INVALID, "SAO enabled without parameters");
@@ -166,7 +170,16 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
         if ((t.flags & RV_TU_DST) && (t.log2 != 2 || t.comp != 0)) return fail(RV_E_RANGE, "DST on a non-4x4 luma TU");
         if (t.flags & RV_TU_CODED) {
             if (t.max_x >= n || t.max_y >= n) return fail(RV_E_RANGE, "nonzero bounds outside the TU");
-            if ((uint64_t)t.coef_off + (uint64_t)n * n > p->n_coefs) return fail(RV_E_RANGE, "coefficients outside coefs");
+            if (t.n_nz == 0 || t.n_nz > n * n || (uint64_t)t.coef_off + t.n_nz > p->n_nz)
+                return fail(RV_E_RANGE, "nonzero list outside nz");
+            // The reference transforms only inside the bounds (idct_scalar,
+            // src/dsp/hevc.rs:499-526); a nonzero outside them would be
+            // dropped there and kept here, so it is refused.
+            for (uint32_t k = 0; k < t.n_nz; ++k) {
+                const rv_nz& e = p->nz[t.coef_off + k];
+                if (e.pos >= n * n || ((t.flags & RV_TU_DST) == 0 && ((e.pos & (n - 1)) > t.max_x || (e.pos >> t.log2) > t.max_y)))
+                    return fail(RV_E_RANGE, "nonzero coefficient outside the TU or its bounds");
+            }
         }
         if (t.comp == 0) tu_area += (int64_t)n * n;
     }
@@ -195,10 +208,15 @@ void hevc_bytes(const rv_hevc_pic* p, int64_t out[4]) {
     for (int i = 0; i < p->n_cus; ++i)
         if (p->cus[i].intra) mc += (int64_t)(1 << p->cus[i].log2_size) * (1 << p->cus[i].log2_size) * 3 / 2 * 2;
     out[0] = s * (mc + planes);
-    int64_t coef = 0;
+    int64_t area = 0, nzc = 0;
     for (int i = 0; i < p->n_tus; ++i)
-        if (p->tus[i].flags & RV_TU_CODED) coef += (int64_t)1 << (2 * p->tus[i].log2);
-    out[1] = 2 * coef + 2 * s * coef;
+        if (p->tus[i].flags & RV_TU_CODED) {
+            area += (int64_t)1 << (2 * p->tus[i].log2);
+            nzc += p->tus[i].n_nz;
+        }
+    // Nonzero entries read once (4 bytes each) + the residual add's
+    // read-modify-write of the TU samples.
+    out[1] = 4 * nzc + 2 * s * area;
     bool chroma = false;
     for (int i = 0; i < p->n_cus && !chroma; ++i) chroma = p->cus[i].intra != 0;
     int64_t luma = (int64_t)p->width * p->height;
@@ -310,6 +328,7 @@ int rv_abi_size(int which) {
         case 6: return (int)sizeof(rv_h264_part);
         case 7: return (int)sizeof(rv_h264_blk);
         case 8: return (int)sizeof(rv_h264_pic);
+        case 9: return (int)sizeof(rv_nz);
         default: return -1;
     }
 }
@@ -336,6 +355,7 @@ int rv_init(int device_ordinal) {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             return fail(RV_E_NODEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
         RV_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+        RV_HIP(hipStreamCreateWithFlags(&g_io, hipStreamNonBlocking));
         for (int i = 0; i < kMarks; ++i) RV_HIP(hipEventCreate(&g_marks[i]));
         const char* e = std::getenv("RIVET_HIP_NO_DEBLOCK");
         if (e && *e && *e != '0') g_stage_mask |= 1;
@@ -407,6 +427,13 @@ int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_s
 
 int rv_session_destroy(rv_session* s) {
     if (!s) return fail(RV_E_INVALID, "null session");
+    if (g_defer) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (void* p : s->planes)
+            if (p) g_deferred.push_back(p);
+        delete s;
+        return RV_OK;
+    }
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     for (void* p : s->planes)
         if (p) (void)hipFree(p);
@@ -417,24 +444,38 @@ int rv_session_destroy(rv_session* s) {
 int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb, const void* cr) {
     if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
     if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+    if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+    // Ordered after every kernel launched so far (a slot may be rewritten
+    // only once its readers have run), without waiting for later launches.
+    hipEvent_t ev;
+    RV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    RV_HIP(hipEventRecord(ev, g_stream));
+    RV_HIP(hipStreamWaitEvent(g_io, ev, 0));
     const void* src[3] = {y, cb, cr};
     for (int c = 0; c < 3; ++c)
         RV_HIP(hipMemcpy2DAsync(s->planes[(size_t)slot * 3 + c], (size_t)s->g.pitch[c] * s->g.ss, src[c],
                                 (size_t)s->g.pw[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
-                                hipMemcpyHostToDevice, g_stream));
-    RV_HIP(hipStreamSynchronize(g_stream));
+                                hipMemcpyHostToDevice, g_io));
+    RV_HIP(hipStreamSynchronize(g_io));
+    (void)hipEventDestroy(ev);
     return RV_OK;
 }
 
 int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr) {
     if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
     if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+    if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+    hipEvent_t ev;
+    RV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    RV_HIP(hipEventRecord(ev, g_stream));
+    RV_HIP(hipStreamWaitEvent(g_io, ev, 0));
     void* dst[3] = {y, cb, cr};
     for (int c = 0; c < 3; ++c)
         RV_HIP(hipMemcpy2DAsync(dst[c], (size_t)s->g.pw[c] * s->g.ss, s->planes[(size_t)slot * 3 + c],
                                 (size_t)s->g.pitch[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
-                                hipMemcpyDeviceToHost, g_stream));
-    RV_HIP(hipStreamSynchronize(g_stream));
+                                hipMemcpyDeviceToHost, g_io));
+    RV_HIP(hipStreamSynchronize(g_io));
+    (void)hipEventDestroy(ev);
     return RV_OK;
 }
 
@@ -481,12 +522,11 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         // TUs read one run of memory.
         std::vector<rv_hevc_tu> ctus;
         for (auto& v : bysize) ctus.insert(ctus.end(), v.begin(), v.end());
-        std::vector<int16_t> coefs;
-        coefs.reserve(p->n_coefs);
+        std::vector<rv_nz> coefs;
+        coefs.reserve(p->n_nz);
         for (rv_hevc_tu& t : ctus) {
-            const size_t nn = (size_t)1 << (2 * t.log2);
             const uint32_t off = (uint32_t)coefs.size();
-            coefs.insert(coefs.end(), p->coefs + t.coef_off, p->coefs + t.coef_off + nn);
+            coefs.insert(coefs.end(), p->nz + t.coef_off, p->nz + t.coef_off + t.n_nz);
             t.coef_off = off;
         }
         std::vector<int32_t> intra_idx;
@@ -505,7 +545,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         // Device arena.
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
                     carve_size<rv_hevc_tu>(ltus.size()) + carve_size<rv_hevc_tu>(ctus.size()) +
-                    carve_size<rv::McTile>(tiles.size()) + carve_size<int16_t>(coefs.size()) +
+                    carve_size<rv::McTile>(tiles.size()) + carve_size<rv_nz>(coefs.size()) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(intra_idx.size()) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -527,7 +567,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, ltus.size());
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, ctus.size());
         rv::McTile* h_tiles = carve<rv::McTile>(hp, tiles.size());
-        int16_t* h_coefs = carve<int16_t>(hp, coefs.size());
+        rv_nz* h_coefs = carve<rv_nz>(hp, coefs.size());
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
         int32_t* h_intra = carve<int32_t>(hp, intra_idx.size());
@@ -539,7 +579,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         if (!ltus.empty()) std::memcpy(h_ltus, ltus.data(), ltus.size() * sizeof(rv_hevc_tu));
         if (!ctus.empty()) std::memcpy(h_ctus, ctus.data(), ctus.size() * sizeof(rv_hevc_tu));
         if (!tiles.empty()) std::memcpy(h_tiles, tiles.data(), tiles.size() * sizeof(rv::McTile));
-        if (!coefs.empty()) std::memcpy(h_coefs, coefs.data(), coefs.size() * sizeof(int16_t));
+        if (!coefs.empty()) std::memcpy(h_coefs, coefs.data(), coefs.size() * sizeof(rv_nz));
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
         std::memcpy(h_sao, sao.data(), n_sao * sizeof(rv_sao));
         // Scratch: reconstruction planes and side maps.
@@ -598,7 +638,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.ltus = (const rv_hevc_tu*)dev((char*)h_ltus);
         d.ctus = (const rv_hevc_tu*)dev((char*)h_ctus);
         d.tiles = (const rv::McTile*)dev((char*)h_tiles);
-        d.coefs = (const int16_t*)dev((char*)h_coefs);
+        d.nz = (const rv_nz*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.sao = (const rv_sao*)dev((char*)h_sao);
         d.cuinfo = (uint8_t*)sp;
@@ -767,6 +807,13 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
 
 int rv_job_free(rv_job* j) {
     if (!j) return fail(RV_E_INVALID, "null job");
+    if (g_defer) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (j->arena) g_deferred.push_back(j->arena);
+        if (j->scratch) g_deferred.push_back(j->scratch);
+        delete j;
+        return RV_OK;
+    }
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     if (j->arena) (void)hipFree(j->arena);
     if (j->scratch) (void)hipFree(j->scratch);
@@ -841,6 +888,14 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
 
 int rv_batch_free(rv_batch* b) {
     if (!b) return fail(RV_E_INVALID, "null batch");
+    if (g_defer) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->tables) g_deferred.push_back(b->tables);
+        delete b;
+        return RV_OK;
+    }
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -975,6 +1030,18 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
 int rv_sync(void) {
     if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
     RV_HIP(hipStreamSynchronize(g_stream));
+    std::vector<void*> q;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        q.swap(g_deferred);
+    }
+    for (void* p : q) (void)hipFree(p);
+    return RV_OK;
+}
+
+int rv_defer_free(int enable) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_defer = enable != 0;
     return RV_OK;
 }
 

@@ -47,7 +47,7 @@ struct HevcPicDev {
     const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
     const McTile* tiles;
     const rv_hevc_tu* ctus;  // coded TUs grouped by size, smallest first
-    const int16_t* coefs;
+    const rv_nz* nz;         // nonzero coefficients of ctus, in ctus order
     const uint16_t* intra_pred;
     const rv_sao* sao;
     // Per-4x4 side data (job scratch).

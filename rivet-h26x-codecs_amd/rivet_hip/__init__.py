@@ -253,5 +253,10 @@ def probe_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
     return float(v.value)
 
 
+def defer_free(enable: bool) -> None:
+    """Queue frees until the next sync() instead of synchronizing on each."""
+    check(lib.rv_defer_free(1 if enable else 0))
+
+
 def set_stage_mask(mask: int) -> None:
     check(lib.rv_set_stage_mask(mask))

@@ -52,9 +52,13 @@ class HevcPu(C.Structure):
                 ("pad1", u16), ("mv", Mv * 2)]
 
 
+class Nz(C.Structure):
+    _fields_ = [("pos", u16), ("val", i16)]
+
+
 class HevcTu(C.Structure):
     _fields_ = [("x", u16), ("y", u16), ("comp", u8), ("log2", u8), ("flags", u8), ("max_x", u8), ("max_y", u8),
-                ("pad", u8 * 3), ("coef_off", u32)]
+                ("pad", u8), ("n_nz", u16), ("coef_off", u32)]
 
 
 class HevcPic(C.Structure):
@@ -67,7 +71,7 @@ class HevcPic(C.Structure):
         ("sao_luma", i32), ("sao_chroma", i32),
         ("n_cus", i32), ("n_pus", i32), ("n_tus", i32),
         ("cus", C.POINTER(HevcCu)), ("pus", C.POINTER(HevcPu)), ("tus", C.POINTER(HevcTu)),
-        ("coefs", C.POINTER(i16)), ("n_coefs", u32), ("n_intra_pred", u32),
+        ("nz", C.POINTER(Nz)), ("n_nz", u32), ("n_intra_pred", u32),
         ("intra_pred", C.POINTER(u16)), ("sao", C.POINTER(Sao)),
     ]
 
@@ -112,7 +116,7 @@ class GenH264Cfg(C.Structure):
         ("ref_poc", i32 * 2), ("ref_slot", i32 * 2)]
 
 
-ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic]
+ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz]
 
 # Every extern "C" entry of include/rivet_hip.h and include/rivet_gen.h:
 # name -> (restype, argtypes).
@@ -140,6 +144,7 @@ PROTOTYPES = {
     "rv_mark_wait": (i32, [i32]),
     "rv_mark_elapsed": (i32, [i32, i32, C.POINTER(C.c_float)]),
     "rv_set_stage_mask": (i32, [i32]),
+    "rv_defer_free": (i32, [i32]),
     "rv_job_bytes": (i32, [P, C.POINTER(i64)]),
     "rv_abi_size": (i32, [i32]),
     "rv_probe_copy": (i32, [i64, i32, C.POINTER(C.c_double)]),

@@ -1,0 +1,348 @@
+"""Continuous-batching serve loop over decode sessions (config 5).
+
+Sessions arrive by a Poisson process. Each is admitted when its device
+footprint (DPB planes + the work lists and scratch of its pictures) fits
+the HBM budget, else it waits in FIFO order. An admitted session's DPB
+stays resident: its I picture is uploaded once, and every P/B picture is
+reconstructed on the device from references already there. A picture is
+*ready* when its session is admitted and all its references are complete;
+the loop launches every ready picture of every session together (one
+batched launch per stage), keeps up to two batches in flight, and records
+per-picture latency from ready to complete. The plain path runs the same
+trace with one launch sequence per picture.
+
+GOP (decode order): I0 P3 B1 B2 P6 B4 B5 ... P27 B25 B26 P29 B28. P
+pictures reference the previous anchor, B pictures the anchors around
+them. DPB slots: three anchors in rotation and two B slots.
+
+Work lists are generated before the timed run (they stand for the
+parser's output, produced upstream of this path); uploading them to the
+device happens inside the loop, at admission.
+"""
+from __future__ import annotations
+
+import collections
+import concurrent.futures as cf_
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (RV_CODEC_HEVC, RV_LAUNCH_BATCHED, RV_LAUNCH_PLAIN, Batch, GenPicture, Job, Session, defer_free,
+               gen_planes, mark, mark_done, mark_wait, subseed, sync)
+from . import configs as cf
+
+RESOLUTIONS = [(856, 480), (1280, 720), (1920, 1080), (3840, 2160)]  # 854x480 coded 856 wide
+GOP = 30
+ANCHOR_SLOTS = (0, 1, 2)
+B_SLOTS = (3, 4)
+DPB_SLOTS = 5
+
+
+def gop_order(n: int = GOP):
+    """(poc, kind, refs) in decode order; refs are POCs (L0, L1)."""
+    anchors = list(range(0, n - 2, 3)) + [n - 1]
+    out = [(0, "I", ())]
+    for a0, a1 in zip(anchors, anchors[1:]):
+        out.append((a1, "P", (a0,)))
+        for b in range(a0 + 1, a1):
+            out.append((b, "B", (a0, a1)))
+    return out
+
+
+@dataclass
+class Pic:
+    sess: "Sess"
+    poc: int
+    kind: str
+    refs: tuple
+    slot: int
+    ref_slots: tuple
+    gen: GenPicture | None = None
+    job: Job | None = None
+    ready_t: float = math.nan
+    done_t: float = math.nan
+    launched: bool = False
+    done: bool = False
+    luma: int = 0
+    out: list | None = None
+    fut: object = None
+
+
+@dataclass
+class Sess:
+    idx: int
+    width: int
+    height: int
+    bit_depth: int
+    seed: int
+    arrival: float
+    pics: list = field(default_factory=list)
+    session: Session | None = None
+    admitted_t: float = math.nan
+    footprint: int = 0
+    remaining: int = 0
+
+
+def session_cfgs(n: int, seed: int):
+    """Resolution uniform over the four sizes; 70 % Main 8-bit, 30 % Main10."""
+    rng = np.random.default_rng(subseed(seed, 5, 0, 0) & 0xFFFFFFFF)
+    out = []
+    for i in range(n):
+        w, h = RESOLUTIONS[int(rng.integers(0, 4))]
+        bd = 8 if rng.random() < 0.7 else 10
+        out.append((w, h, bd, subseed(seed, 5, i, 1)))
+    return out
+
+
+def _pic_cfg(w, h, bd, kind, poc, ref_pocs, ref_slots):
+    base = dict(cf.HEVC_C1, width=w, height=h, bit_depth=bd)
+    if kind == "P":
+        base.update(b_picture=0, poc=poc, ref_poc=[ref_pocs[0], ref_pocs[0]], ref_slot=[ref_slots[0], ref_slots[0]])
+    else:
+        base.update(b_picture=1, bi_percent=60, poc=poc, ref_poc=list(ref_pocs), ref_slot=list(ref_slots))
+    return base
+
+
+def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop: int = GOP, threads: int = 16,
+                   sizes=None):
+    """Sessions with generated work lists. rate = 0: all arrive at t = 0."""
+    specs = session_cfgs(n, seed)
+    if sizes is not None:
+        specs = [(sizes[i % len(sizes)][0], sizes[i % len(sizes)][1], s[2], s[3]) for i, s in enumerate(specs)]
+    rng = np.random.default_rng(subseed(seed, 5, 0, 2) & 0xFFFFFFFF)
+    t = 0.0
+    sessions = []
+    for i, (w, h, bd, sseed) in enumerate(specs):
+        if rate > 0:
+            t += float(rng.exponential(1.0 / rate))
+        s = Sess(i, w, h, bd, sseed, t)
+        slot_of = {}
+        anchor_k = 0
+        b_k = 0
+        for poc, kind, refs in gop_order(gop):
+            if kind == "B":
+                slot = B_SLOTS[b_k % 2]
+                b_k += 1
+            else:
+                slot = ANCHOR_SLOTS[anchor_k % 3]
+                anchor_k += 1
+            slot_of[poc] = slot
+            p = Pic(s, poc, kind, refs, slot, tuple(slot_of[r] for r in refs))
+            p.luma = cf.visible_luma(w, h)
+            s.pics.append(p)
+        sessions.append(s)
+
+    def gen(p: Pic):
+        if p.kind == "I":
+            return None
+        s = p.sess
+        cfg = _pic_cfg(s.width, s.height, s.bit_depth, p.kind, p.poc, p.refs, p.ref_slots)
+        return GenPicture(RV_CODEC_HEVC, cfg, subseed(s.seed, 7, p.poc, 0))
+
+    todo = [p for s in sessions for p in s.pics if p.kind != "I"]
+    with cf_.ThreadPoolExecutor(max_workers=threads) as ex:
+        for p, g in zip(todo, ex.map(gen, todo)):
+            p.gen = g
+    return sessions
+
+
+def i_planes(s: Sess):
+    return gen_planes(s.width, s.height, s.bit_depth, subseed(s.seed, 7, 0, 1))
+
+
+def footprint(s: Sess) -> int:
+    """DPB + per picture: reconstruction scratch and its work lists."""
+    dpb = Session.footprint(RV_CODEC_HEVC, s.width, s.height, s.bit_depth, DPB_SLOTS)
+    per = 0
+    for p in s.pics:
+        if p.gen is None:
+            continue
+        c = p.gen.pic.contents
+        per += c.n_nz * 4 + c.n_tus * 16 + c.n_pus * 40 + c.n_cus * 12
+    ss = 2 if s.bit_depth > 8 else 1
+    per += (len(s.pics) - 1) * (s.width * s.height * 3 // 2 * ss + s.width * s.height // 2)
+    return dpb + per
+
+
+def run(sessions, mode: str = "batched", budget: int = 32 << 30, max_batch: int = 64, inflight_max: int = 2,
+        record: bool = False, upload_threads: int = 8):
+    """Serve every session; returns throughput and latency statistics.
+    record=True downloads every reconstructed picture as it completes (into
+    Pic.out; use inflight_max=1 so no later picture has overwritten it).
+    Work-list uploads run on `upload_threads` host threads from admission
+    on, overlapping the reconstruction of earlier pictures."""
+    defer_free(True)
+    ex = cf_.ThreadPoolExecutor(max_workers=upload_threads)
+    try:
+        return _run(sessions, mode, budget, max_batch, inflight_max, record, ex)
+    finally:
+        ex.shutdown(wait=True)
+        sync()
+        defer_free(False)
+
+
+def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
+    t0 = time.perf_counter()
+    arrivals = collections.deque(sorted(sessions, key=lambda s: s.arrival))
+    waiting = collections.deque()
+    active = []
+    used = 0
+    inflight = collections.deque()
+    total = sum(len(s.pics) - 1 for s in sessions)
+    done = 0
+    lat = []
+    batches = 0
+    batch_sizes = []
+    for s in sessions:
+        s.footprint = footprint(s)
+        s.remaining = len(s.pics) - 1
+        for p in s.pics:
+            p.launched = p.done = False
+            p.ready_t = p.done_t = math.nan
+
+    def now():
+        return time.perf_counter() - t0
+
+    def complete(p: Pic, t: float):
+        nonlocal done
+        p.done = True
+        p.done_t = t
+        if p.kind != "I":
+            done += 1
+            lat.append(p.done_t - p.ready_t)
+            if record:
+                p.out = p.sess.session.download(p.slot)
+            p.job.close()
+            p.job = None
+            p.sess.remaining -= 1
+        s = p.sess
+        for q in s.pics:
+            if not q.done and not q.launched and math.isnan(q.ready_t) and q.kind != "I":
+                if all(_pic_by_poc(s, r).done for r in q.refs):
+                    q.ready_t = t
+
+    def admit(s: Sess, t: float):
+        nonlocal used
+        s.session = Session(RV_CODEC_HEVC, s.width, s.height, s.bit_depth, DPB_SLOTS)
+        s.session.upload(ANCHOR_SLOTS[0], i_planes(s))
+        for p in s.pics:
+            if p.gen is not None:
+                p.job = None
+                p.fut = ex.submit(s.session.prepare, p.gen, p.slot)
+        s.admitted_t = t
+        used += s.footprint
+        active.append(s)
+        complete(s.pics[0], t)  # the I picture: host-supplied
+
+    while done < total:
+        t = now()
+        while arrivals and arrivals[0].arrival <= t:
+            waiting.append(arrivals.popleft())
+        while waiting and (used + waiting[0].footprint <= budget or not active):
+            admit(waiting.popleft(), now())
+        # Completions.
+        while inflight and mark_done(inflight[0][0]):
+            _, pics, b = inflight.popleft()
+            tc = now()
+            b.close()
+            for p in pics:
+                complete(p, tc)
+        for s in list(active):
+            if s.remaining == 0:
+                s.session.close()
+                s.session = None
+                used -= s.footprint
+                active.remove(s)
+        # Launch.
+        cand = [p for s in active for p in s.pics if not p.launched and not math.isnan(p.ready_t) and not p.done]
+        ready = []
+        for p in cand:
+            if p.job is None and p.fut is not None and p.fut.done():
+                p.job = p.fut.result()
+                p.fut = None
+            if p.job is not None:
+                ready.append(p)
+        if ready and len(inflight) < inflight_max:
+            ready.sort(key=lambda p: p.ready_t)
+            if mode == "batched":
+                group = ready[:max_batch]
+                b = Batch([p.job for p in group])
+                b.launch(RV_LAUNCH_BATCHED)
+                for p in group:
+                    p.launched = True
+                inflight.append((mark(), group, b))
+                batches += 1
+                batch_sizes.append(len(group))
+            else:
+                for p in ready:
+                    b = Batch([p.job])
+                    b.launch(RV_LAUNCH_PLAIN)
+                    p.launched = True
+                    inflight.append((mark(), [p], b))
+                    batches += 1
+                    batch_sizes.append(1)
+            continue
+        pending = [p.fut for p in cand if p.fut is not None]
+        if inflight and not pending:
+            mark_wait(inflight[0][0])
+        elif pending:
+            # Uploads still running: wake on the first one done (or soon, to
+            # look at completions and arrivals).
+            cf_.wait(pending, timeout=0.0005, return_when=cf_.FIRST_COMPLETED)
+        elif arrivals and not waiting:
+            time.sleep(max(0.0, min(arrivals[0].arrival - now(), 0.01)))
+    wall = now()
+    luma = sum(p.luma for s in sessions for p in s.pics if p.kind != "I")
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "mode": mode,
+        "pictures": total,
+        "wall_s": round(wall, 4),
+        "luma_msamples_per_s": round(luma / wall / 1e6, 1),
+        "latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
+        "latency_ms_max": round(float(lat_ms.max()), 3),
+        "batches": batches,
+        "mean_batch": round(float(np.mean(batch_sizes)), 2) if batch_sizes else 0.0,
+    }
+
+
+def _pic_by_poc(s: Sess, poc: int) -> Pic:
+    for p in s.pics:
+        if p.poc == poc:
+            return p
+    raise KeyError(poc)
+
+
+def experiment(n: int = 64, seed: int = cf.BASE_SEED, budget: int = 16 << 30, load: float = 0.8, threads: int = 16,
+               log=print):
+    """Config 5: capacity at saturation (every session arriving at t = 0),
+    then Poisson arrivals at `load` x capacity, served batched and plain on
+    the same trace."""
+    t = time.perf_counter()
+    sessions = build_sessions(n, seed, rate=0.0, threads=threads)
+    log(f"[serve] generated {n} sessions ({sum(len(s.pics) - 1 for s in sessions)} P/B pictures) in "
+        f"{time.perf_counter() - t:.1f} s")
+    sat = run(sessions, "batched", budget=budget)
+    capacity = n / sat["wall_s"]
+    rate = load * capacity
+    rng = np.random.default_rng(subseed(seed, 5, 0, 3) & 0xFFFFFFFF)
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    for s, a in zip(sessions, arr):
+        s.arrival = float(a)
+    batched = run(sessions, "batched", budget=budget)
+    plain = run(sessions, "plain", budget=budget)
+    return {
+        "sessions": n,
+        "mix": "uniform over 854x480 (coded 856), 1280x720, 1920x1080, 3840x2160; 70% Main 8-bit / 30% Main10; "
+               "GOP 30 IPBB",
+        "hbm_budget_gib": budget / (1 << 30),
+        "capacity_sessions_per_s": round(capacity, 2),
+        "saturation": sat,
+        "poisson_rate_sessions_per_s": round(rate, 2),
+        "batched": batched,
+        "plain": plain,
+        "batched_over_plain": round(batched["luma_msamples_per_s"] / plain["luma_msamples_per_s"], 3),
+    }
