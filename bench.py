@@ -44,6 +44,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "hevc_c3_8x4k_stage0_traffic.json"
+
+
+def pmc_traffic(kernel: str, streams: int):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy kernel of known size), or
+    None when the summary does not cover this kernel and workload."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if d.get("kernel") != kernel or streams != 8:
+        return None
+    return int(d["hbm_bytes_per_launch"])
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -162,7 +179,9 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": pmc_traffic(f"hevc_stage{dom}", args.streams),
+        "traffic_source": TRAFFIC_FILE.relative_to(ROOT).as_posix() + " (rocprofv3 --pmc FETCH_SIZE x2.0 + "
+                          "WRITE_SIZE x1.0, factors from the copy kernel)",
         "algorithmic_bytes": int(stage_bytes[dom]),
         "avg_launch_ms": round(stage_ms[dom], 4),
         "achievable_copy_gbs": round(copy_gbs, 1),

@@ -240,7 +240,11 @@ typedef struct rv_h264_pic {
 /* ------------------------------------------------------------------ */
 
 /* Select the device (mirrors Cpu::detect_honouring_env, src/dsp/mod.rs:
- * 200,304: chosen once). Must precede every device call. */
+ * 200,304: chosen once). Must precede every device call. Returns
+ * RV_E_NODEVICE without a gfx950 device, or when the environment sets
+ * RIVET_HIP_DISABLE=1 (the analogue of H26X_NO_SIMD, src/dsp/mod.rs:
+ * 295-350): the caller then keeps its own CPU tier. There is no CPU
+ * fallback inside this library. */
 int rv_init(int device_ordinal);
 /* "hip-gfx950" once initialised (mirrors Cpu::rung, src/dsp/mod.rs:271). */
 const char* rv_tier_name(void);
@@ -329,6 +333,158 @@ int rv_job_bytes(const rv_job* j, int64_t out[4]);
  * bytes) / average kernel time in GB/s. Also the known-size kernel that
  * calibrates hardware traffic counters. */
 int rv_probe_copy(int64_t bytes, int reps, double* gbs);
+
+/* ------------------------------------------------------------------ */
+/* Per-entry parity API                                                */
+/* ------------------------------------------------------------------ */
+/*
+ * One function per entry of HevcDsp<S> (src/dsp/hevc.rs:165-229) and
+ * H264Dsp<u8> (src/dsp/h264.rs:123-170), arguments in the entry's order
+ * and meaning. Each Rust slice `&[T]` / `&mut [T]` becomes a pointer and
+ * its length in elements (`*_len`). Before anything runs, the call checks
+ * that every element the kernel reads or writes lies inside its slice, and
+ * returns RV_E_RANGE where the reference would panic on an out-of-range
+ * index. Host buffers: the touched span is copied to the device, one
+ * kernel runs on the library stream, written spans are copied back, and
+ * the call returns when they are. Calls serialise on the library lock.
+ * Samples are uint8_t (u8 tables) or uint16_t (u16 tables, 9..16-bit).
+ * Entries whose arguments carry no sample type have one function.
+ *
+ * Not covered: the intra entries (intra_planar / intra_dc /
+ * intra_angular: intra prediction stays with the host decoder) and the
+ * MBAFF-only deblock_luma8_v / deblock_luma8_v_intra.
+ */
+
+/* IdctFn (src/dsp/hevc.rs:18-21): idct[log2 - 2], log2 = 2..5, coeffs
+ * n x n in place; entries outside max_x / max_y are taken as zero. */
+int rv_hevc_idct(int log2, int16_t* coeffs, size_t coeffs_len, int bd_shift, size_t max_x,
+                 size_t max_y);
+/* idst4 (trType 1), 4 x 4 in place. */
+int rv_hevc_idst4(int16_t* coeffs, size_t coeffs_len, int bd_shift, size_t max_x, size_t max_y);
+/* qpel_v2 / epel_v2: vertical 8 / 4 taps over 14-bit rows, >> 6. */
+int rv_hevc_qpel_v2(int16_t* dst, size_t dst_len, const int16_t* src, size_t src_len,
+                    size_t src_stride, size_t w, size_t h, size_t frac);
+int rv_hevc_epel_v2(int16_t* dst, size_t dst_len, const int16_t* src, size_t src_len,
+                    size_t src_stride, size_t w, size_t h, size_t frac);
+
+/* The entries over samples, for S = uint8_t (_u8_) and uint16_t (_u16_).
+ * AddResidualFn, CopyFn, InterpFn, UniFn, BiFn, WeightedUniFn,
+ * WeightedBiFn, UniInterpFn / BiInterpFn (the fused entries; their `tmp`
+ * scratch argument is not needed on the device and is omitted), SaoBandFn,
+ * SaoEdgeFn, LumaDeblockFn (_v: lines `stride` apart, p/q 1 apart; _h the
+ * other way round) and ChromaDeblockFn. Flags are bytes (0 / 1). */
+#define RV_HEVC_SAMPLE_ENTRIES(S, sfx)                                                           \
+    int rv_hevc_##sfx##_add_residual(S* dst, size_t dst_len, size_t stride, const int16_t* res,  \
+                                     size_t res_len, size_t n, int max);                          \
+    int rv_hevc_##sfx##_qpel_copy(int16_t* dst, size_t dst_len, const S* src, size_t src_len,     \
+                                  size_t src_stride, size_t w, size_t h, int shift);              \
+    int rv_hevc_##sfx##_qpel_h(int16_t* dst, size_t dst_len, const S* src, size_t src_len,        \
+                               size_t src_stride, size_t w, size_t h, size_t frac, int shift);    \
+    int rv_hevc_##sfx##_qpel_v(int16_t* dst, size_t dst_len, const S* src, size_t src_len,        \
+                               size_t src_stride, size_t w, size_t h, size_t frac, int shift);    \
+    int rv_hevc_##sfx##_epel_copy(int16_t* dst, size_t dst_len, const S* src, size_t src_len,     \
+                                  size_t src_stride, size_t w, size_t h, int shift);              \
+    int rv_hevc_##sfx##_epel_h(int16_t* dst, size_t dst_len, const S* src, size_t src_len,        \
+                               size_t src_stride, size_t w, size_t h, size_t frac, int shift);    \
+    int rv_hevc_##sfx##_epel_v(int16_t* dst, size_t dst_len, const S* src, size_t src_len,        \
+                               size_t src_stride, size_t w, size_t h, size_t frac, int shift);    \
+    int rv_hevc_##sfx##_uni(S* dst, size_t dst_len, size_t stride, const int16_t* src,            \
+                            size_t src_len, size_t w, size_t h, int shift, int max);              \
+    int rv_hevc_##sfx##_bi(S* dst, size_t dst_len, size_t stride, const int16_t* a, size_t a_len, \
+                           const int16_t* b, size_t b_len, size_t w, size_t h, int shift,         \
+                           int max);                                                              \
+    int rv_hevc_##sfx##_weighted_uni(S* dst, size_t dst_len, size_t stride, const int16_t* src,   \
+                                     size_t src_len, size_t w, size_t h, int log2_wd, int wt,     \
+                                     int o, int max);                                             \
+    int rv_hevc_##sfx##_weighted_bi(S* dst, size_t dst_len, size_t stride, const int16_t* a,      \
+                                    size_t a_len, const int16_t* b, size_t b_len, size_t w,       \
+                                    size_t h, int log2_wd, int w0, int w1, int o0, int o1,        \
+                                    int max);                                                     \
+    int rv_hevc_##sfx##_qpel_uni(S* dst, size_t dst_len, size_t dst_stride, const S* src,         \
+                                 size_t src_len, size_t src_stride, size_t w, size_t h,           \
+                                 size_t fx, size_t fy, unsigned bit_depth);                       \
+    int rv_hevc_##sfx##_epel_uni(S* dst, size_t dst_len, size_t dst_stride, const S* src,         \
+                                 size_t src_len, size_t src_stride, size_t w, size_t h,           \
+                                 size_t fx, size_t fy, unsigned bit_depth);                       \
+    int rv_hevc_##sfx##_qpel_bi(S* dst, size_t dst_len, size_t dst_stride, const S* src,          \
+                                size_t src_len, size_t src_stride, size_t w, size_t h, size_t fx, \
+                                size_t fy, const int16_t* other, size_t other_len,                \
+                                unsigned bit_depth);                                              \
+    int rv_hevc_##sfx##_epel_bi(S* dst, size_t dst_len, size_t dst_stride, const S* src,          \
+                                size_t src_len, size_t src_stride, size_t w, size_t h, size_t fx, \
+                                size_t fy, const int16_t* other, size_t other_len,                \
+                                unsigned bit_depth);                                              \
+    int rv_hevc_##sfx##_sao_band(S* dst, size_t dst_len, size_t dst_stride, const S* src,         \
+                                 size_t src_len, size_t src_stride, size_t w, size_t h,           \
+                                 const int16_t table[32], int shift, int max);                    \
+    int rv_hevc_##sfx##_sao_edge(S* dst, size_t dst_len, const S* src, size_t src_len,            \
+                                 size_t origin, size_t stride, size_t w, size_t h, ptrdiff_t na,  \
+                                 ptrdiff_t nb, const int16_t off[5], int max);                    \
+    int rv_hevc_##sfx##_deblock_luma_v(S* data, size_t data_len, size_t off, size_t stride,       \
+                                       const int32_t beta[2], const int32_t tc[2],                \
+                                       const uint8_t no_p[2], const uint8_t no_q[2], int max);    \
+    int rv_hevc_##sfx##_deblock_luma_h(S* data, size_t data_len, size_t off, size_t stride,       \
+                                       const int32_t beta[2], const int32_t tc[2],                \
+                                       const uint8_t no_p[2], const uint8_t no_q[2], int max);    \
+    int rv_hevc_##sfx##_deblock_chroma_v(S* data, size_t data_len, size_t off, size_t stride,     \
+                                         const int32_t tc[4], const uint8_t no_p[4],              \
+                                         const uint8_t no_q[4], int max);                         \
+    int rv_hevc_##sfx##_deblock_chroma_h(S* data, size_t data_len, size_t off, size_t stride,     \
+                                         const int32_t tc[4], const uint8_t no_p[4],              \
+                                         const uint8_t no_q[4], int max);
+RV_HEVC_SAMPLE_ENTRIES(uint8_t, u8)
+RV_HEVC_SAMPLE_ENTRIES(uint16_t, u16)
+
+/* H264Dsp<u8> (8-bit). Prediction scratch blocks are PRED_STRIDE = 16
+ * samples per row (src/dsp/h264.rs:22-24). */
+#define RV_H264_PRED_STRIDE 16
+#define RV_H264_NO_DC INT32_MIN /* NO_DC (src/dsp/h264.rs:118) */
+/* qpel[pos], pos = yf * 4 + xf: src at the six-tap window's top-left (two
+ * samples left of and two rows above the block). */
+int rv_h264_u8_qpel(int pos, uint8_t* dst, size_t dst_len, const uint8_t* src, size_t src_len,
+                    size_t src_stride, size_t w, size_t h, int max);
+int rv_h264_u8_chroma(uint8_t* dst, size_t dst_len, const uint8_t* src, size_t src_len,
+                      size_t stride, size_t w, size_t h, int xf, int yf);
+int rv_h264_u8_copy(uint8_t* dst, size_t dst_len, size_t stride, const uint8_t* src,
+                    size_t src_len, size_t w, size_t h);
+int rv_h264_u8_avg(uint8_t* dst, size_t dst_len, size_t stride, const uint8_t* a, size_t a_len,
+                   const uint8_t* b, size_t b_len, size_t w, size_t h);
+int rv_h264_u8_weighted_uni(uint8_t* dst, size_t dst_len, size_t stride, const uint8_t* src,
+                            size_t src_len, size_t w, size_t h, int log_wd, int wt, int o,
+                            int max);
+int rv_h264_u8_weighted_bi(uint8_t* dst, size_t dst_len, size_t stride, const uint8_t* a,
+                           size_t a_len, const uint8_t* b, size_t b_len, size_t w, size_t h,
+                           int log_wd, int w0, int w1, int o0, int o1, int max);
+/* Deblocking: 16 luma / 8 chroma lines; tc0[i] < 0 leaves its lines
+ * alone; the _intra entries are bS 4. */
+int rv_h264_u8_deblock_luma_v(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                              int alpha, int beta, const int16_t tc0[4], int max);
+int rv_h264_u8_deblock_luma_h(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                              int alpha, int beta, const int16_t tc0[4], int max);
+int rv_h264_u8_deblock_luma_v_intra(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                                    int alpha, int beta, int max);
+int rv_h264_u8_deblock_luma_h_intra(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                                    int alpha, int beta, int max);
+int rv_h264_u8_deblock_chroma_v(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                                int alpha, int beta, const int16_t tc0[4], int max);
+int rv_h264_u8_deblock_chroma_h(uint8_t* data, size_t data_len, size_t off, size_t stride,
+                                int alpha, int beta, const int16_t tc0[4], int max);
+int rv_h264_u8_deblock_chroma_v_intra(uint8_t* data, size_t data_len, size_t off,
+                                      size_t stride, int alpha, int beta, int max);
+int rv_h264_u8_deblock_chroma_h_intra(uint8_t* data, size_t data_len, size_t off,
+                                      size_t stride, int alpha, int beta, int max);
+/* Transforms: 8-bit arithmetic in wrapping i16 (src/dsp/h264.rs:766-1066). */
+int rv_h264_u8_idct4_add(uint8_t* dst, size_t dst_len, size_t stride, const int16_t coeffs[16],
+                         int max);
+int rv_h264_u8_idct8_add(uint8_t* dst, size_t dst_len, size_t stride, const int16_t coeffs[64],
+                         int max);
+int rv_h264_u8_idct4_dc_add(uint8_t* dst, size_t dst_len, size_t stride, int dc, int max);
+int rv_h264_u8_idct8_dc_add(uint8_t* dst, size_t dst_len, size_t stride, int dc, int max);
+/* residual4: dc == RV_H264_NO_DC keeps coefs[0]; otherwise dc replaces it. */
+int rv_h264_u8_residual4(uint8_t* dst, size_t dst_len, size_t stride, const int32_t coefs[16],
+                         int dc, int max);
+int rv_h264_u8_residual8(uint8_t* dst, size_t dst_len, size_t stride, const int32_t coefs[64],
+                         int max);
 
 /* sizeof of the work-list structs, for bindings to check their layout:
  * 0 rv_hevc_cu, 1 rv_hevc_pu, 2 rv_hevc_tu, 3 rv_sao, 4 rv_hevc_pic,

@@ -61,7 +61,11 @@ _sig = {
     "orc_hevc_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
     "orc_h264_qpel": (None, [C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_h264_chroma": (None, [u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_copy": (None, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
     "orc_h264_avg": (None, [u8p, C.c_int, u8p, u8p, C.c_int, C.c_int]),
+    "orc_h264_idct4_add": (None, [u8p, C.c_int, i16p, C.c_int]),
+    "orc_h264_idct8_add": (None, [u8p, C.c_int, i16p, C.c_int]),
+    "orc_h264_dc_add": (None, [u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_h264_weighted_uni": (None, [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_h264_weighted_bi": (None, [u8p, C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int]),

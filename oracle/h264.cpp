@@ -464,6 +464,14 @@ void orc_h264_qpel(int pos, uint8_t* dst, const uint8_t* src, int stride, int w,
 void orc_h264_chroma(uint8_t* dst, const uint8_t* src, int stride, int w, int h, int xf, int yf) {
     chroma(dst, src, stride, w, h, xf, yf);
 }
+void orc_h264_copy(uint8_t* dst, int stride, const uint8_t* src, int w, int h) { copy_k(dst, stride, src, w, h); }
+void orc_h264_idct4_add(uint8_t* dst, int stride, const int16_t* coeffs, int max) {
+    idct4_add(dst, stride, coeffs, max);
+}
+void orc_h264_idct8_add(uint8_t* dst, int stride, const int16_t* coeffs, int max) {
+    idct8_add(dst, stride, coeffs, max);
+}
+void orc_h264_dc_add(uint8_t* dst, int stride, int dc, int n, int max) { dc_add(dst, stride, dc, n, max); }
 void orc_h264_avg(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w, int h) {
     avg_k(dst, stride, a, b, w, h);
 }

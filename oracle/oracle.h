@@ -70,7 +70,12 @@ int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes,
 /* ---- H.264 dsp entries (H264Dsp<S>, src/dsp/h264.rs:123-170), 8-bit -- */
 void orc_h264_qpel(int pos, uint8_t* dst, const uint8_t* src, int stride, int w, int h, int max);
 void orc_h264_chroma(uint8_t* dst, const uint8_t* src, int stride, int w, int h, int xf, int yf);
+void orc_h264_copy(uint8_t* dst, int stride, const uint8_t* src, int w, int h);
 void orc_h264_avg(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w, int h);
+void orc_h264_idct4_add(uint8_t* dst, int stride, const int16_t* coeffs, int max);
+void orc_h264_idct8_add(uint8_t* dst, int stride, const int16_t* coeffs, int max);
+/* dc_add_scalar, n = 4 or 8. */
+void orc_h264_dc_add(uint8_t* dst, int stride, int dc, int n, int max);
 void orc_h264_weighted_uni(uint8_t* dst, int stride, const uint8_t* src, int w, int h, int log_wd,
                            int wt, int o, int max);
 void orc_h264_weighted_bi(uint8_t* dst, int stride, const uint8_t* a, const uint8_t* b, int w,

@@ -59,6 +59,7 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
 // LDS: W is the window (stride 32), origin two left / two above the block.
 struct Win {
     const int16_t* w;
+    int mx = 255;  // sample maximum
     __device__ int G(int x, int y) const { return w[(y + 2) * 32 + x + 2]; }
     __device__ int b1(int x, int yy) const {
         const int16_t* r = w + yy * 32 + x;
@@ -68,11 +69,11 @@ struct Win {
         const int16_t* c = w + y * 32 + xx;
         return tap6(c[0], c[32], c[64], c[96], c[128], c[160]);
     }
-    __device__ int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, 255); }
-    __device__ int h(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, 255); }
+    __device__ int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, mx); }
+    __device__ int h(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, mx); }
     __device__ int j(int x, int y) const {
         int j1 = tap6(b1(x, y), b1(x, y + 1), b1(x, y + 2), b1(x, y + 3), b1(x, y + 4), b1(x, y + 5));
-        return clampi((j1 + 512) >> 10, 0, 255);
+        return clampi((j1 + 512) >> 10, 0, mx);
     }
 };
 
@@ -234,32 +235,24 @@ __device__ __forceinline__ int16_t ws(int16_t a, int16_t b) { return (int16_t)(u
 __device__ __forceinline__ int16_t sh(int16_t a, int s) { return (int16_t)(a >> s); }
 
 // add_rows_scalar (:803-816).
-__device__ __forceinline__ void add_px(uint8_t* p, int16_t v) {
+__device__ __forceinline__ void add_px(uint8_t* p, int16_t v, int max) {
     int16_t r = sh(wa(v, 32), 6);
-    *p = (uint8_t)clampi((int)wa((int16_t)*p, r), 0, 255);
+    *p = (uint8_t)clampi((int)wa((int16_t)*p, r), 0, max);
 }
 
-// residual4_scalar with narrow4 and idct4_add_scalar / dc_add_scalar.
-__device__ void residual4(uint8_t* dst, int pitch, const int32_t* co, int has_dc, int dc) {
-    int16_t d[16];
-    int ac = 0;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int v = co[i];
-        d[i] = (int16_t)clampi(v, -32768, 32767);
-        if (i) ac |= v;
-    }
-    if (has_dc) d[0] = (int16_t)dc;
-    if (ac == 0) {
-        if (d[0] == 0) return;
-        int v = (int)sh(wa(d[0], 32), 6);
-        for (int y = 0; y < 4; ++y)
-            for (int x = 0; x < 4; ++x) {
-                uint8_t* p = dst + y * pitch + x;
-                *p = (uint8_t)clampi(*p + v, 0, 255);
-            }
-        return;
-    }
+// dc_add_scalar (:984-996), 8-bit: ((dc as i16) + 32) >> 6 in i16, added
+// in i32.
+__device__ void dc_add16(uint8_t* dst, long pitch, int n, int16_t dc, int max) {
+    const int v = (int)sh(wa(dc, 32), 6);
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            uint8_t* p = dst + y * pitch + x;
+            *p = (uint8_t)clampi(*p + v, 0, max);
+        }
+}
+
+// idct4_add_scalar (:774-801) in wrapping i16.
+__device__ void idct4_add16(uint8_t* dst, long pitch, const int16_t* d, int max) {
     int16_t t[16];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -274,11 +267,29 @@ __device__ void residual4(uint8_t* dst, int pitch, const int32_t* co, int has_dc
     for (int j = 0; j < 4; ++j) {
         int16_t f0 = t[j], f1 = t[4 + j], f2 = t[8 + j], f3 = t[12 + j];
         int16_t g0 = wa(f0, f2), g1 = ws(f0, f2), g2 = ws(sh(f1, 1), f3), g3 = wa(f1, sh(f3, 1));
-        add_px(dst + 0 * pitch + j, wa(g0, g3));
-        add_px(dst + 1 * pitch + j, wa(g1, g2));
-        add_px(dst + 2 * pitch + j, ws(g1, g2));
-        add_px(dst + 3 * pitch + j, ws(g0, g3));
+        add_px(dst + 0 * pitch + j, wa(g0, g3), max);
+        add_px(dst + 1 * pitch + j, wa(g1, g2), max);
+        add_px(dst + 2 * pitch + j, ws(g1, g2), max);
+        add_px(dst + 3 * pitch + j, ws(g0, g3), max);
     }
+}
+
+// residual4_scalar with narrow4 and idct4_add_scalar / dc_add_scalar.
+__device__ void residual4(uint8_t* dst, long pitch, const int32_t* co, int has_dc, int dc, int max = 255) {
+    int16_t d[16];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int v = co[i];
+        d[i] = (int16_t)clampi(v, -32768, 32767);
+        if (i) ac |= v;
+    }
+    if (has_dc) d[0] = (int16_t)dc;
+    if (ac == 0) {
+        if (d[0] != 0) dc_add16(dst, pitch, 4, d[0], max);
+        return;
+    }
+    idct4_add16(dst, pitch, d, max);
 }
 
 // idct8_1d_i16 (:913-958).
@@ -302,8 +313,20 @@ __device__ __forceinline__ void idct8_1d(const int16_t* d, int16_t* o) {
     o[7] = ws(b0, b7);
 }
 
+// idct8_add_scalar (:961-982) in wrapping i16: rows, then columns.
+__device__ void idct8_add16(uint8_t* dst, long pitch, const int16_t* d, int max) {
+    int16_t t[64];
+    for (int i = 0; i < 8; ++i) idct8_1d(d + i * 8, t + i * 8);
+    for (int j = 0; j < 8; ++j) {
+        int16_t col[8], o[8];
+        for (int k = 0; k < 8; ++k) col[k] = t[k * 8 + j];
+        idct8_1d(col, o);
+        for (int i = 0; i < 8; ++i) add_px(dst + i * pitch + j, o[i], max);
+    }
+}
+
 // residual8_scalar with narrow8 and idct8_add_scalar / dc_add_scalar.
-__device__ void residual8(uint8_t* dst, int pitch, const int32_t* co) {
+__device__ void residual8(uint8_t* dst, long pitch, const int32_t* co, int max = 255) {
     int16_t d[64];
     int ac = 0;
     for (int i = 0; i < 64; ++i) {
@@ -312,23 +335,10 @@ __device__ void residual8(uint8_t* dst, int pitch, const int32_t* co) {
         if (i) ac |= v;
     }
     if (ac == 0) {
-        if (d[0] == 0) return;
-        int v = (int)sh(wa(d[0], 32), 6);
-        for (int y = 0; y < 8; ++y)
-            for (int x = 0; x < 8; ++x) {
-                uint8_t* p = dst + y * pitch + x;
-                *p = (uint8_t)clampi(*p + v, 0, 255);
-            }
+        if (d[0] != 0) dc_add16(dst, pitch, 8, d[0], max);
         return;
     }
-    int16_t t[64];
-    for (int i = 0; i < 8; ++i) idct8_1d(d + i * 8, t + i * 8);
-    for (int j = 0; j < 8; ++j) {
-        int16_t col[8], o[8];
-        for (int k = 0; k < 8; ++k) col[k] = t[k * 8 + j];
-        idct8_1d(col, o);
-        for (int i = 0; i < 8; ++i) add_px(dst + i * pitch + j, o[i]);
-    }
+    idct8_add16(dst, pitch, d, max);
 }
 
 __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
@@ -487,15 +497,16 @@ __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
 
 // deblock_line (src/dsp/h264.rs:603-656) on one line, in place through
 // `d` (q0) with `across` crossing the edge.
-__device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alpha, int beta, bool chroma) {
+__device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alpha, int beta, bool chroma,
+                            int max = 255) {
     int p0 = d[-across], p1 = d[-2 * across], q0 = d[0], q1 = d[across];
     if (!(abs(p0 - q0) < alpha && abs(p1 - p0) < beta && abs(q1 - q0) < beta)) return;
     if (chroma) {
         if (!bs4) {
             int tc = tc0 + 1;
             int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-            d[-across] = (uint8_t)clampi(p0 + delta, 0, 255);
-            d[0] = (uint8_t)clampi(q0 - delta, 0, 255);
+            d[-across] = (uint8_t)clampi(p0 + delta, 0, max);
+            d[0] = (uint8_t)clampi(q0 - delta, 0, max);
         } else {
             d[-across] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
             d[0] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
@@ -507,8 +518,8 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
     if (!bs4) {
         int tc = tc0 + (ap < beta) + (aq < beta);
         int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        d[-across] = (uint8_t)clampi(p0 + delta, 0, 255);
-        d[0] = (uint8_t)clampi(q0 - delta, 0, 255);
+        d[-across] = (uint8_t)clampi(p0 + delta, 0, max);
+        d[0] = (uint8_t)clampi(q0 - delta, 0, max);
         if (ap < beta) d[-2 * across] = (uint8_t)(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0));
         if (aq < beta) d[across] = (uint8_t)(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0));
     } else {
@@ -636,7 +647,98 @@ __global__ void __launch_bounds__(64) h264_stage3(BatchDev b) {
     }
 }
 
+// ---------------------------------------------------------------------
+// Per-entry kernel: one call of one H264Dsp<u8> entry (src/dsp/h264.rs:
+// 123-170) on one workgroup of 256 threads, through the device functions
+// of the batched stages. Prediction blocks are 16 samples per row
+// (PRED_STRIDE, :22-24).
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) h264_entry(H264Entry e) {
+    __shared__ int16_t win[21 * 32];
+    const int tid = threadIdx.x;
+    const int n = e.w * e.h;
+    switch (e.op) {
+        case HA_QPEL: {
+            // The (w+5) x (h+5) six-tap window, stride 32 as stage 0 keeps it.
+            for (int i = tid; i < (e.h + 5) * 32; i += 256) {
+                const int r = i >> 5, c = i & 31;
+                if (c < e.w + 5) win[i] = e.src[r * e.src_stride + c];
+            }
+            __syncthreads();
+            const Win q{win, e.max};
+            for (int i = tid; i < n; i += 256) {
+                const int y = i / e.w, x = i - y * e.w;
+                e.dst[y * 16 + x] = (uint8_t)qpel_at(q, e.pos, x, y);
+            }
+            break;
+        }
+        case HA_CHROMA: {  // chroma_scalar (:499-520)
+            const int a = (8 - e.xf) * (8 - e.yf), b = e.xf * (8 - e.yf), c = (8 - e.xf) * e.yf, d = e.xf * e.yf;
+            for (int i = tid; i < n; i += 256) {
+                const int y = i / e.w, x = i - y * e.w;
+                const uint8_t* r0 = e.src + y * e.src_stride + x;
+                const uint8_t* r1 = r0 + e.src_stride;
+                e.dst[y * 16 + x] = (uint8_t)((a * r0[0] + b * r0[1] + c * r1[0] + d * r1[1] + 32) >> 6);
+            }
+            break;
+        }
+        case HA_COPY:
+        case HA_AVG:
+        case HA_WUNI:
+        case HA_WBI:  // copy / avg / weighted_uni / weighted_bi (:522-597)
+            for (int i = tid; i < n; i += 256) {
+                const int y = i / e.w, x = i - y * e.w;
+                const int va = e.src[y * 16 + x];
+                int v;
+                if (e.op == HA_COPY) v = va;
+                else if (e.op == HA_AVG) v = (va + e.b[y * 16 + x] + 1) >> 1;
+                else if (e.op == HA_WUNI)
+                    v = e.log_wd >= 1 ? clampi(((va * e.w0 + (1 << (e.log_wd - 1))) >> e.log_wd) + e.o0, 0, e.max)
+                                      : clampi(va * e.w0 + e.o0, 0, e.max);
+                else
+                    v = clampi(((va * e.w0 + e.b[y * 16 + x] * e.w1 + (1 << e.log_wd)) >> (e.log_wd + 1)) +
+                                   ((e.o0 + e.o1 + 1) >> 1),
+                               0, e.max);
+                e.dst[y * e.stride + x] = (uint8_t)v;
+            }
+            break;
+        case HA_DBK: {  // deblock_luma_lines<16> / deblock_chroma_scalar (:658-763)
+            const int lines = e.chroma ? 8 : 16;
+            if (tid < lines) {
+                const int t = e.intra ? 0 : e.tc0[tid / (e.chroma ? 2 : 4)];
+                if (e.intra || t >= 0) {
+                    const long along = e.vertical ? e.stride : 1, across = e.vertical ? 1 : e.stride;
+                    line_filter(e.dst + e.off + tid * along, across, e.intra != 0, t, e.alpha, e.beta, e.chroma != 0,
+                                e.max);
+                }
+            }
+            break;
+        }
+        case HA_IDCT4:
+            if (tid == 0) idct4_add16(e.dst, e.stride, (const int16_t*)e.coefs, e.max);
+            break;
+        case HA_IDCT8:
+            if (tid == 0) idct8_add16(e.dst, e.stride, (const int16_t*)e.coefs, e.max);
+            break;
+        case HA_DC:
+            if (tid == 0) dc_add16(e.dst, e.stride, e.n, (int16_t)e.dc, e.max);
+            break;
+        case HA_RES4:
+            if (tid == 0) residual4(e.dst, e.stride, (const int32_t*)e.coefs, e.has_dc, e.dc, e.max);
+            break;
+        case HA_RES8:
+            if (tid == 0) residual8(e.dst, e.stride, (const int32_t*)e.coefs, e.max);
+            break;
+        default: break;
+    }
+}
+
 }  // namespace
+
+hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(h264_entry, dim3(blocks), dim3(256), 0, st, e);
+    return hipGetLastError();
+}
 
 void h264_stage_items(H264PicDev& P) {
     P.items[0] = P.n_mbs + P.n_parts;

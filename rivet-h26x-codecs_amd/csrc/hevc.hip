@@ -788,17 +788,18 @@ __device__ __forceinline__ int chroma_tc(const HevcPicDev& P, int qp_p, int qp_q
 }
 
 // chroma_edge_scalar (src/dsp/hevc.rs:1326-1353), two lines. d points at
-// q0 of line 0; `step` crosses the edge, `along` moves between lines.
+// q0 of line 0; `step` crosses the edge, `along` moves between lines;
+// no_p / no_q leave that side unwritten (filter exemptions).
 template <typename S>
-__device__ void chroma_lines(S* d, long step, long along, int tc, int max) {
+__device__ void chroma_lines(S* d, long step, long along, int tc, int max, bool no_p = false, bool no_q = false) {
     if (tc == 0) return;
 #pragma unroll
     for (int line = 0; line < 2; ++line) {
         S* b = d + line * along;
         int p0 = b[-step], p1 = b[-2 * step], q0 = b[0], q1 = b[step];
         int delta = clampi((((q0 - p0) << 2) + p1 - q1 + 4) >> 3, -tc, tc);
-        b[-step] = (S)clampi(p0 + delta, 0, max);
-        b[0] = (S)clampi(q0 - delta, 0, max);
+        if (!no_p) b[-step] = (S)clampi(p0 + delta, 0, max);
+        if (!no_q) b[0] = (S)clampi(q0 - delta, 0, max);
     }
 }
 
@@ -1065,7 +1066,244 @@ __global__ void __launch_bounds__(256) hevc_stage4(BatchDev b) {
     else sao_ctb<uint16_t>(P, rx, ry, threadIdx.x);
 }
 
+// ---------------------------------------------------------------------
+// Per-entry kernels: one call of one HevcDsp entry (src/dsp/hevc.rs:
+// 165-229) on one workgroup (transforms, fused MC, deblocking) or a small
+// grid (element-wise entries), with the scalar kernels' arithmetic and the
+// device functions the batched stages use.
+// ---------------------------------------------------------------------
+
+// idct_scalar<N> (src/dsp/hevc.rs:500-526): entries beyond max_x / max_y
+// are zero, as idct1 never reads them; lane x transforms column x, then
+// lane y row y.
+template <int N>
+__device__ void entry_idct(const HevcEntry& e, int16_t* t) {
+    int16_t* c = (int16_t*)e.dst;
+    const int tid = threadIdx.x;
+    if (tid < N) {
+        int s[N], o[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] = (tid <= e.max_x && k <= e.max_y) ? c[k * N + tid] : 0;
+        idct1d<N>(s, o);
+#pragma unroll
+        for (int y = 0; y < N; ++y) t[y * 33 + tid] = (int16_t)clampi((o[y] + 64) >> 7, -32768, 32767);
+    }
+    __syncthreads();
+    if (tid < N) {
+        int s[N], o[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] = t[tid * 33 + k];
+        idct1d<N>(s, o);
+        const int sh = e.shift, rnd = 1 << (sh - 1);
+#pragma unroll
+        for (int x = 0; x < N; ++x) c[tid * N + x] = (int16_t)clampi((o[x] + rnd) >> sh, -32768, 32767);
+    }
+}
+
+// idst4_scalar (src/dsp/hevc.rs:529-556); the bounds are not used.
+__device__ void entry_idst(const HevcEntry& e, int16_t* t) {
+    int16_t* c = (int16_t*)e.dst;
+    const int tid = threadIdx.x;
+    if (tid < 4) {
+        int s[4], o[4];
+        for (int k = 0; k < 4; ++k) s[k] = c[k * 4 + tid];
+        idst1d(s, o);
+        for (int i = 0; i < 4; ++i) t[i * 33 + tid] = (int16_t)clampi((o[i] + 64) >> 7, -32768, 32767);
+    }
+    __syncthreads();
+    if (tid < 4) {
+        int s[4], o[4];
+        for (int k = 0; k < 4; ++k) s[k] = t[tid * 33 + k];
+        idst1d(s, o);
+        const int sh = e.shift, rnd = 1 << (sh - 1);
+        for (int i = 0; i < 4; ++i) c[tid * 4 + i] = (int16_t)clampi((o[i] + rnd) >> sh, -32768, 32767);
+    }
+}
+
+// The element-wise entries: add_residual, copy, the one-stage filters,
+// the combiners and SAO (src/dsp/hevc.rs:558-566, 677-818, 997-1087,
+// 1181-1224). Element i is (y, x) = (i / w, i % w) of the w x h block.
+template <typename S>
+__device__ void entry_elem(const HevcEntry& e) {
+    const int n = e.w * e.h;
+    const int16_t* a16 = (const int16_t*)e.src;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int y = i / e.w, x = i - y * e.w;
+        S* drow = (S*)e.dst + y * e.dst_stride + x;
+        switch (e.op) {
+            case HE_ADD_RES: *drow = (S)clampi((int)*drow + a16[i], 0, e.max); break;
+            case HE_COPY:
+                ((int16_t*)e.dst)[i] = (int16_t)((int)((const S*)e.src)[y * e.src_stride + x] << e.shift);
+                break;
+            case HE_FILT_H:
+            case HE_FILT_V:
+            case HE_FILT_V2: {
+                const int taps = e.luma ? 8 : 4;
+                const long along = e.op == HE_FILT_H ? 1 : e.src_stride;
+                const long base = y * e.src_stride + x;
+                int acc = 0;
+                for (int k = 0; k < taps; ++k) {
+                    const int f = e.luma ? kQpel[e.frac][k] : kEpel[e.frac][k];
+                    const int v = e.op == HE_FILT_V2 ? (int)a16[base + k * along] : (int)((const S*)e.src)[base + k * along];
+                    acc += f * v;
+                }
+                ((int16_t*)e.dst)[i] = (int16_t)(acc >> (e.op == HE_FILT_V2 ? 6 : e.shift));
+                break;
+            }
+            case HE_UNI: {
+                const int rnd = e.shift > 0 ? 1 << (e.shift - 1) : 0;
+                *drow = (S)clampi((a16[i] + rnd) >> e.shift, 0, e.max);
+                break;
+            }
+            case HE_BI:
+                *drow = (S)clampi((a16[i] + e.other[i] + (1 << (e.shift - 1))) >> e.shift, 0, e.max);
+                break;
+            case HE_WUNI:
+                if (e.log2_wd >= 1)
+                    *drow = (S)clampi(((a16[i] * e.w0 + (1 << (e.log2_wd - 1))) >> e.log2_wd) + e.o0, 0, e.max);
+                else
+                    *drow = (S)clampi(a16[i] * e.w0 + e.o0, 0, e.max);
+                break;
+            case HE_WBI:
+                *drow = (S)clampi((a16[i] * e.w0 + e.other[i] * e.w1 + ((e.o0 + e.o1 + 1) << e.log2_wd)) >>
+                                      (e.log2_wd + 1),
+                                  0, e.max);
+                break;
+            case HE_SAO_BAND: {
+                const int v = ((const S*)e.src)[y * e.src_stride + x];
+                *drow = (S)clampi(v + e.table[v >> e.shift], 0, e.max);
+                break;
+            }
+            case HE_SAO_EDGE: {
+                const long at = e.off + y * e.dst_stride + x;
+                const S* s = (const S*)e.src;
+                const int v = s[at], va = s[at + e.na], vb = s[at + e.nb];
+                ((S*)e.dst)[at] = (S)clampi(v + e.table[2 + sgn(v - va) + sgn(v - vb)], 0, e.max);
+                break;
+            }
+            default: break;
+        }
+    }
+}
+
+// qpel_uni / epel_uni / qpel_bi / epel_bi: interp_scalar's two-pass
+// composition (src/dsp/hevc.rs:823-882) then uni_scalar / bi_scalar at
+// 14 - bd / 15 - bd (:887-991). `src` is the filter window's origin.
+template <typename S>
+__device__ void entry_fused(const HevcEntry& e, int16_t* tmp) {
+    const int taps = e.luma ? 8 : 4, reach = e.luma ? 3 : 1;
+    const S* src = (const S*)e.src;
+    const long ss = e.src_stride;
+    const int w = e.w, h = e.h, bd = e.bd;
+    const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
+    auto fxk = [&](int k) { return e.luma ? (int)kQpel[e.fx][k] : (int)kEpel[e.fx][k]; };
+    auto fyk = [&](int k) { return e.luma ? (int)kQpel[e.fy][k] : (int)kEpel[e.fy][k]; };
+    if (e.fx && e.fy) {
+        for (int i = threadIdx.x; i < w * (h + taps - 1); i += 256) {
+            const int y = i / w, x = i - y * w;
+            int acc = 0;
+            for (int k = 0; k < taps; ++k) acc += fxk(k) * src[y * ss + x + k];
+            tmp[i] = (int16_t)(acc >> shift1);
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < w * h; i += 256) {
+        const int y = i / w, x = i - y * w;
+        int p, acc = 0;
+        if (!e.fx && !e.fy) {
+            p = (int16_t)((int)src[(y + reach) * ss + x + reach] << shift3);
+        } else if (!e.fy) {
+            for (int k = 0; k < taps; ++k) acc += fxk(k) * src[(y + reach) * ss + x + k];
+            p = (int16_t)(acc >> shift1);
+        } else if (!e.fx) {
+            for (int k = 0; k < taps; ++k) acc += fyk(k) * src[(y + k) * ss + x + reach];
+            p = (int16_t)(acc >> shift1);
+        } else {
+            for (int k = 0; k < taps; ++k) acc += fyk(k) * tmp[(y + k) * w + x];
+            p = (int16_t)(acc >> 6);
+        }
+        S* d = (S*)e.dst + y * e.dst_stride + x;
+        const int mx = (1 << bd) - 1;
+        if (e.other) {
+            const int sh = 15 - bd;
+            *d = (S)clampi((e.other[i] + p + (1 << (sh - 1))) >> sh, 0, mx);
+        } else {
+            const int sh = 14 - bd, rnd = sh > 0 ? 1 << (sh - 1) : 0;
+            *d = (S)clampi((p + rnd) >> sh, 0, mx);
+        }
+    }
+}
+
+// deblock_luma_scalar (src/dsp/hevc.rs:1355-1382): lane k < 2 filters
+// segment k with luma_segment; an exempt side is not written back.
+template <typename S>
+__device__ void entry_dbk_luma(const HevcEntry& e) {
+    const int seg = threadIdx.x;
+    if (seg >= 2 || (e.beta[seg] == 0 && e.tc[seg] == 0)) return;
+    const long step = e.vertical ? 1 : e.dst_stride, along = e.vertical ? e.dst_stride : 1;
+    S* d = (S*)e.dst + e.off + 4 * seg * along;
+    int s[4][8];
+#pragma unroll
+    for (int line = 0; line < 4; ++line)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[line][k] = d[line * along + (k - 4) * step];
+    bool changed = false;
+    luma_segment(s, e.beta[seg], e.tc[seg], e.max, changed);
+    if (!changed) return;
+#pragma unroll
+    for (int line = 0; line < 4; ++line)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < 4 ? !e.no_p[seg] : !e.no_q[seg]) d[line * along + (k - 4) * step] = (S)s[line][k];
+}
+
+// deblock_chroma_scalar (:1384-1411): lane k < 4 filters segment k.
+template <typename S>
+__device__ void entry_dbk_chroma(const HevcEntry& e) {
+    const int seg = threadIdx.x;
+    if (seg >= 4) return;
+    const long step = e.vertical ? 1 : e.dst_stride, along = e.vertical ? e.dst_stride : 1;
+    chroma_lines<S>((S*)e.dst + e.off + 2 * seg * along, step, along, e.tc[seg], e.max, e.no_p[seg] != 0,
+                    e.no_q[seg] != 0);
+}
+
+__global__ void __launch_bounds__(256) hevc_entry(HevcEntry e) {
+    __shared__ int16_t lds[64 * 71];
+    switch (e.op) {
+        case HE_IDCT:
+            switch (e.w) {
+                case 4: entry_idct<4>(e, lds); break;
+                case 8: entry_idct<8>(e, lds); break;
+                case 16: entry_idct<16>(e, lds); break;
+                default: entry_idct<32>(e, lds); break;
+            }
+            break;
+        case HE_IDST: entry_idst(e, lds); break;
+        case HE_FUSED:
+            if (e.ss == 1) entry_fused<uint8_t>(e, lds);
+            else entry_fused<uint16_t>(e, lds);
+            break;
+        case HE_DBK_LUMA:
+            if (e.ss == 1) entry_dbk_luma<uint8_t>(e);
+            else entry_dbk_luma<uint16_t>(e);
+            break;
+        case HE_DBK_CHROMA:
+            if (e.ss == 1) entry_dbk_chroma<uint8_t>(e);
+            else entry_dbk_chroma<uint16_t>(e);
+            break;
+        default:
+            if (e.ss == 1) entry_elem<uint8_t>(e);
+            else entry_elem<uint16_t>(e);
+            break;
+    }
+}
+
 }  // namespace
+
+hipError_t hevc_entry_launch(const HevcEntry& e, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(hevc_entry, dim3(blocks), dim3(256), 0, st, e);
+    return hipGetLastError();
+}
 
 __global__ void __launch_bounds__(256) rv_copy_probe(uint4* __restrict__ dst, const uint4* __restrict__ src, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];

@@ -16,6 +16,7 @@
 #include "../../include/rivet_hip.h"
 #include "rv_device.h"
 #include "rv_kernels.h"
+#include "rv_runtime.h"
 
 namespace {
 
@@ -46,6 +47,17 @@ int hip_fail(hipError_t e, const char* what) {
     } while (0)
 
 bool ready() { return g_device >= 0 && g_stream; }
+
+}  // namespace
+
+namespace rv {
+std::mutex& lib_mutex() { return g_mu; }
+hipStream_t lib_stream() { return g_stream; }
+bool lib_ready() { return ready(); }
+int lib_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace rv
+
+namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -346,6 +358,10 @@ int rv_init(int device_ordinal) {
             if (g_device == device_ordinal) return RV_OK;
             return fail(RV_E_STATE, "already initialised on another device");
         }
+        // RIVET_HIP_DISABLE=1 (like H26X_NO_SIMD, src/dsp/mod.rs:295-350)
+        // refuses the tier, so the caller keeps its own CPU tables.
+        const char* off = std::getenv("RIVET_HIP_DISABLE");
+        if (off && *off && *off != '0') return fail(RV_E_NODEVICE, "disabled by RIVET_HIP_DISABLE");
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RV_E_NODEVICE, "no HIP device");
         if (device_ordinal < 0 || device_ordinal >= n) return fail(RV_E_NODEVICE, "device ordinal out of range");

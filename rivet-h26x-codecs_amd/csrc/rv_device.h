@@ -96,6 +96,49 @@ __device__ __forceinline__ int xcd_item() {
     return (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
 }
 
+// One call of one kernel-table entry (the per-entry API). Pointers are
+// device copies of the caller's spans, rebased so the entry's own offsets
+// (off, origin) index them as the caller's slices would.
+enum HevcOp : int32_t {
+    HE_IDCT, HE_IDST, HE_ADD_RES, HE_COPY, HE_FILT_H, HE_FILT_V, HE_FILT_V2, HE_UNI, HE_BI, HE_WUNI,
+    HE_WBI, HE_FUSED, HE_SAO_BAND, HE_SAO_EDGE, HE_DBK_LUMA, HE_DBK_CHROMA
+};
+struct HevcEntry {
+    int32_t op, ss;       // HevcOp; bytes per sample (1, 2)
+    int32_t luma;         // 8-tap (1) or 4-tap (0) filters
+    int32_t vertical;     // deblocking: _v (1) or _h (0)
+    void* dst;
+    const void* src;
+    const int16_t* other; // bi: the second 14-bit input
+    int64_t dst_stride, src_stride;
+    int64_t off;          // deblocking: q0 of line 0; SAO edge: origin
+    int64_t na, nb;       // SAO edge neighbour offsets
+    int32_t w, h;         // block size (idct: n)
+    int32_t frac, fx, fy, shift, max, bd;
+    int32_t log2_wd, w0, w1, o0, o1;
+    int32_t max_x, max_y;
+    int32_t beta[2], tc[4];
+    uint8_t no_p[4], no_q[4];
+    int16_t table[32];    // SAO band table / edge offsets (5)
+};
+
+enum H264Op : int32_t {
+    HA_QPEL, HA_CHROMA, HA_COPY, HA_AVG, HA_WUNI, HA_WBI, HA_DBK, HA_IDCT4, HA_IDCT8, HA_DC, HA_RES4, HA_RES8
+};
+struct H264Entry {
+    int32_t op, pos;
+    uint8_t* dst;
+    const uint8_t* src;   // or the first operand
+    const uint8_t* b;     // second operand (avg, weighted_bi)
+    const void* coefs;    // int16 (idct*_add) or int32 (residual*)
+    int64_t stride, src_stride, off;
+    int32_t w, h, xf, yf, max;
+    int32_t log_wd, w0, w1, o0, o1;
+    int32_t alpha, beta, chroma, vertical, intra;
+    int32_t dc, has_dc, n;
+    int16_t tc0[4];
+};
+
 // Per-batch tables: the descriptors and, per stage, prefix sums of the
 // per-picture work-item counts.
 constexpr int kStages = 5;

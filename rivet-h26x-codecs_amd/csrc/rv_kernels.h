@@ -10,10 +10,13 @@ namespace rv {
 // HEVC (hevc.hip).
 void hevc_stage_items(HevcPicDev& P);
 hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
+// One per-entry call: `blocks` workgroups of 256 threads.
+hipError_t hevc_entry_launch(const HevcEntry& e, int blocks, hipStream_t st);
 
 // H.264 (h264.hip).
 void h264_stage_items(H264PicDev& P);
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
+hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st);
 
 // Plain copy of `bytes` (a multiple of 4096) with 16-byte vectors.
 hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStream_t st);

@@ -157,6 +157,58 @@ PROTOTYPES = {
     "rv_gen_subseed": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
 }
 
+# The per-entry API (include/rivet_hip.h): buffers as void*, lengths,
+# strides and sizes as size_t, neighbour offsets as ptrdiff_t.
+_sz, _ss, _u = C.c_size_t, C.c_ssize_t, C.c_uint
+ENTRY_PROTOTYPES = {
+    "rv_hevc_idct": (i32, [i32, P, _sz, i32, _sz, _sz]),
+    "rv_hevc_idst4": (i32, [P, _sz, i32, _sz, _sz]),
+    "rv_hevc_qpel_v2": (i32, [P, _sz, P, _sz, _sz, _sz, _sz, _sz]),
+    "rv_hevc_epel_v2": (i32, [P, _sz, P, _sz, _sz, _sz, _sz, _sz]),
+    "rv_h264_u8_qpel": (i32, [i32, P, _sz, P, _sz, _sz, _sz, _sz, i32]),
+    "rv_h264_u8_chroma": (i32, [P, _sz, P, _sz, _sz, _sz, _sz, i32, i32]),
+    "rv_h264_u8_copy": (i32, [P, _sz, _sz, P, _sz, _sz, _sz]),
+    "rv_h264_u8_avg": (i32, [P, _sz, _sz, P, _sz, P, _sz, _sz, _sz]),
+    "rv_h264_u8_weighted_uni": (i32, [P, _sz, _sz, P, _sz, _sz, _sz, i32, i32, i32, i32]),
+    "rv_h264_u8_weighted_bi": (i32, [P, _sz, _sz, P, _sz, P, _sz, _sz, _sz] + [i32] * 6),
+    "rv_h264_u8_idct4_add": (i32, [P, _sz, _sz, P, i32]),
+    "rv_h264_u8_idct8_add": (i32, [P, _sz, _sz, P, i32]),
+    "rv_h264_u8_idct4_dc_add": (i32, [P, _sz, _sz, i32, i32]),
+    "rv_h264_u8_idct8_dc_add": (i32, [P, _sz, _sz, i32, i32]),
+    "rv_h264_u8_residual4": (i32, [P, _sz, _sz, P, i32, i32]),
+    "rv_h264_u8_residual8": (i32, [P, _sz, _sz, P, i32]),
+}
+for _e in ("luma_v", "luma_h", "chroma_v", "chroma_h"):
+    ENTRY_PROTOTYPES[f"rv_h264_u8_deblock_{_e}"] = (i32, [P, _sz, _sz, _sz, i32, i32, P, i32])
+    ENTRY_PROTOTYPES[f"rv_h264_u8_deblock_{_e}_intra"] = (i32, [P, _sz, _sz, _sz, i32, i32, i32])
+for _s in ("u8", "u16"):
+    _t = {
+        "add_residual": [P, _sz, _sz, P, _sz, _sz, i32],
+        "qpel_copy": [P, _sz, P, _sz, _sz, _sz, _sz, i32],
+        "epel_copy": [P, _sz, P, _sz, _sz, _sz, _sz, i32],
+        "qpel_h": [P, _sz, P, _sz, _sz, _sz, _sz, _sz, i32],
+        "qpel_v": [P, _sz, P, _sz, _sz, _sz, _sz, _sz, i32],
+        "epel_h": [P, _sz, P, _sz, _sz, _sz, _sz, _sz, i32],
+        "epel_v": [P, _sz, P, _sz, _sz, _sz, _sz, _sz, i32],
+        "uni": [P, _sz, _sz, P, _sz, _sz, _sz, i32, i32],
+        "bi": [P, _sz, _sz, P, _sz, P, _sz, _sz, _sz, i32, i32],
+        "weighted_uni": [P, _sz, _sz, P, _sz, _sz, _sz, i32, i32, i32, i32],
+        "weighted_bi": [P, _sz, _sz, P, _sz, P, _sz, _sz, _sz] + [i32] * 6,
+        "qpel_uni": [P, _sz, _sz, P, _sz, _sz, _sz, _sz, _sz, _sz, _u],
+        "epel_uni": [P, _sz, _sz, P, _sz, _sz, _sz, _sz, _sz, _sz, _u],
+        "qpel_bi": [P, _sz, _sz, P, _sz, _sz, _sz, _sz, _sz, _sz, P, _sz, _u],
+        "epel_bi": [P, _sz, _sz, P, _sz, _sz, _sz, _sz, _sz, _sz, P, _sz, _u],
+        "sao_band": [P, _sz, _sz, P, _sz, _sz, _sz, _sz, P, i32, i32],
+        "sao_edge": [P, _sz, P, _sz, _sz, _sz, _sz, _sz, _ss, _ss, P, i32],
+        "deblock_luma_v": [P, _sz, _sz, _sz, P, P, P, P, i32],
+        "deblock_luma_h": [P, _sz, _sz, _sz, P, P, P, P, i32],
+        "deblock_chroma_v": [P, _sz, _sz, _sz, P, P, P, i32],
+        "deblock_chroma_h": [P, _sz, _sz, _sz, P, P, P, i32],
+    }
+    for _n, _a in _t.items():
+        ENTRY_PROTOTYPES[f"rv_hevc_{_s}_{_n}"] = (i32, _a)
+PROTOTYPES.update(ENTRY_PROTOTYPES)
+
 
 def load() -> C.CDLL:
     if not LIB_PATH.exists():
