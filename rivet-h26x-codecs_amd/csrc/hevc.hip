@@ -21,17 +21,47 @@
 namespace rv {
 namespace {
 
-// Table 8-12 / 8-13 (src/hevc/tables_gen.rs:173-191).
-__constant__ int8_t kQpel[4][8] = {
-    {0, 0, 0, 0, 0, 0, 0, 0},
-    {-1, 4, -10, 58, 17, -5, 1, 0},
-    {-1, 4, -11, 40, 40, -11, 4, -1},
-    {0, 1, -5, 17, 58, -10, 4, -1},
+// Table 8-12 / 8-13 (src/hevc/tables_gen.rs:173-191); row 0 unused.
+struct FilterTables {
+    int8_t q[4][8];  // luma taps by quarter fraction
+    int8_t e[8][4];  // chroma taps by eighth fraction
+    // The same taps as packed i16 pairs for v_dot2_i32_i16: luma even pairs
+    // (f0 f1)(f2 f3)(f4 f5)(f6 f7), odd pairs (0 f0)(f1 f2)(f3 f4)(f5 f6)
+    // (f7 0); chroma even (f0 f1)(f2 f3), odd (0 f0)(f1 f2)(f3 0). Odd pairs
+    // serve outputs whose inputs start one sample after a pair boundary.
+    // Row 0 holds the identity (64 at the centre tap) instead of zeros, so
+    // every fraction takes the two-pass path: for bd <= 12 a whole-sample
+    // first stage gives x << (6 - shift1) exactly, and the second stage's
+    // >> 6 then equals the one-stage filters' >> shift1 and the copy's
+    // << (14 - bd) (src/dsp/hevc.rs:823-882).
+    alignas(16) uint32_t qpk[4][12];
+    alignas(16) uint32_t epk[8][8];
 };
-__constant__ int8_t kEpel[8][4] = {
-    {0, 0, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
-    {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2},
-};
+constexpr uint32_t pair16(int lo, int hi) { return (uint32_t)(lo & 0xFFFF) | ((uint32_t)(hi & 0xFFFF) << 16); }
+constexpr FilterTables make_filter_tables() {
+    FilterTables t{{{0, 0, 0, 0, 0, 0, 0, 0},
+                    {-1, 4, -10, 58, 17, -5, 1, 0},
+                    {-1, 4, -11, 40, 40, -11, 4, -1},
+                    {0, 1, -5, 17, 58, -10, 4, -1}},
+                   {{0, 0, 0, 0}, {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
+                    {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2}},
+                   {},
+                   {}};
+    for (int f = 0; f < 4; ++f) {
+        int g[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // g[k + 1] = tap k, g[0] = g[9] = 0
+        for (int k = 0; k < 8; ++k) g[k + 1] = f ? t.q[f][k] : (k == 3 ? 64 : 0);
+        for (int i = 0; i < 4; ++i) t.qpk[f][i] = pair16(g[2 * i + 1], g[2 * i + 2]);
+        for (int i = 0; i < 5; ++i) t.qpk[f][4 + i] = pair16(g[2 * i], g[2 * i + 1]);
+    }
+    for (int f = 0; f < 8; ++f) {
+        int g[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) g[k + 1] = f ? t.e[f][k] : (k == 1 ? 64 : 0);
+        for (int i = 0; i < 2; ++i) t.epk[f][i] = pair16(g[2 * i + 1], g[2 * i + 2]);
+        for (int i = 0; i < 3; ++i) t.epk[f][2 + i] = pair16(g[2 * i], g[2 * i + 1]);
+    }
+    return t;
+}
+__constant__ FilterTables kF = make_filter_tables();
 // tC' / beta' (src/hevc/tables_gen.rs:195-199).
 __constant__ uint8_t kTc[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
                                 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4,
@@ -73,90 +103,6 @@ __device__ void intra_copy(const HevcPicDev& P, const rv_hevc_cu& cu, int tid) {
 
 typedef short short2v __attribute__((ext_vector_type(2)));
 
-// Staged window of one component and list: (bw + TAPS - 1) x (bh + TAPS - 1)
-// samples at (xi - REACH, yi - REACH), coordinates clamped to the picture
-// (the replicated border of src/hevc/frame.rs:169-214 and the gather of
-// src/hevc/inter.rs:116-123 read the same samples). Lanes cover WIN_W-wide
-// row slots; NLD loads per lane, all issued before any is used.
-template <int TAPS, int WIN_W, int NLD>
-struct WinLoad {
-    int v[NLD];
-    template <typename S>
-    __device__ __forceinline__ void issue(const S* __restrict__ ref, int pitch, int pw, int ph, int xi, int yi,
-                                          int tid) {
-        constexpr int REACH = TAPS / 2 - 1;
-#pragma unroll
-        for (int k = 0; k < NLD; ++k) {
-            // Clamped, so always a valid address; slots past the window
-            // are staged but never read.
-            const int e = tid + 64 * k, r = e / WIN_W, c = e % WIN_W;
-            const int gx = clampi(xi - REACH + c, 0, pw - 1), gy = clampi(yi - REACH + r, 0, ph - 1);
-            v[k] = (int)ref[gy * pitch + gx];
-        }
-    }
-    __device__ __forceinline__ void store(int16_t* win, int tid) const {
-#pragma unroll
-        for (int k = 0; k < NLD; ++k) {
-            const int e = tid + 64 * k;
-            if (e < (TAPS == 8 ? 23 : 11) * WIN_W) win[e] = (int16_t)v[k];
-        }
-    }
-};
-
-// The 14-bit prediction of one component and list from its staged window
-// (interp_scalar, src/dsp/hevc.rs:823-882), NOUT outputs per lane in
-// OUT_W-wide slots. Intermediates of the two-pass case are truncated to i16
-// as the reference's first stage stores them.
-template <int TAPS, int WIN_W, int OUT_W, int NOUT>
-__device__ void mc_predict(const int16_t* win, int16_t* tmp, int fx, int fy, int bw, int bh, int bd, int tid,
-                           int out[NOUT]) {
-    constexpr int REACH = TAPS / 2 - 1;
-    const int hh = bh + TAPS - 1;
-    const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
-    const int8_t* fxt = TAPS == 8 ? kQpel[fx] : kEpel[fx];
-    const int8_t* fyt = TAPS == 8 ? kQpel[fy] : kEpel[fy];
-    if (fx != 0 && fy != 0) {
-        for (int e = tid; e < hh * OUT_W; e += 64) {
-            int r = e / OUT_W, c = e - r * OUT_W;
-            if (c < bw) {
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < TAPS; ++k) acc += fxt[k] * (int)win[r * WIN_W + c + k];
-                tmp[r * OUT_W + c] = (int16_t)(acc >> shift1);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k2 = 0; k2 < NOUT; ++k2) {
-        int o = tid + 64 * k2;
-        int oy = o / OUT_W, ox = o - oy * OUT_W;
-        int v = 0;
-        if (ox < bw && oy < bh) {
-            if (fx == 0 && fy == 0) {
-                v = (int16_t)((int)win[(oy + REACH) * WIN_W + ox + REACH] << shift3);
-            } else if (fy == 0) {
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < TAPS; ++k) acc += fxt[k] * (int)win[(oy + REACH) * WIN_W + ox + k];
-                v = (int16_t)(acc >> shift1);
-            } else if (fx == 0) {
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < TAPS; ++k) acc += fyt[k] * (int)win[(oy + k) * WIN_W + ox + REACH];
-                v = (int16_t)(acc >> shift1);
-            } else {
-                int acc = 0;
-#pragma unroll
-                for (int k = 0; k < TAPS; ++k) acc += fyt[k] * (int)tmp[(oy + k) * OUT_W + ox];
-                v = (int16_t)(acc >> 6);
-            }
-        }
-        out[k2] = v;
-    }
-    __syncthreads();
-}
-
 // The combine of predict_block (src/hevc/inter.rs:712-741) with the
 // scalar combiners (src/dsp/hevc.rs:997-1087).
 __device__ __forceinline__ int combine(const HevcPicDev& P, int c, int dir, const int8_t* ref_idx, int a, int b,
@@ -192,25 +138,33 @@ __device__ __forceinline__ short2v pk(int lo, int hi) {
     r.y = (short)hi;
     return r;
 }
+__device__ __forceinline__ short2v as2(uint32_t v) { return __builtin_bit_cast(short2v, v); }
 __device__ __forceinline__ int dot2(short2v a, short2v b, int c) { return __builtin_amdgcn_sdot2(a, b, c, false); }
 
-// The eight taps of one phase as packed pairs: even[i] = (f[2i], f[2i+1])
-// for outputs whose inputs start on a pair boundary, odd[i] = (f[2i-1],
-// f[2i]) (f[-1] = f[8] = 0) for outputs starting one sample later.
+// The taps of one phase as packed pairs (FilterTables::qpk / epk), per lane:
+// the blocks of one wave have their own vectors.
 struct Taps8 {
     short2v e[4], o[5];
-    __device__ explicit Taps8(int frac) {
-        int f[10];
-        f[0] = 0;
-        f[9] = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) f[k + 1] = kQpel[frac][k];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) e[i] = pk(f[2 * i + 1], f[2 * i + 2]);
-#pragma unroll
-        for (int i = 0; i < 5; ++i) o[i] = pk(f[2 * i], f[2 * i + 1]);
-    }
 };
+__device__ __forceinline__ Taps8 luma_taps(int frac) {
+    const uint4* p = (const uint4*)kF.qpk[frac];
+    const uint4 a = p[0], b = p[1], c = p[2];
+    Taps8 t;
+    t.e[0] = as2(a.x), t.e[1] = as2(a.y), t.e[2] = as2(a.z), t.e[3] = as2(a.w);
+    t.o[0] = as2(b.x), t.o[1] = as2(b.y), t.o[2] = as2(b.z), t.o[3] = as2(b.w), t.o[4] = as2(c.x);
+    return t;
+}
+struct Taps4 {
+    short2v e[2], o[3];
+};
+__device__ __forceinline__ Taps4 chroma_taps(int frac) {
+    const uint4* p = (const uint4*)kF.epk[frac];
+    const uint4 a = p[0], b = p[1];
+    Taps4 t;
+    t.e[0] = as2(a.x), t.e[1] = as2(a.y);
+    t.o[0] = as2(a.z), t.o[1] = as2(a.w), t.o[2] = as2(b.x);
+    return t;
+}
 
 // Four consecutive 8-tap outputs from twelve inputs held as six packed
 // pairs p[0..5] (inputs 2i, 2i+1): output j = sum_k f[k] in[j + k], exact
@@ -222,159 +176,181 @@ __device__ __forceinline__ void filt4(const short2v (&p)[6], const Taps8& t, int
     acc[3] = dot2(p[5], t.o[4], dot2(p[4], t.o[3], dot2(p[3], t.o[2], dot2(p[2], t.o[1], dot2(p[1], t.o[0], 0)))));
 }
 
+// The same for four 4-tap outputs from eight inputs (four pairs).
+__device__ __forceinline__ void filt4c(const short2v (&p)[4], const Taps4& t, int (&acc)[4]) {
+    acc[0] = dot2(p[1], t.e[1], dot2(p[0], t.e[0], 0));
+    acc[1] = dot2(p[2], t.o[2], dot2(p[1], t.o[1], dot2(p[0], t.o[0], 0)));
+    acc[2] = dot2(p[2], t.e[1], dot2(p[1], t.e[0], 0));
+    acc[3] = dot2(p[3], t.o[2], dot2(p[2], t.o[1], dot2(p[1], t.o[0], 0)));
+}
+
 __device__ __forceinline__ void load_pairs(const int16_t* src, short2v (&p)[6]) {
     // Three 8-byte LDS reads; src is 8-byte aligned.
     const uint2 a = *(const uint2*)src, b = *(const uint2*)(src + 4), c = *(const uint2*)(src + 8);
-    p[0] = __builtin_bit_cast(short2v, a.x);
-    p[1] = __builtin_bit_cast(short2v, a.y);
-    p[2] = __builtin_bit_cast(short2v, b.x);
-    p[3] = __builtin_bit_cast(short2v, b.y);
-    p[4] = __builtin_bit_cast(short2v, c.x);
-    p[5] = __builtin_bit_cast(short2v, c.y);
+    p[0] = as2(a.x), p[1] = as2(a.y), p[2] = as2(b.x), p[3] = as2(b.y), p[4] = as2(c.x), p[5] = as2(c.y);
 }
 
-// Luma 14-bit prediction of one list from its staged window (interp_scalar,
-// src/dsp/hevc.rs:823-882; 8-tap QPEL_FILTERS). Lane (x = tid & 15,
-// y0 = 4 * (tid >> 4)) produces rows y0..y0+3 of column x. The horizontal
-// pass reads window rows as 8-byte LDS vectors, four outputs per task, and
-// writes them transposed (tmpT[x * 24 + row]) so the vertical pass reads
-// its eleven inputs as three 8-byte vectors too. First-stage values are
-// truncated to i16 as the reference stores them.
-__device__ void luma_predict(const int16_t* win, int16_t* tmpT, int fx, int fy, int tw, int th, int bd, int tid,
-                             int out[4]) {
-    const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
-    if (fx != 0) {
-        const Taps8 t(fx);
-        const int r0 = fy ? 0 : 3, nr = fy ? th + 7 : th, ng = tw >> 2;
-        for (int e = tid; e < nr * 4; e += 64) {
-            const int r = e >> 2, g = e & 3;
-            if (g < ng) {
-                short2v p[6];
-                load_pairs(win + (r0 + r) * 32 + 4 * g, p);
-                int acc[4];
-                filt4(p, t, acc);
+// N (8 or 12) samples of one reference row from column gx on, as N / 2
+// packed i16 pairs. The row index is clamped to the plane; when all N
+// columns are inside it they come from aligned dword loads and alignbyte,
+// otherwise from a gather with clamped columns. Both read the samples of
+// the reference's replicated border (src/hevc/frame.rs:169-214) and
+// clamped gather (src/hevc/inter.rs:116-123). The aligned loads may read
+// up to 4 bytes past the last sample of a row: planes carry that slack.
+template <typename S, int N>
+__device__ __forceinline__ void ref_row(const S* __restrict__ pl, int pitch, int pw, int ph, int gx, int gy,
+                                        short2v (&p)[N / 2]) {
+    const S* row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
+    if (gx >= 0 && gx + N - 1 < pw) {
+        const uintptr_t a = (uintptr_t)(row + gx);
+        const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)(a & 3);
+        constexpr int ND = N * sizeof(S) / 4 + 1;  // dwords covering the span
+        uint32_t d[ND];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) tmpT[(4 * g + i) * 24 + r] = (int16_t)(acc[i] >> shift1);
+        for (int i = 0; i < ND; ++i) d[i] = w[i];
+        if constexpr (sizeof(S) == 1) {
+#pragma unroll
+            for (int i = 0; i < N / 4; ++i) {
+                const uint32_t v = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+                p[2 * i] = as2(__builtin_amdgcn_perm(0u, v, 0x0c010c00u));
+                p[2 * i + 1] = as2(__builtin_amdgcn_perm(0u, v, 0x0c030c02u));
             }
-        }
-        __syncthreads();
-    }
-    const int x = tid & 15, y0 = (tid >> 4) * 4;
-    out[0] = out[1] = out[2] = out[3] = 0;
-    if (x < tw && y0 < th) {
-        if (fx == 0 && fy == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) out[k] = (int16_t)((int)win[(y0 + 3 + k) * 32 + x + 3] << shift3);
-        } else if (fy == 0) {
-            const uint2 a = *(const uint2*)(tmpT + x * 24 + y0);
-            const short2v lo = __builtin_bit_cast(short2v, a.x), hi = __builtin_bit_cast(short2v, a.y);
-            out[0] = lo.x;
-            out[1] = lo.y;
-            out[2] = hi.x;
-            out[3] = hi.y;
         } else {
-            const Taps8 t(fy);
-            short2v p[6];
-            int sh;
-            if (fx == 0) {
-                int sv[12];
 #pragma unroll
-                for (int k = 0; k < 11; ++k) sv[k] = win[(y0 + k) * 32 + x + 3];
-                sv[11] = 0;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) p[i] = pk(sv[2 * i], sv[2 * i + 1]);
-                sh = shift1;
-            } else {
-                load_pairs(tmpT + x * 24 + y0, p);
-                sh = 6;
-            }
-            int acc[4];
-            filt4(p, t, acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) out[i] = (int16_t)(acc[i] >> sh);
+            for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sh));
         }
+        return;
     }
-    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i)
+        p[i] = pk(row[clampi(gx + 2 * i, 0, pw - 1)], row[clampi(gx + 2 * i + 1, 0, pw - 1)]);
 }
 
-// LDS of one MC work item: luma windows per list, chroma windows per list
-// and component, and the first-stage rows of the two-pass case.
-struct McLds {
-    int16_t luma[2][23 * 32];
-    int16_t chroma[2][2][11 * 16];
-    int16_t tmp[16 * 24];
+// Plane c of reference ri of list l, chosen per lane among the picture's
+// (uniform) reference table.
+__device__ __forceinline__ const void* ref_plane(const HevcPicDev& P, int l, int ri, int c) {
+    const void* p0 = P.ref[l][0][c];
+    const void* p1 = P.ref[l][1][c];
+    const void* p2 = P.ref[l][2][c];
+    const void* p3 = P.ref[l][3][c];
+    return ri == 0 ? p0 : ri == 1 ? p1 : ri == 2 ? p2 : p3;
+}
+
+// LDS of one MC work item (four blocks, both lists): first-stage outputs,
+// transposed ([column][row]) so the second stage reads a column as 8-byte
+// (luma) or 4-byte (chroma) vectors.
+struct alignas(16) McLds {
+    int16_t ltmp[2][4][8 * 16];
+    int16_t ctmp[2][4][2][4 * 8];
 };
 
-// All window loads of one MC tile: luma and both chroma components of
-// each list it uses.
-struct TileLoads {
-    WinLoad<8, 32, 12> l[2];
-    WinLoad<4, 16, 3> c[2][2];
-};
-
+// Motion compensation of four blocks of at most 8x8 (predict_block,
+// src/hevc/inter.rs:555-742, with interp_scalar and the combiners,
+// src/dsp/hevc.rs:823-1087). Lanes 16q..16q+15 serve block first + q.
+// First stage: lane task (row, 4-column group) of the luma support (bh + 7
+// rows) and (component, row) of the chroma support (ch + 3 rows), read
+// straight from the reference planes. Second stage: luma lane (x = s & 7,
+// rows 4 (s >> 3) ..+3), chroma lane (component s >> 3, x = s & 3, rows
+// 2 ((s >> 2) & 1) ..+1). Every lane reaches the barrier.
 template <typename S>
-__device__ __forceinline__ void tile_issue_t(const HevcPicDev& P, const McTile& t, TileLoads& w, int tid) {
-    const int cx = t.x >> 1, cy = t.y >> 1;
+__device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
+    const int slot = tid >> 4, s = tid & 15;
+    // The wave's four records (uniform loads: the list is padded to a
+    // multiple of four with empty records, dir 0).
+    const uint4* rec = (const uint4*)(P.tiles + first);
+    const uint4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+    const uint4 q = slot == 0 ? q0 : slot == 1 ? q1 : slot == 2 ? q2 : q3;
+    const McTile t = __builtin_bit_cast(McTile, q);
+    const int bw = t.tw, bh = t.th, dir = t.dir, bd = P.bd;
+    const int shift1 = min(bd, 12) - 8;
+    int ri[2], mvx[2], mvy[2];
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-        if (!((t.dir >> l) & 1)) continue;
-        const int mvx = t.mv[l].x, mvy = t.mv[l].y;
-        const int ri = (t.refs >> (4 * l)) & 15;
-        w.l[l].issue((const S*)P.ref[l][ri][0], P.pitch[0], P.pw[0], P.ph[0], t.x + (mvx >> 2),
-                     t.y + (mvy >> 2), tid);
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-            w.c[l][c].issue((const S*)P.ref[l][ri][c + 1], P.pitch[c + 1], P.pw[c + 1], P.ph[c + 1],
-                            cx + (mvx >> 3), cy + (mvy >> 3), tid);
+        ri[l] = (t.refs >> (4 * l)) & 15;
+        mvx[l] = t.mv[l].x;
+        mvy[l] = t.mv[l].y;
     }
-}
-
-__device__ __forceinline__ void tile_issue(const HevcPicDev& P, const McTile& t, TileLoads& w, int tid) {
-    if (P.bd == 8) tile_issue_t<uint8_t>(P, t, w, tid);
-    else tile_issue_t<uint16_t>(P, t, w, tid);
-}
-
-// Stage the loaded windows, predict, combine and write one tile.
-template <typename S>
-__device__ void tile_finish_t(const HevcPicDev& P, const McTile& t, const TileLoads& w, int tid, McLds& L) {
-    const int x = t.x, y = t.y, tw = t.tw, th = t.th, bd = P.bd, dir = t.dir;
-    const int cw = tw >> 1, ch = th >> 1, cx = x >> 1, cy = y >> 1;
-    const int8_t ref_idx[2] = {(int8_t)(t.refs & 15), (int8_t)(t.refs >> 4)};
+    // 1. First (horizontal) stage.
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
-        w.l[l].store(L.luma[l], tid);
+        {
+            const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
+            const Taps8 tp = luma_taps(mvx[l] & 3);
+            const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
+            const int nt = (bh + 7) * 2, ng = bw >> 2;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) w.c[l][c].store(L.chroma[l][c], tid);
+            for (int j = 0; j < 2; ++j) {
+                const int e = s + 16 * j, r = e >> 1, g = e & 1;
+                if (e < nt && g < ng) {
+                    short2v p[6];
+                    ref_row<S, 12>(ref, P.pitch[0], P.pw[0], P.ph[0], x0 + 4 * g, y0 + r, p);
+                    int acc[4];
+                    filt4(p, tp, acc);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
+                }
+            }
+        }
+        {
+            const int nr = (bh >> 1) + 3;
+            if (s < 2 * nr) {  // component c, row r of its support
+                const int c = s >= nr ? 1 : 0, r = s - nr * c;
+                const S* ref = (const S*)(c ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
+                const Taps4 tp = chroma_taps(mvx[l] & 7);
+                short2v p[4];
+                ref_row<S, 8>(ref, P.pitch[1], P.pw[1], P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
+                              (t.y >> 1) + (mvy[l] >> 3) - 1 + r, p);
+                int acc[4];
+                filt4c(p, tp, acc);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) L.ctmp[l][slot][c][i * 8 + r] = (int16_t)(acc[i] >> shift1);
+            }
+        }
     }
     __syncthreads();
-    // Luma: lane (x, 4-row group), four outputs.
+    // 2. Second (vertical) stage, combine, store.
+    const int8_t ref_idx[2] = {(int8_t)ri[0], (int8_t)ri[1]};
     {
+        const int x = s & 7, y0 = (s >> 3) * 4;
         int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            luma_predict(L.luma[l], L.tmp, t.mv[l].x & 3, t.mv[l].y & 3, tw, th, bd, tid, pred[l]);
-        }
-        S* dst = (S*)P.cur[0];
-        const int ox = tid & 15, oy0 = (tid >> 4) * 4;
-        if (ox < tw && oy0 < th) {
+            const Taps8 tp = luma_taps(mvy[l] & 3);
+            short2v p[6];
+            load_pairs(L.ltmp[l][slot] + x * 16 + y0, p);
+            int acc[4];
+            filt4(p, tp, acc);
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                dst[(y + oy0 + k) * P.pitch[0] + x + ox] = (S)combine(P, 0, dir, ref_idx, pred[0][k], pred[1][k], bd);
+            for (int i = 0; i < 4; ++i) pred[l][i] = (int16_t)(acc[i] >> 6);
+        }
+        if (dir && x < bw && y0 < bh) {
+            S* dst = (S*)P.cur[0] + (size_t)(t.y + y0) * P.pitch[0] + t.x + x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) dst[k * P.pitch[0]] = (S)combine(P, 0, dir, ref_idx, pred[0][k], pred[1][k], bd);
         }
     }
     // Chroma (4:2:0): eighth-sample vectors in chroma units
-    // (src/hevc/inter.rs:581-594), outputs in 8-wide slots, one per lane.
-    for (int c = 1; c < 3; ++c) {
-        int pred[2][1] = {{0}, {0}};
+    // (src/hevc/inter.rs:581-594).
+    {
+        const int c = s >> 3, x = s & 3, y0 = ((s >> 2) & 1) * 2;
+        int pred[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            mc_predict<4, 16, 8, 1>(L.chroma[l][c - 1], L.tmp, t.mv[l].x & 7, t.mv[l].y & 7, cw, ch, bd, tid, pred[l]);
+            const Taps4 tp = chroma_taps(mvy[l] & 7);
+            const uint32_t* T = (const uint32_t*)(L.ctmp[l][slot][c] + x * 8 + y0);
+            const short2v p0 = as2(T[0]), p1 = as2(T[1]), p2 = as2(T[2]);
+            pred[l][0] = (int16_t)(dot2(p1, tp.e[1], dot2(p0, tp.e[0], 0)) >> 6);
+            pred[l][1] = (int16_t)(dot2(p2, tp.o[2], dot2(p1, tp.o[1], dot2(p0, tp.o[0], 0))) >> 6);
         }
-        const int oy = tid >> 3, ox = tid & 7;
-        if (ox < cw && oy < ch) {
-            S* dst = (S*)P.cur[c];
-            dst[(cy + oy) * P.pitch[c] + cx + ox] = (S)combine(P, c, dir, ref_idx, pred[0][0], pred[1][0], bd);
+        if (dir && x < (bw >> 1) && y0 < (bh >> 1)) {
+            const int pitch = P.pitch[1 + c];
+            S* dst = (S*)P.cur[1 + c] + (size_t)((t.y >> 1) + y0) * pitch + (t.x >> 1) + x;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) dst[k * pitch] = (S)combine(P, 1 + c, dir, ref_idx, pred[0][k], pred[1][k], bd);
         }
     }
 }
@@ -415,7 +391,7 @@ __device__ void side_entry(const HevcPicDev& P, int k) {
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage0(BatchDev b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) hevc_stage0(BatchDev b) {
     __shared__ McLds lds;
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = xcd_item();
@@ -436,16 +412,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) he
         return;
     }
     i -= P.n_intra;
-    McTile t;
-    *(uint4*)&t = *(const uint4*)&P.tiles[i];
-    TileLoads w;
-    if (P.bd == 8) {
-        tile_issue_t<uint8_t>(P, t, w, tid);
-        tile_finish_t<uint8_t>(P, t, w, tid, lds);
-    } else {
-        tile_issue_t<uint16_t>(P, t, w, tid);
-        tile_finish_t<uint16_t>(P, t, w, tid, lds);
-    }
+    if (P.bd == 8) mc_item<uint8_t>(P, 4 * i, tid, lds);
+    else mc_item<uint16_t>(P, 4 * i, tid, lds);
 }
 
 // ---------------------------------------------------------------------
@@ -1143,7 +1111,7 @@ __device__ void entry_elem(const HevcEntry& e) {
                 const long base = y * e.src_stride + x;
                 int acc = 0;
                 for (int k = 0; k < taps; ++k) {
-                    const int f = e.luma ? kQpel[e.frac][k] : kEpel[e.frac][k];
+                    const int f = e.luma ? kF.q[e.frac][k] : kF.e[e.frac][k];
                     const int v = e.op == HE_FILT_V2 ? (int)a16[base + k * along] : (int)((const S*)e.src)[base + k * along];
                     acc += f * v;
                 }
@@ -1196,8 +1164,8 @@ __device__ void entry_fused(const HevcEntry& e, int16_t* tmp) {
     const long ss = e.src_stride;
     const int w = e.w, h = e.h, bd = e.bd;
     const int shift1 = min(bd, 12) - 8, shift3 = 14 - bd;
-    auto fxk = [&](int k) { return e.luma ? (int)kQpel[e.fx][k] : (int)kEpel[e.fx][k]; };
-    auto fyk = [&](int k) { return e.luma ? (int)kQpel[e.fy][k] : (int)kEpel[e.fy][k]; };
+    auto fxk = [&](int k) { return e.luma ? (int)kF.q[e.fx][k] : (int)kF.e[e.fx][k]; };
+    auto fyk = [&](int k) { return e.luma ? (int)kF.q[e.fy][k] : (int)kF.e[e.fy][k]; };
     if (e.fx && e.fy) {
         for (int i = threadIdx.x; i < w * (h + taps - 1); i += 256) {
             const int y = i / w, x = i - y * w;
@@ -1319,7 +1287,7 @@ hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStrea
 // Work items per stage for one picture (the grid unit of each stage).
 void hevc_stage_items(HevcPicDev& P) {
     P.n_side = (P.n_cus + P.n_pus + P.n_ltus + 63) / 64;
-    P.items[0] = P.n_side + P.n_intra + P.n_tiles;
+    P.items[0] = P.n_side + P.n_intra + (P.n_tiles + 3) / 4;
     P.tx_items[0] = 0;
     for (int c = 0; c < 4; ++c) {
         int tpw = 16 >> c;

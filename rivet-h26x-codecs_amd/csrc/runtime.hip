@@ -61,6 +61,11 @@ namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Bytes after each DPB plane: motion compensation reads reference rows
+// with aligned dword loads that may extend up to 4 bytes past a row's last
+// sample (past the plane's end on its last row).
+constexpr size_t kPlaneSlack = 256;
+
 struct PlaneGeom {
     int pw[3], ph[3], pitch[3];
     size_t bytes[3];
@@ -401,7 +406,7 @@ int64_t rv_session_footprint(int codec, int width, int height, int bit_depth, in
     // DPB + one job's reconstruction planes and side maps + work lists
     // (bounded by the coefficients of a fully coded picture).
     int64_t lists = (int64_t)width * height * 3 / 2 * (codec == RV_CODEC_H264 ? 4 : 2) / 4 + n4 * 32;
-    return (int64_t)dpb_slots * pic + pic + n4 * 16 + lists;
+    return (int64_t)dpb_slots * (pic + 3 * (int64_t)kPlaneSlack) + pic + n4 * 16 + lists;
 }
 
 int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_slots, rv_session** out) {
@@ -425,7 +430,7 @@ int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_s
         s->g = plane_geom(width, height, bit_depth);
         s->planes.assign((size_t)dpb_slots * 3, nullptr);
         for (int i = 0; i < dpb_slots * 3; ++i) {
-            hipError_t e = hipMalloc(&s->planes[i], s->g.bytes[i % 3]);
+            hipError_t e = hipMalloc(&s->planes[i], s->g.bytes[i % 3] + kPlaneSlack);
             if (e != hipSuccess) {
                 for (void* p : s->planes)
                     if (p) (void)hipFree(p);
@@ -507,17 +512,17 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         const int W = p->width, H = p->height;
         // Host-side lists: MC tiles, luma TUs, coded TUs (4x4 first).
         std::vector<rv::McTile> tiles;
-        tiles.reserve((size_t)p->n_pus * 2);
+        tiles.reserve((size_t)p->n_pus * 4);
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
-            for (int dy = 0; dy < u.h; dy += 16)
-                for (int dx = 0; dx < u.w; dx += 16) {
+            for (int dy = 0; dy < u.h; dy += 8)
+                for (int dx = 0; dx < u.w; dx += 8) {
                     rv::McTile t;
                     std::memset(&t, 0, sizeof(t));
                     t.x = (uint16_t)(u.x + dx);
                     t.y = (uint16_t)(u.y + dy);
-                    t.tw = (uint8_t)std::min(16, u.w - dx);
-                    t.th = (uint8_t)std::min(16, u.h - dy);
+                    t.tw = (uint8_t)std::min(8, u.w - dx);
+                    t.th = (uint8_t)std::min(8, u.h - dy);
                     t.dir = u.dir;
                     t.refs = 0;
                     for (int l = 0; l < 2; ++l) {
@@ -526,6 +531,13 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     }
                     tiles.push_back(t);
                 }
+        }
+        // A wave serves four blocks and reads their records together: pad
+        // with empty blocks (dir 0) to a multiple of four.
+        while (tiles.size() % 4) {
+            rv::McTile z;
+            std::memset(&z, 0, sizeof(z));
+            tiles.push_back(z);
         }
         std::vector<rv_hevc_tu> ltus, bysize[4];
         for (int i = 0; i < p->n_tus; ++i) {

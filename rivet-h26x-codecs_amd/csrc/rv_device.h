@@ -9,13 +9,13 @@
 
 namespace rv {
 
-// One motion-compensation work item: a tile of at most 16x16 luma samples
-// (and its two 4:2:0 chroma tiles) of one prediction unit, carrying the
-// PU's motion so a wave reads one record per item. Splitting PUs into
-// tiles on the host keeps work items of even size.
+// One motion-compensation block: at most 8x8 luma samples (and its two
+// 4:2:0 chroma blocks) of one prediction unit, carrying the PU's motion so
+// a lane group reads one record per block. Splitting PUs into blocks on the
+// host keeps work of even size; a wave serves four blocks.
 struct alignas(16) McTile {
-    uint16_t x, y;    // tile top-left, luma samples
-    uint8_t tw, th;   // tile size, luma samples (multiples of 4, <= 16)
+    uint16_t x, y;    // block top-left, luma samples
+    uint8_t tw, th;   // block size, luma samples (4 or 8)
     uint8_t dir;      // bit 0: list 0, bit 1: list 1
     uint8_t refs;     // ref_idx of list 0 in bits 0..3, list 1 in bits 4..7
     rv_mv mv[2];
