@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 
 #include "rv_device.h"
 #include "rv_kernels.h"
@@ -141,28 +142,33 @@ __device__ __forceinline__ short2v pk(int lo, int hi) {
 __device__ __forceinline__ short2v as2(uint32_t v) { return __builtin_bit_cast(short2v, v); }
 __device__ __forceinline__ int dot2(short2v a, short2v b, int c) { return __builtin_amdgcn_sdot2(a, b, c, false); }
 
-// The taps of one phase as packed pairs (FilterTables::qpk / epk), per lane:
-// the blocks of one wave have their own vectors.
+// The taps of one phase as packed pairs (FilterTables::qpk / epk), per lane
+// (the blocks of one wave have their own vectors), read from a copy of the
+// packed tables in LDS: `tab` holds qpk (4 x 12 words) then epk (8 x 8).
+constexpr int kTapWords = 4 * 12 + 8 * 8;
+static_assert(offsetof(FilterTables, epk) == offsetof(FilterTables, qpk) + sizeof(uint32_t) * 48,
+              "qpk and epk are contiguous");
 struct Taps8 {
     short2v e[4], o[5];
 };
-__device__ __forceinline__ Taps8 luma_taps(int frac) {
-    const uint4* p = (const uint4*)kF.qpk[frac];
-    const uint4 a = p[0], b = p[1], c = p[2];
+__device__ __forceinline__ Taps8 luma_taps(const uint32_t* tab, int frac) {
+    const uint4* p = (const uint4*)(tab + 12 * frac);
+    const uint4 a = p[0], b = p[1];
+    const uint32_t c = tab[12 * frac + 8];
     Taps8 t;
     t.e[0] = as2(a.x), t.e[1] = as2(a.y), t.e[2] = as2(a.z), t.e[3] = as2(a.w);
-    t.o[0] = as2(b.x), t.o[1] = as2(b.y), t.o[2] = as2(b.z), t.o[3] = as2(b.w), t.o[4] = as2(c.x);
+    t.o[0] = as2(b.x), t.o[1] = as2(b.y), t.o[2] = as2(b.z), t.o[3] = as2(b.w), t.o[4] = as2(c);
     return t;
 }
 struct Taps4 {
     short2v e[2], o[3];
 };
-__device__ __forceinline__ Taps4 chroma_taps(int frac) {
-    const uint4* p = (const uint4*)kF.epk[frac];
-    const uint4 a = p[0], b = p[1];
+__device__ __forceinline__ Taps4 chroma_taps(const uint32_t* tab, int frac) {
+    const uint4 a = *(const uint4*)(tab + 48 + 8 * frac);
+    const uint32_t b = tab[48 + 8 * frac + 4];
     Taps4 t;
     t.e[0] = as2(a.x), t.e[1] = as2(a.y);
-    t.o[0] = as2(a.z), t.o[1] = as2(a.w), t.o[2] = as2(b.x);
+    t.o[0] = as2(a.z), t.o[1] = as2(a.w), t.o[2] = as2(b);
     return t;
 }
 
@@ -197,14 +203,34 @@ __device__ __forceinline__ void load_pairs(const int16_t* src, short2v (&p)[6]) 
 // the reference's replicated border (src/hevc/frame.rs:169-214) and
 // clamped gather (src/hevc/inter.rs:116-123). The aligned loads may read
 // up to 4 bytes past the last sample of a row: planes carry that slack.
+// The clamped gather of ref_row, for spans that cross the plane's left or
+// right edge. Kept out of line: inlined, its per-column clamps are hoisted
+// above the branch and paid by every span.
+template <int N>
+struct Pairs {
+    uint32_t v[N / 2];
+};
+template <typename S, int N>
+__device__ __attribute__((noinline)) Pairs<N> ref_row_clamped(const S* __restrict__ row, int pw, int gx) {
+    Pairs<N> r;
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        const int a = row[clampi(gx + 2 * i, 0, pw - 1)], b = row[clampi(gx + 2 * i + 1, 0, pw - 1)];
+        r.v[i] = (uint32_t)(a & 0xFFFF) | ((uint32_t)b << 16);
+    }
+    return r;
+}
+
 template <typename S, int N>
 __device__ __forceinline__ void ref_row(const S* __restrict__ pl, int pitch, int pw, int ph, int gx, int gy,
                                         short2v (&p)[N / 2]) {
+    // Rows start 256-byte aligned, so the span's dwords are at the row
+    // pointer plus its byte offset rounded down.
     const S* row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
     if (gx >= 0 && gx + N - 1 < pw) {
-        const uintptr_t a = (uintptr_t)(row + gx);
-        const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
-        const uint32_t sh = (uint32_t)(a & 3);
+        const uint32_t off = (uint32_t)gx * sizeof(S);
+        const uint32_t* w = (const uint32_t*)((const char*)row + (off & ~3u));
+        const uint32_t sh = off & 3u;
         constexpr int ND = N * sizeof(S) / 4 + 1;  // dwords covering the span
         uint32_t d[ND];
 #pragma unroll
@@ -222,9 +248,9 @@ __device__ __forceinline__ void ref_row(const S* __restrict__ pl, int pitch, int
         }
         return;
     }
+    const Pairs<N> v = ref_row_clamped<S, N>(row, pw, gx);
 #pragma unroll
-    for (int i = 0; i < N / 2; ++i)
-        p[i] = pk(row[clampi(gx + 2 * i, 0, pw - 1)], row[clampi(gx + 2 * i + 1, 0, pw - 1)]);
+    for (int i = 0; i < N / 2; ++i) p[i] = as2(v.v[i]);
 }
 
 // Plane c of reference ri of list l, chosen per lane among the picture's
@@ -241,6 +267,7 @@ __device__ __forceinline__ const void* ref_plane(const HevcPicDev& P, int l, int
 // transposed ([column][row]) so the second stage reads a column as 8-byte
 // (luma) or 4-byte (chroma) vectors.
 struct alignas(16) McLds {
+    uint32_t taps[kTapWords];  // packed filter pairs, copied from kF
     int16_t ltmp[2][4][8 * 16];
     int16_t ctmp[2][4][2][4 * 8];
 };
@@ -262,6 +289,11 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
     const uint4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
     const uint4 q = slot == 0 ? q0 : slot == 1 ? q1 : slot == 2 ? q2 : q3;
     const McTile t = __builtin_bit_cast(McTile, q);
+    // The packed taps, read per lane below: an LDS copy costs two coalesced
+    // loads per wave instead of one per lane and tap set.
+    const uint32_t* kt = &kF.qpk[0][0];
+    for (int i = tid; i < kTapWords; i += 64) L.taps[i] = kt[i];
+    __syncthreads();
     const int bw = t.tw, bh = t.th, dir = t.dir, bd = P.bd;
     const int shift1 = min(bd, 12) - 8;
     int ri[2], mvx[2], mvy[2];
@@ -277,7 +309,7 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
         if (!((dir >> l) & 1)) continue;
         {
             const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
-            const Taps8 tp = luma_taps(mvx[l] & 3);
+            const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
             const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
             const int nt = (bh + 7) * 2, ng = bw >> 2;
 #pragma unroll
@@ -298,7 +330,7 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
             if (s < 2 * nr) {  // component c, row r of its support
                 const int c = s >= nr ? 1 : 0, r = s - nr * c;
                 const S* ref = (const S*)(c ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
-                const Taps4 tp = chroma_taps(mvx[l] & 7);
+                const Taps4 tp = chroma_taps(L.taps, mvx[l] & 7);
                 short2v p[4];
                 ref_row<S, 8>(ref, P.pitch[1], P.pw[1], P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
                               (t.y >> 1) + (mvy[l] >> 3) - 1 + r, p);
@@ -318,7 +350,7 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
 #pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            const Taps8 tp = luma_taps(mvy[l] & 3);
+            const Taps8 tp = luma_taps(L.taps, mvy[l] & 3);
             short2v p[6];
             load_pairs(L.ltmp[l][slot] + x * 16 + y0, p);
             int acc[4];
@@ -340,7 +372,7 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
 #pragma unroll
         for (int l = 0; l < 2; ++l) {
             if (!((dir >> l) & 1)) continue;
-            const Taps4 tp = chroma_taps(mvy[l] & 7);
+            const Taps4 tp = chroma_taps(L.taps, mvy[l] & 7);
             const uint32_t* T = (const uint32_t*)(L.ctmp[l][slot][c] + x * 8 + y0);
             const short2v p0 = as2(T[0]), p1 = as2(T[1]), p2 = as2(T[2]);
             pred[l][0] = (int16_t)(dot2(p1, tp.e[1], dot2(p0, tp.e[0], 0)) >> 6);
@@ -391,7 +423,7 @@ __device__ void side_entry(const HevcPicDev& P, int k) {
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) hevc_stage0(BatchDev b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage0(BatchDev b) {
     __shared__ McLds lds;
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = xcd_item();

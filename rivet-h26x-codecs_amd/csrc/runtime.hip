@@ -557,9 +557,22 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             coefs.insert(coefs.end(), p->nz + t.coef_off, p->nz + t.coef_off + t.n_nz);
             t.coef_off = off;
         }
+        // Side data (stage 0) gives one thread to each CU, PU and luma TU,
+        // which writes all of the entry's 4x4 cells: ordered by size, the
+        // 64 threads of a wave have equal work. Within a size, picture
+        // order is kept. The PU index stored per cell refers to this order,
+        // and the deblocking filter reads motion through it.
+        std::vector<rv_hevc_cu> cus(p->cus, p->cus + p->n_cus);
+        std::vector<rv_hevc_pu> pus(p->pus, p->pus + p->n_pus);
+        std::stable_sort(cus.begin(), cus.end(),
+                         [](const rv_hevc_cu& a, const rv_hevc_cu& b) { return a.log2_size > b.log2_size; });
+        std::stable_sort(pus.begin(), pus.end(),
+                         [](const rv_hevc_pu& a, const rv_hevc_pu& b) { return a.w * a.h > b.w * b.h; });
+        std::stable_sort(ltus.begin(), ltus.end(),
+                         [](const rv_hevc_tu& a, const rv_hevc_tu& b) { return a.log2 > b.log2; });
         std::vector<int32_t> intra_idx;
         for (int i = 0; i < p->n_cus; ++i)
-            if (p->cus[i].intra) intra_idx.push_back(i);
+            if (cus[i].intra) intra_idx.push_back(i);
         int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
         size_t n_sao = (size_t)wc * hc * 3;
         std::vector<rv_sao> sao(n_sao, rv_sao{0, 0, {0, 0, 0, 0}});
@@ -602,8 +615,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         if (!intra_idx.empty()) std::memcpy(h_intra, intra_idx.data(), intra_idx.size() * sizeof(int32_t));
         const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
-        if (p->n_pus) std::memcpy(h_pus, p->pus, p->n_pus * sizeof(rv_hevc_pu));
-        if (p->n_cus) std::memcpy(h_cus, p->cus, p->n_cus * sizeof(rv_hevc_cu));
+        if (p->n_pus) std::memcpy(h_pus, pus.data(), p->n_pus * sizeof(rv_hevc_pu));
+        if (p->n_cus) std::memcpy(h_cus, cus.data(), p->n_cus * sizeof(rv_hevc_cu));
         if (!ltus.empty()) std::memcpy(h_ltus, ltus.data(), ltus.size() * sizeof(rv_hevc_tu));
         if (!ctus.empty()) std::memcpy(h_ctus, ctus.data(), ctus.size() * sizeof(rv_hevc_tu));
         if (!tiles.empty()) std::memcpy(h_tiles, tiles.data(), tiles.size() * sizeof(rv::McTile));
