@@ -73,7 +73,22 @@ __constant__ uint8_t kBeta[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 __device__ __forceinline__ int sgn(int v) { return (v > 0) - (v < 0); }
 
-__device__ __forceinline__ int find_pic(const int32_t* pre, int n, int it) {
+// The batch tables and picture descriptors are written before the launch
+// and only read while the stage kernels run. Reading them through the
+// constant address space lets uniform reads use scalar loads (the compiler
+// cannot otherwise rule out that the kernel's own stores alias them).
+typedef const HevcPicDev __attribute__((address_space(4))) ConstPic;
+typedef ConstPic& PicRef;
+typedef const int32_t __attribute__((address_space(4))) ConstI32;
+
+__device__ __forceinline__ PicRef picture(const BatchDev& b, int p) {
+    return *(ConstPic*)b.hevc[p];
+}
+__device__ __forceinline__ ConstI32* prefix(const BatchDev& b, int stage) {
+    return (ConstI32*)(b.prefix + stage * (b.n + 1));
+}
+
+__device__ __forceinline__ int find_pic(ConstI32* pre, int n, int it) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -88,7 +103,7 @@ __device__ __forceinline__ int find_pic(const int32_t* pre, int n, int it) {
 // ---------------------------------------------------------------------
 
 template <typename S>
-__device__ void intra_copy(const HevcPicDev& P, const rv_hevc_cu& cu, int tid) {
+__device__ void intra_copy(PicRef P, const rv_hevc_cu& cu, int tid) {
     int s = 1 << cu.log2_size;
     const uint16_t* pr = P.intra_pred + cu.pred_off;
     for (int c = 0; c < 3; ++c) {
@@ -104,13 +119,30 @@ __device__ void intra_copy(const HevcPicDev& P, const rv_hevc_cu& cu, int tid) {
 
 typedef short short2v __attribute__((ext_vector_type(2)));
 
+// Weighting of one component of one block: the explicit weights and
+// offsets of its references (Weighting::Explicit, src/hevc/inter.rs:71-84),
+// chosen per lane from the picture's (uniform) tables.
+struct Wp {
+    int w[2], o[2];
+};
+__device__ __forceinline__ int pick4(int k, int a, int b, int c, int d) { return k == 0 ? a : k == 1 ? b : k == 2 ? c : d; }
+__device__ __forceinline__ Wp weights(PicRef P, int c, const int (&ri)[2]) {
+    Wp r;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        r.w[l] = pick4(ri[l], P.wp_w[l][0][c], P.wp_w[l][1][c], P.wp_w[l][2][c], P.wp_w[l][3][c]);
+        r.o[l] = pick4(ri[l], P.wp_o[l][0][c], P.wp_o[l][1][c], P.wp_o[l][2][c], P.wp_o[l][3][c]);
+    }
+    return r;
+}
+
 // The combine of predict_block (src/hevc/inter.rs:712-741) with the
-// scalar combiners (src/dsp/hevc.rs:997-1087).
-__device__ __forceinline__ int combine(const HevcPicDev& P, int c, int dir, const int8_t* ref_idx, int a, int b,
-                                       int bd) {
+// scalar combiners (src/dsp/hevc.rs:997-1087): default uni / bi, or
+// explicit weights (lwd = log2_wd of the component).
+__device__ __forceinline__ int combine(bool weighted, int lwd, const Wp& wp, int dir, int a, int b, int bd) {
     const int max = (1 << bd) - 1;
     const bool both = dir == 3;
-    if (!P.weighted) {
+    if (!weighted) {
         if (!both) {
             int shift = 14 - bd;
             int v = dir == 1 ? a : b;
@@ -119,19 +151,14 @@ __device__ __forceinline__ int combine(const HevcPicDev& P, int c, int dir, cons
         int shift = 15 - bd;
         return clampi((a + b + (1 << (shift - 1))) >> shift, 0, max);
     }
-    const int lwd = P.log2_wd[c];
     if (!both) {
-        int l = dir == 1 ? 0 : 1;
-        int v = l == 0 ? a : b;
-        int wt = P.wp_w[l][ref_idx[l]][c], o = P.wp_o[l][ref_idx[l]][c];
+        const int l = dir == 1 ? 0 : 1;
+        const int v = l == 0 ? a : b, wt = wp.w[l], o = wp.o[l];
         if (lwd >= 1) return clampi(((v * wt + (1 << (lwd - 1))) >> lwd) + o, 0, max);
         return clampi(v * wt + o, 0, max);
     }
-    int w0 = P.wp_w[0][ref_idx[0]][c], w1 = P.wp_w[1][ref_idx[1]][c];
-    int o0 = P.wp_o[0][ref_idx[0]][c], o1 = P.wp_o[1][ref_idx[1]][c];
-    return clampi((a * w0 + b * w1 + ((o0 + o1 + 1) << lwd)) >> (lwd + 1), 0, max);
+    return clampi((a * wp.w[0] + b * wp.w[1] + ((wp.o[0] + wp.o[1] + 1) << lwd)) >> (lwd + 1), 0, max);
 }
-
 
 __device__ __forceinline__ short2v pk(int lo, int hi) {
     short2v r;
@@ -255,7 +282,7 @@ __device__ __forceinline__ void ref_row(const S* __restrict__ pl, int pitch, int
 
 // Plane c of reference ri of list l, chosen per lane among the picture's
 // (uniform) reference table.
-__device__ __forceinline__ const void* ref_plane(const HevcPicDev& P, int l, int ri, int c) {
+__device__ __forceinline__ const void* ref_plane(PicRef P, int l, int ri, int c) {
     const void* p0 = P.ref[l][0][c];
     const void* p1 = P.ref[l][1][c];
     const void* p2 = P.ref[l][2][c];
@@ -281,7 +308,8 @@ struct alignas(16) McLds {
 // rows 4 (s >> 3) ..+3), chroma lane (component s >> 3, x = s & 3, rows
 // 2 ((s >> 2) & 1) ..+1). Every lane reaches the barrier.
 template <typename S>
-__device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
+__device__ void mc_item(PicRef P, int first, int tid, McLds& L) {
+    first = __builtin_amdgcn_readfirstlane(first);
     const int slot = tid >> 4, s = tid & 15;
     // The wave's four records (uniform loads: the list is padded to a
     // multiple of four with empty records, dir 0).
@@ -343,7 +371,7 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
     }
     __syncthreads();
     // 2. Second (vertical) stage, combine, store.
-    const int8_t ref_idx[2] = {(int8_t)ri[0], (int8_t)ri[1]};
+    const bool weighted = P.weighted != 0;
     {
         const int x = s & 7, y0 = (s >> 3) * 4;
         int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -359,9 +387,12 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
             for (int i = 0; i < 4; ++i) pred[l][i] = (int16_t)(acc[i] >> 6);
         }
         if (dir && x < bw && y0 < bh) {
+            Wp wp = {{1, 1}, {0, 0}};
+            if (weighted) wp = weights(P, 0, ri);
+            const int lwd = P.log2_wd[0];
             S* dst = (S*)P.cur[0] + (size_t)(t.y + y0) * P.pitch[0] + t.x + x;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) dst[k * P.pitch[0]] = (S)combine(P, 0, dir, ref_idx, pred[0][k], pred[1][k], bd);
+            for (int k = 0; k < 4; ++k) dst[k * P.pitch[0]] = (S)combine(weighted, lwd, wp, dir, pred[0][k], pred[1][k], bd);
         }
     }
     // Chroma (4:2:0): eighth-sample vectors in chroma units
@@ -379,17 +410,23 @@ __device__ void mc_item(const HevcPicDev& P, int first, int tid, McLds& L) {
             pred[l][1] = (int16_t)(dot2(p2, tp.o[2], dot2(p1, tp.o[1], dot2(p0, tp.o[0], 0))) >> 6);
         }
         if (dir && x < (bw >> 1) && y0 < (bh >> 1)) {
-            const int pitch = P.pitch[1 + c];
-            S* dst = (S*)P.cur[1 + c] + (size_t)((t.y >> 1) + y0) * pitch + (t.x >> 1) + x;
+            Wp wp = {{1, 1}, {0, 0}};
+            if (weighted) {
+                const Wp cb = weights(P, 1, ri), cr = weights(P, 2, ri);
+                wp = c ? cr : cb;
+            }
+            const int lwd = c ? P.log2_wd[2] : P.log2_wd[1];
+            const int pitch = P.pitch[1];  // Cb and Cr share the geometry
+            S* dst = (S*)(c ? P.cur[2] : P.cur[1]) + (size_t)((t.y >> 1) + y0) * pitch + (t.x >> 1) + x;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) dst[k * pitch] = (S)combine(P, 1 + c, dir, ref_idx, pred[0][k], pred[1][k], bd);
+            for (int k = 0; k < 2; ++k) dst[k * pitch] = (S)combine(weighted, lwd, wp, dir, pred[0][k], pred[1][k], bd);
         }
     }
 }
 
 // Side data of one CU, PU or luma TU, one thread each
 // (src/hevc/ctu.rs:751-800 for CUs and PUs, :1493-1510 for TUs).
-__device__ void side_entry(const HevcPicDev& P, int k) {
+__device__ void side_entry(PicRef P, int k) {
     if (k < P.n_cus) {
         const rv_hevc_cu cu = P.cus[k];
         const int n4 = 1 << (cu.log2_size - 2), bx0 = cu.x >> 2, by0 = cu.y >> 2;
@@ -425,12 +462,14 @@ __device__ void side_entry(const HevcPicDev& P, int k) {
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage0(BatchDev b) {
     __shared__ McLds lds;
-    const int32_t* pre = b.prefix + 0 * (b.n + 1);
+    ConstI32* pre = prefix(b, 0);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
-    const int p = find_pic(pre, b.n, it);
-    const HevcPicDev& P = *b.hevc[p];
-    int i = it - pre[p];
+    // Uniform by construction; readfirstlane keeps the compiler's
+    // uniformity analysis (and scalar loads of the descriptor) on it.
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
+    PicRef P = picture(b, p);
+    int i = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const int tid = threadIdx.x;
     if (i < P.n_side) {
         side_entry(P, i * 64 + tid);
@@ -532,7 +571,7 @@ __device__ __forceinline__ void idst1d(const int (&s)[4], int (&o)[4]) {
 // 7 and clip to i16, rows, round by bdShift = 20 - BitDepth and clip — then
 // add_residual_scalar (:558-566).
 template <int N, typename S>
-__device__ void tu_group(const HevcPicDev& P, int first, int count, int tid, int16_t* lds) {
+__device__ void tu_group(PicRef P, int first, int count, int tid, int16_t* lds) {
     constexpr int TPW = 64 / N, LS = N + 1;  // TUs per wave, padded LDS row
     const int slot = tid / N, lane = tid % N;
     const bool active = slot < count;
@@ -590,7 +629,7 @@ __device__ void tu_group(const HevcPicDev& P, int first, int count, int tid, int
 }
 
 template <typename S>
-__device__ void tx_item(const HevcPicDev& P, int i, int tid, int16_t* lds) {
+__device__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
     int cls = 0;
     while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
     const int tpw = 16 >> cls;
@@ -611,7 +650,7 @@ struct MotionRef {
     int mv[2][2];
 };
 
-__device__ __forceinline__ MotionRef motion_of(const HevcPicDev& P, uint32_t pm) {
+__device__ __forceinline__ MotionRef motion_of(PicRef P, uint32_t pm) {
     MotionRef m;
     const uint32_t idx = pm & 0x3FFFFFFFu;
     if (idx >= (uint32_t)P.n_pus) {
@@ -623,9 +662,11 @@ __device__ __forceinline__ MotionRef motion_of(const HevcPicDev& P, uint32_t pm)
     const rv_hevc_pu pu = P.pus[idx];
     for (int l = 0; l < 2; ++l) {
         m.uses[l] = (pu.dir >> l) & 1;
-        int ri = m.uses[l] ? pu.ref_idx[l] : 0;
-        m.delta[l] = m.uses[l] ? P.ref_delta[l][ri] : 0;
-        m.lt[l] = m.uses[l] ? P.ref_lt[l][ri] : 0;
+        const int ri = m.uses[l] ? pu.ref_idx[l] : 0;
+        const int d = pick4(ri, P.ref_delta[l][0], P.ref_delta[l][1], P.ref_delta[l][2], P.ref_delta[l][3]);
+        const int lt = pick4(ri, P.ref_lt[l][0], P.ref_lt[l][1], P.ref_lt[l][2], P.ref_lt[l][3]);
+        m.delta[l] = m.uses[l] ? d : 0;
+        m.lt[l] = m.uses[l] ? lt : 0;
         m.mv[l][0] = pu.mv[l].x;
         m.mv[l][1] = pu.mv[l].y;
     }
@@ -662,7 +703,7 @@ __device__ int motion_bs(const MotionRef& a, const MotionRef& b) {
 
 // boundary_strengths (src/hevc/deblock.rs:62-180), one 4x4 block per
 // thread: bS of its left and top edges on the 8x8 grid; one slice, one tile.
-__device__ void block_bs(const HevcPicDev& P, int q) {
+__device__ void block_bs(PicRef P, int q) {
     const int w4 = P.w4;
     const int bx = q % w4, by = q / w4, x = bx * 4, y = by * 4;
     int out = 0;
@@ -694,12 +735,14 @@ __device__ void block_bs(const HevcPicDev& P, int q) {
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage1(BatchDev b) {
     __shared__ int16_t lds[2 * 32 * 33];
-    const int32_t* pre = b.prefix + 1 * (b.n + 1);
+    ConstI32* pre = prefix(b, 1);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
-    const int p = find_pic(pre, b.n, it);
-    const HevcPicDev& P = *b.hevc[p];
-    int i = it - pre[p];
+    // Uniform by construction; readfirstlane keeps the compiler's
+    // uniformity analysis (and scalar loads of the descriptor) on it.
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
+    PicRef P = picture(b, p);
+    int i = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const int tid = threadIdx.x;
     if (i < P.tx_items[4]) {
         if (P.bd == 8) tx_item<uint8_t>(P, i, tid, lds);
@@ -766,7 +809,7 @@ __device__ void luma_segment(int s[4][8], int beta, int tc, int max, bool& chang
 }
 
 // Luma beta / tc of one segment (src/hevc/deblock.rs:264-279).
-__device__ __forceinline__ void luma_thresholds(const HevcPicDev& P, int b, int qp_p, int qp_q, int* beta, int* tc) {
+__device__ __forceinline__ void luma_thresholds(PicRef P, int b, int qp_p, int qp_q, int* beta, int* tc) {
     int qp = (qp_p + qp_q + 1) >> 1, sh = P.bd - 8;
     *beta = (int)kBeta[clampi(qp + P.beta_offset, 0, 51)] << sh;
     *tc = (int)kTc[clampi(qp + 2 * (b - 1) + P.tc_offset, 0, 53)] << sh;
@@ -781,7 +824,7 @@ __device__ __forceinline__ int chroma_qp420(int qpi) {
 }
 
 // Chroma tc for Cb (c = 0) / Cr (c = 1) (src/hevc/deblock.rs:281-294).
-__device__ __forceinline__ int chroma_tc(const HevcPicDev& P, int qp_p, int qp_q, int c) {
+__device__ __forceinline__ int chroma_tc(PicRef P, int qp_p, int qp_q, int c) {
     int qpi = ((qp_p + qp_q + 1) >> 1) + (c == 0 ? P.cb_qp_offset : P.cr_qp_offset);
     int qpc = qpi < 0 ? qpi : chroma_qp420(qpi);
     return (int)kTc[clampi(qpc + 2 + P.tc_offset, 0, 53)] << (P.bd - 8);
@@ -804,7 +847,7 @@ __device__ void chroma_lines(S* d, long step, long along, int tc, int max, bool 
 }
 
 template <typename S>
-__device__ void deblock_v_seg(const HevcPicDev& P, int t) {
+__device__ void deblock_v_seg(PicRef P, int t) {
     const int nx = P.w >> 3;
     const int kx = t % nx, j = t / nx;
     if (kx == 0 || j >= P.h4) return;
@@ -847,7 +890,7 @@ __device__ void deblock_v_seg(const HevcPicDev& P, int t) {
 }
 
 template <typename S>
-__device__ void deblock_h_seg(const HevcPicDev& P, int t) {
+__device__ void deblock_h_seg(PicRef P, int t) {
     const int w4 = P.w4;
     const int i = t % w4, e = 1 + t / w4;
     const int y = e * 8;
@@ -890,11 +933,13 @@ __device__ void deblock_h_seg(const HevcPicDev& P, int t) {
 }
 
 __global__ void __launch_bounds__(256) hevc_stage2(BatchDev b) {
-    const int32_t* pre = b.prefix + 2 * (b.n + 1);
+    ConstI32* pre = prefix(b, 2);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
-    const int p = find_pic(pre, b.n, it);
-    const HevcPicDev& P = *b.hevc[p];
+    // Uniform by construction; readfirstlane keeps the compiler's
+    // uniformity analysis (and scalar loads of the descriptor) on it.
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
+    PicRef P = picture(b, p);
     if (!P.deblock) return;
     int t = (it - pre[p]) * 256 + threadIdx.x;
     if (t >= (P.w >> 3) * P.h4) return;
@@ -903,11 +948,13 @@ __global__ void __launch_bounds__(256) hevc_stage2(BatchDev b) {
 }
 
 __global__ void __launch_bounds__(256) hevc_stage3(BatchDev b) {
-    const int32_t* pre = b.prefix + 3 * (b.n + 1);
+    ConstI32* pre = prefix(b, 3);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
-    const int p = find_pic(pre, b.n, it);
-    const HevcPicDev& P = *b.hevc[p];
+    // Uniform by construction; readfirstlane keeps the compiler's
+    // uniformity analysis (and scalar loads of the descriptor) on it.
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
+    PicRef P = picture(b, p);
     if (!P.deblock) return;
     int t = (it - pre[p]) * 256 + threadIdx.x;
     if (t >= ((P.h >> 3) - 1) * P.w4 || P.h < 16) return;
@@ -927,7 +974,7 @@ __global__ void __launch_bounds__(256) hevc_stage3(BatchDev b) {
 // samples left and right of the group are loaded first; the class decides
 // which of them are the two neighbours (static register indices).
 template <typename S>
-__device__ void sao_group(const HevcPicDev& P, int c, int x0, int y, const rv_sao& sp, bool on) {
+__device__ void sao_group(PicRef P, int c, int x0, int y, const rv_sao& sp, bool on) {
     constexpr int G = 16 / sizeof(S);
     const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c];
     const S* src = (const S*)P.cur[c];
@@ -1008,7 +1055,7 @@ __device__ void sao_group(const HevcPicDev& P, int c, int x0, int y, const rv_sa
 
 // One sample, for CTBs narrower than a vector group.
 template <typename S>
-__device__ void sao_sample(const HevcPicDev& P, int c, int x, int y, const rv_sao& sp, bool on) {
+__device__ void sao_sample(PicRef P, int c, int x, int y, const rv_sao& sp, bool on) {
     const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c], max = (1 << P.bd) - 1;
     const S* src = (const S*)P.cur[c];
     const int vv = src[(size_t)y * pitch + x];
@@ -1033,7 +1080,7 @@ __device__ void sao_sample(const HevcPicDev& P, int c, int x, int y, const rv_sa
 // One CTB per workgroup (256 lanes), all three components: the parameters
 // are uniform across each wave, so the type and class branches are scalar.
 template <typename S>
-__device__ void sao_ctb(const HevcPicDev& P, int rx, int ry, int tid) {
+__device__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
     constexpr int G = 16 / sizeof(S);
     for (int c = 0; c < 3; ++c) {
         const int sh = c ? 1 : 0;
@@ -1055,11 +1102,13 @@ __device__ void sao_ctb(const HevcPicDev& P, int rx, int ry, int tid) {
 }
 
 __global__ void __launch_bounds__(256) hevc_stage4(BatchDev b) {
-    const int32_t* pre = b.prefix + 4 * (b.n + 1);
+    ConstI32* pre = prefix(b, 4);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
-    const int p = find_pic(pre, b.n, it);
-    const HevcPicDev& P = *b.hevc[p];
+    // Uniform by construction; readfirstlane keeps the compiler's
+    // uniformity analysis (and scalar loads of the descriptor) on it.
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
+    PicRef P = picture(b, p);
     const int i = it - pre[p];
     const int ry = i / P.wc, rx = i - ry * P.wc;
     if (P.bd == 8) sao_ctb<uint8_t>(P, rx, ry, threadIdx.x);
