@@ -263,13 +263,19 @@ typedef struct rv_session rv_session;
  * width x height (luma, 4:2:0) at bit_depth. */
 int rv_session_create(int codec, int width, int height, int bit_depth,
                       int dpb_slots, rv_session** out);
+/* Returns at once; the DPB's memory is reused once earlier work is done. */
 int rv_session_destroy(rv_session* s);
 /* Device bytes a session of this shape holds (DPB + scratch), for
  * admission by footprint. */
 int64_t rv_session_footprint(int codec, int width, int height, int bit_depth,
                              int dpb_slots);
 /* Host planes are tightly packed (row stride = plane width), 1 byte per
- * sample at 8 bits, 2 (little-endian uint16) above. */
+ * sample at 8 bits, 2 (little-endian uint16) above. Both calls return when
+ * the copy is complete. Each first waits for the latest launch that
+ * included a job of this session (an upload must not overwrite a slot
+ * that is still being read; a download must see the picture written
+ * there), and never for other sessions' work. Calls from several threads
+ * may overlap. */
 int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb,
                   const void* cr);
 int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr);
@@ -313,10 +319,12 @@ int rv_mark_wait(int id);
 /* Milliseconds between two markers (both complete). */
 int rv_mark_elapsed(int id_start, int id_end, float* ms);
 
-/* Deferred frees: with enable = 1, rv_job_free / rv_batch_free /
- * rv_session_destroy queue the object instead of synchronizing the compute
- * stream, and rv_sync() (after synchronizing) releases the queue. For loops
- * that keep launching while pictures complete. */
+/* Kept for callers of the earlier deferred-free switch; has no effect.
+ * rv_job_free, rv_batch_free and rv_session_destroy never wait for the GPU:
+ * device memory goes back to the library's pool once the work launched
+ * before the free has completed (a completion event), and later prepares
+ * and sessions reuse it instead of allocating. A batch launched from a
+ * captured graph is the exception: freeing it waits for the graph. */
 int rv_defer_free(int enable);
 
 /* Stage switches (mirror H26X_NO_DEBLOCK / H26X_NO_SAO,

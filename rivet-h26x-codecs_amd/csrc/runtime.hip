@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -25,10 +26,8 @@ std::mutex g_mu;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 hipStream_t g_io = nullptr;  // plane uploads / downloads
+hipStream_t g_up = nullptr;  // work-list and batch-table uploads
 int g_stage_mask = 0;
-bool g_defer = false;
-std::vector<void*> g_deferred;  // device allocations freed at the next rv_sync
-std::vector<void*> g_deferred_host;
 constexpr int kMarks = 4096;
 hipEvent_t g_marks[kMarks];
 int g_next_mark = 0;
@@ -72,6 +71,122 @@ struct PlaneGeom {
     int ss;
 };
 
+// ---- Device buffers recycled by completion ----------------------------
+// Job arenas, job scratch and batch tables come from one pool. A released
+// buffer goes back to the pool once an event recorded on the kernel stream
+// at release has completed, so a free never waits for the GPU and, once
+// the pool has warmed up, a prepare allocates nothing.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+std::mutex g_pool_mu;
+std::vector<DevBuf> g_pool;
+std::vector<std::pair<hipEvent_t, DevBuf>> g_pool_pending;
+std::vector<hipEvent_t> g_pool_events;
+
+// Completed releases back to the pool; the caller holds g_pool_mu.
+void pool_reap() {
+    for (size_t i = 0; i < g_pool_pending.size();) {
+        if (hipEventQuery(g_pool_pending[i].first) == hipSuccess) {
+            g_pool.push_back(g_pool_pending[i].second);
+            g_pool_events.push_back(g_pool_pending[i].first);
+            g_pool_pending[i] = g_pool_pending.back();
+            g_pool_pending.pop_back();
+        } else {
+            ++i;
+        }
+    }
+}
+
+// The smallest free buffer of at least `bytes`, else a new one with a
+// quarter of headroom.
+bool pool_get(size_t bytes, DevBuf* out) {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        pool_reap();
+        int best = -1;
+        for (int i = 0; i < (int)g_pool.size(); ++i)
+            if (g_pool[i].cap >= bytes && (best < 0 || g_pool[i].cap < g_pool[best].cap)) best = i;
+        if (best >= 0) {
+            *out = g_pool[best];
+            g_pool[best] = g_pool.back();
+            g_pool.pop_back();
+            return true;
+        }
+    }
+    const size_t cap = bytes + bytes / 4 + 4096;
+    void* p = nullptr;
+    if (hipMalloc(&p, cap) != hipSuccess) return false;
+    *out = DevBuf{p, cap};
+    return true;
+}
+
+// Release after everything launched so far on the kernel stream. Callers
+// hold g_mu, so the event never lands in another thread's graph capture.
+void pool_put(const DevBuf& b) {
+    if (!b.p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipEvent_t ev = nullptr;
+    if (!g_pool_events.empty()) {
+        ev = g_pool_events.back();
+        g_pool_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipStreamSynchronize(g_stream);
+        g_pool.push_back(b);
+        return;
+    }
+    (void)hipEventRecord(ev, g_stream);
+    g_pool_pending.emplace_back(ev, b);
+}
+
+// Pinned host staging of the calling thread (work lists and batch tables
+// are assembled here and copied with one DMA transfer).
+struct Staging {
+    char* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    char* get(size_t n) {
+        if (n > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            const size_t c = n + n / 4 + 4096;
+            if (hipHostMalloc((void**)&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+            cap = c;
+        }
+        return p;
+    }
+    ~Staging() {
+        if (p) (void)hipHostFree(p);
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+thread_local Staging t_stage;
+
+// Copy `bytes` of the staging buffer to `dst` on the upload stream, zero
+// `zero_bytes` at `zero`, and wait for both.
+hipError_t stage_upload(void* dst, size_t bytes, void* zero, size_t zero_bytes) {
+    hipError_t e = hipMemcpyAsync(dst, t_stage.p, bytes, hipMemcpyHostToDevice, g_up);
+    if (e == hipSuccess && zero_bytes) e = hipMemsetAsync(zero, 0, zero_bytes, g_up);
+    if (e == hipSuccess && !t_stage.ev) e = hipEventCreateWithFlags(&t_stage.ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(t_stage.ev, g_up);
+    if (e == hipSuccess) e = hipEventSynchronize(t_stage.ev);
+    return e;
+}
+
+// An event recorded after one launch, shared by the sessions it covered;
+// recycled when the last of them lets go.
+std::vector<hipEvent_t> g_launch_free;  // under g_pool_mu
+struct LaunchEv {
+    hipEvent_t e = nullptr;
+    ~LaunchEv() {
+        if (!e) return;
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        g_launch_free.push_back(e);
+    }
+};
+
 PlaneGeom plane_geom(int w, int h, int bd) {
     PlaneGeom g;
     g.ss = bd > 8 ? 2 : 1;
@@ -91,14 +206,19 @@ struct rv_session {
     int codec, w, h, bd, slots;
     PlaneGeom g;
     std::vector<void*> planes;  // slots * 3
+    std::vector<DevBuf> bufs;   // the planes' pool buffers
+    // Completion of the latest launch that included a job of this session
+    // (null before the first): plane copies wait for this, not for the
+    // other streams' work. Set and read under g_mu.
+    std::shared_ptr<LaunchEv> last;
 };
 
 struct rv_job {
     rv_session* s = nullptr;
     int codec = 0;
     int out_slot = 0;
-    void* arena = nullptr;   // work lists + descriptor
-    void* scratch = nullptr; // reconstruction planes + side maps
+    DevBuf arena;    // work lists + descriptor
+    DevBuf scratch;  // reconstruction planes + side maps
     rv::HevcPicDev hd{};
     rv::H264PicDev h4{};
     void* desc_dev = nullptr;  // device copy of the descriptor
@@ -112,7 +232,7 @@ struct rv_batch {
     std::vector<rv_job*> hevc, h264;
     rv::BatchDev bh{}, b4{};
     std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
-    void* tables = nullptr;
+    DevBuf tables;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -126,6 +246,33 @@ template <typename T> T* carve(char*& p, size_t n) {
 }
 template <typename T> size_t carve_size(size_t n) { return align_up(n * sizeof(T), 256); }
 
+// Record the end of a launch for the sessions of `jobs`; the caller holds
+// g_mu and has just enqueued the launch on g_stream.
+int note_launch(const std::vector<rv_job*>& jobs) {
+    auto ev = std::make_shared<LaunchEv>();
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_launch_free.empty()) {
+            ev->e = g_launch_free.back();
+            g_launch_free.pop_back();
+        }
+    }
+    if (!ev->e) RV_HIP(hipEventCreateWithFlags(&ev->e, hipEventDisableTiming));
+    RV_HIP(hipEventRecord(ev->e, g_stream));
+    for (rv_job* j : jobs) j->s->last = ev;
+    return RV_OK;
+}
+
+// Wait on the host for the latest launch that covered `s`.
+hipError_t wait_session(rv_session* s) {
+    std::shared_ptr<LaunchEv> ev;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        ev = s->last;
+    }
+    return ev ? hipEventSynchronize(ev->e) : hipSuccess;
+}
+
 int launch_codec(const rv::BatchDev& b, const std::vector<int32_t>& pre, int n, bool hevc, hipStream_t st) {
     for (int s = 0; s < rv::kStages; ++s) {
         int total = pre[(size_t)s * (n + 1) + n];
@@ -134,6 +281,16 @@ int launch_codec(const rv::BatchDev& b, const std::vector<int32_t>& pre, int n, 
     }
     return RV_OK;
 }
+
+// Return a job's buffers to the pool (after the work launched so far).
+void job_release(rv_job* j) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    pool_put(j->arena);
+    pool_put(j->scratch);
+    j->arena = DevBuf{};
+    j->scratch = DevBuf{};
+}
+
 
 // ---- HEVC job preparation -------------------------------------------
 int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
@@ -206,40 +363,6 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
     return RV_OK;
 }
 
-void hevc_bytes(const rv_hevc_pic* p, int64_t out[4]) {
-    const int64_t s = p->bit_depth > 8 ? 2 : 1;
-    int64_t mc = 0;
-    for (int i = 0; i < p->n_pus; ++i) {
-        const rv_hevc_pu& u = p->pus[i];
-        bool uni_default = u.dir != 3 && !p->weighted;
-        for (int l = 0; l < 2; ++l) {
-            if (!((u.dir >> l) & 1)) continue;
-            int mvx = u.mv[l].x, mvy = u.mv[l].y;
-            bool dl = uni_default && (mvx & 3) == 0 && (mvy & 3) == 0;
-            bool dc = uni_default && (mvx & 7) == 0 && (mvy & 7) == 0;
-            mc += dl ? (int64_t)u.w * u.h : (int64_t)(u.w + 7) * (u.h + 7);
-            mc += 2 * (dc ? (int64_t)(u.w / 2) * (u.h / 2) : (int64_t)(u.w / 2 + 3) * (u.h / 2 + 3));
-        }
-    }
-    int64_t planes = (int64_t)p->width * p->height * 3 / 2;
-    for (int i = 0; i < p->n_cus; ++i)
-        if (p->cus[i].intra) mc += (int64_t)(1 << p->cus[i].log2_size) * (1 << p->cus[i].log2_size) * 3 / 2 * 2;
-    out[0] = s * (mc + planes);
-    int64_t area = 0, nzc = 0;
-    for (int i = 0; i < p->n_tus; ++i)
-        if (p->tus[i].flags & RV_TU_CODED) {
-            area += (int64_t)1 << (2 * p->tus[i].log2);
-            nzc += p->tus[i].n_nz;
-        }
-    // Nonzero entries read once (4 bytes each) + the residual add's
-    // read-modify-write of the TU samples.
-    out[1] = 4 * nzc + 2 * s * area;
-    bool chroma = false;
-    for (int i = 0; i < p->n_cus && !chroma; ++i) chroma = p->cus[i].intra != 0;
-    int64_t luma = (int64_t)p->width * p->height;
-    out[2] = p->deblock ? 2 * (2 * s * (luma + (chroma ? luma / 2 : 0))) : 0;
-    out[3] = 2 * s * planes;
-}
 
 // implicit_pair (src/h264/recon.rs:214-233): weights at logWD 5 from POC
 // distances; (32, 32) for long-term references, equal POCs or a ratio out
@@ -377,6 +500,7 @@ int rv_init(int device_ordinal) {
             return fail(RV_E_NODEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
         RV_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
         RV_HIP(hipStreamCreateWithFlags(&g_io, hipStreamNonBlocking));
+        RV_HIP(hipStreamCreateWithFlags(&g_up, hipStreamNonBlocking));
         for (int i = 0; i < kMarks; ++i) RV_HIP(hipEventCreate(&g_marks[i]));
         const char* e = std::getenv("RIVET_HIP_NO_DEBLOCK");
         if (e && *e && *e != '0') g_stage_mask |= 1;
@@ -429,15 +553,27 @@ int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_s
         s->slots = dpb_slots;
         s->g = plane_geom(width, height, bit_depth);
         s->planes.assign((size_t)dpb_slots * 3, nullptr);
+        s->bufs.assign((size_t)dpb_slots * 3, DevBuf{});
         for (int i = 0; i < dpb_slots * 3; ++i) {
-            hipError_t e = hipMalloc(&s->planes[i], s->g.bytes[i % 3] + kPlaneSlack);
-            if (e != hipSuccess) {
-                for (void* p : s->planes)
-                    if (p) (void)hipFree(p);
+            if (!pool_get(s->g.bytes[i % 3] + kPlaneSlack, &s->bufs[i])) {
+                std::lock_guard<std::mutex> lk(g_mu);
+                for (const DevBuf& b : s->bufs) pool_put(b);
                 delete s;
                 return fail(RV_E_NOMEM, "DPB allocation failed");
             }
-            (void)hipMemset(s->planes[i], 0, s->g.bytes[i % 3]);
+            s->planes[i] = s->bufs[i].p;
+        }
+        // Zeroed DPB slots (a slot read before any picture is written there
+        // reads zeros), on the upload stream, waited for here.
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < dpb_slots * 3 && e == hipSuccess; ++i)
+            e = hipMemsetAsync(s->planes[i], 0, s->g.bytes[i % 3], g_up);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_up);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> lk(g_mu);
+            for (const DevBuf& b : s->bufs) pool_put(b);
+            delete s;
+            return hip_fail(e, "DPB clear");
         }
         *out = s;
         return RV_OK;
@@ -448,16 +584,10 @@ int rv_session_create(int codec, int width, int height, int bit_depth, int dpb_s
 
 int rv_session_destroy(rv_session* s) {
     if (!s) return fail(RV_E_INVALID, "null session");
-    if (g_defer) {
-        std::lock_guard<std::mutex> lk(g_mu);
-        for (void* p : s->planes)
-            if (p) g_deferred.push_back(p);
-        delete s;
-        return RV_OK;
-    }
-    if (g_stream) (void)hipStreamSynchronize(g_stream);
-    for (void* p : s->planes)
-        if (p) (void)hipFree(p);
+    // Never waits: the planes return to the pool once the work launched so
+    // far has completed (plane uploads and downloads are synchronous).
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (const DevBuf& b : s->bufs) pool_put(b);
     delete s;
     return RV_OK;
 }
@@ -466,19 +596,27 @@ int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb, const 
     if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
     if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
     if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
-    // Ordered after every kernel launched so far (a slot may be rewritten
-    // only once its readers have run), without waiting for later launches.
-    hipEvent_t ev;
-    RV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    RV_HIP(hipEventRecord(ev, g_stream));
-    RV_HIP(hipStreamWaitEvent(g_io, ev, 0));
+    // A slot may be rewritten only once its readers have run: wait for the
+    // latest launch that covered this session (none for a new session),
+    // never for other sessions' work. The planes go through the calling
+    // thread's pinned staging in one DMA transfer per plane.
+    RV_HIP(wait_session(s));
+    size_t row[3], off[3], total = 0;
+    for (int c = 0; c < 3; ++c) {
+        row[c] = (size_t)s->g.pw[c] * s->g.ss;
+        off[c] = total;
+        total += row[c] * s->g.ph[c];
+    }
+    char* st = t_stage.get(total);
+    if (!st) return fail(RV_E_NOMEM, "pinned staging allocation failed");
     const void* src[3] = {y, cb, cr};
+    for (int c = 0; c < 3; ++c) memcpy(st + off[c], src[c], row[c] * s->g.ph[c]);
     for (int c = 0; c < 3; ++c)
-        RV_HIP(hipMemcpy2DAsync(s->planes[(size_t)slot * 3 + c], (size_t)s->g.pitch[c] * s->g.ss, src[c],
-                                (size_t)s->g.pw[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
-                                hipMemcpyHostToDevice, g_io));
-    RV_HIP(hipStreamSynchronize(g_io));
-    (void)hipEventDestroy(ev);
+        RV_HIP(hipMemcpy2DAsync(s->planes[(size_t)slot * 3 + c], (size_t)s->g.pitch[c] * s->g.ss, st + off[c],
+                                row[c], row[c], s->g.ph[c], hipMemcpyHostToDevice, g_up));
+    if (!t_stage.ev) RV_HIP(hipEventCreateWithFlags(&t_stage.ev, hipEventDisableTiming));
+    RV_HIP(hipEventRecord(t_stage.ev, g_up));
+    RV_HIP(hipEventSynchronize(t_stage.ev));
     return RV_OK;
 }
 
@@ -486,17 +624,14 @@ int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr) {
     if (!s || !y || !cb || !cr) return fail(RV_E_INVALID, "null argument");
     if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
     if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
-    hipEvent_t ev;
-    RV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    RV_HIP(hipEventRecord(ev, g_stream));
-    RV_HIP(hipStreamWaitEvent(g_io, ev, 0));
+    // The slot's writer is at latest this session's last launch.
+    RV_HIP(wait_session(s));
     void* dst[3] = {y, cb, cr};
     for (int c = 0; c < 3; ++c)
         RV_HIP(hipMemcpy2DAsync(dst[c], (size_t)s->g.pw[c] * s->g.ss, s->planes[(size_t)slot * 3 + c],
                                 (size_t)s->g.pitch[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
                                 hipMemcpyDeviceToHost, g_io));
     RV_HIP(hipStreamSynchronize(g_io));
-    (void)hipEventDestroy(ev);
     return RV_OK;
 }
 
@@ -510,121 +645,163 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         int rc = validate_hevc(s, p);
         if (rc) return rc;
         const int W = p->width, H = p->height;
-        // Host-side lists: MC tiles, luma TUs, coded TUs (4x4 first).
-        std::vector<rv::McTile> tiles;
-        tiles.reserve((size_t)p->n_pus * 4);
+        // Pass 1: counts. Every list is then written once, straight into the
+        // pinned staging buffer, in the order the kernels read it:
+        //  - MC blocks of at most 8x8, in PU order, padded with empty blocks
+        //    to a multiple of four (a wave reads four records together);
+        //  - CUs, PUs and luma TUs by size, largest first (side data gives
+        //    each one thread that writes all its 4x4 cells, so a wave's
+        //    threads have equal work; picture order within a size; the PU
+        //    index stored per cell refers to this order, and the deblocking
+        //    filter reads motion through it);
+        //  - coded TUs by size class, smallest first, their nonzero
+        //    coefficients repacked in that order so a wave's TUs read one run
+        //    of memory.
+        size_t n_tiles = 0;
+        int64_t b_mc = 0;
+        const bool weighted = p->weighted != 0;
+        std::vector<int> pu_cnt(258, 0);
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
-            for (int dy = 0; dy < u.h; dy += 8)
-                for (int dx = 0; dx < u.w; dx += 8) {
-                    rv::McTile t;
-                    std::memset(&t, 0, sizeof(t));
-                    t.x = (uint16_t)(u.x + dx);
-                    t.y = (uint16_t)(u.y + dy);
-                    t.tw = (uint8_t)std::min(8, u.w - dx);
-                    t.th = (uint8_t)std::min(8, u.h - dy);
-                    t.dir = u.dir;
-                    t.refs = 0;
-                    for (int l = 0; l < 2; ++l) {
-                        if ((u.dir >> l) & 1) t.refs |= (uint8_t)(u.ref_idx[l] << (4 * l));
-                        t.mv[l] = u.mv[l];
-                    }
-                    tiles.push_back(t);
-                }
+            n_tiles += (size_t)((u.w + 7) >> 3) * ((u.h + 7) >> 3);
+            ++pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4)) + 1];
+            // Algorithmic bytes of motion compensation (the reference window
+            // per list; whole-sample default-weighted uni reads the block).
+            const bool uni_default = u.dir != 3 && !weighted;
+            for (int l = 0; l < 2; ++l) {
+                if (!((u.dir >> l) & 1)) continue;
+                const int mvx = u.mv[l].x, mvy = u.mv[l].y;
+                const bool dl = uni_default && (mvx & 3) == 0 && (mvy & 3) == 0;
+                const bool dc = uni_default && (mvx & 7) == 0 && (mvy & 7) == 0;
+                b_mc += dl ? (int64_t)u.w * u.h : (int64_t)(u.w + 7) * (u.h + 7);
+                b_mc += 2 * (dc ? (int64_t)(u.w / 2) * (u.h / 2) : (int64_t)(u.w / 2 + 3) * (u.h / 2 + 3));
+            }
         }
-        // A wave serves four blocks and reads their records together: pad
-        // with empty blocks (dir 0) to a multiple of four.
-        while (tiles.size() % 4) {
-            rv::McTile z;
-            std::memset(&z, 0, sizeof(z));
-            tiles.push_back(z);
+        const size_t n_tiles_pad = (n_tiles + 3) & ~(size_t)3;
+        int cu_cnt[5] = {0, 0, 0, 0, 0}, n_intra = 0;
+        int64_t intra_area = 0;
+        for (int i = 0; i < p->n_cus; ++i) {
+            const rv_hevc_cu& c = p->cus[i];
+            ++cu_cnt[(6 - c.log2_size) + 1];
+            if (c.intra) {
+                ++n_intra;
+                intra_area += (int64_t)1 << (2 * c.log2_size);
+            }
         }
-        std::vector<rv_hevc_tu> ltus, bysize[4];
+        int ltu_cnt[5] = {0, 0, 0, 0, 0}, cls_cnt[4] = {0, 0, 0, 0};
+        size_t cls_nz[4] = {0, 0, 0, 0}, n_ltus = 0, n_ctus = 0, n_nzs = 0;
+        int64_t coded_area = 0;
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
-            if (t.comp == 0) ltus.push_back(t);
-            if (t.flags & RV_TU_CODED) bysize[t.log2 - 2].push_back(t);
+            if (t.comp == 0) {
+                ++n_ltus;
+                ++ltu_cnt[(5 - t.log2) + 1];
+            }
+            if (t.flags & RV_TU_CODED) {
+                ++cls_cnt[t.log2 - 2];
+                cls_nz[t.log2 - 2] += t.n_nz;
+                ++n_ctus;
+                n_nzs += t.n_nz;
+                coded_area += (int64_t)1 << (2 * t.log2);
+            }
         }
-        // Coded TUs in kernel order (size classes, each in picture order),
-        // their coefficients repacked contiguously in that order so a wave's
-        // TUs read one run of memory.
-        std::vector<rv_hevc_tu> ctus;
-        for (auto& v : bysize) ctus.insert(ctus.end(), v.begin(), v.end());
-        std::vector<rv_nz> coefs;
-        coefs.reserve(p->n_nz);
-        for (rv_hevc_tu& t : ctus) {
-            const uint32_t off = (uint32_t)coefs.size();
-            coefs.insert(coefs.end(), p->nz + t.coef_off, p->nz + t.coef_off + t.n_nz);
-            t.coef_off = off;
-        }
-        // Side data (stage 0) gives one thread to each CU, PU and luma TU,
-        // which writes all of the entry's 4x4 cells: ordered by size, the
-        // 64 threads of a wave have equal work. Within a size, picture
-        // order is kept. The PU index stored per cell refers to this order,
-        // and the deblocking filter reads motion through it.
-        std::vector<rv_hevc_cu> cus(p->cus, p->cus + p->n_cus);
-        std::vector<rv_hevc_pu> pus(p->pus, p->pus + p->n_pus);
-        std::stable_sort(cus.begin(), cus.end(),
-                         [](const rv_hevc_cu& a, const rv_hevc_cu& b) { return a.log2_size > b.log2_size; });
-        std::stable_sort(pus.begin(), pus.end(),
-                         [](const rv_hevc_pu& a, const rv_hevc_pu& b) { return a.w * a.h > b.w * b.h; });
-        std::stable_sort(ltus.begin(), ltus.end(),
-                         [](const rv_hevc_tu& a, const rv_hevc_tu& b) { return a.log2 > b.log2; });
-        std::vector<int32_t> intra_idx;
-        for (int i = 0; i < p->n_cus; ++i)
-            if (cus[i].intra) intra_idx.push_back(i);
-        int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
-        size_t n_sao = (size_t)wc * hc * 3;
-        std::vector<rv_sao> sao(n_sao, rv_sao{0, 0, {0, 0, 0, 0}});
-        if (p->sao) std::memcpy(sao.data(), p->sao, n_sao * sizeof(rv_sao));
+        for (int k = 0; k < 257; ++k) pu_cnt[(size_t)k + 1] += pu_cnt[(size_t)k];
+        for (int k = 0; k < 4; ++k) cu_cnt[k + 1] += cu_cnt[k];
+        for (int k = 0; k < 4; ++k) ltu_cnt[k + 1] += ltu_cnt[k];
+        const int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
+        const size_t n_sao = (size_t)wc * hc * 3;
 
         rv_job* j = new rv_job();
         j->s = s;
         j->codec = RV_CODEC_HEVC;
         j->out_slot = out_slot;
-        hevc_bytes(p, j->bytes);
+        {
+            // Algorithmic bytes per kernel class (mc, tx, dbk, sao).
+            const int64_t ss = p->bit_depth > 8 ? 2 : 1, luma = (int64_t)W * H, planes = luma * 3 / 2;
+            j->bytes[0] = ss * (b_mc + planes + intra_area * 3 / 2 * 2);
+            j->bytes[1] = 4 * (int64_t)n_nzs + 2 * ss * coded_area;
+            j->bytes[2] = p->deblock ? 2 * (2 * ss * (luma + (n_intra ? luma / 2 : 0))) : 0;
+            j->bytes[3] = 2 * ss * planes;
+        }
         // Device arena.
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
-                    carve_size<rv_hevc_tu>(ltus.size()) + carve_size<rv_hevc_tu>(ctus.size()) +
-                    carve_size<rv::McTile>(tiles.size()) + carve_size<rv_nz>(coefs.size()) +
+                    carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
+                    carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv_nz>(n_nzs) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
-                    carve_size<int32_t>(intra_idx.size()) +
+                    carve_size<int32_t>(n_intra) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4);
         size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
                      carve_size<int8_t>(n4) + carve_size<uint32_t>(n4) + 1024;
-        if (hipMalloc(&j->arena, sz) != hipSuccess || hipMalloc(&j->scratch, scr) != hipSuccess) {
-            if (j->arena) (void)hipFree(j->arena);
+        char* hbase = t_stage.get(sz);
+        if (!hbase || !pool_get(sz, &j->arena) || !pool_get(scr, &j->scratch)) {
+            job_release(j);
             delete j;
             return fail(RV_E_NOMEM, "job allocation failed");
         }
-        std::vector<char> host(sz, 0);
-        char* hp = host.data();
-        char* base = (char*)j->arena;
-        auto dev = [&](char* hptr) { return base + (hptr - host.data()); };
+        char* hp = hbase;
+        char* base = (char*)j->arena.p;
+        auto dev = [&](char* hptr) { return base + (hptr - hbase); };
         rv::HevcPicDev* hdesc = carve<rv::HevcPicDev>(hp, 1);
         rv_hevc_pu* h_pus = carve<rv_hevc_pu>(hp, p->n_pus);
         rv_hevc_cu* h_cus = carve<rv_hevc_cu>(hp, p->n_cus);
-        rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, ltus.size());
-        rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, ctus.size());
-        rv::McTile* h_tiles = carve<rv::McTile>(hp, tiles.size());
-        rv_nz* h_coefs = carve<rv_nz>(hp, coefs.size());
+        rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, n_ltus);
+        rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, n_ctus);
+        rv::McTile* h_tiles = carve<rv::McTile>(hp, n_tiles_pad);
+        rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
-        int32_t* h_intra = carve<int32_t>(hp, intra_idx.size());
-        if (!intra_idx.empty()) std::memcpy(h_intra, intra_idx.data(), intra_idx.size() * sizeof(int32_t));
+        int32_t* h_intra = carve<int32_t>(hp, n_intra);
         const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
-        if (p->n_pus) std::memcpy(h_pus, pus.data(), p->n_pus * sizeof(rv_hevc_pu));
-        if (p->n_cus) std::memcpy(h_cus, cus.data(), p->n_cus * sizeof(rv_hevc_cu));
-        if (!ltus.empty()) std::memcpy(h_ltus, ltus.data(), ltus.size() * sizeof(rv_hevc_tu));
-        if (!ctus.empty()) std::memcpy(h_ctus, ctus.data(), ctus.size() * sizeof(rv_hevc_tu));
-        if (!tiles.empty()) std::memcpy(h_tiles, tiles.data(), tiles.size() * sizeof(rv::McTile));
-        if (!coefs.empty()) std::memcpy(h_coefs, coefs.data(), coefs.size() * sizeof(rv_nz));
+        // Pass 2: write.
+        size_t nt = 0;
+        for (int i = 0; i < p->n_pus; ++i) {
+            const rv_hevc_pu& u = p->pus[i];
+            h_pus[pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4))]++] = u;
+            uint8_t refs = 0;
+            for (int l = 0; l < 2; ++l)
+                if ((u.dir >> l) & 1) refs |= (uint8_t)(u.ref_idx[l] << (4 * l));
+            for (int dy = 0; dy < u.h; dy += 8)
+                for (int dx = 0; dx < u.w; dx += 8) {
+                    rv::McTile& t = h_tiles[nt++];
+                    t.x = (uint16_t)(u.x + dx);
+                    t.y = (uint16_t)(u.y + dy);
+                    t.tw = (uint8_t)std::min(8, u.w - dx);
+                    t.th = (uint8_t)std::min(8, u.h - dy);
+                    t.dir = u.dir;
+                    t.refs = refs;
+                    t.mv[0] = u.mv[0];
+                    t.mv[1] = u.mv[1];
+                }
+        }
+        if (n_tiles_pad > nt) std::memset(h_tiles + nt, 0, (n_tiles_pad - nt) * sizeof(rv::McTile));
+        for (int i = 0, ni = 0; i < p->n_cus; ++i) {
+            const rv_hevc_cu& c = p->cus[i];
+            const int at = cu_cnt[6 - c.log2_size]++;
+            h_cus[at] = c;
+            if (c.intra) h_intra[ni++] = at;
+        }
+        size_t cls_at[4], nz_at[4];
+        for (int k = 0, a = 0; k < 4; a += cls_cnt[k], ++k) cls_at[k] = (size_t)a;
+        for (int k = 0; k < 4; ++k) nz_at[k] = k ? nz_at[k - 1] + cls_nz[k - 1] : 0;
+        for (int i = 0; i < p->n_tus; ++i) {
+            const rv_hevc_tu& t = p->tus[i];
+            if (t.comp == 0) h_ltus[ltu_cnt[5 - t.log2]++] = t;
+            if (t.flags & RV_TU_CODED) {
+                const int k = t.log2 - 2;
+                rv_hevc_tu& c = h_ctus[cls_at[k]++];
+                c = t;
+                c.coef_off = (uint32_t)nz_at[k];
+                std::memcpy(h_coefs + nz_at[k], p->nz + t.coef_off, t.n_nz * sizeof(rv_nz));
+                nz_at[k] += t.n_nz;
+            }
+        }
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
-        std::memcpy(h_sao, sao.data(), n_sao * sizeof(rv_sao));
+        if (p->sao) std::memcpy(h_sao, p->sao, n_sao * sizeof(rv_sao));
+        else std::memset(h_sao, 0, n_sao * sizeof(rv_sao));
         // Scratch: reconstruction planes and side maps.
-        char* sp = (char*)j->scratch;
+        char* sp = (char*)j->scratch.p;
         rv::HevcPicDev& d = j->hd;
         std::memset(&d, 0, sizeof(d));
         d.w = W;
@@ -665,12 +842,12 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.sao_chroma = (g_stage_mask & 2) ? 0 : p->sao_chroma;
         d.n_cus = p->n_cus;
         d.n_pus = p->n_pus;
-        d.n_ltus = (int)ltus.size();
-        d.n_tiles = (int)tiles.size();
-        d.n_intra = (int)intra_idx.size();
+        d.n_ltus = (int)n_ltus;
+        d.n_tiles = (int)n_tiles_pad;
+        d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
         for (int c = 0, acc = 0; c < 4; ++c) {
-            d.n_tx[c] = (int)bysize[c].size();
+            d.n_tx[c] = cls_cnt[c];
             d.tx_first[c] = acc;
             acc += d.n_tx[c];
         }
@@ -703,11 +880,11 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
         j->single.h264 = nullptr;
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
-        hipError_t e = hipMemcpy(j->arena, host.data(), sz, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(j->scratch, 0, scr);
+        // The per-4x4 maps start zeroed (the planes are written in full).
+        const size_t maps = carve_size<uint8_t>(n4) * 3 + carve_size<int8_t>(n4) + carve_size<uint32_t>(n4);
+        hipError_t e = stage_upload(j->arena.p, sz, d.cuinfo, maps);
         if (e != hipSuccess) {
-            (void)hipFree(j->arena);
-            (void)hipFree(j->scratch);
+            job_release(j);
             delete j;
             return hip_fail(e, "job upload");
         }
@@ -743,15 +920,15 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4), nmb = (size_t)mbw * mbh;
         size_t scr = carve_size<uint32_t>(n4) + carve_size<uint32_t>(nmb * 4 + mbh + 1) + carve_size<uint32_t>(nmb * 4) + 1024;
-        if (hipMalloc(&j->arena, sz) != hipSuccess || hipMalloc(&j->scratch, scr) != hipSuccess) {
-            if (j->arena) (void)hipFree(j->arena);
+        char* hbase = t_stage.get(sz);
+        if (!hbase || !pool_get(sz, &j->arena) || !pool_get(scr, &j->scratch)) {
+            job_release(j);
             delete j;
             return fail(RV_E_NOMEM, "job allocation failed");
         }
-        std::vector<char> host(sz, 0);
-        char* hp = host.data();
-        char* base = (char*)j->arena;
-        auto dev = [&](char* hptr) { return base + (hptr - host.data()); };
+        char* hp = hbase;
+        char* base = (char*)j->arena.p;
+        auto dev = [&](char* hptr) { return base + (hptr - hbase); };
         rv::H264PicDev* hdesc = carve<rv::H264PicDev>(hp, 1);
         rv_h264_mb* h_mbs = carve<rv_h264_mb>(hp, p->n_mbs);
         rv_h264_part* h_parts = carve<rv_h264_part>(hp, p->n_parts);
@@ -813,7 +990,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.blks = (const rv_h264_blk*)dev((char*)h_blks);
         d.coefs = (const int32_t*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
-        char* sp = (char*)j->scratch;
+        char* sp = (char*)j->scratch.p;
         d.partmap = (uint32_t*)sp;
         sp += carve_size<uint32_t>(n4);
         d.bsv = (uint32_t*)sp;  // nmb * 4 strengths, then the wavefront counters
@@ -831,11 +1008,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->single.hevc = nullptr;
         j->single.h264 = (const rv::H264PicDev* const*)dev((char*)h_ptr);
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
-        hipError_t e = hipMemcpy(j->arena, host.data(), sz, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(j->scratch, 0, scr);
+        hipError_t e = stage_upload(j->arena.p, sz, j->scratch.p, scr);
         if (e != hipSuccess) {
-            (void)hipFree(j->arena);
-            (void)hipFree(j->scratch);
+            job_release(j);
             delete j;
             return hip_fail(e, "job upload");
         }
@@ -848,16 +1023,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
 
 int rv_job_free(rv_job* j) {
     if (!j) return fail(RV_E_INVALID, "null job");
-    if (g_defer) {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (j->arena) g_deferred.push_back(j->arena);
-        if (j->scratch) g_deferred.push_back(j->scratch);
-        delete j;
-        return RV_OK;
-    }
-    if (g_stream) (void)hipStreamSynchronize(g_stream);
-    if (j->arena) (void)hipFree(j->arena);
-    if (j->scratch) (void)hipFree(j->scratch);
+    // Never waits: the buffers return to the pool once the work launched
+    // so far (which includes every launch of this job) has completed.
+    job_release(j);
     delete j;
     return RV_OK;
 }
@@ -892,14 +1060,14 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
                     carve_size<int32_t>(b->pre_4.size());
-        std::vector<char> host(sz, 0);
-        if (hipMalloc(&b->tables, sz) != hipSuccess) {
+        char* hbase = t_stage.get(sz);
+        if (!hbase || !pool_get(sz, &b->tables)) {
             delete b;
             return fail(RV_E_NOMEM, "batch tables");
         }
-        char* hp = host.data();
-        char* base = (char*)b->tables;
-        auto dev = [&](char* h) { return base + (h - host.data()); };
+        char* hp = hbase;
+        char* base = (char*)b->tables.p;
+        auto dev = [&](char* h) { return base + (h - hbase); };
         void** ph = carve<void*>(hp, nh);
         void** p4 = carve<void*>(hp, n4);
         int32_t* preh = carve<int32_t>(hp, b->pre_h.size());
@@ -914,9 +1082,12 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);
-        hipError_t e = hipMemcpy(b->tables, host.data(), sz, hipMemcpyHostToDevice);
+        hipError_t e = stage_upload(b->tables.p, sz, nullptr, 0);
         if (e != hipSuccess) {
-            (void)hipFree(b->tables);
+            {
+                std::lock_guard<std::mutex> lk(g_mu);
+                pool_put(b->tables);
+            }
             delete b;
             return hip_fail(e, "batch upload");
         }
@@ -929,18 +1100,16 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
 
 int rv_batch_free(rv_batch* b) {
     if (!b) return fail(RV_E_INVALID, "null batch");
-    if (g_defer) {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (b->exec) (void)hipGraphExecDestroy(b->exec);
-        if (b->graph) (void)hipGraphDestroy(b->graph);
-        if (b->tables) g_deferred.push_back(b->tables);
-        delete b;
-        return RV_OK;
+    std::lock_guard<std::mutex> lk(g_mu);
+    // A captured graph is destroyed only once it has finished; batches
+    // launched without one free at once, their tables returning to the
+    // pool when the work launched so far has completed.
+    if (b->exec) {
+        (void)hipStreamSynchronize(g_stream);
+        (void)hipGraphExecDestroy(b->exec);
     }
-    if (g_stream) (void)hipStreamSynchronize(g_stream);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    if (b->tables) (void)hipFree(b->tables);
+    pool_put(b->tables);
     delete b;
     return RV_OK;
 }
@@ -974,9 +1143,12 @@ int rv_batch_launch(rv_batch* b, int mode) {
                 int rc = launch_codec(j->single, pre, 1, j->codec == RV_CODEC_HEVC, g_stream);
                 if (rc) return rc;
             }
-            return RV_OK;
+            return note_launch(b->jobs);
         }
-        if (mode == RV_LAUNCH_BATCHED) return launch_batched(b, g_stream);
+        if (mode == RV_LAUNCH_BATCHED) {
+            int rc = launch_batched(b, g_stream);
+            return rc ? rc : note_launch(b->jobs);
+        }
         if (mode == RV_LAUNCH_GRAPH) {
             if (!b->exec) {
                 RV_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
@@ -989,7 +1161,7 @@ int rv_batch_launch(rv_batch* b, int mode) {
                 RV_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
             }
             RV_HIP(hipGraphLaunch(b->exec, g_stream));
-            return RV_OK;
+            return note_launch(b->jobs);
         }
         return fail(RV_E_INVALID, "unknown launch mode");
     } catch (...) {
@@ -1029,7 +1201,8 @@ int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         for (int s = 0; s < rv::kStages; ++s) out_ms[s] = (float)(acc[s] / reps);
-        return rc;
+        const int rn = note_launch(b->jobs);
+        return rc ? rc : rn;
     } catch (...) {
         return fail(RV_E_HIP, "exception in rv_batch_profile");
     }
@@ -1071,18 +1244,13 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
 int rv_sync(void) {
     if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
     RV_HIP(hipStreamSynchronize(g_stream));
-    std::vector<void*> q;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        q.swap(g_deferred);
-    }
-    for (void* p : q) (void)hipFree(p);
     return RV_OK;
 }
 
 int rv_defer_free(int enable) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_defer = enable != 0;
+    // Frees never wait for the GPU (buffers return to the pool by
+    // completion event), so there is nothing left to defer.
+    (void)enable;
     return RV_OK;
 }
 

@@ -23,14 +23,17 @@ from __future__ import annotations
 
 import collections
 import concurrent.futures as cf_
+import heapq
+import itertools
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (RV_CODEC_HEVC, RV_LAUNCH_BATCHED, RV_LAUNCH_PLAIN, Batch, GenPicture, Job, Session, defer_free,
-               gen_planes, mark, mark_done, mark_wait, subseed, sync)
+from . import (RV_CODEC_HEVC, RV_LAUNCH_BATCHED, RV_LAUNCH_PLAIN, Batch, GenPicture, Job, Session, gen_planes, mark,
+               mark_done, mark_wait, subseed, sync)
 from . import configs as cf
 
 RESOLUTIONS = [(856, 480), (1280, 720), (1920, 1080), (3840, 2160)]  # 854x480 coded 856 wide
@@ -68,6 +71,7 @@ class Pic:
     luma: int = 0
     out: list | None = None
     fut: object = None
+    dependents: list = field(default_factory=list)  # pictures referencing this one
 
 
 @dataclass
@@ -83,6 +87,7 @@ class Sess:
     admitted_t: float = math.nan
     footprint: int = 0
     remaining: int = 0
+    iplanes: list | None = None  # the host-decoded I picture (input data)
 
 
 def session_cfgs(n: int, seed: int):
@@ -132,6 +137,10 @@ def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop
             p = Pic(s, poc, kind, refs, slot, tuple(slot_of[r] for r in refs))
             p.luma = cf.visible_luma(w, h)
             s.pics.append(p)
+        by_poc = {q.poc: q for q in s.pics}
+        for q in s.pics:
+            for r in q.refs:
+                by_poc[r].dependents.append(q)
         sessions.append(s)
 
     def gen(p: Pic):
@@ -145,6 +154,8 @@ def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop
     with cf_.ThreadPoolExecutor(max_workers=threads) as ex:
         for p, g in zip(todo, ex.map(gen, todo)):
             p.gen = g
+        for s, planes in zip(sessions, ex.map(i_planes, sessions)):
+            s.iplanes = planes
     return sessions
 
 
@@ -173,21 +184,22 @@ def run(sessions, mode: str = "batched", budget: int = 32 << 30, max_batch: int 
     Pic.out; use inflight_max=1 so no later picture has overwritten it).
     Work-list uploads run on `upload_threads` host threads from admission
     on, overlapping the reconstruction of earlier pictures."""
-    defer_free(True)
     ex = cf_.ThreadPoolExecutor(max_workers=upload_threads)
     try:
         return _run(sessions, mode, budget, max_batch, inflight_max, record, ex)
     finally:
         ex.shutdown(wait=True)
         sync()
-        defer_free(False)
 
 
 def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
+    """The loop is event-driven: finished uploads (posted by the upload
+    threads) and pictures made ready by a completed reference feed a heap
+    of launchable pictures ordered by ready time."""
     t0 = time.perf_counter()
     arrivals = collections.deque(sorted(sessions, key=lambda s: s.arrival))
     waiting = collections.deque()
-    active = []
+    active = {}  # id(session) -> session
     used = 0
     inflight = collections.deque()
     total = sum(len(s.pics) - 1 for s in sessions)
@@ -195,79 +207,97 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
     lat = []
     batches = 0
     batch_sizes = []
+    launchable = []  # heap of (ready_t, seq, pic): ready and uploaded
+    seq = itertools.count()
+    uploaded = collections.deque()  # appended by upload threads
+    wake = threading.Event()
     for s in sessions:
         s.footprint = footprint(s)
         s.remaining = len(s.pics) - 1
         for p in s.pics:
             p.launched = p.done = False
             p.ready_t = p.done_t = math.nan
+            p.job = p.fut = None
 
     def now():
         return time.perf_counter() - t0
 
+    def on_uploaded(p: Pic):
+        def cb(_fut):
+            uploaded.append(p)
+            wake.set()
+        return cb
+
+    def make_ready(p: Pic, t: float):
+        p.ready_t = t
+        if p.job is not None:
+            heapq.heappush(launchable, (t, next(seq), p))
+
     def complete(p: Pic, t: float):
-        nonlocal done
+        nonlocal done, used
         p.done = True
         p.done_t = t
+        s = p.sess
         if p.kind != "I":
             done += 1
             lat.append(p.done_t - p.ready_t)
             if record:
-                p.out = p.sess.session.download(p.slot)
+                p.out = s.session.download(p.slot)
             p.job.close()
             p.job = None
-            p.sess.remaining -= 1
-        s = p.sess
-        for q in s.pics:
-            if not q.done and not q.launched and math.isnan(q.ready_t) and q.kind != "I":
-                if all(_pic_by_poc(s, r).done for r in q.refs):
-                    q.ready_t = t
+            s.remaining -= 1
+        for q in p.dependents:
+            if math.isnan(q.ready_t) and all(_pic_by_poc(s, r).done for r in q.refs):
+                make_ready(q, t)
+        if s.remaining == 0 and id(s) in active:
+            s.session.close()
+            s.session = None
+            used -= s.footprint
+            del active[id(s)]
 
     def admit(s: Sess, t: float):
+        """Open the session and hand its uploads to the loader threads: the
+        host-decoded I picture first, then every P/B work list. The I
+        picture completes when its planes are resident."""
         nonlocal used
         s.session = Session(RV_CODEC_HEVC, s.width, s.height, s.bit_depth, DPB_SLOTS)
-        s.session.upload(ANCHOR_SLOTS[0], i_planes(s))
         for p in s.pics:
-            if p.gen is not None:
-                p.job = None
+            if p.gen is None:
+                p.fut = ex.submit(s.session.upload, ANCHOR_SLOTS[0], s.iplanes)
+            else:
                 p.fut = ex.submit(s.session.prepare, p.gen, p.slot)
+            p.fut.add_done_callback(on_uploaded(p))
         s.admitted_t = t
         used += s.footprint
-        active.append(s)
-        complete(s.pics[0], t)  # the I picture: host-supplied
+        active[id(s)] = s
 
     while done < total:
+        wake.clear()
         t = now()
         while arrivals and arrivals[0].arrival <= t:
             waiting.append(arrivals.popleft())
         while waiting and (used + waiting[0].footprint <= budget or not active):
             admit(waiting.popleft(), now())
-        # Completions.
         while inflight and mark_done(inflight[0][0]):
             _, pics, b = inflight.popleft()
             tc = now()
             b.close()
             for p in pics:
                 complete(p, tc)
-        for s in list(active):
-            if s.remaining == 0:
-                s.session.close()
-                s.session = None
-                used -= s.footprint
-                active.remove(s)
-        # Launch.
-        cand = [p for s in active for p in s.pics if not p.launched and not math.isnan(p.ready_t) and not p.done]
-        ready = []
-        for p in cand:
-            if p.job is None and p.fut is not None and p.fut.done():
-                p.job = p.fut.result()
-                p.fut = None
-            if p.job is not None:
-                ready.append(p)
-        if ready and len(inflight) < inflight_max:
-            ready.sort(key=lambda p: p.ready_t)
+        while uploaded:
+            p = uploaded.popleft()
+            r = p.fut.result()
+            p.fut = None
+            if p.gen is None:
+                p.ready_t = now()
+                complete(p, p.ready_t)
+                continue
+            p.job = r
+            if not math.isnan(p.ready_t) and not p.launched:
+                heapq.heappush(launchable, (p.ready_t, next(seq), p))
+        if launchable and len(inflight) < inflight_max:
+            group = [heapq.heappop(launchable)[2] for _ in range(min(max_batch, len(launchable)))]
             if mode == "batched":
-                group = ready[:max_batch]
                 b = Batch([p.job for p in group])
                 b.launch(RV_LAUNCH_BATCHED)
                 for p in group:
@@ -276,7 +306,7 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
                 batches += 1
                 batch_sizes.append(len(group))
             else:
-                for p in ready:
+                for p in group:
                     b = Batch([p.job])
                     b.launch(RV_LAUNCH_PLAIN)
                     p.launched = True
@@ -284,15 +314,14 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
                     batches += 1
                     batch_sizes.append(1)
             continue
-        pending = [p.fut for p in cand if p.fut is not None]
-        if inflight and not pending:
+        if len(inflight) >= inflight_max:
             mark_wait(inflight[0][0])
-        elif pending:
-            # Uploads still running: wake on the first one done (or soon, to
-            # look at completions and arrivals).
-            cf_.wait(pending, timeout=0.0005, return_when=cf_.FIRST_COMPLETED)
-        elif arrivals and not waiting:
+        elif inflight:
+            wake.wait(0.0002)  # an upload, or soon a completion to collect
+        elif arrivals and not waiting and not active:
             time.sleep(max(0.0, min(arrivals[0].arrival - now(), 0.01)))
+        else:
+            wake.wait(0.001 if not arrivals else max(0.0, min(arrivals[0].arrival - now(), 0.001)))
     wall = now()
     luma = sum(p.luma for s in sessions for p in s.pics if p.kind != "I")
     lat_ms = np.array(lat) * 1e3
@@ -325,15 +354,15 @@ def experiment(n: int = 64, seed: int = cf.BASE_SEED, budget: int = 16 << 30, lo
     sessions = build_sessions(n, seed, rate=0.0, threads=threads)
     log(f"[serve] generated {n} sessions ({sum(len(s.pics) - 1 for s in sessions)} P/B pictures) in "
         f"{time.perf_counter() - t:.1f} s")
-    sat = run(sessions, "batched", budget=budget)
+    sat = run(sessions, "batched", budget=budget, upload_threads=threads)
     capacity = n / sat["wall_s"]
     rate = load * capacity
     rng = np.random.default_rng(subseed(seed, 5, 0, 3) & 0xFFFFFFFF)
     arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
     for s, a in zip(sessions, arr):
         s.arrival = float(a)
-    batched = run(sessions, "batched", budget=budget)
-    plain = run(sessions, "plain", budget=budget)
+    batched = run(sessions, "batched", budget=budget, upload_threads=threads)
+    plain = run(sessions, "plain", budget=budget, upload_threads=threads)
     return {
         "sessions": n,
         "mix": "uniform over 854x480 (coded 856), 1280x720, 1920x1080, 3840x2160; 70% Main 8-bit / 30% Main10; "

@@ -1,0 +1,161 @@
+"""Serve-loop scheduling on the CPU: the loop runs against stand-in sessions,
+batches and completion markers that check its ordering rules — a picture
+is launched only after its references completed and after its session's
+I picture is resident, sessions are admitted within the HBM budget and
+closed once their last picture completes, and every picture completes."""
+import threading
+
+import pytest
+
+from rivet_hip import serve
+
+
+class World:
+    def __init__(self, polls_to_done=2):
+        self.lock = threading.Lock()
+        self.polls_to_done = polls_to_done
+        self.marks = {}      # id -> polls left
+        self.open = {}       # id(FakeSession) of the open sessions
+        self.closed = 0
+        self.launched = []
+        self.by_gen = {}     # id(gen) -> Pic
+        self.errors = []
+
+    def check(self, cond, msg):
+        if not cond:
+            self.errors.append(msg)
+
+
+def install(monkeypatch, world, sessions):
+    for s in sessions:
+        for p in s.pics:
+            if p.gen is not None:
+                world.by_gen[id(p.gen)] = p
+
+    class FakeJob:
+        def __init__(self, pic):
+            self.pic = pic
+            self.closed = False
+
+        def close(self):
+            world.check(not self.closed, "job closed twice")
+            self.closed = True
+
+    class FakeSession:
+        @staticmethod
+        def footprint(codec, w, h, bd, slots):
+            return w * h * 3 // 2 * slots
+
+        def __init__(self, codec, w, h, bd, slots):
+            self.resident = False
+            with world.lock:
+                world.open[id(self)] = 0
+
+        def upload(self, slot, planes):
+            world.check(slot == serve.ANCHOR_SLOTS[0], "I picture uploaded to another slot")
+            self.resident = True
+
+        def prepare(self, gen, slot):
+            p = world.by_gen[id(gen)]
+            world.check(slot == p.slot, "work list prepared for another slot")
+            return FakeJob(p)
+
+        def close(self):
+            with world.lock:
+                del world.open[id(self)]
+                world.closed += 1
+
+    class FakeBatch:
+        def __init__(self, jobs):
+            self.pics = [j.pic for j in jobs]
+
+        def launch(self, mode):
+            for p in self.pics:
+                world.check(p.sess.session.resident, f"session {p.sess.idx} launched before its I picture")
+                for r in p.refs:
+                    ref = next(q for q in p.sess.pics if q.poc == r)
+                    world.check(ref.done, f"poc {p.poc} launched before reference {r} completed")
+                world.check(not p.launched, f"poc {p.poc} launched twice")
+            world.launched.extend(self.pics)
+
+        def close(self):
+            pass
+
+    counter = iter(range(1, 1 << 30))
+
+    def mark():
+        i = next(counter)
+        world.marks[i] = world.polls_to_done
+        return i
+
+    def mark_done(i):
+        world.marks[i] -= 1
+        return world.marks[i] <= 0
+
+    def mark_wait(i):
+        world.marks[i] = 0
+
+    monkeypatch.setattr(serve, "Session", FakeSession)
+    monkeypatch.setattr(serve, "Batch", FakeBatch)
+    monkeypatch.setattr(serve, "mark", mark)
+    monkeypatch.setattr(serve, "mark_done", mark_done)
+    monkeypatch.setattr(serve, "mark_wait", mark_wait)
+    monkeypatch.setattr(serve, "sync", lambda: None)
+
+
+@pytest.fixture(scope="module")
+def sessions():
+    return serve.build_sessions(6, sizes=[(256, 128), (416, 240)], gop=9, threads=4)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("inflight", [1, 2])
+def test_every_picture_completes_in_dependency_order(monkeypatch, sessions, mode, inflight):
+    w = World()
+    install(monkeypatch, w, sessions)
+    st = serve.run(sessions, mode, inflight_max=inflight, upload_threads=3)
+    assert not w.errors, w.errors[:5]
+    total = sum(len(s.pics) - 1 for s in sessions)
+    assert st["pictures"] == total and len(w.launched) == total
+    assert w.closed == len(sessions) and not w.open
+    assert all(p.done for s in sessions for p in s.pics)
+    assert st["latency_ms_p50"] >= 0 and st["latency_ms_max"] >= st["latency_ms_p99"]
+    if mode == "plain":
+        assert st["mean_batch"] == 1.0
+
+
+def test_admission_respects_the_hbm_budget(monkeypatch, sessions):
+    w = World()
+    install(monkeypatch, w, sessions)
+    fps = [serve.footprint(s) for s in sessions]
+    budget = max(fps) + min(fps) - 1  # never room for the two largest together
+    seen = []
+    orig_init = serve.Session.__init__
+
+    def init(self, *a):
+        orig_init(self, *a)
+        seen.append(len(w.open))
+
+    monkeypatch.setattr(serve.Session, "__init__", init)
+    st = serve.run(sessions, "batched", budget=budget, upload_threads=2)
+    assert not w.errors, w.errors[:5]
+    assert st["pictures"] == sum(len(s.pics) - 1 for s in sessions)
+    # With this budget at most two sessions are ever open together.
+    assert max(seen) <= 2
+
+
+def test_a_session_larger_than_the_budget_runs_alone(monkeypatch, sessions):
+    w = World()
+    install(monkeypatch, w, sessions)
+    opened = []
+    orig_init = serve.Session.__init__
+
+    def init(self, *a):
+        orig_init(self, *a)
+        opened.append(len(w.open))
+
+    monkeypatch.setattr(serve.Session, "__init__", init)
+    st = serve.run(sessions, "batched", budget=1, upload_threads=2)
+    assert not w.errors, w.errors[:5]
+    assert st["pictures"] == sum(len(s.pics) - 1 for s in sessions)
+    assert max(opened) == 1  # admitted one at a time, each when nothing else is open
