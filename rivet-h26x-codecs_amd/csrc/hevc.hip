@@ -628,16 +628,93 @@ __device__ void tu_group(PicRef P, int first, int count, int tid, int16_t* lds) 
     }
 }
 
+// 4x4 TUs, one per lane (64 per wave), same arithmetic as tu_group<4>:
+// the lane scatters its nonzero list into its own 32 bytes of LDS (no other
+// lane touches them, so no barrier), reads the dense raster back as two
+// 16-byte loads, runs both passes in registers and adds the residual one
+// 4-sample row (one 4- or 8-byte access) at a time.
+template <typename S>
+__device__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) {
+    if (tid >= count) return;
+    const rv_hevc_tu tu = P.ctus[first + tid];
+    int16_t* c = lds + tid * 16;
+    uint4* c4 = (uint4*)c;
+    c4[0] = make_uint4(0, 0, 0, 0);
+    c4[1] = make_uint4(0, 0, 0, 0);
+    // rv_nz is {uint16 pos, int16 val}: one dword, 4-byte aligned.
+    const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
+    const int n = tu.n_nz;
+    for (int k = 0; k < n; k += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = k + i < n ? z[k + i] : 0u;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (k + i < n) c[v[i] & 15] = (int16_t)(v[i] >> 16);
+    }
+    const uint4 r0 = c4[0], r1 = c4[1];
+    const uint32_t w[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    int m[16];  // raster: m[y * 4 + x]
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        m[2 * j] = (int)(int16_t)(w[j] & 0xFFFF);
+        m[2 * j + 1] = (int)(int16_t)(w[j] >> 16);
+    }
+    const bool dst = tu.flags & RV_TU_DST;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        const int s_[4] = {m[x], m[4 + x], m[8 + x], m[12 + x]};
+        int o[4];
+        if (dst) idst1d(s_, o);
+        else idct1d<4>(s_, o);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) m[y * 4 + x] = clampi((o[y] + 64) >> 7, -32768, 32767);
+    }
+    const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1);
+    const int max = (1 << P.bd) - 1, pitch = P.pitch[tu.comp];
+    S* d = (S*)P.cur[tu.comp] + (size_t)tu.y * pitch + tu.x;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const int s_[4] = {m[y * 4], m[y * 4 + 1], m[y * 4 + 2], m[y * 4 + 3]};
+        int o[4];
+        if (dst) idst1d(s_, o);
+        else idct1d<4>(s_, o);
+        int r[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) r[x] = clampi((o[x] + round2) >> bd_shift, -32768, 32767);
+        S* row = d + (size_t)y * pitch;
+        if constexpr (sizeof(S) == 1) {
+            const uint32_t old = *(const uint32_t*)row;
+            uint32_t nw = 0;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) nw |= (uint32_t)clampi((int)((old >> (8 * x)) & 0xFF) + r[x], 0, max) << (8 * x);
+            *(uint32_t*)row = nw;
+        } else {
+            const uint2 old = *(const uint2*)row;
+            const uint32_t ow[2] = {old.x, old.y};
+            uint32_t nw[2] = {0, 0};
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+                nw[x >> 1] |= (uint32_t)clampi((int)((ow[x >> 1] >> (16 * (x & 1))) & 0xFFFF) + r[x], 0, max)
+                              << (16 * (x & 1));
+            *(uint2*)row = make_uint2(nw[0], nw[1]);
+        }
+    }
+}
+
+// Coded TUs per work item (one wave) of each size class.
+constexpr int kTusPerItem[4] = {64, 8, 4, 2};
+
 template <typename S>
 __device__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
     int cls = 0;
     while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
-    const int tpw = 16 >> cls;
+    const int tpw = cls == 0 ? 64 : 16 >> cls;
     const int k = i - P.tx_items[cls];
     const int first = P.tx_first[cls] + k * tpw;
     const int count = min(tpw, P.n_tx[cls] - k * tpw);
     switch (cls) {
-        case 0: tu_group<4, S>(P, first, count, tid, lds); break;
+        case 0: tu4_lane<S>(P, first, count, tid, lds); break;
         case 1: tu_group<8, S>(P, first, count, tid, lds); break;
         case 2: tu_group<16, S>(P, first, count, tid, lds); break;
         default: tu_group<32, S>(P, first, count, tid, lds); break;
@@ -649,29 +726,6 @@ struct MotionRef {
     int delta[2], lt[2];
     int mv[2][2];
 };
-
-__device__ __forceinline__ MotionRef motion_of(PicRef P, uint32_t pm) {
-    MotionRef m;
-    const uint32_t idx = pm & 0x3FFFFFFFu;
-    if (idx >= (uint32_t)P.n_pus) {
-        m.uses[0] = m.uses[1] = 0;
-        m.delta[0] = m.delta[1] = m.lt[0] = m.lt[1] = 0;
-        m.mv[0][0] = m.mv[0][1] = m.mv[1][0] = m.mv[1][1] = 0;
-        return m;
-    }
-    const rv_hevc_pu pu = P.pus[idx];
-    for (int l = 0; l < 2; ++l) {
-        m.uses[l] = (pu.dir >> l) & 1;
-        const int ri = m.uses[l] ? pu.ref_idx[l] : 0;
-        const int d = pick4(ri, P.ref_delta[l][0], P.ref_delta[l][1], P.ref_delta[l][2], P.ref_delta[l][3]);
-        const int lt = pick4(ri, P.ref_lt[l][0], P.ref_lt[l][1], P.ref_lt[l][2], P.ref_lt[l][3]);
-        m.delta[l] = m.uses[l] ? d : 0;
-        m.lt[l] = m.uses[l] ? lt : 0;
-        m.mv[l][0] = pu.mv[l].x;
-        m.mv[l][1] = pu.mv[l].y;
-    }
-    return m;
-}
 
 __device__ __forceinline__ bool mv_far(const int* p, const int* q) {
     return abs(p[0] - q[0]) >= 4 || abs(p[1] - q[1]) >= 4;
@@ -701,36 +755,120 @@ __device__ int motion_bs(const MotionRef& a, const MotionRef& b) {
     return c1 && c2;
 }
 
-// boundary_strengths (src/hevc/deblock.rs:62-180), one 4x4 block per
-// thread: bS of its left and top edges on the 8x8 grid; one slice, one tile.
-__device__ void block_bs(PicRef P, int q) {
-    const int w4 = P.w4;
-    const int bx = q % w4, by = q / w4, x = bx * 4, y = by * 4;
-    int out = 0;
-    if (P.deblock) {
-        const uint8_t ci = P.cuinfo[q], ti = P.tuinfo[q];
-        const uint32_t pm = P.pumap[q];
-        const bool intra_q = ci & 1;
-        const bool tu_l = ((ci >> 1) | (ti >> 1)) & 1, pu_l = (pm >> 30) & 1;
-        const bool tu_t = ((ci >> 2) | (ti >> 2)) & 1, pu_t = (pm >> 31) & 1;
-        if (x > 0 && (x & 7) == 0 && (tu_l || pu_l)) {
-            int p = q - 1;
-            int b;
-            if (intra_q || (P.cuinfo[p] & 1)) b = 2;
-            else if (tu_l && ((P.tuinfo[p] | ti) & 1)) b = 1;
-            else b = motion_bs(motion_of(P, P.pumap[p]), motion_of(P, pm));
-            out |= b;
-        }
-        if (y > 0 && (y & 7) == 0 && (tu_t || pu_t)) {
-            int p = q - w4;
-            int b;
-            if (intra_q || (P.cuinfo[p] & 1)) b = 2;
-            else if (tu_t && ((P.tuinfo[p] | ti) & 1)) b = 1;
-            else b = motion_bs(motion_of(P, P.pumap[p]), motion_of(P, pm));
-            out |= b << 2;
-        }
+// The fields of a PU that motion_bs reads, from its dwords 1..4 (dir,
+// ref_idx[2], mv[0], mv[1]); rv_hevc_pu is 20 bytes, so every record is
+// 4-byte aligned. No PU (index past the list): zeros, i.e. no list used.
+__device__ __forceinline__ uint4 pu_motion_words(PicRef P, uint32_t pm) {
+    const uint32_t idx = pm & 0x3FFFFFFFu;
+    if (idx >= (uint32_t)P.n_pus) return make_uint4(0, 0, 0, 0);
+    const uint32_t* w = (const uint32_t*)P.pus + (size_t)idx * 5 + 1;
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ MotionRef motion_from(PicRef P, uint4 w) {
+    MotionRef m;
+    const int dir = (w.x >> 16) & 0xFF;
+    for (int l = 0; l < 2; ++l) {
+        m.uses[l] = (dir >> l) & 1;
+        const int ri = m.uses[l] ? (int)(int8_t)((w.y >> (8 * l)) & 0xFF) : 0;
+        const int d = pick4(ri, P.ref_delta[l][0], P.ref_delta[l][1], P.ref_delta[l][2], P.ref_delta[l][3]);
+        const int lt = pick4(ri, P.ref_lt[l][0], P.ref_lt[l][1], P.ref_lt[l][2], P.ref_lt[l][3]);
+        m.delta[l] = m.uses[l] ? d : 0;
+        m.lt[l] = m.uses[l] ? lt : 0;
+        const uint32_t mv = l ? w.w : w.z;
+        m.mv[l][0] = (int)(int16_t)(mv & 0xFFFF);
+        m.mv[l][1] = (int)(int16_t)(mv >> 16);
     }
-    P.bs[q] = (uint8_t)out;
+    return m;
+}
+
+// One edge of boundary_strengths (src/hevc/deblock.rs:62-180): block q
+// with edge flags (tu_edge, pu_edge) against neighbour p. Returns -1 when
+// the motion comparison decides (both PUs inter, no intra or coded-TU rule
+// applies, different PUs); a PU against itself has bS 0 (motion_bs of
+// equal motion).
+__device__ __forceinline__ int edge_rule(bool on, bool tu_edge, bool pu_edge, int ci_p, int ci_q, int ti_p, int ti_q,
+                                         uint32_t pm_p, uint32_t pm_q) {
+    if (!on || !(tu_edge || pu_edge)) return 0;
+    if ((ci_q | ci_p) & 1) return 2;
+    if (tu_edge && ((ti_p | ti_q) & 1)) return 1;
+    if ((pm_p & 0x3FFFFFFFu) == (pm_q & 0x3FFFFFFFu)) return 0;
+    return -1;
+}
+
+// bS of one 8x8 area (blocks (bx, by), (bx+1, by), (bx, by+1), (bx+1,
+// by+1) of 4x4, bx and by even) per thread: the left edges of the two left
+// blocks and the top edges of the two top blocks lie on the 8x8 grid; the
+// other edges carry no bS. Map bytes of horizontally adjacent block pairs
+// are read together (w4 is even, so every pair starts at an even index);
+// the PU records of the edges that need a motion comparison are all
+// requested before any is used. Output byte per 4x4 block: bits 0..1 bS of
+// its left edge, bits 2..3 of its top edge.
+__device__ void area_bs(PicRef P, int a) {
+    const int w4 = P.w4, aw = w4 >> 1;
+    const int bx = (a % aw) * 2, by = (a / aw) * 2;
+    const size_t q = (size_t)by * w4 + bx;
+    uint32_t top_pair = 0, bottom_pair = 0;
+    if (P.deblock) {
+        const uint16_t* cu = (const uint16_t*)P.cuinfo;
+        const uint16_t* tu = (const uint16_t*)P.tuinfo;
+        const uint2* pmp = (const uint2*)P.pumap;
+        const bool has_l = bx > 0, has_t = by > 0;
+        const size_t h0 = q >> 1, h1 = (q + w4) >> 1;  // pair indices of rows by, by+1
+        const uint32_t c0 = cu[h0], c1 = cu[h1], t0 = tu[h0], t1 = tu[h1];
+        const uint2 m0 = pmp[h0], m1 = pmp[h1];
+        uint32_t cl0 = 0, cl1 = 0, tl0 = 0, tl1 = 0, ct = 0, tt = 0;
+        uint2 ml0 = make_uint2(0, 0), ml1 = ml0, mt = ml0;
+        if (has_l) {
+            cl0 = cu[h0 - 1];
+            cl1 = cu[h1 - 1];
+            tl0 = tu[h0 - 1];
+            tl1 = tu[h1 - 1];
+            ml0 = pmp[h0 - 1];
+            ml1 = pmp[h1 - 1];
+        }
+        if (has_t) {
+            const size_t ht = (q - w4) >> 1;
+            ct = cu[ht];
+            tt = tu[ht];
+            mt = pmp[ht];
+        }
+        // Edges: 0 = left of (bx, by), 1 = left of (bx, by+1), 2 = top of
+        // (bx, by), 3 = top of (bx+1, by).
+        int ci_q[4] = {(int)(c0 & 0xFF), (int)(c1 & 0xFF), (int)(c0 & 0xFF), (int)(c0 >> 8)};
+        int ti_q[4] = {(int)(t0 & 0xFF), (int)(t1 & 0xFF), (int)(t0 & 0xFF), (int)(t0 >> 8)};
+        uint32_t pm_q[4] = {m0.x, m1.x, m0.x, m0.y};
+        int ci_p[4] = {(int)(cl0 >> 8), (int)(cl1 >> 8), (int)(ct & 0xFF), (int)(ct >> 8)};
+        int ti_p[4] = {(int)(tl0 >> 8), (int)(tl1 >> 8), (int)(tt & 0xFF), (int)(tt >> 8)};
+        uint32_t pm_p[4] = {ml0.y, ml1.y, mt.x, mt.y};
+        int bs[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool vert = e < 2;
+            const int sh = vert ? 1 : 2;  // CU / TU edge-flag bit
+            const bool tu_edge = ((ci_q[e] >> sh) | (ti_q[e] >> sh)) & 1;
+            const bool pu_edge = (pm_q[e] >> (vert ? 30 : 31)) & 1;
+            bs[e] = edge_rule(vert ? has_l : has_t, tu_edge, pu_edge, ci_p[e], ci_q[e], ti_p[e], ti_q[e], pm_p[e],
+                              pm_q[e]);
+        }
+        uint4 wq[4], wp[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            wq[e] = wp[e] = make_uint4(0, 0, 0, 0);
+            if (bs[e] < 0) {
+                wq[e] = pu_motion_words(P, pm_q[e]);
+                wp[e] = pu_motion_words(P, pm_p[e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (bs[e] < 0) bs[e] = motion_bs(motion_from(P, wp[e]), motion_from(P, wq[e]));
+        top_pair = (uint32_t)(bs[0] | (bs[2] << 2)) | ((uint32_t)(bs[3] << 2) << 8);
+        bottom_pair = (uint32_t)bs[1];
+    }
+    uint16_t* out = (uint16_t*)P.bs;
+    out[q >> 1] = (uint16_t)top_pair;
+    out[(q + w4) >> 1] = (uint16_t)bottom_pair;
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage1(BatchDev b) {
@@ -750,8 +888,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) he
         return;
     }
     i -= P.tx_items[4];
-    int q = i * 64 + tid;
-    if (q < P.w4 * P.h4) block_bs(P, q);
+    const int a = i * 64 + tid;
+    if (a < (P.w4 >> 1) * (P.h4 >> 1)) area_bs(P, a);
 }
 
 // ---------------------------------------------------------------------
@@ -1371,10 +1509,10 @@ void hevc_stage_items(HevcPicDev& P) {
     P.items[0] = P.n_side + P.n_intra + (P.n_tiles + 3) / 4;
     P.tx_items[0] = 0;
     for (int c = 0; c < 4; ++c) {
-        int tpw = 16 >> c;
+        const int tpw = kTusPerItem[c];
         P.tx_items[c + 1] = P.tx_items[c] + (P.n_tx[c] + tpw - 1) / tpw;
     }
-    P.items[1] = P.tx_items[4] + (P.w4 * P.h4 + 63) / 64;
+    P.items[1] = P.tx_items[4] + ((P.w4 >> 1) * (P.h4 >> 1) + 63) / 64;
     P.items[2] = ((P.w >> 3) * P.h4 + 255) / 256;
     P.items[3] = (P.h >= 16) ? ((((P.h >> 3) - 1) * P.w4) + 255) / 256 : 0;
     P.items[4] = P.wc * P.hc;
