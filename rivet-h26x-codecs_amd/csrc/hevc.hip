@@ -566,72 +566,151 @@ __device__ __forceinline__ void idst1d(const int (&s)[4], int (&o)[4]) {
     o[3] = 84 * s[0] - 74 * s[1] + 55 * s[2] - 29 * s[3];
 }
 
-// One wave reconstructs 64 / N coded TUs of size N, N lanes each:
-// idct_scalar / idst4_scalar (src/dsp/hevc.rs:499-556) — columns, round by
-// 7 and clip to i16, rows, round by bdShift = 20 - BitDepth and clip — then
-// add_residual_scalar (:558-566).
-template <int N, typename S>
-__device__ void tu_group(PicRef P, int first, int count, int tid, int16_t* lds) {
-    constexpr int TPW = 64 / N, LS = N + 1;  // TUs per wave, padded LDS row
-    const int slot = tid / N, lane = tid % N;
-    const bool active = slot < count;
-    const rv_hevc_tu tu = P.ctus[first + (active ? slot : 0)];
-    int16_t* t = lds + slot * N * LS;
-    const bool dst = N == 4 && (tu.flags & RV_TU_DST);
-    (void)TPW;
-    // The dense raster in LDS: zeros, then the TU's nonzero list.
-    for (int e = lane; e < N * LS; e += N) t[e] = 0;
-    __syncthreads();
-    if (active)
-        for (int k = lane; k < tu.n_nz; k += N) {
-            const rv_nz z = P.nz[tu.coef_off + k];
-            t[(z.pos >> (31 - __builtin_clz(N))) * LS + (z.pos & (N - 1))] = z.val;
-        }
-    __syncthreads();
-    if (active) {
-        // Column `lane`, in place.
-        int s[N], o[N];
+// Coded TUs per work item (one wave) of each size class (4, 8, 16, 32):
+// 64 / TPW lanes per TU.
+__host__ __device__ constexpr int tus_per_item(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : cls == 2 ? 16 : 4; }
+
+// Add one row of N residuals to N samples at d (16 * k or 8 bytes,
+// aligned), clipping to [0, max] (add_residual_scalar, src/dsp/hevc.rs:558-566).
+template <typename S, int N>
+__device__ __forceinline__ void add_row(S* d, const int (&r)[N], int max) {
+    constexpr int BYTES = N * (int)sizeof(S), W = BYTES / 4, SPW = 4 / (int)sizeof(S);
+    constexpr uint32_t MASK = sizeof(S) == 1 ? 0xFFu : 0xFFFFu;
+    uint32_t w[W];
+    if constexpr (BYTES == 8) {
+        const uint2 u = *(const uint2*)d;
+        w[0] = u.x;
+        w[1] = u.y;
+    } else {
 #pragma unroll
-        for (int j = 0; j < N; ++j) s[j] = t[j * LS + lane];
-        if constexpr (N == 4) {
-            if (dst) idst1d(s, o);
-            else idct1d<4>(s, o);
-        } else {
-            idct1d<N>(s, o);
+        for (int v = 0; v < W / 4; ++v) {
+            const uint4 u = ((const uint4*)d)[v];
+            w[4 * v] = u.x;
+            w[4 * v + 1] = u.y;
+            w[4 * v + 2] = u.z;
+            w[4 * v + 3] = u.w;
         }
-#pragma unroll
-        for (int y = 0; y < N; ++y) t[y * LS + lane] = (int16_t)clampi((o[y] + 64) >> 7, -32768, 32767);
     }
-    __syncthreads();
-    const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1);
-    if (active) {
-        // Row `lane`, in place.
-        int s[N], o[N];
+    uint32_t o[W];
 #pragma unroll
-        for (int j = 0; j < N; ++j) s[j] = t[lane * LS + j];
-        if constexpr (N == 4) {
-            if (dst) idst1d(s, o);
-            else idct1d<4>(s, o);
-        } else {
-            idct1d<N>(s, o);
+    for (int k = 0; k < W; ++k) {
+        o[k] = 0;
+#pragma unroll
+        for (int e = 0; e < SPW; ++e) {
+            const int sh = e * 8 * (int)sizeof(S);
+            o[k] |= (uint32_t)clampi((int)((w[k] >> sh) & MASK) + r[k * SPW + e], 0, max) << sh;
         }
-#pragma unroll
-        for (int x = 0; x < N; ++x) t[lane * LS + x] = (int16_t)clampi((o[x] + round2) >> bd_shift, -32768, 32767);
     }
-    __syncthreads();
-    if (active) {
-        // Residual add: lane = column, coalesced along each row.
-        const int max = (1 << P.bd) - 1, pitch = P.pitch[tu.comp];
-        S* d = (S*)P.cur[tu.comp] + (size_t)tu.y * pitch + tu.x + lane;
-#pragma unroll 4
-        for (int y = 0; y < N; ++y) d[(size_t)y * pitch] = (S)clampi((int)d[(size_t)y * pitch] + t[y * LS + lane], 0, max);
+    if constexpr (BYTES == 8) {
+        *(uint2*)d = make_uint2(o[0], o[1]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < W / 4; ++v) ((uint4*)d)[v] = make_uint4(o[4 * v], o[4 * v + 1], o[4 * v + 2], o[4 * v + 3]);
     }
 }
 
-// 4x4 TUs, one per lane (64 per wave), same arithmetic as tu_group<4>:
-// the lane scatters its nonzero list into its own 32 bytes of LDS (no other
-// lane touches them, so no barrier), reads the dense raster back as two
-// 16-byte loads, runs both passes in registers and adds the residual one
+// One wave reconstructs 64 / L coded TUs of size N (N >= 8), L lanes per
+// TU: idct_scalar (src/dsp/hevc.rs:499-527) — columns, round by 7 and clip
+// to i16, rows, round by bdShift = 20 - BitDepth and clip — then the
+// residual add. The TU's dense raster lives in LDS (rows of N, TUs padded
+// apart so the column reads of one half-wave hit distinct banks): each
+// lane zeroes 1/L of it, scatters every L-th nonzero, transforms C = N / L
+// adjacent columns (one C-sample LDS read per row), then C whole rows
+// (16-byte LDS reads), adding each row to the picture with 16-byte
+// accesses.
+template <int N, int L, typename S>
+__device__ void tu_lanes(PicRef P, int first, int count, int tid, int16_t* lds) {
+    constexpr int C = N / L, PAD = N == 8 ? 8 : N == 16 ? 16 : 32, STRIDE = N * N + PAD;
+    constexpr int ZV = N * N * 2 / L / 16;  // 16-byte zero stores per lane
+    static_assert(C >= 2 && C <= 4 && ZV >= 1, "lane split");
+    const int slot = tid / L, sub = tid % L;
+    const bool active = slot < count;
+    const rv_hevc_tu tu = P.ctus[first + (active ? slot : 0)];
+    int16_t* t = lds + slot * STRIDE;
+    uint4* tz = (uint4*)t + sub * ZV;
+#pragma unroll
+    for (int i = 0; i < ZV; ++i) tz[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    if (active) {
+        const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
+        const int n = tu.n_nz;
+        for (int k = sub; k < n; k += 4 * L) {
+            uint32_t v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = k + i * L < n ? z[k + i * L] : 0u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (k + i * L < n) t[(v[i] & 0xFFFF) & (N * N - 1)] = (int16_t)(v[i] >> 16);
+        }
+    }
+    __syncthreads();
+    if (active) {
+        int col[C][N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const int16_t* src = t + j * N + sub * C;
+            if constexpr (C == 4) {
+                const uint2 u = *(const uint2*)src;
+                col[0][j] = (int)(int16_t)(u.x & 0xFFFF);
+                col[1][j] = (int)(int16_t)(u.x >> 16);
+                col[2][j] = (int)(int16_t)(u.y & 0xFFFF);
+                col[3][j] = (int)(int16_t)(u.y >> 16);
+            } else {
+                const uint32_t u = *(const uint32_t*)src;
+                col[0][j] = (int)(int16_t)(u & 0xFFFF);
+                col[1][j] = (int)(int16_t)(u >> 16);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            int o[N];
+            idct1d<N>(col[c], o);
+#pragma unroll
+            for (int y = 0; y < N; ++y) col[c][y] = clampi((o[y] + 64) >> 7, -32768, 32767);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            int16_t* dst = t + j * N + sub * C;
+            if constexpr (C == 4) {
+                *(uint2*)dst = make_uint2((uint32_t)(col[0][j] & 0xFFFF) | ((uint32_t)col[1][j] << 16),
+                                          (uint32_t)(col[2][j] & 0xFFFF) | ((uint32_t)col[3][j] << 16));
+            } else {
+                *(uint32_t*)dst = (uint32_t)(col[0][j] & 0xFFFF) | ((uint32_t)col[1][j] << 16);
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1);
+        const int max = (1 << P.bd) - 1, pitch = P.pitch[tu.comp];
+        S* d = (S*)P.cur[tu.comp] + (size_t)tu.y * pitch + tu.x;
+        for (int r = 0; r < C; ++r) {
+            const int y = sub * C + r;
+            const uint4* src = (const uint4*)(t + y * N);
+            int s_[N], o[N];
+#pragma unroll
+            for (int v = 0; v < N / 8; ++v) {
+                const uint4 u = src[v];
+                const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    s_[8 * v + 2 * e] = (int)(int16_t)(w[e] & 0xFFFF);
+                    s_[8 * v + 2 * e + 1] = (int)(int16_t)(w[e] >> 16);
+                }
+            }
+            idct1d<N>(s_, o);
+#pragma unroll
+            for (int x = 0; x < N; ++x) o[x] = clampi((o[x] + round2) >> bd_shift, -32768, 32767);
+            add_row<S, N>(d + (size_t)y * pitch, o, max);
+        }
+    }
+}
+
+// 4x4 TUs, one per lane (64 per wave): idct_scalar / idst4_scalar
+// (src/dsp/hevc.rs:499-556) and the residual add. The lane scatters its
+// nonzero list into its own 32 bytes of LDS (no other lane touches them,
+// so no barrier), reads the dense raster back as two 16-byte loads, runs
+// both passes in registers and adds the residual one
 // 4-sample row (one 4- or 8-byte access) at a time.
 template <typename S>
 __device__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) {
@@ -702,22 +781,19 @@ __device__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) 
     }
 }
 
-// Coded TUs per work item (one wave) of each size class.
-constexpr int kTusPerItem[4] = {64, 8, 4, 2};
-
 template <typename S>
 __device__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
     int cls = 0;
     while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
-    const int tpw = cls == 0 ? 64 : 16 >> cls;
+    const int tpw = tus_per_item(cls);
     const int k = i - P.tx_items[cls];
     const int first = P.tx_first[cls] + k * tpw;
     const int count = min(tpw, P.n_tx[cls] - k * tpw);
     switch (cls) {
         case 0: tu4_lane<S>(P, first, count, tid, lds); break;
-        case 1: tu_group<8, S>(P, first, count, tid, lds); break;
-        case 2: tu_group<16, S>(P, first, count, tid, lds); break;
-        default: tu_group<32, S>(P, first, count, tid, lds); break;
+        case 1: tu_lanes<8, 64 / tus_per_item(1), S>(P, first, count, tid, lds); break;
+        case 2: tu_lanes<16, 64 / tus_per_item(2), S>(P, first, count, tid, lds); break;
+        default: tu_lanes<32, 64 / tus_per_item(3), S>(P, first, count, tid, lds); break;
     }
 }
 
@@ -871,8 +947,9 @@ __device__ void area_bs(PicRef P, int a) {
     out[(q + w4) >> 1] = (uint16_t)bottom_pair;
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage1(BatchDev b) {
-    __shared__ int16_t lds[2 * 32 * 33];
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) hevc_stage1(BatchDev b) {
+    // The largest class layout: 16 TUs of 16x16, each padded by 16 entries.
+    __shared__ __align__(16) int16_t lds[tus_per_item(2) * (16 * 16 + 16)];
     ConstI32* pre = prefix(b, 1);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
@@ -1509,7 +1586,7 @@ void hevc_stage_items(HevcPicDev& P) {
     P.items[0] = P.n_side + P.n_intra + (P.n_tiles + 3) / 4;
     P.tx_items[0] = 0;
     for (int c = 0; c < 4; ++c) {
-        const int tpw = kTusPerItem[c];
+        const int tpw = tus_per_item(c);
         P.tx_items[c + 1] = P.tx_items[c] + (P.n_tx[c] + tpw - 1) / tpw;
     }
     P.items[1] = P.tx_items[4] + ((P.w4 >> 1) * (P.h4 >> 1) + 63) / 64;
