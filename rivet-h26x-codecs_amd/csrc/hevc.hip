@@ -224,60 +224,70 @@ __device__ __forceinline__ void load_pairs(const int16_t* src, short2v (&p)[6]) 
 }
 
 // N (8 or 12) samples of one reference row from column gx on, as N / 2
-// packed i16 pairs. The row index is clamped to the plane; when all N
-// columns are inside it they come from aligned dword loads and alignbyte,
-// otherwise from a gather with clamped columns. Both read the samples of
-// the reference's replicated border (src/hevc/frame.rs:169-214) and
-// clamped gather (src/hevc/inter.rs:116-123). The aligned loads may read
-// up to 4 bytes past the last sample of a row: planes carry that slack.
-// The clamped gather of ref_row, for spans that cross the plane's left or
-// right edge. Kept out of line: inlined, its per-column clamps are hoisted
-// above the branch and paid by every span.
-template <int N>
-struct Pairs {
-    uint32_t v[N / 2];
+// packed i16 pairs, in two steps so that a wave can issue the loads of all
+// its spans before it uses any: span_at() clamps the row to the plane and
+// places a window of N columns inside it (at gx when the span lies inside
+// the plane), read as aligned dwords; span_pairs() extracts the samples
+// with alignbyte or, for a span crossing the plane's left or right edge,
+// picks column clamp(gx + i, 0, pw - 1) of the window for each sample.
+// Both give the samples of the reference's replicated border
+// (src/hevc/frame.rs:169-214) and clamped gather (src/hevc/inter.rs:116-123).
+template <typename S, int N>
+struct Span {
+    static constexpr int ND = N * (int)sizeof(S) / 4 + 1;  // dwords covering the span
+    const S* row;       // the clamped row
+    const uint32_t* w;  // first dword
+    uint32_t sh;        // byte offset of the first sample in w[0]
+    int gx;
+    bool fast;          // all N columns inside the plane
 };
 template <typename S, int N>
-__device__ __attribute__((noinline)) Pairs<N> ref_row_clamped(const S* __restrict__ row, int pw, int gx) {
-    Pairs<N> r;
-#pragma unroll
-    for (int i = 0; i < N / 2; ++i) {
-        const int a = row[clampi(gx + 2 * i, 0, pw - 1)], b = row[clampi(gx + 2 * i + 1, 0, pw - 1)];
-        r.v[i] = (uint32_t)(a & 0xFFFF) | ((uint32_t)b << 16);
-    }
-    return r;
+__device__ __forceinline__ Span<S, N> span_at(const S* pl, int pitch, int pw, int ph, int gx, int gy) {
+    Span<S, N> sp;
+    sp.row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
+    sp.fast = gx >= 0 && gx + N - 1 < pw;
+    // In-plane start; reads may run past the row's last sample by less
+    // than N + 2 samples, inside the row pitch or the plane's slack.
+    const int cx = max(min(gx, pw - N), 0);
+    const uint32_t off = (uint32_t)cx * sizeof(S);
+    sp.w = (const uint32_t*)((const char*)sp.row + (off & ~3u));
+    sp.sh = off & 3u;
+    sp.gx = gx;
+    return sp;
 }
-
 template <typename S, int N>
-__device__ __forceinline__ void ref_row(const S* __restrict__ pl, int pitch, int pw, int ph, int gx, int gy,
-                                        short2v (&p)[N / 2]) {
-    // Rows start 256-byte aligned, so the span's dwords are at the row
-    // pointer plus its byte offset rounded down.
-    const S* row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
-    if (gx >= 0 && gx + N - 1 < pw) {
-        const uint32_t off = (uint32_t)gx * sizeof(S);
-        const uint32_t* w = (const uint32_t*)((const char*)row + (off & ~3u));
-        const uint32_t sh = off & 3u;
-        constexpr int ND = N * sizeof(S) / 4 + 1;  // dwords covering the span
-        uint32_t d[ND];
+__device__ __forceinline__ void span_pairs(const Span<S, N>& sp, const uint32_t (&d)[Span<S, N>::ND], int pw,
+                                           short2v (&p)[N / 2]) {
+    constexpr int ND = Span<S, N>::ND;
+    if (!sp.fast) {
+        // The window starts at column cx = max(min(gx, pw - N), 0), byte
+        // sp.sh of d[0]; every clamped column lies in [cx, cx + N).
+        const int cx = max(min(sp.gx, pw - N), 0);
+        uint32_t smp[N];
 #pragma unroll
-        for (int i = 0; i < ND; ++i) d[i] = w[i];
-        if constexpr (sizeof(S) == 1) {
+        for (int i = 0; i < N; ++i) {
+            const uint32_t b = (uint32_t)(clampi(sp.gx + i, 0, pw - 1) - cx) * sizeof(S) + sp.sh;
+            const uint32_t wi = b >> 2;
+            uint32_t v = d[0];
 #pragma unroll
-            for (int i = 0; i < N / 4; ++i) {
-                const uint32_t v = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
-                p[2 * i] = as2(__builtin_amdgcn_perm(0u, v, 0x0c010c00u));
-                p[2 * i + 1] = as2(__builtin_amdgcn_perm(0u, v, 0x0c030c02u));
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sh));
+            for (int k = 1; k < ND; ++k) v = wi == (uint32_t)k ? d[k] : v;
+            smp[i] = (v >> (8 * (b & 3))) & (sizeof(S) == 1 ? 0xFFu : 0xFFFFu);
         }
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) p[i] = as2(smp[2 * i] | (smp[2 * i + 1] << 16));
         return;
     }
-    const Pairs<N> v = ref_row_clamped<S, N>(row, pw, gx);
+    if constexpr (sizeof(S) == 1) {
 #pragma unroll
-    for (int i = 0; i < N / 2; ++i) p[i] = as2(v.v[i]);
+        for (int i = 0; i < N / 4; ++i) {
+            const uint32_t v = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sp.sh);
+            p[2 * i] = as2(__builtin_amdgcn_perm(0u, v, 0x0c010c00u));
+            p[2 * i + 1] = as2(__builtin_amdgcn_perm(0u, v, 0x0c030c02u));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sp.sh));
+    }
 }
 
 // Plane c of reference ri of list l, chosen per lane among the picture's
@@ -318,10 +328,10 @@ __device__ void mc_item(PicRef P, int first, int tid, McLds& L) {
     const uint4 q = slot == 0 ? q0 : slot == 1 ? q1 : slot == 2 ? q2 : q3;
     const McTile t = __builtin_bit_cast(McTile, q);
     // The packed taps, read per lane below: an LDS copy costs two coalesced
-    // loads per wave instead of one per lane and tap set.
+    // loads per wave instead of one per lane and tap set. Issued first,
+    // written to LDS once the reference loads are in flight.
     const uint32_t* kt = &kF.qpk[0][0];
-    for (int i = tid; i < kTapWords; i += 64) L.taps[i] = kt[i];
-    __syncthreads();
+    const uint32_t tap0 = kt[tid], tap1 = tid + 64 < kTapWords ? kt[tid + 64] : 0u;
     const int bw = t.tw, bh = t.th, dir = t.dir, bd = P.bd;
     const int shift1 = min(bd, 12) - 8;
     int ri[2], mvx[2], mvy[2];
@@ -331,42 +341,74 @@ __device__ void mc_item(PicRef P, int first, int tid, McLds& L) {
         mvx[l] = t.mv[l].x;
         mvy[l] = t.mv[l].y;
     }
-    // 1. First (horizontal) stage.
+    // 1. First (horizontal) stage. Every span of both lists is located and
+    // its loads issued before any is filtered: one memory round trip.
+    const int pw0 = P.pw[0], pw1 = P.pw[1];
+    const int nt = (bh + 7) * 2, ng = bw >> 2, nr = (bh >> 1) + 3;
+    const int cc = s >= nr ? 1 : 0, cr = s - nr * cc;  // chroma task: component, row
+    Span<S, 12> ls[2][2];
+    Span<S, 8> cs[2];
+    bool lact[2][2], cact[2];
+    uint32_t ld[2][2][Span<S, 12>::ND], cd[2][Span<S, 8>::ND];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const bool use = (dir >> l) & 1;
+        const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
+        const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = s + 16 * j, r = e >> 1, g = e & 1;
+            lact[l][j] = use && e < nt && g < ng;
+            ls[l][j] = span_at<S, 12>(ref, P.pitch[0], pw0, P.ph[0], x0 + 4 * g, y0 + r);
+        }
+        const S* cref = (const S*)(cc ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
+        cact[l] = use && s < 2 * nr;
+        cs[l] = span_at<S, 8>(cref, P.pitch[1], pw1, P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
+                              (t.y >> 1) + (mvy[l] >> 3) - 1 + cr);
+    }
+    // Left unset for inactive tasks (never read): a zero default would
+    // need a register copy of each loaded value, i.e. a wait.
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (lact[l][j]) {
+#pragma unroll
+                for (int i = 0; i < Span<S, 12>::ND; ++i) ld[l][j][i] = ls[l][j].w[i];
+            }
+        }
+        if (cact[l]) {
+#pragma unroll
+            for (int i = 0; i < Span<S, 8>::ND; ++i) cd[l][i] = cs[l].w[i];
+        }
+    }
+    L.taps[tid] = tap0;
+    if (tid + 64 < kTapWords) L.taps[tid + 64] = tap1;
+    __syncthreads();
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
-        {
-            const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
-            const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
-            const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
-            const int nt = (bh + 7) * 2, ng = bw >> 2;
+        const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 2; ++j) {
+            if (lact[l][j]) {
                 const int e = s + 16 * j, r = e >> 1, g = e & 1;
-                if (e < nt && g < ng) {
-                    short2v p[6];
-                    ref_row<S, 12>(ref, P.pitch[0], P.pw[0], P.ph[0], x0 + 4 * g, y0 + r, p);
-                    int acc[4];
-                    filt4(p, tp, acc);
+                short2v p[6];
+                span_pairs<S, 12>(ls[l][j], ld[l][j], pw0, p);
+                int acc[4];
+                filt4(p, tp, acc);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
-                }
+                for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
             }
         }
-        {
-            const int nr = (bh >> 1) + 3;
-            if (s < 2 * nr) {  // component c, row r of its support
-                const int c = s >= nr ? 1 : 0, r = s - nr * c;
-                const S* ref = (const S*)(c ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
-                const Taps4 tp = chroma_taps(L.taps, mvx[l] & 7);
-                short2v p[4];
-                ref_row<S, 8>(ref, P.pitch[1], P.pw[1], P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
-                              (t.y >> 1) + (mvy[l] >> 3) - 1 + r, p);
-                int acc[4];
-                filt4c(p, tp, acc);
+        if (cact[l]) {
+            const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
+            short2v p[4];
+            span_pairs<S, 8>(cs[l], cd[l], pw1, p);
+            int acc[4];
+            filt4c(p, tp4, acc);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) L.ctmp[l][slot][c][i * 8 + r] = (int16_t)(acc[i] >> shift1);
-            }
+            for (int i = 0; i < 4; ++i) L.ctmp[l][slot][cc][i * 8 + cr] = (int16_t)(acc[i] >> shift1);
         }
     }
     __syncthreads();
@@ -424,6 +466,159 @@ __device__ void mc_item(PicRef P, int first, int tid, McLds& L) {
     }
 }
 
+// LDS of one 16x16 tile: first-stage outputs of its shared window,
+// transposed: luma [list][column 0..15][row 0..22, one pad], chroma
+// [list][component][column 0..7][row 0..10, one pad].
+struct alignas(16) McLds16 {
+    uint32_t taps[kTapWords];
+    int16_t lt[2][16 * 24];
+    int16_t ct[2][2][8 * 12];
+};
+
+typedef const uint4 __attribute__((address_space(4))) ConstU4;
+
+// Motion compensation of one 16x16 tile of a PU (predict_block,
+// src/hevc/inter.rs:555-742, with interp_scalar and the combiners,
+// src/dsp/hevc.rs:823-1087), the same arithmetic as mc_item over one
+// window: luma first stage 23 rows x 4 groups of 4 columns (task e = tid,
+// tid + 64), chroma 2 components x 11 rows x 2 groups (task e = tid); the
+// second stage gives lane (x = tid & 15, rows 4 (tid >> 4) ..+3) of luma
+// and lane (component tid >> 5, x = tid & 7, rows 2 ((tid >> 3) & 3) ..+1)
+// of chroma. Every lane reaches the barriers.
+template <typename S>
+__device__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) {
+    k = __builtin_amdgcn_readfirstlane(k);
+    const McTile t = __builtin_bit_cast(McTile, *(ConstU4*)(P.tiles16 + k));
+    const uint32_t* kt = &kF.qpk[0][0];
+    const uint32_t tap0 = kt[tid], tap1 = tid + 64 < kTapWords ? kt[tid + 64] : 0u;
+    const int dir = t.dir, bd = P.bd;
+    const int shift1 = min(bd, 12) - 8;
+    int ri[2], mvx[2], mvy[2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        ri[l] = (t.refs >> (4 * l)) & 15;
+        mvx[l] = t.mv[l].x;
+        mvy[l] = t.mv[l].y;
+    }
+    const int pw0 = P.pw[0], pw1 = P.pw[1];
+    const int cc = tid >= 22 ? 1 : 0, crem = tid - 22 * cc, cr = crem >> 1, cg = crem & 1;
+    Span<S, 12> ls[2][2];
+    Span<S, 8> cs[2];
+    bool lact[2][2], cact[2];
+    uint32_t ld[2][2][Span<S, 12>::ND], cd[2][Span<S, 8>::ND];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const bool use = (dir >> l) & 1;
+        const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
+        const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = tid + 64 * j, r = e >> 2, g = e & 3;
+            lact[l][j] = use && e < 92;
+            ls[l][j] = span_at<S, 12>(ref, P.pitch[0], pw0, P.ph[0], x0 + 4 * g, y0 + r);
+        }
+        const S* cref = (const S*)(cc ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
+        cact[l] = use && tid < 44;
+        cs[l] = span_at<S, 8>(cref, P.pitch[1], pw1, P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1 + 4 * cg,
+                              (t.y >> 1) + (mvy[l] >> 3) - 1 + cr);
+    }
+    // All loads issued before any is used; unset for inactive tasks.
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (lact[l][j]) {
+#pragma unroll
+                for (int i = 0; i < Span<S, 12>::ND; ++i) ld[l][j][i] = ls[l][j].w[i];
+            }
+        }
+        if (cact[l]) {
+#pragma unroll
+            for (int i = 0; i < Span<S, 8>::ND; ++i) cd[l][i] = cs[l].w[i];
+        }
+    }
+    L.taps[tid] = tap0;
+    if (tid + 64 < kTapWords) L.taps[tid + 64] = tap1;
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+        const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (lact[l][j]) {
+                const int e = tid + 64 * j, r = e >> 2, g = e & 3;
+                short2v p[6];
+                span_pairs<S, 12>(ls[l][j], ld[l][j], pw0, p);
+                int acc[4];
+                filt4(p, tp, acc);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) L.lt[l][(4 * g + i) * 24 + r] = (int16_t)(acc[i] >> shift1);
+            }
+        }
+        if (cact[l]) {
+            const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
+            short2v p[4];
+            span_pairs<S, 8>(cs[l], cd[l], pw1, p);
+            int acc[4];
+            filt4c(p, tp4, acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) L.ct[l][cc][(4 * cg + i) * 12 + cr] = (int16_t)(acc[i] >> shift1);
+        }
+    }
+    __syncthreads();
+    const bool weighted = P.weighted != 0;
+    {
+        const int x = tid & 15, y0 = (tid >> 4) * 4;
+        int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+            if (!((dir >> l) & 1)) continue;
+            const Taps8 tp = luma_taps(L.taps, mvy[l] & 3);
+            short2v p[6];
+            load_pairs(L.lt[l] + x * 24 + y0, p);
+            int acc[4];
+            filt4(p, tp, acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) pred[l][i] = (int16_t)(acc[i] >> 6);
+        }
+        if (dir) {
+            Wp wp = {{1, 1}, {0, 0}};
+            if (weighted) wp = weights(P, 0, ri);
+            const int lwd = P.log2_wd[0];
+            S* dst = (S*)P.cur[0] + (size_t)(t.y + y0) * P.pitch[0] + t.x + x;
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2)
+                dst[k2 * P.pitch[0]] = (S)combine(weighted, lwd, wp, dir, pred[0][k2], pred[1][k2], bd);
+        }
+    }
+    {
+        const int c = tid >> 5, x = tid & 7, y0 = ((tid >> 3) & 3) * 2;
+        int pred[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+            if (!((dir >> l) & 1)) continue;
+            const Taps4 tp = chroma_taps(L.taps, mvy[l] & 7);
+            const uint32_t* T = (const uint32_t*)(L.ct[l][c] + x * 12 + y0);
+            const short2v p0 = as2(T[0]), p1 = as2(T[1]), p2 = as2(T[2]);
+            pred[l][0] = (int16_t)(dot2(p1, tp.e[1], dot2(p0, tp.e[0], 0)) >> 6);
+            pred[l][1] = (int16_t)(dot2(p2, tp.o[2], dot2(p1, tp.o[1], dot2(p0, tp.o[0], 0))) >> 6);
+        }
+        if (dir) {
+            Wp wp = {{1, 1}, {0, 0}};
+            if (weighted) {
+                const Wp cb = weights(P, 1, ri), cr2 = weights(P, 2, ri);
+                wp = c ? cr2 : cb;
+            }
+            const int lwd = c ? P.log2_wd[2] : P.log2_wd[1];
+            const int pitch = P.pitch[1];
+            S* dst = (S*)(c ? P.cur[2] : P.cur[1]) + (size_t)((t.y >> 1) + y0) * pitch + (t.x >> 1) + x;
+#pragma unroll
+            for (int k2 = 0; k2 < 2; ++k2) dst[k2 * pitch] = (S)combine(weighted, lwd, wp, dir, pred[0][k2], pred[1][k2], bd);
+        }
+    }
+}
+
 // Side data of one CU, PU or luma TU, one thread each
 // (src/hevc/ctu.rs:751-800 for CUs and PUs, :1493-1510 for TUs).
 __device__ void side_entry(PicRef P, int k) {
@@ -460,8 +655,11 @@ __device__ void side_entry(PicRef P, int k) {
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) hevc_stage0(BatchDev b) {
-    __shared__ McLds lds;
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) hevc_stage0(BatchDev b) {
+    __shared__ union {
+        McLds blocks;
+        McLds16 tile;
+    } lds;
     ConstI32* pre = prefix(b, 0);
     const int it = xcd_item();
     if (it >= pre[b.n]) return;
@@ -483,8 +681,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) he
         return;
     }
     i -= P.n_intra;
-    if (P.bd == 8) mc_item<uint8_t>(P, 4 * i, tid, lds);
-    else mc_item<uint16_t>(P, 4 * i, tid, lds);
+    if (i < P.n_tiles16) {
+        if (P.bd == 8) mc_tile16<uint8_t>(P, i, tid, lds.tile);
+        else mc_tile16<uint16_t>(P, i, tid, lds.tile);
+        return;
+    }
+    i -= P.n_tiles16;
+    if (P.bd == 8) mc_item<uint8_t>(P, 4 * i, tid, lds.blocks);
+    else mc_item<uint16_t>(P, 4 * i, tid, lds.blocks);
 }
 
 // ---------------------------------------------------------------------
@@ -1583,7 +1787,7 @@ hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStrea
 // Work items per stage for one picture (the grid unit of each stage).
 void hevc_stage_items(HevcPicDev& P) {
     P.n_side = (P.n_cus + P.n_pus + P.n_ltus + 63) / 64;
-    P.items[0] = P.n_side + P.n_intra + (P.n_tiles + 3) / 4;
+    P.items[0] = P.n_side + P.n_intra + P.n_tiles16 + (P.n_tiles + 3) / 4;
     P.tx_items[0] = 0;
     for (int c = 0; c < 4; ++c) {
         const int tpw = tus_per_item(c);

@@ -657,13 +657,17 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         //  - coded TUs by size class, smallest first, their nonzero
         //    coefficients repacked in that order so a wave's TUs read one run
         //    of memory.
-        size_t n_tiles = 0;
+        size_t n_tiles = 0, n_tiles16 = 0;
         int64_t b_mc = 0;
         const bool weighted = p->weighted != 0;
         std::vector<int> pu_cnt(258, 0);
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
-            n_tiles += (size_t)((u.w + 7) >> 3) * ((u.h + 7) >> 3);
+            // The 16-aligned core in 16x16 tiles; the right and bottom
+            // strips in blocks of at most 8x8.
+            const int w16 = u.w & ~15, h16 = u.h & ~15;
+            n_tiles16 += (size_t)(w16 >> 4) * (h16 >> 4);
+            n_tiles += (size_t)((u.w - w16 + 7) >> 3) * ((u.h + 7) >> 3) + (size_t)(w16 >> 3) * ((u.h - h16 + 7) >> 3);
             ++pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4)) + 1];
             // Algorithmic bytes of motion compensation (the reference window
             // per list; whole-sample default-weighted uni reads the block).
@@ -726,7 +730,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         // Device arena.
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
                     carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
-                    carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv_nz>(n_nzs) +
+                    carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
+                    carve_size<rv_nz>(n_nzs) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -748,6 +753,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, n_ltus);
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, n_ctus);
         rv::McTile* h_tiles = carve<rv::McTile>(hp, n_tiles_pad);
+        rv::McTile* h_tiles16 = carve<rv::McTile>(hp, n_tiles16);
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
@@ -755,25 +761,29 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         // Pass 2: write.
-        size_t nt = 0;
+        size_t nt = 0, nt16 = 0;
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
             h_pus[pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4))]++] = u;
             uint8_t refs = 0;
             for (int l = 0; l < 2; ++l)
                 if ((u.dir >> l) & 1) refs |= (uint8_t)(u.ref_idx[l] << (4 * l));
+            auto tile = [&](rv::McTile& t, int dx, int dy, int tw, int th) {
+                t.x = (uint16_t)(u.x + dx);
+                t.y = (uint16_t)(u.y + dy);
+                t.tw = (uint8_t)tw;
+                t.th = (uint8_t)th;
+                t.dir = u.dir;
+                t.refs = refs;
+                t.mv[0] = u.mv[0];
+                t.mv[1] = u.mv[1];
+            };
+            const int w16 = u.w & ~15, h16 = u.h & ~15;
+            for (int dy = 0; dy < h16; dy += 16)
+                for (int dx = 0; dx < w16; dx += 16) tile(h_tiles16[nt16++], dx, dy, 16, 16);
             for (int dy = 0; dy < u.h; dy += 8)
-                for (int dx = 0; dx < u.w; dx += 8) {
-                    rv::McTile& t = h_tiles[nt++];
-                    t.x = (uint16_t)(u.x + dx);
-                    t.y = (uint16_t)(u.y + dy);
-                    t.tw = (uint8_t)std::min(8, u.w - dx);
-                    t.th = (uint8_t)std::min(8, u.h - dy);
-                    t.dir = u.dir;
-                    t.refs = refs;
-                    t.mv[0] = u.mv[0];
-                    t.mv[1] = u.mv[1];
-                }
+                for (int dx = dy < h16 ? w16 : 0; dx < u.w; dx += 8)
+                    tile(h_tiles[nt++], dx, dy, std::min(8, u.w - dx), std::min(8, u.h - dy));
         }
         if (n_tiles_pad > nt) std::memset(h_tiles + nt, 0, (n_tiles_pad - nt) * sizeof(rv::McTile));
         for (int i = 0, ni = 0; i < p->n_cus; ++i) {
@@ -844,6 +854,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.n_pus = p->n_pus;
         d.n_ltus = (int)n_ltus;
         d.n_tiles = (int)n_tiles_pad;
+        d.n_tiles16 = (int)n_tiles16;
         d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
         for (int c = 0, acc = 0; c < 4; ++c) {
@@ -856,6 +867,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.ltus = (const rv_hevc_tu*)dev((char*)h_ltus);
         d.ctus = (const rv_hevc_tu*)dev((char*)h_ctus);
         d.tiles = (const rv::McTile*)dev((char*)h_tiles);
+        d.tiles16 = (const rv::McTile*)dev((char*)h_tiles16);
         d.nz = (const rv_nz*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.sao = (const rv_sao*)dev((char*)h_sao);

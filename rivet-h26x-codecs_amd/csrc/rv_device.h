@@ -9,13 +9,14 @@
 
 namespace rv {
 
-// One motion-compensation block: at most 8x8 luma samples (and its two
-// 4:2:0 chroma blocks) of one prediction unit, carrying the PU's motion so
-// a lane group reads one record per block. Splitting PUs into blocks on the
-// host keeps work of even size; a wave serves four blocks.
+// One motion-compensation block of one prediction unit, carrying the PU's
+// motion: either a 16x16 tile of the PU's 16-aligned core (one per wave,
+// one shared reference window) or an at most 8x8 block of what remains (a
+// wave serves four), each with its two 4:2:0 chroma blocks. Splitting PUs
+// on the host keeps work of even size.
 struct alignas(16) McTile {
     uint16_t x, y;    // block top-left, luma samples
-    uint8_t tw, th;   // block size, luma samples (4 or 8)
+    uint8_t tw, th;   // block size, luma samples (16, or 4 or 8)
     uint8_t dir;      // bit 0: list 0, bit 1: list 1
     uint8_t refs;     // ref_idx of list 0 in bits 0..3, list 1 in bits 4..7
     rv_mv mv[2];
@@ -36,7 +37,8 @@ struct HevcPicDev {
     int32_t deblock, beta_offset, tc_offset, cb_qp_offset, cr_qp_offset;
     int32_t sao_luma, sao_chroma;
     // Work lists.
-    int32_t n_cus, n_pus, n_ltus, n_tiles;
+    int32_t n_cus, n_pus, n_ltus, n_tiles;  // n_tiles: small blocks, a multiple of 4
+    int32_t n_tiles16;                      // 16x16 tiles
     int32_t n_side, n_intra;  // stage-0 side-data items (64 list entries each), intra CUs
     const int32_t* intra_idx; // indices of the intra CUs in cus
     int32_t n_tx[4];         // coded TUs per size (4, 8, 16, 32), in that order
@@ -45,7 +47,8 @@ struct HevcPicDev {
     const rv_hevc_cu* cus;
     const rv_hevc_pu* pus;
     const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
-    const McTile* tiles;
+    const McTile* tiles;     // small blocks
+    const McTile* tiles16;   // 16x16 tiles
     const rv_hevc_tu* ctus;  // coded TUs grouped by size, smallest first
     const rv_nz* nz;         // nonzero coefficients of ctus, in ctus order
     const uint16_t* intra_pred;
