@@ -619,39 +619,68 @@ __device__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) {
     }
 }
 
-// Side data of one CU, PU or luma TU, one thread each
-// (src/hevc/ctu.rs:751-800 for CUs and PUs, :1493-1510 for TUs).
-__device__ void side_entry(PicRef P, int k) {
-    if (k < P.n_cus) {
-        const rv_hevc_cu cu = P.cus[k];
-        const int n4 = 1 << (cu.log2_size - 2), bx0 = cu.x >> 2, by0 = cu.y >> 2;
-        for (int by = by0; by < by0 + n4; ++by)
-            for (int bx = bx0; bx < bx0 + n4; ++bx) {
-                const size_t q = (size_t)by * P.w4 + bx;
-                const bool left = bx == bx0, top = by == by0;
-                P.qp[q] = cu.qp_y;
-                P.cuinfo[q] = (uint8_t)(cu.intra | (left << 1) | (top << 2));
-                if (cu.intra) P.pumap[q] = 0x3FFFFFFFu | ((uint32_t)left << 30) | ((uint32_t)top << 31);
-            }
-        return;
+// Side data (src/hevc/ctu.rs:751-800 for CUs and PUs, :1493-1510 for
+// TUs): per 4x4 cell, a CU writes QpY and its intra / edge flags (and, if
+// intra, the no-PU marker with edge flags into pumap), a PU its index and
+// edge flags into pumap, a luma TU its cbf and edge flags. One work item
+// writes kSideCellsPerItem consecutive cells of one class, 64 per round;
+// each round's entry records are requested before any cell is written.
+typedef const SideClass __attribute__((address_space(4))) ConstSide;
+
+__device__ void side_item(PicRef P, int item, int tid) {
+    ConstSide* cl = (ConstSide*)P.sclass;
+    int lo = 0, hi = P.n_sclass - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cl[mid].item0 <= item) lo = mid;
+        else hi = mid - 1;
     }
-    k -= P.n_cus;
-    if (k < P.n_pus) {
-        const rv_hevc_pu pu = P.pus[k];
-        const int bx0 = pu.x >> 2, by0 = pu.y >> 2, nw = pu.w >> 2, nh = pu.h >> 2;
-        for (int by = by0; by < by0 + nh; ++by)
-            for (int bx = bx0; bx < bx0 + nw; ++bx)
-                P.pumap[(size_t)by * P.w4 + bx] = (uint32_t)k | ((uint32_t)(bx == bx0) << 30) | ((uint32_t)(by == by0) << 31);
-        return;
+    const int kind = cl[lo].kind, first = cl[lo].first, count = cl[lo].count, cw = cl[lo].cw, ch = cl[lo].ch;
+    const int cells = cw * ch, base = (item - cl[lo].item0) * kSideCellsPerItem;
+    constexpr int R = kSideCellsPerItem / 64;
+    int e[R], cx[R], cy[R];
+    bool on[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int g = base + r * 64 + tid;
+        e[r] = g / cells;
+        on[r] = e[r] < count;
+        const int cell = g - e[r] * cells;
+        cy[r] = cell / cw;
+        cx[r] = cell - cy[r] * cw;
     }
-    k -= P.n_pus;
-    if (k < P.n_ltus) {
-        const rv_hevc_tu tu = P.ltus[k];
-        const int n4 = 1 << (tu.log2 - 2), bx0 = tu.x >> 2, by0 = tu.y >> 2;
-        const uint8_t cbf = (tu.flags & RV_TU_CODED) ? 1 : 0;
-        for (int by = by0; by < by0 + n4; ++by)
-            for (int bx = bx0; bx < bx0 + n4; ++bx)
-                P.tuinfo[(size_t)by * P.w4 + bx] = (uint8_t)(cbf | ((bx == bx0) << 1) | ((by == by0) << 2));
+    // Entry records as dwords: CU {x | y << 16, log2 | qp << 8 | intra << 16},
+    // PU {x | y << 16}, TU {x | y << 16, comp | log2 << 8 | flags << 16}.
+    const uint32_t* recs = kind == SIDE_CU ? (const uint32_t*)P.cus
+                           : kind == SIDE_PU ? (const uint32_t*)P.pus : (const uint32_t*)P.ltus;
+    const int stride = kind == SIDE_CU ? 3 : kind == SIDE_PU ? 5 : 4;  // dwords per record
+    uint32_t w0[R], w1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (on[r]) {
+            const uint32_t* rp = recs + (size_t)(first + e[r]) * stride;
+            w0[r] = rp[0];
+            w1[r] = kind == SIDE_PU ? 0u : rp[1];
+        }
+    }
+    const int w4 = P.w4;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!on[r]) continue;
+        const int bx = (int)(w0[r] & 0xFFFF) / 4 + cx[r], by = (int)(w0[r] >> 16) / 4 + cy[r];
+        const size_t q = (size_t)by * w4 + bx;
+        const uint32_t left = cx[r] == 0, top = cy[r] == 0;
+        if (kind == SIDE_CU) {
+            const uint32_t intra = ((w1[r] >> 16) & 0xFF) ? 1u : 0u;
+            P.qp[q] = (int8_t)((w1[r] >> 8) & 0xFF);
+            P.cuinfo[q] = (uint8_t)(intra | (left << 1) | (top << 2));
+            if (intra) P.pumap[q] = 0x3FFFFFFFu | (left << 30) | (top << 31);
+        } else if (kind == SIDE_PU) {
+            P.pumap[q] = (uint32_t)(first + e[r]) | (left << 30) | (top << 31);
+        } else {
+            const uint32_t cbf = ((w1[r] >> 16) & RV_TU_CODED) ? 1u : 0u;
+            P.tuinfo[q] = (uint8_t)(cbf | (left << 1) | (top << 2));
+        }
     }
 }
 
@@ -670,7 +699,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) he
     int i = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const int tid = threadIdx.x;
     if (i < P.n_side) {
-        side_entry(P, i * 64 + tid);
+        side_item(P, i, tid);
         return;
     }
     i -= P.n_side;
@@ -1786,7 +1815,6 @@ hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStrea
 
 // Work items per stage for one picture (the grid unit of each stage).
 void hevc_stage_items(HevcPicDev& P) {
-    P.n_side = (P.n_cus + P.n_pus + P.n_ltus + 63) / 64;
     P.items[0] = P.n_side + P.n_intra + P.n_tiles16 + (P.n_tiles + 3) / 4;
     P.tx_items[0] = 0;
     for (int c = 0; c < 4; ++c) {

@@ -647,20 +647,23 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         const int W = p->width, H = p->height;
         // Pass 1: counts. Every list is then written once, straight into the
         // pinned staging buffer, in the order the kernels read it:
-        //  - MC blocks of at most 8x8, in PU order, padded with empty blocks
-        //    to a multiple of four (a wave reads four records together);
-        //  - CUs, PUs and luma TUs by size, largest first (side data gives
-        //    each one thread that writes all its 4x4 cells, so a wave's
-        //    threads have equal work; picture order within a size; the PU
-        //    index stored per cell refers to this order, and the deblocking
-        //    filter reads motion through it);
+        //  - MC: the 16-aligned core of each PU as 16x16 tiles, and the rest
+        //    as blocks of at most 8x8, padded with empty blocks to a multiple
+        //    of four (a wave reads four records together), in PU order;
+        //  - CUs by size, PUs by shape and luma TUs by size, largest first
+        //    (side data writes one class of equal shapes per work item, lane
+        //    by cell; picture order within a class; the PU index stored per
+        //    cell refers to this order, and the deblocking filter reads
+        //    motion through it);
         //  - coded TUs by size class, smallest first, their nonzero
         //    coefficients repacked in that order so a wave's TUs read one run
         //    of memory.
         size_t n_tiles = 0, n_tiles16 = 0;
         int64_t b_mc = 0;
         const bool weighted = p->weighted != 0;
-        std::vector<int> pu_cnt(258, 0);
+        // PUs by shape (w, h descending), the side-data classes.
+        auto pu_bucket = [](const rv_hevc_pu& u) { return (size_t)((16 - u.w / 4) * 16 + (16 - u.h / 4)); };
+        std::vector<int> pu_cnt(257, 0);
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
             // The 16-aligned core in 16x16 tiles; the right and bottom
@@ -668,7 +671,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             const int w16 = u.w & ~15, h16 = u.h & ~15;
             n_tiles16 += (size_t)(w16 >> 4) * (h16 >> 4);
             n_tiles += (size_t)((u.w - w16 + 7) >> 3) * ((u.h + 7) >> 3) + (size_t)(w16 >> 3) * ((u.h - h16 + 7) >> 3);
-            ++pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4)) + 1];
+            ++pu_cnt[pu_bucket(u) + 1];
             // Algorithmic bytes of motion compensation (the reference window
             // per list; whole-sample default-weighted uni reads the block).
             const bool uni_default = u.dir != 3 && !weighted;
@@ -709,9 +712,24 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                 coded_area += (int64_t)1 << (2 * t.log2);
             }
         }
-        for (int k = 0; k < 257; ++k) pu_cnt[(size_t)k + 1] += pu_cnt[(size_t)k];
+        for (int k = 0; k < 256; ++k) pu_cnt[(size_t)k + 1] += pu_cnt[(size_t)k];
         for (int k = 0; k < 4; ++k) cu_cnt[k + 1] += cu_cnt[k];
         for (int k = 0; k < 4; ++k) ltu_cnt[k + 1] += ltu_cnt[k];
+        // Side-data classes over the sorted lists: CUs (64 .. 8), PUs by
+        // shape, luma TUs (32 .. 4); empty classes are left out.
+        rv::SideClass sclass[rv::kMaxSideClasses];
+        int n_sclass = 0, n_side = 0;
+        auto add_class = [&](int kind, int first, int count, int cw, int ch) {
+            if (count <= 0) return;
+            sclass[n_sclass++] = rv::SideClass{kind, first, count, cw, ch, n_side};
+            n_side += (int)(((int64_t)count * cw * ch + rv::kSideCellsPerItem - 1) / rv::kSideCellsPerItem);
+        };
+        for (int k = 0; k < 4; ++k)
+            add_class(rv::SIDE_CU, cu_cnt[k], cu_cnt[k + 1] - cu_cnt[k], 16 >> k, 16 >> k);
+        for (int b = 0; b < 256; ++b)
+            add_class(rv::SIDE_PU, pu_cnt[(size_t)b], pu_cnt[(size_t)b + 1] - pu_cnt[(size_t)b], 16 - b / 16, 16 - b % 16);
+        for (int k = 0; k < 4; ++k)
+            add_class(rv::SIDE_TU, ltu_cnt[k], ltu_cnt[k + 1] - ltu_cnt[k], 8 >> k, 8 >> k);
         const int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
         const size_t n_sao = (size_t)wc * hc * 3;
 
@@ -731,6 +749,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
                     carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
+                    carve_size<rv::SideClass>(n_sclass) +
                     carve_size<rv_nz>(n_nzs) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) +
@@ -754,6 +773,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, n_ctus);
         rv::McTile* h_tiles = carve<rv::McTile>(hp, n_tiles_pad);
         rv::McTile* h_tiles16 = carve<rv::McTile>(hp, n_tiles16);
+        rv::SideClass* h_sclass = carve<rv::SideClass>(hp, n_sclass);
+        if (n_sclass) std::memcpy(h_sclass, sclass, n_sclass * sizeof(rv::SideClass));
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
@@ -764,7 +785,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         size_t nt = 0, nt16 = 0;
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
-            h_pus[pu_cnt[(size_t)(256 - ((u.w * u.h) >> 4))]++] = u;
+            h_pus[pu_cnt[pu_bucket(u)]++] = u;
             uint8_t refs = 0;
             for (int l = 0; l < 2; ++l)
                 if ((u.dir >> l) & 1) refs |= (uint8_t)(u.ref_idx[l] << (4 * l));
@@ -855,6 +876,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.n_ltus = (int)n_ltus;
         d.n_tiles = (int)n_tiles_pad;
         d.n_tiles16 = (int)n_tiles16;
+        d.n_side = n_side;
+        d.n_sclass = n_sclass;
+        d.sclass = (const rv::SideClass*)dev((char*)h_sclass);
         d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
         for (int c = 0, acc = 0; c < 4; ++c) {

@@ -23,6 +23,22 @@ struct alignas(16) McTile {
 };
 static_assert(sizeof(McTile) == 16, "one 16-byte load per tile");
 
+// Side data (the per-4x4 maps) is written per cell: the CUs, PUs and luma
+// TUs of one shape form a class (consecutive in their sorted lists), and
+// one work item covers kSideCellsPerItem consecutive cells of a class,
+// lane by lane, so a row of an entry is written by adjacent lanes.
+enum SideKind : int32_t { SIDE_CU, SIDE_PU, SIDE_TU };
+struct SideClass {
+    int32_t kind;    // SideKind
+    int32_t first;   // index of the class's first entry in its list
+    int32_t count;   // entries
+    int32_t cw, ch;  // entry size in 4x4 cells
+    int32_t item0;   // the class's first work item
+};
+constexpr int kSideCellsPerItem = 256;
+// Classes: CU sizes 64..8, PU shapes (w, h in 4..64), luma TU sizes 32..4.
+constexpr int kMaxSideClasses = 4 + 256 + 4;
+
 // Per-picture descriptor of an HEVC job, resident in device memory.
 struct HevcPicDev {
     int32_t w, h, bd, log2_ctb, wc, hc, w4, h4;
@@ -39,7 +55,9 @@ struct HevcPicDev {
     // Work lists.
     int32_t n_cus, n_pus, n_ltus, n_tiles;  // n_tiles: small blocks, a multiple of 4
     int32_t n_tiles16;                      // 16x16 tiles
-    int32_t n_side, n_intra;  // stage-0 side-data items (64 list entries each), intra CUs
+    int32_t n_side, n_intra;  // stage-0 side-data items, intra CUs
+    int32_t n_sclass;         // side-data classes
+    const SideClass* sclass;
     const int32_t* intra_idx; // indices of the intra CUs in cus
     int32_t n_tx[4];         // coded TUs per size (4, 8, 16, 32), in that order
     int32_t tx_first[4];     // index in ctus of each size's first TU
