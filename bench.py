@@ -227,6 +227,7 @@ def main() -> int:
             "stage_ms": {k: round(v, 4) for k, v in zip(STAGES, stage_ms)},
             "stage_gbs": {k: round(v, 1) for k, v in zip(STAGES, stage_gbs)},
             "stage_bytes": {k: int(v) for k, v in zip(STAGES, stage_bytes)},
+            "kernel_variants": {k: names[v] for k, names, v in zip(STAGES, rh.variant_names(), batch.variants())},
         },
     }
     if not args.graph_only and not args.no_serve:

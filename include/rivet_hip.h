@@ -327,6 +327,27 @@ int rv_mark_elapsed(int id_start, int id_end, float* ms);
  * captured graph is the exception: freeing it waits for the graph. */
 int rv_defer_free(int enable);
 
+/* Kernel variants and the autotuner. No counterpart in the reference,
+ * whose tables have one kernel per SIMD tier; the reference's C3
+ * configuration asks for tile shapes from an autotuner. Each HEVC stage has
+ * variants: variant 0 serves any picture; the others only 8-bit or only
+ * deeper pictures, compiled for one sample path with their own register and
+ * occupancy budgets. Every variant gives identical output.
+ * rv_batch_tune times each variant the batch's pictures allow inside full
+ * reconstructions of the batch (one untimed run, then `reps` timed runs; the
+ * minimum of the stage's time counts), keeps the fastest per stage for the
+ * batch, and records it for its depth class (all 8-bit, all deeper, mixed).
+ * A batch without its own choice launches with its class's recorded
+ * variants. RV_LAUNCH_GRAPH tunes a batch before its first capture when its
+ * class has no record, unless RIVET_HIP_TUNE=0 is set. H.264 stages have
+ * one variant each. */
+int rv_batch_tune(rv_batch* b, int reps);
+int rv_batch_variants(const rv_batch* b, int out[5]);
+/* RV_E_RANGE when the batch's sample depths rule the variant out. */
+int rv_batch_set_variant(rv_batch* b, int stage, int variant);
+int rv_variant_count(int codec, int stage);
+const char* rv_variant_name(int codec, int stage, int variant);
+
 /* Stage switches (mirror H26X_NO_DEBLOCK / H26X_NO_SAO,
  * src/hevc/decoder.rs:1247-1248): bit 0 skip deblocking, bit 1 skip SAO.
  * Also read once from RIVET_HIP_NO_DEBLOCK / RIVET_HIP_NO_SAO. */

@@ -98,12 +98,21 @@ __device__ __forceinline__ int find_pic(ConstI32* pre, int n, int it) {
     return lo;
 }
 
+// Kernel variants by sample depth: D = 0 serves any picture, D = 1 only
+// 8-bit pictures and D = 2 only deeper ones, so the specialised variants
+// compile one sample path and need fewer registers. A batch picks among
+// the variants its pictures allow (the autotuner times them).
+template <int D>
+__device__ __forceinline__ bool eight_bit(int bd) {
+    return D == 1 ? true : D == 2 ? false : bd == 8;
+}
+
 // ---------------------------------------------------------------------
 // Stage 0: side data, intra blocks, motion compensation
 // ---------------------------------------------------------------------
 
 template <typename S>
-__device__ void intra_copy(PicRef P, const rv_hevc_cu& cu, int tid) {
+__device__ __forceinline__ void intra_copy(PicRef P, const rv_hevc_cu& cu, int tid) {
     int s = 1 << cu.log2_size;
     const uint16_t* pr = P.intra_pred + cu.pred_off;
     for (int c = 0; c < 3; ++c) {
@@ -318,7 +327,7 @@ struct alignas(16) McLds {
 // rows 4 (s >> 3) ..+3), chroma lane (component s >> 3, x = s & 3, rows
 // 2 ((s >> 2) & 1) ..+1). Every lane reaches the barrier.
 template <typename S>
-__device__ void mc_item(PicRef P, int first, int tid, McLds& L) {
+__device__ __forceinline__ void mc_item(PicRef P, int first, int tid, McLds& L) {
     first = __builtin_amdgcn_readfirstlane(first);
     const int slot = tid >> 4, s = tid & 15;
     // The wave's four records (uniform loads: the list is padded to a
@@ -486,7 +495,7 @@ typedef const uint4 __attribute__((address_space(4))) ConstU4;
 // and lane (component tid >> 5, x = tid & 7, rows 2 ((tid >> 3) & 3) ..+1)
 // of chroma. Every lane reaches the barriers.
 template <typename S>
-__device__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) {
+__device__ __forceinline__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) {
     k = __builtin_amdgcn_readfirstlane(k);
     const McTile t = __builtin_bit_cast(McTile, *(ConstU4*)(P.tiles16 + k));
     const uint32_t* kt = &kF.qpk[0][0];
@@ -627,7 +636,7 @@ __device__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) {
 // each round's entry records are requested before any cell is written.
 typedef const SideClass __attribute__((address_space(4))) ConstSide;
 
-__device__ void side_item(PicRef P, int item, int tid) {
+__device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     ConstSide* cl = (ConstSide*)P.sclass;
     int lo = 0, hi = P.n_sclass - 1;
     while (lo < hi) {
@@ -684,7 +693,8 @@ __device__ void side_item(PicRef P, int item, int tid) {
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) hevc_stage0(BatchDev b) {
+template <int D, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_stage0(BatchDev b) {
     __shared__ union {
         McLds blocks;
         McLds16 tile;
@@ -705,18 +715,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) he
     i -= P.n_side;
     if (i < P.n_intra) {
         const rv_hevc_cu cu = P.cus[P.intra_idx[i]];
-        if (P.bd == 8) intra_copy<uint8_t>(P, cu, tid);
+        if (eight_bit<D>(P.bd)) intra_copy<uint8_t>(P, cu, tid);
         else intra_copy<uint16_t>(P, cu, tid);
         return;
     }
     i -= P.n_intra;
     if (i < P.n_tiles16) {
-        if (P.bd == 8) mc_tile16<uint8_t>(P, i, tid, lds.tile);
+        if (eight_bit<D>(P.bd)) mc_tile16<uint8_t>(P, i, tid, lds.tile);
         else mc_tile16<uint16_t>(P, i, tid, lds.tile);
         return;
     }
     i -= P.n_tiles16;
-    if (P.bd == 8) mc_item<uint8_t>(P, 4 * i, tid, lds.blocks);
+    if (eight_bit<D>(P.bd)) mc_item<uint8_t>(P, 4 * i, tid, lds.blocks);
     else mc_item<uint16_t>(P, 4 * i, tid, lds.blocks);
 }
 
@@ -852,7 +862,7 @@ __device__ __forceinline__ void add_row(S* d, const int (&r)[N], int max) {
 // (16-byte LDS reads), adding each row to the picture with 16-byte
 // accesses.
 template <int N, int L, typename S>
-__device__ void tu_lanes(PicRef P, int first, int count, int tid, int16_t* lds) {
+__device__ __forceinline__ void tu_lanes(PicRef P, int first, int count, int tid, int16_t* lds) {
     constexpr int C = N / L, PAD = N == 8 ? 8 : N == 16 ? 16 : 32, STRIDE = N * N + PAD;
     constexpr int ZV = N * N * 2 / L / 16;  // 16-byte zero stores per lane
     static_assert(C >= 2 && C <= 4 && ZV >= 1, "lane split");
@@ -946,7 +956,7 @@ __device__ void tu_lanes(PicRef P, int first, int count, int tid, int16_t* lds) 
 // both passes in registers and adds the residual one
 // 4-sample row (one 4- or 8-byte access) at a time.
 template <typename S>
-__device__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) {
+__device__ __forceinline__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) {
     if (tid >= count) return;
     const rv_hevc_tu tu = P.ctus[first + tid];
     int16_t* c = lds + tid * 16;
@@ -1015,7 +1025,7 @@ __device__ void tu4_lane(PicRef P, int first, int count, int tid, int16_t* lds) 
 }
 
 template <typename S>
-__device__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
+__device__ __forceinline__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
     int cls = 0;
     while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
     const int tpw = tus_per_item(cls);
@@ -1113,7 +1123,7 @@ __device__ __forceinline__ int edge_rule(bool on, bool tu_edge, bool pu_edge, in
 // the PU records of the edges that need a motion comparison are all
 // requested before any is used. Output byte per 4x4 block: bits 0..1 bS of
 // its left edge, bits 2..3 of its top edge.
-__device__ void area_bs(PicRef P, int a) {
+__device__ __forceinline__ void area_bs(PicRef P, int a) {
     const int w4 = P.w4, aw = w4 >> 1;
     const int bx = (a % aw) * 2, by = (a / aw) * 2;
     const size_t q = (size_t)by * w4 + bx;
@@ -1180,7 +1190,8 @@ __device__ void area_bs(PicRef P, int a) {
     out[(q + w4) >> 1] = (uint16_t)bottom_pair;
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) hevc_stage1(BatchDev b) {
+template <int D, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_stage1(BatchDev b) {
     // The largest class layout: 16 TUs of 16x16, each padded by 16 entries.
     __shared__ __align__(16) int16_t lds[tus_per_item(2) * (16 * 16 + 16)];
     ConstI32* pre = prefix(b, 1);
@@ -1193,7 +1204,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) he
     int i = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const int tid = threadIdx.x;
     if (i < P.tx_items[4]) {
-        if (P.bd == 8) tx_item<uint8_t>(P, i, tid, lds);
+        if (eight_bit<D>(P.bd)) tx_item<uint8_t>(P, i, tid, lds);
         else tx_item<uint16_t>(P, i, tid, lds);
         return;
     }
@@ -1422,7 +1433,7 @@ __global__ void __launch_bounds__(256) hevc_stage3(BatchDev b) {
 // samples left and right of the group are loaded first; the class decides
 // which of them are the two neighbours (static register indices).
 template <typename S>
-__device__ void sao_group(PicRef P, int c, int x0, int y, const rv_sao& sp, bool on) {
+__device__ __forceinline__ void sao_group(PicRef P, int c, int x0, int y, const rv_sao& sp, bool on) {
     constexpr int G = 16 / sizeof(S);
     const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c];
     const S* src = (const S*)P.cur[c];
@@ -1503,7 +1514,7 @@ __device__ void sao_group(PicRef P, int c, int x0, int y, const rv_sao& sp, bool
 
 // One sample, for CTBs narrower than a vector group.
 template <typename S>
-__device__ void sao_sample(PicRef P, int c, int x, int y, const rv_sao& sp, bool on) {
+__device__ __forceinline__ void sao_sample(PicRef P, int c, int x, int y, const rv_sao& sp, bool on) {
     const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c], max = (1 << P.bd) - 1;
     const S* src = (const S*)P.cur[c];
     const int vv = src[(size_t)y * pitch + x];
@@ -1528,7 +1539,7 @@ __device__ void sao_sample(PicRef P, int c, int x, int y, const rv_sao& sp, bool
 // One CTB per workgroup (256 lanes), all three components: the parameters
 // are uniform across each wave, so the type and class branches are scalar.
 template <typename S>
-__device__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
+__device__ __forceinline__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
     constexpr int G = 16 / sizeof(S);
     for (int c = 0; c < 3; ++c) {
         const int sh = c ? 1 : 0;
@@ -1549,6 +1560,7 @@ __device__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
     }
 }
 
+template <int D>
 __global__ void __launch_bounds__(256) hevc_stage4(BatchDev b) {
     ConstI32* pre = prefix(b, 4);
     const int it = xcd_item();
@@ -1559,7 +1571,7 @@ __global__ void __launch_bounds__(256) hevc_stage4(BatchDev b) {
     PicRef P = picture(b, p);
     const int i = it - pre[p];
     const int ry = i / P.wc, rx = i - ry * P.wc;
-    if (P.bd == 8) sao_ctb<uint8_t>(P, rx, ry, threadIdx.x);
+    if (eight_bit<D>(P.bd)) sao_ctb<uint8_t>(P, rx, ry, threadIdx.x);
     else sao_ctb<uint16_t>(P, rx, ry, threadIdx.x);
 }
 
@@ -1827,18 +1839,54 @@ void hevc_stage_items(HevcPicDev& P) {
     P.items[4] = P.wc * P.hc;
 }
 
-hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
+namespace {
+
+// Kernel variants per stage: [0] serves any picture; the others are
+// depth-specialised (eight_bit<D>) with their own occupancy targets.
+struct StageVariant {
+    const void* fn;
+    int depth;    // 0 any, 1 8-bit pictures only, 2 deeper pictures only
+    int threads;  // workgroup size
+    const char* name;
+};
+#define RV_VARIANT(k, d, t, n) \
+    StageVariant { reinterpret_cast<const void*>(&k), d, t, n }
+const StageVariant kStage0[] = {RV_VARIANT((hevc_stage0<0, 5>), 0, 64, "any/5"),
+                                RV_VARIANT((hevc_stage0<1, 6>), 1, 64, "8-bit/6"),
+                                RV_VARIANT((hevc_stage0<1, 7>), 1, 64, "8-bit/7"),
+                                RV_VARIANT((hevc_stage0<2, 5>), 2, 64, "deep/5")};
+const StageVariant kStage1[] = {RV_VARIANT((hevc_stage1<0, 4>), 0, 64, "any/4"),
+                                RV_VARIANT((hevc_stage1<1, 4>), 1, 64, "8-bit/4"),
+                                RV_VARIANT((hevc_stage1<2, 4>), 2, 64, "deep/4")};
+const StageVariant kStage2[] = {RV_VARIANT(hevc_stage2, 0, 256, "any")};
+const StageVariant kStage3[] = {RV_VARIANT(hevc_stage3, 0, 256, "any")};
+const StageVariant kStage4[] = {RV_VARIANT(hevc_stage4<0>, 0, 256, "any"), RV_VARIANT(hevc_stage4<1>, 1, 256, "8-bit"),
+                                RV_VARIANT(hevc_stage4<2>, 2, 256, "deep")};
+#undef RV_VARIANT
+struct StageTable {
+    const StageVariant* v;
+    int n;
+};
+const StageTable kStageTables[kStages] = {{kStage0, 4}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
+
+}  // namespace
+
+int hevc_stage_variants(int stage) { return stage >= 0 && stage < kStages ? kStageTables[stage].n : 0; }
+int hevc_variant_depth(int stage, int v) {
+    return v >= 0 && v < hevc_stage_variants(stage) ? kStageTables[stage].v[v].depth : -1;
+}
+const char* hevc_variant_name(int stage, int v) {
+    return v >= 0 && v < hevc_stage_variants(stage) ? kStageTables[stage].v[v].name : "";
+}
+
+hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st, int variant) {
     if (total <= 0) return hipSuccess;
+    if (variant < 0 || variant >= hevc_stage_variants(stage)) return hipErrorInvalidValue;
+    const StageVariant& k = kStageTables[stage].v[variant];
     const dim3 grid(8 * ((total + 7) / 8));  // see xcd_item()
-    switch (stage) {
-        case 0: hipLaunchKernelGGL(hevc_stage0, grid, dim3(64), 0, st, b); break;
-        case 1: hipLaunchKernelGGL(hevc_stage1, grid, dim3(64), 0, st, b); break;
-        case 2: hipLaunchKernelGGL(hevc_stage2, grid, dim3(256), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(hevc_stage3, grid, dim3(256), 0, st, b); break;
-        case 4: hipLaunchKernelGGL(hevc_stage4, grid, dim3(256), 0, st, b); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    BatchDev arg = b;
+    void* args[] = {&arg};
+    return hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
 }
 
 }  // namespace rv

@@ -230,6 +230,9 @@ struct rv_job {
 struct rv_batch {
     std::vector<rv_job*> jobs;
     std::vector<rv_job*> hevc, h264;
+    int depth_cls = 0;               // HEVC pictures: 1 all 8-bit, 2 all deeper, 0 mixed
+    int var[rv::kStages] = {0, 0, 0, 0, 0};  // HEVC kernel variant per stage
+    bool var_set = false;            // chosen by rv_batch_set_variant / rv_batch_tune
     rv::BatchDev bh{}, b4{};
     std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
     DevBuf tables;
@@ -273,14 +276,29 @@ hipError_t wait_session(rv_session* s) {
     return ev ? hipEventSynchronize(ev->e) : hipSuccess;
 }
 
-int launch_codec(const rv::BatchDev& b, const std::vector<int32_t>& pre, int n, bool hevc, hipStream_t st) {
+// Tuned HEVC kernel variants per depth class (1 all 8-bit, 2 all deeper,
+// 0 mixed), shared by every batch of that class; under g_mu.
+int g_tuned_var[3][rv::kStages];
+bool g_tuned[3] = {false, false, false};
+
+bool tuning_enabled() {
+    const char* e = std::getenv("RIVET_HIP_TUNE");
+    return !(e && e[0] == '0');
+}
+
+// `var`: the HEVC variant of each stage (null: variant 0).
+int launch_codec(const rv::BatchDev& b, const std::vector<int32_t>& pre, int n, bool hevc, hipStream_t st,
+                 const int* var = nullptr) {
     for (int s = 0; s < rv::kStages; ++s) {
         int total = pre[(size_t)s * (n + 1) + n];
-        hipError_t e = hevc ? rv::hevc_launch_stage(s, b, total, st) : rv::h264_launch_stage(s, b, total, st);
+        hipError_t e = hevc ? rv::hevc_launch_stage(s, b, total, st, var ? var[s] : 0)
+                            : rv::h264_launch_stage(s, b, total, st);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
     }
     return RV_OK;
 }
+
+int depth_class_of(int bd) { return bd == 8 ? 1 : 2; }
 
 // Return a job's buffers to the pool (after the work launched so far).
 void job_release(rv_job* j) {
@@ -1086,6 +1104,14 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
             (jobs[i]->codec == RV_CODEC_HEVC ? b->hevc : b->h264).push_back(jobs[i]);
         }
         int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
+        for (int i = 0; i < nh; ++i) {
+            const int c = depth_class_of(b->hevc[i]->hd.bd);
+            b->depth_cls = i == 0 ? c : (b->depth_cls == c ? c : 0);
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            if (g_tuned[b->depth_cls]) std::memcpy(b->var, g_tuned_var[b->depth_cls], sizeof(b->var));
+        }
         b->pre_h.assign((size_t)rv::kStages * (nh + 1), 0);
         b->pre_4.assign((size_t)rv::kStages * (n4 + 1), 0);
         for (int s = 0; s < rv::kStages; ++s) {
@@ -1150,10 +1176,13 @@ int rv_batch_free(rv_batch* b) {
     return RV_OK;
 }
 
+// Caller holds g_mu. Unless chosen for this batch, the variants are the
+// tuned ones of its depth class (if any), taken at each launch and capture.
 static int launch_batched(rv_batch* b, hipStream_t st) {
+    if (!b->var_set && g_tuned[b->depth_cls]) std::memcpy(b->var, g_tuned_var[b->depth_cls], sizeof(b->var));
     int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
     if (nh) {
-        int rc = launch_codec(b->bh, b->pre_h, nh, true, st);
+        int rc = launch_codec(b->bh, b->pre_h, nh, true, st, b->var);
         if (rc) return rc;
     }
     if (n4) {
@@ -1161,6 +1190,121 @@ static int launch_batched(rv_batch* b, hipStream_t st) {
         if (rc) return rc;
     }
     return RV_OK;
+}
+
+// Times every variant of each multi-variant HEVC stage that the batch's
+// pictures allow, inside full reconstruction runs of the batch (stage 0
+// rewrites every sample, so runs repeat exactly): one untimed run, then
+// `reps` timed ones, the minimum of the stage's event time counting. Keeps
+// the fastest per stage for the batch and its depth class. Caller holds
+// g_mu; nothing is captured meanwhile.
+static int tune_batch(rv_batch* b, int reps) {
+    const int nh = (int)b->hevc.size();
+    if (!nh) return RV_OK;
+    int var[rv::kStages];
+    std::memcpy(var, b->var, sizeof(var));
+    hipEvent_t ev[2];
+    RV_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return fail(RV_E_HIP, "hipEventCreate");
+    }
+    int rc = RV_OK;
+    for (int s = 0; s < rv::kStages && rc == RV_OK; ++s) {
+        const int nv = rv::hevc_stage_variants(s);
+        if (nv <= 1) continue;
+        int best = 0;
+        float best_ms = 1e30f;
+        for (int v = 0; v < nv && rc == RV_OK; ++v) {
+            const int d = rv::hevc_variant_depth(s, v);
+            if (d != 0 && d != b->depth_cls) continue;
+            var[s] = v;
+            float t_min = 1e30f;
+            for (int r = 0; r <= reps && rc == RV_OK; ++r) {
+                for (int k = 0; k < rv::kStages && rc == RV_OK; ++k) {
+                    if (k == s && hipEventRecord(ev[0], g_stream) != hipSuccess) rc = fail(RV_E_HIP, "event record");
+                    hipError_t e = rv::hevc_launch_stage(k, b->bh, b->pre_h[(size_t)k * (nh + 1) + nh], g_stream, var[k]);
+                    if (e != hipSuccess) rc = hip_fail(e, "kernel launch");
+                    if (k == s && rc == RV_OK && hipEventRecord(ev[1], g_stream) != hipSuccess)
+                        rc = fail(RV_E_HIP, "event record");
+                }
+                if (rc == RV_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = fail(RV_E_HIP, "event wait");
+                float ms = 0;
+                if (rc == RV_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) rc = fail(RV_E_HIP, "event time");
+                if (r > 0) t_min = std::min(t_min, ms);
+            }
+            if (t_min < best_ms) {
+                best_ms = t_min;
+                best = v;
+            }
+        }
+        var[s] = best;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    std::memcpy(b->var, var, sizeof(var));
+    b->var_set = true;
+    std::memcpy(g_tuned_var[b->depth_cls], var, sizeof(var));
+    g_tuned[b->depth_cls] = true;
+    return note_launch(b->hevc);
+}
+
+int rv_batch_tune(rv_batch* b, int reps) {
+    try {
+        if (!b) return fail(RV_E_INVALID, "null batch");
+        if (reps < 1 || reps > 100) return fail(RV_E_RANGE, "reps outside 1..100");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        std::lock_guard<std::mutex> lk(g_mu);
+        const int rc = tune_batch(b, reps);
+        if (rc == RV_OK && b->exec) {  // re-captured at the next graph launch
+            (void)hipStreamSynchronize(g_stream);
+            (void)hipGraphExecDestroy(b->exec);
+            (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+        }
+        return rc;
+    } catch (...) {
+        return fail(RV_E_HIP, "exception in rv_batch_tune");
+    }
+}
+
+int rv_batch_variants(const rv_batch* b, int out[5]) {
+    if (!b || !out) return fail(RV_E_INVALID, "null argument");
+    for (int s = 0; s < rv::kStages; ++s) out[s] = b->var[s];
+    return RV_OK;
+}
+
+int rv_batch_set_variant(rv_batch* b, int stage, int variant) {
+    if (!b) return fail(RV_E_INVALID, "null batch");
+    if (stage < 0 || stage >= rv::kStages || variant < 0 || variant >= rv::hevc_stage_variants(stage))
+        return fail(RV_E_RANGE, "stage or variant out of range");
+    const int d = rv::hevc_variant_depth(stage, variant);
+    if (d != 0 && (b->hevc.empty() || d != b->depth_cls))
+        return fail(RV_E_RANGE, "variant not valid for the batch's sample depths");
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (b->exec && b->var[stage] != variant) {  // re-captured at the next graph launch
+        (void)hipStreamSynchronize(g_stream);
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    b->var[stage] = variant;
+    b->var_set = true;
+    return RV_OK;
+}
+
+int rv_variant_count(int codec, int stage) {
+    if (stage < 0 || stage >= rv::kStages) return 0;
+    if (codec == RV_CODEC_HEVC) return rv::hevc_stage_variants(stage);
+    return codec == RV_CODEC_H264 ? 1 : 0;
+}
+
+const char* rv_variant_name(int codec, int stage, int variant) {
+    if (codec == RV_CODEC_HEVC) return rv::hevc_variant_name(stage, variant);
+    return codec == RV_CODEC_H264 && stage >= 0 && stage < rv::kStages && variant == 0 ? "any" : "";
 }
 
 int rv_batch_launch(rv_batch* b, int mode) {
@@ -1176,7 +1320,9 @@ int rv_batch_launch(rv_batch* b, int mode) {
                     pre[s * 2] = 0;
                     pre[s * 2 + 1] = items[s];
                 }
-                int rc = launch_codec(j->single, pre, 1, j->codec == RV_CODEC_HEVC, g_stream);
+                const int cls = j->codec == RV_CODEC_HEVC ? depth_class_of(j->hd.bd) : 0;
+                int rc = launch_codec(j->single, pre, 1, j->codec == RV_CODEC_HEVC, g_stream,
+                                      g_tuned[cls] ? g_tuned_var[cls] : nullptr);
                 if (rc) return rc;
             }
             return note_launch(b->jobs);
@@ -1186,6 +1332,10 @@ int rv_batch_launch(rv_batch* b, int mode) {
             return rc ? rc : note_launch(b->jobs);
         }
         if (mode == RV_LAUNCH_GRAPH) {
+            if (!b->exec && tuning_enabled() && !b->hevc.empty() && !g_tuned[b->depth_cls]) {
+                const int rc = tune_batch(b, 3);
+                if (rc) return rc;
+            }
             if (!b->exec) {
                 RV_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
                 int rc = launch_batched(b, g_stream);
@@ -1219,7 +1369,8 @@ int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
             for (int s = 0; s < rv::kStages && rc == RV_OK; ++s) {
                 int nh = (int)b->hevc.size(), n4 = (int)b->h264.size();
                 if (nh) {
-                    hipError_t e = rv::hevc_launch_stage(s, b->bh, b->pre_h[(size_t)s * (nh + 1) + nh], g_stream);
+                    hipError_t e =
+                        rv::hevc_launch_stage(s, b->bh, b->pre_h[(size_t)s * (nh + 1) + nh], g_stream, b->var[s]);
                     if (e != hipSuccess) rc = hip_fail(e, "kernel launch");
                 }
                 if (n4 && rc == RV_OK) {

@@ -9,7 +9,12 @@ namespace rv {
 
 // HEVC (hevc.hip).
 void hevc_stage_items(HevcPicDev& P);
-hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
+// Stage kernels come in variants (variant 0 serves any picture; the
+// others only pictures of one depth class).
+int hevc_stage_variants(int stage);
+int hevc_variant_depth(int stage, int variant);  // 0 any, 1 8-bit only, 2 deeper only
+const char* hevc_variant_name(int stage, int variant);
+hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st, int variant = 0);
 // One per-entry call: `blocks` workgroups of 256 threads.
 hipError_t hevc_entry_launch(const HevcEntry& e, int blocks, hipStream_t st);
 

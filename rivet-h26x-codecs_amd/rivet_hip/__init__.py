@@ -207,6 +207,20 @@ class Batch:
         check(lib.rv_batch_profile(self._h, reps, out))
         return list(out)
 
+    def tune(self, reps: int = 3) -> list:
+        """Time every kernel variant the batch allows; returns the chosen
+        variant per stage (also recorded for the batch's depth class)."""
+        check(lib.rv_batch_tune(self._h, reps))
+        return self.variants()
+
+    def variants(self) -> list:
+        out = (C.c_int32 * 5)()
+        check(lib.rv_batch_variants(self._h, out))
+        return list(out)
+
+    def set_variant(self, stage: int, variant: int) -> None:
+        check(lib.rv_batch_set_variant(self._h, stage, variant))
+
     def close(self) -> None:
         if self._h and self._h.value:
             check(lib.rv_batch_free(self._h))
@@ -217,6 +231,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def variant_names(codec: int = RV_CODEC_HEVC) -> list:
+    """Names of the kernel variants of each stage."""
+    return [[lib.rv_variant_name(codec, s, v).decode() for v in range(lib.rv_variant_count(codec, s))]
+            for s in range(5)]
 
 
 def sync() -> None:

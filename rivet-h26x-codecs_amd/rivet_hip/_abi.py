@@ -139,6 +139,11 @@ PROTOTYPES = {
     "rv_batch_launch": (i32, [P, i32]),
     "rv_sync": (i32, []),
     "rv_batch_profile": (i32, [P, i32, C.POINTER(C.c_float)]),
+    "rv_batch_tune": (i32, [P, i32]),
+    "rv_batch_variants": (i32, [P, C.POINTER(i32)]),
+    "rv_batch_set_variant": (i32, [P, i32, i32]),
+    "rv_variant_count": (i32, [i32, i32]),
+    "rv_variant_name": (C.c_char_p, [i32, i32, i32]),
     "rv_mark": (i32, [C.POINTER(i32)]),
     "rv_mark_query": (i32, [i32]),
     "rv_mark_wait": (i32, [i32]),

@@ -99,3 +99,19 @@ def test_disable_switch_refuses_the_tier():
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True, timeout=120)
     rc, msg = out.stdout.split(" ", 1)
     assert int(rc) == -2 and "RIVET_HIP_DISABLE" in msg
+
+
+def test_kernel_variant_tables():
+    """Variant tables answer without a device: HEVC stages 0, 1 and 4 have a
+    generic variant first and depth-specialised ones after it; H.264 stages
+    have one each; out-of-range queries give 0 / an empty name."""
+    import rivet_hip as rh
+
+    hevc = rh.variant_names(rh.RV_CODEC_HEVC)
+    assert [len(v) for v in hevc] == [4, 3, 1, 1, 3]
+    for stage in hevc:
+        assert stage[0].startswith("any")
+        assert all(n.startswith(("8-bit", "deep")) for n in stage[1:])
+    assert rh.variant_names(rh.RV_CODEC_H264) == [["any"]] * 5
+    assert rh.lib.rv_variant_count(0, 5) == 0 and rh.lib.rv_variant_count(7, 0) == 0
+    assert rh.lib.rv_variant_name(0, 0, 9) == b""
