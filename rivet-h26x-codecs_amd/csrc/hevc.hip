@@ -241,26 +241,29 @@ __device__ __forceinline__ void load_pairs(const int16_t* src, short2v (&p)[6]) 
 // picks column clamp(gx + i, 0, pw - 1) of the window for each sample.
 // Both give the samples of the reference's replicated border
 // (src/hevc/frame.rs:169-214) and clamped gather (src/hevc/inter.rs:116-123).
+// A span keeps only its first dword and its column (the window's start
+// column, byte shift and whether it crosses an edge follow from the
+// column), so little stays live while a wave's loads are in flight.
 template <typename S, int N>
 struct Span {
     static constexpr int ND = N * (int)sizeof(S) / 4 + 1;  // dwords covering the span
-    const S* row;       // the clamped row
     const uint32_t* w;  // first dword
-    uint32_t sh;        // byte offset of the first sample in w[0]
-    int gx;
-    bool fast;          // all N columns inside the plane
+    int gx;             // column of the span's first sample
 };
+// The window's first column: gx when the span lies inside the plane, else
+// the in-plane start that holds every clamped column. Reads may run past
+// the row's last sample by less than N + 2 samples, inside the row pitch
+// or the plane's slack.
+template <int N>
+__device__ __forceinline__ int span_cx(int gx, int pw) {
+    return max(min(gx, pw - N), 0);
+}
 template <typename S, int N>
 __device__ __forceinline__ Span<S, N> span_at(const S* pl, int pitch, int pw, int ph, int gx, int gy) {
     Span<S, N> sp;
-    sp.row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
-    sp.fast = gx >= 0 && gx + N - 1 < pw;
-    // In-plane start; reads may run past the row's last sample by less
-    // than N + 2 samples, inside the row pitch or the plane's slack.
-    const int cx = max(min(gx, pw - N), 0);
-    const uint32_t off = (uint32_t)cx * sizeof(S);
-    sp.w = (const uint32_t*)((const char*)sp.row + (off & ~3u));
-    sp.sh = off & 3u;
+    const S* row = pl + (size_t)clampi(gy, 0, ph - 1) * pitch;
+    const uint32_t off = (uint32_t)span_cx<N>(gx, pw) * sizeof(S);
+    sp.w = (const uint32_t*)((const char*)row + (off & ~3u));
     sp.gx = gx;
     return sp;
 }
@@ -268,14 +271,14 @@ template <typename S, int N>
 __device__ __forceinline__ void span_pairs(const Span<S, N>& sp, const uint32_t (&d)[Span<S, N>::ND], int pw,
                                            short2v (&p)[N / 2]) {
     constexpr int ND = Span<S, N>::ND;
-    if (!sp.fast) {
-        // The window starts at column cx = max(min(gx, pw - N), 0), byte
-        // sp.sh of d[0]; every clamped column lies in [cx, cx + N).
-        const int cx = max(min(sp.gx, pw - N), 0);
+    const int cx = span_cx<N>(sp.gx, pw);
+    const uint32_t sh = ((uint32_t)cx * sizeof(S)) & 3u;  // byte offset of column cx in d[0]
+    if (sp.gx < 0 || sp.gx + N - 1 >= pw) {
+        // Every clamped column lies in [cx, cx + N).
         uint32_t smp[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const uint32_t b = (uint32_t)(clampi(sp.gx + i, 0, pw - 1) - cx) * sizeof(S) + sp.sh;
+            const uint32_t b = (uint32_t)(clampi(sp.gx + i, 0, pw - 1) - cx) * sizeof(S) + sh;
             const uint32_t wi = b >> 2;
             uint32_t v = d[0];
 #pragma unroll
@@ -289,13 +292,13 @@ __device__ __forceinline__ void span_pairs(const Span<S, N>& sp, const uint32_t 
     if constexpr (sizeof(S) == 1) {
 #pragma unroll
         for (int i = 0; i < N / 4; ++i) {
-            const uint32_t v = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sp.sh);
+            const uint32_t v = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
             p[2 * i] = as2(__builtin_amdgcn_perm(0u, v, 0x0c010c00u));
             p[2 * i + 1] = as2(__builtin_amdgcn_perm(0u, v, 0x0c030c02u));
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sp.sh));
+        for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sh));
     }
 }
 
@@ -326,7 +329,7 @@ struct alignas(16) McLds {
 // straight from the reference planes. Second stage: luma lane (x = s & 7,
 // rows 4 (s >> 3) ..+3), chroma lane (component s >> 3, x = s & 3, rows
 // 2 ((s >> 2) & 1) ..+1). Every lane reaches the barrier.
-template <typename S>
+template <typename S, int LB>
 __device__ __forceinline__ void mc_item(PicRef P, int first, int tid, McLds& L) {
     first = __builtin_amdgcn_readfirstlane(first);
     const int slot = tid >> 4, s = tid & 15;
@@ -350,74 +353,82 @@ __device__ __forceinline__ void mc_item(PicRef P, int first, int tid, McLds& L) 
         mvx[l] = t.mv[l].x;
         mvy[l] = t.mv[l].y;
     }
-    // 1. First (horizontal) stage. Every span of both lists is located and
-    // its loads issued before any is filtered: one memory round trip.
+    // 1. First (horizontal) stage. The spans of LB lists at a time (both,
+    // or one when registers are short) are located and their loads issued
+    // before any is filtered: one memory round trip per group of lists.
     const int pw0 = P.pw[0], pw1 = P.pw[1];
     const int nt = (bh + 7) * 2, ng = bw >> 2, nr = (bh >> 1) + 3;
     const int cc = s >= nr ? 1 : 0, cr = s - nr * cc;  // chroma task: component, row
-    Span<S, 12> ls[2][2];
-    Span<S, 8> cs[2];
-    bool lact[2][2], cact[2];
-    uint32_t ld[2][2][Span<S, 12>::ND], cd[2][Span<S, 8>::ND];
 #pragma unroll
-    for (int l = 0; l < 2; ++l) {
-        const bool use = (dir >> l) & 1;
-        const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
-        const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
+    for (int g0 = 0; g0 < 2; g0 += LB) {
+        Span<S, 12> ls[LB][2];
+        Span<S, 8> cs[LB];
+        bool lact[LB][2], cact[LB];
+        uint32_t ld[LB][2][Span<S, 12>::ND], cd[LB][Span<S, 8>::ND];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int e = s + 16 * j, r = e >> 1, g = e & 1;
-            lact[l][j] = use && e < nt && g < ng;
-            ls[l][j] = span_at<S, 12>(ref, P.pitch[0], pw0, P.ph[0], x0 + 4 * g, y0 + r);
-        }
-        const S* cref = (const S*)(cc ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
-        cact[l] = use && s < 2 * nr;
-        cs[l] = span_at<S, 8>(cref, P.pitch[1], pw1, P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
-                              (t.y >> 1) + (mvy[l] >> 3) - 1 + cr);
-    }
-    // Left unset for inactive tasks (never read): a zero default would
-    // need a register copy of each loaded value, i.e. a wait.
+        for (int k = 0; k < LB; ++k) {
+            const int l = g0 + k;
+            const bool use = (dir >> l) & 1;
+            const S* ref = (const S*)ref_plane(P, l, ri[l], 0);
+            const int x0 = t.x + (mvx[l] >> 2) - 3, y0 = t.y + (mvy[l] >> 2) - 3;
 #pragma unroll
-    for (int l = 0; l < 2; ++l) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (lact[l][j]) {
-#pragma unroll
-                for (int i = 0; i < Span<S, 12>::ND; ++i) ld[l][j][i] = ls[l][j].w[i];
-            }
-        }
-        if (cact[l]) {
-#pragma unroll
-            for (int i = 0; i < Span<S, 8>::ND; ++i) cd[l][i] = cs[l].w[i];
-        }
-    }
-    L.taps[tid] = tap0;
-    if (tid + 64 < kTapWords) L.taps[tid + 64] = tap1;
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < 2; ++l) {
-        if (!((dir >> l) & 1)) continue;
-        const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (lact[l][j]) {
+            for (int j = 0; j < 2; ++j) {
                 const int e = s + 16 * j, r = e >> 1, g = e & 1;
-                short2v p[6];
-                span_pairs<S, 12>(ls[l][j], ld[l][j], pw0, p);
-                int acc[4];
-                filt4(p, tp, acc);
+                lact[k][j] = use && e < nt && g < ng;
+                ls[k][j] = span_at<S, 12>(ref, P.pitch[0], pw0, P.ph[0], x0 + 4 * g, y0 + r);
+            }
+            const S* cref = (const S*)(cc ? ref_plane(P, l, ri[l], 2) : ref_plane(P, l, ri[l], 1));
+            cact[k] = use && s < 2 * nr;
+            cs[k] = span_at<S, 8>(cref, P.pitch[1], pw1, P.ph[1], (t.x >> 1) + (mvx[l] >> 3) - 1,
+                                  (t.y >> 1) + (mvy[l] >> 3) - 1 + cr);
+        }
+        // Left unset for inactive tasks (never read): a zero default would
+        // need a register copy of each loaded value, i.e. a wait.
 #pragma unroll
-                for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
+        for (int k = 0; k < LB; ++k) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (lact[k][j]) {
+#pragma unroll
+                    for (int i = 0; i < Span<S, 12>::ND; ++i) ld[k][j][i] = ls[k][j].w[i];
+                }
+            }
+            if (cact[k]) {
+#pragma unroll
+                for (int i = 0; i < Span<S, 8>::ND; ++i) cd[k][i] = cs[k].w[i];
             }
         }
-        if (cact[l]) {
-            const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
-            short2v p[4];
-            span_pairs<S, 8>(cs[l], cd[l], pw1, p);
-            int acc[4];
-            filt4c(p, tp4, acc);
+        if (g0 == 0) {
+            L.taps[tid] = tap0;
+            if (tid + 64 < kTapWords) L.taps[tid + 64] = tap1;
+            __syncthreads();
+        }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) L.ctmp[l][slot][cc][i * 8 + cr] = (int16_t)(acc[i] >> shift1);
+        for (int k = 0; k < LB; ++k) {
+            const int l = g0 + k;
+            if (!((dir >> l) & 1)) continue;
+            const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (lact[k][j]) {
+                    const int e = s + 16 * j, r = e >> 1, g = e & 1;
+                    short2v p[6];
+                    span_pairs<S, 12>(ls[k][j], ld[k][j], pw0, p);
+                    int acc[4];
+                    filt4(p, tp, acc);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
+                }
+            }
+            if (cact[k]) {
+                const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
+                short2v p[4];
+                span_pairs<S, 8>(cs[k], cd[k], pw1, p);
+                int acc[4];
+                filt4c(p, tp4, acc);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) L.ctmp[l][slot][cc][i * 8 + cr] = (int16_t)(acc[i] >> shift1);
+            }
         }
     }
     __syncthreads();
@@ -693,7 +704,7 @@ __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     }
 }
 
-template <int D, int W>
+template <int D, int W, int LB = 2>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_stage0(BatchDev b) {
     __shared__ union {
         McLds blocks;
@@ -726,8 +737,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
         return;
     }
     i -= P.n_tiles16;
-    if (eight_bit<D>(P.bd)) mc_item<uint8_t>(P, 4 * i, tid, lds.blocks);
-    else mc_item<uint16_t>(P, 4 * i, tid, lds.blocks);
+    if (eight_bit<D>(P.bd)) mc_item<uint8_t, LB>(P, 4 * i, tid, lds.blocks);
+    else mc_item<uint16_t, LB>(P, 4 * i, tid, lds.blocks);
 }
 
 // ---------------------------------------------------------------------
@@ -1854,7 +1865,10 @@ struct StageVariant {
 const StageVariant kStage0[] = {RV_VARIANT((hevc_stage0<0, 5>), 0, 64, "any/5"),
                                 RV_VARIANT((hevc_stage0<1, 6>), 1, 64, "8-bit/6"),
                                 RV_VARIANT((hevc_stage0<1, 7>), 1, 64, "8-bit/7"),
-                                RV_VARIANT((hevc_stage0<2, 5>), 2, 64, "deep/5")};
+                                RV_VARIANT((hevc_stage0<1, 8, 1>), 1, 64, "8-bit/8 per-list"),
+                                RV_VARIANT((hevc_stage0<2, 5>), 2, 64, "deep/5"),
+                                RV_VARIANT((hevc_stage0<2, 6, 1>), 2, 64, "deep/6 per-list"),
+                                RV_VARIANT((hevc_stage0<0, 6, 1>), 0, 64, "any/6 per-list")};
 const StageVariant kStage1[] = {RV_VARIANT((hevc_stage1<0, 4>), 0, 64, "any/4"),
                                 RV_VARIANT((hevc_stage1<1, 4>), 1, 64, "8-bit/4"),
                                 RV_VARIANT((hevc_stage1<2, 4>), 2, 64, "deep/4")};
@@ -1867,7 +1881,7 @@ struct StageTable {
     const StageVariant* v;
     int n;
 };
-const StageTable kStageTables[kStages] = {{kStage0, 4}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
+const StageTable kStageTables[kStages] = {{kStage0, 7}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
 
 }  // namespace
 

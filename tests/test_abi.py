@@ -103,15 +103,17 @@ def test_disable_switch_refuses_the_tier():
 
 def test_kernel_variant_tables():
     """Variant tables answer without a device: HEVC stages 0, 1 and 4 have a
-    generic variant first and depth-specialised ones after it; H.264 stages
-    have one each; out-of-range queries give 0 / an empty name."""
+    generic variant first and further generic or depth-specialised ones after
+    it; H.264 stages have one each; out-of-range queries give 0 / an empty
+    name."""
     import rivet_hip as rh
 
     hevc = rh.variant_names(rh.RV_CODEC_HEVC)
-    assert [len(v) for v in hevc] == [4, 3, 1, 1, 3]
+    assert [len(v) for v in hevc] == [7, 3, 1, 1, 3]
     for stage in hevc:
         assert stage[0].startswith("any")
-        assert all(n.startswith(("8-bit", "deep")) for n in stage[1:])
+        assert all(n.startswith(("8-bit", "deep", "any")) for n in stage[1:])
+    assert sum(n.startswith(("8-bit", "deep")) for n in hevc[0]) >= 2
     assert rh.variant_names(rh.RV_CODEC_H264) == [["any"]] * 5
     assert rh.lib.rv_variant_count(0, 5) == 0 and rh.lib.rv_variant_count(7, 0) == 0
     assert rh.lib.rv_variant_name(0, 0, 9) == b""
