@@ -37,8 +37,15 @@ struct FilterTables {
     // << (14 - bd) (src/dsp/hevc.rs:823-882).
     alignas(16) uint32_t qpk[4][12];
     alignas(16) uint32_t epk[8][8];
+    // Signed-byte taps for v_dot4_i32_i8 (the 8-bit first stage): luma taps
+    // 0..3 then 4..7, chroma taps 0..3, one dword each; row 0 the identity.
+    alignas(16) uint32_t qb[4][2];
+    uint32_t eb[8];
 };
 constexpr uint32_t pair16(int lo, int hi) { return (uint32_t)(lo & 0xFFFF) | ((uint32_t)(hi & 0xFFFF) << 16); }
+constexpr uint32_t bytes4(int a, int b, int c, int d) {
+    return (uint32_t)(a & 0xFF) | ((uint32_t)(b & 0xFF) << 8) | ((uint32_t)(c & 0xFF) << 16) | ((uint32_t)(d & 0xFF) << 24);
+}
 constexpr FilterTables make_filter_tables() {
     FilterTables t{{{0, 0, 0, 0, 0, 0, 0, 0},
                     {-1, 4, -10, 58, 17, -5, 1, 0},
@@ -53,12 +60,15 @@ constexpr FilterTables make_filter_tables() {
         for (int k = 0; k < 8; ++k) g[k + 1] = f ? t.q[f][k] : (k == 3 ? 64 : 0);
         for (int i = 0; i < 4; ++i) t.qpk[f][i] = pair16(g[2 * i + 1], g[2 * i + 2]);
         for (int i = 0; i < 5; ++i) t.qpk[f][4 + i] = pair16(g[2 * i], g[2 * i + 1]);
+        t.qb[f][0] = bytes4(g[1], g[2], g[3], g[4]);
+        t.qb[f][1] = bytes4(g[5], g[6], g[7], g[8]);
     }
     for (int f = 0; f < 8; ++f) {
         int g[6] = {0, 0, 0, 0, 0, 0};
         for (int k = 0; k < 4; ++k) g[k + 1] = f ? t.e[f][k] : (k == 1 ? 64 : 0);
         for (int i = 0; i < 2; ++i) t.epk[f][i] = pair16(g[2 * i + 1], g[2 * i + 2]);
         for (int i = 0; i < 3; ++i) t.epk[f][2 + i] = pair16(g[2 * i], g[2 * i + 1]);
+        t.eb[f] = bytes4(g[1], g[2], g[3], g[4]);
     }
     return t;
 }
@@ -178,12 +188,16 @@ __device__ __forceinline__ short2v pk(int lo, int hi) {
 __device__ __forceinline__ short2v as2(uint32_t v) { return __builtin_bit_cast(short2v, v); }
 __device__ __forceinline__ int dot2(short2v a, short2v b, int c) { return __builtin_amdgcn_sdot2(a, b, c, false); }
 
-// The taps of one phase as packed pairs (FilterTables::qpk / epk), per lane
-// (the blocks of one wave have their own vectors), read from a copy of the
-// packed tables in LDS: `tab` holds qpk (4 x 12 words) then epk (8 x 8).
-constexpr int kTapWords = 4 * 12 + 8 * 8;
-static_assert(offsetof(FilterTables, epk) == offsetof(FilterTables, qpk) + sizeof(uint32_t) * 48,
-              "qpk and epk are contiguous");
+// The taps of one phase as packed pairs (FilterTables::qpk / epk) or signed
+// bytes (qb / eb), per lane (the blocks of one wave have their own
+// vectors), read from a copy of the packed tables in LDS: `tab` holds qpk
+// (4 x 12 words), epk (8 x 8), qb (4 x 2) and eb (8).
+constexpr int kTapWords = 4 * 12 + 8 * 8 + 4 * 2 + 8;
+constexpr int kQbWord = 4 * 12 + 8 * 8, kEbWord = kQbWord + 4 * 2;  // offsets of qb, eb
+static_assert(offsetof(FilterTables, epk) == offsetof(FilterTables, qpk) + sizeof(uint32_t) * 48 &&
+                  offsetof(FilterTables, qb) == offsetof(FilterTables, qpk) + sizeof(uint32_t) * kQbWord &&
+                  offsetof(FilterTables, eb) == offsetof(FilterTables, qpk) + sizeof(uint32_t) * kEbWord,
+              "the packed tap tables are contiguous");
 struct Taps8 {
     short2v e[4], o[5];
 };
@@ -206,6 +220,31 @@ __device__ __forceinline__ Taps4 chroma_taps(const uint32_t* tab, int frac) {
     t.e[0] = as2(a.x), t.e[1] = as2(a.y);
     t.o[0] = as2(a.z), t.o[1] = as2(a.w), t.o[2] = as2(b);
     return t;
+}
+__device__ __forceinline__ uint2 luma_taps_b(const uint32_t* tab, int frac) {
+    return *(const uint2*)(tab + kQbWord + 2 * frac);
+}
+__device__ __forceinline__ uint32_t chroma_taps_b(const uint32_t* tab, int frac) { return tab[kEbWord + frac]; }
+
+// 8-bit first stage with v_dot4_i32_i8: the span's bytes x are biased to
+// x - 128 (top bit flipped) to make them signed; every filter's taps sum to
+// 64, so starting each sum at 128 * 64 restores sum f x exactly. w holds
+// the span's samples in byte order, 4 per dword.
+__device__ __forceinline__ int dot4(uint32_t a, uint32_t b, int c) {
+    return __builtin_amdgcn_sdot4((int)a, (int)b, c, false);
+}
+__device__ __forceinline__ void filt4_b(const uint32_t (&w)[3], uint2 t, int (&acc)[4]) {
+    const uint32_t x0 = w[0] ^ 0x80808080u, x1 = w[1] ^ 0x80808080u, x2 = w[2] ^ 0x80808080u;
+    acc[0] = dot4(x1, t.y, dot4(x0, t.x, 8192));
+#pragma unroll
+    for (int j = 1; j < 4; ++j)
+        acc[j] = dot4(__builtin_amdgcn_alignbyte(x2, x1, j), t.y, dot4(__builtin_amdgcn_alignbyte(x1, x0, j), t.x, 8192));
+}
+__device__ __forceinline__ void filt4c_b(const uint32_t (&w)[2], uint32_t t, int (&acc)[4]) {
+    const uint32_t x0 = w[0] ^ 0x80808080u, x1 = w[1] ^ 0x80808080u;
+    acc[0] = dot4(x0, t, 8192);
+#pragma unroll
+    for (int j = 1; j < 4; ++j) acc[j] = dot4(__builtin_amdgcn_alignbyte(x1, x0, j), t, 8192);
 }
 
 // Four consecutive 8-tap outputs from twelve inputs held as six packed
@@ -267,26 +306,27 @@ __device__ __forceinline__ Span<S, N> span_at(const S* pl, int pitch, int pw, in
     sp.gx = gx;
     return sp;
 }
+// Sample i of a span crossing the plane's left or right edge: column
+// clamp(gx + i, 0, pw - 1) of the span's (clamped) row, read again from
+// memory. The row starts cx samples, rounded down to a dword, before the
+// span's first dword. Such spans are rare (the picture's outer columns).
+template <typename S, int N>
+__device__ __forceinline__ uint32_t span_clamped(const Span<S, N>& sp, int pw, int cx, int i) {
+    const S* row = (const S*)((const char*)sp.w - (((uint32_t)cx * sizeof(S)) & ~3u));
+    return row[clampi(sp.gx + i, 0, pw - 1)];
+}
+template <typename S, int N>
+__device__ __forceinline__ bool span_inside(const Span<S, N>& sp, int pw) {
+    return sp.gx >= 0 && sp.gx + N - 1 < pw;
+}
 template <typename S, int N>
 __device__ __forceinline__ void span_pairs(const Span<S, N>& sp, const uint32_t (&d)[Span<S, N>::ND], int pw,
                                            short2v (&p)[N / 2]) {
-    constexpr int ND = Span<S, N>::ND;
     const int cx = span_cx<N>(sp.gx, pw);
     const uint32_t sh = ((uint32_t)cx * sizeof(S)) & 3u;  // byte offset of column cx in d[0]
-    if (sp.gx < 0 || sp.gx + N - 1 >= pw) {
-        // Every clamped column lies in [cx, cx + N).
-        uint32_t smp[N];
+    if (!span_inside(sp, pw)) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const uint32_t b = (uint32_t)(clampi(sp.gx + i, 0, pw - 1) - cx) * sizeof(S) + sh;
-            const uint32_t wi = b >> 2;
-            uint32_t v = d[0];
-#pragma unroll
-            for (int k = 1; k < ND; ++k) v = wi == (uint32_t)k ? d[k] : v;
-            smp[i] = (v >> (8 * (b & 3))) & (sizeof(S) == 1 ? 0xFFu : 0xFFFFu);
-        }
-#pragma unroll
-        for (int i = 0; i < N / 2; ++i) p[i] = as2(smp[2 * i] | (smp[2 * i + 1] << 16));
+        for (int i = 0; i < N / 2; ++i) p[i] = as2(span_clamped(sp, pw, cx, 2 * i) | (span_clamped(sp, pw, cx, 2 * i + 1) << 16));
         return;
     }
     if constexpr (sizeof(S) == 1) {
@@ -299,6 +339,52 @@ __device__ __forceinline__ void span_pairs(const Span<S, N>& sp, const uint32_t 
     } else {
 #pragma unroll
         for (int i = 0; i < N / 2; ++i) p[i] = as2(__builtin_amdgcn_alignbyte(d[i + 1], d[i], sh));
+    }
+}
+// The 8-bit span's N samples as bytes, 4 per dword.
+template <int N>
+__device__ __forceinline__ void span_bytes(const Span<uint8_t, N>& sp, const uint32_t (&d)[Span<uint8_t, N>::ND],
+                                           int pw, uint32_t (&w)[N / 4]) {
+    const int cx = span_cx<N>(sp.gx, pw);
+    const uint32_t sh = (uint32_t)cx & 3u;
+    if (!span_inside(sp, pw)) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i)
+            w[i] = span_clamped(sp, pw, cx, 4 * i) | (span_clamped(sp, pw, cx, 4 * i + 1) << 8) |
+                   (span_clamped(sp, pw, cx, 4 * i + 2) << 16) | (span_clamped(sp, pw, cx, 4 * i + 3) << 24);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+}
+
+// First-stage outputs of one task: 4 luma outputs from a 12-sample span or
+// 4 chroma outputs from an 8-sample span, at phase `frac`; dot4 for 8-bit
+// samples, dot2 on i16 pairs above.
+template <typename S>
+__device__ __forceinline__ void first_luma(const Span<S, 12>& sp, const uint32_t (&d)[Span<S, 12>::ND], int pw,
+                                           const uint32_t* tab, int frac, int (&acc)[4]) {
+    if constexpr (sizeof(S) == 1) {
+        uint32_t w[3];
+        span_bytes<12>(sp, d, pw, w);
+        filt4_b(w, luma_taps_b(tab, frac), acc);
+    } else {
+        short2v p[6];
+        span_pairs<S, 12>(sp, d, pw, p);
+        filt4(p, luma_taps(tab, frac), acc);
+    }
+}
+template <typename S>
+__device__ __forceinline__ void first_chroma(const Span<S, 8>& sp, const uint32_t (&d)[Span<S, 8>::ND], int pw,
+                                             const uint32_t* tab, int frac, int (&acc)[4]) {
+    if constexpr (sizeof(S) == 1) {
+        uint32_t w[2];
+        span_bytes<8>(sp, d, pw, w);
+        filt4c_b(w, chroma_taps_b(tab, frac), acc);
+    } else {
+        short2v p[4];
+        span_pairs<S, 8>(sp, d, pw, p);
+        filt4c(p, chroma_taps(tab, frac), acc);
     }
 }
 
@@ -407,25 +493,19 @@ __device__ __forceinline__ void mc_item(PicRef P, int first, int tid, McLds& L) 
         for (int k = 0; k < LB; ++k) {
             const int l = g0 + k;
             if (!((dir >> l) & 1)) continue;
-            const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 if (lact[k][j]) {
                     const int e = s + 16 * j, r = e >> 1, g = e & 1;
-                    short2v p[6];
-                    span_pairs<S, 12>(ls[k][j], ld[k][j], pw0, p);
                     int acc[4];
-                    filt4(p, tp, acc);
+                    first_luma<S>(ls[k][j], ld[k][j], pw0, L.taps, mvx[l] & 3, acc);
 #pragma unroll
                     for (int i = 0; i < 4; ++i) L.ltmp[l][slot][(4 * g + i) * 16 + r] = (int16_t)(acc[i] >> shift1);
                 }
             }
             if (cact[k]) {
-                const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
-                short2v p[4];
-                span_pairs<S, 8>(cs[k], cd[k], pw1, p);
                 int acc[4];
-                filt4c(p, tp4, acc);
+                first_chroma<S>(cs[k], cd[k], pw1, L.taps, mvx[l] & 7, acc);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) L.ctmp[l][slot][cc][i * 8 + cr] = (int16_t)(acc[i] >> shift1);
             }
@@ -563,25 +643,19 @@ __device__ __forceinline__ void mc_tile16(PicRef P, int k, int tid, McLds16& L) 
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
-        const Taps8 tp = luma_taps(L.taps, mvx[l] & 3);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             if (lact[l][j]) {
                 const int e = tid + 64 * j, r = e >> 2, g = e & 3;
-                short2v p[6];
-                span_pairs<S, 12>(ls[l][j], ld[l][j], pw0, p);
                 int acc[4];
-                filt4(p, tp, acc);
+                first_luma<S>(ls[l][j], ld[l][j], pw0, L.taps, mvx[l] & 3, acc);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) L.lt[l][(4 * g + i) * 24 + r] = (int16_t)(acc[i] >> shift1);
             }
         }
         if (cact[l]) {
-            const Taps4 tp4 = chroma_taps(L.taps, mvx[l] & 7);
-            short2v p[4];
-            span_pairs<S, 8>(cs[l], cd[l], pw1, p);
             int acc[4];
-            filt4c(p, tp4, acc);
+            first_chroma<S>(cs[l], cd[l], pw1, L.taps, mvx[l] & 7, acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) L.ct[l][cc][(4 * cg + i) * 12 + cr] = (int16_t)(acc[i] >> shift1);
         }
