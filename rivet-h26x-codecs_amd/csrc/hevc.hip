@@ -960,15 +960,26 @@ __device__ __forceinline__ void tu_lanes(PicRef P, int first, int count, int tid
     for (int i = 0; i < ZV; ++i) tz[i] = make_uint4(0, 0, 0, 0);
     __syncthreads();
     if (active) {
+        // Lane `sub` scatters its own contiguous run of the TU's entries, a
+        // multiple of 4 long, 8 entries (two 16-byte loads) per round trip;
+        // a chunk may read up to 3 entries past the run, still inside the
+        // work-list arena.
         const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
-        const int n = tu.n_nz;
-        for (int k = sub; k < n; k += 4 * L) {
-            uint32_t v[4];
+        const int n = tu.n_nz, per = ((n + L - 1) / L + 3) & ~3;
+        const int lo = sub * per, hi = min(lo + per, n);
+        for (int k = lo; k < hi; k += 8) {
+            uint32_t v[8];
+            if (k < hi) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = k + i * L < n ? z[k + i * L] : 0u;
+                for (int i = 0; i < 4; ++i) v[i] = z[k + i];
+            }
+            if (k + 4 < hi) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (k + i * L < n) t[(v[i] & 0xFFFF) & (N * N - 1)] = (int16_t)(v[i] >> 16);
+                for (int i = 4; i < 8; ++i) v[i] = z[k + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (k + i < hi) t[(v[i] & 0xFFFF) & (N * N - 1)] = (int16_t)(v[i] >> 16);
         }
     }
     __syncthreads();
@@ -1048,17 +1059,23 @@ __device__ __forceinline__ void tu4_lane(PicRef P, int first, int count, int tid
     uint4* c4 = (uint4*)c;
     c4[0] = make_uint4(0, 0, 0, 0);
     c4[1] = make_uint4(0, 0, 0, 0);
-    // rv_nz is {uint16 pos, int16 val}: one dword, 4-byte aligned.
+    // rv_nz is {uint16 pos, int16 val}: one dword, 4-byte aligned. All of
+    // the TU's (at most 16) entries are requested at once, in 4-entry
+    // chunks (one 16-byte load each; the last may read up to 3 entries past
+    // the run, still inside the work-list arena), before any is stored.
     const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
     const int n = tu.n_nz;
-    for (int k = 0; k < n; k += 4) {
-        uint32_t v[4];
+    uint32_t v[16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = k + i < n ? z[k + i] : 0u;
+    for (int q = 0; q < 4; ++q) {
+        if (4 * q < n) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (k + i < n) c[v[i] & 15] = (int16_t)(v[i] >> 16);
+            for (int i = 0; i < 4; ++i) v[4 * q + i] = z[4 * q + i];
+        }
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (i < n) c[v[i] & 15] = (int16_t)(v[i] >> 16);
     const uint4 r0 = c4[0], r1 = c4[1];
     const uint32_t w[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
     int m[16];  // raster: m[y * 4 + x]
