@@ -243,6 +243,9 @@ def main() -> int:
         log(f"[bench] serve: batched {sv['batched']['luma_msamples_per_s']} Msample/s p50 "
             f"{sv['batched']['latency_ms_p50']} ms p99 {sv['batched']['latency_ms_p99']} ms; plain "
             f"{sv['plain']['luma_msamples_per_s']} Msample/s p99 {sv['plain']['latency_ms_p99']} ms")
+        dv = sv["device"]
+        log(f"[bench] serve, work lists prepared in advance: batched {dv['batched']['luma_msamples_per_s']} "
+            f"Msample/s, plain {dv['plain']['luma_msamples_per_s']} Msample/s ({dv['batched_over_plain']}x)")
         out["serve_p99_ms"] = sv["batched"]["latency_ms_p99"]
         out["extra"]["serve"] = sv
     print(json.dumps(out), flush=True)

@@ -338,9 +338,10 @@ int rv_defer_free(int enable);
  * minimum of the stage's time counts), keeps the fastest per stage for the
  * batch, and records it for its depth class (all 8-bit, all deeper, mixed).
  * A batch without its own choice launches with its class's recorded
- * variants. RV_LAUNCH_GRAPH tunes a batch before its first capture when its
- * class has no record, unless RIVET_HIP_TUNE=0 is set. H.264 stages have
- * one variant each. */
+ * variants; when its class has no record yet, its first RV_LAUNCH_BATCHED
+ * or RV_LAUNCH_GRAPH launch tunes it first (a few full reconstructions of
+ * the batch), unless RIVET_HIP_TUNE=0 is set. H.264 stages have one variant
+ * each. */
 int rv_batch_tune(rv_batch* b, int reps);
 int rv_batch_variants(const rv_batch* b, int out[5]);
 /* RV_E_RANGE when the batch's sample depths rule the variant out. */

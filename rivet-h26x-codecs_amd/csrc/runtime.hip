@@ -1328,11 +1328,15 @@ int rv_batch_launch(rv_batch* b, int mode) {
             return note_launch(b->jobs);
         }
         if (mode == RV_LAUNCH_BATCHED) {
+            if (!b->var_set && tuning_enabled() && !b->hevc.empty() && !g_tuned[b->depth_cls]) {
+                const int rc = tune_batch(b, 2);
+                if (rc) return rc;
+            }
             int rc = launch_batched(b, g_stream);
             return rc ? rc : note_launch(b->jobs);
         }
         if (mode == RV_LAUNCH_GRAPH) {
-            if (!b->exec && tuning_enabled() && !b->hevc.empty() && !g_tuned[b->depth_cls]) {
+            if (!b->exec && !b->var_set && tuning_enabled() && !b->hevc.empty() && !g_tuned[b->depth_cls]) {
                 const int rc = tune_batch(b, 3);
                 if (rc) return rc;
             }

@@ -178,21 +178,24 @@ def footprint(s: Sess) -> int:
 
 
 def run(sessions, mode: str = "batched", budget: int = 32 << 30, max_batch: int = 64, inflight_max: int = 2,
-        record: bool = False, upload_threads: int = 8):
+        record: bool = False, upload_threads: int = 8, preload: bool = False):
     """Serve every session; returns throughput and latency statistics.
     record=True downloads every reconstructed picture as it completes (into
     Pic.out; use inflight_max=1 so no later picture has overwritten it).
     Work-list uploads run on `upload_threads` host threads from admission
-    on, overlapping the reconstruction of earlier pictures."""
+    on, overlapping the reconstruction of earlier pictures. preload=True
+    opens every session and prepares every work list before the clock
+    starts (arrivals and the budget are ignored): the loop then measures the
+    device side alone, scheduling and launching."""
     ex = cf_.ThreadPoolExecutor(max_workers=upload_threads)
     try:
-        return _run(sessions, mode, budget, max_batch, inflight_max, record, ex)
+        return _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload)
     finally:
         ex.shutdown(wait=True)
         sync()
 
 
-def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
+def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=False):
     """The loop is event-driven: finished uploads (posted by the upload
     threads) and pictures made ready by a completed reference feed a heap
     of launchable pictures ordered by ready time."""
@@ -271,6 +274,12 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
         used += s.footprint
         active[id(s)] = s
 
+    if preload:
+        while arrivals:
+            admit(arrivals.popleft(), 0.0)
+        cf_.wait([p.fut for s in sessions for p in s.pics if p.fut is not None])
+        t0 = time.perf_counter()
+
     while done < total:
         wake.clear()
         t = now()
@@ -279,9 +288,10 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
         while waiting and (used + waiting[0].footprint <= budget or not active):
             admit(waiting.popleft(), now())
         while inflight and mark_done(inflight[0][0]):
-            _, pics, b = inflight.popleft()
+            _, pics, bs = inflight.popleft()
             tc = now()
-            b.close()
+            for b in bs:
+                b.close()
             for p in pics:
                 complete(p, tc)
         while uploaded:
@@ -298,11 +308,15 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
         if launchable and len(inflight) < inflight_max:
             group = [heapq.heappop(launchable)[2] for _ in range(min(max_batch, len(launchable)))]
             if mode == "batched":
-                b = Batch([p.job for p in group])
-                b.launch(RV_LAUNCH_BATCHED)
+                # One batch per sample depth class, so each runs the kernel
+                # variants tuned for its class.
+                bs = [Batch([p.job for p in part]) for part in
+                      ([p for p in group if p.sess.bit_depth == 8], [p for p in group if p.sess.bit_depth != 8]) if part]
+                for b in bs:
+                    b.launch(RV_LAUNCH_BATCHED)
                 for p in group:
                     p.launched = True
-                inflight.append((mark(), group, b))
+                inflight.append((mark(), group, bs))
                 batches += 1
                 batch_sizes.append(len(group))
             else:
@@ -310,7 +324,7 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
                     b = Batch([p.job])
                     b.launch(RV_LAUNCH_PLAIN)
                     p.launched = True
-                    inflight.append((mark(), [p], b))
+                    inflight.append((mark(), [p], [b]))
                     batches += 1
                     batch_sizes.append(1)
             continue
@@ -326,7 +340,7 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex):
     luma = sum(p.luma for s in sessions for p in s.pics if p.kind != "I")
     lat_ms = np.array(lat) * 1e3
     return {
-        "mode": mode,
+        "mode": mode + (" (prepared in advance)" if preload else ""),
         "pictures": total,
         "wall_s": round(wall, 4),
         "luma_msamples_per_s": round(luma / wall / 1e6, 1),
@@ -363,6 +377,9 @@ def experiment(n: int = 64, seed: int = cf.BASE_SEED, budget: int = 16 << 30, lo
         s.arrival = float(a)
     batched = run(sessions, "batched", budget=budget, upload_threads=threads)
     plain = run(sessions, "plain", budget=budget, upload_threads=threads)
+    # The device side alone: every work list prepared before the clock.
+    dev_b = run(sessions, "batched", upload_threads=threads, preload=True)
+    dev_p = run(sessions, "plain", upload_threads=threads, preload=True)
     return {
         "sessions": n,
         "mix": "uniform over 854x480 (coded 856), 1280x720, 1920x1080, 3840x2160; 70% Main 8-bit / 30% Main10; "
@@ -374,4 +391,6 @@ def experiment(n: int = 64, seed: int = cf.BASE_SEED, budget: int = 16 << 30, lo
         "batched": batched,
         "plain": plain,
         "batched_over_plain": round(batched["luma_msamples_per_s"] / plain["luma_msamples_per_s"], 3),
+        "device": {"batched": dev_b, "plain": dev_p,
+                   "batched_over_plain": round(dev_b["luma_msamples_per_s"] / dev_p["luma_msamples_per_s"], 3)},
     }

@@ -124,6 +124,17 @@ def test_every_picture_completes_in_dependency_order(monkeypatch, sessions, mode
         assert st["mean_batch"] == 1.0
 
 
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_preloaded_run_prepares_everything_first(monkeypatch, sessions, mode):
+    w = World()
+    install(monkeypatch, w, sessions)
+    st = serve.run(sessions, mode, upload_threads=3, preload=True)
+    assert not w.errors, w.errors[:5]
+    assert st["pictures"] == sum(len(s.pics) - 1 for s in sessions)
+    assert st["mode"].startswith(mode) and "prepared" in st["mode"]
+    assert w.closed == len(sessions) and not w.open
+
+
 def test_admission_respects_the_hbm_budget(monkeypatch, sessions):
     w = World()
     install(monkeypatch, w, sessions)
