@@ -87,6 +87,57 @@ def build_workload(rh, cf, streams: int):
     return pics, refs, sessions, jobs
 
 
+def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int):
+    """Configuration 2 (HEVC Main10 1080p B, explicit WP, intra with DST) or
+    4 (H.264 High 1080p B, implicit weights) as a batched workload: n pictures
+    of the configuration (the first is the configuration's own seed), one
+    graph launch per step, every plane checked against the oracle."""
+    codec, cfg, seed0 = cf.single(config)
+    w, h, bd = cfg["width"], cfg["height"], cfg.get("bit_depth", 8)
+    pics, refs, sessions, jobs = [], [], [], []
+    for k in range(n):
+        seed = seed0 if k == 0 else rh.subseed(seed0, 11, k, 0)
+        g = rh.GenPicture(codec, cfg, seed)
+        r = [rh.gen_planes(w, h, bd, cf.ref_seed(seed, slot)) for slot in range(2)]
+        s = rh.Session(codec, w, h, bd, 3)
+        for slot in range(2):
+            s.upload(slot, r[slot])
+        pics.append(g)
+        refs.append(r)
+        sessions.append(s)
+        jobs.append(s.prepare(g, 2))
+    batch = rh.Batch(jobs)
+    batch.launch(rh.RV_LAUNCH_GRAPH)
+    rh.sync()
+    ok = True
+    for g, r, s in zip(pics, refs, sessions):
+        got, want = s.download(2), oracle.reconstruct(codec, g.pic, r + [None], 0)
+        ok = ok and all(np.array_equal(got[c], want[c]) for c in range(3))
+    t = time_steps(rh, batch, rh.RV_LAUNCH_GRAPH, steps, warmup)
+    stage_ms = batch.profile(10)
+    nb = {"mc": 0, "tx": 0, "dbk": 0, "sao": 0}
+    for j in jobs:
+        for key, v in j.bytes().items():
+            nb[key] += v
+    stage_bytes = [nb["mc"], nb["tx"], nb["dbk"] / 2, nb["dbk"] / 2, nb["sao"]]
+    luma = n * cf.visible_luma(w, h)
+    out = {
+        "workload": f"{n} x config {config} ({'HEVC' if codec == rh.RV_CODEC_HEVC else 'H.264'} {w}x{h}, "
+                    f"{bd}-bit), batched, hipGraph replay",
+        "bit_exact": ok,
+        "ms_per_step": round(t * 1e3, 4),
+        "luma_msamples_per_s": round(luma / t / 1e6, 1) if ok else None,
+        "stage_ms": {k: round(v, 4) for k, v in zip(STAGES, stage_ms)},
+        "stage_gbs": {k: round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0
+                      for k, b, ms in zip(STAGES, stage_bytes, stage_ms)},
+    }
+    batch.close()
+    for j, s in zip(jobs, sessions):
+        j.close()
+        s.close()
+    return out
+
+
 def time_steps(rh, batch, mode, steps, warmup):
     for _ in range(warmup):
         batch.launch(mode)
@@ -108,6 +159,7 @@ def main() -> int:
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the 256 MiB copy probe once (counter calibration) and exit")
     ap.add_argument("--no-serve", action="store_true", help="skip the serve-loop measurement (config 5)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the batched configs 2 and 4")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -230,6 +282,13 @@ def main() -> int:
             "kernel_variants": {k: names[v] for k, names, v in zip(STAGES, rh.variant_names(), batch.variants())},
         },
     }
+    if not args.graph_only and not args.no_configs:
+        out["extra"]["configs"] = {}
+        for config in (2, 4):
+            r = measure_config(rh, cf, oracle, config, 8, max(10, args.steps // 4), 3)
+            log(f"[bench] config {config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms, "
+                f"{r['luma_msamples_per_s']} Msample/s; stages ms {r['stage_ms']}")
+            out["extra"]["configs"][str(config)] = r
     if not args.graph_only and not args.no_serve:
         from rivet_hip import serve
 
