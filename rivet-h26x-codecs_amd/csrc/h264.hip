@@ -3,11 +3,11 @@
 //   stage 0  intra prediction blocks, motion-compensated partitions,
 //            per-4x4 partition map
 //   stage 1  4x4 / 8x8 inverse transforms + residual add (one block per lane)
-//   stage 2  boundary strengths per macroblock; wavefront counters reset
-//   stage 3  deblocking as a macroblock wavefront: one workgroup per MB row
-//            (rows taken in dispatch order from a ticket), MB (x, y) after
-//            MB (x + 1, y - 1) through per-row progress counters — the
-//            raster order of the reference (src/h264/deblock.rs:617-813)
+//   stage 2  boundary strengths and filter thresholds per macroblock edge
+//   stage 3  deblocking as a macroblock wavefront inside one workgroup per
+//            picture: MB (x, y) after MB (x - 1, y) and MB (x + 1, y - 1),
+//            which gives the raster order's result (src/h264/deblock.rs:
+//            617-813); carried samples stay in LDS
 //   stage 4  (none: the picture is reconstructed in its DPB slot)
 //
 // Arithmetic restates src/dsp/h264.rs bit for bit (wrapping i16 transforms
@@ -417,6 +417,24 @@ __device__ __forceinline__ uint32_t partof(const H264PicDev& P, int bx, int by) 
     return P.partmap[(size_t)by * P.w4 + bx];
 }
 
+// The parameters of one edge with packed strengths `bs` and QP average `qa`
+// (ThrTable::get, src/h264/deblock.rs:311-344; filter_edge, :192-213):
+// x = alpha | beta << 8 | bS-4 flag << 16; y = one byte per 4-line
+// segment, 0 where the segment is left alone, else tC0 + 1 (1 under bS 4,
+// which byte 0 of the edge decides for all four).
+__device__ uint2 edge_words(const H264PicDev& P, uint32_t bs, int qa) {
+    if (bs == 0) return make_uint2(0, 0);
+    const int ia = clampi(qa + P.offset_a, 0, 51), ib = clampi(qa + P.offset_b, 0, 51);
+    const bool bs4 = (bs & 0xFF) == 4;
+    uint32_t segs = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int s = (bs >> (8 * k)) & 3;
+        const uint32_t v = bs4 ? 1u : s ? (uint32_t)kTc0[ia][s - 1] + 1 : 0u;
+        segs |= v << (8 * k);
+    }
+    return make_uint2((uint32_t)kAlpha[ia] | (uint32_t)kBeta[ib] << 8 | (bs4 ? 1u << 16 : 0u), segs);
+}
+
 __device__ void mb_bs(const H264PicDev& P, int addr) {
     const int mbx = addr % P.mbw, mby = addr / P.mbw;
     const rv_h264_mb m = P.mbs[addr];
@@ -474,10 +492,31 @@ __device__ void mb_bs(const H264PicDev& P, int addr) {
                 }
         }
     }
+    // The edge parameters of this macroblock, one word per edge and segment.
+    const rv_h264_mb ml = P.mbs[fl ? addr - 1 : addr], ma = P.mbs[ft ? addr - P.mbw : addr];
+    const bool odd = !m.transform_8x8;
+    uint32_t w[kH264DbkWords];
     for (int e = 0; e < 4; ++e) {
-        P.bsv[addr * 4 + e] = bv[e];
-        P.bsh[addr * 4 + e] = bh[e];
+        const bool on = e == 0 || (e & 1) == 0 || odd;  // 8x8 transform: no luma edges 1, 3
+        const uint2 v = edge_words(P, on ? bv[e] : 0u, e == 0 ? (ml.qp + m.qp + 1) >> 1 : m.qp);
+        const uint2 h = edge_words(P, on ? bh[e] : 0u, e == 0 ? (ma.qp + m.qp + 1) >> 1 : m.qp);
+        for (int k = 0; k < 4; ++k) {
+            w[k * 4 + e] = v.x | ((v.y >> (8 * k)) & 0xFF) << 24;
+            w[16 + k * 4 + e] = h.x | ((h.y >> (8 * k)) & 0xFF) << 24;
+        }
     }
+    for (int c = 0; c < 2; ++c)
+        for (int e = 0; e < 2; ++e) {
+            // Chroma edges 0 and 4 take the strengths of luma edges 0 and 8.
+            const uint2 v = edge_words(P, bv[2 * e], e == 0 ? (ml.qpc[c] + m.qpc[c] + 1) >> 1 : m.qpc[c]);
+            const uint2 h = edge_words(P, bh[2 * e], e == 0 ? (ma.qpc[c] + m.qpc[c] + 1) >> 1 : m.qpc[c]);
+            for (int k = 0; k < 4; ++k) {
+                w[32 + c * 8 + k * 2 + e] = v.x | ((v.y >> (8 * k)) & 0xFF) << 24;
+                w[48 + c * 8 + k * 2 + e] = h.x | ((h.y >> (8 * k)) & 0xFF) << 24;
+            }
+        }
+    uint4* dst = (uint4*)(P.dbk + (size_t)addr * kH264DbkWords);
+    for (int k = 0; k < kH264DbkWords / 4; ++k) dst[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
 __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
@@ -486,8 +525,6 @@ __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
     const int p = find_pic(pre, b.n, it);
     const H264PicDev& P = *b.h264[p];
     const int i = (it - pre[p]) * 64 + threadIdx.x;
-    // Wavefront counters: ticket (entry 0) and per-row progress (1..mbh).
-    if (i <= P.mbh) P.bsv[(size_t)P.n_mbs * 4 + i] = 0;
     if (i < P.n_mbs) mb_bs(P, i);
 }
 
@@ -542,108 +579,312 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
     }
 }
 
-// One line of one edge: `seg` is the line's segment (bS byte), `qa` the
-// QP average of the edge (ThrTable::get, :311-344).
-__device__ __forceinline__ void edge_line(const H264PicDev& P, uint8_t* d, long across, uint32_t bs, int seg, int qa,
-                                          bool chroma) {
-    if (bs == 0) return;
-    const int ia = clampi(qa + P.offset_a, 0, 51), ib = clampi(qa + P.offset_b, 0, 51);
-    const int alpha = kAlpha[ia], beta = kBeta[ib];
-    const bool bs4 = (bs & 0xFF) == 4;  // byte 0 decides (filter_edge, :192-213)
-    int tc0 = 0;
-    if (!bs4) {
-        int s = (bs >> (8 * seg)) & 3;
-        if (s == 0) return;  // tC0 = -1: leave the line alone
-        tc0 = kTc0[ia][s - 1];
-    }
-    line_filter(d, across, bs4, tc0, alpha, beta, chroma);
+// Stage 3 filters each picture in one workgroup of 1024 threads: 64 slots
+// of 16 lanes, slot s taking the macroblock rows y = s (mod 64). Row y
+// starts `lag` steps after row y - 1 and takes one macroblock per step, so
+// MB (x, y) runs after MB (x + 1, y - 1) and MB (x - 1, y) — the two it
+// reads — with one barrier per step. lag >= 2, and 63 * lag >= mbw so a
+// slot's next row starts after the row below it has read all its strips.
+//
+// Dependencies through the samples. MB (x, y) reads its own unfiltered
+// samples, columns 12..15 of MB (x - 1, y) as that MB left them, and rows
+// 12..15 of MB (x, y - 1) after MB (x + 1, y - 1) filtered its left edge.
+// It changes columns 13..15 of MB (x - 1, y), rows 13..15 of MB (x, y - 1)
+// and itself. So every sample is read once from the picture (before any
+// filtering of it) and written once (when final): its macroblock's rows
+// 0..12 x columns 0..11 by the macroblock itself, columns 12..15 by the
+// next macroblock of the row (its left edge), rows 13..15 by the
+// macroblock below (its top edge) — or sooner at the right and bottom
+// borders. Chroma likewise with columns / rows 7 and 6..7. The carried
+// samples stay on chip:
+//   tile  the macroblock after its vertical edges (luma 16 x 16, Cb and Cr
+//         8 x 8), rewritten after the horizontal edges in the columns the
+//         next macroblock's left edge reads (luma 12..15, chroma 6..7);
+//   ring  per macroblock of the slot's row, rows 12..15 of each luma
+//         column and rows 6..7 of each chroma column, final but for the
+//         row below's top edge, which reads entry x at its step x.
+// A lane filters luma row r and chroma row r & 7 of Cb (r < 8) or Cr
+// across the vertical edges, then luma column r and chroma column r & 7
+// across the horizontal edges, the samples of a line in registers.
+
+constexpr int kDbkSlots = 64;
+constexpr int kDbkTile = 96;  // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
+constexpr int kDbkRing = 24;  // words per entry: 16 luma columns, 16 chroma (16-bit)
+
+__host__ __device__ inline int dbk_lag(int mbw) {
+    return max(2, (mbw + kDbkSlots - 2) / (kDbkSlots - 1));
 }
 
-__device__ void deblock_mb(const H264PicDev& P, int mbx, int mby, int tid) {
-    const int addr = mby * P.mbw + mbx;
-    const rv_h264_mb m = P.mbs[addr];
-    const rv_h264_mb ml = P.mbs[mbx > 0 ? addr - 1 : addr];
-    const rv_h264_mb ma = P.mbs[mby > 0 ? addr - P.mbw : addr];
-    const uint32_t* bv = P.bsv + addr * 4;
-    const uint32_t* bh = P.bsh + addr * 4;
-    const bool odd = !m.transform_8x8;
-    // Luma: lanes 0..15 are the lines; chroma: 16..23 Cb, 24..31 Cr.
-    if (tid < 16) {
-        uint8_t* Y = (uint8_t*)P.cur[0];
-        const int pitch = P.pitch[0];
-        uint8_t* row = Y + (size_t)(mby * 16 + tid) * pitch + mbx * 16;
-        for (int e = 0; e < 4; ++e) {
-            if (e > 0 && (e & 1) && !odd) continue;
-            int qa = e == 0 ? ((ml.qp + m.qp + 1) >> 1) : m.qp;
-            edge_line(P, row + e * 4, 1, bv[e], tid >> 2, qa, false);
-        }
-    } else if (tid < 32) {
-        const int c = tid < 24 ? 1 : 2, line = tid & 7;
-        uint8_t* pl = (uint8_t*)P.cur[c];
-        uint8_t* row = pl + (size_t)(mby * 8 + line) * P.pitch[c] + mbx * 8;
-        int qa0 = (ml.qpc[c - 1] + m.qpc[c - 1] + 1) >> 1;
-        edge_line(P, row, 1, bv[0], line >> 1, qa0, true);
-        edge_line(P, row + 4, 1, bv[2], line >> 1, m.qpc[c - 1], true);
-    }
-    __syncthreads();
-    if (tid < 16) {
-        uint8_t* Y = (uint8_t*)P.cur[0];
-        const long pitch = P.pitch[0];
-        uint8_t* col = Y + (size_t)(mby * 16) * pitch + mbx * 16 + tid;
-        for (int e = 0; e < 4; ++e) {
-            if (e > 0 && (e & 1) && !odd) continue;
-            int qa = e == 0 ? ((ma.qp + m.qp + 1) >> 1) : m.qp;
-            edge_line(P, col + e * 4 * pitch, pitch, bh[e], tid >> 2, qa, false);
-        }
-    } else if (tid < 32) {
-        const int c = tid < 24 ? 1 : 2, line = tid & 7;
-        uint8_t* pl = (uint8_t*)P.cur[c];
-        const long pitch = P.pitch[c];
-        uint8_t* col = pl + (size_t)(mby * 8) * pitch + mbx * 8 + line;
-        int qa0 = (ma.qpc[c - 1] + m.qpc[c - 1] + 1) >> 1;
-        edge_line(P, col, pitch, bh[0], line >> 1, qa0, true);
-        edge_line(P, col + 4 * pitch, pitch, bh[2], line >> 1, m.qpc[c - 1], true);
-    }
-    __syncthreads();
+__device__ __forceinline__ int absd(int a, int b) { return abs(a - b); }
+
+__device__ __forceinline__ uint32_t pk4(int a, int b, int c, int d) {
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x04000c0cu);
+    return lo | hi;
 }
 
-// Spin bound: far beyond any real wait (a row waits for one macroblock of
-// the row above); a run that reaches it abandons the row instead of hanging.
-constexpr long kSpinLimit = 1L << 26;
+template <int O, int N> __device__ __forceinline__ void unpk4(int (&v)[N], uint32_t w) {
+    v[O] = w & 0xFF;
+    v[O + 1] = (w >> 8) & 0xFF;
+    v[O + 2] = (w >> 16) & 0xFF;
+    v[O + 3] = w >> 24;
+}
 
-__global__ void __launch_bounds__(64) h264_stage3(BatchDev b) {
+// deblock_line (src/dsp/h264.rs:603-656) on one line held in registers:
+// p3..q3 = v[Q - 4 .. Q + 3]. `a` is the line's parameter word (alpha |
+// beta << 8 | bS-4 flag << 16 | tcb << 24; tcb 0: leave the line alone,
+// else tC0 + 1). kLuma: the luma filter (p2, q2 read, p2..q2 written);
+// else chroma (p1..q1 read, p0, q0 written). kBs4: a macroblock edge,
+// which may carry bS 4.
+template <int Q, bool kLuma, bool kBs4, int N>
+__device__ __forceinline__ void dbk_line(int (&v)[N], uint32_t a) {
+    const int alpha = a & 0xFF, beta = (a >> 8) & 0xFF;
+    const uint32_t tcb = a >> 24;
+    const int p0 = v[Q - 1], p1 = v[Q - 2], q0 = v[Q], q1 = v[Q + 1];
+    const int d0 = absd(p0, q0);
+    const bool on = tcb != 0 && d0 < alpha && absd(p1, p0) < beta && absd(q1, q0) < beta;
+    const int tc0 = (int)tcb - 1;
+    const int delta0 = (((q0 - p0) << 2) + (p1 - q1) + 4) >> 3;
+    if constexpr (kLuma) {
+        const int p2 = v[Q - 3], q2 = v[Q + 2];
+        const bool lp = absd(p2, p0) < beta, lq = absd(q2, q0) < beta;
+        const int tc = tc0 + (int)lp + (int)lq;
+        const int delta = clampi(delta0, -tc, tc);
+        int P0 = clampi(p0 + delta, 0, 255), Q0 = clampi(q0 - delta, 0, 255);
+        const int avg = (p0 + q0 + 1) >> 1;
+        int P1 = lp ? p1 + clampi((p2 + avg - (p1 << 1)) >> 1, -tc0, tc0) : p1;
+        int Q1 = lq ? q1 + clampi((q2 + avg - (q1 << 1)) >> 1, -tc0, tc0) : q1;
+        int P2 = p2, Q2 = q2;
+        if constexpr (kBs4) {
+            if ((a >> 16) & 1) {
+                const int p3 = v[Q - 4], q3 = v[Q + 3];
+                const bool strong = d0 < ((alpha >> 2) + 2);
+                if (lp && strong) {
+                    P0 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+                    P1 = (p2 + p1 + p0 + q0 + 2) >> 2;
+                    P2 = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
+                } else {
+                    P0 = (2 * p1 + p0 + q1 + 2) >> 2;
+                    P1 = p1;
+                }
+                if (lq && strong) {
+                    Q0 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+                    Q1 = (p0 + q0 + q1 + q2 + 2) >> 2;
+                    Q2 = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
+                } else {
+                    Q0 = (2 * q1 + q0 + p1 + 2) >> 2;
+                    Q1 = q1;
+                }
+            }
+        }
+        if (on) {
+            v[Q - 3] = P2, v[Q - 2] = P1, v[Q - 1] = P0;
+            v[Q] = Q0, v[Q + 1] = Q1, v[Q + 2] = Q2;
+        }
+    } else {
+        const int tc = tc0 + 1;
+        const int delta = clampi(delta0, -tc, tc);
+        int P0 = clampi(p0 + delta, 0, 255), Q0 = clampi(q0 - delta, 0, 255);
+        if constexpr (kBs4) {
+            if ((a >> 16) & 1) {
+                P0 = (2 * p1 + p0 + q1 + 2) >> 2;
+                Q0 = (2 * q1 + q0 + p1 + 2) >> 2;
+            }
+        }
+        if (on) v[Q - 1] = P0, v[Q] = Q0;
+    }
+}
+
+// One lane's inputs for one macroblock, fetched a step ahead: its samples
+// and the parameter words of its lines' segments.
+struct DbkIn {
+    uint4 y;       // luma row r
+    uint2 c;       // chroma row r & 7 of the lane's plane
+    uint4 lv, lh;  // luma vertical, horizontal edges 0..3
+    uint2 cv, ch;  // chroma vertical, horizontal edges 0, 4
+};
+
+// Global-memory pointers: pointers read from a descriptor are generic, and
+// generic (flat) accesses count against the LDS wait counter too.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p) { return (gptr<T>)p; }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 gld4(gptr<const void> p) {
+    const u32x4 v = *(gptr<const u32x4>)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gld2(gptr<const void> p) {
+    const u32x2 v = *(gptr<const u32x2>)p;
+    return make_uint2(v.x, v.y);
+}
+
+// The picture fields stage 3 uses, read once: its byte stores could alias
+// the descriptor as far as the compiler knows.
+struct DbkPic {
+    gptr<uint8_t> y;
+    gptr<uint8_t> c;  // the lane's chroma plane
+    size_t py, pc;
+    gptr<const uint32_t> prm;
+    int mbw, mbh;
+};
+
+__device__ __forceinline__ void dbk_fetch(const DbkPic& D, int x, int y, int r, DbkIn& in) {
+    const int pl = r >> 3, rr = r & 7;
+    in.y = gld4(D.y + (size_t)(16 * y + r) * D.py + 16 * x);
+    in.c = gld2(D.c + (size_t)(8 * y + rr) * D.pc + 8 * x);
+    const gptr<const uint32_t> pb = D.prm + ((size_t)y * D.mbw + x) * kH264DbkWords;
+    const int sl = r >> 2, sc = rr >> 1;  // segments of the luma and chroma lines
+    in.lv = gld4(pb + sl * 4), in.lh = gld4(pb + 16 + sl * 4);
+    in.cv = gld2(pb + 32 + pl * 8 + sc * 2), in.ch = gld2(pb + 48 + pl * 8 + sc * 2);
+}
+
+// Workgroup barrier that orders LDS only: the next step's prefetched
+// loads and this step's picture stores stay in flight across it (every
+// picture sample is stored once and read only before it is stored).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Where slot `slot` stands at step t = q * lag + rem: row y = slot + 64k,
+// the latest of its rows started (y * lag <= t), column x = t - y * lag.
+__device__ __forceinline__ bool dbk_at(int q, int rem, int slot, int lag, int mbw, int mbh, int& x, int& y) {
+    if (q < slot) return false;
+    y = slot + ((q - slot) & ~(kDbkSlots - 1));
+    x = (q - y) * lag + rem;
+    return y < mbh && x < mbw;
+}
+
+__global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
+    extern __shared__ uint32_t dl[];
     const int32_t* pre = b.prefix + 3 * (b.n + 1);
-    const int it = blockIdx.x;
-    const int p = find_pic(pre, b.n, it);
+    const int p = find_pic(pre, b.n, (int)blockIdx.x);
     const H264PicDev& P = *b.h264[p];
     if (!P.deblock) return;
-    const int tid = threadIdx.x;
-    uint32_t* ctr = P.bsv + (size_t)P.n_mbs * 4;  // [0] ticket, [1 + y] progress
-    __shared__ int row_s;
-    if (tid == 0) row_s = (int)atomicAdd(&ctr[0], 1u);
-    __syncthreads();
-    const int y = row_s;
-    if (y >= P.mbh) return;
-    __shared__ int ok;
-    for (int x = 0; x < P.mbw; ++x) {
-        if (y > 0) {
-            const uint32_t need = (uint32_t)min(x + 2, P.mbw);
-            if (tid == 0) {
-                long spins = 0;
-                while (__hip_atomic_load(&ctr[y], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need &&
-                       ++spins < kSpinLimit)
-                    __builtin_amdgcn_s_sleep(2);
-                ok = spins < kSpinLimit;
+    const int slot = (int)threadIdx.x >> 4, r = (int)threadIdx.x & 15;
+    const int pl = r >> 3, rr = r & 7;  // chroma plane, row / column of this lane
+    const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (size_t)P.pitch[0],
+                   (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh};
+    const int mbw = D.mbw, mbh = D.mbh, lag = dbk_lag(mbw), C = lag + 1;
+    const int per = kDbkTile + kDbkRing * C;
+    uint32_t* const tile = dl + slot * per;
+    uint8_t* const tb = (uint8_t*)tile;
+    uint32_t* const ring = tile + kDbkTile;
+    const uint32_t* const up = dl + ((slot + kDbkSlots - 1) & (kDbkSlots - 1)) * per + kDbkTile;
+    const gptr<uint8_t> Y = D.y;
+    const gptr<uint8_t> Cp = D.c;
+    const size_t py = D.py, pc = D.pc;
+    const int steps = (mbh - 1) * lag + mbw;
+
+    int q = 0, rem = 0;  // the next step is q * lag + rem
+    int x = 0, y = 0, ex = 0;
+    bool act = dbk_at(q, rem, slot, lag, mbw, mbh, x, y);
+    DbkIn ba, bb;  // this step's inputs and the next step's, alternating
+    if (act) dbk_fetch(D, x, y, r, ba);
+    // One step: filter the macroblock `in` holds, fetch the next into `nx`.
+    auto step = [&](const DbkIn& in, DbkIn& nx) __attribute__((always_inline)) {
+        // Wait here for this step's inputs (fetched a step ago) and the last
+        // step's stores, before the next fetch: later waits then never hold
+        // the next step's loads (the vector memory counter retires in order).
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        const int cx = x, cy = y, cex = ex;
+        const bool cact = act;
+        if (++rem == lag) rem = 0, ++q;
+        act = dbk_at(q, rem, slot, lag, mbw, mbh, x, y);
+        if (act) {
+            ex = x == 0 ? 0 : (ex + 1 == C ? 0 : ex + 1);  // x % C
+            dbk_fetch(D, x, y, r, nx);
+        }
+        if (cact) {
+            const bool lastc = cx == mbw - 1, lastr = cy == mbh - 1;
+            // Vertical edges: luma row r, chroma row rr.
+            int v[20], w[10];
+            unpk4<0>(v, tile[r * 4 + 3]);  // previous macroblock, columns 12..15
+            unpk4<4>(v, in.y.x), unpk4<8>(v, in.y.y), unpk4<12>(v, in.y.z), unpk4<16>(v, in.y.w);
+            {
+                const uint32_t lc = tile[64 + pl * 16 + rr * 2 + 1];  // columns 4..7
+                w[0] = (lc >> 16) & 0xFF, w[1] = lc >> 24;
+                unpk4<2>(w, in.c.x), unpk4<6>(w, in.c.y);
             }
-            __syncthreads();
-            if (!ok) return;
-            __threadfence();
+            dbk_line<4, true, true>(v, in.lv.x);
+            dbk_line<8, true, false>(v, in.lv.y);
+            dbk_line<12, true, false>(v, in.lv.z);
+            dbk_line<16, true, false>(v, in.lv.w);
+            dbk_line<2, false, true>(w, in.cv.x);
+            dbk_line<6, false, false>(w, in.cv.y);
+            *(uint4*)&tile[r * 4] =
+                make_uint4(pk4(v[4], v[5], v[6], v[7]), pk4(v[8], v[9], v[10], v[11]),
+                           pk4(v[12], v[13], v[14], v[15]), pk4(v[16], v[17], v[18], v[19]));
+            *(uint2*)&tile[64 + pl * 16 + rr * 2] = make_uint2(pk4(w[2], w[3], w[4], w[5]), pk4(w[6], w[7], w[8], w[9]));
+            if (cx > 0) {
+                // The left neighbour's columns 12..15 (chroma 7): final but
+                // for rows 13..15 (chroma 7) outside the last row.
+                if (r <= 12 || lastr)
+                    *(gptr<uint32_t>)(Y + (size_t)(16 * cy + r) * py + 16 * cx - 4) = pk4(v[0], v[1], v[2], v[3]);
+                if (rr <= 6 || lastr) Cp[(size_t)(8 * cy + rr) * pc + 8 * cx - 1] = (uint8_t)w[1];
+                if (!lastr) {
+                    uint8_t* e = (uint8_t*)(ring + (cex == 0 ? C - 1 : cex - 1) * kDbkRing);
+                    if (r >= 12)
+                        for (int k = 0; k < 4; ++k) e[(12 + k) * 4 + r - 12] = (uint8_t)v[k];
+                    if (rr >= 6) e[64 + (pl * 8 + 7) * 2 + rr - 6] = (uint8_t)w[1];
+                }
+            }
+            // The slot's lanes are one wave's: LDS order within the wave.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            // Horizontal edges: luma column r, chroma column rr; rows above
+            // from the ring of the row above (unused in row 0: no top edge).
+            int h[20], g[10];
+            const uint32_t* ue = up + cex * kDbkRing;
+            unpk4<0>(h, ue[r]);
+            for (int k = 0; k < 16; ++k) h[4 + k] = tb[k * 16 + r];
+            {
+                const uint32_t u = ((const uint16_t*)(ue + 16))[pl * 8 + rr];
+                g[0] = u & 0xFF, g[1] = u >> 8;
+            }
+            for (int k = 0; k < 8; ++k) g[2 + k] = tb[256 + pl * 64 + k * 8 + rr];
+            dbk_line<4, true, true>(h, in.lh.x);
+            dbk_line<8, true, false>(h, in.lh.y);
+            dbk_line<12, true, false>(h, in.lh.z);
+            dbk_line<16, true, false>(h, in.lh.w);
+            dbk_line<2, false, true>(g, in.ch.x);
+            dbk_line<6, false, false>(g, in.ch.y);
+            gptr<uint8_t> col = Y + (size_t)(16 * cy) * py + 16 * cx + r;
+            if (cy > 0) {  // rows 13..15 of the macroblock above are final
+                col[-3 * (long)py] = (uint8_t)h[1];
+                col[-2 * (long)py] = (uint8_t)h[2];
+                col[-(long)py] = (uint8_t)h[3];
+            }
+            if (r <= 11 || lastc) {
+                for (int k = 0; k < 13; ++k) col[k * py] = (uint8_t)h[4 + k];
+                if (lastr) {
+                    for (int k = 13; k < 16; ++k) col[k * py] = (uint8_t)h[4 + k];
+                } else {
+                    ring[cex * kDbkRing + r] = pk4(h[16], h[17], h[18], h[19]);
+                }
+            }
+            gptr<uint8_t> ccol = Cp + (size_t)(8 * cy) * pc + 8 * cx + rr;
+            if (cy > 0) ccol[-(long)pc] = (uint8_t)g[1];
+            if (rr <= 6 || lastc) {
+                for (int k = 0; k < 7; ++k) ccol[k * pc] = (uint8_t)g[2 + k];
+                if (lastr) ccol[7 * pc] = (uint8_t)g[9];
+                else ((uint16_t*)(ring + cex * kDbkRing + 16))[pl * 8 + rr] = (uint16_t)(g[8] | g[9] << 8);
+            }
+            if (!lastc) {  // the columns the next macroblock's left edge reads
+                if (r >= 12)
+                    for (int k = 0; k < 16; ++k) tb[k * 16 + r] = (uint8_t)h[4 + k];
+                if (rr >= 6)
+                    for (int k = 0; k < 8; ++k) tb[256 + pl * 64 + k * 8 + rr] = (uint8_t)g[2 + k];
+            }
         }
-        deblock_mb(P, x, y, tid);
-        if (tid == 0) {
-            __threadfence();
-            __hip_atomic_store(&ctr[1 + y], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        lds_barrier();
+    };
+    for (int t = 0; t < steps; t += 2) {
+        step(ba, bb);
+        if (t + 1 < steps) step(bb, ba);
     }
 }
 
@@ -744,8 +985,12 @@ void h264_stage_items(H264PicDev& P) {
     P.items[0] = P.n_mbs + P.n_parts;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
     P.items[2] = (P.n_mbs + 1 + 63) / 64;
-    P.items[3] = P.deblock ? P.mbh : 0;
+    P.items[3] = P.deblock ? 1 : 0;  // one workgroup per picture
     P.items[4] = 0;
+}
+
+int h264_deblock_lds(int mbw) {
+    return kDbkSlots * (kDbkTile + kDbkRing * (dbk_lag(mbw) + 1)) * (int)sizeof(uint32_t);
 }
 
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
@@ -754,7 +999,7 @@ hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
         case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
         case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(64), 0, st, b); break;
+        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(1024), h264_deblock_lds(b.mbw_max), st, b); break;
         default: return hipSuccess;
     }
     return hipGetLastError();

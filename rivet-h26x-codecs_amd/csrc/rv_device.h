@@ -81,6 +81,13 @@ struct HevcPicDev {
     int32_t items[5];
 };
 
+// Deblocking parameters of one H.264 macroblock: per edge and 4-line
+// segment one word alpha | beta << 8 | bS-4 flag << 16 | s << 24, s = 0
+// where the segment is left alone, else tC0 + 1. Luma vertical edges as
+// [segment][edge 0..3], luma horizontal likewise, then chroma vertical as
+// [plane][segment][edge 0, 4], chroma horizontal likewise.
+constexpr int kH264DbkWords = 64;
+
 // Per-picture descriptor of an H.264 job.
 struct H264PicDev {
     int32_t w, h, mbw, mbh, w4, h4;
@@ -102,8 +109,7 @@ struct H264PicDev {
     const int32_t* coefs;
     const uint16_t* intra_pred;
     uint32_t* partmap;   // per 4x4: partition index (0xFFFFFFFF = intra)
-    uint32_t* bsv;       // per MB x 4 edges: packed vertical bS (byte/segment)
-    uint32_t* bsh;       // per MB x 4 edges: packed horizontal bS
+    uint32_t* dbk;       // per MB: kH264DbkWords words of edge parameters
     int32_t items[5];
 };
 
@@ -168,6 +174,7 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
+    int32_t mbw_max;                // H.264: widest picture, in macroblocks
 };
 
 }  // namespace rv
