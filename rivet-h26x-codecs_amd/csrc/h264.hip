@@ -615,7 +615,8 @@ __host__ __device__ inline int dbk_lag(int mbw) {
     return max(2, (mbw + kDbkSlots - 2) / (kDbkSlots - 1));
 }
 
-__device__ __forceinline__ int absd(int a, int b) { return abs(a - b); }
+// |a - b| of two samples (0..255) in one v_sad_u8.
+__device__ __forceinline__ int absd(int a, int b) { return (int)__builtin_amdgcn_sad_u8((uint32_t)a, (uint32_t)b, 0u); }
 
 __device__ __forceinline__ uint32_t pk4(int a, int b, int c, int d) {
     const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
@@ -722,16 +723,17 @@ __device__ __forceinline__ uint2 gld2(gptr<const void> p) {
 // The picture fields stage 3 uses, read once: its byte stores could alias
 // the descriptor as far as the compiler knows.
 struct DbkPic {
-    gptr<uint8_t> y;
+    gptr<uint8_t> y;  // uniform: luma addresses are 32-bit offsets from it
     gptr<uint8_t> c;  // the lane's chroma plane
-    size_t py, pc;
+    uint32_t py;
+    size_t pc;
     gptr<const uint32_t> prm;
     int mbw, mbh;
 };
 
 __device__ __forceinline__ void dbk_fetch(const DbkPic& D, int x, int y, int r, DbkIn& in) {
     const int pl = r >> 3, rr = r & 7;
-    in.y = gld4(D.y + (size_t)(16 * y + r) * D.py + 16 * x);
+    in.y = gld4(D.y + ((uint32_t)(16 * y + r) * D.py + 16 * x));
     in.c = gld2(D.c + (size_t)(8 * y + rr) * D.pc + 8 * x);
     const gptr<const uint32_t> pb = D.prm + ((size_t)y * D.mbw + x) * kH264DbkWords;
     const int sl = r >> 2, sc = rr >> 1;  // segments of the luma and chroma lines
@@ -765,7 +767,7 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
     if (!P.deblock) return;
     const int slot = (int)threadIdx.x >> 4, r = (int)threadIdx.x & 15;
     const int pl = r >> 3, rr = r & 7;  // chroma plane, row / column of this lane
-    const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (size_t)P.pitch[0],
+    const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
                    (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh};
     const int mbw = D.mbw, mbh = D.mbh, lag = dbk_lag(mbw), C = lag + 1;
     const int per = kDbkTile + kDbkRing * C;
@@ -775,7 +777,8 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
     const uint32_t* const up = dl + ((slot + kDbkSlots - 1) & (kDbkSlots - 1)) * per + kDbkTile;
     const gptr<uint8_t> Y = D.y;
     const gptr<uint8_t> Cp = D.c;
-    const size_t py = D.py, pc = D.pc;
+    const uint32_t py = D.py;
+    const size_t pc = D.pc;
     const int steps = (mbh - 1) * lag + mbw;
 
     int q = 0, rem = 0;  // the next step is q * lag + rem
@@ -822,7 +825,7 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
                 // The left neighbour's columns 12..15 (chroma 7): final but
                 // for rows 13..15 (chroma 7) outside the last row.
                 if (r <= 12 || lastr)
-                    *(gptr<uint32_t>)(Y + (size_t)(16 * cy + r) * py + 16 * cx - 4) = pk4(v[0], v[1], v[2], v[3]);
+                    *(gptr<uint32_t>)(Y + ((uint32_t)(16 * cy + r) * py + 16 * cx - 4)) = pk4(v[0], v[1], v[2], v[3]);
                 if (rr <= 6 || lastr) Cp[(size_t)(8 * cy + rr) * pc + 8 * cx - 1] = (uint8_t)w[1];
                 if (!lastr) {
                     uint8_t* e = (uint8_t*)(ring + (cex == 0 ? C - 1 : cex - 1) * kDbkRing);
@@ -852,16 +855,16 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
             dbk_line<16, true, false>(h, in.lh.w);
             dbk_line<2, false, true>(g, in.ch.x);
             dbk_line<6, false, false>(g, in.ch.y);
-            gptr<uint8_t> col = Y + (size_t)(16 * cy) * py + 16 * cx + r;
+            const uint32_t col = (uint32_t)(16 * cy) * py + 16 * cx + r;
             if (cy > 0) {  // rows 13..15 of the macroblock above are final
-                col[-3 * (long)py] = (uint8_t)h[1];
-                col[-2 * (long)py] = (uint8_t)h[2];
-                col[-(long)py] = (uint8_t)h[3];
+                Y[col - 3 * py] = (uint8_t)h[1];
+                Y[col - 2 * py] = (uint8_t)h[2];
+                Y[col - py] = (uint8_t)h[3];
             }
             if (r <= 11 || lastc) {
-                for (int k = 0; k < 13; ++k) col[k * py] = (uint8_t)h[4 + k];
+                for (int k = 0; k < 13; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
                 if (lastr) {
-                    for (int k = 13; k < 16; ++k) col[k * py] = (uint8_t)h[4 + k];
+                    for (int k = 13; k < 16; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
                 } else {
                     ring[cex * kDbkRing + r] = pk4(h[16], h[17], h[18], h[19]);
                 }
