@@ -525,6 +525,8 @@ __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
     const int p = find_pic(pre, b.n, it);
     const H264PicDev& P = *b.h264[p];
     const int i = (it - pre[p]) * 64 + threadIdx.x;
+    // The deblocking exchange and band ticket start at zero every launch.
+    for (size_t k = i; k < (size_t)P.n_xch; k += (size_t)P.items[2] * 64) P.xch[k] = 0;
     if (i < P.n_mbs) mb_bs(P, i);
 }
 
@@ -579,12 +581,14 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
     }
 }
 
-// Stage 3 filters each picture in one workgroup of 1024 threads: 64 slots
-// of 16 lanes, slot s taking the macroblock rows y = s (mod 64). Row y
-// starts `lag` steps after row y - 1 and takes one macroblock per step, so
-// MB (x, y) runs after MB (x + 1, y - 1) and MB (x - 1, y) — the two it
-// reads — with one barrier per step. lag >= 2, and 63 * lag >= mbw so a
-// slot's next row starts after the row below it has read all its strips.
+// Stage 3 cuts each picture into bands of 16 macroblock rows, one
+// workgroup of 256 threads per band: 16 slots of 16 lanes, slot s filtering
+// row 16 * band + s. Row s of a band starts 2 steps after row s - 1 and
+// takes one macroblock per step, so MB (x, y) runs after MB (x + 1, y - 1)
+// and MB (x - 1, y) — the two it reads — with one barrier per step. Bands
+// take their index from a per-picture ticket, so a band's predecessor is
+// always running. The first row of a band takes the strips of the row
+// above from the band before through the exchange below.
 //
 // Dependencies through the samples. MB (x, y) reads its own unfiltered
 // samples, columns 12..15 of MB (x - 1, y) as that MB left them, and rows
@@ -602,18 +606,27 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 //         next macroblock's left edge reads (luma 12..15, chroma 6..7);
 //   ring  per macroblock of the slot's row, rows 12..15 of each luma
 //         column and rows 6..7 of each chroma column, final but for the
-//         row below's top edge, which reads entry x at its step x.
+//         row below's top edge, which reads entry x at its step x (3
+//         entries: the row below runs 2 macroblocks behind).
+//   exchange  the last row of a band publishes each ring entry, once
+//         complete, as 24 64-bit words (data | tag << 32, tag = x + 1),
+//         relaxed atomics at device scope: each word is coherent on its own,
+//         so the first row of the next band, which polls its words until the
+//         tag matches, needs no cache-wide release or acquire. The strength
+//         stage zeroes the exchange and the ticket before every launch.
 // A lane filters luma row r and chroma row r & 7 of Cb (r < 8) or Cr
 // across the vertical edges, then luma column r and chroma column r & 7
 // across the horizontal edges, the samples of a line in registers.
 
-constexpr int kDbkSlots = 64;
-constexpr int kDbkTile = 96;  // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
-constexpr int kDbkRing = 24;  // words per entry: 16 luma columns, 16 chroma (16-bit)
-
-__host__ __device__ inline int dbk_lag(int mbw) {
-    return max(2, (mbw + kDbkSlots - 2) / (kDbkSlots - 1));
-}
+constexpr int kDbkSlots = 16;  // rows per band
+constexpr int kDbkTile = 96;   // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
+constexpr int kDbkRing = 24;   // words per entry: 16 luma columns, 16 chroma (16-bit)
+constexpr int kDbkRingN = 3;   // entries per slot
+constexpr int kDbkPer = kDbkTile + kDbkRing * kDbkRingN;
+// Poll bound of the exchange: far beyond any real wait (the band before is
+// running and publishes within a few steps); a poll that reaches it goes on
+// with what it read rather than hang the workgroup.
+constexpr int kDbkSpin = 1 << 18;
 
 // |a - b| of two samples (0..255) in one v_sad_u8.
 __device__ __forceinline__ int absd(int a, int b) { return (int)__builtin_amdgcn_sad_u8((uint32_t)a, (uint32_t)b, 0u); }
@@ -697,12 +710,15 @@ __device__ __forceinline__ void dbk_line(int (&v)[N], uint32_t a) {
 }
 
 // One lane's inputs for one macroblock, fetched a step ahead: its samples
-// and the parameter words of its lines' segments.
+// and the parameter words of its lines' segments; in a band's first row,
+// also its exchange words (luma column r; the chroma pair holding column
+// r & 7 of its plane).
 struct DbkIn {
     uint4 y;       // luma row r
     uint2 c;       // chroma row r & 7 of the lane's plane
     uint4 lv, lh;  // luma vertical, horizontal edges 0..3
     uint2 cv, ch;  // chroma vertical, horizontal edges 0, 4
+    uint64_t il, ic;
 };
 
 // Global-memory pointers: pointers read from a descriptor are generic, and
@@ -750,42 +766,65 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Where slot `slot` stands at step t = q * lag + rem: row y = slot + 64k,
-// the latest of its rows started (y * lag <= t), column x = t - y * lag.
-__device__ __forceinline__ bool dbk_at(int q, int rem, int slot, int lag, int mbw, int mbh, int& x, int& y) {
-    if (q < slot) return false;
-    y = slot + ((q - slot) & ~(kDbkSlots - 1));
-    x = (q - y) * lag + rem;
-    return y < mbh && x < mbw;
+__device__ __forceinline__ uint64_t xch_load(gptr<const uint64_t> p) {
+    return __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xch_store(gptr<uint64_t> p, uint64_t v) {
+    __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Poll an exchange word until it carries `tag`.
+__device__ __forceinline__ uint64_t xch_wait(gptr<const uint64_t> p, uint64_t v, uint32_t tag) {
+    for (int n = 0; (uint32_t)(v >> 32) != tag && n < kDbkSpin; ++n) {
+        __builtin_amdgcn_s_sleep(1);
+        v = xch_load(p);
+    }
+    return v;
 }
 
-__global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
-    extern __shared__ uint32_t dl[];
+__global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
+    __shared__ uint32_t dl[kDbkSlots * kDbkPer];
+    __shared__ int band_s;
     const int32_t* pre = b.prefix + 3 * (b.n + 1);
     const int p = find_pic(pre, b.n, (int)blockIdx.x);
     const H264PicDev& P = *b.h264[p];
     if (!P.deblock) return;
+    const gptr<uint64_t> xch = to_global(P.xch);  // [0] ticket, then the boundaries
+    if (threadIdx.x == 0)
+        band_s = (int)__hip_atomic_fetch_add((uint64_t*)xch, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int band = band_s;
+    if (band >= h264_deblock_bands(P.mbh)) return;  // only with a stale ticket
     const int slot = (int)threadIdx.x >> 4, r = (int)threadIdx.x & 15;
     const int pl = r >> 3, rr = r & 7;  // chroma plane, row / column of this lane
     const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
                    (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh};
-    const int mbw = D.mbw, mbh = D.mbh, lag = dbk_lag(mbw), C = lag + 1;
-    const int per = kDbkTile + kDbkRing * C;
-    uint32_t* const tile = dl + slot * per;
+    const int mbw = D.mbw, mbh = D.mbh;
+    const int y0 = band * kDbkSlots, rows = min(kDbkSlots, mbh - y0);
+    const bool first = slot == 0 && band > 0;                       // imports its strips
+    const bool last = slot == rows - 1 && y0 + rows < mbh;          // publishes its strips
+    // Exchange words of this lane: imported (boundary band - 1) and
+    // published (boundary band), macroblock x at + 24 x.
+    const gptr<const uint64_t> imp_l = xch + 1 + (size_t)max(band - 1, 0) * mbw * kDbkRing + r;
+    const gptr<const uint64_t> imp_c = imp_l + 16 - r + (r >> 1);
+    const gptr<uint64_t> pub = xch + 1 + (size_t)band * mbw * kDbkRing + r;
+    uint32_t* const tile = dl + slot * kDbkPer;
     uint8_t* const tb = (uint8_t*)tile;
     uint32_t* const ring = tile + kDbkTile;
-    const uint32_t* const up = dl + ((slot + kDbkSlots - 1) & (kDbkSlots - 1)) * per + kDbkTile;
+    const uint32_t* const up = dl + max(slot - 1, 0) * kDbkPer + kDbkTile;  // read by slots > 0
     const gptr<uint8_t> Y = D.y;
     const gptr<uint8_t> Cp = D.c;
     const uint32_t py = D.py;
     const size_t pc = D.pc;
-    const int steps = (mbh - 1) * lag + mbw;
+    const int y = y0 + slot;
+    const int steps = 2 * (rows - 1) + mbw + (y0 + rows < mbh ? 2 : 0);
 
-    int q = 0, rem = 0;  // the next step is q * lag + rem
-    int x = 0, y = 0, ex = 0;
-    bool act = dbk_at(q, rem, slot, lag, mbw, mbh, x, y);
+    int x = -2 * slot, ex = 0;  // the slot's macroblock at the next step, x % 3
+    bool act = slot < rows && x == 0;
     DbkIn ba, bb;  // this step's inputs and the next step's, alternating
-    if (act) dbk_fetch(D, x, y, r, ba);
+    if (act) {
+        dbk_fetch(D, 0, y, r, ba);
+        if (first) ba.il = xch_load(imp_l), ba.ic = xch_load(imp_c);
+    }
     // One step: filter the macroblock `in` holds, fetch the next into `nx`.
     auto step = [&](const DbkIn& in, DbkIn& nx) __attribute__((always_inline)) {
         // Wait here for this step's inputs (fetched a step ago) and the last
@@ -794,11 +833,21 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         const int cx = x, cy = y, cex = ex;
         const bool cact = act;
-        if (++rem == lag) rem = 0, ++q;
-        act = dbk_at(q, rem, slot, lag, mbw, mbh, x, y);
+        if (last) {  // publish ring entry x - 2, complete since the last step
+            const int xp = cx - 2;
+            if (xp >= 0 && xp < mbw) {
+                const uint32_t* e = ring + (cex == 0 ? 1 : cex == 1 ? 2 : 0) * kDbkRing;
+                const uint64_t tag = (uint64_t)(xp + 1) << 32;
+                xch_store(pub + (size_t)xp * kDbkRing, tag | e[r]);
+                if (r < 8) xch_store(pub + (size_t)xp * kDbkRing + 16, tag | e[16 + r]);
+            }
+        }
+        ++x;
+        act = slot < rows && x >= 0 && x < mbw;
+        ex = x <= 0 ? 0 : (ex + 1 == kDbkRingN ? 0 : ex + 1);  // x % 3
         if (act) {
-            ex = x == 0 ? 0 : (ex + 1 == C ? 0 : ex + 1);  // x % C
             dbk_fetch(D, x, y, r, nx);
+            if (first) nx.il = xch_load(imp_l + (size_t)x * kDbkRing), nx.ic = xch_load(imp_c + (size_t)x * kDbkRing);
         }
         if (cact) {
             const bool lastc = cx == mbw - 1, lastr = cy == mbh - 1;
@@ -828,7 +877,7 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
                     *(gptr<uint32_t>)(Y + ((uint32_t)(16 * cy + r) * py + 16 * cx - 4)) = pk4(v[0], v[1], v[2], v[3]);
                 if (rr <= 6 || lastr) Cp[(size_t)(8 * cy + rr) * pc + 8 * cx - 1] = (uint8_t)w[1];
                 if (!lastr) {
-                    uint8_t* e = (uint8_t*)(ring + (cex == 0 ? C - 1 : cex - 1) * kDbkRing);
+                    uint8_t* e = (uint8_t*)(ring + (cex == 0 ? kDbkRingN - 1 : cex - 1) * kDbkRing);
                     if (r >= 12)
                         for (int k = 0; k < 4; ++k) e[(12 + k) * 4 + r - 12] = (uint8_t)v[k];
                     if (rr >= 6) e[64 + (pl * 8 + 7) * 2 + rr - 6] = (uint8_t)w[1];
@@ -839,15 +888,25 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             // Horizontal edges: luma column r, chroma column rr; rows above
-            // from the ring of the row above (unused in row 0: no top edge).
+            // from the ring of the row above, or the exchange in a band's
+            // first row (unused in row 0: no top edge).
             int h[20], g[10];
-            const uint32_t* ue = up + cex * kDbkRing;
-            unpk4<0>(h, ue[r]);
-            for (int k = 0; k < 16; ++k) h[4 + k] = tb[k * 16 + r];
-            {
-                const uint32_t u = ((const uint16_t*)(ue + 16))[pl * 8 + rr];
-                g[0] = u & 0xFF, g[1] = u >> 8;
+            uint32_t ul, uc;
+            if (slot > 0) {
+                const uint32_t* ue = up + cex * kDbkRing;
+                ul = ue[r];
+                uc = ((const uint16_t*)(ue + 16))[pl * 8 + rr];
+            } else if (first) {
+                const uint32_t tag = (uint32_t)cx + 1;
+                ul = (uint32_t)xch_wait(imp_l + (size_t)cx * kDbkRing, in.il, tag);
+                uc = (uint32_t)xch_wait(imp_c + (size_t)cx * kDbkRing, in.ic, tag);
+                uc = (uc >> (16 * (r & 1))) & 0xFFFF;
+            } else {
+                ul = uc = 0;
             }
+            unpk4<0>(h, ul);
+            for (int k = 0; k < 16; ++k) h[4 + k] = tb[k * 16 + r];
+            g[0] = uc & 0xFF, g[1] = uc >> 8;
             for (int k = 0; k < 8; ++k) g[2 + k] = tb[256 + pl * 64 + k * 8 + rr];
             dbk_line<4, true, true>(h, in.lh.x);
             dbk_line<8, true, false>(h, in.lh.y);
@@ -885,9 +944,9 @@ __global__ void __launch_bounds__(1024) h264_stage3(BatchDev b) {
         }
         lds_barrier();
     };
-    for (int t = 0; t < steps; t += 2) {
+    for (int s2 = 0; s2 < steps; s2 += 2) {
         step(ba, bb);
-        if (t + 1 < steps) step(bb, ba);
+        if (s2 + 1 < steps) step(bb, ba);
     }
 }
 
@@ -988,12 +1047,12 @@ void h264_stage_items(H264PicDev& P) {
     P.items[0] = P.n_mbs + P.n_parts;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
     P.items[2] = (P.n_mbs + 1 + 63) / 64;
-    P.items[3] = P.deblock ? 1 : 0;  // one workgroup per picture
+    P.items[3] = P.deblock ? h264_deblock_bands(P.mbh) : 0;  // one workgroup per band
     P.items[4] = 0;
 }
 
-int h264_deblock_lds(int mbw) {
-    return kDbkSlots * (kDbkTile + kDbkRing * (dbk_lag(mbw) + 1)) * (int)sizeof(uint32_t);
+size_t h264_deblock_xch_words(int mbw, int mbh) {
+    return 1 + (size_t)(h264_deblock_bands(mbh) - 1) * mbw * kDbkRing;
 }
 
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
@@ -1002,7 +1061,7 @@ hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
         case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
         case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(1024), h264_deblock_lds(b.mbw_max), st, b); break;
+        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(256), 0, st, b); break;
         default: return hipSuccess;
     }
     return hipGetLastError();

@@ -28,9 +28,6 @@ hipStream_t g_stream = nullptr;
 hipStream_t g_io = nullptr;  // plane uploads / downloads
 hipStream_t g_up = nullptr;  // work-list and batch-table uploads
 int g_stage_mask = 0;
-// LDS a workgroup may take on the device (stage 3 of H.264 sizes its
-// wavefront to the picture width).
-int g_lds_max = 65536;
 constexpr int kMarks = 4096;
 hipEvent_t g_marks[kMarks];
 int g_next_mark = 0;
@@ -406,8 +403,6 @@ int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
         return fail(RV_E_RANGE, "picture shape differs from the session");
     const int W = p->width, H = p->height, mbw = W / 16, mbh = H / 16;
     if (p->n_mbs != mbw * mbh || !p->mbs) return fail(RV_E_RANGE, "n_mbs must cover the picture");
-    if (p->deblock && rv::h264_deblock_lds(mbw) > g_lds_max)
-        return fail(RV_E_RANGE, "picture too wide for the deblocking wavefront's LDS");
     for (int l = 0; l < 2; ++l) {
         if (p->num_ref[l] < 0 || p->num_ref[l] > RV_MAX_REFS) return fail(RV_E_RANGE, "num_ref outside 0..4");
         for (int r = 0; r < p->num_ref[l]; ++r) {
@@ -521,7 +516,6 @@ int rv_init(int device_ordinal) {
         RV_HIP(hipGetDeviceProperties(&prop, device_ordinal));
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             return fail(RV_E_NODEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
-        g_lds_max = (int)std::min<size_t>(prop.sharedMemPerBlock, 65536);
         RV_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
         RV_HIP(hipStreamCreateWithFlags(&g_io, hipStreamNonBlocking));
         RV_HIP(hipStreamCreateWithFlags(&g_up, hipStreamNonBlocking));
@@ -979,7 +973,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
                     carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4), nmb = (size_t)mbw * mbh;
-        size_t scr = carve_size<uint32_t>(n4) + carve_size<uint32_t>(nmb * rv::kH264DbkWords) + 1024;
+        const size_t nx = rv::h264_deblock_xch_words(mbw, mbh);
+        size_t scr = carve_size<uint32_t>(n4) + carve_size<uint32_t>(nmb * rv::kH264DbkWords) + carve_size<uint64_t>(nx) +
+                     1024;
         char* hbase = t_stage.get(sz);
         if (!hbase || !pool_get(sz, &j->arena) || !pool_get(scr, &j->scratch)) {
             job_release(j);
@@ -1054,6 +1050,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.partmap = (uint32_t*)sp;
         sp += carve_size<uint32_t>(n4);
         d.dbk = (uint32_t*)sp;
+        sp += carve_size<uint32_t>(nmb * rv::kH264DbkWords);
+        d.xch = (uint64_t*)sp;
+        d.n_xch = (int64_t)nx;
         rv::h264_stage_items(d);
         *hdesc = d;
         j->desc_dev = dev((char*)hdesc);
@@ -1065,7 +1064,6 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->single.n = 1;
         j->single.hevc = nullptr;
         j->single.h264 = (const rv::H264PicDev* const*)dev((char*)h_ptr);
-        j->single.mbw_max = mbw;
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
         hipError_t e = stage_upload(j->arena.p, sz, j->scratch.p, scr);
         if (e != hipSuccess) {
@@ -1149,8 +1147,6 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);
-        b->b4.mbw_max = 0;
-        for (int i = 0; i < n4; ++i) b->b4.mbw_max = std::max(b->b4.mbw_max, b->h264[i]->h4.mbw);
         hipError_t e = stage_upload(b->tables.p, sz, nullptr, 0);
         if (e != hipSuccess) {
             {

@@ -88,6 +88,9 @@ struct HevcPicDev {
 // [plane][segment][edge 0, 4], chroma horizontal likewise.
 constexpr int kH264DbkWords = 64;
 
+// Macroblock rows per deblocking band (one workgroup each).
+__host__ __device__ inline int h264_deblock_bands(int mbh) { return (mbh + 15) / 16; }
+
 // Per-picture descriptor of an H.264 job.
 struct H264PicDev {
     int32_t w, h, mbw, mbh, w4, h4;
@@ -110,6 +113,8 @@ struct H264PicDev {
     const uint16_t* intra_pred;
     uint32_t* partmap;   // per 4x4: partition index (0xFFFFFFFF = intra)
     uint32_t* dbk;       // per MB: kH264DbkWords words of edge parameters
+    uint64_t* xch;       // deblocking: band ticket, then the strips crossing bands
+    int64_t n_xch;       // words of xch
     int32_t items[5];
 };
 
@@ -174,7 +179,6 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
-    int32_t mbw_max;                // H.264: widest picture, in macroblocks
 };
 
 }  // namespace rv

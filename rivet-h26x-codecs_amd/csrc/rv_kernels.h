@@ -21,8 +21,8 @@ hipError_t hevc_entry_launch(const HevcEntry& e, int blocks, hipStream_t st);
 // H.264 (h264.hip).
 void h264_stage_items(H264PicDev& P);
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st);
-// LDS bytes of the deblocking wavefront for pictures `mbw` macroblocks wide.
-int h264_deblock_lds(int mbw);
+// 64-bit words of the deblocking exchange of one picture.
+size_t h264_deblock_xch_words(int mbw, int mbh);
 hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st);
 
 // Plain copy of `bytes` (a multiple of 4096) with 16-byte vectors.
