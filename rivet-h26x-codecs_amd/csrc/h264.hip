@@ -98,17 +98,6 @@ __device__ int qpel_at(const Win& q, int pos, int x, int y) {
     }
 }
 
-// Stage a ww x hh window at (x0, y0) of a plane, clamped to the picture
-// (the replicated border of src/h264/frame.rs and the gather of
-// src/h264/inter.rs:127-139 read the same samples). Stride 32.
-__device__ void stage_window(const uint8_t* __restrict__ pl, int pitch, int pw, int ph, int x0, int y0, int ww, int hh,
-                             int16_t* win, int tid) {
-    for (int e = tid; e < hh * 32; e += 64) {
-        int r = e >> 5, c = e & 31;
-        if (c < ww) win[r * 32 + c] = pl[(size_t)clampi(y0 + r, 0, ph - 1) * pitch + clampi(x0 + c, 0, pw - 1)];
-    }
-}
-
 // The combine of predict_partition (src/h264/inter.rs:292-331) with
 // copy / avg / weighted_uni / weighted_bi (src/dsp/h264.rs:522-597).
 __device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, const int8_t* ri, int a, int b) {
@@ -140,6 +129,12 @@ __device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, cons
     return clampi(((a * w0 + b * w1 + (1 << log_wd)) >> (log_wd + 1)) + ((o0 + o1 + 1) >> 1), 0, 255);
 }
 
+// LDS of one partition: the luma windows of both lists (up to 21 x 21,
+// stride 32), then the chroma windows of both lists and planes (up to
+// 9 x 9, stride 16).
+constexpr int kLumaWin = 21 * 32, kChromaWin = 9 * 16;
+constexpr int kPartLds = 2 * kLumaWin + 4 * kChromaWin;
+
 __device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
     const rv_h264_part pt = P.parts[idx];
     const int x = pt.x, y = pt.y, w = pt.w, h = pt.h, dir = pt.dir;
@@ -148,21 +143,64 @@ __device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
         int bx = (x >> 2) + e % (w >> 2), by = (y >> 2) + e / (w >> 2);
         P.partmap[(size_t)by * P.w4 + bx] = (uint32_t)idx;
     }
+    // Every window's samples are requested before any is stored, so the
+    // workgroup waits for memory once. Windows clamp to the picture (the
+    // replicated border of src/h264/frame.rs and the gather of
+    // src/h264/inter.rs:127-139 read the same samples). Luma: lane column
+    // tid & 31, rows tid >> 5 + 2k; chroma: column tid & 15, rows tid >> 4 + 4k.
+    const int lc = tid & 31, lr = tid >> 5, cc = tid & 15, cr = tid >> 4;
+    const int ww = w + 5, hh = h + 5, cw = w >> 1, ch = h >> 1;
+    int lv[2][11], cv[2][2][3];
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+        const int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
+        const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
+        const int x0 = x + (mvx >> 2) - 2, y0 = y + (mvy >> 2) - 2;
+        const int xs = clampi(x0 + lc, 0, P.pw[0] - 1);
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            const int r = lr + 2 * k;
+            if (lc < ww && r < hh) lv[l][k] = ref[(size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] + xs];
+        }
+        const int xc0 = (x >> 1) + (mvx >> 3), yc0 = (y >> 1) + (mvy >> 3);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint8_t* rc = (const uint8_t*)P.ref[l][pt.ref_idx[l]][1 + c];
+            const int xcs = clampi(xc0 + cc, 0, P.pw[1 + c] - 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int r = cr + 4 * k;
+                if (cc <= cw && r <= ch)
+                    cv[l][c][k] = rc[(size_t)clampi(yc0 + r, 0, P.ph[1 + c] - 1) * P.pitch[1 + c] + xcs];
+            }
+        }
+    }
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1)) continue;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            const int r = lr + 2 * k;
+            if (lc < ww && r < hh) win[l * kLumaWin + r * 32 + lc] = (int16_t)lv[l][k];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int r = cr + 4 * k;
+                if (cc <= cw && r <= ch) win[2 * kLumaWin + (l * 2 + c) * kChromaWin + r * 16 + cc] = (int16_t)cv[l][c][k];
+            }
+    }
+    __syncthreads();
     int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
-        const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
-        int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
-        int xi = x + (mvx >> 2), yi = y + (mvy >> 2), pos = (mvy & 3) * 4 + (mvx & 3);
-        stage_window(ref, P.pitch[0], P.pw[0], P.ph[0], xi - 2, yi - 2, w + 5, h + 5, win, tid);
-        __syncthreads();
-        Win q{win};
+        const int pos = (pt.mv[l].y & 3) * 4 + (pt.mv[l].x & 3);
+        Win q{win + l * kLumaWin};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
             if (ox < w && oy < h) pred[l][k] = qpel_at(q, pos, ox, oy);
         }
-        __syncthreads();
     }
     uint8_t* dst = (uint8_t*)P.cur[0];
 #pragma unroll
@@ -171,34 +209,25 @@ __device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
         if (ox < w && oy < h) dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (uint8_t)combine(P, 0, dir, pt.ref_idx, pred[0][k], pred[1][k]);
     }
     // Chroma: eighth-sample bilinear (src/h264/inter.rs:243-282), 4:2:0.
-    const int cw = w >> 1, ch = h >> 1;
-    for (int c = 1; c < 3; ++c) {
-        int cp[2] = {0, 0};
-        for (int l = 0; l < 2; ++l) {
-            if (!((dir >> l) & 1)) continue;
-            const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][c];
-            int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
-            int xci = (x >> 1) + (mvx >> 3), yci = (y >> 1) + (mvy >> 3), xf = mvx & 7, yf = mvy & 7;
-            stage_window(ref, P.pitch[c], P.pw[c], P.ph[c], xci, yci, cw + 1, ch + 1, win, tid);
-            __syncthreads();
-            int ox = tid & 7, oy = tid >> 3;
-            if (ox < cw && oy < ch) {
-                int a = (8 - xf) * (8 - yf), b = xf * (8 - yf), cc = (8 - xf) * yf, d = xf * yf;
-                const int16_t* r0 = win + oy * 32 + ox;
-                cp[l] = (a * r0[0] + b * r0[1] + cc * r0[32] + d * r0[33] + 32) >> 6;
+    const int ox = tid & 7, oy = tid >> 3;
+    if (ox < cw && oy < ch) {
+        for (int c = 1; c < 3; ++c) {
+            int cp[2] = {0, 0};
+            for (int l = 0; l < 2; ++l) {
+                if (!((dir >> l) & 1)) continue;
+                const int xf = pt.mv[l].x & 7, yf = pt.mv[l].y & 7;
+                const int a = (8 - xf) * (8 - yf), bb = xf * (8 - yf), c2 = (8 - xf) * yf, d = xf * yf;
+                const int16_t* r0 = win + 2 * kLumaWin + (l * 2 + c - 1) * kChromaWin + oy * 16 + ox;
+                cp[l] = (a * r0[0] + bb * r0[1] + c2 * r0[16] + d * r0[17] + 32) >> 6;
             }
-            __syncthreads();
-        }
-        int ox = tid & 7, oy = tid >> 3;
-        if (ox < cw && oy < ch) {
-            uint8_t* d = (uint8_t*)P.cur[c];
-            d[(size_t)((y >> 1) + oy) * P.pitch[c] + (x >> 1) + ox] = (uint8_t)combine(P, c, dir, pt.ref_idx, cp[0], cp[1]);
+            uint8_t* dc = (uint8_t*)P.cur[c];
+            dc[(size_t)((y >> 1) + oy) * P.pitch[c] + (x >> 1) + ox] = (uint8_t)combine(P, c, dir, pt.ref_idx, cp[0], cp[1]);
         }
     }
 }
 
 __global__ void __launch_bounds__(64) h264_stage0(BatchDev b) {
-    __shared__ int16_t win[21 * 32];
+    __shared__ int16_t win[kPartLds];
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = blockIdx.x;
     const int p = find_pic(pre, b.n, it);
