@@ -160,6 +160,9 @@ def main() -> int:
                     help="run only the 256 MiB copy probe once (counter calibration) and exit")
     ap.add_argument("--no-serve", action="store_true", help="skip the serve-loop measurement (config 5)")
     ap.add_argument("--no-configs", action="store_true", help="skip the batched configs 2 and 4")
+    ap.add_argument("--only-config", type=int, choices=(2, 4), default=0,
+                    help="measure only 8 pictures of configuration 2 or 4, batched (for kernel profiling), "
+                         "print its JSON and exit")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -177,6 +180,11 @@ def main() -> int:
         gbs = rh.probe_copy_gbs(256 << 20, 1)
         log(f"[bench] copy probe: 256 MiB read + 256 MiB written, {gbs:.1f} GB/s")
         return 0
+    if args.only_config:
+        r = measure_config(rh, cf, oracle, args.only_config, 8, args.steps, args.warmup)
+        log(f"[bench] config {args.only_config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms")
+        print(json.dumps(r), flush=True)
+        return 0 if r["bit_exact"] else 1
     log(f"[bench] tier {rh.tier_name()}; generating {args.streams} x 3840x2160 pictures")
     pics, refs, sessions, jobs = build_workload(rh, cf, args.streams)
     batch = rh.Batch(jobs)
