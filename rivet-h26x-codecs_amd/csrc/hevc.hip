@@ -1558,44 +1558,92 @@ __device__ __forceinline__ void sao_group(PicRef P, int c, int x0, int y, const 
     int o[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) o[k] = v[1][k + 1];
+    // The offsets as a byte table (offset + 128) read with v_perm_b32: one
+    // select for what is otherwise a chain of compares. Offsets beyond a
+    // signed byte (13 bits and deeper) take the chain.
+    const int o0 = sp.offsets[0], o1 = sp.offsets[1], o2 = sp.offsets[2], o3 = sp.offsets[3];
+    const bool tbl = (abs(o0) | abs(o1) | abs(o2) | abs(o3)) <= 127;
     if (on && sp.type_idx == 1) {
         const int shift = P.bd - 5;
+        if (tbl) {
+            // Bands 0..3 of the table carry their offsets, any other band 0.
+            const uint32_t lo = (uint32_t)(o0 + 128) | (uint32_t)(o1 + 128) << 8 | (uint32_t)(o2 + 128) << 16 |
+                                (uint32_t)(o3 + 128) << 24;
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            int band = ((o[k] >> shift) - sp.band_or_class) & 31;
-            if (band < 4) o[k] = clampi(o[k] + sp.offsets[band], 0, max);
+            for (int k = 0; k < G; ++k) {
+                const uint32_t band = (uint32_t)((o[k] >> shift) - sp.band_or_class) & 31;
+                const int off = (int)__builtin_amdgcn_perm(128u, lo, min(band, 4u) | 0x0c0c0c00u);
+                o[k] = clampi(o[k] + off - 128, 0, max);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                int band = ((o[k] >> shift) - sp.band_or_class) & 31;
+                if (band < 4) o[k] = clampi(o[k] + sp.offsets[band], 0, max);
+            }
         }
     } else if (on && sp.type_idx == 2) {
         const int cls = sp.band_or_class & 3;
         const int vy = cls == 0 ? 0 : -1;
         const bool rows_ok = y + vy >= 0 && y - vy < ph && y - vy >= 0 && y + vy < ph;
-        const int o0 = sp.offsets[0], o1 = sp.offsets[1], o2 = sp.offsets[2], o3 = sp.offsets[3];
+        // edgeIdx 0..4 (sign sum + 2) picks byte edgeIdx of {hi:lo}.
+        const uint32_t lo = (uint32_t)(o0 + 128) | (uint32_t)(o1 + 128) << 8 | 128u << 16 | (uint32_t)(o2 + 128) << 24;
+        const uint32_t hi = (uint32_t)(o3 + 128);
         auto apply = [&](int k, int a, int bb) {
-            const int e = sgn(o[k] - a) + sgn(o[k] - bb);  // edgeIdx - 2
-            const int off = e == -2 ? o0 : e == -1 ? o1 : e == 0 ? 0 : e == 1 ? o2 : o3;
-            o[k] = clampi(o[k] + off, 0, max);
+            if (tbl) {
+                const int s1 = clampi(o[k] - a, -1, 1), s2 = clampi(o[k] - bb, -1, 1);
+                const uint32_t sel = (uint32_t)(s1 + s2) + 0x0c0c0c02u;
+                o[k] = clampi(o[k] + (int)__builtin_amdgcn_perm(hi, lo, sel) - 128, 0, max);
+            } else {
+                const int e = sgn(o[k] - a) + sgn(o[k] - bb);  // edgeIdx - 2
+                const int off = e == -2 ? o0 : e == -1 ? o1 : e == 0 ? 0 : e == 1 ? o2 : o3;
+                o[k] = clampi(o[k] + off, 0, max);
+            }
         };
+        // Groups clear of the left and right picture edges skip the
+        // per-sample column checks.
+        const bool inner = x0 >= 1 && x0 + G < pw;
         if (rows_ok) {
             switch (cls) {
                 case 0:  // (x - 1, y), (x + 1, y)
+                    if (inner) {
 #pragma unroll
-                    for (int k = 0; k < G; ++k)
-                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[1][k], v[1][k + 2]);
+                        for (int k = 0; k < G; ++k) apply(k, v[1][k], v[1][k + 2]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < G; ++k)
+                            if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[1][k], v[1][k + 2]);
+                    }
                     break;
                 case 1:  // (x, y - 1), (x, y + 1)
+                    if (inner) {
 #pragma unroll
-                    for (int k = 0; k < G; ++k)
-                        if (x0 + k < pw) apply(k, v[0][k + 1], v[2][k + 1]);
+                        for (int k = 0; k < G; ++k) apply(k, v[0][k + 1], v[2][k + 1]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < G; ++k)
+                            if (x0 + k < pw) apply(k, v[0][k + 1], v[2][k + 1]);
+                    }
                     break;
                 case 2:  // (x - 1, y - 1), (x + 1, y + 1)
+                    if (inner) {
 #pragma unroll
-                    for (int k = 0; k < G; ++k)
-                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k], v[2][k + 2]);
+                        for (int k = 0; k < G; ++k) apply(k, v[0][k], v[2][k + 2]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < G; ++k)
+                            if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k], v[2][k + 2]);
+                    }
                     break;
                 default:  // (x + 1, y - 1), (x - 1, y + 1)
+                    if (inner) {
 #pragma unroll
-                    for (int k = 0; k < G; ++k)
-                        if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k + 2], v[2][k]);
+                        for (int k = 0; k < G; ++k) apply(k, v[0][k + 2], v[2][k]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < G; ++k)
+                            if (x0 + k - 1 >= 0 && x0 + k + 1 < pw) apply(k, v[0][k + 2], v[2][k]);
+                    }
                     break;
             }
         }
