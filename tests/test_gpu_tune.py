@@ -96,6 +96,10 @@ def test_mixed_depth_batch_keeps_the_generic_variants(device):
     items = _setup([(cf.HEVC_C1, 31337), (dict(cf.HEVC_C2, width=416, height=240), 31338)])
     b = rh.Batch([j for _, _, j, _ in items])
     _sweep(b, items, 0, "mixed batch")
-    assert b.tune(1) == [0, 0, 0, 0, 0]
+    # The tuner may pick any variant that serves every depth (a timing
+    # choice); none specialised to one depth.
+    names = rh.variant_names()
+    chosen = b.tune(1)
+    assert all(names[s][v].startswith("any") for s, v in enumerate(chosen)), chosen
     _check(b, items, rh.RV_LAUNCH_GRAPH, "mixed graph")
     _close(b, items)
