@@ -67,6 +67,7 @@ class Pic:
     ready_t: float = math.nan
     done_t: float = math.nan
     launched: bool = False
+    queued: bool = False  # in the loop's heap of launchable pictures
     done: bool = False
     luma: int = 0
     out: list | None = None
@@ -197,8 +198,15 @@ def run(sessions, mode: str = "batched", budget: int = 32 << 30, max_batch: int 
 
 def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=False):
     """The loop is event-driven: finished uploads (posted by the upload
-    threads) and pictures made ready by a completed reference feed a heap
-    of launchable pictures ordered by ready time."""
+    threads) and launched references feed a heap of launchable pictures.
+    Every launch goes to the library's one in-order stream, so a picture
+    may launch as soon as its references have launched: the stream runs it
+    after them, and the next batch is queued while the last one runs. At
+    most `inflight_max` launch groups (one loop iteration's launches) are
+    outstanding in either mode. A picture's latency runs from when it was
+    ready (references complete, work list uploaded) to its completion.
+    With record=True a picture launches only after its references
+    complete, so the download at its completion sees its own output."""
     t0 = time.perf_counter()
     arrivals = collections.deque(sorted(sessions, key=lambda s: s.arrival))
     waiting = collections.deque()
@@ -214,11 +222,13 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
     seq = itertools.count()
     uploaded = collections.deque()  # appended by upload threads
     wake = threading.Event()
+    groups = collections.Counter()  # launch group -> its inflight entries
+    gid = itertools.count()
     for s in sessions:
         s.footprint = footprint(s)
         s.remaining = len(s.pics) - 1
         for p in s.pics:
-            p.launched = p.done = False
+            p.launched = p.done = p.queued = False
             p.ready_t = p.done_t = math.nan
             p.job = p.fut = None
 
@@ -231,10 +241,20 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             wake.set()
         return cb
 
+    def eligible(q: Pic) -> bool:
+        if q.job is None or q.launched or q.queued:
+            return False
+        refs = [_pic_by_poc(q.sess, r) for r in q.refs]
+        return all(r.done or (r.launched and not record) for r in refs)
+
+    def push(q: Pic, t: float):
+        q.queued = True
+        heapq.heappush(launchable, (t, next(seq), q))
+
     def make_ready(p: Pic, t: float):
         p.ready_t = t
-        if p.job is not None:
-            heapq.heappush(launchable, (t, next(seq), p))
+        if eligible(p):
+            push(p, t)
 
     def complete(p: Pic, t: float):
         nonlocal done, used
@@ -288,7 +308,10 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         while waiting and (used + waiting[0].footprint <= budget or not active):
             admit(waiting.popleft(), now())
         while inflight and mark_done(inflight[0][0]):
-            _, pics, bs = inflight.popleft()
+            _, pics, bs, g = inflight.popleft()
+            groups[g] -= 1
+            if groups[g] == 0:
+                del groups[g]
             tc = now()
             for b in bs:
                 b.close()
@@ -303,10 +326,11 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
                 complete(p, p.ready_t)
                 continue
             p.job = r
-            if not math.isnan(p.ready_t) and not p.launched:
-                heapq.heappush(launchable, (p.ready_t, next(seq), p))
-        if launchable and len(inflight) < inflight_max:
+            if eligible(p):
+                push(p, now())
+        if launchable and len(groups) < inflight_max:
             group = [heapq.heappop(launchable)[2] for _ in range(min(max_batch, len(launchable)))]
+            g = next(gid)
             if mode == "batched":
                 # One batch per sample depth class, so each runs the kernel
                 # variants tuned for its class.
@@ -316,7 +340,8 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
                     b.launch(RV_LAUNCH_BATCHED)
                 for p in group:
                     p.launched = True
-                inflight.append((mark(), group, bs))
+                inflight.append((mark(), group, bs, g))
+                groups[g] += 1
                 batches += 1
                 batch_sizes.append(len(group))
             else:
@@ -324,11 +349,18 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
                     b = Batch([p.job])
                     b.launch(RV_LAUNCH_PLAIN)
                     p.launched = True
-                    inflight.append((mark(), [p], [b]))
+                    inflight.append((mark(), [p], [b], g))
+                    groups[g] += 1
                     batches += 1
                     batch_sizes.append(1)
+            # Their dependents may launch next: the stream orders them.
+            tl = now()
+            for p in group:
+                for q in p.dependents:
+                    if eligible(q):
+                        push(q, tl)
             continue
-        if len(inflight) >= inflight_max:
+        if len(groups) >= inflight_max:
             mark_wait(inflight[0][0])
         elif inflight:
             wake.wait(0.0002)  # an upload, or soon a completion to collect
