@@ -56,18 +56,18 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
 }
 
 // qpel_scalar<XF, YF> (src/dsp/h264.rs:427-493) over a window staged in
-// LDS: W is the window (stride 32), origin two left / two above the block.
-struct Win {
+// LDS: w is the window (stride LS), origin two left / two above the block.
+template <int LS> struct WinS {
     const int16_t* w;
     int mx = 255;  // sample maximum
-    __device__ int G(int x, int y) const { return w[(y + 2) * 32 + x + 2]; }
+    __device__ int G(int x, int y) const { return w[(y + 2) * LS + x + 2]; }
     __device__ int b1(int x, int yy) const {
-        const int16_t* r = w + yy * 32 + x;
+        const int16_t* r = w + yy * LS + x;
         return tap6(r[0], r[1], r[2], r[3], r[4], r[5]);
     }
     __device__ int h1(int xx, int y) const {
-        const int16_t* c = w + y * 32 + xx;
-        return tap6(c[0], c[32], c[64], c[96], c[128], c[160]);
+        const int16_t* c = w + y * LS + xx;
+        return tap6(c[0], c[LS], c[2 * LS], c[3 * LS], c[4 * LS], c[5 * LS]);
     }
     __device__ int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, mx); }
     __device__ int h(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, mx); }
@@ -76,8 +76,9 @@ struct Win {
         return clampi((j1 + 512) >> 10, 0, mx);
     }
 };
+using Win = WinS<32>;
 
-__device__ int qpel_at(const Win& q, int pos, int x, int y) {
+template <class Q> __device__ int qpel_at(const Q& q, int pos, int x, int y) {
     switch (pos) {
         case 0: return q.G(x, y);
         case 1: return (q.G(x, y) + q.b(x, y) + 1) >> 1;
@@ -129,96 +130,113 @@ __device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, cons
     return clampi(((a * w0 + b * w1 + (1 << log_wd)) >> (log_wd + 1)) + ((o0 + o1 + 1) >> 1), 0, 255);
 }
 
-// LDS of one partition: the luma windows of both lists (up to 21 x 21,
-// stride 32), then the chroma windows of both lists and planes (up to
-// 9 x 9, stride 16).
-constexpr int kLumaWin = 21 * 32, kChromaWin = 9 * 16;
-constexpr int kPartLds = 2 * kLumaWin + 4 * kChromaWin;
+// One partition on NT lanes: 64 for partitions up to 16 x 16, 16 for
+// those up to 8 x 8 (four per wave). Its LDS holds the luma windows of
+// both lists (up to (w + 5) x (h + 5), stride LS), then the chroma windows
+// of both lists and planes (up to (w/2 + 1) x (h/2 + 1), stride CS).
+template <int NT> struct PartShape {
+    static constexpr int MW = NT == 64 ? 16 : 8;     // largest width / height
+    static constexpr int LS = NT == 64 ? 32 : 16;    // luma window stride
+    static constexpr int LR = MW + 5;                // luma window rows
+    static constexpr int CS = NT == 64 ? 16 : 8;     // chroma window stride
+    static constexpr int CR = MW / 2 + 1;            // chroma window rows
+    static constexpr int LWIN = LR * LS, CWIN = CR * CS;
+    static constexpr int LDS = 2 * LWIN + 4 * CWIN;  // int16 per partition
+    static constexpr int LPASS = (LR * LS + NT - 1) / NT, CPASS = (CR * CS + NT - 1) / NT;
+};
+constexpr int kStage0Lds = PartShape<64>::LDS > 4 * PartShape<16>::LDS ? PartShape<64>::LDS : 4 * PartShape<16>::LDS;
 
-__device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
-    const rv_h264_part pt = P.parts[idx];
-    const int x = pt.x, y = pt.y, w = pt.w, h = pt.h, dir = pt.dir;
+// `valid` false: a lane group without a partition, which still meets the
+// workgroup barrier.
+template <int NT>
+__device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win, bool valid) {
+    using S = PartShape<NT>;
+    const rv_h264_part pt = P.parts[valid ? idx : 0];
+    const int x = pt.x, y = pt.y, w = pt.w, h = pt.h, dir = valid ? pt.dir : 0;
     // Partition map for the deblocking filter.
-    for (int e = tid; e < (w >> 2) * (h >> 2); e += 64) {
+    for (int e = tid; valid && e < (w >> 2) * (h >> 2); e += NT) {
         int bx = (x >> 2) + e % (w >> 2), by = (y >> 2) + e / (w >> 2);
         P.partmap[(size_t)by * P.w4 + bx] = (uint32_t)idx;
     }
     // Every window's samples are requested before any is stored, so the
     // workgroup waits for memory once. Windows clamp to the picture (the
     // replicated border of src/h264/frame.rs and the gather of
-    // src/h264/inter.rs:127-139 read the same samples). Luma: lane column
-    // tid & 31, rows tid >> 5 + 2k; chroma: column tid & 15, rows tid >> 4 + 4k.
-    const int lc = tid & 31, lr = tid >> 5, cc = tid & 15, cr = tid >> 4;
+    // src/h264/inter.rs:127-139 read the same samples).
     const int ww = w + 5, hh = h + 5, cw = w >> 1, ch = h >> 1;
-    int lv[2][11], cv[2][2][3];
+    int lv[2][S::LPASS], cv[2][2][S::CPASS];
+#pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
         const int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
         const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
         const int x0 = x + (mvx >> 2) - 2, y0 = y + (mvy >> 2) - 2;
-        const int xs = clampi(x0 + lc, 0, P.pw[0] - 1);
 #pragma unroll
-        for (int k = 0; k < 11; ++k) {
-            const int r = lr + 2 * k;
-            if (lc < ww && r < hh) lv[l][k] = ref[(size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] + xs];
+        for (int k = 0; k < S::LPASS; ++k) {
+            const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
+            if (c < ww && r < hh)
+                lv[l][k] = ref[(size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] + clampi(x0 + c, 0, P.pw[0] - 1)];
         }
         const int xc0 = (x >> 1) + (mvx >> 3), yc0 = (y >> 1) + (mvy >> 3);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const uint8_t* rc = (const uint8_t*)P.ref[l][pt.ref_idx[l]][1 + c];
-            const int xcs = clampi(xc0 + cc, 0, P.pw[1 + c] - 1);
+        for (int c2 = 0; c2 < 2; ++c2) {
+            const uint8_t* rc = (const uint8_t*)P.ref[l][pt.ref_idx[l]][1 + c2];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int r = cr + 4 * k;
-                if (cc <= cw && r <= ch)
-                    cv[l][c][k] = rc[(size_t)clampi(yc0 + r, 0, P.ph[1 + c] - 1) * P.pitch[1 + c] + xcs];
+            for (int k = 0; k < S::CPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::CS, c = e % S::CS;
+                if (c <= cw && r <= ch)
+                    cv[l][c2][k] = rc[(size_t)clampi(yc0 + r, 0, P.ph[1 + c2] - 1) * P.pitch[1 + c2] +
+                                      clampi(xc0 + c, 0, P.pw[1 + c2] - 1)];
             }
         }
     }
+#pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
 #pragma unroll
-        for (int k = 0; k < 11; ++k) {
-            const int r = lr + 2 * k;
-            if (lc < ww && r < hh) win[l * kLumaWin + r * 32 + lc] = (int16_t)lv[l][k];
+        for (int k = 0; k < S::LPASS; ++k) {
+            const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
+            if (c < ww && r < hh) win[l * S::LWIN + r * S::LS + c] = (int16_t)lv[l][k];
         }
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c2 = 0; c2 < 2; ++c2)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int r = cr + 4 * k;
-                if (cc <= cw && r <= ch) win[2 * kLumaWin + (l * 2 + c) * kChromaWin + r * 16 + cc] = (int16_t)cv[l][c][k];
+            for (int k = 0; k < S::CPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::CS, c = e % S::CS;
+                if (c <= cw && r <= ch) win[2 * S::LWIN + (l * 2 + c2) * S::CWIN + r * S::CS + c] = (int16_t)cv[l][c2][k];
             }
     }
     __syncthreads();
+    if (!valid) return;
     int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
         const int pos = (pt.mv[l].y & 3) * 4 + (pt.mv[l].x & 3);
-        Win q{win + l * kLumaWin};
+        const WinS<S::LS> q{win + l * S::LWIN};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
+            const int o = tid + NT * k, oy = o / S::MW, ox = o % S::MW;
             if (ox < w && oy < h) pred[l][k] = qpel_at(q, pos, ox, oy);
         }
     }
     uint8_t* dst = (uint8_t*)P.cur[0];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        int o = tid + 64 * k, oy = o >> 4, ox = o & 15;
+        const int o = tid + NT * k, oy = o / S::MW, ox = o % S::MW;
         if (ox < w && oy < h) dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (uint8_t)combine(P, 0, dir, pt.ref_idx, pred[0][k], pred[1][k]);
     }
     // Chroma: eighth-sample bilinear (src/h264/inter.rs:243-282), 4:2:0.
-    const int ox = tid & 7, oy = tid >> 3;
+    const int ox = tid % (S::MW / 2), oy = tid / (S::MW / 2);
     if (ox < cw && oy < ch) {
         for (int c = 1; c < 3; ++c) {
             int cp[2] = {0, 0};
-            for (int l = 0; l < 2; ++l) {
+        #pragma unroll
+    for (int l = 0; l < 2; ++l) {
                 if (!((dir >> l) & 1)) continue;
                 const int xf = pt.mv[l].x & 7, yf = pt.mv[l].y & 7;
                 const int a = (8 - xf) * (8 - yf), bb = xf * (8 - yf), c2 = (8 - xf) * yf, d = xf * yf;
-                const int16_t* r0 = win + 2 * kLumaWin + (l * 2 + c - 1) * kChromaWin + oy * 16 + ox;
-                cp[l] = (a * r0[0] + bb * r0[1] + c2 * r0[16] + d * r0[17] + 32) >> 6;
+                const int16_t* r0 = win + 2 * S::LWIN + (l * 2 + c - 1) * S::CWIN + oy * S::CS + ox;
+                cp[l] = (a * r0[0] + bb * r0[1] + c2 * r0[S::CS] + d * r0[S::CS + 1] + 32) >> 6;
             }
             uint8_t* dc = (uint8_t*)P.cur[c];
             dc[(size_t)((y >> 1) + oy) * P.pitch[c] + (x >> 1) + ox] = (uint8_t)combine(P, c, dir, pt.ref_idx, cp[0], cp[1]);
@@ -227,7 +245,7 @@ __device__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win) {
 }
 
 __global__ void __launch_bounds__(64) h264_stage0(BatchDev b) {
-    __shared__ int16_t win[kPartLds];
+    __shared__ int16_t win[kStage0Lds];
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = blockIdx.x;
     const int p = find_pic(pre, b.n, it);
@@ -253,7 +271,15 @@ __global__ void __launch_bounds__(64) h264_stage0(BatchDev b) {
         return;
     }
     i -= P.n_mbs;
-    mc_part(P, i, tid, win);
+    // Partitions above 8 x 8 one per workgroup, then the others four per
+    // workgroup, 16 lanes each (the host orders them so).
+    if (i < P.n_big) {
+        mc_part<64>(P, P.part_order[i], tid, win, true);
+        return;
+    }
+    const int g = tid >> 4, k = 4 * (i - P.n_big) + g;
+    const bool valid = k < P.n_parts - P.n_big;
+    mc_part<16>(P, valid ? P.part_order[P.n_big + k] : 0, tid & 15, win + g * PartShape<16>::LDS, valid);
 }
 
 // ---------------------------------------------------------------------
@@ -1073,7 +1099,7 @@ hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st) {
 }
 
 void h264_stage_items(H264PicDev& P) {
-    P.items[0] = P.n_mbs + P.n_parts;
+    P.items[0] = P.n_mbs + P.n_big + (P.n_parts - P.n_big + 3) / 4;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
     P.items[2] = (P.n_mbs + 1 + 63) / 64;
     P.items[3] = P.deblock ? h264_deblock_bands(P.mbh) : 0;  // one workgroup per band

@@ -969,7 +969,8 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->out_slot = out_slot;
         h264_bytes(p, j->bytes);
         size_t sz = carve_size<rv::H264PicDev>(1) + carve_size<rv_h264_mb>(p->n_mbs) +
-                    carve_size<rv_h264_part>(p->n_parts) + carve_size<rv_h264_blk>(blks.size()) +
+                    carve_size<rv_h264_part>(p->n_parts) + carve_size<uint32_t>(p->n_parts) +
+                    carve_size<rv_h264_blk>(blks.size()) +
                     carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4), nmb = (size_t)mbw * mbh;
@@ -988,6 +989,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         rv::H264PicDev* hdesc = carve<rv::H264PicDev>(hp, 1);
         rv_h264_mb* h_mbs = carve<rv_h264_mb>(hp, p->n_mbs);
         rv_h264_part* h_parts = carve<rv_h264_part>(hp, p->n_parts);
+        uint32_t* h_order = carve<uint32_t>(hp, p->n_parts);
         rv_h264_blk* h_blks = carve<rv_h264_blk>(hp, blks.size());
         int32_t* h_coefs = carve<int32_t>(hp, p->n_coefs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
@@ -995,6 +997,13 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         std::memcpy(h_mbs, p->mbs, p->n_mbs * sizeof(rv_h264_mb));
         if (p->n_parts) std::memcpy(h_parts, p->parts, p->n_parts * sizeof(rv_h264_part));
+        // Motion compensation takes partitions above 8 x 8 one per
+        // workgroup and the rest four per workgroup: the larger first.
+        int n_big = 0;
+        for (int i = 0; i < p->n_parts; ++i)
+            if (p->parts[i].w > 8 || p->parts[i].h > 8) h_order[n_big++] = (uint32_t)i;
+        for (int i = 0, k = n_big; i < p->n_parts; ++i)
+            if (!(p->parts[i].w > 8 || p->parts[i].h > 8)) h_order[k++] = (uint32_t)i;
         if (!blks.empty()) std::memcpy(h_blks, blks.data(), blks.size() * sizeof(rv_h264_blk));
         if (p->n_coefs) std::memcpy(h_coefs, p->coefs, p->n_coefs * sizeof(int32_t));
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
@@ -1043,6 +1052,8 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.n_blks8 = (int)b8.size();
         d.mbs = (const rv_h264_mb*)dev((char*)h_mbs);
         d.parts = (const rv_h264_part*)dev((char*)h_parts);
+        d.part_order = (const uint32_t*)dev((char*)h_order);
+        d.n_big = n_big;
         d.blks = (const rv_h264_blk*)dev((char*)h_blks);
         d.coefs = (const int32_t*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
