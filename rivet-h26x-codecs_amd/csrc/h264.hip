@@ -396,6 +396,132 @@ __device__ void residual8(uint8_t* dst, long pitch, const int32_t* co, int max =
     idct8_add16(dst, pitch, d, max);
 }
 
+// Stage 1 reads and writes a block a row at a time: a 4-sample row is one
+// dword, an 8-sample row two (block columns are multiples of the block
+// size and plane rows 256-byte aligned), each byte updated as add_px /
+// dc_add16 do; coefficients load as 16-byte vectors when so aligned.
+__device__ __forceinline__ uint32_t add_px_word(uint32_t px, const int16_t* r, int max) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int16_t v = sh(wa(r[k], 32), 6);
+        o |= (uint32_t)clampi((int)wa((int16_t)((px >> (8 * k)) & 0xFF), v), 0, max) << (8 * k);
+    }
+    return o;
+}
+__device__ __forceinline__ uint32_t dc_word(uint32_t px, int v, int max) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o |= (uint32_t)clampi((int)((px >> (8 * k)) & 0xFF) + v, 0, max) << (8 * k);
+    return o;
+}
+template <int N> __device__ __forceinline__ void load_coefs(const int32_t* co, int (&v)[N]) {
+    if (((uintptr_t)co & 15) == 0) {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const int4 t = ((const int4*)co)[q];
+            v[4 * q] = t.x, v[4 * q + 1] = t.y, v[4 * q + 2] = t.z, v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = co[i];
+    }
+}
+
+// residual4 with idct4_add16 / dc_add16, by rows.
+__device__ __forceinline__ void residual4_rows(uint8_t* dst, long pitch, const int32_t* co, int has_dc, int dc) {
+    int c[16];
+    load_coefs<16>(co, c);
+    int16_t d[16];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        d[i] = (int16_t)clampi(c[i], -32768, 32767);
+        if (i) ac |= c[i];
+    }
+    if (has_dc) d[0] = (int16_t)dc;
+    if (ac == 0) {
+        if (d[0] != 0) {
+            const int v = (int)sh(wa(d[0], 32), 6);
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                uint32_t* r = (uint32_t*)(dst + y * pitch);
+                *r = dc_word(*r, v, 255);
+            }
+        }
+        return;
+    }
+    int16_t t[16], res[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int16_t d0 = d[i * 4], d1 = d[i * 4 + 1], d2 = d[i * 4 + 2], d3 = d[i * 4 + 3];
+        int16_t e0 = wa(d0, d2), e1 = ws(d0, d2), e2 = ws(sh(d1, 1), d3), e3 = wa(d1, sh(d3, 1));
+        t[i * 4] = wa(e0, e3);
+        t[i * 4 + 1] = wa(e1, e2);
+        t[i * 4 + 2] = ws(e1, e2);
+        t[i * 4 + 3] = ws(e0, e3);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int16_t f0 = t[j], f1 = t[4 + j], f2 = t[8 + j], f3 = t[12 + j];
+        int16_t g0 = wa(f0, f2), g1 = ws(f0, f2), g2 = ws(sh(f1, 1), f3), g3 = wa(f1, sh(f3, 1));
+        res[j] = wa(g0, g3);
+        res[4 + j] = wa(g1, g2);
+        res[8 + j] = ws(g1, g2);
+        res[12 + j] = ws(g0, g3);
+    }
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        uint32_t* r = (uint32_t*)(dst + y * pitch);
+        *r = add_px_word(*r, res + 4 * y, 255);
+    }
+}
+
+// residual8 with idct8_add16 / dc_add16, by rows.
+__device__ __forceinline__ void residual8_rows(uint8_t* dst, long pitch, const int32_t* co) {
+    int c[64];
+    load_coefs<64>(co, c);
+    int16_t d[64];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        d[i] = (int16_t)clampi(c[i], -32768, 32767);
+        if (i) ac |= c[i];
+    }
+    if (ac == 0) {
+        if (d[0] != 0) {
+            const int v = (int)sh(wa(d[0], 32), 6);
+#pragma unroll
+            for (int y = 0; y < 8; ++y) {
+                uint2* r = (uint2*)(dst + y * pitch);
+                uint2 px = *r;
+                px.x = dc_word(px.x, v, 255), px.y = dc_word(px.y, v, 255);
+                *r = px;
+            }
+        }
+        return;
+    }
+    int16_t t[64], res[64];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) idct8_1d(d + i * 8, t + i * 8);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int16_t col[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) col[k] = t[k * 8 + j];
+        idct8_1d(col, o);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) res[i * 8 + j] = o[i];
+    }
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+        uint2* r = (uint2*)(dst + y * pitch);
+        uint2 px = *r;
+        px.x = add_px_word(px.x, res + 8 * y, 255), px.y = add_px_word(px.y, res + 8 * y + 4, 255);
+        *r = px;
+    }
+}
+
 __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     const int32_t* pre = b.prefix + 1 * (b.n + 1);
     const int it = blockIdx.x;
@@ -414,8 +540,8 @@ __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     }
     const rv_h264_blk bk = P.blks[k];
     uint8_t* dst = (uint8_t*)P.cur[bk.comp] + (size_t)bk.y * P.pitch[bk.comp] + bk.x;
-    if (bk.size == 8) residual8(dst, P.pitch[bk.comp], P.coefs + bk.coef_off);
-    else residual4(dst, P.pitch[bk.comp], P.coefs + bk.coef_off, bk.has_dc, bk.dc);
+    if (bk.size == 8) residual8_rows(dst, P.pitch[bk.comp], P.coefs + bk.coef_off);
+    else residual4_rows(dst, P.pitch[bk.comp], P.coefs + bk.coef_off, bk.has_dc, bk.dc);
 }
 
 // ---------------------------------------------------------------------
