@@ -47,18 +47,25 @@ def log(*a):
 TRAFFIC_FILE = ROOT / "profiles" / "hevc_c3_8x4k_stage0_traffic.json"
 
 
-def pmc_traffic(kernel: str, streams: int):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
-    scaled by what the counter reads for a copy kernel of known size), or
-    None when the summary does not cover this kernel and workload."""
+def pmc_traffic(kernel: str, streams: int, variant: str = ""):
+    """(HBM bytes per launch, kernel symbol) of `kernel` on this workload,
+    from the committed rocprofv3 --pmc summary: FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by what the counter reads for a copy kernel
+    of known size, for the kernel variant `variant` when the summary holds
+    its counters. (None, None) when it does not cover this kernel and
+    workload."""
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
-        return None
+        return None, None
     if d.get("kernel") != kernel or streams != 8:
-        return None
-    return int(d["hbm_bytes_per_launch"])
+        return None, None
+    sym = d.get("variant_symbols", {}).get(variant)
+    kib = d.get("all_stage0_variants_per_dispatch_kib", {}).get(sym) if sym else None
+    if kib:
+        cal = d["calibration"]
+        return int(round((kib["fetch"] * cal["fetch_factor"] + kib["write"] * cal["write_factor"]) * 1024)), sym
+    return int(d["hbm_bytes_per_launch"]), d.get("variant", "").split(":")[0]
 
 
 def cpu_model() -> str:
@@ -232,6 +239,8 @@ def main() -> int:
     stage_gbs = [b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0 for b, ms in zip(stage_bytes, stage_ms)]
     dom = int(np.argmax(stage_ms))
     value = luma / t_graph / 1e6 if bit_exact else None
+    chosen = {k: names[v] for k, names, v in zip(STAGES, rh.variant_names(), batch.variants())}
+    traffic, traffic_kernel = pmc_traffic(f"hevc_stage{dom}", args.streams, chosen[STAGES[dom]])
     roofline = {
         "bound": "hbm",
         "kernel": f"hevc_stage{dom} ({STAGES[dom]})",
@@ -239,9 +248,9 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
-        "traffic": pmc_traffic(f"hevc_stage{dom}", args.streams),
-        "traffic_source": TRAFFIC_FILE.relative_to(ROOT).as_posix() + " (rocprofv3 --pmc FETCH_SIZE x2.0 + "
-                          "WRITE_SIZE x1.0, factors from the copy kernel)",
+        "traffic": traffic,
+        "traffic_source": TRAFFIC_FILE.relative_to(ROOT).as_posix() + f" ({traffic_kernel}: rocprofv3 --pmc "
+                          "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from the copy kernel)",
         "algorithmic_bytes": int(stage_bytes[dom]),
         "avg_launch_ms": round(stage_ms[dom], 4),
         "achievable_copy_gbs": round(copy_gbs, 1),
