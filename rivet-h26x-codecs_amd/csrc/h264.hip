@@ -616,88 +616,64 @@ __device__ uint2 edge_words(const H264PicDev& P, uint32_t bs, int qa) {
     return make_uint2((uint32_t)kAlpha[ia] | (uint32_t)kBeta[ib] << 8 | (bs4 ? 1u << 16 : 0u), segs);
 }
 
-__device__ void mb_bs(const H264PicDev& P, int addr) {
+// Packed strengths (one byte per 4-line segment) of edge e (0..3) of
+// macroblock addr, vertical or horizontal: bS 4 on an intra macroblock's
+// macroblock edges and 3 inside it; else 2 where a side has coefficients,
+// else motion, only across a partition boundary inside the macroblock.
+__device__ uint32_t edge_bs(const H264PicDev& P, int addr, const rv_h264_mb& m, bool vert, int e) {
     const int mbx = addr % P.mbw, mby = addr / P.mbw;
-    const rv_h264_mb m = P.mbs[addr];
-    uint32_t bv[4] = {0, 0, 0, 0}, bh[4] = {0, 0, 0, 0};
-    const bool fl = mbx > 0, ft = mby > 0;
     const int bx0 = mbx * 4, by0 = mby * 4;
-    if (m.intra) {
-        if (fl) bv[0] = 0x04040404u;
-        if (ft) bh[0] = 0x04040404u;
-        for (int e = 1; e < 4; ++e) bv[e] = bh[e] = 0x03030303u;
-    } else {
-        const uint32_t nz = m.nz_mask;
-        const int step = m.transform_8x8 ? 2 : 1;
-        // Internal edges: a coefficient on either side settles bS 2; else
-        // motion, only across a partition boundary.
-        for (int e = step; e < 4; e += step)
-            for (int k = 0; k < 4; ++k) {
-                // Vertical edge e, block row k.
-                int pb = k * 4 + e - 1, qb = k * 4 + e;
-                uint32_t v;
-                if (((nz >> pb) | (nz >> qb)) & 1) v = 2;
-                else if (partof(P, bx0 + e - 1, by0 + k) != partof(P, bx0 + e, by0 + k))
-                    v = motion_bs(mot_of(P, bx0 + e - 1, by0 + k), mot_of(P, bx0 + e, by0 + k));
-                else v = 0;
-                bv[e] |= v << (8 * k);
-                // Horizontal edge e, block column k.
-                pb = (e - 1) * 4 + k;
-                qb = e * 4 + k;
-                if (((nz >> pb) | (nz >> qb)) & 1) v = 2;
-                else if (partof(P, bx0 + k, by0 + e - 1) != partof(P, bx0 + k, by0 + e))
-                    v = motion_bs(mot_of(P, bx0 + k, by0 + e - 1), mot_of(P, bx0 + k, by0 + e));
-                else v = 0;
-                bh[e] |= v << (8 * k);
-            }
-        if (fl) {
-            const rv_h264_mb ml = P.mbs[addr - 1];
-            if (ml.intra) bv[0] = 0x04040404u;
-            else
-                for (int k = 0; k < 4; ++k) {
-                    uint32_t v;
-                    if (((ml.nz_mask >> (k * 4 + 3)) | (nz >> (k * 4))) & 1) v = 2;
-                    else v = motion_bs(mot_of(P, bx0 - 1, by0 + k), mot_of(P, bx0, by0 + k));
-                    bv[0] |= v << (8 * k);
-                }
-        }
-        if (ft) {
-            const rv_h264_mb ma = P.mbs[addr - P.mbw];
-            if (ma.intra) bh[0] = 0x04040404u;
-            else
-                for (int k = 0; k < 4; ++k) {
-                    uint32_t v;
-                    if (((ma.nz_mask >> (12 + k)) | (nz >> k)) & 1) v = 2;
-                    else v = motion_bs(mot_of(P, bx0 + k, by0 - 1), mot_of(P, bx0 + k, by0));
-                    bh[0] |= v << (8 * k);
-                }
-        }
-    }
-    // The edge parameters of this macroblock, one word per edge and segment.
-    const rv_h264_mb ml = P.mbs[fl ? addr - 1 : addr], ma = P.mbs[ft ? addr - P.mbw : addr];
-    const bool odd = !m.transform_8x8;
-    uint32_t w[kH264DbkWords];
-    for (int e = 0; e < 4; ++e) {
-        const bool on = e == 0 || (e & 1) == 0 || odd;  // 8x8 transform: no luma edges 1, 3
-        const uint2 v = edge_words(P, on ? bv[e] : 0u, e == 0 ? (ml.qp + m.qp + 1) >> 1 : m.qp);
-        const uint2 h = edge_words(P, on ? bh[e] : 0u, e == 0 ? (ma.qp + m.qp + 1) >> 1 : m.qp);
+    const bool has = vert ? mbx > 0 : mby > 0;  // the macroblock edge exists
+    if (m.intra) return e == 0 ? (has ? 0x04040404u : 0u) : 0x03030303u;
+    const uint32_t nz = m.nz_mask;
+    uint32_t w = 0;
+    if (e > 0) {
+        if (m.transform_8x8 && (e & 1)) return 0u;
         for (int k = 0; k < 4; ++k) {
-            w[k * 4 + e] = v.x | ((v.y >> (8 * k)) & 0xFF) << 24;
-            w[16 + k * 4 + e] = h.x | ((h.y >> (8 * k)) & 0xFF) << 24;
+            // Vertical: block row k across columns e - 1 | e; horizontal: the transpose.
+            const int pb = vert ? k * 4 + e - 1 : (e - 1) * 4 + k, qb = vert ? k * 4 + e : e * 4 + k;
+            const int px = vert ? bx0 + e - 1 : bx0 + k, py = vert ? by0 + k : by0 + e - 1;
+            const int qx = vert ? bx0 + e : bx0 + k, qy = vert ? by0 + k : by0 + e;
+            uint32_t v;
+            if (((nz >> pb) | (nz >> qb)) & 1) v = 2;
+            else if (partof(P, px, py) != partof(P, qx, qy)) v = motion_bs(mot_of(P, px, py), mot_of(P, qx, qy));
+            else v = 0;
+            w |= v << (8 * k);
         }
+        return w;
     }
-    for (int c = 0; c < 2; ++c)
-        for (int e = 0; e < 2; ++e) {
-            // Chroma edges 0 and 4 take the strengths of luma edges 0 and 8.
-            const uint2 v = edge_words(P, bv[2 * e], e == 0 ? (ml.qpc[c] + m.qpc[c] + 1) >> 1 : m.qpc[c]);
-            const uint2 h = edge_words(P, bh[2 * e], e == 0 ? (ma.qpc[c] + m.qpc[c] + 1) >> 1 : m.qpc[c]);
-            for (int k = 0; k < 4; ++k) {
-                w[32 + c * 8 + k * 2 + e] = v.x | ((v.y >> (8 * k)) & 0xFF) << 24;
-                w[48 + c * 8 + k * 2 + e] = h.x | ((h.y >> (8 * k)) & 0xFF) << 24;
-            }
-        }
-    uint4* dst = (uint4*)(P.dbk + (size_t)addr * kH264DbkWords);
-    for (int k = 0; k < kH264DbkWords / 4; ++k) dst[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    if (!has) return 0u;
+    const rv_h264_mb n = P.mbs[vert ? addr - 1 : addr - P.mbw];
+    if (n.intra) return 0x04040404u;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t v;
+        const bool coef = vert ? (((n.nz_mask >> (k * 4 + 3)) | (nz >> (k * 4))) & 1)
+                               : (((n.nz_mask >> (12 + k)) | (nz >> k)) & 1);
+        if (coef) v = 2;
+        else if (vert) v = motion_bs(mot_of(P, bx0 - 1, by0 + k), mot_of(P, bx0, by0 + k));
+        else v = motion_bs(mot_of(P, bx0 + k, by0 - 1), mot_of(P, bx0 + k, by0));
+        w |= v << (8 * k);
+    }
+    return w;
+}
+
+// One edge's parameter words (layout of kH264DbkWords): its luma words and,
+// for luma edges 0 and 2 (chroma edges 0 and 4), both chroma planes'.
+__device__ void edge_params(const H264PicDev& P, int addr, bool vert, int e) {
+    const rv_h264_mb m = P.mbs[addr];
+    const uint32_t bs = edge_bs(P, addr, m, vert, e);
+    const int mbx = addr % P.mbw, mby = addr / P.mbw;
+    const bool has = vert ? mbx > 0 : mby > 0;
+    const rv_h264_mb n = P.mbs[has ? (vert ? addr - 1 : addr - P.mbw) : addr];  // QP neighbour of edge 0
+    uint32_t* w = P.dbk + (size_t)addr * kH264DbkWords + (vert ? 0 : 16);
+    const bool on = e == 0 || (e & 1) == 0 || !m.transform_8x8;  // 8x8 transform: no luma edges 1, 3
+    const uint2 lw = edge_words(P, on ? bs : 0u, e == 0 ? (n.qp + m.qp + 1) >> 1 : m.qp);
+    for (int k = 0; k < 4; ++k) w[k * 4 + e] = lw.x | ((lw.y >> (8 * k)) & 0xFF) << 24;
+    if (e & 1) return;
+    for (int c = 0; c < 2; ++c) {
+        const uint2 cw = edge_words(P, bs, e == 0 ? (n.qpc[c] + m.qpc[c] + 1) >> 1 : m.qpc[c]);
+        for (int k = 0; k < 4; ++k) w[32 + c * 8 + k * 2 + (e >> 1)] = cw.x | ((cw.y >> (8 * k)) & 0xFF) << 24;
+    }
 }
 
 __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
@@ -708,7 +684,8 @@ __global__ void __launch_bounds__(64) h264_stage2(BatchDev b) {
     const int i = (it - pre[p]) * 64 + threadIdx.x;
     // The deblocking exchange and band ticket start at zero every launch.
     for (size_t k = i; k < (size_t)P.n_xch; k += (size_t)P.items[2] * 64) P.xch[k] = 0;
-    if (i < P.n_mbs) mb_bs(P, i);
+    // One thread per macroblock edge: bits 0..1 the edge, bit 2 horizontal.
+    if (i < P.n_mbs * 8) edge_params(P, i >> 3, !((i >> 2) & 1), i & 3);
 }
 
 // ---------------------------------------------------------------------
@@ -1227,7 +1204,7 @@ hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st) {
 void h264_stage_items(H264PicDev& P) {
     P.items[0] = P.n_mbs + P.n_big + (P.n_parts - P.n_big + 3) / 4;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
-    P.items[2] = (P.n_mbs + 1 + 63) / 64;
+    P.items[2] = (P.n_mbs * 8 + 63) / 64;
     P.items[3] = P.deblock ? h264_deblock_bands(P.mbh) : 0;  // one workgroup per band
     P.items[4] = 0;
 }
