@@ -143,6 +143,8 @@ template <int NT> struct PartShape {
     static constexpr int LWIN = LR * LS, CWIN = CR * CS;
     static constexpr int LDS = 2 * LWIN + 4 * CWIN;  // int16 per partition
     static constexpr int LPASS = (LR * LS + NT - 1) / NT, CPASS = (CR * CS + NT - 1) / NT;
+    static constexpr int ND = MW / 4 + 2;  // aligned dwords covering a luma window row
+    static constexpr int DPASS = (LR * ND + NT - 1) / NT;
 };
 constexpr int kStage0Lds = PartShape<64>::LDS > 4 * PartShape<16>::LDS ? PartShape<64>::LDS : 4 * PartShape<16>::LDS;
 
@@ -161,20 +163,37 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
     // Every window's samples are requested before any is stored, so the
     // workgroup waits for memory once. Windows clamp to the picture (the
     // replicated border of src/h264/frame.rs and the gather of
-    // src/h264/inter.rs:127-139 read the same samples).
+    // src/h264/inter.rs:127-139 read the same samples). A luma window whose
+    // columns lie inside the plane is read as aligned dwords of clamped
+    // rows; one crossing the left or right edge sample by sample.
     const int ww = w + 5, hh = h + 5, cw = w >> 1, ch = h >> 1;
     int lv[2][S::LPASS], cv[2][2][S::CPASS];
+    int shl[2] = {0, 0}, ndl[2] = {0, 0};
+    bool ins[2] = {false, false};
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
         const int mvx = pt.mv[l].x, mvy = pt.mv[l].y;
         const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
         const int x0 = x + (mvx >> 2) - 2, y0 = y + (mvy >> 2) - 2;
+        const int sh = x0 & 3, nd = (sh + ww + 3) >> 2;
+        shl[l] = sh, ndl[l] = nd;
+        ins[l] = x0 >= 0 && x0 - sh + 4 * nd <= P.pw[0];
+        if (ins[l]) {
 #pragma unroll
-        for (int k = 0; k < S::LPASS; ++k) {
-            const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
-            if (c < ww && r < hh)
-                lv[l][k] = ref[(size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] + clampi(x0 + c, 0, P.pw[0] - 1)];
+            for (int k = 0; k < S::DPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::ND, d = e - r * S::ND;
+                if (d < nd && r < hh)
+                    lv[l][k] = (int)*(const uint32_t*)(ref + (size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] +
+                                                       (x0 - sh) + 4 * d);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S::LPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
+                if (c < ww && r < hh)
+                    lv[l][k] = ref[(size_t)clampi(y0 + r, 0, P.ph[0] - 1) * P.pitch[0] + clampi(x0 + c, 0, P.pw[0] - 1)];
+            }
         }
         const int xc0 = (x >> 1) + (mvx >> 3), yc0 = (y >> 1) + (mvy >> 3);
 #pragma unroll
@@ -192,10 +211,23 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
+        if (ins[l]) {
 #pragma unroll
-        for (int k = 0; k < S::LPASS; ++k) {
-            const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
-            if (c < ww && r < hh) win[l * S::LWIN + r * S::LS + c] = (int16_t)lv[l][k];
+            for (int k = 0; k < S::DPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::ND, d = e - r * S::ND;
+                if (d < ndl[l] && r < hh)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int c = 4 * d - shl[l] + j;
+                        if (c >= 0 && c < ww) win[l * S::LWIN + r * S::LS + c] = (int16_t)(((uint32_t)lv[l][k] >> (8 * j)) & 0xFF);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S::LPASS; ++k) {
+                const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
+                if (c < ww && r < hh) win[l * S::LWIN + r * S::LS + c] = (int16_t)lv[l][k];
+            }
         }
 #pragma unroll
         for (int c2 = 0; c2 < 2; ++c2)
@@ -244,7 +276,7 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
     }
 }
 
-__global__ void __launch_bounds__(64) h264_stage0(BatchDev b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h264_stage0(BatchDev b) {
     __shared__ int16_t win[kStage0Lds];
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = blockIdx.x;
