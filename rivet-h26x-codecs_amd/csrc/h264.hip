@@ -773,12 +773,17 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 
 // Stage 3 cuts each picture into bands of 16 macroblock rows, one
 // workgroup of 256 threads per band: 16 slots of 16 lanes, slot s filtering
-// row 16 * band + s. Row s of a band starts 2 steps after row s - 1 and
-// takes one macroblock per step, so MB (x, y) runs after MB (x + 1, y - 1)
-// and MB (x - 1, y) — the two it reads — with one barrier per step. Bands
-// take their index from a per-picture ticket, so a band's predecessor is
-// always running. The first row of a band takes the strips of the row
-// above from the band before through the exchange below.
+// row 16 * band + s. Each row takes one macroblock per step, with one
+// barrier per step, and each step filters the vertical edges before the
+// horizontal ones. MB (x, y) needs MB (x - 1, y) complete and, for its top
+// edge only, MB (x + 1, y - 1)'s vertical edges (the left edge of that MB
+// filters the bottom-right corner of MB (x, y - 1)). So row s starts one
+// step after row s - 1 when both are in one wave (the wave's program order
+// puts the row above's vertical edges first) and two steps after it across
+// waves (behind the barrier): dbk_lag(). Bands take their index from a
+// per-picture ticket, so a band's predecessor is always running. The first
+// row of a band takes the strips of the row above from the band before
+// through the exchange below.
 //
 // Dependencies through the samples. MB (x, y) reads its own unfiltered
 // samples, columns 12..15 of MB (x - 1, y) as that MB left them, and rows
@@ -797,7 +802,8 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 //   ring  per macroblock of the slot's row, rows 12..15 of each luma
 //         column and rows 6..7 of each chroma column, final but for the
 //         row below's top edge, which reads entry x at its step x (3
-//         entries: the row below runs 2 macroblocks behind).
+//         entries: across waves the row below runs 2 macroblocks behind,
+//         while this row fills entries x + 1 and x + 2).
 //   exchange  the last row of a band publishes each ring entry, once
 //         complete, as 24 64-bit words (data | tag << 32, tag = x + 1),
 //         relaxed atomics at device scope: each word is coherent on its own,
@@ -817,6 +823,11 @@ constexpr int kDbkPer = kDbkTile + kDbkRing * kDbkRingN;
 // running and publishes within a few steps); a poll that reaches it goes on
 // with what it read rather than hang the workgroup.
 constexpr int kDbkSpin = 1 << 18;
+static_assert(kDbkSlots * 16 == 256 && 4 * 16 == 64, "four slots of 16 lanes per 64-lane wave");
+
+// The step at which slot s reaches macroblock 0: one step after slot s - 1
+// in the same wave, two across a wave boundary.
+__device__ __forceinline__ int dbk_lag(int s) { return s + (s >> 2); }
 
 // |a - b| of two samples (0..255) in one v_sad_u8.
 __device__ __forceinline__ int absd(int a, int b) { return (int)__builtin_amdgcn_sad_u8((uint32_t)a, (uint32_t)b, 0u); }
@@ -1006,9 +1017,9 @@ __global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
     const uint32_t py = D.py;
     const size_t pc = D.pc;
     const int y = y0 + slot;
-    const int steps = 2 * (rows - 1) + mbw + (y0 + rows < mbh ? 2 : 0);
+    const int steps = dbk_lag(rows - 1) + mbw + (y0 + rows < mbh ? 2 : 0);
 
-    int x = -2 * slot, ex = 0;  // the slot's macroblock at the next step, x % 3
+    int x = -dbk_lag(slot), ex = 0;  // the slot's macroblock at the next step, x % 3
     bool act = slot < rows && x == 0;
     DbkIn ba, bb;  // this step's inputs and the next step's, alternating
     if (act) {
