@@ -772,8 +772,10 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 }
 
 // Stage 3 cuts each picture into bands of 16 macroblock rows, one
-// workgroup of 256 threads per band: 16 slots of 16 lanes, slot s filtering
-// row 16 * band + s. Each row takes one macroblock per step, with one
+// workgroup of 512 threads per band: 16 slots, slot s filtering row
+// 16 * band + s with 16 luma lanes (waves 0..3) and 16 chroma lanes (waves
+// 4..7); luma and chroma filtering are independent, so each lane's chain of
+// edges per step is one plane's. Each row takes one macroblock per step, with one
 // barrier per step, and each step filters the vertical edges before the
 // horizontal ones. MB (x, y) needs MB (x - 1, y) complete and, for its top
 // edge only, MB (x + 1, y - 1)'s vertical edges (the left edge of that MB
@@ -810,9 +812,10 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 //         so the first row of the next band, which polls its words until the
 //         tag matches, needs no cache-wide release or acquire. The strength
 //         stage zeroes the exchange and the ticket before every launch.
-// A lane filters luma row r and chroma row r & 7 of Cb (r < 8) or Cr
-// across the vertical edges, then luma column r and chroma column r & 7
-// across the horizontal edges, the samples of a line in registers.
+// Lane r of a slot filters luma row r (luma waves), or chroma row r & 7 of
+// Cb (r < 8) or Cr (chroma waves), across the vertical edges, then the
+// column of the same index across the horizontal edges, the samples of a
+// line in registers.
 
 constexpr int kDbkSlots = 16;  // rows per band
 constexpr int kDbkTile = 96;   // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
@@ -910,16 +913,15 @@ __device__ __forceinline__ void dbk_line(int (&v)[N], uint32_t a) {
     }
 }
 
-// One lane's inputs for one macroblock, fetched a step ahead: its samples
-// and the parameter words of its lines' segments; in a band's first row,
-// also its exchange words (luma column r; the chroma pair holding column
-// r & 7 of its plane).
+// One lane's inputs for one macroblock, fetched a step ahead: its line's
+// samples (luma row r, or chroma row r & 7 of the lane's plane in .x, .y)
+// and the parameter words of the line's segments across the vertical and
+// the horizontal edges (luma edges 0..3, chroma edges 0 and 4 in .x, .y);
+// in a band's first row, also its exchange word (luma column r; the chroma
+// pair holding column r & 7 of its plane).
 struct DbkIn {
-    uint4 y;       // luma row r
-    uint2 c;       // chroma row r & 7 of the lane's plane
-    uint4 lv, lh;  // luma vertical, horizontal edges 0..3
-    uint2 cv, ch;  // chroma vertical, horizontal edges 0, 4
-    uint64_t il, ic;
+    uint4 s, pv, ph;
+    uint64_t ix;
 };
 
 // Global-memory pointers: pointers read from a descriptor are generic, and
@@ -948,14 +950,19 @@ struct DbkPic {
     int mbw, mbh;
 };
 
-__device__ __forceinline__ void dbk_fetch(const DbkPic& D, int x, int y, int r, DbkIn& in) {
-    const int pl = r >> 3, rr = r & 7;
-    in.y = gld4(D.y + ((uint32_t)(16 * y + r) * D.py + 16 * x));
-    in.c = gld2(D.c + (size_t)(8 * y + rr) * D.pc + 8 * x);
+__device__ __forceinline__ void dbk_fetch(const DbkPic& D, bool lu, int x, int y, int r, DbkIn& in) {
     const gptr<const uint32_t> pb = D.prm + ((size_t)y * D.mbw + x) * kH264DbkWords;
-    const int sl = r >> 2, sc = rr >> 1;  // segments of the luma and chroma lines
-    in.lv = gld4(pb + sl * 4), in.lh = gld4(pb + 16 + sl * 4);
-    in.cv = gld2(pb + 32 + pl * 8 + sc * 2), in.ch = gld2(pb + 48 + pl * 8 + sc * 2);
+    if (lu) {
+        const int sl = r >> 2;  // segment of the line
+        in.s = gld4(D.y + ((uint32_t)(16 * y + r) * D.py + 16 * x));
+        in.pv = gld4(pb + sl * 4), in.ph = gld4(pb + 16 + sl * 4);
+    } else {
+        const int pl = r >> 3, rr = r & 7, sc = rr >> 1;  // plane, line, segment
+        const uint2 c = gld2(D.c + (size_t)(8 * y + rr) * D.pc + 8 * x);
+        const uint2 v = gld2(pb + 32 + pl * 8 + sc * 2), h = gld2(pb + 48 + pl * 8 + sc * 2);
+        // Whole vectors on both paths keep DbkIn in registers.
+        in.s = make_uint4(c.x, c.y, 0u, 0u), in.pv = make_uint4(v.x, v.y, 0u, 0u), in.ph = make_uint4(h.x, h.y, 0u, 0u);
+    }
 }
 
 // Workgroup barrier that orders LDS only: the next step's prefetched
@@ -982,7 +989,7 @@ __device__ __forceinline__ uint64_t xch_wait(gptr<const uint64_t> p, uint64_t v,
     return v;
 }
 
-__global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
+__global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
     __shared__ uint32_t dl[kDbkSlots * kDbkPer];
     __shared__ int band_s;
     const int32_t* pre = b.prefix + 3 * (b.n + 1);
@@ -995,19 +1002,23 @@ __global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
     __syncthreads();
     const int band = band_s;
     if (band >= h264_deblock_bands(P.mbh)) return;  // only with a stale ticket
-    const int slot = (int)threadIdx.x >> 4, r = (int)threadIdx.x & 15;
-    const int pl = r >> 3, rr = r & 7;  // chroma plane, row / column of this lane
+    const bool lu = threadIdx.x < 256;  // waves 0..3 luma, 4..7 chroma
+    const int t = (int)threadIdx.x & 255;
+    const int slot = t >> 4, r = t & 15;
+    const int pl = r >> 3, rr = r & 7;  // chroma lanes: plane, row / column
     const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
                    (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh};
     const int mbw = D.mbw, mbh = D.mbh;
     const int y0 = band * kDbkSlots, rows = min(kDbkSlots, mbh - y0);
-    const bool first = slot == 0 && band > 0;                       // imports its strips
-    const bool last = slot == rows - 1 && y0 + rows < mbh;          // publishes its strips
+    const bool first = slot == 0 && band > 0;  // imports its strips
+    // Publishes its strips: the band's last row, one word per luma lane and
+    // per chroma lane 0..7.
+    const bool last = slot == rows - 1 && y0 + rows < mbh && (lu || r < 8);
     // Exchange words of this lane: imported (boundary band - 1) and
     // published (boundary band), macroblock x at + 24 x.
-    const gptr<const uint64_t> imp_l = xch + 1 + (size_t)max(band - 1, 0) * mbw * kDbkRing + r;
-    const gptr<const uint64_t> imp_c = imp_l + 16 - r + (r >> 1);
-    const gptr<uint64_t> pub = xch + 1 + (size_t)band * mbw * kDbkRing + r;
+    const gptr<const uint64_t> imp = xch + 1 + (size_t)max(band - 1, 0) * mbw * kDbkRing + (lu ? r : 16 + (r >> 1));
+    const int pw = lu ? r : 16 + r;  // ring word this lane publishes
+    const gptr<uint64_t> pub = xch + 1 + (size_t)band * mbw * kDbkRing + pw;
     uint32_t* const tile = dl + slot * kDbkPer;
     uint8_t* const tb = (uint8_t*)tile;
     uint32_t* const ring = tile + kDbkTile;
@@ -1023,8 +1034,8 @@ __global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
     bool act = slot < rows && x == 0;
     DbkIn ba, bb;  // this step's inputs and the next step's, alternating
     if (act) {
-        dbk_fetch(D, 0, y, r, ba);
-        if (first) ba.il = xch_load(imp_l), ba.ic = xch_load(imp_c);
+        dbk_fetch(D, lu, 0, y, r, ba);
+        if (first) ba.ix = xch_load(imp);
     }
     // One step: filter the macroblock `in` holds, fetch the next into `nx`.
     auto step = [&](const DbkIn& in, DbkIn& nx) __attribute__((always_inline)) {
@@ -1038,108 +1049,107 @@ __global__ void __launch_bounds__(256) h264_stage3(BatchDev b) {
             const int xp = cx - 2;
             if (xp >= 0 && xp < mbw) {
                 const uint32_t* e = ring + (cex == 0 ? 1 : cex == 1 ? 2 : 0) * kDbkRing;
-                const uint64_t tag = (uint64_t)(xp + 1) << 32;
-                xch_store(pub + (size_t)xp * kDbkRing, tag | e[r]);
-                if (r < 8) xch_store(pub + (size_t)xp * kDbkRing + 16, tag | e[16 + r]);
+                xch_store(pub + (size_t)xp * kDbkRing, ((uint64_t)(xp + 1) << 32) | e[pw]);
             }
         }
         ++x;
         act = slot < rows && x >= 0 && x < mbw;
         ex = x <= 0 ? 0 : (ex + 1 == kDbkRingN ? 0 : ex + 1);  // x % 3
         if (act) {
-            dbk_fetch(D, x, y, r, nx);
-            if (first) nx.il = xch_load(imp_l + (size_t)x * kDbkRing), nx.ic = xch_load(imp_c + (size_t)x * kDbkRing);
+            dbk_fetch(D, lu, x, y, r, nx);
+            if (first) nx.ix = xch_load(imp + (size_t)x * kDbkRing);
         }
         if (cact) {
             const bool lastc = cx == mbw - 1, lastr = cy == mbh - 1;
-            // Vertical edges: luma row r, chroma row rr.
-            int v[20], w[10];
-            unpk4<0>(v, tile[r * 4 + 3]);  // previous macroblock, columns 12..15
-            unpk4<4>(v, in.y.x), unpk4<8>(v, in.y.y), unpk4<12>(v, in.y.z), unpk4<16>(v, in.y.w);
-            {
-                const uint32_t lc = tile[64 + pl * 16 + rr * 2 + 1];  // columns 4..7
+            uint8_t* const le = (uint8_t*)(ring + (cex == 0 ? kDbkRingN - 1 : cex - 1) * kDbkRing);  // entry x - 1
+            // Vertical edges: luma row r or chroma row rr. Then the left
+            // neighbour's columns 12..15 (chroma 7) are final but for rows
+            // 13..15 (chroma 7) outside the last row, which go to its ring
+            // entry.
+            if (lu) {
+                int v[20];
+                unpk4<0>(v, tile[r * 4 + 3]);  // previous macroblock, columns 12..15
+                unpk4<4>(v, in.s.x), unpk4<8>(v, in.s.y), unpk4<12>(v, in.s.z), unpk4<16>(v, in.s.w);
+                dbk_line<4, true, true>(v, in.pv.x);
+                dbk_line<8, true, false>(v, in.pv.y);
+                dbk_line<12, true, false>(v, in.pv.z);
+                dbk_line<16, true, false>(v, in.pv.w);
+                *(uint4*)&tile[r * 4] =
+                    make_uint4(pk4(v[4], v[5], v[6], v[7]), pk4(v[8], v[9], v[10], v[11]),
+                               pk4(v[12], v[13], v[14], v[15]), pk4(v[16], v[17], v[18], v[19]));
+                if (cx > 0) {
+                    if (r <= 12 || lastr)
+                        *(gptr<uint32_t>)(Y + ((uint32_t)(16 * cy + r) * py + 16 * cx - 4)) = pk4(v[0], v[1], v[2], v[3]);
+                    if (!lastr && r >= 12)
+                        for (int k = 0; k < 4; ++k) le[(12 + k) * 4 + r - 12] = (uint8_t)v[k];
+                }
+            } else {
+                int w[10];
+                const uint32_t lc = tile[64 + pl * 16 + rr * 2 + 1];  // previous macroblock, columns 4..7
                 w[0] = (lc >> 16) & 0xFF, w[1] = lc >> 24;
-                unpk4<2>(w, in.c.x), unpk4<6>(w, in.c.y);
-            }
-            dbk_line<4, true, true>(v, in.lv.x);
-            dbk_line<8, true, false>(v, in.lv.y);
-            dbk_line<12, true, false>(v, in.lv.z);
-            dbk_line<16, true, false>(v, in.lv.w);
-            dbk_line<2, false, true>(w, in.cv.x);
-            dbk_line<6, false, false>(w, in.cv.y);
-            *(uint4*)&tile[r * 4] =
-                make_uint4(pk4(v[4], v[5], v[6], v[7]), pk4(v[8], v[9], v[10], v[11]),
-                           pk4(v[12], v[13], v[14], v[15]), pk4(v[16], v[17], v[18], v[19]));
-            *(uint2*)&tile[64 + pl * 16 + rr * 2] = make_uint2(pk4(w[2], w[3], w[4], w[5]), pk4(w[6], w[7], w[8], w[9]));
-            if (cx > 0) {
-                // The left neighbour's columns 12..15 (chroma 7): final but
-                // for rows 13..15 (chroma 7) outside the last row.
-                if (r <= 12 || lastr)
-                    *(gptr<uint32_t>)(Y + ((uint32_t)(16 * cy + r) * py + 16 * cx - 4)) = pk4(v[0], v[1], v[2], v[3]);
-                if (rr <= 6 || lastr) Cp[(size_t)(8 * cy + rr) * pc + 8 * cx - 1] = (uint8_t)w[1];
-                if (!lastr) {
-                    uint8_t* e = (uint8_t*)(ring + (cex == 0 ? kDbkRingN - 1 : cex - 1) * kDbkRing);
-                    if (r >= 12)
-                        for (int k = 0; k < 4; ++k) e[(12 + k) * 4 + r - 12] = (uint8_t)v[k];
-                    if (rr >= 6) e[64 + (pl * 8 + 7) * 2 + rr - 6] = (uint8_t)w[1];
+                unpk4<2>(w, in.s.x), unpk4<6>(w, in.s.y);
+                dbk_line<2, false, true>(w, in.pv.x);
+                dbk_line<6, false, false>(w, in.pv.y);
+                *(uint2*)&tile[64 + pl * 16 + rr * 2] = make_uint2(pk4(w[2], w[3], w[4], w[5]), pk4(w[6], w[7], w[8], w[9]));
+                if (cx > 0) {
+                    if (rr <= 6 || lastr) Cp[(size_t)(8 * cy + rr) * pc + 8 * cx - 1] = (uint8_t)w[1];
+                    if (!lastr && rr >= 6) le[64 + (pl * 8 + 7) * 2 + rr - 6] = (uint8_t)w[1];
                 }
             }
-            // The slot's lanes are one wave's: LDS order within the wave.
+            // A slot's lanes, and the rows one step apart, are one wave's:
+            // LDS order within the wave.
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-            // Horizontal edges: luma column r, chroma column rr; rows above
-            // from the ring of the row above, or the exchange in a band's
-            // first row (unused in row 0: no top edge).
-            int h[20], g[10];
-            uint32_t ul, uc;
+            // Horizontal edges: luma column r or chroma column rr; rows
+            // above from the ring of the row above, or the exchange in a
+            // band's first row (unused in row 0: no top edge).
+            uint32_t u = 0;
             if (slot > 0) {
                 const uint32_t* ue = up + cex * kDbkRing;
-                ul = ue[r];
-                uc = ((const uint16_t*)(ue + 16))[pl * 8 + rr];
+                u = lu ? ue[r] : ((const uint16_t*)(ue + 16))[pl * 8 + rr];
             } else if (first) {
-                const uint32_t tag = (uint32_t)cx + 1;
-                ul = (uint32_t)xch_wait(imp_l + (size_t)cx * kDbkRing, in.il, tag);
-                uc = (uint32_t)xch_wait(imp_c + (size_t)cx * kDbkRing, in.ic, tag);
-                uc = (uc >> (16 * (r & 1))) & 0xFFFF;
-            } else {
-                ul = uc = 0;
+                u = (uint32_t)xch_wait(imp + (size_t)cx * kDbkRing, in.ix, (uint32_t)cx + 1);
+                if (!lu) u = (u >> (16 * (r & 1))) & 0xFFFF;
             }
-            unpk4<0>(h, ul);
-            for (int k = 0; k < 16; ++k) h[4 + k] = tb[k * 16 + r];
-            g[0] = uc & 0xFF, g[1] = uc >> 8;
-            for (int k = 0; k < 8; ++k) g[2 + k] = tb[256 + pl * 64 + k * 8 + rr];
-            dbk_line<4, true, true>(h, in.lh.x);
-            dbk_line<8, true, false>(h, in.lh.y);
-            dbk_line<12, true, false>(h, in.lh.z);
-            dbk_line<16, true, false>(h, in.lh.w);
-            dbk_line<2, false, true>(g, in.ch.x);
-            dbk_line<6, false, false>(g, in.ch.y);
-            const uint32_t col = (uint32_t)(16 * cy) * py + 16 * cx + r;
-            if (cy > 0) {  // rows 13..15 of the macroblock above are final
-                Y[col - 3 * py] = (uint8_t)h[1];
-                Y[col - 2 * py] = (uint8_t)h[2];
-                Y[col - py] = (uint8_t)h[3];
-            }
-            if (r <= 11 || lastc) {
-                for (int k = 0; k < 13; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
-                if (lastr) {
-                    for (int k = 13; k < 16; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
-                } else {
-                    ring[cex * kDbkRing + r] = pk4(h[16], h[17], h[18], h[19]);
+            if (lu) {
+                int h[20];
+                unpk4<0>(h, u);
+                for (int k = 0; k < 16; ++k) h[4 + k] = tb[k * 16 + r];
+                dbk_line<4, true, true>(h, in.ph.x);
+                dbk_line<8, true, false>(h, in.ph.y);
+                dbk_line<12, true, false>(h, in.ph.z);
+                dbk_line<16, true, false>(h, in.ph.w);
+                const uint32_t col = (uint32_t)(16 * cy) * py + 16 * cx + r;
+                if (cy > 0) {  // rows 13..15 of the macroblock above are final
+                    Y[col - 3 * py] = (uint8_t)h[1];
+                    Y[col - 2 * py] = (uint8_t)h[2];
+                    Y[col - py] = (uint8_t)h[3];
                 }
-            }
-            gptr<uint8_t> ccol = Cp + (size_t)(8 * cy) * pc + 8 * cx + rr;
-            if (cy > 0) ccol[-(long)pc] = (uint8_t)g[1];
-            if (rr <= 6 || lastc) {
-                for (int k = 0; k < 7; ++k) ccol[k * pc] = (uint8_t)g[2 + k];
-                if (lastr) ccol[7 * pc] = (uint8_t)g[9];
-                else ((uint16_t*)(ring + cex * kDbkRing + 16))[pl * 8 + rr] = (uint16_t)(g[8] | g[9] << 8);
-            }
-            if (!lastc) {  // the columns the next macroblock's left edge reads
-                if (r >= 12)
+                if (r <= 11 || lastc) {
+                    for (int k = 0; k < 13; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
+                    if (lastr) {
+                        for (int k = 13; k < 16; ++k) Y[col + k * py] = (uint8_t)h[4 + k];
+                    } else {
+                        ring[cex * kDbkRing + r] = pk4(h[16], h[17], h[18], h[19]);
+                    }
+                }
+                if (!lastc && r >= 12)  // the columns the next macroblock's left edge reads
                     for (int k = 0; k < 16; ++k) tb[k * 16 + r] = (uint8_t)h[4 + k];
-                if (rr >= 6)
+            } else {
+                int g[10];
+                g[0] = u & 0xFF, g[1] = u >> 8;
+                for (int k = 0; k < 8; ++k) g[2 + k] = tb[256 + pl * 64 + k * 8 + rr];
+                dbk_line<2, false, true>(g, in.ph.x);
+                dbk_line<6, false, false>(g, in.ph.y);
+                gptr<uint8_t> ccol = Cp + (size_t)(8 * cy) * pc + 8 * cx + rr;
+                if (cy > 0) ccol[-(long)pc] = (uint8_t)g[1];
+                if (rr <= 6 || lastc) {
+                    for (int k = 0; k < 7; ++k) ccol[k * pc] = (uint8_t)g[2 + k];
+                    if (lastr) ccol[7 * pc] = (uint8_t)g[9];
+                    else ((uint16_t*)(ring + cex * kDbkRing + 16))[pl * 8 + rr] = (uint16_t)(g[8] | g[9] << 8);
+                }
+                if (!lastc && rr >= 6)
                     for (int k = 0; k < 8; ++k) tb[256 + pl * 64 + k * 8 + rr] = (uint8_t)g[2 + k];
             }
         }
@@ -1262,7 +1272,7 @@ hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
         case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
         case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(256), 0, st, b); break;
+        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(512), 0, st, b); break;
         default: return hipSuccess;
     }
     return hipGetLastError();
