@@ -130,10 +130,33 @@ __device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, cons
     return clampi(((a * w0 + b * w1 + (1 << log_wd)) >> (log_wd + 1)) + ((o0 + o1 + 1) >> 1), 0, 255);
 }
 
+// The window of qpel_scalar for stage 0: the luma window as bytes (stride
+// LS; w points at column 0, which sits `sh` bytes into its row) and the
+// six-tap sums b1 of its rows at columns 0..w-1 (stride BS), computed once
+// per partition.
+template <int LS, int BS> struct WinB {
+    const uint8_t* w;
+    const int16_t* hb;
+    int mx = 255;  // sample maximum
+    __device__ int G(int x, int y) const { return w[(y + 2) * LS + x + 2]; }
+    __device__ int b1(int x, int yy) const { return hb[yy * BS + x]; }
+    __device__ int h1(int xx, int y) const {
+        const uint8_t* c = w + y * LS + xx;
+        return tap6(c[0], c[LS], c[2 * LS], c[3 * LS], c[4 * LS], c[5 * LS]);
+    }
+    __device__ int b(int x, int y) const { return clampi((b1(x, y + 2) + 16) >> 5, 0, mx); }
+    __device__ int h(int x, int y) const { return clampi((h1(x + 2, y) + 16) >> 5, 0, mx); }
+    __device__ int j(int x, int y) const {
+        int j1 = tap6(b1(x, y), b1(x, y + 1), b1(x, y + 2), b1(x, y + 3), b1(x, y + 4), b1(x, y + 5));
+        return clampi((j1 + 512) >> 10, 0, mx);
+    }
+};
+
 // One partition on NT lanes: 64 for partitions up to 16 x 16, 16 for
-// those up to 8 x 8 (four per wave). Its LDS holds the luma windows of
-// both lists (up to (w + 5) x (h + 5), stride LS), then the chroma windows
-// of both lists and planes (up to (w/2 + 1) x (h/2 + 1), stride CS).
+// those up to 8 x 8 (four per wave). Its LDS (bytes) holds the luma windows
+// of both lists (up to (w + 5) x (h + 5), stride LS), the chroma windows of
+// both lists and planes (up to (w/2 + 1) x (h/2 + 1), stride CS), then the
+// six-tap row sums of both luma windows (int16, (h + 5) x w, stride MW).
 template <int NT> struct PartShape {
     static constexpr int MW = NT == 64 ? 16 : 8;     // largest width / height
     static constexpr int LS = NT == 64 ? 32 : 16;    // luma window stride
@@ -141,17 +164,41 @@ template <int NT> struct PartShape {
     static constexpr int CS = NT == 64 ? 16 : 8;     // chroma window stride
     static constexpr int CR = MW / 2 + 1;            // chroma window rows
     static constexpr int LWIN = LR * LS, CWIN = CR * CS;
-    static constexpr int LDS = 2 * LWIN + 4 * CWIN;  // int16 per partition
+    static constexpr int BWIN = LR * MW;             // int16 per list
+    static constexpr int BOFF = 2 * LWIN + 4 * CWIN;  // bytes before the row sums
+    static constexpr int LDS = BOFF + 4 * BWIN;       // bytes per partition
     static constexpr int LPASS = (LR * LS + NT - 1) / NT, CPASS = (CR * CS + NT - 1) / NT;
     static constexpr int ND = MW / 4 + 2;  // aligned dwords covering a luma window row
     static constexpr int DPASS = (LR * ND + NT - 1) / NT;
+    static constexpr int QN = MW / 4;      // four-column groups of a row-sum row
+    static constexpr int BPASS = (LR * QN + NT - 1) / NT;
+    static_assert(BOFF % 8 == 0 && LDS % 8 == 0 && 4 * ND <= LS, "LDS layout");
 };
 constexpr int kStage0Lds = PartShape<64>::LDS > 4 * PartShape<16>::LDS ? PartShape<64>::LDS : 4 * PartShape<16>::LDS;
 
+// Six-tap sums 1, -5, 20, 20, -5, 1 of four consecutive window columns
+// c..c+3 with v_dot4_i32_i8: the samples x are biased to x - 128 (top bit
+// flipped) to make them signed, and the taps sum to 32, so each sum starts
+// at 128 * 32. d holds the window bytes from column c - sh on (sh 0..3) in
+// byte order; the taps of bytes 6 and 7 of each 8-byte group are 0.
+__device__ __forceinline__ void tap6x4(const uint32_t (&d)[4], uint32_t sh, int (&o)[4]) {
+    constexpr uint32_t t0 = 0x1414FB01u, t1 = 0x000001FBu;
+    const uint32_t x0 = d[0] ^ 0x80808080u, x1 = d[1] ^ 0x80808080u, x2 = d[2] ^ 0x80808080u,
+                   x3 = d[3] ^ 0x80808080u;
+    const uint32_t e0 = __builtin_amdgcn_alignbyte(x1, x0, sh), e1 = __builtin_amdgcn_alignbyte(x2, x1, sh),
+                   e2 = __builtin_amdgcn_alignbyte(x3, x2, sh);
+    o[0] = __builtin_amdgcn_sdot4((int)e1, (int)t1, __builtin_amdgcn_sdot4((int)e0, (int)t0, 4096, false), false);
+#pragma unroll
+    for (int j = 1; j < 4; ++j)
+        o[j] = __builtin_amdgcn_sdot4((int)__builtin_amdgcn_alignbyte(e2, e1, j), (int)t1,
+                                      __builtin_amdgcn_sdot4((int)__builtin_amdgcn_alignbyte(e1, e0, j), (int)t0, 4096, false),
+                                      false);
+}
+
 // `valid` false: a lane group without a partition, which still meets the
-// workgroup barrier.
+// workgroup barriers.
 template <int NT>
-__device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, int16_t* win, bool valid) {
+__device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, uint8_t* lds, bool valid) {
     using S = PartShape<NT>;
     const rv_h264_part pt = P.parts[valid ? idx : 0];
     const int x = pt.x, y = pt.y, w = pt.w, h = pt.h, dir = valid ? pt.dir : 0;
@@ -164,8 +211,9 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
     // workgroup waits for memory once. Windows clamp to the picture (the
     // replicated border of src/h264/frame.rs and the gather of
     // src/h264/inter.rs:127-139 read the same samples). A luma window whose
-    // columns lie inside the plane is read as aligned dwords of clamped
-    // rows; one crossing the left or right edge sample by sample.
+    // columns lie inside the plane is read and stored as aligned dwords of
+    // clamped rows, column 0 at byte sh = x0 & 3 of its row; one crossing
+    // the left or right edge sample by sample, column 0 at byte 0.
     const int ww = w + 5, hh = h + 5, cw = w >> 1, ch = h >> 1;
     int lv[2][S::LPASS], cv[2][2][S::CPASS];
     int shl[2] = {0, 0}, ndl[2] = {0, 0};
@@ -177,8 +225,8 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
         const uint8_t* ref = (const uint8_t*)P.ref[l][pt.ref_idx[l]][0];
         const int x0 = x + (mvx >> 2) - 2, y0 = y + (mvy >> 2) - 2;
         const int sh = x0 & 3, nd = (sh + ww + 3) >> 2;
-        shl[l] = sh, ndl[l] = nd;
         ins[l] = x0 >= 0 && x0 - sh + 4 * nd <= P.pw[0];
+        shl[l] = ins[l] ? sh : 0, ndl[l] = nd;
         if (ins[l]) {
 #pragma unroll
             for (int k = 0; k < S::DPASS; ++k) {
@@ -211,22 +259,18 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
+        uint8_t* lw = lds + l * S::LWIN;
         if (ins[l]) {
 #pragma unroll
             for (int k = 0; k < S::DPASS; ++k) {
                 const int e = tid + NT * k, r = e / S::ND, d = e - r * S::ND;
-                if (d < ndl[l] && r < hh)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int c = 4 * d - shl[l] + j;
-                        if (c >= 0 && c < ww) win[l * S::LWIN + r * S::LS + c] = (int16_t)(((uint32_t)lv[l][k] >> (8 * j)) & 0xFF);
-                    }
+                if (d < ndl[l] && r < hh) *(uint32_t*)(lw + r * S::LS + 4 * d) = (uint32_t)lv[l][k];
             }
         } else {
 #pragma unroll
             for (int k = 0; k < S::LPASS; ++k) {
                 const int e = tid + NT * k, r = e / S::LS, c = e % S::LS;
-                if (c < ww && r < hh) win[l * S::LWIN + r * S::LS + c] = (int16_t)lv[l][k];
+                if (c < ww && r < hh) lw[r * S::LS + c] = (uint8_t)lv[l][k];
             }
         }
 #pragma unroll
@@ -234,8 +278,33 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
 #pragma unroll
             for (int k = 0; k < S::CPASS; ++k) {
                 const int e = tid + NT * k, r = e / S::CS, c = e % S::CS;
-                if (c <= cw && r <= ch) win[2 * S::LWIN + (l * 2 + c2) * S::CWIN + r * S::CS + c] = (int16_t)cv[l][c2][k];
+                if (c <= cw && r <= ch) lds[2 * S::LWIN + (l * 2 + c2) * S::CWIN + r * S::CS + c] = (uint8_t)cv[l][c2][k];
             }
+    }
+    __syncthreads();
+    // Row sums b1 of every window row at columns 0..w-1, for a list whose
+    // horizontal phase is fractional (the positions that use b or j).
+    int16_t* const hb = (int16_t*)(lds + S::BOFF);
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        if (!((dir >> l) & 1) || !(pt.mv[l].x & 3)) continue;
+        const uint32_t* lw = (const uint32_t*)(lds + l * S::LWIN);
+#pragma unroll
+        for (int k = 0; k < S::BPASS; ++k) {
+            const int e = tid + NT * k, r = e / S::QN, qd = e - r * S::QN;
+            if (r < hh && 4 * qd < w) {
+                // Window bytes 4 qd .. 4 qd + 15 of row r; the last dword may
+                // lie in the next row (or the chroma windows after the last
+                // row), under taps that are 0.
+                const uint32_t* src = lw + r * (S::LS / 4) + qd;
+                const uint32_t d[4] = {src[0], src[1], src[2], src[3]};
+                int o[4];
+                tap6x4(d, (uint32_t)shl[l], o);
+                *(uint2*)(hb + l * S::BWIN + r * S::MW + 4 * qd) =
+                    make_uint2(__builtin_amdgcn_perm((uint32_t)o[1], (uint32_t)o[0], 0x05040100u),
+                               __builtin_amdgcn_perm((uint32_t)o[3], (uint32_t)o[2], 0x05040100u));
+            }
+        }
     }
     __syncthreads();
     if (!valid) return;
@@ -244,7 +313,7 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
         const int pos = (pt.mv[l].y & 3) * 4 + (pt.mv[l].x & 3);
-        const WinS<S::LS> q{win + l * S::LWIN};
+        const WinB<S::LS, S::MW> q{lds + l * S::LWIN + shl[l], hb + l * S::BWIN};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int o = tid + NT * k, oy = o / S::MW, ox = o % S::MW;
@@ -262,12 +331,12 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
     if (ox < cw && oy < ch) {
         for (int c = 1; c < 3; ++c) {
             int cp[2] = {0, 0};
-        #pragma unroll
-    for (int l = 0; l < 2; ++l) {
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
                 if (!((dir >> l) & 1)) continue;
                 const int xf = pt.mv[l].x & 7, yf = pt.mv[l].y & 7;
                 const int a = (8 - xf) * (8 - yf), bb = xf * (8 - yf), c2 = (8 - xf) * yf, d = xf * yf;
-                const int16_t* r0 = win + 2 * S::LWIN + (l * 2 + c - 1) * S::CWIN + oy * S::CS + ox;
+                const uint8_t* r0 = lds + 2 * S::LWIN + (l * 2 + c - 1) * S::CWIN + oy * S::CS + ox;
                 cp[l] = (a * r0[0] + bb * r0[1] + c2 * r0[S::CS] + d * r0[S::CS + 1] + 32) >> 6;
             }
             uint8_t* dc = (uint8_t*)P.cur[c];
@@ -277,7 +346,7 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, i
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h264_stage0(BatchDev b) {
-    __shared__ int16_t win[kStage0Lds];
+    __shared__ __attribute__((aligned(16))) uint8_t win[kStage0Lds];
     const int32_t* pre = b.prefix + 0 * (b.n + 1);
     const int it = blockIdx.x;
     const int p = find_pic(pre, b.n, it);
