@@ -130,6 +130,18 @@ __device__ __forceinline__ int combine(const H264PicDev& P, int c, int dir, cons
     return clampi(((a * w0 + b * w1 + (1 << log_wd)) >> (log_wd + 1)) + ((o0 + o1 + 1) >> 1), 0, 255);
 }
 
+// qpel_at as terms: every position is one term, or the rounded mean of two,
+// among G (full sample at column offset bit 4, row offset bit 5), b
+// (horizontal half sample, row offset bit 6), h (vertical half sample,
+// column offset bit 7) and j (centre half sample). One byte per position.
+constexpr uint32_t kQG = 1, kQB = 2, kQH = 4, kQJ = 8;
+__device__ __forceinline__ uint32_t qpel_terms(int pos) {
+    // Positions 0..7: G, G+b, b, G(x+1)+b, G+h, b+h, b+j, b+h(x+1);
+    // 8..15: h, h+j, j, j+h(x+1), G(y+1)+h, h+b(y+1), j+b(y+1), h(x+1)+b(y+1).
+    constexpr uint64_t lo = 0x860A060513020301ull, hi = 0xC64A46258C080C04ull;
+    return (uint32_t)(((pos & 8) ? hi : lo) >> (8 * (pos & 7))) & 0xFFu;
+}
+
 // The window of qpel_scalar for stage 0: the luma window as bytes (stride
 // LS; w points at column 0, which sits `sh` bytes into its row) and the
 // six-tap sums b1 of its rows at columns 0..w-1 (stride BS), computed once
@@ -314,11 +326,45 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, u
         if (!((dir >> l) & 1)) continue;
         const int pos = (pt.mv[l].y & 3) * 4 + (pt.mv[l].x & 3);
         const WinB<S::LS, S::MW> q{lds + l * S::LWIN + shl[l], hb + l * S::BWIN};
+        // Each term once for the lane's four samples; a term no lane of the
+        // wave uses is skipped. Lanes outside the partition compute on
+        // in-window samples and store nothing.
+        const uint32_t t = qpel_terms(pos);
+        int s[4] = {0, 0, 0, 0};
+        if (t & kQG) {
+            const int dx = (t >> 4) & 1, dy = (t >> 5) & 1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int o = tid + NT * k, oy = o / S::MW, ox = o % S::MW;
-            if (ox < w && oy < h) pred[l][k] = qpel_at(q, pos, ox, oy);
+            for (int k = 0; k < 4; ++k) {
+                const int o = tid + NT * k;
+                s[k] += q.G(o % S::MW + dx, o / S::MW + dy);
+            }
         }
+        if (t & kQB) {
+            const int dy = (t >> 6) & 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int o = tid + NT * k;
+                s[k] += q.b(o % S::MW, o / S::MW + dy);
+            }
+        }
+        if (t & kQH) {
+            const int dx = (t >> 7) & 1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int o = tid + NT * k;
+                s[k] += q.h(o % S::MW + dx, o / S::MW);
+            }
+        }
+        if (t & kQJ) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int o = tid + NT * k;
+                s[k] += q.j(o % S::MW, o / S::MW);
+            }
+        }
+        const bool two = __builtin_popcount(t & 15u) == 2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pred[l][k] = two ? (s[k] + 1) >> 1 : s[k];
     }
     uint8_t* dst = (uint8_t*)P.cur[0];
 #pragma unroll
