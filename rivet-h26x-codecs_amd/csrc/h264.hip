@@ -399,11 +399,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h2
     const H264PicDev& P = *b.h264[p];
     int i = it - pre[p];
     const int tid = threadIdx.x;
-    if (i < P.n_mbs) {
+    if (i < P.n_intra) {
         // Intra macroblock: host-supplied prediction; no partition.
-        const rv_h264_mb m = P.mbs[i];
-        if (!m.intra) return;
-        int mx = (i % P.mbw) * 16, my = (i / P.mbw) * 16;
+        const int a = (int)P.part_order[P.n_parts + i];
+        const rv_h264_mb m = P.mbs[a];
+        const int mx = (a % P.mbw) * 16, my = (a / P.mbw) * 16;
         const uint16_t* pr = P.intra_pred + m.pred_off;
         for (int e = tid; e < 256; e += 64) {
             int yy = e >> 4, xx = e & 15;
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h2
         if (tid < 16) P.partmap[(size_t)((my >> 2) + (tid >> 2)) * P.w4 + (mx >> 2) + (tid & 3)] = 0xFFFFFFFFu;
         return;
     }
-    i -= P.n_mbs;
+    i -= P.n_intra;
     // Partitions above 8 x 8 one per workgroup, then the others four per
     // workgroup, 16 lanes each (the host orders them so).
     if (i < P.n_big) {
@@ -1370,7 +1370,7 @@ hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st) {
 }
 
 void h264_stage_items(H264PicDev& P) {
-    P.items[0] = P.n_mbs + P.n_big + (P.n_parts - P.n_big + 3) / 4;
+    P.items[0] = P.n_intra + P.n_big + (P.n_parts - P.n_big + 3) / 4;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
     P.items[2] = (P.n_mbs * 8 + 63) / 64;
     P.items[3] = P.deblock ? h264_deblock_bands(P.mbh) : 0;  // one workgroup per band

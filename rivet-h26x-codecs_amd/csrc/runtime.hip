@@ -968,8 +968,10 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->codec = RV_CODEC_H264;
         j->out_slot = out_slot;
         h264_bytes(p, j->bytes);
+        int n_intra = 0;
+        for (int a = 0; a < p->n_mbs; ++a) n_intra += p->mbs[a].intra ? 1 : 0;
         size_t sz = carve_size<rv::H264PicDev>(1) + carve_size<rv_h264_mb>(p->n_mbs) +
-                    carve_size<rv_h264_part>(p->n_parts) + carve_size<uint32_t>(p->n_parts) +
+                    carve_size<rv_h264_part>(p->n_parts) + carve_size<uint32_t>(p->n_parts + n_intra) +
                     carve_size<rv_h264_blk>(blks.size()) +
                     carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -989,7 +991,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         rv::H264PicDev* hdesc = carve<rv::H264PicDev>(hp, 1);
         rv_h264_mb* h_mbs = carve<rv_h264_mb>(hp, p->n_mbs);
         rv_h264_part* h_parts = carve<rv_h264_part>(hp, p->n_parts);
-        uint32_t* h_order = carve<uint32_t>(hp, p->n_parts);
+        uint32_t* h_order = carve<uint32_t>(hp, p->n_parts + n_intra);
         rv_h264_blk* h_blks = carve<rv_h264_blk>(hp, blks.size());
         int32_t* h_coefs = carve<int32_t>(hp, p->n_coefs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
@@ -1004,6 +1006,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
             if (p->parts[i].w > 8 || p->parts[i].h > 8) h_order[n_big++] = (uint32_t)i;
         for (int i = 0, k = n_big; i < p->n_parts; ++i)
             if (!(p->parts[i].w > 8 || p->parts[i].h > 8)) h_order[k++] = (uint32_t)i;
+        // Then the intra macroblocks, one workgroup each.
+        for (int a = 0, k = p->n_parts; a < p->n_mbs; ++a)
+            if (p->mbs[a].intra) h_order[k++] = (uint32_t)a;
         if (!blks.empty()) std::memcpy(h_blks, blks.data(), blks.size() * sizeof(rv_h264_blk));
         if (p->n_coefs) std::memcpy(h_coefs, p->coefs, p->n_coefs * sizeof(int32_t));
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
@@ -1054,6 +1059,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.parts = (const rv_h264_part*)dev((char*)h_parts);
         d.part_order = (const uint32_t*)dev((char*)h_order);
         d.n_big = n_big;
+        d.n_intra = n_intra;
         d.blks = (const rv_h264_blk*)dev((char*)h_blks);
         d.coefs = (const int32_t*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);

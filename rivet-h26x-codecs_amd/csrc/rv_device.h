@@ -108,8 +108,10 @@ struct H264PicDev {
     int32_t n_mbs, n_parts, n_blks4, n_blks8;
     const rv_h264_mb* mbs;
     const rv_h264_part* parts;
-    const uint32_t* part_order;  // partition indices: above 8 x 8 first (n_big), then the rest
-    int32_t n_big;
+    // Partition indices, above 8 x 8 first (n_big), then the rest; then
+    // the indices of the n_intra intra macroblocks.
+    const uint32_t* part_order;
+    int32_t n_big, n_intra;
     const rv_h264_blk* blks;  // 4x4 first, then 8x8
     const int32_t* coefs;
     const uint16_t* intra_pred;
