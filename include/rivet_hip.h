@@ -280,6 +280,24 @@ int rv_dpb_upload(rv_session* s, int slot, const void* y, const void* cb,
                   const void* cr);
 int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr);
 
+/* Decoded picture hash (H.265 D.3.20) of DPB slot `slot`, computed on the
+ * device without a plane download. Replaces the hashing half of
+ * hash::verify (src/hevc/hash.rs:109-158): per component, the plane's
+ * samples in raster order as bytes, low byte first above 8 bits, then
+ * `type` (the SEI's hash_type):
+ *   RV_HASH_MD5       md5 (:191-250): out = 3 x 16 bytes (Y, Cb, Cr);
+ *   RV_HASH_CRC       crc16 (:160-174): out = 3 x uint16;
+ *   RV_HASH_CHECKSUM  checksum (:176-189): out = 3 x uint32.
+ * Comparing with the SEI's values is the caller's (PartialEq on
+ * PictureHash, :12-21). Waits like rv_plane_download for the slot's
+ * writer. MD5 is serial within a component: about 150 ms for a 4K
+ * 8-bit luma plane, for verification rather than the decode loop; CRC and
+ * checksum take microseconds. RV_E_INVALID for an unknown type. */
+#define RV_HASH_MD5 0
+#define RV_HASH_CRC 1
+#define RV_HASH_CHECKSUM 2
+int rv_picture_hash(rv_session* s, int slot, int type, void* out);
+
 /* A prepared picture: the work lists copied into device memory (they are
  * not referenced after the call returns). The job writes its picture into
  * DPB slot out_slot of its session when launched; it may be launched many

@@ -74,6 +74,9 @@ _sig = {
     "orc_h264_residual8": (None, [u8p, C.c_int, i32p, C.c_int]),
     "orc_h264_implicit_pair": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p]),
     "orc_h264_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+    "orc_crc16": (C.c_uint16, [P, C.c_size_t]),
+    "orc_checksum": (C.c_uint32, [P, C.c_size_t, C.c_int, C.c_int]),
+    "orc_md5": (None, [P, C.c_size_t, u8p]),
 }
 for _n, (_r, _a) in _sig.items():
     _f = getattr(lib, _n)
@@ -84,6 +87,42 @@ for _n, (_r, _a) in _sig.items():
 def ptr(a: np.ndarray, t=None):
     a = np.ascontiguousarray(a)
     return a.ctypes.data_as(t) if t is not None else C.c_void_p(a.ctypes.data)
+
+
+def component_message(plane: np.ndarray) -> bytes:
+    """One component's hash message (src/hevc/hash.rs:131-142): its samples
+    in raster order as bytes, low byte first for uint16 planes."""
+    a = np.ascontiguousarray(plane)
+    return a.astype("<u2").tobytes() if a.dtype == np.uint16 else a.astype(np.uint8).tobytes()
+
+
+def crc16(msg: bytes) -> int:
+    return int(lib.orc_crc16(msg, len(msg)))
+
+
+def checksum(msg: bytes, width: int, wide: bool) -> int:
+    return int(lib.orc_checksum(msg, len(msg), width, int(wide)))
+
+
+def md5(msg: bytes) -> bytes:
+    out = (C.c_uint8 * 16)()
+    lib.orc_md5(msg, len(msg), out)
+    return bytes(out)
+
+
+def picture_hash(planes, kind: int) -> list:
+    """The three component hashes of hash::verify (src/hevc/hash.rs:109-158)
+    for kind 0 (MD5), 1 (CRC) or 2 (checksum)."""
+    out = []
+    for p in planes:
+        m = component_message(p)
+        if kind == 0:
+            out.append(md5(m))
+        elif kind == 1:
+            out.append(crc16(m))
+        else:
+            out.append(checksum(m, p.shape[1], p.dtype == np.uint16))
+    return out
 
 
 def reconstruct(codec: int, pic, slots: list, stop_after: int = 0):

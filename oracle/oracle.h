@@ -92,6 +92,14 @@ void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long
 int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots,
                          void* const out[3], int stop_after);
 
+/* ---- Decoded picture hash (src/hevc/hash.rs) -------------------------
+ * Over one component's message (n bytes): crc16 (:160-174), checksum
+ * (:176-189; width in samples, wide = two bytes per sample) and md5
+ * (:191-250). */
+uint16_t orc_crc16(const uint8_t* data, size_t n);
+uint32_t orc_checksum(const uint8_t* data, size_t n, int width, int wide);
+void orc_md5(const uint8_t* data, size_t n, uint8_t out[16]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["hevc.hip", "h264.hip", "runtime.hip", "entry.hip"]
+HIP_SOURCES = ["hevc.hip", "h264.hip", "runtime.hip", "entry.hip", "hash.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 

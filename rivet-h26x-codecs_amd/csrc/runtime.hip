@@ -653,6 +653,51 @@ int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr) {
     return RV_OK;
 }
 
+int rv_picture_hash(rv_session* s, int slot, int type, void* out) {
+    try {
+        if (!s || !out) return fail(RV_E_INVALID, "null argument");
+        if (type < RV_HASH_MD5 || type > RV_HASH_CHECKSUM) return fail(RV_E_INVALID, "unknown hash type");
+        if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        rv::HashPlanes hp;
+        for (int c = 0; c < 3; ++c) {
+            hp.p[c] = (const uint8_t*)s->planes[(size_t)slot * 3 + c];
+            hp.w[c] = s->g.pw[c];
+            hp.h[c] = s->g.ph[c];
+            hp.pitch[c] = s->g.pitch[c] * s->g.ss;
+        }
+        hp.ss = s->g.ss;
+        DevBuf buf;
+        if (!pool_get(rv::hash_scratch_bytes(type, hp), &buf)) return fail(RV_E_NOMEM, "hash scratch allocation failed");
+        hipEvent_t ev = nullptr;
+        int rc = RV_OK;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "event create");
+        } else {
+            // On the kernel stream, after the launch that wrote the slot.
+            std::lock_guard<std::mutex> lk(g_mu);
+            e = rv::hash_launch(type, hp, buf.p, g_stream);
+            if (e == hipSuccess) e = hipEventRecord(ev, g_stream);
+            if (e != hipSuccess) rc = hip_fail(e, "picture hash launch");
+        }
+        if (rc == RV_OK) {
+            const size_t n = type == RV_HASH_MD5 ? 48 : type == RV_HASH_CRC ? 6 : 12;
+            e = hipEventSynchronize(ev);
+            if (e == hipSuccess) e = hipMemcpy(out, buf.p, n, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "picture hash result");
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            pool_put(buf);
+        }
+        if (ev) (void)hipEventDestroy(ev);
+        return rc;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_picture_hash");
+    }
+}
+
 int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** out) {
     try {
         if (!s || !p || !out) return fail(RV_E_INVALID, "null argument");

@@ -25,6 +25,18 @@ hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_
 size_t h264_deblock_xch_words(int mbw, int mbh);
 hipError_t h264_entry_launch(const H264Entry& e, int blocks, hipStream_t st);
 
+// Picture hash (hash.hip): the three planes of one picture, each `w` x `h`
+// samples of `ss` bytes at a row pitch of `pitch` bytes.
+struct HashPlanes {
+    const uint8_t* p[3];
+    int w[3], h[3], pitch[3];
+    int ss;
+};
+// Device scratch for hash_launch (including the 48-byte result at its
+// start: 3 x 16 bytes, 3 x uint16 or 3 x uint32 by type).
+size_t hash_scratch_bytes(int type, const HashPlanes& H);
+hipError_t hash_launch(int type, const HashPlanes& H, void* scratch, hipStream_t st);
+
 // Plain copy of `bytes` (a multiple of 4096) with 16-byte vectors.
 hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStream_t st);
 

@@ -13,8 +13,8 @@ from typing import Sequence
 import numpy as np
 
 from . import _abi
-from ._abi import (RV_CODEC_H264, RV_CODEC_HEVC, RV_LAUNCH_BATCHED, RV_LAUNCH_GRAPH,  # noqa: F401
-                   RV_LAUNCH_PLAIN)
+from ._abi import (RV_CODEC_H264, RV_CODEC_HEVC, RV_HASH_CHECKSUM, RV_HASH_CRC, RV_HASH_MD5,  # noqa: F401
+                   RV_LAUNCH_BATCHED, RV_LAUNCH_GRAPH, RV_LAUNCH_PLAIN)
 
 lib = _abi.load()
 
@@ -141,6 +141,20 @@ class Session:
         ps = [np.empty(s, dtype=dt) for s in plane_shapes(self.width, self.height)]
         check(lib.rv_plane_download(self._h, slot, *(p.ctypes.data for p in ps)))
         return ps
+
+    def picture_hash(self, slot: int, kind: int) -> list:
+        """Decoded picture hash of DPB slot `slot`, computed on the device
+        (rv_picture_hash; hash::verify, src/hevc/hash.rs:109-158): per
+        component (Y, Cb, Cr), a 16-byte MD5 digest (RV_HASH_MD5), the CRC-16
+        (RV_HASH_CRC) or the checksum (RV_HASH_CHECKSUM)."""
+        buf = (C.c_uint8 * 48)()
+        check(lib.rv_picture_hash(self._h, slot, kind, buf))
+        raw = bytes(buf)
+        if kind == RV_HASH_MD5:
+            return [raw[16 * c:16 * c + 16] for c in range(3)]
+        if kind == RV_HASH_CRC:
+            return [int(v) for v in np.frombuffer(raw[:6], dtype="<u2")]
+        return [int(v) for v in np.frombuffer(raw[:12], dtype="<u4")]
 
     def prepare(self, pic: GenPicture | C._Pointer, out_slot: int) -> "Job":
         return Job(self, pic, out_slot)

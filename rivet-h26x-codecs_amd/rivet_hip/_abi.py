@@ -29,6 +29,9 @@ RV_H264_IMPLICIT = 2
 RV_LAUNCH_PLAIN = 0
 RV_LAUNCH_BATCHED = 1
 RV_LAUNCH_GRAPH = 2
+RV_HASH_MD5 = 0
+RV_HASH_CRC = 1
+RV_HASH_CHECKSUM = 2
 
 i8, u8, i16, u16, i32, u32, i64 = C.c_int8, C.c_uint8, C.c_int16, C.c_uint16, C.c_int32, C.c_uint32, C.c_int64
 R = RV_MAX_REFS
@@ -131,6 +134,7 @@ PROTOTYPES = {
     "rv_session_footprint": (i64, [i32, i32, i32, i32, i32]),
     "rv_dpb_upload": (i32, [P, i32, P, P, P]),
     "rv_plane_download": (i32, [P, i32, P, P, P]),
+    "rv_picture_hash": (i32, [P, i32, i32, P]),
     "rv_hevc_prepare": (i32, [P, C.POINTER(HevcPic), i32, C.POINTER(P)]),
     "rv_h264_prepare": (i32, [P, C.POINTER(H264Pic), i32, C.POINTER(P)]),
     "rv_job_free": (i32, [P]),

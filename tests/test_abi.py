@@ -117,3 +117,11 @@ def test_kernel_variant_tables():
     assert rh.variant_names(rh.RV_CODEC_H264) == [["any"]] * 5
     assert rh.lib.rv_variant_count(0, 5) == 0 and rh.lib.rv_variant_count(7, 0) == 0
     assert rh.lib.rv_variant_name(0, 0, 9) == b""
+
+
+def test_picture_hash_refuses_bad_arguments_without_a_device():
+    from rivet_hip import lib
+
+    buf = (C.c_uint8 * 48)()
+    assert lib.rv_picture_hash(None, 0, 0, buf) == -1  # null session
+    assert lib.rv_picture_hash(None, 0, 9, buf) == -1
