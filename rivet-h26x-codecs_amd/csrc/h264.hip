@@ -320,47 +320,42 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, u
     }
     __syncthreads();
     if (!valid) return;
+    // Lane tid takes samples o = tid + NT k (k < kn) of the partition in
+    // raster order over its own width: w h / NT of them (exact for every
+    // partition shape a group serves).
+    const int lw = 31 - __builtin_clz((unsigned)w), kn = (w * h) / NT;
     int pred[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (!((dir >> l) & 1)) continue;
         const int pos = (pt.mv[l].y & 3) * 4 + (pt.mv[l].x & 3);
         const WinB<S::LS, S::MW> q{lds + l * S::LWIN + shl[l], hb + l * S::BWIN};
-        // Each term once for the lane's four samples; a term no lane of the
-        // wave uses is skipped. Lanes outside the partition compute on
-        // in-window samples and store nothing.
+        // Each term once for the lane's samples; a term no lane of the wave
+        // uses is skipped.
         const uint32_t t = qpel_terms(pos);
         int s[4] = {0, 0, 0, 0};
         if (t & kQG) {
             const int dx = (t >> 4) & 1, dy = (t >> 5) & 1;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int o = tid + NT * k;
-                s[k] += q.G(o % S::MW + dx, o / S::MW + dy);
-            }
+            for (int k = 0; k < 4; ++k)
+                if (k < kn) s[k] += q.G(((tid + NT * k) & (w - 1)) + dx, ((tid + NT * k) >> lw) + dy);
         }
         if (t & kQB) {
             const int dy = (t >> 6) & 1;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int o = tid + NT * k;
-                s[k] += q.b(o % S::MW, o / S::MW + dy);
-            }
+            for (int k = 0; k < 4; ++k)
+                if (k < kn) s[k] += q.b((tid + NT * k) & (w - 1), ((tid + NT * k) >> lw) + dy);
         }
         if (t & kQH) {
             const int dx = (t >> 7) & 1;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int o = tid + NT * k;
-                s[k] += q.h(o % S::MW + dx, o / S::MW);
-            }
+            for (int k = 0; k < 4; ++k)
+                if (k < kn) s[k] += q.h(((tid + NT * k) & (w - 1)) + dx, (tid + NT * k) >> lw);
         }
         if (t & kQJ) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int o = tid + NT * k;
-                s[k] += q.j(o % S::MW, o / S::MW);
-            }
+            for (int k = 0; k < 4; ++k)
+                if (k < kn) s[k] += q.j((tid + NT * k) & (w - 1), (tid + NT * k) >> lw);
         }
         const bool two = __builtin_popcount(t & 15u) == 2;
 #pragma unroll
@@ -369,8 +364,8 @@ __device__ __forceinline__ void mc_part(const H264PicDev& P, int idx, int tid, u
     uint8_t* dst = (uint8_t*)P.cur[0];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int o = tid + NT * k, oy = o / S::MW, ox = o % S::MW;
-        if (ox < w && oy < h) dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (uint8_t)combine(P, 0, dir, pt.ref_idx, pred[0][k], pred[1][k]);
+        const int o = tid + NT * k, oy = o >> lw, ox = o & (w - 1);
+        if (k < kn) dst[(size_t)(y + oy) * P.pitch[0] + x + ox] = (uint8_t)combine(P, 0, dir, pt.ref_idx, pred[0][k], pred[1][k]);
     }
     // Chroma: eighth-sample bilinear (src/h264/inter.rs:243-282), 4:2:0.
     const int ox = tid % (S::MW / 2), oy = tid / (S::MW / 2);

@@ -1051,6 +1051,14 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
             if (p->parts[i].w > 8 || p->parts[i].h > 8) h_order[n_big++] = (uint32_t)i;
         for (int i = 0, k = n_big; i < p->n_parts; ++i)
             if (!(p->parts[i].w > 8 || p->parts[i].h > 8)) h_order[k++] = (uint32_t)i;
+        // Each lane takes w h / (lanes per partition) samples, so the
+        // partitions that share a workgroup are grouped by area: largest
+        // first in each class, picture order within an area.
+        auto by_area = [&](uint32_t a, uint32_t b) {
+            return p->parts[a].w * p->parts[a].h > p->parts[b].w * p->parts[b].h;
+        };
+        std::stable_sort(h_order, h_order + n_big, by_area);
+        std::stable_sort(h_order + n_big, h_order + p->n_parts, by_area);
         // Then the intra macroblocks, one workgroup each.
         for (int a = 0, k = p->n_parts; a < p->n_mbs; ++a)
             if (p->mbs[a].intra) h_order[k++] = (uint32_t)a;
