@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
     const uint32_t py = D.py;
     const size_t pc = D.pc;
     const int y = y0 + slot;
-    const int steps = dbk_lag(rows - 1) + mbw + (y0 + rows < mbh ? 2 : 0);
+    const int steps = dbk_lag(rows - 1) + mbw + (y0 + rows < mbh ? 1 : 0);
 
     int x = -dbk_lag(slot), ex = 0;  // the slot's macroblock at the next step, x % 3
     bool act = slot < rows && x == 0;
@@ -1155,13 +1155,6 @@ __global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         const int cx = x, cy = y, cex = ex;
         const bool cact = act;
-        if (last) {  // publish ring entry x - 2, complete since the last step
-            const int xp = cx - 2;
-            if (xp >= 0 && xp < mbw) {
-                const uint32_t* e = ring + (cex == 0 ? 1 : cex == 1 ? 2 : 0) * kDbkRing;
-                xch_store(pub + (size_t)xp * kDbkRing, ((uint64_t)(xp + 1) << 32) | e[pw]);
-            }
-        }
         ++x;
         act = slot < rows && x >= 0 && x < mbw;
         ex = x <= 0 ? 0 : (ex + 1 == kDbkRingN ? 0 : ex + 1);  // x % 3
@@ -1169,8 +1162,8 @@ __global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
             dbk_fetch(D, lu, x, y, r, nx);
             if (first) nx.ix = xch_load(imp + (size_t)x * kDbkRing);
         }
+        const bool lastc = cx == mbw - 1, lastr = cy == mbh - 1;
         if (cact) {
-            const bool lastc = cx == mbw - 1, lastr = cy == mbh - 1;
             uint8_t* const le = (uint8_t*)(ring + (cex == 0 ? kDbkRingN - 1 : cex - 1) * kDbkRing);  // entry x - 1
             // Vertical edges: luma row r or chroma row rr. Then the left
             // neighbour's columns 12..15 (chroma 7) are final but for rows
@@ -1206,11 +1199,24 @@ __global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
                     if (!lastr && rr >= 6) le[64 + (pl * 8 + 7) * 2 + rr - 6] = (uint8_t)w[1];
                 }
             }
-            // A slot's lanes, and the rows one step apart, are one wave's:
-            // LDS order within the wave.
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        }
+        // A slot's lanes, and the rows one step apart, are one wave's: LDS
+        // order within the wave.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        // The band's last row publishes ring entry x - 1: its columns
+        // 12..15 are final since this step's vertical edges, the others
+        // since the last step's horizontal ones (all of them at the row's
+        // end, which takes one more step).
+        if (last) {
+            const int xp = cx - 1;
+            if (xp >= 0 && xp < mbw) {
+                const uint32_t* e = ring + (cex == 0 ? kDbkRingN - 1 : cex - 1) * kDbkRing;
+                xch_store(pub + (size_t)xp * kDbkRing, ((uint64_t)(xp + 1) << 32) | e[pw]);
+            }
+        }
+        if (cact) {
             // Horizontal edges: luma column r or chroma column rr; rows
             // above from the ring of the row above, or the exchange in a
             // band's first row (unused in row 0: no top edge).
