@@ -881,11 +881,12 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
     }
 }
 
-// Stage 3 cuts each picture into bands of 16 macroblock rows, one
-// workgroup of 512 threads per band: 16 slots, slot s filtering row
-// 16 * band + s with 16 luma lanes (waves 0..3) and 16 chroma lanes (waves
-// 4..7); luma and chroma filtering are independent, so each lane's chain of
-// edges per step is one plane's. Each row takes one macroblock per step, with one
+// Stage 3 cuts each picture into bands of kH264DbkRows (16) macroblock
+// rows, one workgroup per band: a slot per row, slot s filtering row
+// kH264DbkRows * band + s with 16 luma lanes (the first half of the
+// workgroup's waves) and 16 chroma lanes (the second half); luma and
+// chroma filtering are independent, so each lane's chain of edges per
+// step is one plane's. Each row takes one macroblock per step, with one
 // barrier per step, and each step filters the vertical edges before the
 // horizontal ones. MB (x, y) needs MB (x - 1, y) complete and, for its top
 // edge only, MB (x + 1, y - 1)'s vertical edges (the left edge of that MB
@@ -927,7 +928,8 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 // column of the same index across the horizontal edges, the samples of a
 // line in registers.
 
-constexpr int kDbkSlots = 16;  // rows per band
+constexpr int kDbkSlots = kH264DbkRows;  // rows per band
+constexpr int kDbkThreads = 32 * kDbkSlots;  // 16 luma and 16 chroma lanes per row
 constexpr int kDbkTile = 96;   // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
 constexpr int kDbkRing = 24;   // words per entry: 16 luma columns, 16 chroma (16-bit)
 constexpr int kDbkRingN = 3;   // entries per slot
@@ -936,7 +938,7 @@ constexpr int kDbkPer = kDbkTile + kDbkRing * kDbkRingN;
 // running and publishes within a few steps); a poll that reaches it goes on
 // with what it read rather than hang the workgroup.
 constexpr int kDbkSpin = 1 << 18;
-static_assert(kDbkSlots * 16 == 256 && 4 * 16 == 64, "four slots of 16 lanes per 64-lane wave");
+static_assert(kDbkSlots % 4 == 0 && kDbkThreads <= 1024, "four slots of 16 lanes per 64-lane wave");
 
 // The step at which slot s reaches macroblock 0: one step after slot s - 1
 // in the same wave, two across a wave boundary.
@@ -1099,7 +1101,7 @@ __device__ __forceinline__ uint64_t xch_wait(gptr<const uint64_t> p, uint64_t v,
     return v;
 }
 
-__global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
+__global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
     __shared__ uint32_t dl[kDbkSlots * kDbkPer];
     __shared__ int band_s;
     const int32_t* pre = b.prefix + 3 * (b.n + 1);
@@ -1112,8 +1114,8 @@ __global__ void __launch_bounds__(512) h264_stage3(BatchDev b) {
     __syncthreads();
     const int band = band_s;
     if (band >= h264_deblock_bands(P.mbh)) return;  // only with a stale ticket
-    const bool lu = threadIdx.x < 256;  // waves 0..3 luma, 4..7 chroma
-    const int t = (int)threadIdx.x & 255;
+    const bool lu = (int)threadIdx.x < kDbkThreads / 2;  // luma waves first, then chroma
+    const int t = (int)threadIdx.x & (kDbkThreads / 2 - 1);
     const int slot = t >> 4, r = t & 15;
     const int pl = r >> 3, rr = r & 7;  // chroma lanes: plane, row / column
     const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
@@ -1388,7 +1390,7 @@ hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
         case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
         case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
-        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(512), 0, st, b); break;
+        case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(kDbkThreads), 0, st, b); break;
         default: return hipSuccess;
     }
     return hipGetLastError();

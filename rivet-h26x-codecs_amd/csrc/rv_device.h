@@ -89,7 +89,10 @@ struct HevcPicDev {
 constexpr int kH264DbkWords = 64;
 
 // Macroblock rows per deblocking band (one workgroup each).
-__host__ __device__ inline int h264_deblock_bands(int mbh) { return (mbh + 15) / 16; }
+// Macroblock rows per band of the H.264 deblocking wavefront (one
+// workgroup of 32 threads per row).
+constexpr int kH264DbkRows = 16;
+__host__ __device__ inline int h264_deblock_bands(int mbh) { return (mbh + kH264DbkRows - 1) / kH264DbkRows; }
 
 // Per-picture descriptor of an H.264 job.
 struct H264PicDev {
