@@ -307,16 +307,16 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             waiting.append(arrivals.popleft())
         while waiting and (used + waiting[0].footprint <= budget or not active):
             admit(waiting.popleft(), now())
+        # Finished groups free their place in flight now; their batches and
+        # jobs are released after the next launch, so that host work
+        # overlaps the GPU work queued behind it.
+        fin = []
         while inflight and mark_done(inflight[0][0]):
             _, pics, bs, g = inflight.popleft()
             groups[g] -= 1
             if groups[g] == 0:
                 del groups[g]
-            tc = now()
-            for b in bs:
-                b.close()
-            for p in pics:
-                complete(p, tc)
+            fin.append((now(), pics, bs))
         while uploaded:
             p = uploaded.popleft()
             r = p.fut.result()
@@ -328,6 +328,7 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             p.job = r
             if eligible(p):
                 push(p, now())
+        launched = False
         if launchable and len(groups) < inflight_max:
             group = [heapq.heappop(launchable)[2] for _ in range(min(max_batch, len(launchable)))]
             g = next(gid)
@@ -359,6 +360,13 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
                 for q in p.dependents:
                     if eligible(q):
                         push(q, tl)
+            launched = True
+        for tc, pics, bs in fin:
+            for b in bs:
+                b.close()
+            for p in pics:
+                complete(p, tc)
+        if launched or fin:
             continue
         if len(groups) >= inflight_max:
             mark_wait(inflight[0][0])
