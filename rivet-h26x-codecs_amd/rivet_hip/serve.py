@@ -73,6 +73,13 @@ class Pic:
     out: list | None = None
     fut: object = None
     dependents: list = field(default_factory=list)  # pictures referencing this one
+    # Earlier pictures (decode order) of the session that write this
+    # picture's slot, or read it as a reference: this picture may overwrite
+    # the slot only after they have launched (completed, for writers, when
+    # outputs are recorded).
+    prior_writers: list = field(default_factory=list)
+    prior_readers: list = field(default_factory=list)
+    slot_successors: list = field(default_factory=list)  # pictures waiting on this one's use of a slot
 
 
 @dataclass
@@ -139,9 +146,17 @@ def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop
             p.luma = cf.visible_luma(w, h)
             s.pics.append(p)
         by_poc = {q.poc: q for q in s.pics}
-        for q in s.pics:
+        for i, q in enumerate(s.pics):
             for r in q.refs:
                 by_poc[r].dependents.append(q)
+            for e in s.pics[:i]:
+                if e.slot == q.slot:
+                    q.prior_writers.append(e)
+                elif q.slot in e.ref_slots:
+                    q.prior_readers.append(e)
+                else:
+                    continue
+                e.slot_successors.append(q)
         sessions.append(s)
 
     def gen(p: Pic):
@@ -182,7 +197,8 @@ def run(sessions, mode: str = "batched", budget: int = 32 << 30, max_batch: int 
         record: bool = False, upload_threads: int = 8, preload: bool = False):
     """Serve every session; returns throughput and latency statistics.
     record=True downloads every reconstructed picture as it completes (into
-    Pic.out; use inflight_max=1 so no later picture has overwritten it).
+    Pic.out); a later picture that writes the same DPB slot launches only
+    after that download.
     Work-list uploads run on `upload_threads` host threads from admission
     on, overlapping the reconstruction of earlier pictures. preload=True
     opens every session and prepares every work list before the clock
@@ -201,12 +217,17 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
     threads) and launched references feed a heap of launchable pictures.
     Every launch goes to the library's one in-order stream, so a picture
     may launch as soon as its references have launched: the stream runs it
-    after them, and the next batch is queued while the last one runs. At
-    most `inflight_max` launch groups (one loop iteration's launches) are
-    outstanding in either mode. A picture's latency runs from when it was
-    ready (references complete, work list uploaded) to its completion.
-    With record=True a picture launches only after its references
-    complete, so the download at its completion sees its own output."""
+    after them, and the next batch is queued while the last one runs. It
+    also waits until every earlier picture of its session that uses its
+    DPB slot (writes it, or reads it as a reference) has launched in an
+    earlier group, so no slot is overwritten before it is read, and two
+    writers of a slot never share a launch. At most `inflight_max` launch
+    groups (one loop iteration's launches) are outstanding in either mode.
+    A picture's latency runs from when it was ready (references complete,
+    work list uploaded) to its completion. With record=True a picture
+    launches only after its references and the earlier writers of its slot
+    have completed, and finished groups are downloaded before the next
+    launch, so each download sees its picture's own output."""
     t0 = time.perf_counter()
     arrivals = collections.deque(sorted(sessions, key=lambda s: s.arrival))
     waiting = collections.deque()
@@ -242,10 +263,20 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         return cb
 
     def eligible(q: Pic) -> bool:
+        """References launched (completed when recording: the download at
+        their completion must see their own output), and the slot free:
+        every earlier picture that writes q's slot, or reads it as a
+        reference, launched in an earlier group, so the stream runs it
+        first. When recording, earlier writers must also have completed,
+        since their download happens at completion."""
         if q.job is None or q.launched or q.queued:
             return False
         refs = [_pic_by_poc(q.sess, r) for r in q.refs]
-        return all(r.done or (r.launched and not record) for r in refs)
+        if not all(r.done or (r.launched and not record) for r in refs):
+            return False
+        if not all(e.done or (e.launched and not record) for e in q.prior_writers):
+            return False
+        return all(e.done or e.launched for e in q.prior_readers)
 
     def push(q: Pic, t: float):
         q.queued = True
@@ -272,11 +303,21 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         for q in p.dependents:
             if math.isnan(q.ready_t) and all(_pic_by_poc(s, r).done for r in q.refs):
                 make_ready(q, t)
+        for q in p.slot_successors:
+            if eligible(q):
+                push(q, t)
         if s.remaining == 0 and id(s) in active:
             s.session.close()
             s.session = None
             used -= s.footprint
             del active[id(s)]
+
+    def release(fin):
+        for tc, pics, bs in fin:
+            for b in bs:
+                b.close()
+            for p in pics:
+                complete(p, tc)
 
     def admit(s: Sess, t: float):
         """Open the session and hand its uploads to the loader threads: the
@@ -317,6 +358,11 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             if groups[g] == 0:
                 del groups[g]
             fin.append((now(), pics, bs))
+        if record:
+            # A recorded picture is downloaded before any later picture can
+            # be launched into its slot.
+            release(fin)
+            fin = []
         while uploaded:
             p = uploaded.popleft()
             r = p.fut.result()
@@ -354,18 +400,15 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
                     groups[g] += 1
                     batches += 1
                     batch_sizes.append(1)
-            # Their dependents may launch next: the stream orders them.
+            # Their dependents, and pictures waiting for their slots, may
+            # launch next: the stream orders them.
             tl = now()
             for p in group:
-                for q in p.dependents:
+                for q in itertools.chain(p.dependents, p.slot_successors):
                     if eligible(q):
                         push(q, tl)
             launched = True
-        for tc, pics, bs in fin:
-            for b in bs:
-                b.close()
-            for p in pics:
-                complete(p, tc)
+        release(fin)
         if launched or fin:
             continue
         if len(groups) >= inflight_max:

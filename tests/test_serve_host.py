@@ -2,10 +2,15 @@
 batches and completion markers that check its ordering rules — a picture
 is launched only after its references were launched (in an earlier launch
 group: the library's one stream runs them first), or completed when the
-loop records outputs, and after its session's I picture is resident;
+loop records outputs, and after its session's I picture is resident; each
+reference slot still holds the reference when a picture launches (no later
+anchor or B picture overwrote it first), no two pictures of a launch write
+one slot, and a recorded download returns the picture's own output;
 sessions are admitted within the HBM budget and closed once their last
 picture completes, and every picture completes."""
+import random
 import threading
+import time
 
 import pytest
 
@@ -13,8 +18,11 @@ from rivet_hip import serve
 
 
 class World:
-    def __init__(self, polls_to_done=2, record=False):
+    def __init__(self, polls_to_done=2, record=False, prepare_delay_ms=0.0, seed=0, slow_b=False):
         self.record = record
+        self.prepare_delay_ms = prepare_delay_ms
+        self.slow_b = slow_b  # every B picture's upload takes prepare_delay_ms, anchors none
+        self.rng = random.Random(seed)
         self.lock = threading.Lock()
         self.polls_to_done = polls_to_done
         self.marks = {}      # id -> polls left
@@ -30,7 +38,9 @@ class World:
 
 
 def install(monkeypatch, world, sessions):
+    by_planes = {}
     for s in sessions:
+        by_planes[id(s.iplanes)] = s.pics[0]
         for p in s.pics:
             if p.gen is not None:
                 world.by_gen[id(p.gen)] = p
@@ -51,20 +61,28 @@ def install(monkeypatch, world, sessions):
 
         def __init__(self, codec, w, h, bd, slots):
             self.resident = False
+            self.writer = {}  # slot -> the picture whose output it holds (in stream order)
             with world.lock:
                 world.open[id(self)] = 0
 
         def upload(self, slot, planes):
             world.check(slot == serve.ANCHOR_SLOTS[0], "I picture uploaded to another slot")
+            self.writer[slot] = by_planes[id(planes)]
             self.resident = True
 
         def prepare(self, gen, slot):
             p = world.by_gen[id(gen)]
             world.check(slot == p.slot, "work list prepared for another slot")
+            if world.slow_b:
+                time.sleep(world.prepare_delay_ms / 1e3 if p.kind == "B" else 0.0)
+            elif world.prepare_delay_ms:
+                with world.lock:
+                    d = world.rng.uniform(0.0, world.prepare_delay_ms)
+                time.sleep(d / 1e3)
             return FakeJob(p)
 
         def download(self, slot):
-            return slot
+            return self.writer.get(slot)
 
         def close(self):
             with world.lock:
@@ -76,15 +94,28 @@ def install(monkeypatch, world, sessions):
             self.pics = [j.pic for j in jobs]
 
         def launch(self, mode):
+            # Every picture of a launch reads its references (stage 0) before
+            # any writes its slot (the last stage), so reads are checked
+            # against the slots as they were before this launch.
+            written = set()
             for p in self.pics:
-                world.check(p.sess.session.resident, f"session {p.sess.idx} launched before its I picture")
-                for r in p.refs:
+                sess = p.sess.session
+                world.check(sess.resident, f"session {p.sess.idx} launched before its I picture")
+                for r, sl in zip(p.refs, p.ref_slots):
                     ref = next(q for q in p.sess.pics if q.poc == r)
                     if world.record:
                         world.check(ref.done, f"poc {p.poc} launched before reference {r} completed")
                     else:
                         world.check(ref.done or ref.launched, f"poc {p.poc} launched before reference {r}")
+                    w = sess.writer.get(sl)
+                    world.check(w is ref, f"session {p.sess.idx} poc {p.poc} read slot {sl} holding poc "
+                                          f"{None if w is None else w.poc} instead of reference {r}")
                 world.check(not p.launched, f"poc {p.poc} launched twice")
+                key = (id(sess), p.slot)
+                world.check(key not in written, f"two pictures of one launch write slot {p.slot}")
+                written.add(key)
+            for p in self.pics:
+                p.sess.session.writer[p.slot] = p
             world.launched.extend(self.pics)
 
         def close(self):
@@ -140,7 +171,24 @@ def test_recording_waits_for_completed_references(monkeypatch, sessions, mode):
     st = serve.run(sessions, mode, inflight_max=2, record=True, upload_threads=3)
     assert not w.errors, w.errors[:5]
     assert st["pictures"] == sum(len(s.pics) - 1 for s in sessions)
-    assert all(p.out == p.slot for s in sessions for p in s.pics if p.kind != "I")
+    assert all(p.out is p for s in sessions for p in s.pics if p.kind != "I")
+
+
+@pytest.mark.parametrize("record", [False, True])
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_slots_are_not_overwritten_before_they_are_read(monkeypatch, sessions, mode, record, seed):
+    # Work lists finish uploading out of order: every B picture 30 ms late
+    # (seed 0), or each 0-8 ms at random. Later anchors are then ready
+    # before the B pictures that read the slot they overwrite, and the
+    # loop must hold them back.
+    w = World(record=record, prepare_delay_ms=30.0 if seed == 0 else 8.0, seed=seed, slow_b=seed == 0)
+    install(monkeypatch, w, sessions)
+    st = serve.run(sessions, mode, inflight_max=2, record=record, upload_threads=4)
+    assert not w.errors, w.errors[:5]
+    assert st["pictures"] == sum(len(s.pics) - 1 for s in sessions)
+    if record:
+        assert all(p.out is p for s in sessions for p in s.pics if p.kind != "I")
 
 
 def test_references_launch_ahead_of_completion(monkeypatch, sessions):
