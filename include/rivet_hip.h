@@ -298,6 +298,20 @@ int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr);
 #define RV_HASH_CHECKSUM 2
 int rv_picture_hash(rv_session* s, int slot, int type, void* out);
 
+/* rv_picture_hash that allocates nothing. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture elsewhere in the process, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, results and ordering after the
+ * slot's writer; the hash kernels' working memory is a fixed device array
+ * of the library (about 51 KiB per device, held until the process ends)
+ * instead of a buffer from the library's pool, and the result returns on
+ * the kernel stream instead of through an event. Pictures of more than
+ * RV_HASH_NOALLOC_MAX_ROWS rows (8K UHD has 4320) are refused with
+ * RV_E_RANGE. Calls serialise on a lock of their own, since they share
+ * the array; the array keeps the last call's contents. */
+#define RV_HASH_NOALLOC_MAX_ROWS 4352
+int rv_picture_hash_noalloc(rv_session* s, int slot, int type, void* out);
+
 /* A prepared picture: the work lists copied into device memory (they are
  * not referenced after the call returns). The job writes its picture into
  * DPB slot out_slot of its session when launched; it may be launched many

@@ -49,6 +49,20 @@ bool ready() { return g_device >= 0 && g_stream; }
 
 }  // namespace
 
+// Device memory of rv_picture_hash_noalloc: the hash kernels' results
+// (256 bytes) and one CRC partial per row and component, for pictures of at
+// most RV_HASH_NOALLOC_MAX_ROWS rows. Each device holds its own copy from
+// module load to process end. Host code never passes this name to a
+// kernel (on the host it is a stand-in, not a device address): rv_init
+// looks up the device address once, and the entry point uses that.
+static __device__ __align__(256) uint8_t scratch[256 + 3 * RV_HASH_NOALLOC_MAX_ROWS * sizeof(uint32_t)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device ordinal
+std::mutex g_noalloc_mu;           // one allocation-free hash at a time uses the array
+}  // namespace
+
 namespace rv {
 std::mutex& lib_mutex() { return g_mu; }
 hipStream_t lib_stream() { return g_stream; }
@@ -520,6 +534,11 @@ int rv_init(int device_ordinal) {
         RV_HIP(hipStreamCreateWithFlags(&g_io, hipStreamNonBlocking));
         RV_HIP(hipStreamCreateWithFlags(&g_up, hipStreamNonBlocking));
         for (int i = 0; i < kMarks; ++i) RV_HIP(hipEventCreate(&g_marks[i]));
+        if (device_ordinal < kMaxDevices) {
+            void* ptr = nullptr;
+            RV_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratch_dev[device_ordinal] = ptr;
+        }
         const char* e = std::getenv("RIVET_HIP_NO_DEBLOCK");
         if (e && *e && *e != '0') g_stage_mask |= 1;
         e = std::getenv("RIVET_HIP_NO_SAO");
@@ -695,6 +714,42 @@ int rv_picture_hash(rv_session* s, int slot, int type, void* out) {
         return rc;
     } catch (...) {
         return fail(RV_E_NOMEM, "exception in rv_picture_hash");
+    }
+}
+
+int rv_picture_hash_noalloc(rv_session* s, int slot, int type, void* out) {
+    try {
+        if (!s || !out) return fail(RV_E_INVALID, "null argument");
+        if (type < RV_HASH_MD5 || type > RV_HASH_CHECKSUM) return fail(RV_E_INVALID, "unknown hash type");
+        if (slot < 0 || slot >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        rv::HashPlanes hp;
+        for (int c = 0; c < 3; ++c) {
+            hp.p[c] = (const uint8_t*)s->planes[(size_t)slot * 3 + c];
+            hp.w[c] = s->g.pw[c];
+            hp.h[c] = s->g.ph[c];
+            hp.pitch[c] = s->g.pitch[c] * s->g.ss;
+        }
+        hp.ss = s->g.ss;
+        if (rv::hash_scratch_bytes(type, hp) > sizeof(scratch))
+            return fail(RV_E_RANGE, "picture taller than RV_HASH_NOALLOC_MAX_ROWS rows");
+        void* buf = g_device < kMaxDevices ? g_scratch_dev[g_device] : nullptr;
+        if (!buf) return fail(RV_E_STATE, "no device address for the hash array");
+        const size_t n = type == RV_HASH_MD5 ? 48 : type == RV_HASH_CRC ? 6 : 12;
+        std::lock_guard<std::mutex> hk(g_noalloc_mu);
+        {
+            // After the launch that wrote the slot; the result comes back on
+            // the same stream, so no event is needed.
+            std::lock_guard<std::mutex> lk(g_mu);
+            hipError_t e = rv::hash_launch(type, hp, buf, g_stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(out, buf, n, hipMemcpyDeviceToHost, g_stream);
+            if (e != hipSuccess) return hip_fail(e, "picture hash launch");
+        }
+        hipError_t e = hipStreamSynchronize(g_stream);
+        if (e != hipSuccess) return hip_fail(e, "picture hash result");
+        return RV_OK;
+    } catch (...) {
+        return fail(RV_E_INVALID, "exception in rv_picture_hash_noalloc");
     }
 }
 
