@@ -59,6 +59,8 @@ _sig = {
     "orc_hevc_deblock_luma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, i32p, u8p, u8p, C.c_int]),
     "orc_hevc_deblock_chroma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, u8p, u8p, C.c_int]),
     "orc_hevc_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+    "orc_hevc_bs_map": (C.c_int, [P, P, P]),
+    "orc_h264_bs_map": (C.c_int, [P, P]),
     "orc_h264_qpel": (None, [C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_h264_chroma": (None, [u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_h264_copy": (None, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
@@ -123,6 +125,30 @@ def picture_hash(planes, kind: int) -> list:
         else:
             out.append(checksum(m, p.shape[1], p.dtype == np.uint16))
     return out
+
+
+def hevc_bs_map(pic):
+    """(ver, hor) boundary-strength maps of an HEVC picture description, one
+    entry per 4x4 block (orc_hevc_bs_map)."""
+    c = pic.contents
+    ver = np.zeros((c.height // 4, c.width // 4), np.uint8)
+    hor = np.zeros_like(ver)
+    rc = lib.orc_hevc_bs_map(C.cast(pic, C.c_void_p), ver.ctypes.data, hor.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {rc}")
+    return ver, hor
+
+
+def h264_bs_map(pic):
+    """Per macroblock (mby, mbx): bS[2][4][4] = [vertical | horizontal][edge
+    0..3][segment 0..3] (orc_h264_bs_map)."""
+    c = pic.contents
+    raw = np.zeros(c.n_mbs * 8, np.uint32)
+    rc = lib.orc_h264_bs_map(C.cast(pic, C.c_void_p), raw.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed: {rc}")
+    b = raw.view(np.uint8).reshape(c.height // 16, c.width // 16, 2, 4, 4)
+    return b
 
 
 def reconstruct(codec: int, pic, slots: list, stop_after: int = 0):

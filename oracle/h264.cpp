@@ -454,6 +454,153 @@ void filter_edge(bool chroma_k, bool vertical, uint8_t* data, long off, long str
     }
 }
 
+// Per-4x4 motion per list (BlockMotion, src/h264/frame.rs:42-74), raster
+// per macroblock (index mb * 16 + by * 4 + bx), and the partition of each
+// 4x4 block (raster over the picture), from the partition list.
+struct MbSide {
+    int mbw = 0, mbh = 0, w4 = 0;
+    std::vector<BlockMotion> motion[2];
+    std::vector<int> part_of;
+};
+
+int build_side(const rv_h264_pic* pic, MbSide& sd) {
+    int W = pic->width, H = pic->height;
+    sd.mbw = W / 16;
+    sd.mbh = H / 16;
+    sd.w4 = W / 4;
+    BlockMotion none{{0, 0}, -1, -1};
+    for (auto& m : sd.motion) m.assign((size_t)sd.mbw * sd.mbh * 16, none);
+    sd.part_of.assign((size_t)sd.w4 * (H / 4), -1);
+    for (int i = 0; i < pic->n_parts; ++i) {
+        const rv_h264_part& pt = pic->parts[i];
+        int x = pt.x, y = pt.y, w = pt.w, h = pt.h;
+        if (!(pt.dir & 3)) return RV_E_RANGE;
+        for (int l = 0; l < 2; ++l) {
+            if (!((pt.dir >> l) & 1)) continue;
+            int ri = pt.ref_idx[l];
+            if (ri < 0 || ri >= RV_MAX_REFS || ri >= pic->num_ref[l]) return RV_E_RANGE;
+            for (int by = y / 4; by < (y + h) / 4; ++by)
+                for (int bx = x / 4; bx < (x + w) / 4; ++bx) {
+                    int mb = (by / 4) * sd.mbw + bx / 4, blk = (by % 4) * 4 + (bx % 4);
+                    BlockMotion& bm = sd.motion[l][(size_t)mb * 16 + blk];
+                    bm.mv[0] = pt.mv[l].x;
+                    bm.mv[1] = pt.mv[l].y;
+                    bm.ref_idx = ri;
+                    bm.ref_id = pic->ref_slot[l][ri];
+                }
+        }
+        for (int by = y / 4; by < (y + h) / 4; ++by)
+            for (int bx = x / 4; bx < (x + w) / 4; ++bx) sd.part_of[(size_t)by * sd.w4 + bx] = i;
+    }
+    return RV_OK;
+}
+
+// part_edges (MbInfo, src/h264/mb.rs:350-365) from the partitions: bit
+// e * 4 + k is set where the 4x4 blocks either side of internal edge e
+// (1..3) at line k belong to different partitions.
+void part_edges(const MbSide& sd, int mbx, int mby, uint32_t* pe_v, uint32_t* pe_h) {
+    *pe_v = *pe_h = 0;
+    const int bx = mbx * 4, by = mby * 4, w4 = sd.w4;
+    for (int e = 1; e < 4; ++e)
+        for (int k = 0; k < 4; ++k) {
+            if (sd.part_of[(size_t)(by + k) * w4 + bx + e - 1] != sd.part_of[(size_t)(by + k) * w4 + bx + e])
+                *pe_v |= 1u << (e * 4 + k);
+            if (sd.part_of[(size_t)(by + e - 1) * w4 + bx + k] != sd.part_of[(size_t)(by + e) * w4 + bx + k])
+                *pe_h |= 1u << (e * 4 + k);
+        }
+}
+
+// The packed strengths of macroblock `addr` (deblock_mb_rows' bS
+// derivation, src/h264/deblock.rs:617-756): bs_v[e] / bs_h[e] for edge e
+// (0 = the macroblock edge), one byte per 4-sample segment. Frame
+// picture, one slice.
+void mb_strengths(const rv_h264_pic* pic, const MbSide& sd, int addr, uint32_t bs_v[4], uint32_t bs_h[4]) {
+    const int mbw = sd.mbw, mbx = addr % mbw, mby = addr / mbw;
+    const rv_h264_mb& m = pic->mbs[addr];
+    const int left = mbx > 0 ? addr - 1 : addr, above = mby > 0 ? addr - mbw : addr;
+    const bool filter_left = mbx > 0, filter_top = mby > 0;
+    for (int e = 0; e < 4; ++e) bs_v[e] = bs_h[e] = 0;
+    const bool internal_odd = !m.transform_8x8;
+    uint32_t pe_v, pe_h;
+    part_edges(sd, mbx, mby, &pe_v, &pe_h);
+    if (m.intra) {
+        if (filter_left) bs_v[0] = 0x04040404;
+        if (filter_top) bs_h[0] = 0x04040404;
+        for (int e = 1; e < 4; ++e) bs_v[e] = bs_h[e] = 0x03030303;
+        return;
+    }
+    const BlockMotion* m0 = sd.motion[0].data();
+    const BlockMotion* m1 = sd.motion[1].data();
+    auto edge_motion = [&](uint32_t rest, uint32_t chg, auto bs_of) -> uint32_t {
+        if (rest == 0) return 0;
+        uint32_t packed = 0, held = 0;
+        bool known = false;
+        for (int k = 0; k < 4; ++k) {
+            if ((chg >> k) & 1) known = false;
+            if (!((rest >> k) & 1)) continue;
+            if (!known) {
+                held = (uint32_t)bs_of(k);
+                known = true;
+            }
+            packed |= held << (k * 8);
+        }
+        return packed;
+    };
+    uint32_t nz = m.nz_mask;
+    int step = internal_odd ? 1 : 2;
+    // internal_edge (src/h264/deblock.rs:255-282).
+    auto internal = [&](uint32_t coef, uint32_t part, int e, bool vertical) -> uint32_t {
+        uint32_t c = vertical ? gather_col(coef, e - 1) : (coef >> ((e - 1) * 4)) & 0xF;
+        uint32_t bs = BS2_SPREAD[c];
+        uint32_t rest = ((part >> (e * 4)) & 0xF) & ~c;
+        for (int k = 0; k < 4; ++k) {
+            if (!((rest >> k) & 1)) continue;
+            int pb = vertical ? k * 4 + e - 1 : (e - 1) * 4 + k;
+            int qb = vertical ? k * 4 + e : e * 4 + k;
+            bs |= (uint32_t)motion_bs(m0, m1, addr * 16 + pb, addr * 16 + qb) << (k * 8);
+        }
+        return bs;
+    };
+    if (nz != 0 || pe_v != 0) {
+        uint32_t coef_v = nz | (nz >> 1);
+        for (int e = step; e < 4; e += step) bs_v[e] = internal(coef_v, pe_v, e, true);
+    }
+    if (nz != 0 || pe_h != 0) {
+        uint32_t coef_h = nz | (nz >> 4);
+        for (int e = step; e < 4; e += step) bs_h[e] = internal(coef_h, pe_h, e, false);
+    }
+    if (filter_left) {
+        const rv_h264_mb& ml = pic->mbs[left];
+        if (ml.intra) {
+            bs_v[0] = 0x04040404;
+        } else {
+            uint32_t lpv, lph;
+            part_edges(sd, mbx - 1, mby, &lpv, &lph);
+            uint32_t c = gather_col(ml.nz_mask, 3) | gather_col(nz, 0);
+            uint32_t rest = ~c & 0xF;
+            uint32_t chg = gather_col(lph, 3) | gather_col(pe_h, 0);
+            bs_v[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
+                          return motion_bs(m0, m1, left * 16 + k * 4 + 3, addr * 16 + k * 4);
+                      });
+        }
+    }
+    if (filter_top) {
+        const rv_h264_mb& ma = pic->mbs[above];
+        if (ma.intra) {
+            bs_h[0] = 0x04040404;
+        } else {
+            uint32_t apv, aph;
+            part_edges(sd, mbx, mby - 1, &apv, &aph);
+            uint32_t c = ((ma.nz_mask >> 12) | nz) & 0xF;
+            uint32_t rest = ~c & 0xF;
+            uint32_t chg = gather_col(apv, 3) | gather_col(pe_v, 0);
+            bs_h[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
+                          return motion_bs(m0, m1, above * 16 + 12 + k, addr * 16 + k);
+                      });
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -501,6 +648,17 @@ void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long
     out[1] = w1;
 }
 
+int orc_h264_bs_map(const rv_h264_pic* pic, uint32_t* out) {
+    if (!pic || !out) return RV_E_INVALID;
+    int W = pic->width, H = pic->height;
+    if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || pic->n_mbs != (W / 16) * (H / 16)) return RV_E_RANGE;
+    MbSide sd;
+    int rc = build_side(pic, sd);
+    if (rc) return rc;
+    for (int a = 0; a < pic->n_mbs; ++a) mb_strengths(pic, sd, a, out + 8 * a, out + 8 * a + 4);
+    return RV_OK;
+}
+
 int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
                          int stop_after) {
     if (!pic || !slot_planes || !out) return RV_E_INVALID;
@@ -526,14 +684,9 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
     };
     Frame cur;
     cur.init(W, H);
-    int w4 = W / 4, h4 = H / 4;
-    // Per-4x4 motion per list (BlockMotion, src/h264/frame.rs:42-74), raster
-    // per macroblock: index mb * 16 + (by * 4 + bx).
-    BlockMotion none{{0, 0}, -1, -1};
-    std::vector<BlockMotion> motion[2] = {std::vector<BlockMotion>((size_t)mbw * mbh * 16, none),
-                                          std::vector<BlockMotion>((size_t)mbw * mbh * 16, none)};
-    std::vector<int> part_of((size_t)w4 * h4, -1);
-    (void)h4;
+    MbSide sd;
+    int rc = build_side(pic, sd);
+    if (rc) return rc;
     // Intra macroblocks: host-supplied prediction.
     for (int a = 0; a < pic->n_mbs; ++a) {
         const rv_h264_mb& m = pic->mbs[a];
@@ -572,20 +725,7 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
             for (int c = 1; c < 3; ++c)
                 interp(rf[l]->p[c], xci, yci, cw + 1, ch + 1, window,
                        [&](const uint8_t* s, long st) { chroma(pred[l][c].data(), s, st, cw, ch, xf, yf); });
-            // Motion for the deblocking filter.
-            for (int by = y / 4; by < (y + h) / 4; ++by)
-                for (int bx = x / 4; bx < (x + w) / 4; ++bx) {
-                    int mb = (by / 4) * mbw + bx / 4, blk = (by % 4) * 4 + (bx % 4);
-                    BlockMotion& bm = motion[l][(size_t)mb * 16 + blk];
-                    bm.mv[0] = mvx;
-                    bm.mv[1] = mvy;
-                    bm.ref_idx = ri;
-                    bm.ref_id = pic->ref_slot[l][ri];
-                }
         }
-        if (!rf[0] && !rf[1]) return RV_E_RANGE;
-        for (int by = y / 4; by < (y + h) / 4; ++by)
-            for (int bx = x / 4; bx < (x + w) / 4; ++bx) part_of[(size_t)by * w4 + bx] = i;
         // Weighting (src/h264/recon.rs:142-170).
         bool both = rf[0] && rf[1];
         int single = rf[0] ? 0 : 1;
@@ -653,105 +793,15 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
             for (int k = 0; k < 3; ++k) thr[qa].lut[k + 1] = (int16_t)TC0[ia][k];
         }
         auto get = [&](int qp_p, int qp_q) -> const Thr& { return thr[clampi((qp_p + qp_q + 1) >> 1, 0, 51)]; };
-        // part_edges (MbInfo, src/h264/mb.rs:350-365) from the partitions.
-        auto part_edges = [&](int mbx, int mby, uint32_t* pe_v, uint32_t* pe_h) {
-            *pe_v = *pe_h = 0;
-            for (int e = 1; e < 4; ++e)
-                for (int k = 0; k < 4; ++k) {
-                    int bx = mbx * 4, by = mby * 4;
-                    if (part_of[(size_t)(by + k) * w4 + bx + e - 1] != part_of[(size_t)(by + k) * w4 + bx + e])
-                        *pe_v |= 1u << (e * 4 + k);
-                    if (part_of[(size_t)(by + e - 1) * w4 + bx + k] != part_of[(size_t)(by + e) * w4 + bx + k])
-                        *pe_h |= 1u << (e * 4 + k);
-                }
-        };
-        const BlockMotion* m0 = motion[0].data();
-        const BlockMotion* m1 = motion[1].data();
-        auto edge_motion = [&](uint32_t rest, uint32_t chg, auto bs_of) -> uint32_t {
-            if (rest == 0) return 0;
-            uint32_t packed = 0, held = 0;
-            bool known = false;
-            for (int k = 0; k < 4; ++k) {
-                if ((chg >> k) & 1) known = false;
-                if (!((rest >> k) & 1)) continue;
-                if (!known) {
-                    held = (uint32_t)bs_of(k);
-                    known = true;
-                }
-                packed |= held << (k * 8);
-            }
-            return packed;
-        };
         for (int mby = 0; mby < mbh; ++mby)
             for (int mbx = 0; mbx < mbw; ++mbx) {
                 int addr = mby * mbw + mbx;
                 const rv_h264_mb& m = pic->mbs[addr];
                 int left = mbx > 0 ? addr - 1 : addr, above = mby > 0 ? addr - mbw : addr;
-                bool filter_left = mbx > 0, filter_top = mby > 0;
-                uint32_t bs_v[4] = {0, 0, 0, 0}, bs_h[4] = {0, 0, 0, 0};
                 bool internal_odd = !m.transform_8x8;
-                uint32_t pe_v, pe_h;
-                part_edges(mbx, mby, &pe_v, &pe_h);
-                if (m.intra) {
-                    if (filter_left) bs_v[0] = 0x04040404;
-                    if (filter_top) bs_h[0] = 0x04040404;
-                    for (int e = 1; e < 4; ++e) bs_v[e] = bs_h[e] = 0x03030303;
-                } else {
-                    uint32_t nz = m.nz_mask;
-                    int step = internal_odd ? 1 : 2;
-                    // internal_edge (src/h264/deblock.rs:255-282).
-                    auto internal = [&](uint32_t coef, uint32_t part, int e, bool vertical) -> uint32_t {
-                        uint32_t c = vertical ? gather_col(coef, e - 1) : (coef >> ((e - 1) * 4)) & 0xF;
-                        uint32_t bs = BS2_SPREAD[c];
-                        uint32_t rest = ((part >> (e * 4)) & 0xF) & ~c;
-                        for (int k = 0; k < 4; ++k) {
-                            if (!((rest >> k) & 1)) continue;
-                            int pb = vertical ? k * 4 + e - 1 : (e - 1) * 4 + k;
-                            int qb = vertical ? k * 4 + e : e * 4 + k;
-                            bs |= (uint32_t)motion_bs(m0, m1, addr * 16 + pb, addr * 16 + qb) << (k * 8);
-                        }
-                        return bs;
-                    };
-                    if (nz != 0 || pe_v != 0) {
-                        uint32_t coef_v = nz | (nz >> 1);
-                        for (int e = step; e < 4; e += step) bs_v[e] = internal(coef_v, pe_v, e, true);
-                    }
-                    if (nz != 0 || pe_h != 0) {
-                        uint32_t coef_h = nz | (nz >> 4);
-                        for (int e = step; e < 4; e += step) bs_h[e] = internal(coef_h, pe_h, e, false);
-                    }
-                    if (filter_left) {
-                        const rv_h264_mb& ml = pic->mbs[left];
-                        if (ml.intra) {
-                            bs_v[0] = 0x04040404;
-                        } else {
-                            uint32_t lpv, lph;
-                            part_edges(mbx - 1, mby, &lpv, &lph);
-                            uint32_t c = gather_col(ml.nz_mask, 3) | gather_col(nz, 0);
-                            uint32_t rest = ~c & 0xF;
-                            uint32_t chg = gather_col(lph, 3) | gather_col(pe_h, 0);
-                            bs_v[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
-                                          return motion_bs(m0, m1, left * 16 + k * 4 + 3, addr * 16 + k * 4);
-                                      });
-                        }
-                    }
-                    if (filter_top) {
-                        const rv_h264_mb& ma = pic->mbs[above];
-                        if (ma.intra) {
-                            bs_h[0] = 0x04040404;
-                        } else {
-                            uint32_t apv, aph;
-                            part_edges(mbx, mby - 1, &apv, &aph);
-                            uint32_t c = ((ma.nz_mask >> 12) | nz) & 0xF;
-                            uint32_t rest = ~c & 0xF;
-                            uint32_t chg = gather_col(apv, 3) | gather_col(pe_v, 0);
-                            bs_h[0] = BS2_SPREAD[c] | edge_motion(rest, chg, [&](int k) {
-                                          return motion_bs(m0, m1, above * 16 + 12 + k, addr * 16 + k);
-                                      });
-                        }
-                    }
-                    if ((bs_v[0] | bs_v[1] | bs_v[2] | bs_v[3] | bs_h[0] | bs_h[1] | bs_h[2] | bs_h[3]) == 0) continue;
-                }
+                uint32_t bs_v[4], bs_h[4];
+                mb_strengths(pic, sd, addr, bs_v, bs_h);
+                if ((bs_v[0] | bs_v[1] | bs_v[2] | bs_v[3] | bs_h[0] | bs_h[1] | bs_h[2] | bs_h[3]) == 0) continue;
                 const rv_h264_mb& ml = pic->mbs[left];
                 const rv_h264_mb& ma = pic->mbs[above];
                 // filter_luma_style (src/h264/deblock.rs:352-451).

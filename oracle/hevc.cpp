@@ -851,6 +851,75 @@ void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv
 // ---------------------------------------------------------------------
 // Whole picture
 // ---------------------------------------------------------------------
+
+// The per-4x4 side data the deblocking filter reads (PicInfo,
+// src/hevc/pic.rs:64-96), from the work lists: QP, prediction mode and
+// coding-block edges per CU (src/hevc/ctu.rs:751-800), motion and
+// prediction-block edges per PU, transform edges and cbf per luma TU
+// (src/hevc/ctu.rs:1493-1510).
+int build_info(const rv_hevc_pic* pic, PicInfo& info) {
+    int W = pic->width, H = pic->height;
+    info.log2_ctb = pic->log2_ctb;
+    info.w4 = W / 4;
+    info.h4 = H / 4;
+    int ctb = 1 << pic->log2_ctb;
+    info.wc = (W + ctb - 1) / ctb;
+    info.hc = (H + ctb - 1) / ctb;
+    size_t n4 = (size_t)info.w4 * info.h4;
+    info.pred_mode.assign(n4, 0);
+    info.cbf_luma.assign(n4, 0);
+    info.edges.assign(n4, 0);
+    info.filter_exempt.assign(n4, 0);
+    info.qp_y.assign(n4, 0);
+    MotionInfo none{};
+    none.ref_idx[0] = none.ref_idx[1] = -1;
+    info.motion.assign(n4, none);
+    for (int i = 0; i < pic->n_cus; ++i) {
+        const rv_hevc_cu& cu = pic->cus[i];
+        int s = 1 << cu.log2_size;
+        fill4(info.qp_y, info.w4, cu.x, cu.y, s, s, (int8_t)cu.qp_y);
+        for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 1;
+        for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 4;
+        if (cu.intra) {
+            fill4(info.pred_mode, info.w4, cu.x, cu.y, s, s, (uint8_t)1);
+            MotionInfo mi = none;
+            mi.flags = 4;
+            fill4(info.motion, info.w4, cu.x, cu.y, s, s, mi);
+            // PU edges of the single intra PB.
+            for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 2;
+            for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 8;
+        }
+    }
+    for (int i = 0; i < pic->n_pus; ++i) {
+        const rv_hevc_pu& pu = pic->pus[i];
+        if (!(pu.dir & 3)) return RV_E_RANGE;
+        MotionInfo mi = none;
+        for (int l = 0; l < 2; ++l) {
+            if (!((pu.dir >> l) & 1)) continue;
+            int ri = pu.ref_idx[l];
+            if (ri < 0 || ri >= RV_MAX_REFS || ri >= pic->num_ref[l]) return RV_E_RANGE;
+            mi.ref_idx[l] = ri;
+            mi.mv[l][0] = pu.mv[l].x;
+            mi.mv[l][1] = pu.mv[l].y;
+            mi.ref_delta[l] = pic->poc - pic->ref_poc[l][ri];
+            if (pic->ref_long_term[l][ri]) mi.flags |= 1 << l;
+        }
+        fill4(info.motion, info.w4, pu.x, pu.y, pu.w, pu.h, mi);
+        for (int yy = pu.y; yy < std::min(pu.y + pu.h, H); yy += 4) info.edges[info.idx4(pu.x, yy)] |= 2;
+        for (int xx = pu.x; xx < std::min(pu.x + pu.w, W); xx += 4) info.edges[info.idx4(xx, pu.y)] |= 8;
+    }
+    for (int i = 0; i < pic->n_tus; ++i) {
+        const rv_hevc_tu& tu = pic->tus[i];
+        int n = 1 << tu.log2;
+        if (tu.comp == 0) {
+            for (int yy = tu.y; yy < std::min(tu.y + n, H); yy += 4) info.edges[info.idx4(tu.x, yy)] |= 1;
+            for (int xx = tu.x; xx < std::min(tu.x + n, W); xx += 4) info.edges[info.idx4(xx, tu.y)] |= 4;
+            if (tu.flags & RV_TU_CODED) fill4(info.cbf_luma, info.w4, tu.x, tu.y, n, n, (uint8_t)1);
+        }
+    }
+    return RV_OK;
+}
+
 template <typename S>
 int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
                 int stop_after) {
@@ -874,74 +943,37 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
     Frame<S> cur;
     cur.init(W, H);
     PicInfo info;
-    info.log2_ctb = pic->log2_ctb;
-    info.w4 = W / 4;
-    info.h4 = H / 4;
-    int ctb = 1 << pic->log2_ctb;
-    info.wc = (W + ctb - 1) / ctb;
-    info.hc = (H + ctb - 1) / ctb;
-    size_t n4 = (size_t)info.w4 * info.h4;
-    info.pred_mode.assign(n4, 0);
-    info.cbf_luma.assign(n4, 0);
-    info.edges.assign(n4, 0);
-    info.filter_exempt.assign(n4, 0);
-    info.qp_y.assign(n4, 0);
-    MotionInfo none{};
-    none.ref_idx[0] = none.ref_idx[1] = -1;
-    info.motion.assign(n4, none);
+    int rc = build_info(pic, info);
+    if (rc) return rc;
     int max = (1 << bd) - 1;
 
-    // Coding units: QP, prediction mode, CU edges (src/hevc/ctu.rs:751-800);
-    // intra CUs get their host-supplied prediction.
+    // Intra CUs get their host-supplied prediction.
     for (int i = 0; i < pic->n_cus; ++i) {
         const rv_hevc_cu& cu = pic->cus[i];
+        if (!cu.intra) continue;
         int s = 1 << cu.log2_size;
-        fill4(info.qp_y, info.w4, cu.x, cu.y, s, s, (int8_t)cu.qp_y);
-        for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 1;
-        for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 4;
-        if (cu.intra) {
-            fill4(info.pred_mode, info.w4, cu.x, cu.y, s, s, (uint8_t)1);
-            MotionInfo mi = none;
-            mi.flags = 4;
-            fill4(info.motion, info.w4, cu.x, cu.y, s, s, mi);
-            // PU edges of the single intra PB.
-            for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 2;
-            for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 8;
-            const uint16_t* pr = pic->intra_pred + cu.pred_off;
-            for (int c = 0; c < 3; ++c) {
-                int n = c ? s / 2 : s, px = c ? cu.x / 2 : cu.x, py = c ? cu.y / 2 : cu.y;
-                Plane<S>& pl = cur.p[c];
-                for (int yy = 0; yy < n; ++yy)
-                    for (int xx = 0; xx < n; ++xx) pl.data[pl.offset(px + xx, py + yy)] = (S)pr[yy * n + xx];
-                pr += n * n;
-            }
+        const uint16_t* pr = pic->intra_pred + cu.pred_off;
+        for (int c = 0; c < 3; ++c) {
+            int n = c ? s / 2 : s, px = c ? cu.x / 2 : cu.x, py = c ? cu.y / 2 : cu.y;
+            Plane<S>& pl = cur.p[c];
+            for (int yy = 0; yy < n; ++yy)
+                for (int xx = 0; xx < n; ++xx) pl.data[pl.offset(px + xx, py + yy)] = (S)pr[yy * n + xx];
+            pr += n * n;
         }
     }
-    // Prediction units: motion side data, PU edges, and the prediction.
+    // Prediction units.
     Mc<S> mc;
     for (int i = 0; i < pic->n_pus; ++i) {
         const rv_hevc_pu& pu = pic->pus[i];
-        MotionInfo mi = none;
         const Frame<S>* rf[2] = {nullptr, nullptr};
         int mv[2][2] = {{0, 0}, {0, 0}};
         for (int l = 0; l < 2; ++l) {
             if (!((pu.dir >> l) & 1)) continue;
-            int ri = pu.ref_idx[l];
-            if (ri < 0 || ri >= RV_MAX_REFS || ri >= pic->num_ref[l]) return RV_E_RANGE;
-            rf[l] = ref_frame(pic->ref_slot[l][ri]);
+            rf[l] = ref_frame(pic->ref_slot[l][pu.ref_idx[l]]);
             if (!rf[l]) return RV_E_RANGE;
             mv[l][0] = pu.mv[l].x;
             mv[l][1] = pu.mv[l].y;
-            mi.ref_idx[l] = ri;
-            mi.mv[l][0] = mv[l][0];
-            mi.mv[l][1] = mv[l][1];
-            mi.ref_delta[l] = pic->poc - pic->ref_poc[l][ri];
-            if (pic->ref_long_term[l][ri]) mi.flags |= 1 << l;
         }
-        if (!rf[0] && !rf[1]) return RV_E_RANGE;
-        fill4(info.motion, info.w4, pu.x, pu.y, pu.w, pu.h, mi);
-        for (int yy = pu.y; yy < std::min(pu.y + pu.h, H); yy += 4) info.edges[info.idx4(pu.x, yy)] |= 2;
-        for (int xx = pu.x; xx < std::min(pu.x + pu.w, W); xx += 4) info.edges[info.idx4(xx, pu.y)] |= 8;
         // Weighting (src/hevc/ctu.rs:1254-1292).
         Weight wt[3];
         for (int c = 0; c < 3; ++c) {
@@ -958,18 +990,8 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
         }
         predict_block(mc, cur, bd, pu.x, pu.y, pu.w, pu.h, rf, mv, wt);
     }
-    // Transform units: edges and cbf (src/hevc/ctu.rs:1493-1510), then the
-    // residual (src/hevc/ctu.rs:2031-2088).
+    // Residual (src/hevc/ctu.rs:2031-2088).
     std::vector<int16_t> coeffs(32 * 32);
-    for (int i = 0; i < pic->n_tus; ++i) {
-        const rv_hevc_tu& tu = pic->tus[i];
-        int n = 1 << tu.log2;
-        if (tu.comp == 0) {
-            for (int yy = tu.y; yy < std::min(tu.y + n, H); yy += 4) info.edges[info.idx4(tu.x, yy)] |= 1;
-            for (int xx = tu.x; xx < std::min(tu.x + n, W); xx += 4) info.edges[info.idx4(xx, tu.y)] |= 4;
-            if (tu.flags & RV_TU_CODED) fill4(info.cbf_luma, info.w4, tu.x, tu.y, n, n, (uint8_t)1);
-        }
-    }
     if (stop_after != 1) {
         for (int i = 0; i < pic->n_tus; ++i) {
             const rv_hevc_tu& tu = pic->tus[i];
@@ -1121,6 +1143,19 @@ void orc_hevc_deblock_chroma(int ss, int vertical, void* data, int off, int stri
     }
     long step = vertical ? 1 : stride, along = vertical ? stride : 1;
     ORC_BOTH(ss, deblock_chroma((S*)data, off, step, along, t, np, nq, max));
+}
+
+int orc_hevc_bs_map(const rv_hevc_pic* pic, uint8_t* ver, uint8_t* hor) {
+    if (!pic || !ver || !hor) return RV_E_INVALID;
+    if (pic->width <= 0 || pic->height <= 0 || (pic->width & 7) || (pic->height & 7)) return RV_E_RANGE;
+    PicInfo info;
+    int rc = build_info(pic, info);
+    if (rc) return rc;
+    std::vector<uint8_t> v, h;
+    boundary_strengths(info, v, h);
+    std::copy(v.begin(), v.end(), ver);
+    std::copy(h.begin(), h.end(), hor);
+    return RV_OK;
 }
 
 int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],

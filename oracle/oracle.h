@@ -67,6 +67,13 @@ void orc_hevc_deblock_chroma(int ss, int vertical, void* data, int off, int stri
 int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots,
                          void* const out[3], int stop_after);
 
+/* Boundary strengths of the deblocking filter (boundary_strengths,
+ * src/hevc/deblock.rs:62-180) from a picture's work lists: ver[by*w4+bx]
+ * for the vertical edge at the left of 4x4 block (bx, by), hor[...] for
+ * the horizontal edge above it; w4 = width/4, h4 = height/4 entries each,
+ * 0 where no edge is filtered. */
+int orc_hevc_bs_map(const rv_hevc_pic* pic, uint8_t* ver, uint8_t* hor);
+
 /* ---- H.264 dsp entries (H264Dsp<S>, src/dsp/h264.rs:123-170), 8-bit -- */
 void orc_h264_qpel(int pos, uint8_t* dst, const uint8_t* src, int stride, int w, int h, int max);
 void orc_h264_chroma(uint8_t* dst, const uint8_t* src, int stride, int w, int h, int xf, int yf);
@@ -91,6 +98,12 @@ void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long
 
 int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots,
                          void* const out[3], int stop_after);
+/* Boundary strengths of the deblocking filter (src/h264/deblock.rs:
+ * 97-292, 617-756) from a picture's work lists: per macroblock (raster),
+ * 8 words: the vertical edges 0..3 (0 = the macroblock's left edge), then
+ * the horizontal edges 0..3 (0 = its top edge). Byte k of a word is the bS
+ * of the edge's 4-sample segment k (top to bottom / left to right). */
+int orc_h264_bs_map(const rv_h264_pic* pic, uint32_t* out);
 
 /* ---- Decoded picture hash (src/hevc/hash.rs) -------------------------
  * Over one component's message (n bytes): crc16 (:160-174), checksum
