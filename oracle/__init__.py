@@ -59,6 +59,7 @@ _sig = {
     "orc_hevc_deblock_luma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, i32p, u8p, u8p, C.c_int]),
     "orc_hevc_deblock_chroma": (None, [C.c_int, C.c_int, P, C.c_int, C.c_int, i32p, u8p, u8p, C.c_int]),
     "orc_hevc_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+    "orc_hevc_reconstruct_mt": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int, C.c_int]),
     "orc_hevc_bs_map": (C.c_int, [P, P, P]),
     "orc_h264_bs_map": (C.c_int, [P, P]),
     "orc_h264_qpel": (None, [C.c_int, u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -151,11 +152,12 @@ def h264_bs_map(pic):
     return b
 
 
-def reconstruct(codec: int, pic, slots: list, stop_after: int = 0):
+def reconstruct(codec: int, pic, slots: list, stop_after: int = 0, threads: int = 1):
     """Run the oracle on a picture description (ctypes pointer).
 
     slots[k] = (y, cb, cr) numpy planes of DPB slot k (or None).
-    Returns the three output planes.
+    Returns the three output planes. threads > 1 (HEVC) splits the
+    picture's work over that many threads.
     """
     c = pic.contents
     w, h = c.width, c.height
@@ -172,8 +174,11 @@ def reconstruct(codec: int, pic, slots: list, stop_after: int = 0):
                 flat[3 * k + j] = a.ctypes.data
     outs = [np.zeros((h, w), dt), np.zeros((h // 2, w // 2), dt), np.zeros((h // 2, w // 2), dt)]
     optr = (C.c_void_p * 3)(*(o.ctypes.data for o in outs))
-    f = lib.orc_hevc_reconstruct if codec == 0 else lib.orc_h264_reconstruct
-    rc = f(C.cast(pic, C.c_void_p), flat, len(slots), optr, stop_after)
+    if codec == 0 and threads > 1:
+        rc = lib.orc_hevc_reconstruct_mt(C.cast(pic, C.c_void_p), flat, len(slots), optr, stop_after, threads)
+    else:
+        f = lib.orc_hevc_reconstruct if codec == 0 else lib.orc_h264_reconstruct
+        rc = f(C.cast(pic, C.c_void_p), flat, len(slots), optr, stop_after)
     if rc != 0:
         raise RuntimeError(f"oracle failed: {rc}")
     return outs

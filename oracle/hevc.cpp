@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "oracle.h"
@@ -433,6 +434,22 @@ template <typename T> void fill4(std::vector<T>& a, int w4, int x, int y, int w,
         for (int bx = x >> 2; bx < (x + w) >> 2; ++bx) a[by * w4 + bx] = v;
 }
 
+// f(lo, hi) over [0, n) split into `threads` contiguous ranges, each on a
+// thread of its own (the baseline's row and block parallelism; the
+// reference spreads a picture's rows over its pool, src/threading.rs:176-356).
+template <typename F> void parallel_for(int n, int threads, F f) {
+    if (threads <= 1 || n <= 1) {
+        f(0, n);
+        return;
+    }
+    const int t = std::min(threads, n);
+    std::vector<std::thread> ts;
+    for (int i = 1; i < t; ++i)
+        ts.emplace_back([&f, n, t, i] { f((int)((long long)n * i / t), (int)((long long)n * (i + 1) / t)); });
+    f(0, (int)((long long)n / t));
+    for (auto& th : ts) th.join();
+}
+
 // chroma_qp (src/hevc/ctu.rs:3072-3084), 4:2:0 (ChromaArrayType 1).
 int chroma_qp420(int qpi) {
     if (qpi < 30) return qpi;
@@ -631,7 +648,7 @@ struct SliceParams {
 // deblock_rows (src/hevc/deblock.rs:210-515) for rows 0..h4 (the whole
 // picture: the standard's order, which row-by-row application equals).
 template <typename S>
-void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, int bd) {
+void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, int bd, int threads) {
     std::vector<uint8_t> ver, hor;
     boundary_strengths(info, ver, hor);
     int w4 = info.w4, h4 = info.h4;
@@ -651,11 +668,17 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
             out[c] = TC_TABLE[clampi(qpc + 2 + sl.tc_offset, 0, 53)] * (1 << sh_c);
         }
     };
+    // Each pass over bands of four 4x4 rows (16 luma rows): a vertical edge
+    // changes only its own rows, and horizontal edges 8 rows apart change
+    // disjoint rows (3 either side) and read none another changes, so bands
+    // run in any order and on any thread.
     for (int pass = 0; pass < 2; ++pass) {
-        const std::vector<uint8_t>& bs = pass == 0 ? ver : hor;
+      const std::vector<uint8_t>& bs = pass == 0 ? ver : hor;
+      parallel_for((h4 + 3) / 4, threads, [&](int band0, int band1) {
+        const int lo = band0 * 4, hi = std::min(band1 * 4, h4);
         Plane<S>& Y = f.p[0];
         if (pass == 0) {
-            for (int by = 0; by < h4; by += 2) {
+            for (int by = lo; by < hi; by += 2) {
                 bool two = by + 1 < h4;
                 for (int bx = 2; bx < w4; bx += 2) {
                     int b0 = bs[by * w4 + bx], b1 = two ? bs[(by + 1) * w4 + bx] : 0;
@@ -672,8 +695,7 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
                 }
             }
         } else {
-            int by0 = 2;
-            for (int by = by0; by < h4; by += 2) {
+            for (int by = std::max(lo, 2); by < hi; by += 2) {
                 for (int bx = 0; bx < w4; bx += 2) {
                     int b0 = bs[by * w4 + bx], b1 = bx + 1 < w4 ? bs[by * w4 + bx + 1] : 0;
                     if ((b0 | b1) == 0) continue;
@@ -693,7 +715,7 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
         Plane<S>& Cb = f.p[1];
         Plane<S>& Cr = f.p[2];
         if (pass == 0) {
-            for (int by = 0; by < h4; by += 4) {
+            for (int by = lo; by < hi; by += 4) {
                 int cnt = std::min(h4 - by, 4);
                 for (int bx = 0; bx < w4; bx += 4) {
                     bool any = false;
@@ -717,7 +739,7 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
                 }
             }
         } else {
-            for (int by = 0; by < h4; ++by) {
+            for (int by = lo; by < hi; ++by) {
                 if ((by * 4) % 16 != 0) continue;
                 for (int bx = 0; bx < w4; bx += 4) {
                     int cnt = std::min(w4 - bx, 4);
@@ -742,6 +764,7 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
                 }
             }
         }
+      });
     }
 }
 
@@ -820,10 +843,13 @@ void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, in
 }
 
 template <typename S>
-void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv_hevc_pic* pic) {
+void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv_hevc_pic* pic, int threads) {
     int ctb = 1 << info.log2_ctb;
     int pw = f.p[0].width, ph = f.p[0].height;
-    for (int ry = 0; ry < info.hc; ++ry)
+    // CTBs read the deblocked copy and write only their own samples: rows of
+    // CTBs run on any thread.
+    parallel_for(info.hc, threads, [&](int ry0, int ry1) {
+    for (int ry = ry0; ry < ry1; ++ry)
         for (int rx = 0; rx < info.wc; ++rx) {
             const rv_sao* params = &pic->sao[3 * (ry * info.wc + rx)];
             rv_sao eff[3];
@@ -846,6 +872,7 @@ void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv
                 sao_ctb(deb.p[c], f.p[c], info, x0, y0, w, h, sw, sh, pic->bit_depth, eff[c], nb);
             }
         }
+    });
 }
 
 // ---------------------------------------------------------------------
@@ -922,7 +949,7 @@ int build_info(const rv_hevc_pic* pic, PicInfo& info) {
 
 template <typename S>
 int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
-                int stop_after) {
+                int stop_after, int threads) {
     int W = pic->width, H = pic->height, bd = pic->bit_depth;
     if (W <= 0 || H <= 0 || (W & 7) || (H & 7)) return RV_E_RANGE;
     // Reference frames (padded, border extended, as a decoded picture is).
@@ -961,67 +988,82 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
             pr += n * n;
         }
     }
-    // Prediction units.
-    Mc<S> mc;
-    for (int i = 0; i < pic->n_pus; ++i) {
-        const rv_hevc_pu& pu = pic->pus[i];
-        const Frame<S>* rf[2] = {nullptr, nullptr};
-        int mv[2][2] = {{0, 0}, {0, 0}};
-        for (int l = 0; l < 2; ++l) {
-            if (!((pu.dir >> l) & 1)) continue;
-            rf[l] = ref_frame(pic->ref_slot[l][pu.ref_idx[l]]);
-            if (!rf[l]) return RV_E_RANGE;
-            mv[l][0] = pu.mv[l].x;
-            mv[l][1] = pu.mv[l].y;
-        }
-        // Weighting (src/hevc/ctu.rs:1254-1292).
-        Weight wt[3];
-        for (int c = 0; c < 3; ++c) {
-            wt[c].explicit_ = pic->weighted != 0;
-            wt[c].log2_wd = pic->log2_wd[c];
+    // Every reference a PU names, loaded before the PUs run on threads.
+    for (int i = 0; i < pic->n_pus; ++i)
+        for (int l = 0; l < 2; ++l)
+            if (((pic->pus[i].dir >> l) & 1) && !ref_frame(pic->ref_slot[l][pic->pus[i].ref_idx[l]])) return RV_E_RANGE;
+    // Prediction units: disjoint outputs, so ranges of PUs run on threads,
+    // each with its own filter scratch.
+    parallel_for(pic->n_pus, threads, [&](int i0, int i1) {
+        Mc<S> mc;
+        for (int i = i0; i < i1; ++i) {
+            const rv_hevc_pu& pu = pic->pus[i];
+            const Frame<S>* rf[2] = {nullptr, nullptr};
+            int mv[2][2] = {{0, 0}, {0, 0}};
             for (int l = 0; l < 2; ++l) {
-                wt[c].w[l] = 1;
-                wt[c].o[l] = 0;
-                if (rf[l]) {
-                    wt[c].w[l] = pic->wp_w[l][pu.ref_idx[l]][c];
-                    wt[c].o[l] = pic->wp_o[l][pu.ref_idx[l]][c];
+                if (!((pu.dir >> l) & 1)) continue;
+                rf[l] = ref_frame(pic->ref_slot[l][pu.ref_idx[l]]);
+                mv[l][0] = pu.mv[l].x;
+                mv[l][1] = pu.mv[l].y;
+            }
+            // Weighting (src/hevc/ctu.rs:1254-1292).
+            Weight wt[3];
+            for (int c = 0; c < 3; ++c) {
+                wt[c].explicit_ = pic->weighted != 0;
+                wt[c].log2_wd = pic->log2_wd[c];
+                for (int l = 0; l < 2; ++l) {
+                    wt[c].w[l] = 1;
+                    wt[c].o[l] = 0;
+                    if (rf[l]) {
+                        wt[c].w[l] = pic->wp_w[l][pu.ref_idx[l]][c];
+                        wt[c].o[l] = pic->wp_o[l][pu.ref_idx[l]][c];
+                    }
                 }
             }
+            predict_block(mc, cur, bd, pu.x, pu.y, pu.w, pu.h, rf, mv, wt);
         }
-        predict_block(mc, cur, bd, pu.x, pu.y, pu.w, pu.h, rf, mv, wt);
-    }
-    // Residual (src/hevc/ctu.rs:2031-2088).
-    std::vector<int16_t> coeffs(32 * 32);
+    });
+    // Residual (src/hevc/ctu.rs:2031-2088): TUs of a component do not
+    // overlap, so ranges of TUs run on threads.
     if (stop_after != 1) {
         for (int i = 0; i < pic->n_tus; ++i) {
             const rv_hevc_tu& tu = pic->tus[i];
             if (!(tu.flags & RV_TU_CODED)) continue;
             int n = 1 << tu.log2;
             if (tu.coef_off + (uint64_t)tu.n_nz > pic->n_nz) return RV_E_RANGE;
-            // The dense raster the reference's IdctFn takes: the nonzero
-            // list scattered into zeros.
-            std::fill(coeffs.begin(), coeffs.begin() + n * n, (int16_t)0);
-            for (uint32_t k = 0; k < tu.n_nz; ++k) {
-                const rv_nz& e = pic->nz[tu.coef_off + k];
-                if (e.pos >= n * n) return RV_E_RANGE;
-                coeffs[e.pos] = e.val;
-            }
-            int bd_shift = 20 - bd;
-            if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
-            else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);
-            Plane<S>& pl = cur.p[tu.comp];
-            add_residual(&pl.data[pl.offset(tu.x, tu.y)], pl.stride, coeffs.data(), n, max);
+            for (uint32_t k = 0; k < tu.n_nz; ++k)
+                if (pic->nz[tu.coef_off + k].pos >= n * n) return RV_E_RANGE;
         }
+        parallel_for(pic->n_tus, threads, [&](int i0, int i1) {
+            std::vector<int16_t> coeffs(32 * 32);
+            for (int i = i0; i < i1; ++i) {
+                const rv_hevc_tu& tu = pic->tus[i];
+                if (!(tu.flags & RV_TU_CODED)) continue;
+                int n = 1 << tu.log2;
+                // The dense raster the reference's IdctFn takes: the nonzero
+                // list scattered into zeros.
+                std::fill(coeffs.begin(), coeffs.begin() + n * n, (int16_t)0);
+                for (uint32_t k = 0; k < tu.n_nz; ++k) {
+                    const rv_nz& e = pic->nz[tu.coef_off + k];
+                    coeffs[e.pos] = e.val;
+                }
+                int bd_shift = 20 - bd;
+                if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
+                else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);
+                Plane<S>& pl = cur.p[tu.comp];
+                add_residual(&pl.data[pl.offset(tu.x, tu.y)], pl.stride, coeffs.data(), n, max);
+            }
+        });
     }
     if (stop_after == 0 || stop_after >= 3) {
         if (pic->deblock) {
             SliceParams sl{pic->beta_offset, pic->tc_offset, pic->cb_qp_offset, pic->cr_qp_offset};
-            deblock_picture(cur, info, sl, bd);
+            deblock_picture(cur, info, sl, bd, threads);
         }
     }
     if ((stop_after == 0 || stop_after >= 4) && (pic->sao_luma || pic->sao_chroma) && pic->sao) {
         Frame<S> deb = cur;  // the deblocked copy SAO reads (src/hevc/sao.rs:1-4)
-        sao_picture(cur, deb, info, pic);
+        sao_picture(cur, deb, info, pic, threads);
     }
     for (int c = 0; c < 3; ++c) cur.p[c].store((S*)out[c]);
     return RV_OK;
@@ -1158,13 +1200,18 @@ int orc_hevc_bs_map(const rv_hevc_pic* pic, uint8_t* ver, uint8_t* hor) {
     return RV_OK;
 }
 
+int orc_hevc_reconstruct_mt(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots,
+                            void* const out[3], int stop_after, int threads) {
+    if (!pic || !slot_planes || !out || threads < 1) return RV_E_INVALID;
+    if (pic->bit_depth == 8) return reconstruct<uint8_t>(pic, slot_planes, n_slots, out, stop_after, threads);
+    if (pic->bit_depth > 8 && pic->bit_depth <= 12)
+        return reconstruct<uint16_t>(pic, slot_planes, n_slots, out, stop_after, threads);
+    return RV_E_RANGE;
+}
+
 int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
                          int stop_after) {
-    if (!pic || !slot_planes || !out) return RV_E_INVALID;
-    if (pic->bit_depth == 8) return reconstruct<uint8_t>(pic, slot_planes, n_slots, out, stop_after);
-    if (pic->bit_depth > 8 && pic->bit_depth <= 12)
-        return reconstruct<uint16_t>(pic, slot_planes, n_slots, out, stop_after);
-    return RV_E_RANGE;
+    return orc_hevc_reconstruct_mt(pic, slot_planes, n_slots, out, stop_after, 1);
 }
 
 }  // extern "C"

@@ -66,6 +66,11 @@ void orc_hevc_deblock_chroma(int ss, int vertical, void* data, int off, int stri
  * 1 = prediction only, 2 = + residual, 3 = + deblocking, 0/4 = + SAO. */
 int orc_hevc_reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots,
                          void* const out[3], int stop_after);
+/* The same on `threads` threads: PUs and TUs in ranges, deblocking in
+ * bands of 16 luma rows per pass, SAO by CTB rows (the CPU baseline's
+ * parallelism within a picture). Output identical to the above. */
+int orc_hevc_reconstruct_mt(const rv_hevc_pic* pic, const void* const* slot_planes, int n_slots,
+                            void* const out[3], int stop_after, int threads);
 
 /* Boundary strengths of the deblocking filter (boundary_strengths,
  * src/hevc/deblock.rs:62-180) from a picture's work lists: ver[by*w4+bx]

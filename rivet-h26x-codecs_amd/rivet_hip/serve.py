@@ -119,11 +119,15 @@ def _pic_cfg(w, h, bd, kind, poc, ref_pocs, ref_slots):
 
 
 def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop: int = GOP, threads: int = 16,
-                   sizes=None):
-    """Sessions with generated work lists. rate = 0: all arrive at t = 0."""
+                   sizes=None, depths=None):
+    """Sessions with generated work lists. rate = 0: all arrive at t = 0.
+    sizes / depths, when given, replace the drawn resolution / bit depth of
+    session i by entry i modulo their length."""
     specs = session_cfgs(n, seed)
     if sizes is not None:
         specs = [(sizes[i % len(sizes)][0], sizes[i % len(sizes)][1], s[2], s[3]) for i, s in enumerate(specs)]
+    if depths is not None:
+        specs = [(s[0], s[1], depths[i % len(depths)], s[3]) for i, s in enumerate(specs)]
     rng = np.random.default_rng(subseed(seed, 5, 0, 2) & 0xFFFFFFFF)
     t = 0.0
     sessions = []

@@ -785,3 +785,18 @@ def test_the_32_point_matrix_is_the_standards():
             assert m[k, n] == want, (k, n)
     for k, row in ((1, ROW1), (2, ROW2), (3, ROW3), (16, ROW16), (31, ROW31)):
         assert m[k].tolist() == row, k
+
+
+# ---------------------------------------------------------------- the threaded driver
+
+@pytest.mark.parametrize("threads", [2, 3, 7, 16])
+def test_threaded_driver_equals_the_serial_one(threads):
+    """orc_hevc_reconstruct_mt (the CPU baseline's row- and block-parallel
+    driver) gives the serial driver's bytes at every cut point."""
+    for cfg, seed in ((cf.HEVC_C1, 901), (dict(cf.HEVC_C2, width=832, height=480), 902)):
+        g = gen(cfg, seed)
+        refs = refs_of(g, seed)
+        for stop in (1, 2, 3, 0):
+            want = run(g, refs, stop)
+            got = oracle.reconstruct(rh.RV_CODEC_HEVC, g.pic, list(refs) + [None], stop, threads=threads)
+            assert_planes_equal(got, want, f"threads {threads} stop {stop}")
