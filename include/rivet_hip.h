@@ -51,6 +51,10 @@ extern "C" {
 #define RV_E_NOMEM (-4)    /* device or pinned allocation failed        */
 #define RV_E_RANGE (-5)    /* size / depth / fraction outside the range */
 #define RV_E_STATE (-6)    /* call out of order (e.g. job still running) */
+#define RV_E_DEVICE (-7)   /* a kernel's bounded wait gave up since the last
+                              check: outputs of the launches completed since
+                              then are not valid (reported once, by the
+                              synchronising call that sees it) */
 
 /* Limits of the work-list formats. */
 #define RV_MAX_REFS 4 /* reference indices per list (DPB of up to 4 refs) */
@@ -321,6 +325,11 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* pic, int out_slot,
                     rv_job** out);
 int rv_h264_prepare(rv_session* s, const rv_h264_pic* pic, int out_slot,
                     rv_job** out);
+/* rv_job_free, rv_batch_free and rv_session_destroy never wait for the
+ * GPU: device memory goes back to the library's pool once the work launched
+ * before the free has completed (a completion event), and later prepares
+ * and sessions reuse it instead of allocating. A batch launched from a
+ * captured graph is the exception: freeing it waits for the graph. */
 int rv_job_free(rv_job* j);
 
 /* A batch: a set of prepared jobs launched together. Creating it uploads
@@ -351,13 +360,6 @@ int rv_mark_wait(int id);
 /* Milliseconds between two markers (both complete). */
 int rv_mark_elapsed(int id_start, int id_end, float* ms);
 
-/* Kept for callers of the earlier deferred-free switch; has no effect.
- * rv_job_free, rv_batch_free and rv_session_destroy never wait for the GPU:
- * device memory goes back to the library's pool once the work launched
- * before the free has completed (a completion event), and later prepares
- * and sessions reuse it instead of allocating. A batch launched from a
- * captured graph is the exception: freeing it waits for the graph. */
-int rv_defer_free(int enable);
 
 /* Kernel variants and the autotuner. No counterpart in the reference,
  * whose tables have one kernel per SIMD tier; the reference's C3

@@ -1,8 +1,9 @@
-"""Build the HIP tier in-tree: rivet_hip/librivet_hip.so for gfx950.
+"""Build the HIP tier in-tree: rivet_hip/librivet_hip.so for gfx950, and
+rivet_hip/librivet_gen.so, the host-only generator of synthetic work lists.
 
-Each HIP source is compiled with hipcc for gfx950 only; the host-only
-generator with g++. The library is written next to the Python package so
-it travels with the source tree and is what the package loads.
+Each HIP source is compiled with hipcc for gfx950 only; the generator with
+g++. Both libraries are written next to the Python package so they travel
+with the source tree and are what the package loads.
 """
 from __future__ import annotations
 
@@ -15,6 +16,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "rivet_hip" / "librivet_hip.so"
+GEN_OUT = HERE / "rivet_hip" / "librivet_gen.so"
 OBJ = HERE / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -53,12 +55,17 @@ def build(verbose: bool = False) -> Path:
         jobs.append(["g++", *CXX_FLAGS, "-c", str(CSRC / s), "-o", str(o)])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(_run, jobs))
-    objs = [str(OBJ / (s + ".o")) for s in HIP_SOURCES + CXX_SOURCES]
+    objs = [str(OBJ / (s + ".o")) for s in HIP_SOURCES]
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", str(tmp), *objs])
     os.replace(tmp, OUT)
+    # The synthetic work-list generator is a host library of its own: the
+    # device tier carries no test-workload code.
+    gtmp = GEN_OUT.with_suffix(".so.tmp")
+    _run(["g++", "-shared", "-o", str(gtmp), *[str(OBJ / (s + ".o")) for s in CXX_SOURCES]])
+    os.replace(gtmp, GEN_OUT)
     if verbose:
-        print("built", OUT)
+        print("built", OUT, "and", GEN_OUT)
     return OUT
 
 

@@ -934,10 +934,11 @@ constexpr int kDbkTile = 96;   // words: luma 16 x 16 bytes, then Cb and Cr 8 x 
 constexpr int kDbkRing = 24;   // words per entry: 16 luma columns, 16 chroma (16-bit)
 constexpr int kDbkRingN = 3;   // entries per slot
 constexpr int kDbkPer = kDbkTile + kDbkRing * kDbkRingN;
-// Poll bound of the exchange: far beyond any real wait (the band before is
-// running and publishes within a few steps); a poll that reaches it goes on
+// The exchange's poll bound (H264PicDev::xch_spin, 2^18 polls) is far beyond
+// any real wait: the band before is running and publishes within a few
+// steps. A poll that reaches it counts an error on the device, which the
+// host reports at its next synchronisation as RV_E_DEVICE, and goes on
 // with what it read rather than hang the workgroup.
-constexpr int kDbkSpin = 1 << 18;
 static_assert(kDbkSlots % 4 == 0 && kDbkThreads <= 1024, "four slots of 16 lanes per 64-lane wave");
 
 // The step at which slot s reaches macroblock 0: one step after slot s - 1
@@ -1092,12 +1093,15 @@ __device__ __forceinline__ uint64_t xch_load(gptr<const uint64_t> p) {
 __device__ __forceinline__ void xch_store(gptr<uint64_t> p, uint64_t v) {
     __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Poll an exchange word until it carries `tag`.
-__device__ __forceinline__ uint64_t xch_wait(gptr<const uint64_t> p, uint64_t v, uint32_t tag) {
-    for (int n = 0; (uint32_t)(v >> 32) != tag && n < kDbkSpin; ++n) {
+// Poll an exchange word until it carries `tag`, at most P.xch_spin times;
+// a wait that gives up is counted in *P.err.
+__device__ __forceinline__ uint64_t xch_wait(const H264PicDev& P, gptr<const uint64_t> p, uint64_t v, uint32_t tag) {
+    tag ^= P.xch_tag_xor;
+    for (int n = 0; (uint32_t)(v >> 32) != tag && n < P.xch_spin; ++n) {
         __builtin_amdgcn_s_sleep(1);
         v = xch_load(p);
     }
+    if ((uint32_t)(v >> 32) != tag) __hip_atomic_fetch_add(P.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return v;
 }
 
@@ -1227,7 +1231,7 @@ __global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
                 const uint32_t* ue = up + cex * kDbkRing;
                 u = lu ? ue[r] : ((const uint16_t*)(ue + 16))[pl * 8 + rr];
             } else if (first) {
-                u = (uint32_t)xch_wait(imp + (size_t)cx * kDbkRing, in.ix, (uint32_t)cx + 1);
+                u = (uint32_t)xch_wait(P, imp + (size_t)cx * kDbkRing, in.ix, (uint32_t)cx + 1);
                 if (!lu) u = (u >> (16 * (r & 1))) & 0xFFFF;
             }
             if (lu) {

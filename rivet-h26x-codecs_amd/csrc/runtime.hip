@@ -28,6 +28,12 @@ hipStream_t g_stream = nullptr;
 hipStream_t g_io = nullptr;  // plane uploads / downloads
 hipStream_t g_up = nullptr;  // work-list and batch-table uploads
 int g_stage_mask = 0;
+// Device error counter: kernels whose bounded waits give up add to it
+// (H264PicDev::err). The host reports each increase once, as RV_E_DEVICE,
+// from the next synchronising call that reads it.
+uint32_t* g_dev_err = nullptr;
+uint32_t g_err_seen = 0;
+std::mutex g_err_mu;
 constexpr int kMarks = 4096;
 hipEvent_t g_marks[kMarks];
 int g_next_mark = 0;
@@ -46,6 +52,21 @@ int hip_fail(hipError_t e, const char* what) {
     } while (0)
 
 bool ready() { return g_device >= 0 && g_stream; }
+
+// RV_E_DEVICE when the device error counter rose since the last check.
+// Called after a wait, so the launches it covers have completed.
+int check_device_errors() {
+    if (!g_dev_err) return RV_OK;
+    uint32_t n = 0;
+    hipError_t e = hipMemcpy(&n, g_dev_err, sizeof(n), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "device error counter");
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    if (n == g_err_seen) return RV_OK;
+    const uint32_t k = n - g_err_seen;
+    g_err_seen = n;
+    return fail(RV_E_DEVICE, std::to_string(k) + " H.264 deblocking exchange wait(s) gave up (a band's strips "
+                             "never arrived); the pictures launched since the last check are not valid");
+}
 
 }  // namespace
 
@@ -73,6 +94,13 @@ int lib_fail(int code, const std::string& msg) { return fail(code, msg); }
 namespace {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Runs f when the scope ends, on every return path.
+template <typename F> struct OnExit {
+    F f;
+    ~OnExit() { f(); }
+};
+template <typename F> OnExit<F> on_exit(F f) { return OnExit<F>{f}; }
 
 // Bytes after each DPB plane: motion compensation reads reference rows
 // with aligned dword loads that may extend up to 4 bytes past a row's last
@@ -534,6 +562,9 @@ int rv_init(int device_ordinal) {
         RV_HIP(hipStreamCreateWithFlags(&g_io, hipStreamNonBlocking));
         RV_HIP(hipStreamCreateWithFlags(&g_up, hipStreamNonBlocking));
         for (int i = 0; i < kMarks; ++i) RV_HIP(hipEventCreate(&g_marks[i]));
+        RV_HIP(hipMalloc((void**)&g_dev_err, 256));
+        RV_HIP(hipMemset(g_dev_err, 0, 256));
+        g_err_seen = 0;
         if (device_ordinal < kMaxDevices) {
             void* ptr = nullptr;
             RV_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
@@ -669,7 +700,7 @@ int rv_plane_download(rv_session* s, int slot, void* y, void* cb, void* cr) {
                                 (size_t)s->g.pitch[c] * s->g.ss, (size_t)s->g.pw[c] * s->g.ss, s->g.ph[c],
                                 hipMemcpyDeviceToHost, g_io));
     RV_HIP(hipStreamSynchronize(g_io));
-    return RV_OK;
+    return check_device_errors();
 }
 
 int rv_picture_hash(rv_session* s, int slot, int type, void* out) {
@@ -1178,6 +1209,15 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         sp += carve_size<uint32_t>(nmb * rv::kH264DbkWords);
         d.xch = (uint64_t*)sp;
         d.n_xch = (int64_t)nx;
+        // The exchange's poll bound and awaited-tag XOR. A job prepared
+        // while the environment sets RIVET_HIP_TEST_EXCHANGE_MISS=1 awaits
+        // tags that are never published and gives up after 16 polls, so the
+        // error path can be tested with kernels that still finish normally.
+        const char* miss = std::getenv("RIVET_HIP_TEST_EXCHANGE_MISS");
+        const bool force_miss = miss && *miss && *miss != '0';
+        d.xch_spin = force_miss ? 16 : 1 << 18;
+        d.xch_tag_xor = force_miss ? 0x80000000u : 0u;
+        d.err = g_dev_err;
         rv::h264_stage_items(d);
         *hdesc = d;
         j->desc_dev = dev((char*)hdesc);
@@ -1473,9 +1513,11 @@ int rv_batch_launch(rv_batch* b, int mode) {
                 int rc = launch_batched(b, g_stream);
                 hipGraph_t graph = nullptr;
                 hipError_t e = hipStreamEndCapture(g_stream, &graph);
-                if (rc) return rc;
-                if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-                b->graph = graph;
+                if (rc || e != hipSuccess) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc ? rc : hip_fail(e, "hipStreamEndCapture");
+                }
+                b->graph = graph;  // destroyed with the batch from here on
                 RV_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
             }
             RV_HIP(hipGraphLaunch(b->exec, g_stream));
@@ -1492,7 +1534,11 @@ int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
         if (!b || !out_ms || reps <= 0) return fail(RV_E_INVALID, "bad argument");
         if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
         std::lock_guard<std::mutex> lk(g_mu);
-        hipEvent_t ev[rv::kStages + 1];
+        hipEvent_t ev[rv::kStages + 1] = {};
+        auto destroy = on_exit([&] {
+            for (auto& e : ev)
+                if (e) (void)hipEventDestroy(e);
+        });
         for (auto& e : ev) RV_HIP(hipEventCreate(&e));
         double acc[rv::kStages] = {0, 0, 0, 0, 0};
         int rc = RV_OK;
@@ -1518,7 +1564,6 @@ int rv_batch_profile(rv_batch* b, int reps, float out_ms[5]) {
                 acc[s] += ms;
             }
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
         for (int s = 0; s < rv::kStages; ++s) out_ms[s] = (float)(acc[s] / reps);
         const int rn = note_launch(b->jobs);
         return rc ? rc : rn;
@@ -1533,12 +1578,16 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
         if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
         bytes &= ~(int64_t)4095;
         void *a = nullptr, *b = nullptr;
-        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = on_exit([&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
             if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        });
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess)
             return fail(RV_E_NOMEM, "probe buffers");
-        }
         std::lock_guard<std::mutex> lk(g_mu);
-        hipEvent_t e0, e1;
         RV_HIP(hipEventCreate(&e0));
         RV_HIP(hipEventCreate(&e1));
         RV_HIP(hipMemsetAsync(a, 1, bytes, g_stream));
@@ -1549,10 +1598,6 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
         RV_HIP(hipEventSynchronize(e1));
         float ms = 0;
         RV_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFree(a);
-        (void)hipFree(b);
         *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
         return RV_OK;
     } catch (...) {
@@ -1563,14 +1608,7 @@ int rv_probe_copy(int64_t bytes, int reps, double* gbs) {
 int rv_sync(void) {
     if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
     RV_HIP(hipStreamSynchronize(g_stream));
-    return RV_OK;
-}
-
-int rv_defer_free(int enable) {
-    // Frees never wait for the GPU (buffers return to the pool by
-    // completion event), so there is nothing left to defer.
-    (void)enable;
-    return RV_OK;
+    return check_device_errors();
 }
 
 int rv_mark(int* id_out) {
@@ -1595,7 +1633,7 @@ int rv_mark_query(int id) {
 int rv_mark_wait(int id) {
     if (id < 0 || id >= kMarks || !ready()) return fail(RV_E_INVALID, "bad marker");
     RV_HIP(hipEventSynchronize(g_marks[id]));
-    return RV_OK;
+    return check_device_errors();
 }
 
 int rv_mark_elapsed(int id_start, int id_end, float* ms) {

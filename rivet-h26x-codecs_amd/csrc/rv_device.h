@@ -122,6 +122,13 @@ struct H264PicDev {
     uint32_t* dbk;       // per MB: kH264DbkWords words of edge parameters
     uint64_t* xch;       // deblocking: band ticket, then the strips crossing bands
     int64_t n_xch;       // words of xch
+    // The exchange poll's bound, and a value XORed into the awaited tag (0;
+    // non-zero only where a test forces every wait to give up). A wait that
+    // gives up adds 1 to *err, the library's device error counter, which
+    // the host reads at its next synchronisation and reports.
+    int32_t xch_spin;
+    uint32_t xch_tag_xor;
+    uint32_t* err;
     int32_t items[5];
 };
 

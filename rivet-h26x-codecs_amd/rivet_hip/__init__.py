@@ -2,8 +2,9 @@
 
 Host-side wrapper over the C ABI (include/rivet_hip.h): sessions with a
 device-resident DPB, prepared jobs (one picture's work lists), batches,
-and the seeded generator of synthetic work lists. All device work is done
-by librivet_hip.so; nothing here computes picture samples.
+and the seeded generator of synthetic work lists (librivet_gen.so). All
+device work is done by librivet_hip.so; nothing here computes picture
+samples.
 """
 from __future__ import annotations
 
@@ -17,6 +18,7 @@ from ._abi import (RV_CODEC_H264, RV_CODEC_HEVC, RV_HASH_CHECKSUM, RV_HASH_CRC, 
                    RV_LAUNCH_BATCHED, RV_LAUNCH_GRAPH, RV_LAUNCH_PLAIN)
 
 lib = _abi.load()
+genlib = _abi.load_gen()  # the synthetic work-list generator (host only)
 
 
 class RivetError(RuntimeError):
@@ -59,12 +61,12 @@ def gen_planes(width: int, height: int, bit_depth: int, seed: int):
     """Uniform random 4:2:0 planes (the generator's reference pictures)."""
     dt = sample_dtype(bit_depth)
     ys = [np.empty(s, dtype=dt) for s in plane_shapes(width, height)]
-    check(lib.rv_gen_planes(width, height, bit_depth, seed, *(p.ctypes.data for p in ys)))
+    check(genlib.rv_gen_planes(width, height, bit_depth, seed, *(p.ctypes.data for p in ys)))
     return ys
 
 
 def subseed(seed: int, a: int = 0, b: int = 0, c: int = 0) -> int:
-    return int(lib.rv_gen_subseed(seed, a, b, c))
+    return int(genlib.rv_gen_subseed(seed, a, b, c))
 
 
 class GenPicture:
@@ -82,9 +84,9 @@ class GenPicture:
                     getattr(c, k)[:] = v
                 else:
                     setattr(c, k, v)
-            check(lib.rv_gen_hevc(C.byref(c), seed, C.byref(h)))
+            check(genlib.rv_gen_hevc(C.byref(c), seed, C.byref(h)))
             self._h = h
-            self.pic = lib.rv_gen_hevc_pic(h)
+            self.pic = genlib.rv_gen_hevc_pic(h)
         else:
             c = _abi.GenH264Cfg()
             for k, v in cfg.items():
@@ -92,9 +94,9 @@ class GenPicture:
                     getattr(c, k)[:] = v
                 else:
                     setattr(c, k, v)
-            check(lib.rv_gen_h264(C.byref(c), seed, C.byref(h)))
+            check(genlib.rv_gen_h264(C.byref(c), seed, C.byref(h)))
             self._h = h
-            self.pic = lib.rv_gen_h264_pic(h)
+            self.pic = genlib.rv_gen_h264_pic(h)
 
     @property
     def width(self) -> int:
@@ -111,7 +113,7 @@ class GenPicture:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and h.value:
-            lib.rv_gen_free(h)
+            genlib.rv_gen_free(h)
             self._h = None
 
 
@@ -285,11 +287,6 @@ def probe_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
     v = C.c_double()
     check(lib.rv_probe_copy(nbytes, reps, C.byref(v)))
     return float(v.value)
-
-
-def defer_free(enable: bool) -> None:
-    """Queue frees until the next sync() instead of synchronizing on each."""
-    check(lib.rv_defer_free(1 if enable else 0))
 
 
 def set_stage_mask(mask: int) -> None:

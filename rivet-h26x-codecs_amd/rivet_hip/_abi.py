@@ -1,4 +1,5 @@
-"""ctypes mirror of include/rivet_hip.h and include/rivet_gen.h.
+"""ctypes mirror of include/rivet_hip.h (librivet_hip.so) and include/rivet_gen.h
+(librivet_gen.so).
 
 The library is loaded from this directory (built in-tree by build.py);
 there is no fallback: a missing library is an ImportError naming the
@@ -10,6 +11,7 @@ import ctypes as C
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "librivet_hip.so"
+GEN_LIB_PATH = Path(__file__).resolve().parent / "librivet_gen.so"
 
 RV_OK = 0
 RV_E_INVALID = -1
@@ -18,6 +20,7 @@ RV_E_HIP = -3
 RV_E_NOMEM = -4
 RV_E_RANGE = -5
 RV_E_STATE = -6
+RV_E_DEVICE = -7
 RV_MAX_REFS = 4
 RV_TU_DST = 1
 RV_TU_CODED = 2
@@ -121,7 +124,7 @@ class GenH264Cfg(C.Structure):
 
 ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz]
 
-# Every extern "C" entry of include/rivet_hip.h and include/rivet_gen.h:
+# Every extern "C" entry of include/rivet_hip.h (librivet_hip.so):
 # name -> (restype, argtypes).
 P = C.c_void_p
 PROTOTYPES = {
@@ -154,17 +157,9 @@ PROTOTYPES = {
     "rv_mark_wait": (i32, [i32]),
     "rv_mark_elapsed": (i32, [i32, i32, C.POINTER(C.c_float)]),
     "rv_set_stage_mask": (i32, [i32]),
-    "rv_defer_free": (i32, [i32]),
     "rv_job_bytes": (i32, [P, C.POINTER(i64)]),
     "rv_abi_size": (i32, [i32]),
     "rv_probe_copy": (i32, [i64, i32, C.POINTER(C.c_double)]),
-    "rv_gen_hevc": (i32, [C.POINTER(GenHevcCfg), C.c_uint64, C.POINTER(P)]),
-    "rv_gen_h264": (i32, [C.POINTER(GenH264Cfg), C.c_uint64, C.POINTER(P)]),
-    "rv_gen_hevc_pic": (C.POINTER(HevcPic), [P]),
-    "rv_gen_h264_pic": (C.POINTER(H264Pic), [P]),
-    "rv_gen_free": (None, [P]),
-    "rv_gen_planes": (i32, [i32, i32, i32, C.c_uint64, P, P, P]),
-    "rv_gen_subseed": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
 }
 
 # The per-entry API (include/rivet_hip.h): buffers as void*, lengths,
@@ -219,16 +214,38 @@ for _s in ("u8", "u16"):
         ENTRY_PROTOTYPES[f"rv_hevc_{_s}_{_n}"] = (i32, _a)
 PROTOTYPES.update(ENTRY_PROTOTYPES)
 
+# include/rivet_gen.h (librivet_gen.so): the synthetic work-list generator,
+# a library of its own, so the device tier carries no test-workload code.
+GEN_PROTOTYPES = {
+    "rv_gen_hevc": (i32, [C.POINTER(GenHevcCfg), C.c_uint64, C.POINTER(P)]),
+    "rv_gen_h264": (i32, [C.POINTER(GenH264Cfg), C.c_uint64, C.POINTER(P)]),
+    "rv_gen_hevc_pic": (C.POINTER(HevcPic), [P]),
+    "rv_gen_h264_pic": (C.POINTER(H264Pic), [P]),
+    "rv_gen_free": (None, [P]),
+    "rv_gen_planes": (i32, [i32, i32, i32, C.c_uint64, P, P, P]),
+    "rv_gen_subseed": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
+}
+
+
+def _bind(lib: C.CDLL, protos: dict) -> C.CDLL:
+    for name, (res, args) in protos.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load_gen() -> C.CDLL:
+    if not GEN_LIB_PATH.exists():
+        raise ImportError(f"{GEN_LIB_PATH} is missing: run `python rivet-h26x-codecs_amd/build.py` first")
+    return _bind(C.CDLL(str(GEN_LIB_PATH)), GEN_PROTOTYPES)
+
 
 def load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: run `python rivet-h26x-codecs_amd/build.py` "
                           "(or __graft_entry__.build()) first")
-    lib = C.CDLL(str(LIB_PATH))
-    for name, (res, args) in PROTOTYPES.items():
-        f = getattr(lib, name)
-        f.restype = res
-        f.argtypes = args
+    lib = _bind(C.CDLL(str(LIB_PATH)), PROTOTYPES)
     for i, st in enumerate(ABI_STRUCTS):
         n = lib.rv_abi_size(i)
         if n != C.sizeof(st):

@@ -22,14 +22,28 @@ def declared(header: str) -> set:
 
 
 def test_every_declared_function_is_exported():
-    from rivet_hip import _abi, lib
+    from rivet_hip import _abi, genlib, lib
 
-    names = declared("rivet_hip.h") | declared("rivet_gen.h")
+    names = declared("rivet_hip.h")
     assert len(names) > 90
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
     # Every exported prototype in the binding is declared in a header.
     assert set(_abi.PROTOTYPES) <= names, set(_abi.PROTOTYPES) - names
+    gen = declared("rivet_gen.h") - names
+    assert gen and not [n for n in sorted(gen) if not hasattr(genlib, n)]
+    assert set(_abi.GEN_PROTOTYPES) == gen
+
+
+def test_the_device_library_carries_no_generator():
+    """librivet_hip.so exports only the tier's ABI; the synthetic work-list
+    generator lives in librivet_gen.so."""
+    from rivet_hip import _abi
+
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_abi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\b(rv_[a-z0-9_]+)$", out, re.M))
+    assert exported and not [n for n in exported if n.startswith("rv_gen")]
 
 
 def test_per_entry_family_is_complete():
