@@ -78,6 +78,21 @@ int check_device_errors() {
 // looks up the device address once, and the entry point uses that.
 static __device__ __align__(256) uint8_t scratch[256 + 3 * RV_HASH_NOALLOC_MAX_ROWS * sizeof(uint32_t)];
 
+// Small tables written by a kernel whose data travel in its arguments, so
+// that writing them is ordered on the library stream and the host never
+// waits for a copy: a host-to-device copy of a few hundred bytes runs as a
+// blit kernel that can start only once the stage kernels ahead of it have
+// drained, and a host that waited for it could not queue the next batch.
+constexpr int kFillWords = 960;
+struct WordChunk {
+    uint32_t* dst;
+    int32_t n;
+    uint32_t w[kFillWords];
+};
+__global__ void __launch_bounds__(256) rv_fill_words(WordChunk c) {
+    for (int i = threadIdx.x; i < c.n; i += 256) c.dst[i] = c.w[i];
+}
+
 namespace {
 constexpr int kMaxDevices = 64;
 void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device ordinal
@@ -122,17 +137,35 @@ struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
 };
+// An event recorded on the kernel stream after a launch (or other stream
+// work on pooled buffers), shared by the sessions and released buffers it
+// covers; its event is recycled when the last of them lets go.
+std::mutex g_launch_mu;
+std::vector<hipEvent_t> g_launch_free;  // under g_launch_mu
+struct LaunchEv {
+    hipEvent_t e = nullptr;
+    ~LaunchEv() {
+        if (!e) return;
+        std::lock_guard<std::mutex> lk(g_launch_mu);
+        g_launch_free.push_back(e);
+    }
+};
+
 std::mutex g_pool_mu;
 std::vector<DevBuf> g_pool;
-std::vector<std::pair<hipEvent_t, DevBuf>> g_pool_pending;
-std::vector<hipEvent_t> g_pool_events;
+// Released buffers, reusable once the event of the last kernel-stream work
+// enqueued before their release has completed. Every device use of a
+// pooled buffer is kernel-stream work followed by such an event (launches,
+// batch-table fills), so one event per launch covers every buffer released
+// after it: releases cost no stream packet of their own.
+std::vector<std::pair<std::shared_ptr<LaunchEv>, DevBuf>> g_pool_pending;
+std::shared_ptr<LaunchEv> g_last_work;  // under g_pool_mu
 
 // Completed releases back to the pool; the caller holds g_pool_mu.
 void pool_reap() {
     for (size_t i = 0; i < g_pool_pending.size();) {
-        if (hipEventQuery(g_pool_pending[i].first) == hipSuccess) {
+        if (hipEventQuery(g_pool_pending[i].first->e) == hipSuccess) {
             g_pool.push_back(g_pool_pending[i].second);
-            g_pool_events.push_back(g_pool_pending[i].first);
             g_pool_pending[i] = g_pool_pending.back();
             g_pool_pending.pop_back();
         } else {
@@ -169,17 +202,28 @@ bool pool_get(size_t bytes, DevBuf* out) {
 void pool_put(const DevBuf& b) {
     if (!b.p) return;
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    hipEvent_t ev = nullptr;
-    if (!g_pool_events.empty()) {
-        ev = g_pool_events.back();
-        g_pool_events.pop_back();
-    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-        (void)hipStreamSynchronize(g_stream);
-        g_pool.push_back(b);
-        return;
+    if (g_last_work) g_pool_pending.emplace_back(g_last_work, b);
+    else g_pool.push_back(b);  // no kernel-stream work has touched it yet
+}
+
+// Records the event that covers the kernel-stream work enqueued so far
+// (the caller holds g_mu, so nothing else is enqueued meanwhile).
+std::shared_ptr<LaunchEv> note_stream_work(hipError_t* err) {
+    auto ev = std::make_shared<LaunchEv>();
+    {
+        std::lock_guard<std::mutex> lk(g_launch_mu);
+        if (!g_launch_free.empty()) {
+            ev->e = g_launch_free.back();
+            g_launch_free.pop_back();
+        }
     }
-    (void)hipEventRecord(ev, g_stream);
-    g_pool_pending.emplace_back(ev, b);
+    hipError_t e = ev->e ? hipSuccess : hipEventCreateWithFlags(&ev->e, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev->e, g_stream);
+    *err = e;
+    if (e != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_last_work = ev;
+    return ev;
 }
 
 // Pinned host staging of the calling thread (work lists and batch tables
@@ -217,17 +261,6 @@ hipError_t stage_upload(void* dst, size_t bytes, void* zero, size_t zero_bytes) 
     return e;
 }
 
-// An event recorded after one launch, shared by the sessions it covered;
-// recycled when the last of them lets go.
-std::vector<hipEvent_t> g_launch_free;  // under g_pool_mu
-struct LaunchEv {
-    hipEvent_t e = nullptr;
-    ~LaunchEv() {
-        if (!e) return;
-        std::lock_guard<std::mutex> lk(g_pool_mu);
-        g_launch_free.push_back(e);
-    }
-};
 
 PlaneGeom plane_geom(int w, int h, int bd) {
     PlaneGeom g;
@@ -294,16 +327,9 @@ template <typename T> size_t carve_size(size_t n) { return align_up(n * sizeof(T
 // Record the end of a launch for the sessions of `jobs`; the caller holds
 // g_mu and has just enqueued the launch on g_stream.
 int note_launch(const std::vector<rv_job*>& jobs) {
-    auto ev = std::make_shared<LaunchEv>();
-    {
-        std::lock_guard<std::mutex> lk(g_pool_mu);
-        if (!g_launch_free.empty()) {
-            ev->e = g_launch_free.back();
-            g_launch_free.pop_back();
-        }
-    }
-    if (!ev->e) RV_HIP(hipEventCreateWithFlags(&ev->e, hipEventDisableTiming));
-    RV_HIP(hipEventRecord(ev->e, g_stream));
+    hipError_t e = hipSuccess;
+    auto ev = note_stream_work(&e);
+    if (!ev) return hip_fail(e, "launch event");
     for (rv_job* j : jobs) j->s->last = ev;
     return RV_OK;
 }
@@ -1290,8 +1316,9 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
                     carve_size<int32_t>(b->pre_4.size());
-        char* hbase = t_stage.get(sz);
-        if (!hbase || !pool_get(sz, &b->tables)) {
+        std::vector<uint32_t> host(sz / 4 + 1, 0u);
+        char* hbase = (char*)host.data();
+        if (!pool_get(sz, &b->tables)) {
             delete b;
             return fail(RV_E_NOMEM, "batch tables");
         }
@@ -1312,14 +1339,25 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);
-        hipError_t e = stage_upload(b->tables.p, sz, nullptr, 0);
-        if (e != hipSuccess) {
-            {
-                std::lock_guard<std::mutex> lk(g_mu);
-                pool_put(b->tables);
+        // Written on the library stream, ahead of the batch's launches.
+        hipError_t e = hipSuccess;
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            const int nw = (int)(sz / 4);
+            for (int w0 = 0; w0 < nw && e == hipSuccess; w0 += kFillWords) {
+                WordChunk c;
+                c.dst = (uint32_t*)b->tables.p + w0;
+                c.n = std::min(kFillWords, nw - w0);
+                std::memcpy(c.w, host.data() + w0, (size_t)c.n * 4);
+                hipLaunchKernelGGL(rv_fill_words, dim3(1), dim3(256), 0, g_stream, c);
+                e = hipGetLastError();
             }
+            if (e == hipSuccess) (void)note_stream_work(&e);
+            if (e != hipSuccess) pool_put(b->tables);
+        }
+        if (e != hipSuccess) {
             delete b;
-            return hip_fail(e, "batch upload");
+            return hip_fail(e, "batch tables");
         }
         *out = b;
         return RV_OK;

@@ -54,7 +54,7 @@ def gop_order(n: int = GOP):
     return out
 
 
-@dataclass
+@dataclass(eq=False)  # pictures are compared and hashed by identity
 class Pic:
     sess: "Sess"
     poc: int
@@ -80,6 +80,14 @@ class Pic:
     prior_writers: list = field(default_factory=list)
     prior_readers: list = field(default_factory=list)
     slot_successors: list = field(default_factory=list)  # pictures waiting on this one's use of a slot
+    ref_pics: list = field(default_factory=list)  # the references, as pictures
+    # The loop's counters: pictures this one still waits for (to launch, or
+    # to complete), references not yet complete, and who waits for this one.
+    pending: int = 0
+    refs_pending: int = 0
+    started: bool = False  # launched, or (the I picture) resident
+    on_start: list = field(default_factory=list)
+    on_done: list = field(default_factory=list)
 
 
 @dataclass
@@ -151,6 +159,7 @@ def build_sessions(n: int = 64, seed: int = cf.BASE_SEED, rate: float = 0.0, gop
             s.pics.append(p)
         by_poc = {q.poc: q for q in s.pics}
         for i, q in enumerate(s.pics):
+            q.ref_pics = [by_poc[r] for r in q.refs]
             for r in q.refs:
                 by_poc[r].dependents.append(q)
             for e in s.pics[:i]:
@@ -253,9 +262,22 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         s.footprint = footprint(s)
         s.remaining = len(s.pics) - 1
         for p in s.pics:
-            p.launched = p.done = p.queued = False
+            p.launched = p.done = p.queued = p.started = False
             p.ready_t = p.done_t = math.nan
             p.job = p.fut = None
+            p.on_start, p.on_done = [], []
+    # What each picture waits for: its references and the earlier users of
+    # its slot must have launched (references and earlier writers must have
+    # completed when recording). Counted once here, then decremented as
+    # pictures launch and complete.
+    for s in sessions:
+        for q in s.pics:
+            need_done = set(q.ref_pics) | set(q.prior_writers) if record else set()
+            waits = list(dict.fromkeys(q.ref_pics + q.prior_writers + q.prior_readers))
+            q.pending = len(waits)
+            q.refs_pending = len(q.ref_pics)
+            for e in waits:
+                (e.on_done if e in need_done else e.on_start).append(q)
 
     def now():
         return time.perf_counter() - t0
@@ -273,28 +295,29 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         reference, launched in an earlier group, so the stream runs it
         first. When recording, earlier writers must also have completed,
         since their download happens at completion."""
-        if q.job is None or q.launched or q.queued:
-            return False
-        refs = [_pic_by_poc(q.sess, r) for r in q.refs]
-        if not all(r.done or (r.launched and not record) for r in refs):
-            return False
-        if not all(e.done or (e.launched and not record) for e in q.prior_writers):
-            return False
-        return all(e.done or e.launched for e in q.prior_readers)
+        return q.job is not None and q.pending == 0 and not q.launched and not q.queued
 
     def push(q: Pic, t: float):
         q.queued = True
         heapq.heappush(launchable, (t, next(seq), q))
 
-    def make_ready(p: Pic, t: float):
-        p.ready_t = t
-        if eligible(p):
-            push(p, t)
+    def release_waiters(waiters, t: float):
+        for q in waiters:
+            q.pending -= 1
+            if eligible(q):
+                push(q, t)
+
+    def start(p: Pic, t: float):
+        if not p.started:
+            p.started = True
+            release_waiters(p.on_start, t)
 
     def complete(p: Pic, t: float):
         nonlocal done, used
         p.done = True
         p.done_t = t
+        start(p, t)
+        release_waiters(p.on_done, t)
         s = p.sess
         if p.kind != "I":
             done += 1
@@ -305,11 +328,9 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             p.job = None
             s.remaining -= 1
         for q in p.dependents:
-            if math.isnan(q.ready_t) and all(_pic_by_poc(s, r).done for r in q.refs):
-                make_ready(q, t)
-        for q in p.slot_successors:
-            if eligible(q):
-                push(q, t)
+            q.refs_pending -= 1
+            if q.refs_pending == 0:
+                q.ready_t = t
         if s.remaining == 0 and id(s) in active:
             s.session.close()
             s.session = None
@@ -408,9 +429,7 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
             # launch next: the stream orders them.
             tl = now()
             for p in group:
-                for q in itertools.chain(p.dependents, p.slot_successors):
-                    if eligible(q):
-                        push(q, tl)
+                start(p, tl)
             launched = True
         release(fin)
         if launched or fin:
@@ -437,13 +456,6 @@ def _run(sessions, mode, budget, max_batch, inflight_max, record, ex, preload=Fa
         "batches": batches,
         "mean_batch": round(float(np.mean(batch_sizes)), 2) if batch_sizes else 0.0,
     }
-
-
-def _pic_by_poc(s: Sess, poc: int) -> Pic:
-    for p in s.pics:
-        if p.poc == poc:
-            return p
-    raise KeyError(poc)
 
 
 def experiment(n: int = 64, seed: int = cf.BASE_SEED, budget: int = 16 << 30, load: float = 0.8, threads: int = 16,
