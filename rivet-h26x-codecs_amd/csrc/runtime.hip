@@ -1091,9 +1091,14 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
         j->single.h264 = nullptr;
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
-        // The per-4x4 maps start zeroed (the planes are written in full).
-        const size_t maps = carve_size<uint8_t>(n4) * 3 + carve_size<int8_t>(n4) + carve_size<uint32_t>(n4);
-        hipError_t e = stage_upload(j->arena.p, sz, d.cuinfo, maps);
+        // The per-4x4 maps need no clearing: for work lists whose CUs, PUs
+        // and luma TUs tile the picture (checked by area above) stage 0
+        // writes every cell and stage 1 every strength, and a cell left
+        // unwritten by other lists can only give wrong strengths, never an
+        // access outside the job (the PU index is bounds-checked, QP-indexed
+        // tables clamp). Clearing them here would queue a fill kernel behind
+        // the running stage kernels and block this call until it ran.
+        hipError_t e = stage_upload(j->arena.p, sz, nullptr, 0);
         if (e != hipSuccess) {
             job_release(j);
             delete j;
@@ -1256,7 +1261,10 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->single.hevc = nullptr;
         j->single.h264 = (const rv::H264PicDev* const*)dev((char*)h_ptr);
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
-        hipError_t e = stage_upload(j->arena.p, sz, j->scratch.p, scr);
+        // No clearing of the scratch (see rv_hevc_prepare): stage 0 writes
+        // the partition map, stage 2 the edge words and, every launch, the
+        // exchange; partition indices are bounds-checked where read.
+        hipError_t e = stage_upload(j->arena.p, sz, nullptr, 0);
         if (e != hipSuccess) {
             job_release(j);
             delete j;
