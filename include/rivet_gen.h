@@ -6,7 +6,8 @@
  * src/dsp/u16_sweep.rs:39-48). This generator uses the same LCG to build
  * whole pictures of the shapes the benchmark configurations name: a CU
  * quadtree with symmetric / AMP partitions, quarter-sample vectors, a TU
- * quadtree with sparse Laplacian coefficients, per-CU QP, per-CTB SAO
+ * quadtree with sparse Laplacian coefficients (scaled values, or levels
+ * with qP and optional scaling lists), per-CU QP, per-CTB SAO
  * (HEVC); macroblocks with 16x16..4x4 partitions, 4x4 / 8x8 transforms and
  * intra macroblocks (H.264). The generated descriptions are what a parser
  * would hand the device: they feed the product path and the oracle alike.
@@ -36,6 +37,9 @@ typedef struct rv_gen_hevc_cfg {
     int32_t poc;
     int32_t ref_poc[2];     /* POC of the list-0 / list-1 reference        */
     int32_t ref_slot[2];    /* DPB slot of the list-0 / list-1 reference   */
+    int32_t levels;         /* 1: coded TUs carry levels (RV_TU_LEVELS) at
+                               the CU's qP (chroma through Table 8-10)     */
+    int32_t scaling;        /* 1: seeded scaling lists (factors 4..64)     */
 } rv_gen_hevc_cfg;
 
 typedef struct rv_gen_h264_cfg {
@@ -52,6 +56,11 @@ typedef struct rv_gen_h264_cfg {
     int32_t poc;
     int32_t ref_poc[2];
     int32_t ref_slot[2];
+    int32_t levels;         /* 1: blocks carry levels (RV_BLK_LEVELS): chroma
+                               DC groups, and intra macroblocks without 8x8
+                               transforms Intra_16x16 (luma DC group) half
+                               the time                                     */
+    int32_t scaling;        /* 1: seeded scaling lists (factors 4..64)     */
 } rv_gen_h264_cfg;
 
 typedef struct rv_gen rv_gen;

@@ -6,7 +6,8 @@
  *  (1) Work-list formats and the batched picture API. A host decoder (the
  *      parser, which stays on the CPU) describes one inter picture as
  *      host-packed lists — coding units, prediction units, transform units
- *      with their dequantised coefficients, SAO parameters — and the
+ *      with their coefficients (scaled, or levels that the device scales),
+ *      SAO parameters — and the
  *      library reconstructs it on the device: motion compensation, inverse
  *      transform + residual add, deblocking, SAO. Reference pictures stay
  *      resident in device memory (a per-session decoded-picture buffer).
@@ -107,11 +108,12 @@ typedef struct rv_hevc_pu {
     rv_mv mv[2];        /* quarter-sample vectors per list             */
 } rv_hevc_pu;
 
-/* One nonzero scaled transform coefficient: pos = y * n + x in its n x n
- * TU. A coded TU's coefficients are the run nz[coef_off .. coef_off + n_nz)
+/* One nonzero transform coefficient: pos = y * n + x in its n x n TU. A
+ * coded TU's coefficients are the run nz[coef_off .. coef_off + n_nz)
  * (positions distinct); every other coefficient of the TU is zero — the
  * dense n x n raster of IdctFn (src/dsp/hevc.rs:18-21) is these values
- * scattered into zeros. */
+ * scattered into zeros. val is the scaled coefficient d[x][y], or with
+ * RV_TU_LEVELS the level TransCoeffLevel that stage 1 scales. */
 typedef struct rv_nz {
     uint16_t pos;
     int16_t val;
@@ -124,16 +126,40 @@ typedef struct rv_hevc_tu {
     uint16_t x, y;      /* top-left in component samples              */
     uint8_t comp;       /* 0 Y, 1 Cb, 2 Cr                             */
     uint8_t log2;       /* 2..5                                        */
-    uint8_t flags;      /* RV_TU_DST | RV_TU_CODED                     */
+    uint8_t flags;      /* RV_TU_DST | RV_TU_CODED | RV_TU_LEVELS |
+                           RV_TU_INTRA                                 */
     uint8_t max_x;      /* nonzero bounds, inclusive (IdctFn,          */
     uint8_t max_y;      /*   src/dsp/hevc.rs:18-21)                    */
-    uint8_t pad;
+    uint8_t qp;         /* RV_TU_LEVELS: qP of the scaling (Qp'Y, Qp'Cb
+                           or Qp'Cr, QpBdOffset included: 0 ..
+                           51 + 6 * (bit_depth - 8)); else ignored     */
     uint16_t n_nz;      /* coded: number of nonzero coefficients       */
     uint32_t coef_off;  /* coded: first entry in nz                    */
 } rv_hevc_tu;
 
 #define RV_TU_DST 1u   /* 4x4 DST (intra luma 4x4, src/hevc/ctu.rs:2041) */
 #define RV_TU_CODED 2u /* cbf: coefficients present                      */
+/* The TU's nz values are levels: stage 1 scales them (scale_coefficients,
+ * src/hevc/residual.rs:1223-1283, as called at src/hevc/ctu.rs:2016-2031)
+ * with qp and the picture's scaling lists before the inverse transform.
+ * Transform-skip, RDPCM and transquant-bypass TUs have no place in these
+ * lists. */
+#define RV_TU_LEVELS 4u
+/* RV_TU_LEVELS of an intra CU: scaling matrices 0..2 (matrixId = cIdx),
+ * else 3..5 (3 + cIdx), src/hevc/ctu.rs:2021. */
+#define RV_TU_INTRA 8u
+
+/* Resolved scaling factors of a picture (the lists the reference's
+ * ScalingSource::List carries, src/hevc/residual.rs:1209-1215, chosen at
+ * src/hevc/ctu.rs:2016-2029): default, predicted and explicit lists already
+ * resolved by the host. Raster order (y * size + x); the 16x16 and 32x32
+ * factors are the 8x8 grid, each entry covering a 2x2 / 4x4 square,
+ * except (0, 0), which is dc. */
+typedef struct rv_hevc_scaling {
+    uint8_t list4[6][16];     /* sizeId 0, matrixId 0..5               */
+    uint8_t list8[3][6][64];  /* sizeId 1..3 (8x8, 16x16, 32x32)       */
+    uint8_t dc[2][6];         /* sizeId 2..3: scaling_list_dc_coef     */
+} rv_hevc_scaling;
 
 /* One HEVC P/B picture to reconstruct. Planes are the session's DPB
  * slots; ref_slot names them per list and reference index. */
@@ -167,6 +193,9 @@ typedef struct rv_hevc_pic {
     uint32_t n_intra_pred;
     const uint16_t* intra_pred;  /* samples, any depth, as uint16       */
     const rv_sao* sao;           /* 3 per CTB (Y, Cb, Cr), raster CTBs   */
+    /* Scaling lists of RV_TU_LEVELS TUs; NULL = flat 16 (scaling lists
+     * off, ScalingSource::Flat). */
+    const rv_hevc_scaling* scaling;
 } rv_hevc_pic;
 
 /* ------------------------------------------------------------------ */
@@ -205,11 +234,34 @@ typedef struct rv_h264_blk {
     uint16_t x, y;      /* top-left in component samples              */
     uint8_t comp;       /* 0 Y, 1 Cb, 2 Cr                            */
     uint8_t size;       /* 4 or 8                                     */
-    uint8_t has_dc;     /* 1: dc replaces position 0 (chroma DC)      */
-    uint8_t pad;
-    int32_t dc;
+    uint8_t has_dc;     /* 1: dc replaces position 0 (chroma DC, or
+                           the luma DC of an Intra_16x16 macroblock)   */
+    uint8_t flags;      /* RV_BLK_LEVELS                              */
+    int32_t dc;         /* has_dc: the scaled DC; with RV_BLK_LEVELS
+                           the offset in coefs of the DC levels of the
+                           block's group (below)                      */
     uint32_t coef_off;  /* offset of size*size int32 in coefs         */
 } rv_h264_blk;
+
+/* The block's coefs are levels, raster order: stage 1 scales them as the
+ * reference's parser does (dequant_level with the MbDequant tables,
+ * src/h264/mb.rs:1363-1428; LevelScale from Dequant::new,
+ * src/h264/transform.rs:24-56), with the macroblock's QP_Y (luma) or
+ * QP_C (chroma), its intra / inter lists and the picture's scaling lists.
+ * With has_dc, position 0 is not read: dc is the offset of the group's DC
+ * levels, raster — 4 for a chroma component of the macroblock
+ * (chroma_dc_transform_420, src/h264/transform.rs:216-228), 16 for the
+ * luma of an Intra_16x16 macroblock (luma_dc_transform, :179-213), as
+ * called at src/h264/recon.rs:700-716 and 1200-1226 — and the block takes
+ * the transformed, scaled DC of its own position in the group. */
+#define RV_BLK_LEVELS 1u
+
+/* Scaling lists of a picture (ScalingLists, src/h264/sps.rs), resolved by
+ * the host (fall-back rules applied), raster order. */
+typedef struct rv_h264_scaling {
+    uint8_t list4[6][16];   /* Y, Cb, Cr intra; Y, Cb, Cr inter         */
+    uint8_t list8[2][64];   /* Y intra, Y inter                         */
+} rv_h264_scaling;
 
 #define RV_H264_DEFAULT 0
 #define RV_H264_EXPLICIT 1
@@ -237,6 +289,9 @@ typedef struct rv_h264_pic {
     uint32_t n_coefs;
     uint32_t n_intra_pred;
     const uint16_t* intra_pred;
+    /* Scaling lists of RV_BLK_LEVELS blocks; NULL = flat 16
+     * (Flat_4x4_16 / Flat_8x8_16). */
+    const rv_h264_scaling* scaling;
 } rv_h264_pic;
 
 /* ------------------------------------------------------------------ */
@@ -552,8 +607,8 @@ int rv_h264_u8_residual8(uint8_t* dst, size_t dst_len, size_t stride, const int3
 
 /* sizeof of the work-list structs, for bindings to check their layout:
  * 0 rv_hevc_cu, 1 rv_hevc_pu, 2 rv_hevc_tu, 3 rv_sao, 4 rv_hevc_pic,
- * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic, 9 rv_nz;
- * -1 for an unknown index. */
+ * 5 rv_h264_mb, 6 rv_h264_part, 7 rv_h264_blk, 8 rv_h264_pic, 9 rv_nz,
+ * 10 rv_hevc_scaling, 11 rv_h264_scaling; -1 for an unknown index. */
 int rv_abi_size(int which);
 
 #ifdef __cplusplus

@@ -36,6 +36,7 @@ u8p = C.POINTER(C.c_uint8)
 _sig = {
     "orc_hevc_idct": (None, [i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_hevc_idst4": (None, [i16p, C.c_int]),
+    "orc_hevc_scale_coefficients": (None, [i16p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int]),
     "orc_hevc_idct_matrix": (None, [i16p, C.c_int, C.c_int, i16p]),
     "orc_hevc_add_residual": (None, [C.c_int, P, C.c_int, i16p, C.c_int, C.c_int]),
     "orc_hevc_copy": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -75,6 +76,11 @@ _sig = {
     "orc_h264_deblock": (None, [C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int, C.c_int, i16p, C.c_int]),
     "orc_h264_residual4": (None, [u8p, C.c_int, i32p, C.c_int, C.c_int, C.c_int]),
     "orc_h264_residual8": (None, [u8p, C.c_int, i32p, C.c_int]),
+    "orc_h264_level_scale4": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_level_scale8": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "orc_h264_dequant_level": (C.c_int32, [C.c_int32, C.c_int32, C.c_int]),
+    "orc_h264_luma_dc": (None, [i32p, C.c_int, C.c_int]),
+    "orc_h264_chroma_dc": (None, [i32p, C.c_int, C.c_int]),
     "orc_h264_implicit_pair": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p]),
     "orc_h264_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
     "orc_crc16": (C.c_uint16, [P, C.c_size_t]),

@@ -312,6 +312,126 @@ void residual8(uint8_t* dst, long stride, const int32_t* coefs, int max) {
     else if (c[0] != 0) dc_add(dst, stride, c[0], 8, max);
 }
 
+// ---------------------------------------------------------------------
+// Scaling (8.5.9, 8.5.10, 8.5.11, 8.5.12.1, 8.5.13.1)
+// ---------------------------------------------------------------------
+
+// normAdjust4x4(m, i, j) of 8.5.9: v[m][0] where i and j are both even,
+// v[m][1] where both are odd, v[m][2] otherwise (Dequant::new,
+// src/h264/transform.rs:33-41, over src/h264/tables.rs's DEQUANT4_INIT,
+// which orders the same values by the number of odd indices).
+const int NORM4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+// normAdjust8x8(m, i, j): v[m][0] i%4 == 0 && j%4 == 0; v[m][1] i, j odd;
+// v[m][2] i%4 == 2 && j%4 == 2; v[m][3] one index a multiple of 4, the other
+// odd; v[m][4] one a multiple of 4, the other 2 mod 4; v[m][5] otherwise
+// (transform.rs:42-48, via DEQUANT8_INIT_SCAN).
+const int NORM8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
+                         {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
+
+int norm4(int m, int i, int j) {
+    if (i % 2 == 0 && j % 2 == 0) return NORM4[m][0];
+    if (i % 2 == 1 && j % 2 == 1) return NORM4[m][1];
+    return NORM4[m][2];
+}
+int norm8(int m, int i, int j) {
+    if (i % 4 == 0 && j % 4 == 0) return NORM8[m][0];
+    if (i % 2 == 1 && j % 2 == 1) return NORM8[m][1];
+    if (i % 4 == 2 && j % 4 == 2) return NORM8[m][2];
+    if ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) return NORM8[m][3];
+    if ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) return NORM8[m][4];
+    return NORM8[m][5];
+}
+
+// LevelScale4x4(m, i, j) = weightScale4x4(i, j) * normAdjust4x4(m, i, j) at
+// raster pos = i * 4 + j (Dequant::new, transform.rs:24-56); flat lists are
+// 16 everywhere.
+int level_scale4(const rv_h264_scaling* sl, int list, int m, int pos) {
+    return (sl ? sl->list4[list][pos] : 16) * norm4(m, pos / 4, pos % 4);
+}
+int level_scale8(const rv_h264_scaling* sl, int list, int m, int pos) {
+    return (sl ? sl->list8[list][pos] : 16) * norm8(m, pos / 8, pos % 8);
+}
+
+// dequant_level (src/h264/mb.rs:1420-1428): (level * table << shift + 32)
+// >> 6 in wrapping i32, shift = qP / 6 + 2 for 4x4 blocks and qP / 6 for
+// 8x8 (MbDequant::for_mb, :1404-1413) — 8.5.12.1 / 8.5.13.1 folded.
+int32_t dequant_level(int32_t level, int32_t table, int shift) {
+    const uint32_t t = ((uint32_t)level * (uint32_t)table) << shift;
+    return (int32_t)(t + 32u) >> 6;
+}
+
+// luma_dc_transform (transform.rs:179-213): the Intra_16x16 DC levels
+// (raster 4x4) through the inverse Hadamard, rows then columns, then
+// scaled at qP with LevelScale4x4(qP % 6, 0, 0).
+void luma_dc_transform(int32_t* dc, int scale00, int qp) {
+    int32_t t[16], f[16];
+    for (int i = 0; i < 4; ++i) {
+        const int32_t c0 = dc[i * 4], c1 = dc[i * 4 + 1], c2 = dc[i * 4 + 2], c3 = dc[i * 4 + 3];
+        const int32_t e0 = c0 + c1, e1 = c0 - c1, e2 = c2 - c3, e3 = c2 + c3;
+        t[i * 4] = e0 + e3;
+        t[i * 4 + 1] = e0 - e3;
+        t[i * 4 + 2] = e1 - e2;
+        t[i * 4 + 3] = e1 + e2;
+    }
+    for (int j = 0; j < 4; ++j) {
+        const int32_t c0 = t[j], c1 = t[4 + j], c2 = t[8 + j], c3 = t[12 + j];
+        const int32_t e0 = c0 + c1, e1 = c0 - c1, e2 = c2 - c3, e3 = c2 + c3;
+        f[j] = e0 + e3;
+        f[4 + j] = e0 - e3;
+        f[8 + j] = e1 - e2;
+        f[12 + j] = e1 + e2;
+    }
+    const int q6 = qp / 6;
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t p = (uint32_t)f[i] * (uint32_t)scale00;
+        dc[i] = qp >= 36 ? (int32_t)(p << (q6 - 6)) : (int32_t)(p + (1u << (5 - q6))) >> (6 - q6);
+    }
+}
+
+// chroma_dc_transform_420 (transform.rs:216-228): the 2x2 Hadamard, then
+// ((f * LevelScale4x4(qP % 6, 0, 0)) << (qP / 6)) >> 5.
+void chroma_dc_transform_420(int32_t* dc, int scale00, int qp) {
+    const int32_t c0 = dc[0], c1 = dc[1], c2 = dc[2], c3 = dc[3];
+    const int32_t f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
+    for (int i = 0; i < 4; ++i) dc[i] = (int32_t)(((uint32_t)f[i] * (uint32_t)scale00) << (qp / 6)) >> 5;
+}
+
+// The coefficients and DC of a block of levels as the reference's parser
+// and recon hand them to residual4 / residual8 (src/h264/cavlc.rs:300-310,
+// src/h264/recon.rs:700-716, 1200-1226): AC levels through dequant_level
+// with the macroblock's tables; with has_dc, the group's DC levels through
+// the DC transform and this block's entry taken. Returns false for a QP
+// outside 0..51 or a DC group outside coefs.
+bool scale_block(const rv_h264_pic* pic, const rv_h264_blk& bk, int32_t* out, int* dc_out) {
+    const int mbw = pic->width / 16;
+    const bool chroma = bk.comp != 0;
+    const int mbx = chroma ? bk.x / 8 : bk.x / 16, mby = chroma ? bk.y / 8 : bk.y / 16;
+    const rv_h264_mb& m = pic->mbs[mby * mbw + mbx];
+    const int qp = chroma ? m.qpc[bk.comp - 1] : m.qp;
+    if (qp < 0 || qp > 51) return false;
+    const bool inter = !m.intra;
+    const int n = bk.size * bk.size;
+    const int32_t* lv = pic->coefs + bk.coef_off;
+    for (int i = 0; i < n; ++i)
+        out[i] = bk.size == 8 ? dequant_level(lv[i], level_scale8(pic->scaling, inter ? 1 : 0, qp % 6, i), qp / 6)
+                              : dequant_level(lv[i], level_scale4(pic->scaling, (inter ? 3 : 0) + bk.comp, qp % 6, i),
+                                              qp / 6 + 2);
+    *dc_out = 0;
+    if (!bk.has_dc) return true;
+    const int group = chroma ? 4 : 16;
+    if (bk.dc < 0 || (uint64_t)bk.dc + group > pic->n_coefs) return false;
+    int32_t g[16];
+    std::memcpy(g, pic->coefs + bk.dc, group * sizeof(int32_t));
+    if (chroma) {
+        chroma_dc_transform_420(g, level_scale4(pic->scaling, (inter ? 3 : 0) + bk.comp, qp % 6, 0), qp);
+        *dc_out = g[((bk.y % 8) / 4) * 2 + (bk.x % 8) / 4];
+    } else {
+        luma_dc_transform(g, level_scale4(pic->scaling, 0, qp % 6, 0), qp);
+        *dc_out = g[((bk.y % 16) / 4) * 4 + (bk.x % 16) / 4];
+    }
+    return true;
+}
+
 // implicit_pair (src/h264/recon.rs:214-233).
 void implicit_pair(int cur_poc, int poc0, int poc1, bool long0, bool long1, int* w0, int* w1) {
     int tb = clampi(cur_poc - poc0, -128, 127);
@@ -638,6 +758,15 @@ void orc_h264_deblock(int chroma_k, int vertical, uint8_t* data, int off, int st
 void orc_h264_residual4(uint8_t* dst, int stride, const int32_t* coefs, int dc, int has_dc, int max) {
     residual4(dst, stride, coefs, dc, has_dc != 0, max);
 }
+int orc_h264_level_scale4(const rv_h264_scaling* sl, int list, int m, int pos) {
+    return level_scale4(sl, list, m, pos);
+}
+int orc_h264_level_scale8(const rv_h264_scaling* sl, int list, int m, int pos) {
+    return level_scale8(sl, list, m, pos);
+}
+int32_t orc_h264_dequant_level(int32_t level, int32_t table, int shift) { return dequant_level(level, table, shift); }
+void orc_h264_luma_dc(int32_t* dc, int scale00, int qp) { luma_dc_transform(dc, scale00, qp); }
+void orc_h264_chroma_dc(int32_t* dc, int scale00, int qp) { chroma_dc_transform_420(dc, scale00, qp); }
 void orc_h264_residual8(uint8_t* dst, int stride, const int32_t* coefs, int max) {
     residual8(dst, stride, coefs, max);
 }
@@ -777,8 +906,17 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
             if (bk.coef_off + (uint64_t)n * n > pic->n_coefs) return RV_E_RANGE;
             Plane& pl = cur.p[bk.comp];
             uint8_t* dst = &pl.data[pl.offset(bk.x, bk.y)];
-            if (n == 8) residual8(dst, pl.stride, pic->coefs + bk.coef_off, max);
-            else residual4(dst, pl.stride, pic->coefs + bk.coef_off, bk.dc, bk.has_dc != 0, max);
+            if (bk.flags & RV_BLK_LEVELS) {
+                int32_t scaled[64];
+                int dc;
+                if (!scale_block(pic, bk, scaled, &dc)) return RV_E_RANGE;
+                if (n == 8) residual8(dst, pl.stride, scaled, max);
+                else residual4(dst, pl.stride, scaled, dc, bk.has_dc != 0, max);
+            } else if (n == 8) {
+                residual8(dst, pl.stride, pic->coefs + bk.coef_off, max);
+            } else {
+                residual4(dst, pl.stride, pic->coefs + bk.coef_off, bk.dc, bk.has_dc != 0, max);
+            }
         }
     }
     // Deblocking (deblock_mb_rows, src/h264/deblock.rs:567-815), frame

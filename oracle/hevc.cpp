@@ -75,6 +75,49 @@ const Transform32 T32;
 inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 inline int sgn(int v) { return (v > 0) - (v < 0); }
 
+// scale_coefficients (src/hevc/residual.rs:1223-1283), 8.6.3: within the
+// nonzero bounds, d = Clip3(-32768, 32767, ((c * m * levelScale[qP % 6] <<
+// (qP / 6)) + (1 << (bdShift - 1))) >> bdShift), bdShift = BitDepth +
+// Log2(nTbS) - 5. m is 16 for flat scaling (list == nullptr), else the
+// list's factor: list[y * 4 + x] (4x4), list[y * 8 + x] (8x8), dc at (0, 0)
+// and list[(y / r) * 8 + x / r], r = n / 8, for 16x16 / 32x32. The
+// reference's flat fast path (:1243-1254) computes the same values in i32.
+// Transform skip (flat above 4x4) has no place in the work lists.
+void scale_coefficients(int16_t* coeffs, int log2, int qp, int bd, const uint8_t* list, int dc, int max_x,
+                        int max_y) {
+    static const int LEVEL_SCALE[6] = {40, 45, 51, 57, 64, 72};
+    const int n = 1 << log2;
+    const int bd_shift = bd + log2 - 5;
+    const int64_t round = (int64_t)1 << (bd_shift - 1);
+    const int64_t ls = LEVEL_SCALE[qp % 6];
+    const int q6 = qp / 6;
+    for (int y = 0; y <= max_y; ++y)
+        for (int x = 0; x <= max_x; ++x) {
+            const int c = coeffs[y * n + x];
+            if (c == 0) continue;
+            int64_t m = 16;
+            if (list) {
+                if (n == 4) m = list[y * 4 + x];
+                else if (n == 8) m = list[y * 8 + x];
+                else if (x == 0 && y == 0) m = dc;
+                else m = list[(y / (n / 8)) * 8 + x / (n / 8)];
+            }
+            const int64_t v = (((int64_t)c * m * ls) << q6) + round;
+            const int64_t d = v >> bd_shift;
+            coeffs[y * n + x] = (int16_t)(d < -32768 ? -32768 : (d > 32767 ? 32767 : d));
+        }
+}
+
+// The list and DC of a TU (src/hevc/ctu.rs:2016-2029): sizeId = log2 - 2,
+// matrixId = cIdx for intra CUs, 3 + cIdx otherwise; DC 16 below 16x16.
+const uint8_t* tu_scaling(const rv_hevc_scaling* sl, int log2, int comp, bool intra, int* dc) {
+    *dc = 16;
+    if (!sl) return nullptr;
+    const int mid = (intra ? 0 : 3) + comp;
+    if (log2 >= 4) *dc = sl->dc[log2 - 4][mid];
+    return log2 == 2 ? sl->list4[mid] : sl->list8[log2 - 3][mid];
+}
+
 template <typename S> inline int rd(const S* p, long i) { return (int)p[i]; }
 
 // ---------------------------------------------------------------------
@@ -1033,6 +1076,7 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
             if (tu.coef_off + (uint64_t)tu.n_nz > pic->n_nz) return RV_E_RANGE;
             for (uint32_t k = 0; k < tu.n_nz; ++k)
                 if (pic->nz[tu.coef_off + k].pos >= n * n) return RV_E_RANGE;
+            if ((tu.flags & RV_TU_LEVELS) && tu.qp > 51 + 6 * (bd - 8)) return RV_E_RANGE;
         }
         parallel_for(pic->n_tus, threads, [&](int i0, int i1) {
             std::vector<int16_t> coeffs(32 * 32);
@@ -1046,6 +1090,11 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
                 for (uint32_t k = 0; k < tu.n_nz; ++k) {
                     const rv_nz& e = pic->nz[tu.coef_off + k];
                     coeffs[e.pos] = e.val;
+                }
+                if (tu.flags & RV_TU_LEVELS) {
+                    int dc;
+                    const uint8_t* list = tu_scaling(pic->scaling, tu.log2, tu.comp, (tu.flags & RV_TU_INTRA) != 0, &dc);
+                    scale_coefficients(coeffs.data(), tu.log2, tu.qp, bd, list, dc, tu.max_x, tu.max_y);
                 }
                 int bd_shift = 20 - bd;
                 if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
@@ -1077,6 +1126,10 @@ void orc_hevc_idct(int16_t* coeffs, int log2, int bd_shift, int max_x, int max_y
     idct(coeffs, 1 << log2, bd_shift, max_x, max_y);
 }
 void orc_hevc_idst4(int16_t* coeffs, int bd_shift) { idst4(coeffs, bd_shift); }
+void orc_hevc_scale_coefficients(int16_t* coeffs, int log2, int qp, int bit_depth, const uint8_t* list, int dc,
+                                 int max_x, int max_y) {
+    scale_coefficients(coeffs, log2, qp, bit_depth, list, dc, max_x, max_y);
+}
 
 void orc_hevc_idct_matrix(const int16_t* c, int n, int bd_shift, int16_t* out) {
     int step = 32 / n;

@@ -25,6 +25,11 @@ extern "C" {
 /* ---- HEVC dsp entries (HevcDsp<S>, src/dsp/hevc.rs:165-229) ---------- */
 void orc_hevc_idct(int16_t* coeffs, int log2, int bd_shift, int max_x, int max_y);
 void orc_hevc_idst4(int16_t* coeffs, int bd_shift);
+/* scale_coefficients (src/hevc/residual.rs:1223-1283) of one n x n raster
+ * (n = 1 << log2) in place; list = NULL for flat scaling, else the TU's
+ * list (16 or 64 raster entries) and, for 16x16 / 32x32, its dc. */
+void orc_hevc_scale_coefficients(int16_t* coeffs, int log2, int qp, int bit_depth, const uint8_t* list, int dc,
+                                 int max_x, int max_y);
 /* The plain matrix product with the two clips (src/dsp/hevc.rs:1417-1442),
  * an independent statement the butterfly must equal. */
 void orc_hevc_idct_matrix(const int16_t* coeffs, int n, int bd_shift, int16_t* out);
@@ -99,6 +104,16 @@ void orc_h264_deblock(int chroma, int vertical, uint8_t* data, int off, int stri
 void orc_h264_residual4(uint8_t* dst, int stride, const int32_t* coefs, int dc, int has_dc,
                         int max);
 void orc_h264_residual8(uint8_t* dst, int stride, const int32_t* coefs, int max);
+/* LevelScale4x4 / LevelScale8x8 (Dequant::new, src/h264/transform.rs:24-56)
+ * of list `list` at qP % 6 = m, raster pos; sl = NULL: flat 16. */
+int orc_h264_level_scale4(const rv_h264_scaling* sl, int list, int m, int pos);
+int orc_h264_level_scale8(const rv_h264_scaling* sl, int list, int m, int pos);
+/* dequant_level (src/h264/mb.rs:1420-1428). */
+int32_t orc_h264_dequant_level(int32_t level, int32_t table, int shift);
+/* luma_dc_transform / chroma_dc_transform_420 (src/h264/transform.rs:
+ * 179-228), in place on 16 / 4 raster DC levels. */
+void orc_h264_luma_dc(int32_t* dc, int scale00, int qp);
+void orc_h264_chroma_dc(int32_t* dc, int scale00, int qp);
 void orc_h264_implicit_pair(int cur_poc, int poc0, int poc1, int long0, int long1, int32_t out[2]);
 
 int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots,

@@ -50,7 +50,20 @@ struct Lcg {
     }
 };
 
-enum Purpose : uint64_t { STRUCT = 1, COEF = 2, SAO = 3, INTRA = 4, WEIGHTS = 5 };
+enum Purpose : uint64_t { STRUCT = 1, COEF = 2, SAO = 3, INTRA = 4, WEIGHTS = 5, SCALING = 6 };
+
+// Seeded scaling factors, 4..64 each.
+void fill_factors(Lcg& r, uint8_t* p, size_t n) {
+    for (size_t i = 0; i < n; ++i) p[i] = (uint8_t)r.uniform(4, 64);
+}
+
+// QpC as a function of qPi for 4:2:0 (Table 8-10 of H.265).
+int hevc_chroma_qp(int qpi) {
+    static const int T[14] = {29, 30, 31, 32, 33, 33, 34, 34, 35, 35, 36, 36, 37, 37};
+    if (qpi < 30) return qpi;
+    if (qpi > 43) return qpi - 6;
+    return T[qpi - 30];
+}
 
 }  // namespace
 
@@ -68,6 +81,8 @@ struct rv_gen {
     std::vector<rv_h264_part> parts;
     std::vector<rv_h264_blk> blks;
     std::vector<int32_t> coefs32;
+    rv_hevc_scaling hevc_scaling;
+    rv_h264_scaling h264_scaling;
 };
 
 namespace {
@@ -104,7 +119,7 @@ struct HevcBuilder {
         return cnt != 0;
     }
 
-    void add_tu(int comp, int x, int y, int log2, bool dst, bool always) {
+    void add_tu(int comp, int x, int y, int log2, bool dst, bool always, bool intra, int qp_y) {
         rv_hevc_tu tu;
         std::memset(&tu, 0, sizeof(tu));
         tu.x = (uint16_t)x;
@@ -113,6 +128,14 @@ struct HevcBuilder {
         tu.log2 = (uint8_t)log2;
         bool coded = coefs(1 << log2, tu);
         tu.flags = (uint8_t)((dst ? RV_TU_DST : 0) | (coded ? RV_TU_CODED : 0));
+        if (coded && c.levels) {
+            // qP of the TU: Qp'Y, or Qp'C from qPi = Clip3(-QpBdOffsetC, 57,
+            // QpY + pps_cb_qp_offset) (the generated pictures' offsets are 0).
+            const int off = 6 * (c.bit_depth - 8);
+            const int qpi = qp_y < -off ? -off : (qp_y > 57 ? 57 : qp_y);
+            tu.qp = (uint8_t)((comp ? hevc_chroma_qp(qpi) : qp_y) + off);
+            tu.flags |= (uint8_t)(RV_TU_LEVELS | (intra ? RV_TU_INTRA : 0));
+        }
         if (coded || always) g.tus.push_back(tu);
     }
 
@@ -211,15 +234,15 @@ struct HevcBuilder {
         // Transform units: luma (all, coded or not), chroma (coded only).
         for (size_t i = 0; i < leaves.size(); i += 3) {
             int tx = leaves[i], ty = leaves[i + 1], l2 = leaves[i + 2];
-            add_tu(0, tx, ty, l2, intra && l2 == 2, true);
+            add_tu(0, tx, ty, l2, intra && l2 == 2, true, intra, cu.qp_y);
             if (l2 > 2) {
-                add_tu(1, tx / 2, ty / 2, l2 - 1, false, false);
-                add_tu(2, tx / 2, ty / 2, l2 - 1, false, false);
+                add_tu(1, tx / 2, ty / 2, l2 - 1, false, false, intra, cu.qp_y);
+                add_tu(2, tx / 2, ty / 2, l2 - 1, false, false, intra, cu.qp_y);
             } else if ((tx & 7) == 0 && (ty & 7) == 0) {
                 // 4:2:0: the four 4x4 luma TUs of an 8x8 share one 4x4
                 // chroma TU per component (carried with the first).
-                add_tu(1, tx / 2, ty / 2, 2, false, false);
-                add_tu(2, tx / 2, ty / 2, 2, false, false);
+                add_tu(1, tx / 2, ty / 2, 2, false, false, intra, cu.qp_y);
+                add_tu(2, tx / 2, ty / 2, 2, false, false, intra, cu.qp_y);
             }
         }
     }
@@ -282,6 +305,13 @@ struct HevcBuilder {
         p.intra_pred = g.intra_pred.data();
         p.n_intra_pred = (uint32_t)g.intra_pred.size();
         p.sao = g.sao.data();
+        if (c.scaling) {
+            Lcg rsl(rv_gen_subseed(seed, 0, 0, SCALING));
+            fill_factors(rsl, &g.hevc_scaling.list4[0][0], sizeof(g.hevc_scaling.list4));
+            fill_factors(rsl, &g.hevc_scaling.list8[0][0][0], sizeof(g.hevc_scaling.list8));
+            fill_factors(rsl, &g.hevc_scaling.dc[0][0], sizeof(g.hevc_scaling.dc));
+            p.scaling = &g.hevc_scaling;
+        }
     }
 };
 
@@ -292,13 +322,62 @@ const int CHROMA_QP[52] = {0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 1
 struct H264Builder {
     const rv_gen_h264_cfg& c;
     rv_gen& g;
-    Lcg rs, rc, ri;
+    Lcg rs, rc, ri, rsl;
     H264Builder(const rv_gen_h264_cfg& cfg, rv_gen& gen, uint64_t seed)
         : c(cfg), g(gen), rs(rv_gen_subseed(seed, 0, 0, STRUCT)), rc(rv_gen_subseed(seed, 0, 0, COEF)),
-          ri(rv_gen_subseed(seed, 0, 0, INTRA)) {}
+          ri(rv_gen_subseed(seed, 0, 0, INTRA)), rsl(rv_gen_subseed(seed, 0, 0, SCALING)) {}
+
+    int level() {
+        int m = rc.laplace(c.lap_b);
+        return (rc.next() & 1) ? -m : m;
+    }
+
+    // A DC group of n levels (levels mode): its offset in coefs; *any tells
+    // whether one is nonzero.
+    uint32_t dc_group(int n, bool* any) {
+        const uint32_t off = (uint32_t)g.coefs32.size();
+        *any = false;
+        for (int i = 0; i < n; ++i) {
+            int v = rc.permille(c.nz_permille) ? level() : 0;
+            g.coefs32.push_back(v);
+            *any |= v != 0;
+        }
+        return off;
+    }
+
+    // One block of levels whose position 0 comes from the DC group at
+    // group_off (has_dc) or is a level itself; kept when any level of the
+    // block or its group is nonzero.
+    bool level_block(int comp, int x, int y, int n, bool has_dc, uint32_t group_off, bool group_any) {
+        uint32_t off = (uint32_t)g.coefs32.size();
+        g.coefs32.resize(off + n * n, 0);
+        bool any = false;
+        for (int i = has_dc ? 1 : 0; i < n * n; ++i) {
+            if (!rc.permille(c.nz_permille)) continue;
+            g.coefs32[off + i] = level();
+            any = true;
+        }
+        if (!any && !(has_dc && group_any)) {
+            g.coefs32.resize(off);
+            return false;
+        }
+        rv_h264_blk b;
+        std::memset(&b, 0, sizeof(b));
+        b.x = (uint16_t)x;
+        b.y = (uint16_t)y;
+        b.comp = (uint8_t)comp;
+        b.size = (uint8_t)n;
+        b.has_dc = has_dc ? 1 : 0;
+        b.flags = RV_BLK_LEVELS;
+        b.dc = has_dc ? (int32_t)group_off : 0;
+        b.coef_off = off;
+        g.blks.push_back(b);
+        return any;
+    }
 
     // One coded block (or nothing); returns whether any coefficient is set.
     bool block(int comp, int x, int y, int n, bool chroma_dc) {
+        if (c.levels) return level_block(comp, x, y, n, false, 0, false);
         uint32_t off = (uint32_t)g.coefs32.size();
         g.coefs32.resize(off + n * n, 0);
         bool any = false;
@@ -410,13 +489,27 @@ struct H264Builder {
                         nz |= (uint16_t)((1u << (by * 4 + bx)) | (1u << (by * 4 + bx + 1)) |
                                          (1u << ((by + 1) * 4 + bx)) | (1u << ((by + 1) * 4 + bx + 1)));
                 }
+            } else if (c.levels && m.intra && rs.percent(50)) {
+                // Intra_16x16: a luma DC group and sixteen blocks taking it.
+                bool gany;
+                const uint32_t go = dc_group(16, &gany);
+                for (int r = 0; r < 16; ++r)
+                    if (level_block(0, mx + (r % 4) * 4, my + (r / 4) * 4, 4, true, go, gany)) nz |= (uint16_t)(1u << r);
             } else {
                 for (int r = 0; r < 16; ++r)
                     if (block(0, mx + (r % 4) * 4, my + (r / 4) * 4, 4, false)) nz |= (uint16_t)(1u << r);
             }
             m.nz_mask = nz;
-            for (int comp = 1; comp < 3; ++comp)
-                for (int b = 0; b < 4; ++b) block(comp, mx / 2 + (b & 1) * 4, my / 2 + (b >> 1) * 4, 4, true);
+            for (int comp = 1; comp < 3; ++comp) {
+                if (c.levels) {
+                    bool gany;
+                    const uint32_t go = dc_group(4, &gany);
+                    for (int b = 0; b < 4; ++b)
+                        level_block(comp, mx / 2 + (b & 1) * 4, my / 2 + (b >> 1) * 4, 4, true, go, gany);
+                } else {
+                    for (int b = 0; b < 4; ++b) block(comp, mx / 2 + (b & 1) * 4, my / 2 + (b >> 1) * 4, 4, true);
+                }
+            }
             g.mbs[a] = m;
         }
         p.n_mbs = mbw * mbh;
@@ -429,6 +522,11 @@ struct H264Builder {
         p.n_coefs = (uint32_t)g.coefs32.size();
         p.intra_pred = g.intra_pred.data();
         p.n_intra_pred = (uint32_t)g.intra_pred.size();
+        if (c.scaling) {
+            fill_factors(rsl, &g.h264_scaling.list4[0][0], sizeof(g.h264_scaling.list4));
+            fill_factors(rsl, &g.h264_scaling.list8[0][0], sizeof(g.h264_scaling.list8));
+            p.scaling = &g.h264_scaling;
+        }
     }
 };
 

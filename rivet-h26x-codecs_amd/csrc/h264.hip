@@ -571,9 +571,7 @@ template <int N> __device__ __forceinline__ void load_coefs(const int32_t* co, i
 }
 
 // residual4 with idct4_add16 / dc_add16, by rows.
-__device__ __forceinline__ void residual4_rows(uint8_t* dst, long pitch, const int32_t* co, int has_dc, int dc) {
-    int c[16];
-    load_coefs<16>(co, c);
+__device__ __forceinline__ void residual4_rows(uint8_t* dst, long pitch, const int (&c)[16], int has_dc, int dc) {
     int16_t d[16];
     int ac = 0;
 #pragma unroll
@@ -620,9 +618,7 @@ __device__ __forceinline__ void residual4_rows(uint8_t* dst, long pitch, const i
 }
 
 // residual8 with idct8_add16 / dc_add16, by rows.
-__device__ __forceinline__ void residual8_rows(uint8_t* dst, long pitch, const int32_t* co) {
-    int c[64];
-    load_coefs<64>(co, c);
+__device__ __forceinline__ void residual8_rows(uint8_t* dst, long pitch, const int (&c)[64]) {
     int16_t d[64];
     int ac = 0;
 #pragma unroll
@@ -664,6 +660,117 @@ __device__ __forceinline__ void residual8_rows(uint8_t* dst, long pitch, const i
     }
 }
 
+// ---------------------------------------------------------------------
+// Scaling of levels (8.5.9 - 8.5.13.1), as the reference's parser and
+// recon apply it: LevelScale = weightScale * normAdjust (Dequant::new,
+// src/h264/transform.rs:24-56), dequant_level (src/h264/mb.rs:1420-1428)
+// with shift qP / 6 + 2 (4x4) or qP / 6 (8x8), and the DC transforms
+// (transform.rs:179-228).
+// ---------------------------------------------------------------------
+
+// normAdjust4x4 / normAdjust8x8 values of one position class for the six
+// values of qP % 6, one byte each (m = 0 lowest).
+constexpr uint64_t pack6(int a, int b, int c, int d, int e, int f) {
+    return (uint64_t)a | (uint64_t)b << 8 | (uint64_t)c << 16 | (uint64_t)d << 24 | (uint64_t)e << 32 |
+           (uint64_t)f << 40;
+}
+// 4x4 classes: both indices even, both odd, mixed.
+constexpr uint64_t kNorm4[3] = {pack6(10, 11, 13, 14, 16, 18), pack6(16, 18, 20, 23, 25, 29),
+                                pack6(13, 14, 16, 18, 20, 23)};
+// 8x8 classes: (i%4, j%4) both 0; i, j odd; both 2; a 0 and an odd; a 0
+// and a 2; the rest.
+constexpr uint64_t kNorm8[6] = {pack6(20, 22, 26, 28, 32, 36), pack6(18, 19, 23, 25, 28, 32),
+                                pack6(32, 35, 42, 45, 51, 58), pack6(19, 21, 24, 26, 30, 34),
+                                pack6(25, 28, 33, 35, 40, 46), pack6(24, 26, 31, 33, 38, 43)};
+constexpr int norm4_class(int i, int j) { return (i % 2 == 0 && j % 2 == 0) ? 0 : (i % 2 == 1 && j % 2 == 1) ? 1 : 2; }
+constexpr int norm8_class(int i, int j) {
+    return (i % 4 == 0 && j % 4 == 0)                                  ? 0
+           : (i % 2 == 1 && j % 2 == 1)                                ? 1
+           : (i % 4 == 2 && j % 4 == 2)                                ? 2
+           : ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) ? 3
+           : ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) ? 4
+                                                                        : 5;
+}
+__device__ __forceinline__ int norm_of(uint64_t packed, int m) { return (int)((packed >> (8 * m)) & 0xFF); }
+
+// (level * table << shift + 32) >> 6 in wrapping 32-bit arithmetic.
+__device__ __forceinline__ int dequant_level(int level, int table, int shift) {
+    const uint32_t t = ((uint32_t)level * (uint32_t)table) << shift;
+    return (int)(t + 32u) >> 6;
+}
+
+// The weightScale bytes of a list as N / 4 words (all 16 when flat).
+template <int N> __device__ __forceinline__ void load_weights(const uint8_t* list, uint32_t (&w)[N / 4]) {
+    if (list) {
+#pragma unroll
+        for (int q = 0; q < N / 16; ++q) {
+            const uint4 u = ((const uint4*)list)[q];
+            w[4 * q] = u.x, w[4 * q + 1] = u.y, w[4 * q + 2] = u.z, w[4 * q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) w[q] = 0x10101010u;
+    }
+}
+
+__device__ __forceinline__ void dequant4(int (&c)[16], const uint8_t* list, int qp) {
+    const int m = qp % 6, shift = qp / 6 + 2;
+    const int v0 = norm_of(kNorm4[0], m), v1 = norm_of(kNorm4[1], m), v2 = norm_of(kNorm4[2], m);
+    uint32_t w[4];
+    load_weights<16>(list, w);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int ws = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFF);
+        const int k = norm4_class(i >> 2, i & 3);
+        c[i] = dequant_level(c[i], ws * (k == 0 ? v0 : k == 1 ? v1 : v2), shift);
+    }
+}
+
+__device__ __forceinline__ void dequant8(int (&c)[64], const uint8_t* list, int qp) {
+    const int m = qp % 6, shift = qp / 6;
+    const int v0 = norm_of(kNorm8[0], m), v1 = norm_of(kNorm8[1], m), v2 = norm_of(kNorm8[2], m);
+    const int v3 = norm_of(kNorm8[3], m), v4 = norm_of(kNorm8[4], m), v5 = norm_of(kNorm8[5], m);
+    uint32_t w[16];
+    load_weights<64>(list, w);
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        const int ws = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFF);
+        const int k = norm8_class(i >> 3, i & 7);
+        const int v = k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : k == 3 ? v3 : k == 4 ? v4 : v5;
+        c[i] = dequant_level(c[i], ws * v, shift);
+    }
+}
+
+__device__ __forceinline__ int sel4(int k, int a, int b, int c, int d) { return k == 0 ? a : k == 1 ? b : k == 2 ? c : d; }
+
+// Entry `idx` (raster) of luma_dc_transform over the 16 DC levels at g:
+// each row through the Hadamard butterflies (outputs in the order c0+c1+
+// c2+c3, c0+c1-c2-c3, c0-c1-c2+c3, c0-c1+c2-c3), column idx % 4 of the
+// result through them again, entry idx / 4 of that; then the scaling at
+// qP with LevelScale4x4(qP % 6, 0, 0) = scale00.
+__device__ __forceinline__ int luma_dc(const int32_t* g, int idx, int scale00, int qp) {
+    int col[4];
+    const int j = idx & 3, i = idx >> 2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int c0 = g[r * 4], c1 = g[r * 4 + 1], c2 = g[r * 4 + 2], c3 = g[r * 4 + 3];
+        const int e0 = c0 + c1, e1 = c0 - c1, e2 = c2 - c3, e3 = c2 + c3;
+        col[r] = sel4(j, e0 + e3, e0 - e3, e1 - e2, e1 + e2);
+    }
+    const int e0 = col[0] + col[1], e1 = col[0] - col[1], e2 = col[2] - col[3], e3 = col[2] + col[3];
+    const int f = sel4(i, e0 + e3, e0 - e3, e1 - e2, e1 + e2);
+    const int q6 = qp / 6;
+    const uint32_t p = (uint32_t)f * (uint32_t)scale00;
+    return qp >= 36 ? (int)(p << (q6 - 6)) : (int)(p + (1u << (5 - q6))) >> (6 - q6);
+}
+
+// Entry `idx` of chroma_dc_transform_420 over the 4 DC levels at g.
+__device__ __forceinline__ int chroma_dc(const int32_t* g, int idx, int scale00, int qp) {
+    const int c0 = g[0], c1 = g[1], c2 = g[2], c3 = g[3];
+    const int f = sel4(idx, c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3);
+    return (int)(((uint32_t)f * (uint32_t)scale00) << (qp / 6)) >> 5;
+}
+
 __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     const int32_t* pre = b.prefix + 1 * (b.n + 1);
     const int it = blockIdx.x;
@@ -682,8 +789,42 @@ __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     }
     const rv_h264_blk bk = P.blks[k];
     uint8_t* dst = (uint8_t*)P.cur[bk.comp] + (size_t)bk.y * P.pitch[bk.comp] + bk.x;
-    if (bk.size == 8) residual8_rows(dst, P.pitch[bk.comp], P.coefs + bk.coef_off);
-    else residual4_rows(dst, P.pitch[bk.comp], P.coefs + bk.coef_off, bk.has_dc, bk.dc);
+    const int32_t* co = P.coefs + bk.coef_off;
+    const bool levels = bk.flags & RV_BLK_LEVELS;
+    // Levels: the macroblock's QP and intra / inter lists (MbDequant::for_mb,
+    // src/h264/mb.rs:1384-1414).
+    int qp = 0, list = 0;
+    const rv_h264_scaling* sl = P.scaling;
+    if (levels) {
+        const int mb = bk.comp ? (bk.y >> 3) * P.mbw + (bk.x >> 3) : (bk.y >> 4) * P.mbw + (bk.x >> 4);
+        const rv_h264_mb m = P.mbs[mb];
+        qp = bk.comp == 0 ? m.qp : bk.comp == 1 ? m.qpc[0] : m.qpc[1];
+        list = (m.intra ? 0 : 3) + bk.comp;
+    }
+    if (bk.size == 8) {
+        int c[64];
+        load_coefs<64>(co, c);
+        if (levels) dequant8(c, sl ? sl->list8[list == 0 ? 0 : 1] : nullptr, qp);
+        residual8_rows(dst, P.pitch[bk.comp], c);
+    } else {
+        int c[16];
+        load_coefs<16>(co, c);
+        int dc = bk.dc;
+        if (levels) {
+            dequant4(c, sl ? sl->list4[list] : nullptr, qp);
+            if (bk.has_dc) {
+                // The DC of this block's position in its group, scaled with
+                // the group's list: Intra_16x16 luma uses list 0 (recon.rs:
+                // 701-706), chroma the macroblock's (recon.rs:1201-1211).
+                const int32_t* g = P.coefs + bk.dc;
+                const int dl = bk.comp ? list : 0;
+                const int scale00 = (sl ? (int)sl->list4[dl][0] : 16) * norm_of(kNorm4[0], qp % 6);
+                dc = bk.comp ? chroma_dc(g, ((bk.y >> 2) & 1) * 2 + ((bk.x >> 2) & 1), scale00, qp)
+                             : luma_dc(g, ((bk.y >> 2) & 3) * 4 + ((bk.x >> 2) & 3), scale00, qp);
+            }
+        }
+        residual4_rows(dst, P.pitch[bk.comp], c, bk.has_dc, dc);
+    }
 }
 
 // ---------------------------------------------------------------------

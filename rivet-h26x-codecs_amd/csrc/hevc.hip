@@ -937,6 +937,51 @@ __device__ __forceinline__ void add_row(S* d, const int (&r)[N], int max) {
     }
 }
 
+// scale_coefficients (src/hevc/residual.rs:1223-1283) for the levels of
+// one TU: d = Clip3(-32768, 32767, ((c * m * levelScale[qP % 6]) << (qP / 6)
+// + (1 << (bdShift - 1))) >> bdShift), bdShift = BitDepth + log2 - 5. m is
+// 16 without scaling lists, else the TU's list entry (src/hevc/ctu.rs:
+// 2016-2029): raster for 4x4 and 8x8; for 16x16 / 32x32 the 8x8 grid entry
+// covering (x, y), and dc at (0, 0). |c * m * levelScale| < 2^31 (|c| <=
+// 2^15, m <= 255, levelScale <= 72): only the shift needs 64 bits.
+struct LevelScale {
+    int ls, q6, bd_shift, dc;
+    const uint8_t* list;  // null: flat
+};
+
+__device__ __forceinline__ LevelScale level_scale(PicRef P, const rv_hevc_tu& tu, int log2) {
+    LevelScale s;
+    const int qp = tu.qp, q6 = qp / 6;
+    // levelScale = {40, 45, 51, 57, 64, 72}, one byte each.
+    s.ls = (int)((0x484039332D28ull >> (8 * (qp - 6 * q6))) & 0xFF);
+    s.q6 = q6;
+    s.bd_shift = P.bd + log2 - 5;
+    s.list = nullptr;
+    s.dc = 16;
+    if (P.scaling) {
+        const int mid = ((tu.flags & RV_TU_INTRA) ? 0 : 3) + tu.comp;
+        s.list = log2 == 2 ? P.scaling->list4[mid] : P.scaling->list8[log2 - 3][mid];
+        if (log2 >= 4) s.dc = P.scaling->dc[log2 - 4][mid];
+    }
+    return s;
+}
+
+template <int LOG2> __device__ __forceinline__ int16_t scale_level(const LevelScale& s, int c, int pos) {
+    int m = 16;
+    if (s.list) {
+        if constexpr (LOG2 <= 3) {
+            m = s.list[pos];
+        } else {
+            constexpr int R = LOG2 - 3;
+            const int x = pos & ((1 << LOG2) - 1), y = pos >> LOG2;
+            m = pos == 0 ? s.dc : s.list[((y >> R) << 3) + (x >> R)];
+        }
+    }
+    const int64_t v = ((int64_t)(c * m * s.ls) << s.q6) + ((int64_t)1 << (s.bd_shift - 1));
+    const int64_t d = v >> s.bd_shift;
+    return (int16_t)(d < -32768 ? -32768 : (d > 32767 ? 32767 : d));
+}
+
 // One wave reconstructs 64 / L coded TUs of size N (N >= 8), L lanes per
 // TU: idct_scalar (src/dsp/hevc.rs:499-527) — columns, round by 7 and clip
 // to i16, rows, round by bdShift = 20 - BitDepth and clip — then the
@@ -967,6 +1012,10 @@ __device__ __forceinline__ void tu_lanes(PicRef P, int first, int count, int tid
         const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
         const int n = tu.n_nz, per = ((n + L - 1) / L + 3) & ~3;
         const int lo = sub * per, hi = min(lo + per, n);
+        const bool levels = tu.flags & RV_TU_LEVELS;
+        constexpr int LOG2 = N == 8 ? 3 : N == 16 ? 4 : 5;
+        LevelScale sc;
+        if (levels) sc = level_scale(P, tu, LOG2);
         for (int k = lo; k < hi; k += 8) {
             uint32_t v[8];
             if (k < hi) {
@@ -979,7 +1028,11 @@ __device__ __forceinline__ void tu_lanes(PicRef P, int first, int count, int tid
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (k + i < hi) t[(v[i] & 0xFFFF) & (N * N - 1)] = (int16_t)(v[i] >> 16);
+                if (k + i < hi) {
+                    const int pos = (v[i] & 0xFFFF) & (N * N - 1);
+                    const int c = (int)(int16_t)(v[i] >> 16);
+                    t[pos] = levels ? scale_level<LOG2>(sc, c, pos) : (int16_t)c;
+                }
         }
     }
     __syncthreads();
@@ -1073,9 +1126,16 @@ __device__ __forceinline__ void tu4_lane(PicRef P, int first, int count, int tid
             for (int i = 0; i < 4; ++i) v[4 * q + i] = z[4 * q + i];
         }
     }
+    if (tu.flags & RV_TU_LEVELS) {
+        const LevelScale sc = level_scale(P, tu, 2);
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-        if (i < n) c[v[i] & 15] = (int16_t)(v[i] >> 16);
+        for (int i = 0; i < 16; ++i)
+            if (i < n) c[v[i] & 15] = scale_level<2>(sc, (int)(int16_t)(v[i] >> 16), v[i] & 15);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < n) c[v[i] & 15] = (int16_t)(v[i] >> 16);
+    }
     const uint4 r0 = c4[0], r1 = c4[1];
     const uint32_t w[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
     int m[16];  // raster: m[y * 4 + x]

@@ -428,16 +428,23 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
         int n = 1 << t.log2, pw = t.comp ? W / 2 : W, ph = t.comp ? H / 2 : H;
         if ((t.x & 3) || (t.y & 3) || t.x + n > pw || t.y + n > ph) return fail(RV_E_RANGE, "TU outside the plane");
         if ((t.flags & RV_TU_DST) && (t.log2 != 2 || t.comp != 0)) return fail(RV_E_RANGE, "DST on a non-4x4 luma TU");
+        if (t.flags & ~(RV_TU_DST | RV_TU_CODED | RV_TU_LEVELS | RV_TU_INTRA)) return fail(RV_E_RANGE, "unknown TU flag");
+        const bool levels = (t.flags & RV_TU_LEVELS) != 0;
+        if (levels && t.qp > 51 + 6 * (s->bd - 8)) return fail(RV_E_RANGE, "TU qP outside 0..51 + QpBdOffset");
         if (t.flags & RV_TU_CODED) {
             if (t.max_x >= n || t.max_y >= n) return fail(RV_E_RANGE, "nonzero bounds outside the TU");
             if (t.n_nz == 0 || t.n_nz > n * n || (uint64_t)t.coef_off + t.n_nz > p->n_nz)
                 return fail(RV_E_RANGE, "nonzero list outside nz");
-            // The reference transforms only inside the bounds (idct_scalar,
-            // src/dsp/hevc.rs:499-526); a nonzero outside them would be
-            // dropped there and kept here, so it is refused.
+            // The reference transforms (idct_scalar, src/dsp/hevc.rs:499-526)
+            // and scales (scale_coefficients, src/hevc/residual.rs:1255-1256)
+            // only inside the bounds; a nonzero outside them would be
+            // dropped or left unscaled there and used here, so it is refused.
+            // The DST transforms the whole 4x4 and needs no bounds unless
+            // its levels are scaled.
+            const bool bounded = levels || (t.flags & RV_TU_DST) == 0;
             for (uint32_t k = 0; k < t.n_nz; ++k) {
                 const rv_nz& e = p->nz[t.coef_off + k];
-                if (e.pos >= n * n || ((t.flags & RV_TU_DST) == 0 && ((e.pos & (n - 1)) > t.max_x || (e.pos >> t.log2) > t.max_y)))
+                if (e.pos >= n * n || (bounded && ((e.pos & (n - 1)) > t.max_x || (e.pos >> t.log2) > t.max_y)))
                     return fail(RV_E_RANGE, "nonzero coefficient outside the TU or its bounds");
             }
         }
@@ -517,6 +524,16 @@ int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
             b.x + b.size > pw || b.y + b.size > ph)
             return fail(RV_E_RANGE, "residual block outside its plane or of bad size");
         if ((uint64_t)b.coef_off + (uint64_t)b.size * b.size > p->n_coefs) return fail(RV_E_RANGE, "coefficients outside coefs");
+        if (b.flags & ~RV_BLK_LEVELS) return fail(RV_E_RANGE, "unknown block flag");
+        if ((b.flags & RV_BLK_LEVELS) && b.has_dc) {
+            // A DC group: the four chroma DCs of a macroblock's component, or
+            // the sixteen luma DCs of an Intra_16x16 macroblock.
+            const int group = b.comp ? 4 : 16;
+            if (b.size != 4) return fail(RV_E_RANGE, "DC on an 8x8 block");
+            if (!b.comp && !p->mbs[(b.y >> 4) * mbw + (b.x >> 4)].intra)
+                return fail(RV_E_RANGE, "luma DC outside an intra macroblock");
+            if (b.dc < 0 || (uint64_t)b.dc + group > p->n_coefs) return fail(RV_E_RANGE, "DC levels outside coefs");
+        }
     }
     return RV_OK;
 }
@@ -555,6 +572,8 @@ int rv_abi_size(int which) {
         case 7: return (int)sizeof(rv_h264_blk);
         case 8: return (int)sizeof(rv_h264_pic);
         case 9: return (int)sizeof(rv_nz);
+        case 10: return (int)sizeof(rv_hevc_scaling);
+        case 11: return (int)sizeof(rv_h264_scaling);
         default: return -1;
     }
 }
@@ -927,7 +946,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<rv::SideClass>(n_sclass) +
                     carve_size<rv_nz>(n_nzs) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
-                    carve_size<int32_t>(n_intra) +
+                    carve_size<int32_t>(n_intra) + carve_size<rv_hevc_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4);
         size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
@@ -954,6 +973,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
         int32_t* h_intra = carve<int32_t>(hp, n_intra);
+        rv_hevc_scaling* h_scaling = carve<rv_hevc_scaling>(hp, p->scaling ? 1 : 0);
+        if (p->scaling) *h_scaling = *p->scaling;
         const rv::HevcPicDev** h_ptr = carve<const rv::HevcPicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         // Pass 2: write.
@@ -1070,6 +1091,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.nz = (const rv_nz*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.sao = (const rv_sao*)dev((char*)h_sao);
+        d.scaling = p->scaling ? (const rv_hevc_scaling*)dev((char*)h_scaling) : nullptr;
         d.cuinfo = (uint8_t*)sp;
         sp += carve_size<uint8_t>(n4);
         d.tuinfo = (uint8_t*)sp;
@@ -1136,6 +1158,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
                     carve_size<rv_h264_part>(p->n_parts) + carve_size<uint32_t>(p->n_parts + n_intra) +
                     carve_size<rv_h264_blk>(blks.size()) +
                     carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
+                    carve_size<rv_h264_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4), nmb = (size_t)mbw * mbh;
         const size_t nx = rv::h264_deblock_xch_words(mbw, mbh);
@@ -1157,6 +1180,8 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         rv_h264_blk* h_blks = carve<rv_h264_blk>(hp, blks.size());
         int32_t* h_coefs = carve<int32_t>(hp, p->n_coefs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
+        rv_h264_scaling* h_scaling = carve<rv_h264_scaling>(hp, p->scaling ? 1 : 0);
+        if (p->scaling) *h_scaling = *p->scaling;
         const rv::H264PicDev** h_ptr = carve<const rv::H264PicDev*>(hp, 1);
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         std::memcpy(h_mbs, p->mbs, p->n_mbs * sizeof(rv_h264_mb));
@@ -1233,6 +1258,7 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.blks = (const rv_h264_blk*)dev((char*)h_blks);
         d.coefs = (const int32_t*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
+        d.scaling = p->scaling ? (const rv_h264_scaling*)dev((char*)h_scaling) : nullptr;
         char* sp = (char*)j->scratch.p;
         d.partmap = (uint32_t*)sp;
         sp += carve_size<uint32_t>(n4);

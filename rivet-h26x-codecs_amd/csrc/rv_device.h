@@ -71,6 +71,7 @@ struct HevcPicDev {
     const rv_nz* nz;         // nonzero coefficients of ctus, in ctus order
     const uint16_t* intra_pred;
     const rv_sao* sao;
+    const rv_hevc_scaling* scaling;  // levels' scaling lists; null = flat 16
     // Per-4x4 side data (job scratch).
     uint8_t* cuinfo;   // bit 0 intra, bit 1 CU left edge, bit 2 CU top edge
     int8_t* qp;        // QpY
@@ -118,6 +119,7 @@ struct H264PicDev {
     const rv_h264_blk* blks;  // 4x4 first, then 8x8
     const int32_t* coefs;
     const uint16_t* intra_pred;
+    const rv_h264_scaling* scaling;  // levels' scaling lists; null = flat 16
     uint32_t* partmap;   // per 4x4: partition index (0xFFFFFFFF = intra)
     uint32_t* dbk;       // per MB: kH264DbkWords words of edge parameters
     uint64_t* xch;       // deblocking: band ticket, then the strips crossing bands

@@ -80,6 +80,8 @@ class GenPicture:
         if codec == RV_CODEC_HEVC:
             c = _abi.GenHevcCfg()
             for k, v in cfg.items():
+                if not hasattr(c, k):
+                    raise KeyError(f"no generator setting {k!r}")
                 if k in ("ref_poc", "ref_slot"):
                     getattr(c, k)[:] = v
                 else:
@@ -90,6 +92,8 @@ class GenPicture:
         else:
             c = _abi.GenH264Cfg()
             for k, v in cfg.items():
+                if not hasattr(c, k):
+                    raise KeyError(f"no generator setting {k!r}")
                 if k in ("ref_poc", "ref_slot"):
                     getattr(c, k)[:] = v
                 else:

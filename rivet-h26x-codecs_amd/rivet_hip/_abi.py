@@ -24,6 +24,9 @@ RV_E_DEVICE = -7
 RV_MAX_REFS = 4
 RV_TU_DST = 1
 RV_TU_CODED = 2
+RV_TU_LEVELS = 4
+RV_TU_INTRA = 8
+RV_BLK_LEVELS = 1
 RV_CODEC_HEVC = 0
 RV_CODEC_H264 = 1
 RV_H264_DEFAULT = 0
@@ -64,7 +67,11 @@ class Nz(C.Structure):
 
 class HevcTu(C.Structure):
     _fields_ = [("x", u16), ("y", u16), ("comp", u8), ("log2", u8), ("flags", u8), ("max_x", u8), ("max_y", u8),
-                ("pad", u8), ("n_nz", u16), ("coef_off", u32)]
+                ("qp", u8), ("n_nz", u16), ("coef_off", u32)]
+
+
+class HevcScaling(C.Structure):
+    _fields_ = [("list4", (u8 * 16) * 6), ("list8", ((u8 * 64) * 6) * 3), ("dc", (u8 * 6) * 2)]
 
 
 class HevcPic(C.Structure):
@@ -78,7 +85,7 @@ class HevcPic(C.Structure):
         ("n_cus", i32), ("n_pus", i32), ("n_tus", i32),
         ("cus", C.POINTER(HevcCu)), ("pus", C.POINTER(HevcPu)), ("tus", C.POINTER(HevcTu)),
         ("nz", C.POINTER(Nz)), ("n_nz", u32), ("n_intra_pred", u32),
-        ("intra_pred", C.POINTER(u16)), ("sao", C.POINTER(Sao)),
+        ("intra_pred", C.POINTER(u16)), ("sao", C.POINTER(Sao)), ("scaling", C.POINTER(HevcScaling)),
     ]
 
 
@@ -93,8 +100,12 @@ class H264Part(C.Structure):
 
 
 class H264Blk(C.Structure):
-    _fields_ = [("x", u16), ("y", u16), ("comp", u8), ("size", u8), ("has_dc", u8), ("pad", u8), ("dc", i32),
+    _fields_ = [("x", u16), ("y", u16), ("comp", u8), ("size", u8), ("has_dc", u8), ("flags", u8), ("dc", i32),
                 ("coef_off", u32)]
+
+
+class H264Scaling(C.Structure):
+    _fields_ = [("list4", (u8 * 16) * 6), ("list8", (u8 * 64) * 2)]
 
 
 class H264Pic(C.Structure):
@@ -107,22 +118,25 @@ class H264Pic(C.Structure):
         ("n_mbs", i32), ("n_parts", i32), ("n_blks", i32),
         ("mbs", C.POINTER(H264Mb)), ("parts", C.POINTER(H264Part)), ("blks", C.POINTER(H264Blk)),
         ("coefs", C.POINTER(i32)), ("n_coefs", u32), ("n_intra_pred", u32), ("intra_pred", C.POINTER(u16)),
+        ("scaling", C.POINTER(H264Scaling)),
     ]
 
 
 class GenHevcCfg(C.Structure):
     _fields_ = [(n, i32) for n in ("width", "height", "bit_depth", "b_picture", "bi_percent", "explicit_wp",
                                    "intra_percent", "nz_permille", "lap_b", "qp_min", "qp_max", "mv_range",
-                                   "deblock", "sao", "poc")] + [("ref_poc", i32 * 2), ("ref_slot", i32 * 2)]
+                                   "deblock", "sao", "poc")] + [("ref_poc", i32 * 2), ("ref_slot", i32 * 2),
+                                                                ("levels", i32), ("scaling", i32)]
 
 
 class GenH264Cfg(C.Structure):
     _fields_ = [(n, i32) for n in ("width", "height", "b_picture", "bi_percent", "weighting", "intra_percent",
                                    "nz_permille", "lap_b", "qp_min", "qp_max", "mv_range", "deblock", "poc")] + [
-        ("ref_poc", i32 * 2), ("ref_slot", i32 * 2)]
+        ("ref_poc", i32 * 2), ("ref_slot", i32 * 2), ("levels", i32), ("scaling", i32)]
 
 
-ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz]
+ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz, HevcScaling,
+               H264Scaling]
 
 # Every extern "C" entry of include/rivet_hip.h (librivet_hip.so):
 # name -> (restype, argtypes).
