@@ -38,6 +38,7 @@ import numpy as np  # noqa: E402
 METRIC = "bit-exact luma Msample/s 1080p/4K batched; p99 picture latency, serve loop"
 HBM_PEAK_GBS = 8000.0
 STAGES = ["mc", "tx", "dbk_v", "dbk_h", "sao"]
+LEVELS_NOTE = "; coefficients as levels with scaling lists, scaled on the device"
 
 
 def log(*a):
@@ -112,10 +113,11 @@ def cpu_baseline(oracle, pics, refs, cores: int, target_core_s: float = 10.0):
     return first, wall, rounds, inner, one
 
 
-def build_workload(rh, cf, streams: int):
+def build_workload(rh, cf, streams: int, extra_cfg: dict):
     pics, refs, sessions, jobs = [], [], [], []
     for k in range(streams):
         cfg, seed = cf.c3_stream(k)
+        cfg = dict(cfg, **extra_cfg)
         g = rh.GenPicture(rh.RV_CODEC_HEVC, cfg, seed)
         r = [rh.gen_planes(cfg["width"], cfg["height"], cfg["bit_depth"], cf.ref_seed(seed, s)) for s in range(2)]
         s = rh.Session(rh.RV_CODEC_HEVC, cfg["width"], cfg["height"], cfg["bit_depth"], 3)
@@ -128,12 +130,13 @@ def build_workload(rh, cf, streams: int):
     return pics, refs, sessions, jobs
 
 
-def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int):
+def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int, extra_cfg: dict):
     """Configuration 2 (HEVC Main10 1080p B, explicit WP, intra with DST) or
     4 (H.264 High 1080p B, implicit weights) as a batched workload: n pictures
     of the configuration (the first is the configuration's own seed), one
     graph launch per step, every plane checked against the oracle."""
     codec, cfg, seed0 = cf.single(config)
+    cfg = dict(cfg, **extra_cfg)
     w, h, bd = cfg["width"], cfg["height"], cfg.get("bit_depth", 8)
     pics, refs, sessions, jobs = [], [], [], []
     for k in range(n):
@@ -172,7 +175,7 @@ def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int)
     luma = n * cf.visible_luma(w, h)
     out = {
         "workload": f"{n} x config {config} ({'HEVC' if codec == rh.RV_CODEC_HEVC else 'H.264'} {w}x{h}, "
-                    f"{bd}-bit), batched, hipGraph replay",
+                    f"{bd}-bit), batched, hipGraph replay" + LEVELS_NOTE * bool(extra_cfg),
         "bit_exact": ok,
         "ms_per_step": round(t * 1e3, 4),
         "luma_msamples_per_s": round(luma / t / 1e6, 1) if ok else None,
@@ -213,6 +216,9 @@ def main() -> int:
     ap.add_argument("--only-config", type=int, choices=(2, 4), default=0,
                     help="measure only 8 pictures of configuration 2 or 4, batched (for kernel profiling), "
                          "print its JSON and exit")
+    ap.add_argument("--levels", action="store_true",
+                    help="generate coefficients as levels with seeded scaling lists, scaled on the device in "
+                         "stage 1 (same pictures' structure; the coefficient values differ)")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -230,13 +236,14 @@ def main() -> int:
         gbs = rh.probe_copy_gbs(256 << 20, 1)
         log(f"[bench] copy probe: 256 MiB read + 256 MiB written, {gbs:.1f} GB/s")
         return 0
+    extra_cfg = dict(levels=1, scaling=1) if args.levels else {}
     if args.only_config:
-        r = measure_config(rh, cf, oracle, args.only_config, 8, args.steps, args.warmup)
+        r = measure_config(rh, cf, oracle, args.only_config, 8, args.steps, args.warmup, extra_cfg)
         log(f"[bench] config {args.only_config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms")
         print(json.dumps(r), flush=True)
         return 0 if r["bit_exact"] else 1
     log(f"[bench] tier {rh.tier_name()}; generating {args.streams} x 3840x2160 pictures")
-    pics, refs, sessions, jobs = build_workload(rh, cf, args.streams)
+    pics, refs, sessions, jobs = build_workload(rh, cf, args.streams, extra_cfg)
     batch = rh.Batch(jobs)
     luma = sum(cf.visible_luma(g.width, g.height) for g in pics)
 
@@ -304,7 +311,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{args.streams} x HEVC Main 8-bit 4:2:0 3840x2160 P/B pictures (one per stream), "
-                        "batched launch per stage, hipGraph replay",
+                        "batched launch per stage, hipGraph replay" + LEVELS_NOTE * args.levels,
             "streams": args.streams, "width": 3840, "height": 2160, "bit_depth": 8,
             "pictures_per_step": len(pics),
         },
@@ -338,7 +345,7 @@ def main() -> int:
     if not args.graph_only and not args.no_configs:
         out["extra"]["configs"] = {}
         for config in (2, 4):
-            r = measure_config(rh, cf, oracle, config, 8, max(10, args.steps // 4), 3)
+            r = measure_config(rh, cf, oracle, config, 8, max(10, args.steps // 4), 3, extra_cfg)
             log(f"[bench] config {config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms, "
                 f"{r['luma_msamples_per_s']} Msample/s; stages ms {r['stage_ms']}")
             out["extra"]["configs"][str(config)] = r
