@@ -415,16 +415,27 @@ struct alignas(16) McLds {
 // straight from the reference planes. Second stage: luma lane (x = s & 7,
 // rows 4 (s >> 3) ..+3), chroma lane (component s >> 3, x = s & 3, rows
 // 2 ((s >> 2) & 1) ..+1). Every lane reaches the barrier.
-template <typename S, int LB>
+// QUAD: the four blocks are the 8x8 quadrants of 16x16 tile `first` (a
+// window of 15 x 15 luma samples per quadrant instead of one shared
+// 23 x 23 window).
+template <typename S, int LB, bool QUAD = false>
 __device__ __forceinline__ void mc_item(PicRef P, int first, int tid, McLds& L) {
     first = __builtin_amdgcn_readfirstlane(first);
     const int slot = tid >> 4, s = tid & 15;
-    // The wave's four records (uniform loads: the list is padded to a
-    // multiple of four with empty records, dir 0).
-    const uint4* rec = (const uint4*)(P.tiles + first);
-    const uint4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
-    const uint4 q = slot == 0 ? q0 : slot == 1 ? q1 : slot == 2 ? q2 : q3;
-    const McTile t = __builtin_bit_cast(McTile, q);
+    McTile t;
+    if constexpr (QUAD) {
+        t = __builtin_bit_cast(McTile, *(const uint4*)(P.tiles16 + first));
+        t.x = (uint16_t)(t.x + (slot & 1) * 8);
+        t.y = (uint16_t)(t.y + (slot >> 1) * 8);
+        t.tw = t.th = 8;
+    } else {
+        // The wave's four records (uniform loads: the list is padded to a
+        // multiple of four with empty records, dir 0).
+        const uint4* rec = (const uint4*)(P.tiles + first);
+        const uint4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+        const uint4 q = slot == 0 ? q0 : slot == 1 ? q1 : slot == 2 ? q2 : q3;
+        t = __builtin_bit_cast(McTile, q);
+    }
     // The packed taps, read per lane below: an LDS copy costs two coalesced
     // loads per wave instead of one per lane and tap set. Issued first,
     // written to LDS once the reference loads are in flight.
@@ -778,7 +789,10 @@ __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     }
 }
 
-template <int D, int W, int LB = 2>
+// QUAD: 16x16 tiles as four 8x8 quadrants through mc_item (a smaller
+// window per block, no shared one): a tile and window shape the tuner
+// weighs against the shared 16x16 window.
+template <int D, int W, int LB = 2, bool QUAD = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_stage0(BatchDev b) {
     __shared__ union {
         McLds blocks;
@@ -806,8 +820,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
     }
     i -= P.n_intra;
     if (i < P.n_tiles16) {
-        if (eight_bit<D>(P.bd)) mc_tile16<uint8_t>(P, i, tid, lds.tile);
-        else mc_tile16<uint16_t>(P, i, tid, lds.tile);
+        if constexpr (QUAD) {
+            if (eight_bit<D>(P.bd)) mc_item<uint8_t, LB, true>(P, i, tid, lds.blocks);
+            else mc_item<uint16_t, LB, true>(P, i, tid, lds.blocks);
+        } else {
+            if (eight_bit<D>(P.bd)) mc_tile16<uint8_t>(P, i, tid, lds.tile);
+            else mc_tile16<uint16_t>(P, i, tid, lds.tile);
+        }
         return;
     }
     i -= P.n_tiles16;
@@ -2067,7 +2086,9 @@ const StageVariant kStage0[] = {RV_VARIANT((hevc_stage0<0, 5>), 0, 64, "any/5"),
                                 RV_VARIANT((hevc_stage0<1, 8, 1>), 1, 64, "8-bit/8 per-list"),
                                 RV_VARIANT((hevc_stage0<2, 5>), 2, 64, "deep/5"),
                                 RV_VARIANT((hevc_stage0<2, 6, 1>), 2, 64, "deep/6 per-list"),
-                                RV_VARIANT((hevc_stage0<0, 6, 1>), 0, 64, "any/6 per-list")};
+                                RV_VARIANT((hevc_stage0<0, 6, 1>), 0, 64, "any/6 per-list"),
+                                RV_VARIANT((hevc_stage0<1, 8, 1, true>), 1, 64, "8-bit/8 per-list 8x8 windows"),
+                                RV_VARIANT((hevc_stage0<2, 6, 1, true>), 2, 64, "deep/6 per-list 8x8 windows")};
 const StageVariant kStage1[] = {RV_VARIANT((hevc_stage1<0, 4>), 0, 64, "any/4"),
                                 RV_VARIANT((hevc_stage1<1, 4>), 1, 64, "8-bit/4"),
                                 RV_VARIANT((hevc_stage1<2, 4>), 2, 64, "deep/4")};
@@ -2080,7 +2101,7 @@ struct StageTable {
     const StageVariant* v;
     int n;
 };
-const StageTable kStageTables[kStages] = {{kStage0, 7}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
+const StageTable kStageTables[kStages] = {{kStage0, 9}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
 
 }  // namespace
 

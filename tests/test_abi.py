@@ -123,7 +123,10 @@ def test_kernel_variant_tables():
     import rivet_hip as rh
 
     hevc = rh.variant_names(rh.RV_CODEC_HEVC)
-    assert [len(v) for v in hevc] == [7, 3, 1, 1, 3]
+    assert [len(v) for v in hevc] == [9, 3, 1, 1, 3]
+    # Stage 0 also offers a tile / window shape: 16x16 PU cores as four 8x8
+    # blocks with windows of their own.
+    assert sum("8x8 windows" in n for n in hevc[0]) == 2
     for stage in hevc:
         assert stage[0].startswith("any")
         assert all(n.startswith(("8-bit", "deep", "any")) for n in stage[1:])
