@@ -40,6 +40,11 @@ typedef struct rv_gen_hevc_cfg {
     int32_t levels;         /* 1: coded TUs carry levels (RV_TU_LEVELS) at
                                the CU's qP (chroma through Table 8-10)     */
     int32_t scaling;        /* 1: seeded scaling lists (factors 4..64)     */
+    int32_t intra_device;   /* 1: intra_percent of CUs of every size intra,
+                               predicted on the device (RV_INTRA_DEVICE)
+                               with seeded modes; 8x8 CUs split into 4x4
+                               luma TUs are PART_NxN                       */
+    int32_t strong_intra_smoothing, constrained_intra_pred;  /* copied */
 } rv_gen_hevc_cfg;
 
 typedef struct rv_gen_h264_cfg {

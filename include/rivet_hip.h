@@ -85,16 +85,35 @@ typedef struct rv_sao {
 
 /* One coding unit: what the deblocking side data needs (QpY, prediction
  * mode, coding-block edges; src/hevc/ctu.rs:751-800) plus, for an intra
- * CU, where its host-supplied prediction block lives. */
+ * CU, where its host-supplied prediction block lives or the modes the
+ * device predicts it with. */
 typedef struct rv_hevc_cu {
     uint16_t x, y;      /* luma top-left                               */
     uint8_t log2_size;  /* 3..6                                        */
     int8_t qp_y;        /* QpY of the CU                               */
-    uint8_t intra;      /* 1 = intra CU: prediction supplied by host   */
-    uint8_t part_mode;  /* PartMode (informational)                    */
-    uint32_t pred_off;  /* intra: sample offset into intra_pred: Y (s*s)
-                           then Cb and Cr ((s/2)^2 each), raster          */
+    uint8_t intra;      /* 0 inter, RV_INTRA_HOST or RV_INTRA_DEVICE   */
+    uint8_t part_mode;  /* PartMode: 0 PART_2Nx2N, 3 PART_NxN (device
+                           intra reads it; otherwise informational)   */
+    uint32_t pred_off;  /* RV_INTRA_HOST: sample offset into intra_pred:
+                           Y (s*s) then Cb and Cr ((s/2)^2 each), raster */
+    uint8_t luma_mode[4]; /* RV_INTRA_DEVICE: IntraPredModeY (0..34) of
+                           the PU; PART_NxN: of the four PUs, z order  */
+    uint8_t chroma_mode;  /* RV_INTRA_DEVICE: IntraPredModeC (0..34), the
+                           derived mode (intra_chroma_pred_mode 4 and the
+                           mode-34 substitution resolved)              */
+    uint8_t pad[3];
 } rv_hevc_cu;
+
+/* Intra CU whose prediction the host supplies in intra_pred (copied in
+ * stage 0). */
+#define RV_INTRA_HOST 1
+/* Intra CU predicted on the device (src/hevc/intra.rs, called per
+ * transform block from src/hevc/ctu.rs:1727-1833): every TU of the CU is
+ * listed, luma and chroma, coded or not, and the TUs of such CUs appear in
+ * decoding order (CTBs in raster order, z-scan within). Each TU is
+ * predicted from the reconstructed samples around it, then its residual
+ * added, in dependency order after the transform stage. */
+#define RV_INTRA_DEVICE 2
 
 /* One prediction unit (the arguments of predict_block,
  * src/hevc/inter.rs:555-567). */
@@ -196,6 +215,9 @@ typedef struct rv_hevc_pic {
     /* Scaling lists of RV_TU_LEVELS TUs; NULL = flat 16 (scaling lists
      * off, ScalingSource::Flat). */
     const rv_hevc_scaling* scaling;
+    /* Intra prediction on the device (RV_INTRA_DEVICE CUs). */
+    int32_t strong_intra_smoothing;  /* strong_intra_smoothing_enabled_flag */
+    int32_t constrained_intra_pred;  /* constrained_intra_pred_flag         */
 } rv_hevc_pic;
 
 /* ------------------------------------------------------------------ */

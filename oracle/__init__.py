@@ -37,6 +37,10 @@ _sig = {
     "orc_hevc_idct": (None, [i16p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_hevc_idst4": (None, [i16p, C.c_int]),
     "orc_hevc_scale_coefficients": (None, [i16p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_intra_tables": (None, [i32p, i32p]),
+    "orc_hevc_intra_substitute": (None, [P, P, C.c_int, C.c_int]),
+    "orc_hevc_intra_pred": (None, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_hevc_intra_avail": (C.c_int, [P, C.c_int, P]),
     "orc_hevc_idct_matrix": (None, [i16p, C.c_int, C.c_int, i16p]),
     "orc_hevc_add_residual": (None, [C.c_int, P, C.c_int, i16p, C.c_int, C.c_int]),
     "orc_hevc_copy": (None, [C.c_int, i16p, P, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -919,6 +919,182 @@ void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv
 }
 
 // ---------------------------------------------------------------------
+// Intra prediction (src/hevc/intra.rs; the caller's availability and
+// flags, src/hevc/ctu.rs:1727-1833)
+// ---------------------------------------------------------------------
+
+// intraPredAngle, modes 2..34 (Table 8-4), and invAngle, modes 11..25
+// (Table 8-5) (src/hevc/intra.rs:20-27).
+const int INTRA_ANGLE[33] = {32,  26,  21,  17,  13,  9,   5,   2,   0,   -2, -5, -9, -13, -17, -21, -26, -32,
+                             -26, -21, -17, -13, -9,  -5,  -2,  0,   2,   5,  9,  13,  17,  21,  26,  32};
+const int INV_ANGLE[15] = {-4096, -1638, -910, -630, -482, -390, -315, -256, -315, -390, -482, -630, -910, -1638, -4096};
+
+// A block's reference samples in substitution (scan) order: r[0..2n) =
+// p[-1][2n-1] .. p[-1][0], r[2n] = p[-1][-1], r[2n+1..4n] = p[0][-1] ..
+// p[2n-1][-1].
+struct IntraRefs {
+    int n = 0;
+    uint16_t r[129];
+    int left(int y) const { return r[2 * n - 1 - y]; }
+    int top(int x) const { return r[2 * n + 1 + x]; }
+    int corner() const { return r[2 * n]; }
+};
+
+// prepare (src/hevc/intra.rs:159-259), 8.4.4.2.2: unavailable samples take
+// the value of the previous one in scan order; a leading run takes the first
+// available one; with none available, all are 1 << (bitDepth - 1).
+void intra_substitute(IntraRefs& R, const bool* avail, int bd) {
+    const int m = 4 * R.n + 1;
+    int first = -1;
+    for (int j = 0; j < m && first < 0; ++j)
+        if (avail[j]) first = j;
+    if (first < 0) {
+        for (int j = 0; j < m; ++j) R.r[j] = (uint16_t)(1 << (bd - 1));
+        return;
+    }
+    if (!avail[0]) R.r[0] = R.r[first];
+    for (int j = 1; j < m; ++j)
+        if (!avail[j]) R.r[j] = R.r[j - 1];
+}
+
+// smooth (src/hevc/intra.rs:261-309), 8.4.4.2.3: the [1 2 1] filter along
+// the scan order (its ends kept), or for a flat 32x32 luma block under the
+// SPS flag the bilinear strong filter between the corner and the far ends.
+IntraRefs intra_smooth(const IntraRefs& R, bool strong, int bd) {
+    IntraRefs F = R;
+    const int n = R.n, m = 4 * n + 1;
+    const int c = R.corner();
+    const bool bi = strong && n == 32 && std::abs(c + R.top(63) - 2 * R.top(31)) < (1 << (bd - 5)) &&
+                    std::abs(c + R.left(63) - 2 * R.left(31)) < (1 << (bd - 5));
+    if (bi) {
+        for (int i = 0; i < 63; ++i) {
+            F.r[2 * n - 1 - i] = (uint16_t)(((63 - i) * c + (i + 1) * R.left(63) + 32) >> 6);
+            F.r[2 * n + 1 + i] = (uint16_t)(((63 - i) * c + (i + 1) * R.top(63) + 32) >> 6);
+        }
+        return F;
+    }
+    for (int j = 1; j < m - 1; ++j) F.r[j] = (uint16_t)((R.r[j - 1] + 2 * R.r[j] + R.r[j + 1] + 2) >> 2);
+    return F;
+}
+
+// predict_prepared (src/hevc/intra.rs:311-426) with the scalar predictors
+// (src/dsp/hevc.rs:1093-1166): the filter decision (8.4.4.2.3), planar, DC
+// with its edge filter, angular with the ref[] of 8.4.4.2.6 and the
+// boundary filters of modes 10 / 26. `luma`: the component is filtered and
+// gets the boundary filters (4:2:0). Writes dst[y * n + x].
+void intra_pred(int* dst, const IntraRefs& raw, int mode, bool luma, bool strong, int bd) {
+    const int n = raw.n;
+    int log2n = 0;
+    while ((1 << log2n) < n) ++log2n;
+    bool smoothed = false;
+    if (luma && mode != 1 && n != 4) {
+        const int dist = std::min(std::abs(mode - 26), std::abs(mode - 10));
+        const int thres = n == 8 ? 7 : n == 16 ? 1 : n == 32 ? 0 : 10;
+        smoothed = dist > thres;
+    }
+    const IntraRefs R = smoothed ? intra_smooth(raw, strong, bd) : raw;
+    const int max = (1 << bd) - 1;
+    const bool edge = luma && n < 32;
+    if (mode == 0) {
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x)
+                dst[y * n + x] = ((n - 1 - x) * R.left(y) + (x + 1) * R.top(n) + (n - 1 - y) * R.top(x) +
+                                  (y + 1) * R.left(n) + n) >> (log2n + 1);
+        return;
+    }
+    if (mode == 1) {
+        int sum = n;
+        for (int i = 0; i < n; ++i) sum += R.top(i) + R.left(i);
+        const int dc = sum >> (log2n + 1);
+        for (int i = 0; i < n * n; ++i) dst[i] = dc;
+        if (edge) {
+            dst[0] = (R.left(0) + 2 * dc + R.top(0) + 2) >> 2;
+            for (int x = 1; x < n; ++x) dst[x] = (R.top(x) + 3 * dc + 2) >> 2;
+            for (int y = 1; y < n; ++y) dst[y * n] = (R.left(y) + 3 * dc + 2) >> 2;
+        }
+        return;
+    }
+    const int angle = INTRA_ANGLE[mode - 2];
+    const bool vertical = mode >= 18;
+    // ref[k], k = -n .. 2n, at refb[n + k].
+    int refb[3 * 32 + 2] = {};  // one past 3n: read (and unused) when iFact is 0
+    auto main_ = [&](int i) { return vertical ? R.top(i) : R.left(i); };
+    auto side = [&](int i) { return vertical ? R.left(i) : R.top(i); };
+    refb[n] = R.corner();
+    for (int i = 0; i < n; ++i) refb[n + 1 + i] = main_(i);
+    if (angle < 0) {
+        const int last = (n * angle) >> 5;
+        if (last < -1) {
+            const int inv = INV_ANGLE[mode - 11];
+            for (int k = last; k <= -1; ++k) {
+                const int idx = -1 + ((k * inv + 128) >> 8);
+                refb[n + k] = idx < 0 ? R.corner() : side(idx);
+            }
+        }
+    } else {
+        for (int i = n; i < 2 * n; ++i) refb[n + 1 + i] = main_(i);
+    }
+    for (int y = 0; y < n; ++y) {
+        const int i_idx = ((y + 1) * angle) >> 5, i_fact = ((y + 1) * angle) & 31;
+        for (int x = 0; x < n; ++x) {
+            const int a = refb[n + x + i_idx + 1], b = refb[n + x + i_idx + 2];
+            const int v = i_fact ? ((32 - i_fact) * a + i_fact * b + 16) >> 5 : a;
+            dst[vertical ? y * n + x : x * n + y] = v;
+        }
+    }
+    if (edge) {
+        const int c = R.corner();
+        if (mode == 26)
+            for (int y = 0; y < n; ++y) dst[y * n] = clampi(R.top(0) + ((R.left(y) - c) >> 1), 0, max);
+        else if (mode == 10)
+            for (int x = 0; x < n; ++x) dst[x] = clampi(R.left(0) + ((R.top(x) - c) >> 1), 0, max);
+    }
+}
+
+// MinTbAddrZs (6.5.2) at 4x4 granularity for one tile: the CTB's raster
+// address, then the z-order of the 4x4 block inside it.
+uint32_t min_tb_addr_zs(int bx, int by, int log2_ctb, int wc) {
+    const int sh = log2_ctb - 2;
+    uint32_t z = (uint32_t)((by >> sh) * wc + (bx >> sh)) << (2 * sh);
+    for (int i = 0; i < sh; ++i) {
+        const int m = 1 << i;
+        z += (m & bx ? (uint32_t)(m * m) : 0u) + (m & by ? (uint32_t)(2 * m * m) : 0u);
+    }
+    return z;
+}
+
+// The availability of a transform block's reference samples
+// (src/hevc/ctu.rs:1732-1821 with PicInfo::available_at, src/hevc/pic.rs:
+// 344-367): per 4x4 luma block, inside the picture, earlier in z-scan order
+// (one slice and one tile: everything earlier is decoded), and with
+// constrained intra prediction an intra block. avail in scan order.
+void intra_avail(const rv_hevc_pic* pic, const std::vector<uint8_t>& intra4, int comp, int x, int y, int n,
+                 bool* avail) {
+    const int sc = comp ? 1 : 0, w4 = pic->width / 4;
+    const int wc = (pic->width + (1 << pic->log2_ctb) - 1) >> pic->log2_ctb;
+    const uint32_t zc = min_tb_addr_zs((x << sc) >> 2, (y << sc) >> 2, pic->log2_ctb, wc);
+    auto check = [&](int xs, int ys) {
+        const int xl = xs << sc, yl = ys << sc;
+        if (xs < 0 || ys < 0 || xl >= pic->width || yl >= pic->height) return false;
+        const int bx = xl >> 2, by = yl >> 2;
+        if (min_tb_addr_zs(bx, by, pic->log2_ctb, wc) > zc) return false;
+        return !pic->constrained_intra_pred || intra4[(size_t)by * w4 + bx] != 0;
+    };
+    for (int j = 0; j < 2 * n; ++j) avail[j] = check(x - 1, y + 2 * n - 1 - j);
+    avail[2 * n] = check(x - 1, y - 1);
+    for (int j = 0; j < 2 * n; ++j) avail[2 * n + 1 + j] = check(x + j, y - 1);
+}
+
+// IntraPredModeY / IntraPredModeC of a TU of an RV_INTRA_DEVICE CU: the
+// luma mode of the PU holding it (NxN: the quadrant), the chroma mode.
+int tu_intra_mode(const rv_hevc_cu& cu, const rv_hevc_tu& tu) {
+    if (tu.comp) return cu.chroma_mode;
+    if (cu.part_mode != 3) return cu.luma_mode[0];
+    const int h = 1 << (cu.log2_size - 1);
+    return cu.luma_mode[(tu.y >= cu.y + h ? 2 : 0) + (tu.x >= cu.x + h ? 1 : 0)];
+}
+
+// ---------------------------------------------------------------------
 // Whole picture
 // ---------------------------------------------------------------------
 
@@ -1017,10 +1193,10 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
     if (rc) return rc;
     int max = (1 << bd) - 1;
 
-    // Intra CUs get their host-supplied prediction.
+    // RV_INTRA_HOST CUs get their host-supplied prediction.
     for (int i = 0; i < pic->n_cus; ++i) {
         const rv_hevc_cu& cu = pic->cus[i];
-        if (!cu.intra) continue;
+        if (cu.intra != RV_INTRA_HOST) continue;
         int s = 1 << cu.log2_size;
         const uint16_t* pr = pic->intra_pred + cu.pred_off;
         for (int c = 0; c < 3; ++c) {
@@ -1066,8 +1242,26 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
             predict_block(mc, cur, bd, pu.x, pu.y, pu.w, pu.h, rf, mv, wt);
         }
     });
+    // The CU of each 4x4 luma block, where a CU is predicted on the device.
+    std::vector<int32_t> cu_at;
+    for (int i = 0; i < pic->n_cus; ++i)
+        if (pic->cus[i].intra == RV_INTRA_DEVICE) {
+            if (cu_at.empty()) cu_at.assign((size_t)info.w4 * info.h4, -1);
+            const rv_hevc_cu& cu = pic->cus[i];
+            if (cu.part_mode != 0 && cu.part_mode != 3) return RV_E_RANGE;
+            for (int k = 0; k < (cu.part_mode == 3 ? 4 : 1); ++k)
+                if (cu.luma_mode[k] > 34) return RV_E_RANGE;
+            if (cu.chroma_mode > 34) return RV_E_RANGE;
+            fill4(cu_at, info.w4, cu.x, cu.y, 1 << cu.log2_size, 1 << cu.log2_size, (int32_t)i);
+        }
+    auto device_intra_cu = [&](const rv_hevc_tu& tu) -> int {
+        if (cu_at.empty()) return -1;
+        const int sc = tu.comp ? 1 : 0;
+        return cu_at[info.idx4(tu.x << sc, tu.y << sc)];
+    };
     // Residual (src/hevc/ctu.rs:2031-2088): TUs of a component do not
-    // overlap, so ranges of TUs run on threads.
+    // overlap, so ranges of TUs run on threads. The TUs of RV_INTRA_DEVICE
+    // CUs follow, in decoding order.
     if (stop_after != 1) {
         for (int i = 0; i < pic->n_tus; ++i) {
             const rv_hevc_tu& tu = pic->tus[i];
@@ -1078,31 +1272,63 @@ int reconstruct(const rv_hevc_pic* pic, const void* const* slot_planes, int n_sl
                 if (pic->nz[tu.coef_off + k].pos >= n * n) return RV_E_RANGE;
             if ((tu.flags & RV_TU_LEVELS) && tu.qp > 51 + 6 * (bd - 8)) return RV_E_RANGE;
         }
+        auto residual = [&](const rv_hevc_tu& tu, std::vector<int16_t>& coeffs) {
+            int n = 1 << tu.log2;
+            // The dense raster the reference's IdctFn takes: the nonzero
+            // list scattered into zeros.
+            std::fill(coeffs.begin(), coeffs.begin() + n * n, (int16_t)0);
+            for (uint32_t k = 0; k < tu.n_nz; ++k) {
+                const rv_nz& e = pic->nz[tu.coef_off + k];
+                coeffs[e.pos] = e.val;
+            }
+            if (tu.flags & RV_TU_LEVELS) {
+                int dc;
+                const uint8_t* list = tu_scaling(pic->scaling, tu.log2, tu.comp, (tu.flags & RV_TU_INTRA) != 0, &dc);
+                scale_coefficients(coeffs.data(), tu.log2, tu.qp, bd, list, dc, tu.max_x, tu.max_y);
+            }
+            int bd_shift = 20 - bd;
+            if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
+            else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);
+            Plane<S>& pl = cur.p[tu.comp];
+            add_residual(&pl.data[pl.offset(tu.x, tu.y)], pl.stride, coeffs.data(), n, max);
+        };
         parallel_for(pic->n_tus, threads, [&](int i0, int i1) {
             std::vector<int16_t> coeffs(32 * 32);
             for (int i = i0; i < i1; ++i) {
                 const rv_hevc_tu& tu = pic->tus[i];
-                if (!(tu.flags & RV_TU_CODED)) continue;
-                int n = 1 << tu.log2;
-                // The dense raster the reference's IdctFn takes: the nonzero
-                // list scattered into zeros.
-                std::fill(coeffs.begin(), coeffs.begin() + n * n, (int16_t)0);
-                for (uint32_t k = 0; k < tu.n_nz; ++k) {
-                    const rv_nz& e = pic->nz[tu.coef_off + k];
-                    coeffs[e.pos] = e.val;
-                }
-                if (tu.flags & RV_TU_LEVELS) {
-                    int dc;
-                    const uint8_t* list = tu_scaling(pic->scaling, tu.log2, tu.comp, (tu.flags & RV_TU_INTRA) != 0, &dc);
-                    scale_coefficients(coeffs.data(), tu.log2, tu.qp, bd, list, dc, tu.max_x, tu.max_y);
-                }
-                int bd_shift = 20 - bd;
-                if (tu.flags & RV_TU_DST) idst4(coeffs.data(), bd_shift);
-                else idct(coeffs.data(), n, bd_shift, tu.max_x, tu.max_y);
-                Plane<S>& pl = cur.p[tu.comp];
-                add_residual(&pl.data[pl.offset(tu.x, tu.y)], pl.stride, coeffs.data(), n, max);
+                if (!(tu.flags & RV_TU_CODED) || device_intra_cu(tu) >= 0) continue;
+                residual(tu, coeffs);
             }
         });
+        // Intra prediction of each TU, then its residual (src/hevc/ctu.rs:
+        // 1727-1833, 2016-2088), in list (decoding) order.
+        if (!cu_at.empty()) {
+            std::vector<int16_t> coeffs(32 * 32);
+            std::vector<int> pred(32 * 32);
+            for (int i = 0; i < pic->n_tus; ++i) {
+                const rv_hevc_tu& tu = pic->tus[i];
+                const int ci = device_intra_cu(tu);
+                if (ci < 0) continue;
+                const int n = 1 << tu.log2;
+                IntraRefs R;
+                R.n = n;
+                bool avail[129];
+                intra_avail(pic, info.pred_mode, tu.comp, tu.x, tu.y, n, avail);
+                Plane<S>& pl = cur.p[tu.comp];
+                for (int j = 0; j < 4 * n + 1; ++j) {
+                    if (!avail[j]) continue;
+                    const int xs = j < 2 * n ? tu.x - 1 : j == 2 * n ? tu.x - 1 : tu.x + (j - 2 * n - 1);
+                    const int ys = j < 2 * n ? tu.y + (2 * n - 1 - j) : tu.y - 1;
+                    R.r[j] = (uint16_t)pl.data[pl.offset(xs, ys)];
+                }
+                intra_substitute(R, avail, bd);
+                intra_pred(pred.data(), R, tu_intra_mode(pic->cus[ci], tu), tu.comp == 0,
+                           pic->strong_intra_smoothing != 0, bd);
+                for (int y = 0; y < n; ++y)
+                    for (int x = 0; x < n; ++x) pl.data[pl.offset(tu.x + x, tu.y + y)] = (S)pred[y * n + x];
+                if (tu.flags & RV_TU_CODED) residual(tu, coeffs);
+            }
+        }
     }
     if (stop_after == 0 || stop_after >= 3) {
         if (pic->deblock) {
@@ -1126,6 +1352,39 @@ void orc_hevc_idct(int16_t* coeffs, int log2, int bd_shift, int max_x, int max_y
     idct(coeffs, 1 << log2, bd_shift, max_x, max_y);
 }
 void orc_hevc_idst4(int16_t* coeffs, int bd_shift) { idst4(coeffs, bd_shift); }
+void orc_hevc_intra_tables(int32_t* angle33, int32_t* inv15) {
+    for (int i = 0; i < 33; ++i) angle33[i] = INTRA_ANGLE[i];
+    for (int i = 0; i < 15; ++i) inv15[i] = INV_ANGLE[i];
+}
+void orc_hevc_intra_substitute(uint16_t* scan, const uint8_t* avail, int n, int bit_depth) {
+    IntraRefs R;
+    R.n = n;
+    bool a[129] = {};
+    for (int j = 0; j < 4 * n + 1; ++j) {
+        R.r[j] = scan[j];
+        a[j] = avail[j] != 0;
+    }
+    intra_substitute(R, a, bit_depth);
+    for (int j = 0; j < 4 * n + 1; ++j) scan[j] = R.r[j];
+}
+void orc_hevc_intra_pred(int32_t* dst, const uint16_t* scan, int n, int mode, int luma, int strong, int bit_depth) {
+    IntraRefs R;
+    R.n = n;
+    for (int j = 0; j < 4 * n + 1; ++j) R.r[j] = scan[j];
+    intra_pred(dst, R, mode, luma != 0, strong != 0, bit_depth);
+}
+int orc_hevc_intra_avail(const rv_hevc_pic* pic, int tu_index, uint8_t* avail) {
+    PicInfo info;
+    const int rc = build_info(pic, info);
+    if (rc) return rc;
+    if (tu_index < 0 || tu_index >= pic->n_tus) return RV_E_RANGE;
+    const rv_hevc_tu& tu = pic->tus[tu_index];
+    const int n = 1 << tu.log2;
+    bool a[129];
+    intra_avail(pic, info.pred_mode, tu.comp, tu.x, tu.y, n, a);
+    for (int j = 0; j < 4 * n + 1; ++j) avail[j] = a[j] ? 1 : 0;
+    return RV_OK;
+}
 void orc_hevc_scale_coefficients(int16_t* coeffs, int log2, int qp, int bit_depth, const uint8_t* list, int dc,
                                  int max_x, int max_y) {
     scale_coefficients(coeffs, log2, qp, bit_depth, list, dc, max_x, max_y);

@@ -30,6 +30,16 @@ void orc_hevc_idst4(int16_t* coeffs, int bd_shift);
  * list (16 or 64 raster entries) and, for 16x16 / 32x32, its dc. */
 void orc_hevc_scale_coefficients(int16_t* coeffs, int log2, int qp, int bit_depth, const uint8_t* list, int dc,
                                  int max_x, int max_y);
+/* Intra prediction (src/hevc/intra.rs). A block's reference samples in
+ * substitution order: scan[0 .. 2n) = p[-1][2n-1] .. p[-1][0], scan[2n] =
+ * p[-1][-1], scan[2n+1 .. 4n] = p[0][-1] .. p[2n-1][-1]. */
+void orc_hevc_intra_tables(int32_t* angle33, int32_t* inv15);
+void orc_hevc_intra_substitute(uint16_t* scan, const uint8_t* avail, int n, int bit_depth);
+/* predict_prepared with the filter decision; luma = 1 for the luma
+ * component of 4:2:0 (smoothing and boundary filters); dst[y * n + x]. */
+void orc_hevc_intra_pred(int32_t* dst, const uint16_t* scan, int n, int mode, int luma, int strong, int bit_depth);
+/* The availability of TU tu_index's reference samples in scan order. */
+int orc_hevc_intra_avail(const rv_hevc_pic* pic, int tu_index, uint8_t* avail);
 /* The plain matrix product with the two clips (src/dsp/hevc.rs:1417-1442),
  * an independent statement the butterfly must equal. */
 void orc_hevc_idct_matrix(const int16_t* coeffs, int n, int bd_shift, int16_t* out);

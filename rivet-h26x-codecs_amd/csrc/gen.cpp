@@ -172,12 +172,25 @@ struct HevcBuilder {
         int tl = log2 > 5 ? 5 : log2;
         for (int ty = 0; ty < s; ty += 1 << tl)
             for (int tx = 0; tx < s; tx += 1 << tl) tu_tree(x + tx, y + ty, tl, leaves);
-        bool has4 = false;
-        for (size_t i = 0; i < leaves.size(); i += 3)
+        bool has4 = false, all4 = true;
+        for (size_t i = 0; i < leaves.size(); i += 3) {
             if (leaves[i + 2] == 2) has4 = true;
-        bool intra = log2 == 3 && has4 && rs.percent(c.intra_percent);
-        cu.intra = intra ? 1 : 0;
-        if (intra) {
+            else all4 = false;
+        }
+        bool intra;
+        if (c.intra_device) {
+            intra = rs.percent(c.intra_percent);
+            cu.intra = intra ? RV_INTRA_DEVICE : 0;
+        } else {
+            intra = log2 == 3 && has4 && rs.percent(c.intra_percent);
+            cu.intra = intra ? RV_INTRA_HOST : 0;
+        }
+        if (intra && c.intra_device) {
+            const bool nxn = log2 == 3 && all4;
+            cu.part_mode = nxn ? 3 : 0;
+            for (int k = 0; k < (nxn ? 4 : 1); ++k) cu.luma_mode[k] = (uint8_t)rs.uniform(0, 34);
+            cu.chroma_mode = (uint8_t)rs.uniform(0, 34);
+        } else if (intra) {
             cu.part_mode = 0;
             cu.pred_off = (uint32_t)g.intra_pred.size();
             for (int i = 0; i < s * s + 2 * (s / 2) * (s / 2); ++i) g.intra_pred.push_back((uint16_t)ri.uniform(0, max));
@@ -234,15 +247,17 @@ struct HevcBuilder {
         // Transform units: luma (all, coded or not), chroma (coded only).
         for (size_t i = 0; i < leaves.size(); i += 3) {
             int tx = leaves[i], ty = leaves[i + 1], l2 = leaves[i + 2];
+            // Device-intra CUs list every chroma TU, coded or not.
+            const bool all_c = intra && c.intra_device;
             add_tu(0, tx, ty, l2, intra && l2 == 2, true, intra, cu.qp_y);
             if (l2 > 2) {
-                add_tu(1, tx / 2, ty / 2, l2 - 1, false, false, intra, cu.qp_y);
-                add_tu(2, tx / 2, ty / 2, l2 - 1, false, false, intra, cu.qp_y);
+                add_tu(1, tx / 2, ty / 2, l2 - 1, false, all_c, intra, cu.qp_y);
+                add_tu(2, tx / 2, ty / 2, l2 - 1, false, all_c, intra, cu.qp_y);
             } else if ((tx & 7) == 0 && (ty & 7) == 0) {
                 // 4:2:0: the four 4x4 luma TUs of an 8x8 share one 4x4
                 // chroma TU per component (carried with the first).
-                add_tu(1, tx / 2, ty / 2, 2, false, false, intra, cu.qp_y);
-                add_tu(2, tx / 2, ty / 2, 2, false, false, intra, cu.qp_y);
+                add_tu(1, tx / 2, ty / 2, 2, false, all_c, intra, cu.qp_y);
+                add_tu(2, tx / 2, ty / 2, 2, false, all_c, intra, cu.qp_y);
             }
         }
     }
@@ -305,6 +320,8 @@ struct HevcBuilder {
         p.intra_pred = g.intra_pred.data();
         p.n_intra_pred = (uint32_t)g.intra_pred.size();
         p.sao = g.sao.data();
+        p.strong_intra_smoothing = c.strong_intra_smoothing;
+        p.constrained_intra_pred = c.constrained_intra_pred;
         if (c.scaling) {
             Lcg rsl(rv_gen_subseed(seed, 0, 0, SCALING));
             fill_factors(rsl, &g.hevc_scaling.list4[0][0], sizeof(g.hevc_scaling.list4));

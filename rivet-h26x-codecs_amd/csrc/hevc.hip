@@ -758,7 +758,7 @@ __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     // PU {x | y << 16}, TU {x | y << 16, comp | log2 << 8 | flags << 16}.
     const uint32_t* recs = kind == SIDE_CU ? (const uint32_t*)P.cus
                            : kind == SIDE_PU ? (const uint32_t*)P.pus : (const uint32_t*)P.ltus;
-    const int stride = kind == SIDE_CU ? 3 : kind == SIDE_PU ? 5 : 4;  // dwords per record
+    const int stride = kind == SIDE_CU ? 5 : kind == SIDE_PU ? 5 : 4;  // dwords per record
     uint32_t w0[R], w1[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -1392,6 +1392,308 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
     i -= P.tx_items[4];
     const int a = i * 64 + tid;
     if (a < (P.w4 >> 1) * (P.h4 >> 1)) area_bs(P, a);
+}
+
+// ---------------------------------------------------------------------
+// Stage 1, second kernel: intra prediction of RV_INTRA_DEVICE CUs
+// (src/hevc/intra.rs, availability as src/hevc/ctu.rs:1727-1833) with each
+// transform block's residual, in dependency levels. One workgroup per
+// picture; its waves take the TUs of a level, one TU per wave, and the
+// workgroup barrier between levels orders a TU after the TUs whose samples
+// it reads (the host derives the levels from decoding order).
+// ---------------------------------------------------------------------
+constexpr int kIntraWaves = 8;
+
+// One wave's LDS: the reference samples in substitution (scan) order —
+// s[0 .. 2n) = p[-1][2n-1] .. p[-1][0], s[2n] = p[-1][-1], s[2n+1 .. 4n] =
+// p[0][-1] .. p[2n-1][-1] — their smoothed copy, the angular ref[] (biased
+// by n), the prediction and the residual.
+struct IntraLds {
+    uint16_t s[132];
+    uint16_t f[132];
+    uint16_t ref[100];
+    uint16_t pred[32 * 32];
+    int16_t res[32 * 32];
+};
+
+// intraPredAngle, modes 2..34, and invAngle, modes 11..25 (Tables 8-4,
+// 8-5 of H.265).
+__constant__ int8_t kIntraAngle[33] = {32,  26,  21,  17,  13,  9,  5,  2,  0,  -2, -5, -9, -13, -17, -21, -26, -32,
+                                       -26, -21, -17, -13, -9,  -5, -2, 0,  2,  5,  9,  13,  17,  21,  26,  32};
+__constant__ int16_t kInvAngle[15] = {-4096, -1638, -910, -630, -482, -390, -315, -256,
+                                      -315,  -390,  -482, -630, -910, -1638, -4096};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// MinTbAddrZs (6.5.2) of 4x4 luma block (bx, by): CTB raster address, then
+// the block's z order inside the CTB (sh = log2 CTB size - 2).
+__device__ __forceinline__ uint32_t zs4(int bx, int by, int sh, int wc) {
+    const uint32_t m = (1u << sh) - 1;
+    uint32_t ix = (uint32_t)bx & m, iy = (uint32_t)by & m;
+    ix = (ix | (ix << 2)) & 0x33u, ix = (ix | (ix << 1)) & 0x55u;
+    iy = (iy | (iy << 2)) & 0x33u, iy = (iy | (iy << 1)) & 0x55u;
+    return ((uint32_t)((by >> sh) * wc + (bx >> sh)) << (2 * sh)) | ix | (iy << 1);
+}
+
+// The nearest available scan position below j (-1 if none), from the
+// wave's three availability masks.
+__device__ __forceinline__ int prev_avail(int j, uint64_t m0, uint64_t m1, uint64_t m2) {
+    const int seg = j >> 6;
+    const uint64_t below = (1ull << (j & 63)) - 1;
+    const uint64_t ms[3] = {m0, m1, m2};
+    for (int s2 = seg; s2 >= 0; --s2) {
+        const uint64_t mm = (s2 == 0 ? m0 : s2 == 1 ? m1 : m2) & (s2 == seg ? below : ~0ull);
+        if (mm) return s2 * 64 + 63 - __builtin_clzll(mm);
+    }
+    (void)ms;
+    return -1;
+}
+
+template <typename S> __device__ __forceinline__ void store_row4(S* d, const int* v, int count) {
+    // count samples (a multiple of 4) as dwords; rows start at multiples of
+    // 4 samples, so every dword is aligned.
+    if constexpr (sizeof(S) == 1) {
+        for (int k = 0; k < count; k += 4)
+            *(uint32_t*)(d + k) = (uint32_t)v[k] | (uint32_t)v[k + 1] << 8 | (uint32_t)v[k + 2] << 16 |
+                                  (uint32_t)v[k + 3] << 24;
+    } else {
+        for (int k = 0; k < count; k += 2) *(uint32_t*)(d + k) = (uint32_t)v[k] | (uint32_t)v[k + 1] << 16;
+    }
+}
+
+// One transform block: reference samples, substitution, smoothing,
+// prediction (planar, DC, angular, boundary filters), then the residual
+// (the stage-1 transform arithmetic) added once and the block stored.
+// Inlined into the kernel: its LDS stays in the LDS address space (DS
+// instructions, LDS alignment rules). Reached through a generic pointer in
+// a called function, the 16-bit LDS arrays were read with merged 4-byte
+// FLAT loads at 2-byte-aligned addresses, which the hardware refuses.
+template <typename S, int LOG2>
+__device__ __forceinline__ void intra_tu(PicRef P, const IntraTu it, int lane, IntraLds& L) {
+    constexpr int N = 1 << LOG2, M = 4 * N + 1;
+    const rv_hevc_tu tu = it.t;
+    const int mode = it.mode, c = tu.comp, sc = c ? 1 : 0, bd = P.bd, max = (1 << bd) - 1;
+    const int x0 = tu.x, y0 = tu.y, pitch = P.pitch[c];
+    S* plane = (S*)P.cur[c];
+    // 1. Availability per sample (per 4x4 luma block: in the picture,
+    // earlier in z-scan order, intra under constrained intra prediction)
+    // and the available samples.
+    const int sh = P.log2_ctb - 2;
+    const uint32_t zc = zs4((x0 << sc) >> 2, (y0 << sc) >> 2, sh, P.wc);
+    uint64_t mk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int j = lane + 64 * k;
+        const int xs = j < 2 * N + 1 ? x0 - 1 : x0 + (j - 2 * N - 1);
+        const int ys = j < 2 * N ? y0 + 2 * N - 1 - j : y0 - 1;
+        bool ok = j < M && xs >= 0 && ys >= 0;
+        if (ok) {
+            const int xl = xs << sc, yl = ys << sc;
+            ok = xl < P.w && yl < P.h;
+            if (ok) {
+                const int bx = xl >> 2, by = yl >> 2;
+                ok = !(zs4(bx, by, sh, P.wc) > zc) && (!P.cip || (P.cuinfo[(size_t)by * P.w4 + bx] & 1));
+            }
+        }
+        if (ok) L.s[j] = (uint16_t)plane[(size_t)ys * pitch + xs];
+        mk[k] = __ballot(ok);
+    }
+    wave_sync();
+    // 2. Substitution (8.4.4.2.2).
+    if (!(mk[0] | mk[1] | mk[2])) {
+        for (int j = lane; j < M; j += 64) L.s[j] = (uint16_t)(1 << (bd - 1));
+    } else {
+        const int first = mk[0] ? __builtin_ctzll(mk[0]) : mk[1] ? 64 + __builtin_ctzll(mk[1]) : 128 + __builtin_ctzll(mk[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int j = lane + 64 * k;
+            if (j < M && !((mk[k] >> lane) & 1)) {
+                const int src = prev_avail(j, mk[0], mk[1], mk[2]);
+                L.s[j] = L.s[src >= 0 ? src : first];
+            }
+        }
+    }
+    wave_sync();
+    // 3. Smoothing (8.4.4.2.3): luma only in 4:2:0; not DC, not 4x4.
+    bool smoothed = false;
+    if (c == 0 && mode != 1 && N != 4) {
+        const int dist = min(abs(mode - 26), abs(mode - 10));
+        smoothed = dist > (N == 8 ? 7 : N == 16 ? 1 : 0);
+    }
+    const uint16_t* R = L.s;
+    if (smoothed) {
+        const int cr = L.s[2 * N];
+        const bool bi = N == 32 && P.strong && abs(cr + (int)L.s[4 * N] - 2 * (int)L.s[3 * N]) < (1 << (bd - 5)) &&
+                        abs(cr + (int)L.s[0] - 2 * (int)L.s[N]) < (1 << (bd - 5));
+        for (int j = lane; j < M; j += 64) {
+            int v = L.s[j];
+            if (bi) {
+                if (j >= 1 && j < 2 * N) {  // p[-1][i], i = 2n - 1 - j <= 62
+                    const int i = 2 * N - 1 - j;
+                    v = ((63 - i) * cr + (i + 1) * (int)L.s[0] + 32) >> 6;
+                } else if (j > 2 * N && j < 4 * N) {
+                    const int i = j - 2 * N - 1;
+                    v = ((63 - i) * cr + (i + 1) * (int)L.s[4 * N] + 32) >> 6;
+                }
+            } else if (j > 0 && j < M - 1) {
+                v = ((int)L.s[j - 1] + 2 * v + (int)L.s[j + 1] + 2) >> 2;
+            }
+            L.f[j] = (uint16_t)v;
+        }
+        wave_sync();
+        R = L.f;
+    }
+    // 4. Prediction into LDS.
+    const bool edge = c == 0 && N < 32;
+    if (mode == 0) {
+        const int tn = R[3 * N + 1], ln = R[N - 1];
+        for (int o = lane; o < N * N; o += 64) {
+            const int x = o & (N - 1), y = o >> LOG2;
+            L.pred[o] = (uint16_t)(((N - 1 - x) * (int)R[2 * N - 1 - y] + (x + 1) * tn + (N - 1 - y) * (int)R[2 * N + 1 + x] +
+                                    (y + 1) * ln + N) >> (LOG2 + 1));
+        }
+    } else if (mode == 1) {
+        int v = lane < N ? (int)R[2 * N + 1 + lane] : lane < 2 * N ? (int)R[2 * N - 1 - (lane - N)] : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
+        const int dc = (v + N) >> (LOG2 + 1);
+        for (int o = lane; o < N * N; o += 64) {
+            const int x = o & (N - 1), y = o >> LOG2;
+            int p = dc;
+            if (edge) {
+                if (x == 0 && y == 0) p = ((int)R[2 * N - 1] + 2 * dc + (int)R[2 * N + 1] + 2) >> 2;
+                else if (y == 0) p = ((int)R[2 * N + 1 + x] + 3 * dc + 2) >> 2;
+                else if (x == 0) p = ((int)R[2 * N - 1 - y] + 3 * dc + 2) >> 2;
+            }
+            L.pred[o] = (uint16_t)p;
+        }
+    } else {
+        const int angle = kIntraAngle[mode - 2];
+        const bool vert = mode >= 18;
+        // ref[k] at L.ref[N + k], k = -N .. 2N (8.4.4.2.6).
+        const int last = (N * angle) >> 5;
+        const int inv = angle < 0 ? (int)kInvAngle[mode - 11] : 0;
+        for (int q = lane; q <= 3 * N; q += 64) {
+            const int k = q - N;
+            int v = 0;
+            bool set = true;
+            if (k == 0) {
+                v = R[2 * N];
+            } else if (k > 0 && (k <= N || angle >= 0)) {
+                v = vert ? R[2 * N + k] : R[2 * N - k];  // main[k - 1]
+            } else if (k < 0 && angle < 0 && last < -1 && k >= last) {
+                const int idx = -1 + ((k * inv + 128) >> 8);
+                v = idx < 0 ? R[2 * N] : vert ? R[2 * N - 1 - idx] : R[2 * N + 1 + idx];
+            } else {
+                set = false;
+            }
+            if (set) L.ref[q] = (uint16_t)v;
+        }
+        wave_sync();
+        for (int o = lane; o < N * N; o += 64) {
+            const int px = o & (N - 1), py = o >> LOG2;
+            const int kx = vert ? px : py, ky = vert ? py : px;
+            const int i_idx = ((ky + 1) * angle) >> 5, i_fact = ((ky + 1) * angle) & 31;
+            const int a = L.ref[N + kx + i_idx + 1];
+            int v = i_fact ? ((32 - i_fact) * a + i_fact * (int)L.ref[N + kx + i_idx + 2] + 16) >> 5 : a;
+            if (edge) {
+                const int cr = R[2 * N];
+                if (mode == 26 && px == 0) v = clampi((int)R[2 * N + 1] + (((int)R[2 * N - 1 - py] - cr) >> 1), 0, max);
+                else if (mode == 10 && py == 0) v = clampi((int)R[2 * N - 1] + (((int)R[2 * N + 1 + px] - cr) >> 1), 0, max);
+            }
+            L.pred[o] = (uint16_t)v;
+        }
+    }
+    wave_sync();
+    // 5. Residual (as stage 1: columns rounded by 7 and clipped, rows by
+    // 20 - BitDepth), added to the prediction; rows stored as dwords.
+    S* dst = plane + (size_t)y0 * pitch + x0;
+    if (!(tu.flags & RV_TU_CODED)) {
+        if (lane < N) {
+            int v[N];
+#pragma unroll
+            for (int x = 0; x < N; ++x) v[x] = L.pred[lane * N + x];
+            store_row4<S>(dst + (size_t)lane * pitch, v, N);
+        }
+        return;
+    }
+    for (int o = lane; o < N * N; o += 64) L.res[o] = 0;
+    wave_sync();
+    {
+        const uint32_t* z = (const uint32_t*)(P.nz + tu.coef_off);
+        const bool levels = tu.flags & RV_TU_LEVELS;
+        LevelScale scl;
+        if (levels) scl = level_scale(P, tu, LOG2);
+        for (int e = lane; e < tu.n_nz; e += 64) {
+            const uint32_t w = z[e];
+            const int pos = (int)(w & 0xFFFF) & (N * N - 1), cv = (int)(int16_t)(w >> 16);
+            L.res[pos] = levels ? scale_level<LOG2>(scl, cv, pos) : (int16_t)cv;
+        }
+    }
+    wave_sync();
+    const bool dstx = LOG2 == 2 && (tu.flags & RV_TU_DST);
+    if (lane < N) {
+        int col[N], o[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) col[j] = L.res[j * N + lane];
+        if constexpr (N == 4) {
+            if (dstx) idst1d(col, o);
+            else idct1d<4>(col, o);
+        } else {
+            idct1d<N>(col, o);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) L.res[j * N + lane] = (int16_t)clampi((o[j] + 64) >> 7, -32768, 32767);
+    }
+    wave_sync();
+    if (lane < N) {
+        const int bd_shift = 20 - bd, round2 = 1 << (bd_shift - 1);
+        int row[N], o[N];
+#pragma unroll
+        for (int x = 0; x < N; ++x) row[x] = L.res[lane * N + x];
+        if constexpr (N == 4) {
+            if (dstx) idst1d(row, o);
+            else idct1d<4>(row, o);
+        } else {
+            idct1d<N>(row, o);
+        }
+#pragma unroll
+        for (int x = 0; x < N; ++x)
+            o[x] = clampi((int)L.pred[lane * N + x] + clampi((o[x] + round2) >> bd_shift, -32768, 32767), 0, max);
+        store_row4<S>(dst + (size_t)lane * pitch, o, N);
+    }
+}
+
+template <typename S> __device__ __forceinline__ void intra_tu_any(PicRef P, const IntraTu it, int lane, IntraLds& L) {
+    switch (it.t.log2) {
+        case 2: intra_tu<S, 2>(P, it, lane, L); break;
+        case 3: intra_tu<S, 3>(P, it, lane, L); break;
+        case 4: intra_tu<S, 4>(P, it, lane, L); break;
+        default: intra_tu<S, 5>(P, it, lane, L); break;
+    }
+}
+
+__global__ void __launch_bounds__(64 * kIntraWaves) hevc_intra(BatchDev b) {
+    __shared__ IntraLds lds[kIntraWaves];
+    const int p = blockIdx.x;
+    if (p >= b.n) return;
+    PicRef P = picture(b, p);
+    const int levels = P.n_ilevels;
+    if (levels == 0) return;
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    for (int lv = 0; lv < levels; ++lv) {
+        const int k1 = P.ilevel[lv + 1];
+        for (int k = P.ilevel[lv] + wave; k < k1; k += kIntraWaves) {
+            const IntraTu it = P.itus[k];
+            if (P.bd == 8) intra_tu_any<uint8_t>(P, it, lane, lds[wave]);
+            else intra_tu_any<uint16_t>(P, it, lane, lds[wave]);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -2114,13 +2416,20 @@ const char* hevc_variant_name(int stage, int v) {
 }
 
 hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st, int variant) {
-    if (total <= 0) return hipSuccess;
     if (variant < 0 || variant >= hevc_stage_variants(stage)) return hipErrorInvalidValue;
-    const StageVariant& k = kStageTables[stage].v[variant];
-    const dim3 grid(8 * ((total + 7) / 8));  // see xcd_item()
     BatchDev arg = b;
     void* args[] = {&arg};
-    return hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
+    if (total > 0) {
+        const StageVariant& k = kStageTables[stage].v[variant];
+        const dim3 grid(8 * ((total + 7) / 8));  // see xcd_item()
+        const hipError_t e = hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    // Stage 1 ends with the intra prediction of device-intra CUs, one
+    // workgroup per picture (those without any return at once).
+    if (stage == 1 && b.has_intra && b.n > 0)
+        return hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra), dim3(b.n), dim3(64 * kIntraWaves), args, 0, st);
+    return hipSuccess;
 }
 
 }  // namespace rv

@@ -379,7 +379,9 @@ void job_release(rv_job* j) {
 
 
 // ---- HEVC job preparation -------------------------------------------
-int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
+// cu_at: the CU of each 4x4 luma block, filled (and returned non-empty)
+// only when some CU is predicted on the device.
+int validate_hevc(const rv_session* s, const rv_hevc_pic* p, std::vector<int32_t>& cu_at) {
     if (p->width != s->w || p->height != s->h || p->bit_depth != s->bd)
         return fail(RV_E_RANGE, "picture shape differs from the session");
     if (p->log2_ctb < 4 || p->log2_ctb > 6) return fail(RV_E_RANGE, "log2_ctb outside 4..6");
@@ -398,18 +400,33 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
             if (p->log2_wd[c] < 0 || p->log2_wd[c] > 14) return fail(RV_E_RANGE, "log2_wd outside 0..14");
     const int W = p->width, H = p->height;
     int64_t cu_area = 0, pred_area = 0, tu_area = 0;
+    cu_at.clear();
+    std::vector<int32_t> chroma_area;
     for (int i = 0; i < p->n_cus; ++i) {
         const rv_hevc_cu& c = p->cus[i];
         if (c.log2_size < 3 || c.log2_size > 6) return fail(RV_E_RANGE, "CU size outside 8..64");
         int sz = 1 << c.log2_size;
         if ((c.x & 7) || (c.y & 7) || c.x + sz > W || c.y + sz > H) return fail(RV_E_RANGE, "CU outside the picture");
         if (c.qp_y < -6 * (s->bd - 8) || c.qp_y > 51) return fail(RV_E_RANGE, "QpY outside -QpBdOffset..51");
+        if (c.intra > RV_INTRA_DEVICE) return fail(RV_E_RANGE, "CU intra kind");
         cu_area += (int64_t)sz * sz;
-        if (c.intra) {
+        if (c.intra == RV_INTRA_HOST) {
             if ((uint64_t)c.pred_off + (uint64_t)sz * sz * 3 / 2 > p->n_intra_pred)
                 return fail(RV_E_RANGE, "intra prediction outside intra_pred");
-            pred_area += (int64_t)sz * sz;
         }
+        if (c.intra == RV_INTRA_DEVICE) {
+            if (c.part_mode != 0 && c.part_mode != 3) return fail(RV_E_RANGE, "device-intra CU neither 2Nx2N nor NxN");
+            for (int k = 0; k < (c.part_mode == 3 ? 4 : 1); ++k)
+                if (c.luma_mode[k] > 34) return fail(RV_E_RANGE, "intra mode outside 0..34");
+            if (c.chroma_mode > 34) return fail(RV_E_RANGE, "intra mode outside 0..34");
+            if (cu_at.empty()) {
+                cu_at.assign((size_t)(W / 4) * (H / 4), -1);
+                chroma_area.assign(p->n_cus, 0);
+            }
+            for (int by = c.y / 4; by < (c.y + sz) / 4; ++by)
+                for (int bx = c.x / 4; bx < (c.x + sz) / 4; ++bx) cu_at[(size_t)by * (W / 4) + bx] = i;
+        }
+        if (c.intra) pred_area += (int64_t)sz * sz;
     }
     for (int i = 0; i < p->n_pus; ++i) {
         const rv_hevc_pu& u = p->pus[i];
@@ -449,11 +466,69 @@ int validate_hevc(const rv_session* s, const rv_hevc_pic* p) {
             }
         }
         if (t.comp == 0) tu_area += (int64_t)n * n;
+        if (!cu_at.empty()) {
+            const int sc = t.comp ? 1 : 0;
+            const int ci = cu_at[(size_t)((t.y << sc) >> 2) * (W / 4) + ((t.x << sc) >> 2)];
+            if (ci >= 0) {
+                // A TU of a device-intra CU lies inside it, an NxN CU's luma
+                // TUs inside one PU.
+                const rv_hevc_cu& c = p->cus[ci];
+                const int cs = (1 << c.log2_size) >> sc, cx = c.x >> sc, cy = c.y >> sc;
+                if (t.x < cx || t.y < cy || t.x + n > cx + cs || t.y + n > cy + cs)
+                    return fail(RV_E_RANGE, "TU across a device-intra CU");
+                if (!sc && c.part_mode == 3 && n > cs / 2) return fail(RV_E_RANGE, "luma TU across NxN intra PUs");
+                if (sc) chroma_area[ci] += n * n;
+            }
+        }
     }
     const int64_t area = (int64_t)W * H;
     if (cu_area != area || pred_area != area || tu_area != area)
         return fail(RV_E_RANGE, "CU / PU / luma TU lists do not tile the picture");
+    for (size_t i = 0; i < chroma_area.size(); ++i) {
+        const int cs = (1 << p->cus[i].log2_size) / 2;
+        if (p->cus[i].intra == RV_INTRA_DEVICE && chroma_area[i] != 2 * cs * cs)
+            return fail(RV_E_RANGE, "device-intra CU without all its chroma TUs");
+    }
     return RV_OK;
+}
+
+// Dependency levels of the TUs of device-intra CUs (decoding order = list
+// order): a TU's level is one more than the highest level among the TUs
+// whose samples its references cover (its left column and corner,
+// p[-1][-1 .. 2n-1], and its top row, p[0 .. 2n-1][-1]); samples of other
+// CUs are ready before the intra kernel (level 0). Per component, a map of
+// 4x4 cells. Returns the levels (n_levels = max + 1 of them, level 0
+// unused) per TU index, -1 for TUs of other CUs.
+int intra_levels(const rv_hevc_pic* p, const std::vector<int32_t>& cu_at, std::vector<int32_t>& lvl) {
+    const int W = p->width, H = p->height;
+    lvl.assign(p->n_tus, -1);
+    std::vector<int32_t> map[3];
+    map[0].assign((size_t)(W / 4) * (H / 4), 0);
+    map[1].assign((size_t)(W / 8) * (H / 8), 0);
+    map[2].assign(map[1].size(), 0);
+    int top = 0;
+    for (int i = 0; i < p->n_tus; ++i) {
+        const rv_hevc_tu& t = p->tus[i];
+        const int sc = t.comp ? 1 : 0;
+        if (cu_at[(size_t)((t.y << sc) >> 2) * (W / 4) + ((t.x << sc) >> 2)] < 0) continue;
+        const int n = 1 << t.log2, pw = W >> sc, ph = H >> sc, w4 = pw / 4;
+        std::vector<int32_t>& m = map[t.comp];
+        int l = 0;
+        if (t.x > 0) {
+            const int cx = (t.x - 1) >> 2;
+            for (int cy = std::max(t.y - 1, 0) >> 2; cy <= (std::min(t.y + 2 * n, ph) - 1) >> 2; ++cy)
+                l = std::max(l, m[(size_t)cy * w4 + cx]);
+        }
+        if (t.y > 0) {
+            const int cy = (t.y - 1) >> 2;
+            for (int cx = t.x >> 2; cx <= (std::min(t.x + 2 * n, pw) - 1) >> 2; ++cx) l = std::max(l, m[(size_t)cy * w4 + cx]);
+        }
+        lvl[i] = l + 1;
+        top = std::max(top, l + 1);
+        for (int cy = t.y >> 2; cy < (t.y + n) >> 2; ++cy)
+            for (int cx = t.x >> 2; cx < (t.x + n) >> 2; ++cx) m[(size_t)cy * w4 + cx] = l + 1;
+    }
+    return top;
 }
 
 
@@ -836,9 +911,28 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
         if (s->codec != RV_CODEC_HEVC) return fail(RV_E_INVALID, "not an HEVC session");
         if (out_slot < 0 || out_slot >= s->slots) return fail(RV_E_RANGE, "out_slot outside the DPB");
-        int rc = validate_hevc(s, p);
+        std::vector<int32_t> cu_at;
+        int rc = validate_hevc(s, p, cu_at);
         if (rc) return rc;
         const int W = p->width, H = p->height;
+        // Device intra: the TUs of RV_INTRA_DEVICE CUs leave the stage-1
+        // lists for the intra kernel, grouped by dependency level.
+        std::vector<int32_t> ilvl;
+        const int n_ilevels = cu_at.empty() ? 0 : intra_levels(p, cu_at, ilvl);
+        std::vector<int32_t> lvl_cnt(n_ilevels + 1, 0);
+        size_t n_itus = 0, n_inz = 0;
+        int64_t b_intra = 0;
+        const int64_t ssz = p->bit_depth > 8 ? 2 : 1;
+        for (size_t i = 0; i < ilvl.size(); ++i) {
+            if (ilvl[i] < 0) continue;
+            const rv_hevc_tu& t = p->tus[i];
+            ++lvl_cnt[ilvl[i]];
+            ++n_itus;
+            const int n = 1 << t.log2;
+            if (t.flags & RV_TU_CODED) n_inz += t.n_nz;
+            b_intra += ssz * ((int64_t)n * n + 4 * n + 1) + ((t.flags & RV_TU_CODED) ? 4 * (int64_t)t.n_nz : 0);
+        }
+        auto is_itu = [&](int i) { return !ilvl.empty() && ilvl[i] >= 0; };
         // Pass 1: counts. Every list is then written once, straight into the
         // pinned staging buffer, in the order the kernels read it:
         //  - MC: the 16-aligned core of each PU as 16x16 tiles, and the rest
@@ -884,7 +978,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         for (int i = 0; i < p->n_cus; ++i) {
             const rv_hevc_cu& c = p->cus[i];
             ++cu_cnt[(6 - c.log2_size) + 1];
-            if (c.intra) {
+            if (c.intra == RV_INTRA_HOST) {
                 ++n_intra;
                 intra_area += (int64_t)1 << (2 * c.log2_size);
             }
@@ -898,7 +992,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                 ++n_ltus;
                 ++ltu_cnt[(5 - t.log2) + 1];
             }
-            if (t.flags & RV_TU_CODED) {
+            if ((t.flags & RV_TU_CODED) && !is_itu(i)) {
                 ++cls_cnt[t.log2 - 2];
                 cls_nz[t.log2 - 2] += t.n_nz;
                 ++n_ctus;
@@ -935,7 +1029,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             // Algorithmic bytes per kernel class (mc, tx, dbk, sao).
             const int64_t ss = p->bit_depth > 8 ? 2 : 1, luma = (int64_t)W * H, planes = luma * 3 / 2;
             j->bytes[0] = ss * (b_mc + planes + intra_area * 3 / 2 * 2);
-            j->bytes[1] = 4 * (int64_t)n_nzs + 2 * ss * coded_area;
+            j->bytes[1] = 4 * (int64_t)n_nzs + 2 * ss * coded_area + b_intra;
             j->bytes[2] = p->deblock ? 2 * (2 * ss * (luma + (n_intra ? luma / 2 : 0))) : 0;
             j->bytes[3] = 2 * ss * planes;
         }
@@ -944,7 +1038,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
                     carve_size<rv::SideClass>(n_sclass) +
-                    carve_size<rv_nz>(n_nzs) +
+                    carve_size<rv_nz>(n_nzs + n_inz) + carve_size<rv::IntraTu>(n_itus) +
+                    carve_size<int32_t>(n_ilevels + 1) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) + carve_size<rv_hevc_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -969,7 +1064,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv::McTile* h_tiles16 = carve<rv::McTile>(hp, n_tiles16);
         rv::SideClass* h_sclass = carve<rv::SideClass>(hp, n_sclass);
         if (n_sclass) std::memcpy(h_sclass, sclass, n_sclass * sizeof(rv::SideClass));
-        rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs);
+        rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs + n_inz);
+        rv::IntraTu* h_itus = carve<rv::IntraTu>(hp, n_itus);
+        int32_t* h_ilevel = carve<int32_t>(hp, n_ilevels + 1);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
         int32_t* h_intra = carve<int32_t>(hp, n_intra);
@@ -1007,15 +1104,45 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             const rv_hevc_cu& c = p->cus[i];
             const int at = cu_cnt[6 - c.log2_size]++;
             h_cus[at] = c;
-            if (c.intra) h_intra[ni++] = at;
+            if (c.intra == RV_INTRA_HOST) h_intra[ni++] = at;
         }
         size_t cls_at[4], nz_at[4];
         for (int k = 0, a = 0; k < 4; a += cls_cnt[k], ++k) cls_at[k] = (size_t)a;
         for (int k = 0; k < 4; ++k) nz_at[k] = k ? nz_at[k - 1] + cls_nz[k - 1] : 0;
+        // Device-intra TUs by level (list order within a level), their
+        // coefficients after the stage-1 ones.
+        if (n_ilevels) {
+            // Levels run from 1: level l fills [h_ilevel[l - 1], h_ilevel[l]).
+            h_ilevel[0] = 0;
+            for (int l = 1; l <= n_ilevels; ++l) h_ilevel[l] = h_ilevel[l - 1] + lvl_cnt[l];
+            std::vector<int32_t> put(h_ilevel, h_ilevel + n_ilevels + 1);
+            size_t nz_at = n_nzs;
+            for (int i = 0; i < p->n_tus; ++i) {
+                if (!is_itu(i)) continue;
+                const rv_hevc_tu& t = p->tus[i];
+                rv::IntraTu& it = h_itus[put[ilvl[i] - 1]++];
+                std::memset(&it, 0, sizeof(it));
+                it.t = t;
+                const int sc = t.comp ? 1 : 0;
+                const rv_hevc_cu& cu = p->cus[cu_at[(size_t)((t.y << sc) >> 2) * (W / 4) + ((t.x << sc) >> 2)]];
+                if (t.comp) {
+                    it.mode = cu.chroma_mode;
+                } else {
+                    const int h = 1 << (cu.log2_size - 1);
+                    it.mode = cu.part_mode == 3 ? cu.luma_mode[(t.y >= cu.y + h ? 2 : 0) + (t.x >= cu.x + h ? 1 : 0)]
+                                                : cu.luma_mode[0];
+                }
+                if (t.flags & RV_TU_CODED) {
+                    it.t.coef_off = (uint32_t)nz_at;
+                    std::memcpy(h_coefs + nz_at, p->nz + t.coef_off, t.n_nz * sizeof(rv_nz));
+                    nz_at += t.n_nz;
+                }
+            }
+        }
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
             if (t.comp == 0) h_ltus[ltu_cnt[5 - t.log2]++] = t;
-            if (t.flags & RV_TU_CODED) {
+            if ((t.flags & RV_TU_CODED) && !is_itu(i)) {
                 const int k = t.log2 - 2;
                 rv_hevc_tu& c = h_ctus[cls_at[k]++];
                 c = t;
@@ -1092,6 +1219,11 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.sao = (const rv_sao*)dev((char*)h_sao);
         d.scaling = p->scaling ? (const rv_hevc_scaling*)dev((char*)h_scaling) : nullptr;
+        d.n_ilevels = n_ilevels;
+        d.ilevel = (const int32_t*)dev((char*)h_ilevel);
+        d.itus = (const rv::IntraTu*)dev((char*)h_itus);
+        d.strong = p->strong_intra_smoothing != 0;
+        d.cip = p->constrained_intra_pred != 0;
         d.cuinfo = (uint8_t*)sp;
         sp += carve_size<uint8_t>(n4);
         d.tuinfo = (uint8_t*)sp;
@@ -1112,6 +1244,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.n = 1;
         j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
         j->single.h264 = nullptr;
+        j->single.has_intra = n_ilevels > 0;
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
         // The per-4x4 maps need no clearing: for work lists whose CUs, PUs
         // and luma TUs tile the picture (checked by area above) stage 0
@@ -1370,6 +1503,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->bh.n = nh;
         b->bh.hevc = (const rv::HevcPicDev* const*)dev((char*)ph);
         b->bh.prefix = (const int32_t*)dev((char*)preh);
+        b->bh.has_intra = 0;
+        for (int i = 0; i < nh; ++i) b->bh.has_intra |= b->hevc[i]->hd.n_ilevels > 0;
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);

@@ -23,6 +23,15 @@ struct alignas(16) McTile {
 };
 static_assert(sizeof(McTile) == 16, "one 16-byte load per tile");
 
+// One transform block of an RV_INTRA_DEVICE CU: the TU (coef_off rebased
+// into the job's nonzero list) and its intra prediction mode.
+struct alignas(16) IntraTu {
+    rv_hevc_tu t;
+    int32_t mode;  // IntraPredModeY of its PU, or IntraPredModeC
+    int32_t pad[3];
+};
+static_assert(sizeof(IntraTu) == 32, "two 16-byte loads per intra TU");
+
 // Side data (the per-4x4 maps) is written per cell: the CUs, PUs and luma
 // TUs of one shape form a class (consecutive in their sorted lists), and
 // one work item covers kSideCellsPerItem consecutive cells of a class,
@@ -72,6 +81,12 @@ struct HevcPicDev {
     const uint16_t* intra_pred;
     const rv_sao* sao;
     const rv_hevc_scaling* scaling;  // levels' scaling lists; null = flat 16
+    // Intra prediction on the device: the TUs of RV_INTRA_DEVICE CUs by
+    // dependency level; level k holds itus[ilevel[k] .. ilevel[k + 1]).
+    int32_t n_ilevels;
+    const int32_t* ilevel;
+    const IntraTu* itus;
+    int32_t strong, cip;  // strong intra smoothing, constrained intra pred
     // Per-4x4 side data (job scratch).
     uint8_t* cuinfo;   // bit 0 intra, bit 1 CU left edge, bit 2 CU top edge
     int8_t* qp;        // QpY
@@ -195,6 +210,7 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
+    int32_t has_intra;              // some HEVC picture has device-intra TUs
 };
 
 }  // namespace rv

@@ -27,6 +27,8 @@ RV_TU_CODED = 2
 RV_TU_LEVELS = 4
 RV_TU_INTRA = 8
 RV_BLK_LEVELS = 1
+RV_INTRA_HOST = 1
+RV_INTRA_DEVICE = 2
 RV_CODEC_HEVC = 0
 RV_CODEC_H264 = 1
 RV_H264_DEFAULT = 0
@@ -53,7 +55,7 @@ class Sao(C.Structure):
 
 class HevcCu(C.Structure):
     _fields_ = [("x", u16), ("y", u16), ("log2_size", u8), ("qp_y", i8), ("intra", u8), ("part_mode", u8),
-                ("pred_off", u32)]
+                ("pred_off", u32), ("luma_mode", u8 * 4), ("chroma_mode", u8), ("pad", u8 * 3)]
 
 
 class HevcPu(C.Structure):
@@ -86,6 +88,7 @@ class HevcPic(C.Structure):
         ("cus", C.POINTER(HevcCu)), ("pus", C.POINTER(HevcPu)), ("tus", C.POINTER(HevcTu)),
         ("nz", C.POINTER(Nz)), ("n_nz", u32), ("n_intra_pred", u32),
         ("intra_pred", C.POINTER(u16)), ("sao", C.POINTER(Sao)), ("scaling", C.POINTER(HevcScaling)),
+        ("strong_intra_smoothing", i32), ("constrained_intra_pred", i32),
     ]
 
 
@@ -126,7 +129,10 @@ class GenHevcCfg(C.Structure):
     _fields_ = [(n, i32) for n in ("width", "height", "bit_depth", "b_picture", "bi_percent", "explicit_wp",
                                    "intra_percent", "nz_permille", "lap_b", "qp_min", "qp_max", "mv_range",
                                    "deblock", "sao", "poc")] + [("ref_poc", i32 * 2), ("ref_slot", i32 * 2),
-                                                                ("levels", i32), ("scaling", i32)]
+                                                                ("levels", i32), ("scaling", i32),
+                                                                ("intra_device", i32),
+                                                                ("strong_intra_smoothing", i32),
+                                                                ("constrained_intra_pred", i32)]
 
 
 class GenH264Cfg(C.Structure):
