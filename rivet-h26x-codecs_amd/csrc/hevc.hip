@@ -1395,25 +1395,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
 }
 
 // ---------------------------------------------------------------------
-// Stage 1, second kernel: intra prediction of RV_INTRA_DEVICE CUs
-// (src/hevc/intra.rs, availability as src/hevc/ctu.rs:1727-1833) with each
-// transform block's residual, in dependency levels. One workgroup per
-// picture; its waves take the TUs of a level, one TU per wave, and the
-// workgroup barrier between levels orders a TU after the TUs whose samples
-// it reads (the host derives the levels from decoding order).
+// Stage 1, after the transform kernel: intra prediction of RV_INTRA_DEVICE
+// CUs (src/hevc/intra.rs, availability as src/hevc/ctu.rs:1727-1833) with
+// each transform block's residual, one launch per dependency level (the
+// host derives the levels from decoding order).
 // ---------------------------------------------------------------------
-constexpr int kIntraWaves = 8;
-
 // One wave's LDS: the reference samples in substitution (scan) order —
 // s[0 .. 2n) = p[-1][2n-1] .. p[-1][0], s[2n] = p[-1][-1], s[2n+1 .. 4n] =
 // p[0][-1] .. p[2n-1][-1] — their smoothed copy, the angular ref[] (biased
 // by n), the prediction and the residual.
-struct IntraLds {
-    uint16_t s[132];
-    uint16_t f[132];
-    uint16_t ref[100];
-    uint16_t pred[32 * 32];
-    int16_t res[32 * 32];
+// Every array 16-byte aligned, so that the wide LDS reads the compiler
+// forms from consecutive elements are naturally aligned.
+struct alignas(16) IntraLds {
+    alignas(16) uint16_t s[136];
+    alignas(16) uint16_t f[136];
+    alignas(16) uint16_t ref[104];
+    alignas(16) uint16_t pred[32 * 32];
+    alignas(16) int16_t res[32 * 32];
 };
 
 // intraPredAngle, modes 2..34, and invAngle, modes 11..25 (Tables 8-4,
@@ -1677,23 +1675,20 @@ template <typename S> __device__ __forceinline__ void intra_tu_any(PicRef P, con
     }
 }
 
-__global__ void __launch_bounds__(64 * kIntraWaves) hevc_intra(BatchDev b) {
-    __shared__ IntraLds lds[kIntraWaves];
-    const int p = blockIdx.x;
-    if (p >= b.n) return;
+// One dependency level over the batch: one TU per 64-thread workgroup.
+// The kernel boundary between levels orders each TU after the TUs whose
+// samples it reads.
+__global__ void __launch_bounds__(64) hevc_intra_level(BatchDev b, int level) {
+    __shared__ IntraLds lds;
+    ConstI32* pre = (ConstI32*)(b.ipre + (size_t)level * (b.n + 1));
+    const int it = blockIdx.x;
+    if (it >= pre[b.n]) return;
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
     PicRef P = picture(b, p);
-    const int levels = P.n_ilevels;
-    if (levels == 0) return;
-    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-    for (int lv = 0; lv < levels; ++lv) {
-        const int k1 = P.ilevel[lv + 1];
-        for (int k = P.ilevel[lv] + wave; k < k1; k += kIntraWaves) {
-            const IntraTu it = P.itus[k];
-            if (P.bd == 8) intra_tu_any<uint8_t>(P, it, lane, lds[wave]);
-            else intra_tu_any<uint16_t>(P, it, lane, lds[wave]);
-        }
-        __syncthreads();
-    }
+    const int k = __builtin_amdgcn_readfirstlane(P.ilevel[level] + it - pre[p]);
+    const IntraTu t = P.itus[k];
+    if (P.bd == 8) intra_tu_any<uint8_t>(P, t, (int)threadIdx.x, lds);
+    else intra_tu_any<uint16_t>(P, t, (int)threadIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------
@@ -2425,10 +2420,17 @@ hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         const hipError_t e = hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
         if (e != hipSuccess) return e;
     }
-    // Stage 1 ends with the intra prediction of device-intra CUs, one
-    // workgroup per picture (those without any return at once).
-    if (stage == 1 && b.has_intra && b.n > 0)
-        return hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra), dim3(b.n), dim3(64 * kIntraWaves), args, 0, st);
+    // Stage 1 ends with the intra prediction of device-intra CUs: one
+    // launch per dependency level, one TU per workgroup.
+    if (stage == 1)
+        for (int l = 0; l < b.ilevels; ++l) {
+            if (b.itotal[l] <= 0) continue;
+            int lv = l;
+            void* largs[] = {&arg, &lv};
+            const hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra_level), dim3(b.itotal[l]),
+                                                 dim3(64), largs, 0, st);
+            if (e != hipSuccess) return e;
+        }
     return hipSuccess;
 }
 

@@ -299,6 +299,7 @@ struct rv_job {
     void* desc_dev = nullptr;  // device copy of the descriptor
     // Single-picture tables for the plain path.
     rv::BatchDev single{};
+    std::vector<int32_t> itotal;  // device-intra TUs per level (host copy)
     int64_t bytes[4] = {0, 0, 0, 0};
 };
 
@@ -310,6 +311,7 @@ struct rv_batch {
     bool var_set = false;            // chosen by rv_batch_set_variant / rv_batch_tune
     rv::BatchDev bh{}, b4{};
     std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
+    std::vector<int32_t> itotal;        // device-intra TUs per level
     DevBuf tables;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1039,7 +1041,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
                     carve_size<rv::SideClass>(n_sclass) +
                     carve_size<rv_nz>(n_nzs + n_inz) + carve_size<rv::IntraTu>(n_itus) +
-                    carve_size<int32_t>(n_ilevels + 1) +
+                    carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)n_ilevels) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) + carve_size<rv_hevc_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -1067,6 +1069,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs + n_inz);
         rv::IntraTu* h_itus = carve<rv::IntraTu>(hp, n_itus);
         int32_t* h_ilevel = carve<int32_t>(hp, n_ilevels + 1);
+        int32_t* h_ipre = carve<int32_t>(hp, 2 * (size_t)n_ilevels);  // per level {0, count}
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
         int32_t* h_intra = carve<int32_t>(hp, n_intra);
@@ -1115,6 +1118,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             // Levels run from 1: level l fills [h_ilevel[l - 1], h_ilevel[l]).
             h_ilevel[0] = 0;
             for (int l = 1; l <= n_ilevels; ++l) h_ilevel[l] = h_ilevel[l - 1] + lvl_cnt[l];
+            for (int l = 0; l < n_ilevels; ++l) {
+                h_ipre[2 * l] = 0;
+                h_ipre[2 * l + 1] = lvl_cnt[l + 1];
+            }
             std::vector<int32_t> put(h_ilevel, h_ilevel + n_ilevels + 1);
             size_t nz_at = n_nzs;
             for (int i = 0; i < p->n_tus; ++i) {
@@ -1244,7 +1251,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.n = 1;
         j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
         j->single.h264 = nullptr;
-        j->single.has_intra = n_ilevels > 0;
+        j->itotal.assign(lvl_cnt.begin() + (n_ilevels ? 1 : 0), lvl_cnt.end());
+        j->single.ilevels = n_ilevels;
+        j->single.ipre = (const int32_t*)dev((char*)h_ipre);
+        j->single.itotal = j->itotal.data();
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
         // The per-4x4 maps need no clearing: for work lists whose CUs, PUs
         // and luma TUs tile the picture (checked by area above) stage 0
@@ -1481,8 +1491,21 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
             for (int i = 0; i < n4; ++i)
                 b->pre_4[(size_t)s * (n4 + 1) + i + 1] = b->pre_4[(size_t)s * (n4 + 1) + i] + b->h264[i]->h4.items[s];
         }
+        // Device-intra levels: per level, the prefix over the HEVC pictures
+        // of their TU counts.
+        int ilevels = 0;
+        for (int i = 0; i < nh; ++i) ilevels = std::max(ilevels, b->hevc[i]->hd.n_ilevels);
+        std::vector<int32_t> ipre((size_t)ilevels * (nh + 1), 0);
+        b->itotal.assign(ilevels, 0);
+        for (int l = 0; l < ilevels; ++l) {
+            for (int i = 0; i < nh; ++i) {
+                const std::vector<int32_t>& t = b->hevc[i]->itotal;
+                ipre[(size_t)l * (nh + 1) + i + 1] = ipre[(size_t)l * (nh + 1) + i] + (l < (int)t.size() ? t[l] : 0);
+            }
+            b->itotal[l] = ipre[(size_t)l * (nh + 1) + nh];
+        }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
-                    carve_size<int32_t>(b->pre_4.size());
+                    carve_size<int32_t>(b->pre_4.size()) + carve_size<int32_t>(ipre.size());
         std::vector<uint32_t> host(sz / 4 + 1, 0u);
         char* hbase = (char*)host.data();
         if (!pool_get(sz, &b->tables)) {
@@ -1496,6 +1519,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         void** p4 = carve<void*>(hp, n4);
         int32_t* preh = carve<int32_t>(hp, b->pre_h.size());
         int32_t* pre4 = carve<int32_t>(hp, b->pre_4.size());
+        int32_t* hipre = carve<int32_t>(hp, ipre.size());
+        if (!ipre.empty()) std::memcpy(hipre, ipre.data(), ipre.size() * 4);
         for (int i = 0; i < nh; ++i) ph[i] = b->hevc[i]->desc_dev;
         for (int i = 0; i < n4; ++i) p4[i] = b->h264[i]->desc_dev;
         std::memcpy(preh, b->pre_h.data(), b->pre_h.size() * 4);
@@ -1503,8 +1528,9 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->bh.n = nh;
         b->bh.hevc = (const rv::HevcPicDev* const*)dev((char*)ph);
         b->bh.prefix = (const int32_t*)dev((char*)preh);
-        b->bh.has_intra = 0;
-        for (int i = 0; i < nh; ++i) b->bh.has_intra |= b->hevc[i]->hd.n_ilevels > 0;
+        b->bh.ilevels = ilevels;
+        b->bh.ipre = (const int32_t*)dev((char*)hipre);
+        b->bh.itotal = b->itotal.data();
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);

@@ -210,7 +210,13 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
-    int32_t has_intra;              // some HEVC picture has device-intra TUs
+    // Device intra (HEVC): the most dependency levels of any picture; per
+    // level, the prefix over pictures of their TU counts at that level
+    // (device memory, levels x (n + 1)) and the level's total (host
+    // memory: read by the launcher only).
+    int32_t ilevels;
+    const int32_t* ipre;
+    const int32_t* itotal;
 };
 
 }  // namespace rv
