@@ -1396,21 +1396,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
 
 // ---------------------------------------------------------------------
 // Stage 1, after the transform kernel: intra prediction of RV_INTRA_DEVICE
-// CUs (src/hevc/intra.rs, availability as src/hevc/ctu.rs:1727-1833) with
-// each transform block's residual, one launch per dependency level (the
-// host derives the levels from decoding order).
+// CUs (src/hevc/intra.rs, availability as src/hevc/ctu.rs:1727-1833). A
+// preparation kernel computes the residual of every TU at once (no
+// residual depends on predicted samples) and files each TU under its
+// dependency level; then one launch per level (the host derives the levels
+// from decoding order) predicts the level's TUs and adds their residuals.
 // ---------------------------------------------------------------------
-// One wave's LDS: the reference samples in substitution (scan) order —
-// s[0 .. 2n) = p[-1][2n-1] .. p[-1][0], s[2n] = p[-1][-1], s[2n+1 .. 4n] =
-// p[0][-1] .. p[2n-1][-1] — their smoothed copy, the angular ref[] (biased
-// by n), the prediction and the residual.
-// Every array 16-byte aligned, so that the wide LDS reads the compiler
-// forms from consecutive elements are naturally aligned.
+// One wave's reference samples in LDS, in substitution (scan) order from
+// s[7] on — scan index j: 0 .. 2n-1 = p[-1][2n-1] .. p[-1][0], 2n =
+// p[-1][-1], 2n+1 .. 4n = p[0][-1] .. p[2n-1][-1] — their smoothed copy
+// (likewise) and the angular ref[]
+// (biased by n). Every array 16-byte aligned, so that the wide LDS reads
+// the compiler forms from consecutive elements are naturally aligned.
 struct alignas(16) IntraLds {
     alignas(16) uint16_t s[136];
     alignas(16) uint16_t f[136];
     alignas(16) uint16_t ref[104];
-    alignas(16) uint16_t pred[32 * 32];
+};
+struct alignas(16) ResidualLds {
     alignas(16) int16_t res[32 * 32];
 };
 
@@ -1463,162 +1466,13 @@ template <typename S> __device__ __forceinline__ void store_row4(S* d, const int
     }
 }
 
-// One transform block: reference samples, substitution, smoothing,
-// prediction (planar, DC, angular, boundary filters), then the residual
-// (the stage-1 transform arithmetic) added once and the block stored.
-// Inlined into the kernel: its LDS stays in the LDS address space (DS
-// instructions, LDS alignment rules). Reached through a generic pointer in
-// a called function, the 16-bit LDS arrays were read with merged 4-byte
-// FLAT loads at 2-byte-aligned addresses, which the hardware refuses.
-template <typename S, int LOG2>
-__device__ __forceinline__ void intra_tu(PicRef P, const IntraTu it, int lane, IntraLds& L) {
-    constexpr int N = 1 << LOG2, M = 4 * N + 1;
+// The residual of one coded transform block, by the stage-1 arithmetic
+// (levels scaled, columns rounded by 7 and clipped, rows by 20 - BitDepth
+// and clipped to 16 bits), stored row-major at ires + res.
+template <int LOG2>
+__device__ __forceinline__ void intra_residual(PicRef P, const IntraTu it, int lane, ResidualLds& L) {
+    constexpr int N = 1 << LOG2;
     const rv_hevc_tu tu = it.t;
-    const int mode = it.mode, c = tu.comp, sc = c ? 1 : 0, bd = P.bd, max = (1 << bd) - 1;
-    const int x0 = tu.x, y0 = tu.y, pitch = P.pitch[c];
-    S* plane = (S*)P.cur[c];
-    // 1. Availability per sample (per 4x4 luma block: in the picture,
-    // earlier in z-scan order, intra under constrained intra prediction)
-    // and the available samples.
-    const int sh = P.log2_ctb - 2;
-    const uint32_t zc = zs4((x0 << sc) >> 2, (y0 << sc) >> 2, sh, P.wc);
-    uint64_t mk[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int j = lane + 64 * k;
-        const int xs = j < 2 * N + 1 ? x0 - 1 : x0 + (j - 2 * N - 1);
-        const int ys = j < 2 * N ? y0 + 2 * N - 1 - j : y0 - 1;
-        bool ok = j < M && xs >= 0 && ys >= 0;
-        if (ok) {
-            const int xl = xs << sc, yl = ys << sc;
-            ok = xl < P.w && yl < P.h;
-            if (ok) {
-                const int bx = xl >> 2, by = yl >> 2;
-                ok = !(zs4(bx, by, sh, P.wc) > zc) && (!P.cip || (P.cuinfo[(size_t)by * P.w4 + bx] & 1));
-            }
-        }
-        if (ok) L.s[j] = (uint16_t)plane[(size_t)ys * pitch + xs];
-        mk[k] = __ballot(ok);
-    }
-    wave_sync();
-    // 2. Substitution (8.4.4.2.2).
-    if (!(mk[0] | mk[1] | mk[2])) {
-        for (int j = lane; j < M; j += 64) L.s[j] = (uint16_t)(1 << (bd - 1));
-    } else {
-        const int first = mk[0] ? __builtin_ctzll(mk[0]) : mk[1] ? 64 + __builtin_ctzll(mk[1]) : 128 + __builtin_ctzll(mk[2]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int j = lane + 64 * k;
-            if (j < M && !((mk[k] >> lane) & 1)) {
-                const int src = prev_avail(j, mk[0], mk[1], mk[2]);
-                L.s[j] = L.s[src >= 0 ? src : first];
-            }
-        }
-    }
-    wave_sync();
-    // 3. Smoothing (8.4.4.2.3): luma only in 4:2:0; not DC, not 4x4.
-    bool smoothed = false;
-    if (c == 0 && mode != 1 && N != 4) {
-        const int dist = min(abs(mode - 26), abs(mode - 10));
-        smoothed = dist > (N == 8 ? 7 : N == 16 ? 1 : 0);
-    }
-    const uint16_t* R = L.s;
-    if (smoothed) {
-        const int cr = L.s[2 * N];
-        const bool bi = N == 32 && P.strong && abs(cr + (int)L.s[4 * N] - 2 * (int)L.s[3 * N]) < (1 << (bd - 5)) &&
-                        abs(cr + (int)L.s[0] - 2 * (int)L.s[N]) < (1 << (bd - 5));
-        for (int j = lane; j < M; j += 64) {
-            int v = L.s[j];
-            if (bi) {
-                if (j >= 1 && j < 2 * N) {  // p[-1][i], i = 2n - 1 - j <= 62
-                    const int i = 2 * N - 1 - j;
-                    v = ((63 - i) * cr + (i + 1) * (int)L.s[0] + 32) >> 6;
-                } else if (j > 2 * N && j < 4 * N) {
-                    const int i = j - 2 * N - 1;
-                    v = ((63 - i) * cr + (i + 1) * (int)L.s[4 * N] + 32) >> 6;
-                }
-            } else if (j > 0 && j < M - 1) {
-                v = ((int)L.s[j - 1] + 2 * v + (int)L.s[j + 1] + 2) >> 2;
-            }
-            L.f[j] = (uint16_t)v;
-        }
-        wave_sync();
-        R = L.f;
-    }
-    // 4. Prediction into LDS.
-    const bool edge = c == 0 && N < 32;
-    if (mode == 0) {
-        const int tn = R[3 * N + 1], ln = R[N - 1];
-        for (int o = lane; o < N * N; o += 64) {
-            const int x = o & (N - 1), y = o >> LOG2;
-            L.pred[o] = (uint16_t)(((N - 1 - x) * (int)R[2 * N - 1 - y] + (x + 1) * tn + (N - 1 - y) * (int)R[2 * N + 1 + x] +
-                                    (y + 1) * ln + N) >> (LOG2 + 1));
-        }
-    } else if (mode == 1) {
-        int v = lane < N ? (int)R[2 * N + 1 + lane] : lane < 2 * N ? (int)R[2 * N - 1 - (lane - N)] : 0;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
-        const int dc = (v + N) >> (LOG2 + 1);
-        for (int o = lane; o < N * N; o += 64) {
-            const int x = o & (N - 1), y = o >> LOG2;
-            int p = dc;
-            if (edge) {
-                if (x == 0 && y == 0) p = ((int)R[2 * N - 1] + 2 * dc + (int)R[2 * N + 1] + 2) >> 2;
-                else if (y == 0) p = ((int)R[2 * N + 1 + x] + 3 * dc + 2) >> 2;
-                else if (x == 0) p = ((int)R[2 * N - 1 - y] + 3 * dc + 2) >> 2;
-            }
-            L.pred[o] = (uint16_t)p;
-        }
-    } else {
-        const int angle = kIntraAngle[mode - 2];
-        const bool vert = mode >= 18;
-        // ref[k] at L.ref[N + k], k = -N .. 2N (8.4.4.2.6).
-        const int last = (N * angle) >> 5;
-        const int inv = angle < 0 ? (int)kInvAngle[mode - 11] : 0;
-        for (int q = lane; q <= 3 * N; q += 64) {
-            const int k = q - N;
-            int v = 0;
-            bool set = true;
-            if (k == 0) {
-                v = R[2 * N];
-            } else if (k > 0 && (k <= N || angle >= 0)) {
-                v = vert ? R[2 * N + k] : R[2 * N - k];  // main[k - 1]
-            } else if (k < 0 && angle < 0 && last < -1 && k >= last) {
-                const int idx = -1 + ((k * inv + 128) >> 8);
-                v = idx < 0 ? R[2 * N] : vert ? R[2 * N - 1 - idx] : R[2 * N + 1 + idx];
-            } else {
-                set = false;
-            }
-            if (set) L.ref[q] = (uint16_t)v;
-        }
-        wave_sync();
-        for (int o = lane; o < N * N; o += 64) {
-            const int px = o & (N - 1), py = o >> LOG2;
-            const int kx = vert ? px : py, ky = vert ? py : px;
-            const int i_idx = ((ky + 1) * angle) >> 5, i_fact = ((ky + 1) * angle) & 31;
-            const int a = L.ref[N + kx + i_idx + 1];
-            int v = i_fact ? ((32 - i_fact) * a + i_fact * (int)L.ref[N + kx + i_idx + 2] + 16) >> 5 : a;
-            if (edge) {
-                const int cr = R[2 * N];
-                if (mode == 26 && px == 0) v = clampi((int)R[2 * N + 1] + (((int)R[2 * N - 1 - py] - cr) >> 1), 0, max);
-                else if (mode == 10 && py == 0) v = clampi((int)R[2 * N - 1] + (((int)R[2 * N + 1 + px] - cr) >> 1), 0, max);
-            }
-            L.pred[o] = (uint16_t)v;
-        }
-    }
-    wave_sync();
-    // 5. Residual (as stage 1: columns rounded by 7 and clipped, rows by
-    // 20 - BitDepth), added to the prediction; rows stored as dwords.
-    S* dst = plane + (size_t)y0 * pitch + x0;
-    if (!(tu.flags & RV_TU_CODED)) {
-        if (lane < N) {
-            int v[N];
-#pragma unroll
-            for (int x = 0; x < N; ++x) v[x] = L.pred[lane * N + x];
-            store_row4<S>(dst + (size_t)lane * pitch, v, N);
-        }
-        return;
-    }
     for (int o = lane; o < N * N; o += 64) L.res[o] = 0;
     wave_sync();
     {
@@ -1649,7 +1503,7 @@ __device__ __forceinline__ void intra_tu(PicRef P, const IntraTu it, int lane, I
     }
     wave_sync();
     if (lane < N) {
-        const int bd_shift = 20 - bd, round2 = 1 << (bd_shift - 1);
+        const int bd_shift = 20 - P.bd, round2 = 1 << (bd_shift - 1);
         int row[N], o[N];
 #pragma unroll
         for (int x = 0; x < N; ++x) row[x] = L.res[lane * N + x];
@@ -1659,11 +1513,208 @@ __device__ __forceinline__ void intra_tu(PicRef P, const IntraTu it, int lane, I
         } else {
             idct1d<N>(row, o);
         }
+        // Row starts are multiples of 4 samples: whole dwords.
+        uint32_t* d = (uint32_t*)(P.ires + it.res + lane * N);
 #pragma unroll
-        for (int x = 0; x < N; ++x)
-            o[x] = clampi((int)L.pred[lane * N + x] + clampi((o[x] + round2) >> bd_shift, -32768, 32767), 0, max);
-        store_row4<S>(dst + (size_t)lane * pitch, o, N);
+        for (int x = 0; x < N; x += 2) {
+            const int u = clampi((o[x] + round2) >> bd_shift, -32768, 32767);
+            const int v = clampi((o[x + 1] + round2) >> bd_shift, -32768, 32767);
+            d[x >> 1] = (uint32_t)(u & 0xFFFF) | ((uint32_t)v << 16);
+        }
     }
+}
+
+// One transform block: reference samples, substitution, smoothing, then
+// per lane one run of a row predicted (planar, DC, angular, boundary
+// filters), its residual added and the run stored.
+// Inlined into the kernel: its LDS stays in the LDS address space (DS
+// instructions, LDS alignment rules). Reached through a generic pointer in
+// a called function, the 16-bit LDS arrays were read with merged 4-byte
+// FLAT loads at 2-byte-aligned addresses, which the hardware refuses.
+template <typename S, int LOG2>
+__device__ __forceinline__ void intra_tu(PicRef P, const IntraTu it, int lane, IntraLds& L) {
+    constexpr int N = 1 << LOG2, M = 4 * N + 1;
+    // A lane's run: RUN samples of one row (a multiple of 4: whole dwords);
+    // LANES lanes cover the block.
+    constexpr int RUN = N * N / 64 < 4 ? 4 : N * N / 64, PER_ROW = N / RUN, LANES = N * N / RUN;
+    const rv_hevc_tu tu = it.t;
+    const int mode = it.mode, c = tu.comp, sc = c ? 1 : 0, bd = P.bd, max = (1 << bd) - 1;
+    const int x0 = tu.x, y0 = tu.y, pitch = P.pitch[c];
+    S* plane = (S*)P.cur[c];
+    // The scan arrays start 7 samples in: the top row, p[x][-1] at scan
+    // index 2n + 1 + x, then lies on 16-byte boundaries for x a multiple
+    // of 8, where the compiler reads runs of it as single wide loads.
+    uint16_t* const Ls = L.s + 7;
+    uint16_t* const Lf = L.f + 7;
+    const int ry = lane / PER_ROW, rx = (lane % PER_ROW) * RUN;
+    const bool active = lane < LANES;
+    // The run's residual, loaded first so that the load overlaps the rest.
+    // Residuals start at multiples of 16 samples (32 bytes), rows at
+    // multiples of 4: 8-byte loads, or 16-byte ones for 32x32 runs.
+    int res[RUN];
+#pragma unroll
+    for (int k = 0; k < RUN; ++k) res[k] = 0;
+    if (active && it.res >= 0) {
+        const int16_t* r = P.ires + it.res + ry * N + rx;
+        if constexpr (RUN == 4) {
+            const uint2 v = *(const uint2*)r;
+            const uint32_t w[2] = {v.x, v.y};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) res[k] = (int)(int16_t)(w[k >> 1] >> (16 * (k & 1)));
+        } else {
+#pragma unroll
+            for (int q = 0; q < RUN / 8; ++q) {
+                const uint4 v = ((const uint4*)r)[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) res[8 * q + k] = (int)(int16_t)(w[k >> 1] >> (16 * (k & 1)));
+            }
+        }
+    }
+    // 1. Availability per sample (per 4x4 luma block: in the picture,
+    // earlier in z-scan order, intra under constrained intra prediction)
+    // and the available samples.
+    const int sh = P.log2_ctb - 2;
+    const uint32_t zc = zs4((x0 << sc) >> 2, (y0 << sc) >> 2, sh, P.wc);
+    uint64_t mk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int j = lane + 64 * k;
+        const int xs = j < 2 * N + 1 ? x0 - 1 : x0 + (j - 2 * N - 1);
+        const int ys = j < 2 * N ? y0 + 2 * N - 1 - j : y0 - 1;
+        bool ok = j < M && xs >= 0 && ys >= 0;
+        if (ok) {
+            const int xl = xs << sc, yl = ys << sc;
+            ok = xl < P.w && yl < P.h;
+            if (ok) {
+                const int bx = xl >> 2, by = yl >> 2;
+                ok = !(zs4(bx, by, sh, P.wc) > zc) && (!P.cip || (P.cuinfo[(size_t)by * P.w4 + bx] & 1));
+            }
+        }
+        if (ok) Ls[j] = (uint16_t)plane[(size_t)ys * pitch + xs];
+        mk[k] = __ballot(ok);
+    }
+    wave_sync();
+    // 2. Substitution (8.4.4.2.2).
+    if (!(mk[0] | mk[1] | mk[2])) {
+        for (int j = lane; j < M; j += 64) Ls[j] = (uint16_t)(1 << (bd - 1));
+    } else {
+        const int first = mk[0] ? __builtin_ctzll(mk[0]) : mk[1] ? 64 + __builtin_ctzll(mk[1]) : 128 + __builtin_ctzll(mk[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int j = lane + 64 * k;
+            if (j < M && !((mk[k] >> lane) & 1)) {
+                const int src = prev_avail(j, mk[0], mk[1], mk[2]);
+                Ls[j] = Ls[src >= 0 ? src : first];
+            }
+        }
+    }
+    wave_sync();
+    // 3. Smoothing (8.4.4.2.3): luma only in 4:2:0; not DC, not 4x4.
+    bool smoothed = false;
+    if (c == 0 && mode != 1 && N != 4) {
+        const int dist = min(abs(mode - 26), abs(mode - 10));
+        smoothed = dist > (N == 8 ? 7 : N == 16 ? 1 : 0);
+    }
+    const uint16_t* R = Ls;
+    if (smoothed) {
+        const int cr = Ls[2 * N];
+        const bool bi = N == 32 && P.strong && abs(cr + (int)Ls[4 * N] - 2 * (int)Ls[3 * N]) < (1 << (bd - 5)) &&
+                        abs(cr + (int)Ls[0] - 2 * (int)Ls[N]) < (1 << (bd - 5));
+        for (int j = lane; j < M; j += 64) {
+            int v = Ls[j];
+            if (bi) {
+                if (j >= 1 && j < 2 * N) {  // p[-1][i], i = 2n - 1 - j <= 62
+                    const int i = 2 * N - 1 - j;
+                    v = ((63 - i) * cr + (i + 1) * (int)Ls[0] + 32) >> 6;
+                } else if (j > 2 * N && j < 4 * N) {
+                    const int i = j - 2 * N - 1;
+                    v = ((63 - i) * cr + (i + 1) * (int)Ls[4 * N] + 32) >> 6;
+                }
+            } else if (j > 0 && j < M - 1) {
+                v = ((int)Ls[j - 1] + 2 * v + (int)Ls[j + 1] + 2) >> 2;
+            }
+            Lf[j] = (uint16_t)v;
+        }
+        wave_sync();
+        R = Lf;
+    }
+    // 4. Prediction of the lane's run; DC's sum and the angular ref[]
+    // by the whole wave first.
+    const bool edge = c == 0 && N < 32;
+    int dc = 0;
+    if (mode == 1) {
+        int v = lane < N ? (int)R[2 * N + 1 + lane] : lane < 2 * N ? (int)R[2 * N - 1 - (lane - N)] : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
+        dc = (v + N) >> (LOG2 + 1);
+    }
+    const int angle = mode >= 2 ? kIntraAngle[mode - 2] : 0;
+    const bool vert = mode >= 18;
+    if (mode >= 2) {
+        // ref[k] at L.ref[N + k], k = -N .. 2N (8.4.4.2.6).
+        const int last = (N * angle) >> 5;
+        const int inv = angle < 0 ? (int)kInvAngle[mode - 11] : 0;
+        for (int q = lane; q <= 3 * N; q += 64) {
+            const int k = q - N;
+            int v = 0;
+            bool set = true;
+            if (k == 0) {
+                v = R[2 * N];
+            } else if (k > 0 && (k <= N || angle >= 0)) {
+                v = vert ? R[2 * N + k] : R[2 * N - k];  // main[k - 1]
+            } else if (k < 0 && angle < 0 && last < -1 && k >= last) {
+                const int idx = -1 + ((k * inv + 128) >> 8);
+                v = idx < 0 ? R[2 * N] : vert ? R[2 * N - 1 - idx] : R[2 * N + 1 + idx];
+            } else {
+                set = false;
+            }
+            if (set) L.ref[q] = (uint16_t)v;
+        }
+        wave_sync();
+    }
+    if (!active) return;
+    int v[RUN];
+    const int y = ry;
+    if (mode == 0) {
+        const int tn = R[3 * N + 1], ln = R[N - 1], left = R[2 * N - 1 - y];
+#pragma unroll
+        for (int k = 0; k < RUN; ++k) {
+            const int x = rx + k;
+            v[k] = ((N - 1 - x) * left + (x + 1) * tn + (N - 1 - y) * (int)R[2 * N + 1 + x] + (y + 1) * ln + N) >>
+                   (LOG2 + 1);
+        }
+    } else if (mode == 1) {
+#pragma unroll
+        for (int k = 0; k < RUN; ++k) {
+            const int x = rx + k;
+            int p = dc;
+            if (edge) {
+                if (x == 0 && y == 0) p = ((int)R[2 * N - 1] + 2 * dc + (int)R[2 * N + 1] + 2) >> 2;
+                else if (y == 0) p = ((int)R[2 * N + 1 + x] + 3 * dc + 2) >> 2;
+                else if (x == 0) p = ((int)R[2 * N - 1 - y] + 3 * dc + 2) >> 2;
+            }
+            v[k] = p;
+        }
+    } else {
+        const int cr = R[2 * N];
+#pragma unroll
+        for (int k = 0; k < RUN; ++k) {
+            const int px = rx + k, py = y;
+            const int kx = vert ? px : py, ky = vert ? py : px;
+            const int i_idx = ((ky + 1) * angle) >> 5, i_fact = ((ky + 1) * angle) & 31;
+            const int a = L.ref[N + kx + i_idx + 1];
+            int w = i_fact ? ((32 - i_fact) * a + i_fact * (int)L.ref[N + kx + i_idx + 2] + 16) >> 5 : a;
+            if (edge) {
+                if (mode == 26 && px == 0) w = clampi((int)R[2 * N + 1] + (((int)R[2 * N - 1 - py] - cr) >> 1), 0, max);
+                else if (mode == 10 && py == 0) w = clampi((int)R[2 * N - 1] + (((int)R[2 * N + 1 + px] - cr) >> 1), 0, max);
+            }
+            v[k] = w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RUN; ++k) v[k] = clampi(v[k] + res[k], 0, max);
+    store_row4<S>(plane + (size_t)(y0 + ry) * pitch + x0 + rx, v, RUN);
 }
 
 template <typename S> __device__ __forceinline__ void intra_tu_any(PicRef P, const IntraTu it, int lane, IntraLds& L) {
@@ -1675,18 +1726,53 @@ template <typename S> __device__ __forceinline__ void intra_tu_any(PicRef P, con
     }
 }
 
-// One dependency level over the batch: one TU per 64-thread workgroup.
-// The kernel boundary between levels orders each TU after the TUs whose
-// samples it reads.
-__global__ void __launch_bounds__(64) hevc_intra_level(BatchDev b, int level) {
-    __shared__ IntraLds lds;
-    ConstI32* pre = (ConstI32*)(b.ipre + (size_t)level * (b.n + 1));
+__device__ __forceinline__ void intra_residual_any(PicRef P, const IntraTu it, int lane, ResidualLds& L) {
+    switch (it.t.log2) {
+        case 2: intra_residual<2>(P, it, lane, L); break;
+        case 3: intra_residual<3>(P, it, lane, L); break;
+        case 4: intra_residual<4>(P, it, lane, L); break;
+        default: intra_residual<5>(P, it, lane, L); break;
+    }
+}
+
+// Preparation of the device-intra levels, one TU of the batch per 64-thread
+// workgroup: the TU's task in its level's run of the task table and, if
+// coded, its residual.
+__global__ void __launch_bounds__(64) hevc_intra_prep(BatchDev b) {
+    __shared__ ResidualLds lds;
+    ConstI32* pre = (ConstI32*)b.ipre;
     const int it = blockIdx.x;
     if (it >= pre[b.n]) return;
     const int p = __builtin_amdgcn_readfirstlane(find_pic(pre, b.n, it));
     PicRef P = picture(b, p);
-    const int k = __builtin_amdgcn_readfirstlane(P.ilevel[level] + it - pre[p]);
+    const int k = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const IntraTu t = P.itus[k];
+    if (threadIdx.x == 0) {
+        const int l = t.level;
+        ConstI32* lp = (ConstI32*)(b.ipre + (size_t)(l + 1) * (b.n + 1));
+        b.itask[((ConstI32*)b.ibase)[l] + lp[p] + (k - P.ilevel[l])] = IntraTask{b.hevc[p], P.itus + k};
+    }
+    if (t.res >= 0) intra_residual_any(P, t, (int)threadIdx.x, lds);
+}
+
+// The task and its TU read as dword vectors through the constant address
+// space (scalar loads: both are uniform and written before the launch).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(4))) ConstU32x4;
+
+// One dependency level: one TU per 64-thread workgroup, from the level's
+// run of the task table. The kernel boundary between levels orders each
+// TU after the TUs whose samples it reads.
+__global__ void __launch_bounds__(64) hevc_intra_level(const IntraTask* tasks) {
+    __shared__ IntraLds lds;
+    const u32x4 w = ((ConstU32x4*)tasks)[blockIdx.x];
+    IntraTask tk;
+    __builtin_memcpy(&tk, &w, sizeof(tk));
+    PicRef P = *(ConstPic*)tk.pic;
+    const u32x4 u0 = ((ConstU32x4*)tk.tu)[0], u1 = ((ConstU32x4*)tk.tu)[1];
+    IntraTu t;
+    __builtin_memcpy(&t, &u0, 16);
+    __builtin_memcpy((char*)&t + 16, &u1, 16);
     if (P.bd == 8) intra_tu_any<uint8_t>(P, t, (int)threadIdx.x, lds);
     else intra_tu_any<uint16_t>(P, t, (int)threadIdx.x, lds);
 }
@@ -2420,17 +2506,24 @@ hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         const hipError_t e = hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
         if (e != hipSuccess) return e;
     }
-    // Stage 1 ends with the intra prediction of device-intra CUs: one
-    // launch per dependency level, one TU per workgroup.
-    if (stage == 1)
+    // Stage 1 ends with the intra prediction of device-intra CUs: the
+    // preparation over all their TUs, then one launch per dependency level,
+    // one TU per workgroup.
+    if (stage == 1 && b.ilevels > 0 && b.iall > 0) {
+        hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra_prep), dim3(b.iall), dim3(64), args,
+                                       0, st);
+        if (e != hipSuccess) return e;
+        int64_t at = 0;
         for (int l = 0; l < b.ilevels; ++l) {
             if (b.itotal[l] <= 0) continue;
-            int lv = l;
-            void* largs[] = {&arg, &lv};
-            const hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra_level), dim3(b.itotal[l]),
-                                                 dim3(64), largs, 0, st);
+            const IntraTask* tasks = b.itask + at;
+            at += b.itotal[l];
+            void* largs[] = {&tasks};
+            e = hipLaunchKernel(reinterpret_cast<const void*>(&hevc_intra_level), dim3(b.itotal[l]), dim3(64), largs,
+                                0, st);
             if (e != hipSuccess) return e;
         }
+    }
     return hipSuccess;
 }
 

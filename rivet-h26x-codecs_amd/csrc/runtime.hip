@@ -300,6 +300,7 @@ struct rv_job {
     // Single-picture tables for the plain path.
     rv::BatchDev single{};
     std::vector<int32_t> itotal;  // device-intra TUs per level (host copy)
+    int32_t n_itus = 0;           // device-intra TUs
     int64_t bytes[4] = {0, 0, 0, 0};
 };
 
@@ -313,6 +314,7 @@ struct rv_batch {
     std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
     std::vector<int32_t> itotal;        // device-intra TUs per level
     DevBuf tables;
+    DevBuf itask;                       // device-intra task table
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -922,7 +924,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         std::vector<int32_t> ilvl;
         const int n_ilevels = cu_at.empty() ? 0 : intra_levels(p, cu_at, ilvl);
         std::vector<int32_t> lvl_cnt(n_ilevels + 1, 0);
-        size_t n_itus = 0, n_inz = 0;
+        size_t n_itus = 0, n_inz = 0, n_ires = 0;
         int64_t b_intra = 0;
         const int64_t ssz = p->bit_depth > 8 ? 2 : 1;
         for (size_t i = 0; i < ilvl.size(); ++i) {
@@ -931,7 +933,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             ++lvl_cnt[ilvl[i]];
             ++n_itus;
             const int n = 1 << t.log2;
-            if (t.flags & RV_TU_CODED) n_inz += t.n_nz;
+            if (t.flags & RV_TU_CODED) {
+                n_inz += t.n_nz;
+                n_ires += (size_t)n * n;
+            }
             b_intra += ssz * ((int64_t)n * n + 4 * n + 1) + ((t.flags & RV_TU_CODED) ? 4 * (int64_t)t.n_nz : 0);
         }
         auto is_itu = [&](int i) { return !ilvl.empty() && ilvl[i] >= 0; };
@@ -1041,13 +1046,14 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
                     carve_size<rv::SideClass>(n_sclass) +
                     carve_size<rv_nz>(n_nzs + n_inz) + carve_size<rv::IntraTu>(n_itus) +
-                    carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)n_ilevels) +
+                    carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)(n_ilevels + 1)) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) + carve_size<rv_hevc_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4);
         size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
-                     carve_size<int8_t>(n4) + carve_size<uint32_t>(n4) + 1024;
+                     carve_size<int8_t>(n4) + carve_size<uint32_t>(n4) + carve_size<int16_t>(n_ires) +
+                     carve_size<rv::IntraTask>(n_itus) + 1024;
         char* hbase = t_stage.get(sz);
         if (!hbase || !pool_get(sz, &j->arena) || !pool_get(scr, &j->scratch)) {
             job_release(j);
@@ -1069,7 +1075,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs + n_inz);
         rv::IntraTu* h_itus = carve<rv::IntraTu>(hp, n_itus);
         int32_t* h_ilevel = carve<int32_t>(hp, n_ilevels + 1);
-        int32_t* h_ipre = carve<int32_t>(hp, 2 * (size_t)n_ilevels);  // per level {0, count}
+        // {0, all TUs}, then per level {0, count}.
+        int32_t* h_ipre = carve<int32_t>(hp, 2 * (size_t)(n_ilevels + 1));
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_sao* h_sao = carve<rv_sao>(hp, n_sao);
         int32_t* h_intra = carve<int32_t>(hp, n_intra);
@@ -1118,18 +1125,22 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             // Levels run from 1: level l fills [h_ilevel[l - 1], h_ilevel[l]).
             h_ilevel[0] = 0;
             for (int l = 1; l <= n_ilevels; ++l) h_ilevel[l] = h_ilevel[l - 1] + lvl_cnt[l];
+            h_ipre[0] = 0;
+            h_ipre[1] = (int32_t)n_itus;
             for (int l = 0; l < n_ilevels; ++l) {
-                h_ipre[2 * l] = 0;
-                h_ipre[2 * l + 1] = lvl_cnt[l + 1];
+                h_ipre[2 * l + 2] = 0;
+                h_ipre[2 * l + 3] = lvl_cnt[l + 1];
             }
             std::vector<int32_t> put(h_ilevel, h_ilevel + n_ilevels + 1);
-            size_t nz_at = n_nzs;
+            size_t nz_at = n_nzs, res_at = 0;
             for (int i = 0; i < p->n_tus; ++i) {
                 if (!is_itu(i)) continue;
                 const rv_hevc_tu& t = p->tus[i];
                 rv::IntraTu& it = h_itus[put[ilvl[i] - 1]++];
                 std::memset(&it, 0, sizeof(it));
                 it.t = t;
+                it.level = ilvl[i] - 1;
+                it.res = -1;
                 const int sc = t.comp ? 1 : 0;
                 const rv_hevc_cu& cu = p->cus[cu_at[(size_t)((t.y << sc) >> 2) * (W / 4) + ((t.x << sc) >> 2)]];
                 if (t.comp) {
@@ -1143,6 +1154,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     it.t.coef_off = (uint32_t)nz_at;
                     std::memcpy(h_coefs + nz_at, p->nz + t.coef_off, t.n_nz * sizeof(rv_nz));
                     nz_at += t.n_nz;
+                    // Residual offsets are sums of N x N, multiples of 16.
+                    it.res = (int32_t)res_at;
+                    res_at += (size_t)1 << (2 * t.log2);
                 }
             }
         }
@@ -1240,6 +1254,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.qp = (int8_t*)sp;
         sp += carve_size<int8_t>(n4);
         d.pumap = (uint32_t*)sp;
+        sp += carve_size<uint32_t>(n4);
+        d.ires = (int16_t*)sp;
+        sp += carve_size<int16_t>(n_ires);
+        rv::IntraTask* itask = (rv::IntraTask*)sp;
         rv::hevc_stage_items(d);
         *hdesc = d;
         j->desc_dev = dev((char*)hdesc);
@@ -1252,8 +1270,12 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.hevc = (const rv::HevcPicDev* const*)dev((char*)h_ptr);
         j->single.h264 = nullptr;
         j->itotal.assign(lvl_cnt.begin() + (n_ilevels ? 1 : 0), lvl_cnt.end());
+        j->n_itus = (int32_t)n_itus;
         j->single.ilevels = n_ilevels;
         j->single.ipre = (const int32_t*)dev((char*)h_ipre);
+        j->single.ibase = (const int32_t*)dev((char*)h_ilevel);  // one picture: its own level starts
+        j->single.itask = itask;
+        j->single.iall = (int32_t)n_itus;
         j->single.itotal = j->itotal.data();
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
         // The per-4x4 maps need no clearing: for work lists whose CUs, PUs
@@ -1491,24 +1513,34 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
             for (int i = 0; i < n4; ++i)
                 b->pre_4[(size_t)s * (n4 + 1) + i + 1] = b->pre_4[(size_t)s * (n4 + 1) + i] + b->h264[i]->h4.items[s];
         }
-        // Device-intra levels: per level, the prefix over the HEVC pictures
-        // of their TU counts.
+        // Device-intra levels: the prefix over the HEVC pictures of their TU
+        // counts, all levels, then per level; each level's first task.
         int ilevels = 0;
         for (int i = 0; i < nh; ++i) ilevels = std::max(ilevels, b->hevc[i]->hd.n_ilevels);
-        std::vector<int32_t> ipre((size_t)ilevels * (nh + 1), 0);
+        std::vector<int32_t> ipre((size_t)(ilevels + 1) * (nh + 1), 0), ibase((size_t)ilevels + 1, 0);
         b->itotal.assign(ilevels, 0);
+        for (int i = 0; i < nh; ++i) ipre[(size_t)i + 1] = ipre[(size_t)i] + b->hevc[i]->n_itus;
         for (int l = 0; l < ilevels; ++l) {
+            int32_t* row = ipre.data() + (size_t)(l + 1) * (nh + 1);
             for (int i = 0; i < nh; ++i) {
                 const std::vector<int32_t>& t = b->hevc[i]->itotal;
-                ipre[(size_t)l * (nh + 1) + i + 1] = ipre[(size_t)l * (nh + 1) + i] + (l < (int)t.size() ? t[l] : 0);
+                row[i + 1] = row[i] + (l < (int)t.size() ? t[l] : 0);
             }
-            b->itotal[l] = ipre[(size_t)l * (nh + 1) + nh];
+            b->itotal[l] = row[nh];
+            ibase[(size_t)l + 1] = ibase[(size_t)l] + row[nh];
+        }
+        const int32_t iall = ipre[(size_t)nh];
+        if (iall > 0 && !pool_get((size_t)iall * sizeof(rv::IntraTask), &b->itask)) {
+            delete b;
+            return fail(RV_E_NOMEM, "batch intra tasks");
         }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
-                    carve_size<int32_t>(b->pre_4.size()) + carve_size<int32_t>(ipre.size());
+                    carve_size<int32_t>(b->pre_4.size()) + carve_size<int32_t>(ipre.size()) +
+                    carve_size<int32_t>(ibase.size());
         std::vector<uint32_t> host(sz / 4 + 1, 0u);
         char* hbase = (char*)host.data();
         if (!pool_get(sz, &b->tables)) {
+            pool_put(b->itask);
             delete b;
             return fail(RV_E_NOMEM, "batch tables");
         }
@@ -1520,7 +1552,9 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         int32_t* preh = carve<int32_t>(hp, b->pre_h.size());
         int32_t* pre4 = carve<int32_t>(hp, b->pre_4.size());
         int32_t* hipre = carve<int32_t>(hp, ipre.size());
-        if (!ipre.empty()) std::memcpy(hipre, ipre.data(), ipre.size() * 4);
+        std::memcpy(hipre, ipre.data(), ipre.size() * 4);
+        int32_t* hibase = carve<int32_t>(hp, ibase.size());
+        std::memcpy(hibase, ibase.data(), ibase.size() * 4);
         for (int i = 0; i < nh; ++i) ph[i] = b->hevc[i]->desc_dev;
         for (int i = 0; i < n4; ++i) p4[i] = b->h264[i]->desc_dev;
         std::memcpy(preh, b->pre_h.data(), b->pre_h.size() * 4);
@@ -1530,6 +1564,9 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->bh.prefix = (const int32_t*)dev((char*)preh);
         b->bh.ilevels = ilevels;
         b->bh.ipre = (const int32_t*)dev((char*)hipre);
+        b->bh.ibase = (const int32_t*)dev((char*)hibase);
+        b->bh.itask = (rv::IntraTask*)b->itask.p;
+        b->bh.iall = iall;
         b->bh.itotal = b->itotal.data();
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
@@ -1548,7 +1585,10 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
                 e = hipGetLastError();
             }
             if (e == hipSuccess) (void)note_stream_work(&e);
-            if (e != hipSuccess) pool_put(b->tables);
+            if (e != hipSuccess) {
+                pool_put(b->tables);
+                pool_put(b->itask);
+            }
         }
         if (e != hipSuccess) {
             delete b;
@@ -1573,6 +1613,7 @@ int rv_batch_free(rv_batch* b) {
     }
     if (b->graph) (void)hipGraphDestroy(b->graph);
     pool_put(b->tables);
+    pool_put(b->itask);
     delete b;
     return RV_OK;
 }

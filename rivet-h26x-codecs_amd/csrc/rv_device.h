@@ -24,11 +24,14 @@ struct alignas(16) McTile {
 static_assert(sizeof(McTile) == 16, "one 16-byte load per tile");
 
 // One transform block of an RV_INTRA_DEVICE CU: the TU (coef_off rebased
-// into the job's nonzero list) and its intra prediction mode.
+// into the job's nonzero list), its intra prediction mode, where its
+// residual goes and its dependency level.
 struct alignas(16) IntraTu {
     rv_hevc_tu t;
-    int32_t mode;  // IntraPredModeY of its PU, or IntraPredModeC
-    int32_t pad[3];
+    int32_t mode;   // IntraPredModeY of its PU, or IntraPredModeC
+    int32_t res;    // coded: first int16 of its N x N residual in ires; else -1
+    int32_t level;  // dependency level, from 0
+    int32_t pad;
 };
 static_assert(sizeof(IntraTu) == 32, "two 16-byte loads per intra TU");
 
@@ -87,6 +90,7 @@ struct HevcPicDev {
     const int32_t* ilevel;
     const IntraTu* itus;
     int32_t strong, cip;  // strong intra smoothing, constrained intra pred
+    int16_t* ires;        // residuals of the coded itus (job scratch)
     // Per-4x4 side data (job scratch).
     uint8_t* cuinfo;   // bit 0 intra, bit 1 CU left edge, bit 2 CU top edge
     int8_t* qp;        // QpY
@@ -210,13 +214,25 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
-    // Device intra (HEVC): the most dependency levels of any picture; per
-    // level, the prefix over pictures of their TU counts at that level
-    // (device memory, levels x (n + 1)) and the level's total (host
-    // memory: read by the launcher only).
+    // Device intra (HEVC): the most dependency levels of any picture; the
+    // prefix over pictures of their TU counts, all levels first, then per
+    // level (device memory, (levels + 1) x (n + 1)); the first task of each
+    // level in itask (device, levels + 1); the task table, level by level,
+    // written by the preparation kernel (device); the TUs of all levels and
+    // each level's count (host: read by the launcher only).
     int32_t ilevels;
     const int32_t* ipre;
+    const int32_t* ibase;
+    struct IntraTask* itask;
+    int32_t iall;
     const int32_t* itotal;
 };
+
+// Task of a device-intra level kernel: one TU of one picture.
+struct alignas(16) IntraTask {
+    const HevcPicDev* pic;
+    const IntraTu* tu;
+};
+static_assert(sizeof(IntraTask) == 16, "one 16-byte load per task");
 
 }  // namespace rv
