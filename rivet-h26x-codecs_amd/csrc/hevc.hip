@@ -835,6 +835,229 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
 }
 
 // ---------------------------------------------------------------------
+// Stage 0, strips: motion compensation of the blocks of 8-bit pictures
+// whose reference windows lie inside the reference planes, one lane per
+// strip of 4 output columns (McStrip). The lane walks its window row by row: each row's 4
+// first-stage outputs (interp_scalar's horizontal pass, src/dsp/hevc.rs:
+// 823-882) come from one aligned load, and rows are kept in registers as
+// i16 pairs (row 2k, row 2k + 1) of each column, so the vertical pass is
+// v_dot2 over pairs: output row 2m takes pairs m..m+3 with the even taps
+// (f0 f1)..(f6 f7), row 2m + 1 pairs m..m+4 with the odd taps (0 f0)..
+// (f7 0) (chroma: 2 and 3 pairs). An output row is finished, combined
+// over both lists and stored as one dword (8-bit) as soon as its pairs
+// exist, so only the last 5 (chroma 3) pairs of each list stay live.
+// ---------------------------------------------------------------------
+
+// Reference window of one list of one strip lane (8-bit samples): the
+// dword holding its first sample (row 0), the sample's byte offset in it,
+// the row pitch in dwords and the two phases.
+struct StripSrc {
+    const uint32_t* w;
+    uint32_t sh;
+    int pitch4;
+    int fx, fy;
+};
+__device__ __forceinline__ StripSrc strip_src(PicRef P, int l, int ri, int c, int x0, int y0, int fx, int fy) {
+    const char* base = (const char*)ref_plane(P, l, ri, c);
+    const int pitch_b = P.pitch[c];
+    const size_t off = (size_t)y0 * pitch_b + (size_t)x0;
+    StripSrc s;
+    s.w = (const uint32_t*)(base + (off & ~(size_t)3));
+    s.sh = (uint32_t)off & 3u;
+    s.pitch4 = pitch_b >> 2;
+    s.fx = fx;
+    s.fy = fy;
+    return s;
+}
+
+// The dwords of one window row of a strip: 11 (luma) or 7 (chroma) samples
+// after a byte shift of at most 3.
+template <bool LUMA>
+struct StripRow {
+    static constexpr int ND = LUMA ? 4 : 3;
+    uint32_t d[ND];
+};
+template <bool LUMA>
+__device__ __forceinline__ StripRow<LUMA> strip_load(const StripSrc& s, int r) {
+    const uint32_t* w = s.w + (size_t)r * s.pitch4;
+    StripRow<LUMA> v;
+#pragma unroll
+    for (int i = 0; i < StripRow<LUMA>::ND; ++i) v.d[i] = w[i];
+    return v;
+}
+
+// First stage of one loaded row of a strip: 4 outputs (exact at 8 bits,
+// shift1 = 0) from luma (8-tap) or chroma (4-tap) samples.
+template <bool LUMA>
+__device__ __forceinline__ void strip_row(const StripSrc& s, const StripRow<LUMA>& v, const uint32_t* tab,
+                                          int (&acc)[4]) {
+    const uint32_t* d = v.d;
+    if constexpr (LUMA) {
+        const uint32_t x[3] = {__builtin_amdgcn_alignbyte(d[1], d[0], s.sh),
+                               __builtin_amdgcn_alignbyte(d[2], d[1], s.sh),
+                               __builtin_amdgcn_alignbyte(d[3], d[2], s.sh)};
+        filt4_b(x, luma_taps_b(tab, s.fx), acc);
+    } else {
+        const uint32_t x[2] = {__builtin_amdgcn_alignbyte(d[1], d[0], s.sh),
+                               __builtin_amdgcn_alignbyte(d[2], d[1], s.sh)};
+        filt4c_b(x, chroma_taps_b(tab, s.fx), acc);
+    }
+}
+
+// Low halves of lo (row 2k) and hi (row 2k + 1) as one i16 pair.
+__device__ __forceinline__ uint32_t pair_rows(int lo, int hi) {
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+
+// One strip (LUMA: H rows of 8-tap; else H rows of 4-tap chroma), both
+// lists when BI, the one in st.dir otherwise.
+template <bool LUMA, int H, bool BI>
+__device__ __forceinline__ void mc_strip(PicRef P, const McStrip& st, const uint32_t* tab) {
+    constexpr int T = LUMA ? 8 : 4;       // taps
+    constexpr int R = H + T - 1;          // window rows (odd)
+    constexpr int RING = T / 2 + 1;       // live row pairs
+    constexpr int NL = BI ? 2 : 1;
+    const int bd = 8;
+    const int c = LUMA ? 0 : st.comp;
+    const int dir = st.dir;
+    int ri[2] = {st.refs & 15, st.refs >> 4};
+    StripSrc src[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int l = BI ? k : (dir >> 1);
+        const int mvx = st.mv[l].x, mvy = st.mv[l].y;
+        if constexpr (LUMA)
+            src[k] = strip_src(P, l, ri[l], 0, st.x + (mvx >> 2) - 3, st.y + (mvy >> 2) - 3, mvx & 3, mvy & 3);
+        else
+            src[k] = strip_src(P, l, ri[l], c, st.x + (mvx >> 3) - 1, st.y + (mvy >> 3) - 1, mvx & 7, mvy & 7);
+    }
+    const bool weighted = P.weighted != 0;
+    Wp wp = {{1, 1}, {0, 0}};
+    if (weighted) wp = weights(P, c, ri);
+    const int lwd = P.log2_wd[c];
+    const int pitch = P.pitch[c];
+    uint8_t* dst = (uint8_t*)P.cur[c] + (size_t)st.y * pitch + st.x;
+    const int cols = st.cols;
+
+    uint32_t ring[NL][RING][4];
+    // Output row y of list k from the pairs in the ring, as an i16 value;
+    // the second-stage taps (even pairs e, odd pairs o) are read from LDS
+    // per row rather than held in registers.
+    auto vert = [&](int k, int y, int (&out)[4]) {
+        const int m = y >> 1;
+        short2v tp[T / 2 + 1];
+        if constexpr (LUMA) {
+            const Taps8 t = luma_taps(tab, src[k].fy);
+#pragma unroll
+            for (int j = 0; j <= T / 2; ++j) tp[j] = (y & 1) ? t.o[j] : (j < T / 2 ? t.e[j] : t.o[0]);
+        } else {
+            const Taps4 t = chroma_taps(tab, src[k].fy);
+#pragma unroll
+            for (int j = 0; j <= T / 2; ++j) tp[j] = (y & 1) ? t.o[j] : (j < T / 2 ? t.e[j] : t.o[0]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int s = 0;
+            if ((y & 1) == 0) {
+#pragma unroll
+                for (int j = 0; j < T / 2; ++j) s = dot2(as2(ring[k][(m + j) % RING][i]), tp[j], s);
+            } else {
+#pragma unroll
+                for (int j = 0; j <= T / 2; ++j) s = dot2(as2(ring[k][(m + j) % RING][i]), tp[j], s);
+            }
+            out[i] = (int16_t)(s >> 6);
+        }
+    };
+    auto emit = [&](int y) {
+        int a[4], bq[4] = {0, 0, 0, 0};
+        vert(0, y, a);
+        if constexpr (BI) vert(1, y, bq);
+        int v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = BI ? combine(weighted, lwd, wp, 3, a[i], bq[i], bd)
+                                              : combine(weighted, lwd, wp, dir, a[i], a[i], bd);
+        uint8_t* d = dst + (size_t)y * pitch;
+        const uint32_t pk = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+        if (cols == 4) *(uint32_t*)d = pk;
+        else if (cols == 2) *(uint16_t*)d = (uint16_t)pk;
+    };
+    // Row pairs in a software pipeline: the loads of pair kp + PF are
+    // issued before pair kp is filtered (the scheduling barriers keep the
+    // compiler from sinking them to their use), so each wave keeps PF pairs
+    // of every list in flight.
+    constexpr int NPAIR = (R + 1) / 2;
+    constexpr int PF = 1;
+    StripRow<LUMA> buf[PF + 1][NL][2];
+    auto load_pair = [&](int kp, StripRow<LUMA> (&into)[NL][2]) {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            into[k][0] = strip_load<LUMA>(src[k], 2 * kp);
+            if (2 * kp + 1 < R) into[k][1] = strip_load<LUMA>(src[k], 2 * kp + 1);
+        }
+    };
+#pragma unroll
+    for (int kp = 0; kp < PF && kp < NPAIR; ++kp) load_pair(kp, buf[kp]);
+#pragma unroll
+    for (int kp = 0; kp < NPAIR; ++kp) {
+        if (kp + PF < NPAIR) load_pair(kp + PF, buf[(kp + PF) % (PF + 1)]);
+        __builtin_amdgcn_sched_barrier(0);
+        StripRow<LUMA>(&cur)[NL][2] = buf[kp % (PF + 1)];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            int a0[4], a1[4] = {0, 0, 0, 0};
+            strip_row<LUMA>(src[k], cur[k][0], tab, a0);
+            if (2 * kp + 1 < R) strip_row<LUMA>(src[k], cur[k][1], tab, a1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ring[k][kp % RING][i] = pair_rows(a0[i], a1[i]);
+        }
+        // Pair kp is complete: output rows 2kp - T + 1 (odd) and 2kp - T + 2
+        // (even) have all their pairs.
+        const int yo = 2 * kp - T + 1, ye = 2 * kp - T + 2;
+        if (yo >= 1 && yo < H) emit(yo);
+        if (ye >= 0 && ye < H) emit(ye);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+__device__ __forceinline__ void strip_group(PicRef P, int g, const McStrip& st, const uint32_t* tab) {
+    switch (g) {
+        case 0: mc_strip<true, 16, true>(P, st, tab); break;
+        case 1: mc_strip<true, 16, false>(P, st, tab); break;
+        case 2: mc_strip<true, 8, true>(P, st, tab); break;
+        case 3: mc_strip<true, 8, false>(P, st, tab); break;
+        case 4: mc_strip<true, 4, true>(P, st, tab); break;
+        case 5: mc_strip<true, 4, false>(P, st, tab); break;
+        case 6: mc_strip<false, 8, true>(P, st, tab); break;
+        case 7: mc_strip<false, 8, false>(P, st, tab); break;
+        case 8: mc_strip<false, 4, true>(P, st, tab); break;
+        case 9: mc_strip<false, 4, false>(P, st, tab); break;
+        case 10: mc_strip<false, 2, true>(P, st, tab); break;
+        default: mc_strip<false, 2, false>(P, st, tab); break;
+    }
+}
+
+// One wave per 64 strips of one group of one 8-bit picture (b.spre: prefix
+// over pictures of their strip waves).
+template <int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_mc_strips(BatchDev b) {
+    __shared__ uint32_t taps[kTapWords];
+    ConstI32* spre = (ConstI32*)b.spre;
+    const int it = xcd_item();
+    if (it >= spre[b.n]) return;
+    const int tid = threadIdx.x;
+    const uint32_t* kt = &kF.qpk[0][0];
+    taps[tid] = kt[tid];
+    if (tid + 64 < kTapWords) taps[tid + 64] = kt[tid + 64];
+    const int p = __builtin_amdgcn_readfirstlane(find_pic(spre, b.n, it));
+    PicRef P = picture(b, p);
+    const int w = __builtin_amdgcn_readfirstlane(it - spre[p]);
+    const int g = __builtin_amdgcn_readfirstlane((int)((const uint8_t __attribute__((address_space(4)))*)P.swg)[w]);
+    const McStrip st = __builtin_bit_cast(McStrip, *(const uint4*)(P.strips + (size_t)w * 64 + tid));
+    __syncthreads();
+    strip_group(P, g, st, taps);
+}
+
+// ---------------------------------------------------------------------
 // Stage 1: inverse transform + residual add; boundary strengths
 // ---------------------------------------------------------------------
 
@@ -2460,18 +2683,25 @@ struct StageVariant {
     int depth;    // 0 any, 1 8-bit pictures only, 2 deeper pictures only
     int threads;  // workgroup size
     const char* name;
+    const void* strips;  // stage 0: the strip kernel launched with it
 };
 #define RV_VARIANT(k, d, t, n) \
-    StageVariant { reinterpret_cast<const void*>(&k), d, t, n }
-const StageVariant kStage0[] = {RV_VARIANT((hevc_stage0<0, 5>), 0, 64, "any/5"),
-                                RV_VARIANT((hevc_stage0<1, 6>), 1, 64, "8-bit/6"),
-                                RV_VARIANT((hevc_stage0<1, 7>), 1, 64, "8-bit/7"),
-                                RV_VARIANT((hevc_stage0<1, 8, 1>), 1, 64, "8-bit/8 per-list"),
-                                RV_VARIANT((hevc_stage0<2, 5>), 2, 64, "deep/5"),
-                                RV_VARIANT((hevc_stage0<2, 6, 1>), 2, 64, "deep/6 per-list"),
-                                RV_VARIANT((hevc_stage0<0, 6, 1>), 0, 64, "any/6 per-list"),
-                                RV_VARIANT((hevc_stage0<1, 8, 1, true>), 1, 64, "8-bit/8 per-list 8x8 windows"),
-                                RV_VARIANT((hevc_stage0<2, 6, 1, true>), 2, 64, "deep/6 per-list 8x8 windows")};
+    StageVariant { reinterpret_cast<const void*>(&k), d, t, n, nullptr }
+#define RV_VARIANT0(k, s, d, n) \
+    StageVariant { reinterpret_cast<const void*>(&k), d, 64, n, reinterpret_cast<const void*>(&s) }
+// Stage 0 pairs a tile kernel (side data, intra copies, the blocks of
+// deeper pictures and blocks whose windows cross a plane edge) with a strip
+// kernel (the other blocks of 8-bit pictures; deeper pictures have none).
+const StageVariant kStage0[] = {
+    RV_VARIANT0((hevc_stage0<0, 5>), (hevc_mc_strips<3>), 0, "any/5, strips/3"),
+    RV_VARIANT0((hevc_stage0<1, 7>), (hevc_mc_strips<3>), 1, "8-bit/7, strips/3"),
+    RV_VARIANT0((hevc_stage0<1, 8, 1>), (hevc_mc_strips<4>), 1, "8-bit/8 per-list, strips/4"),
+    RV_VARIANT0((hevc_stage0<1, 8, 1>), (hevc_mc_strips<3>), 1, "8-bit/8 per-list, strips/3"),
+    RV_VARIANT0((hevc_stage0<2, 5>), (hevc_mc_strips<3>), 2, "deep/5"),
+    RV_VARIANT0((hevc_stage0<2, 6, 1>), (hevc_mc_strips<3>), 2, "deep/6 per-list"),
+    RV_VARIANT0((hevc_stage0<0, 6, 1>), (hevc_mc_strips<3>), 0, "any/6 per-list, strips/3"),
+    RV_VARIANT0((hevc_stage0<1, 8, 1, true>), (hevc_mc_strips<3>), 1, "8-bit/8 per-list 8x8 windows, strips/3"),
+    RV_VARIANT0((hevc_stage0<2, 6, 1, true>), (hevc_mc_strips<3>), 2, "deep/6 per-list 8x8 windows")};
 const StageVariant kStage1[] = {RV_VARIANT((hevc_stage1<0, 4>), 0, 64, "any/4"),
                                 RV_VARIANT((hevc_stage1<1, 4>), 1, 64, "8-bit/4"),
                                 RV_VARIANT((hevc_stage1<2, 4>), 2, 64, "deep/4")};
@@ -2479,6 +2709,7 @@ const StageVariant kStage2[] = {RV_VARIANT(hevc_stage2, 0, 256, "any")};
 const StageVariant kStage3[] = {RV_VARIANT(hevc_stage3, 0, 256, "any")};
 const StageVariant kStage4[] = {RV_VARIANT(hevc_stage4<0>, 0, 256, "any"), RV_VARIANT(hevc_stage4<1>, 1, 256, "8-bit"),
                                 RV_VARIANT(hevc_stage4<2>, 2, 256, "deep")};
+#undef RV_VARIANT0
 #undef RV_VARIANT
 struct StageTable {
     const StageVariant* v;
@@ -2504,6 +2735,13 @@ hipError_t hevc_launch_stage(int stage, const BatchDev& b, int total, hipStream_
         const StageVariant& k = kStageTables[stage].v[variant];
         const dim3 grid(8 * ((total + 7) / 8));  // see xcd_item()
         const hipError_t e = hipLaunchKernel(k.fn, grid, dim3(k.threads), args, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    // Stage 0 also runs the strip kernel (blocks with interior windows), of
+    // the stage variant's depth class.
+    if (stage == 0 && b.stotal > 0) {
+        const void* fn = kStageTables[0].v[variant].strips;
+        const hipError_t e = hipLaunchKernel(fn, dim3(8 * ((b.stotal + 7) / 8)), dim3(64), args, 0, st);
         if (e != hipSuccess) return e;
     }
     // Stage 1 ends with the intra prediction of device-intra CUs: the

@@ -959,13 +959,52 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         // PUs by shape (w, h descending), the side-data classes.
         auto pu_bucket = [](const rv_hevc_pu& u) { return (size_t)((16 - u.w / 4) * 16 + (16 - u.h / 4)); };
         std::vector<int> pu_cnt(257, 0);
+        // MC blocks of a PU: the 16-aligned core in 16x16 tiles, the right
+        // and bottom strips in blocks of at most 8x8. A block of an 8-bit
+        // picture whose windows (luma 8-tap, chroma 4-tap, every list) lie
+        // inside the planes goes to the strip kernel, the others to the
+        // tile kernel.
+        const int cw_pl = W / 2, ch_pl = H / 2;
+        // Strips serve 8-bit pictures only (deeper ones take the tile kernel).
+        const bool strips_ok = p->bit_depth == 8;
+        auto interior = [&](const rv_hevc_pu& u, int dx, int dy, int tw, int th) {
+            if (!strips_ok) return false;
+            for (int l = 0; l < 2; ++l) {
+                if (!((u.dir >> l) & 1)) continue;
+                const int mvx = u.mv[l].x, mvy = u.mv[l].y;
+                const int lx = u.x + dx + (mvx >> 2) - 3, ly = u.y + dy + (mvy >> 2) - 3;
+                if (lx < 0 || lx + tw + 7 > W || ly < 0 || ly + th + 7 > H) return false;
+                const int cx = (u.x + dx) / 2 + (mvx >> 3) - 1, cy = (u.y + dy) / 2 + (mvy >> 3) - 1;
+                if (cx < 0 || cx + tw / 2 + 3 > cw_pl || cy < 0 || cy + th / 2 + 3 > ch_pl) return false;
+            }
+            return true;
+        };
+        auto for_each_block = [](const rv_hevc_pu& u, auto&& f) {
+            const int w16 = u.w & ~15, h16 = u.h & ~15;
+            for (int dy = 0; dy < h16; dy += 16)
+                for (int dx = 0; dx < w16; dx += 16) f(dx, dy, 16, 16);
+            for (int dy = 0; dy < u.h; dy += 8)
+                for (int dx = dy < h16 ? w16 : 0; dx < u.w; dx += 8) f(dx, dy, std::min(8, u.w - dx), std::min(8, u.h - dy));
+        };
+        auto luma_group = [](int th, bool bi) { return (th == 16 ? 0 : th == 8 ? 2 : 4) + (bi ? 0 : 1); };
+        auto chroma_group = [](int th, bool bi) { return 6 + (th == 16 ? 0 : th == 8 ? 2 : 4) + (bi ? 0 : 1); };
+        // Strip segments: band of 128 luma rows (one band below 1024 rows),
+        // then group.
+        const int band_sh = H >= 1024 ? 7 : 14, n_bands = ((H - 1) >> band_sh) + 1;
+        std::vector<size_t> s_cnt((size_t)n_bands * rv::kStripGroups, 0);
+        auto seg = [&](int y, int g) { return (size_t)(y >> band_sh) * rv::kStripGroups + g; };
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
-            // The 16-aligned core in 16x16 tiles; the right and bottom
-            // strips in blocks of at most 8x8.
-            const int w16 = u.w & ~15, h16 = u.h & ~15;
-            n_tiles16 += (size_t)(w16 >> 4) * (h16 >> 4);
-            n_tiles += (size_t)((u.w - w16 + 7) >> 3) * ((u.h + 7) >> 3) + (size_t)(w16 >> 3) * ((u.h - h16 + 7) >> 3);
+            for_each_block(u, [&](int dx, int dy, int tw, int th) {
+                if (interior(u, dx, dy, tw, th)) {
+                    s_cnt[seg(u.y + dy, luma_group(th, u.dir == 3))] += (size_t)(tw / 4);
+                    s_cnt[seg(u.y + dy, chroma_group(th, u.dir == 3))] += (size_t)2 * ((tw / 2 + 3) / 4);
+                } else if (tw == 16 && th == 16) {
+                    ++n_tiles16;
+                } else {
+                    ++n_tiles;
+                }
+            });
             ++pu_cnt[pu_bucket(u) + 1];
             // Algorithmic bytes of motion compensation (the reference window
             // per list; whole-sample default-weighted uni reads the block).
@@ -980,6 +1019,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             }
         }
         const size_t n_tiles_pad = (n_tiles + 3) & ~(size_t)3;
+        std::vector<int32_t> s_wave0(s_cnt.size() + 1, 0);  // first wave of each segment
+        for (size_t k = 0; k < s_cnt.size(); ++k) s_wave0[k + 1] = s_wave0[k] + (int32_t)((s_cnt[k] + 63) / 64);
+        const int32_t n_swaves = s_wave0.back();
+        const size_t n_strips = (size_t)n_swaves * 64;
         int cu_cnt[5] = {0, 0, 0, 0, 0}, n_intra = 0;
         int64_t intra_area = 0;
         for (int i = 0; i < p->n_cus; ++i) {
@@ -1044,6 +1087,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
                     carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
+                    carve_size<rv::McStrip>(n_strips) + carve_size<uint8_t>(n_swaves) + carve_size<int32_t>(2) +
                     carve_size<rv::SideClass>(n_sclass) +
                     carve_size<rv_nz>(n_nzs + n_inz) + carve_size<rv::IntraTu>(n_itus) +
                     carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)(n_ilevels + 1)) +
@@ -1070,6 +1114,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, n_ctus);
         rv::McTile* h_tiles = carve<rv::McTile>(hp, n_tiles_pad);
         rv::McTile* h_tiles16 = carve<rv::McTile>(hp, n_tiles16);
+        rv::McStrip* h_strips = carve<rv::McStrip>(hp, n_strips);
+        uint8_t* h_swg = carve<uint8_t>(hp, n_swaves);
+        int32_t* h_spre = carve<int32_t>(hp, 2);
         rv::SideClass* h_sclass = carve<rv::SideClass>(hp, n_sclass);
         if (n_sclass) std::memcpy(h_sclass, sclass, n_sclass * sizeof(rv::SideClass));
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs + n_inz);
@@ -1086,6 +1133,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         int32_t* h_pre = carve<int32_t>(hp, rv::kStages * 2);
         // Pass 2: write.
         size_t nt = 0, nt16 = 0;
+        std::vector<size_t> s_at(s_cnt.size());
+        for (size_t k = 0; k < s_cnt.size(); ++k) s_at[k] = (size_t)s_wave0[k] * 64;
         for (int i = 0; i < p->n_pus; ++i) {
             const rv_hevc_pu& u = p->pus[i];
             h_pus[pu_cnt[pu_bucket(u)]++] = u;
@@ -1102,14 +1151,43 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                 t.mv[0] = u.mv[0];
                 t.mv[1] = u.mv[1];
             };
-            const int w16 = u.w & ~15, h16 = u.h & ~15;
-            for (int dy = 0; dy < h16; dy += 16)
-                for (int dx = 0; dx < w16; dx += 16) tile(h_tiles16[nt16++], dx, dy, 16, 16);
-            for (int dy = 0; dy < u.h; dy += 8)
-                for (int dx = dy < h16 ? w16 : 0; dx < u.w; dx += 8)
-                    tile(h_tiles[nt++], dx, dy, std::min(8, u.w - dx), std::min(8, u.h - dy));
+            auto strip = [&](size_t k, int x, int y, int cols, int comp) {
+                rv::McStrip& t = h_strips[s_at[k]++];
+                t.x = (uint16_t)x;
+                t.y = (uint16_t)y;
+                t.cols = (uint8_t)cols;
+                t.dir = u.dir;
+                t.refs = refs;
+                t.comp = (uint8_t)comp;
+                t.mv[0] = u.mv[0];
+                t.mv[1] = u.mv[1];
+            };
+            for_each_block(u, [&](int dx, int dy, int tw, int th) {
+                if (interior(u, dx, dy, tw, th)) {
+                    const size_t gl = seg(u.y + dy, luma_group(th, u.dir == 3)),
+                                 gc = seg(u.y + dy, chroma_group(th, u.dir == 3));
+                    for (int sx = 0; sx < tw; sx += 4) strip(gl, u.x + dx + sx, u.y + dy, 4, 0);
+                    for (int comp = 1; comp <= 2; ++comp)
+                        for (int sx = 0; sx < tw / 2; sx += 4)
+                            strip(gc, (u.x + dx) / 2 + sx, (u.y + dy) / 2, std::min(4, tw / 2 - sx), comp);
+                } else if (tw == 16 && th == 16) {
+                    tile(h_tiles16[nt16++], dx, dy, 16, 16);
+                } else {
+                    tile(h_tiles[nt++], dx, dy, tw, th);
+                }
+            });
         }
         if (n_tiles_pad > nt) std::memset(h_tiles + nt, 0, (n_tiles_pad - nt) * sizeof(rv::McTile));
+        // Each segment padded to whole waves with its first strip, no
+        // columns written; each wave's group.
+        for (size_t k = 0; k < s_cnt.size(); ++k) {
+            const size_t g0 = (size_t)s_wave0[k] * 64;
+            for (size_t i = s_at[k]; i < (size_t)s_wave0[k + 1] * 64; ++i) {
+                h_strips[i] = h_strips[g0];
+                h_strips[i].cols = 0;
+            }
+            for (int32_t w = s_wave0[k]; w < s_wave0[k + 1]; ++w) h_swg[w] = (uint8_t)(k % rv::kStripGroups);
+        }
         for (int i = 0, ni = 0; i < p->n_cus; ++i) {
             const rv_hevc_cu& c = p->cus[i];
             const int at = cu_cnt[6 - c.log2_size]++;
@@ -1236,6 +1314,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.ctus = (const rv_hevc_tu*)dev((char*)h_ctus);
         d.tiles = (const rv::McTile*)dev((char*)h_tiles);
         d.tiles16 = (const rv::McTile*)dev((char*)h_tiles16);
+        d.strips = (const rv::McStrip*)dev((char*)h_strips);
+        d.swg = (const uint8_t*)dev((char*)h_swg);
+        d.n_swaves = n_swaves;
         d.nz = (const rv_nz*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.sao = (const rv_sao*)dev((char*)h_sao);
@@ -1278,6 +1359,10 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         j->single.iall = (int32_t)n_itus;
         j->single.itotal = j->itotal.data();
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
+        h_spre[0] = 0;
+        h_spre[1] = n_swaves;
+        j->single.spre = (const int32_t*)dev((char*)h_spre);
+        j->single.stotal = h_spre[1];
         // The per-4x4 maps need no clearing: for work lists whose CUs, PUs
         // and luma TUs tile the picture (checked by area above) stage 0
         // writes every cell and stage 1 every strength, and a cell left
@@ -1529,6 +1614,10 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
             b->itotal[l] = row[nh];
             ibase[(size_t)l + 1] = ibase[(size_t)l] + row[nh];
         }
+        // Motion-compensation strips: the prefix over the HEVC pictures of
+        // their waves.
+        std::vector<int32_t> spre((size_t)nh + 1, 0);
+        for (int i = 0; i < nh; ++i) spre[(size_t)i + 1] = spre[(size_t)i] + b->hevc[i]->hd.n_swaves;
         const int32_t iall = ipre[(size_t)nh];
         if (iall > 0 && !pool_get((size_t)iall * sizeof(rv::IntraTask), &b->itask)) {
             delete b;
@@ -1536,7 +1625,7 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
                     carve_size<int32_t>(b->pre_4.size()) + carve_size<int32_t>(ipre.size()) +
-                    carve_size<int32_t>(ibase.size());
+                    carve_size<int32_t>(ibase.size()) + carve_size<int32_t>(spre.size());
         std::vector<uint32_t> host(sz / 4 + 1, 0u);
         char* hbase = (char*)host.data();
         if (!pool_get(sz, &b->tables)) {
@@ -1555,6 +1644,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         std::memcpy(hipre, ipre.data(), ipre.size() * 4);
         int32_t* hibase = carve<int32_t>(hp, ibase.size());
         std::memcpy(hibase, ibase.data(), ibase.size() * 4);
+        int32_t* hspre = carve<int32_t>(hp, spre.size());
+        std::memcpy(hspre, spre.data(), spre.size() * 4);
         for (int i = 0; i < nh; ++i) ph[i] = b->hevc[i]->desc_dev;
         for (int i = 0; i < n4; ++i) p4[i] = b->h264[i]->desc_dev;
         std::memcpy(preh, b->pre_h.data(), b->pre_h.size() * 4);
@@ -1568,6 +1659,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->bh.itask = (rv::IntraTask*)b->itask.p;
         b->bh.iall = iall;
         b->bh.itotal = b->itotal.data();
+        b->bh.spre = (const int32_t*)dev((char*)hspre);
+        b->bh.stotal = spre[(size_t)nh];
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);

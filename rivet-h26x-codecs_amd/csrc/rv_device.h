@@ -23,6 +23,27 @@ struct alignas(16) McTile {
 };
 static_assert(sizeof(McTile) == 16, "one 16-byte load per tile");
 
+// One strip of motion compensation: 4 output columns (2 for the chroma of
+// 4-wide blocks) by H rows of one component of a block whose reference
+// windows all lie inside the reference planes. A lane computes a strip's
+// whole first stage (H + 7 luma or H + 3 chroma rows) in registers, so no
+// window is shared and no sample is clamped. Strips are grouped by kind,
+// height and list use; each group's strips of one band of the picture are
+// padded to whole waves with copies of their first strip that have
+// cols = 0.
+struct alignas(16) McStrip {
+    uint16_t x, y;  // top-left output sample, in samples of the component
+    uint8_t cols;   // output columns: 4, 2, or 0 for padding
+    uint8_t dir;    // bit 0: list 0, bit 1: list 1
+    uint8_t refs;   // ref_idx of list 0 in bits 0..3, list 1 in bits 4..7
+    uint8_t comp;   // 0 luma, 1 Cb, 2 Cr
+    rv_mv mv[2];    // the PU's luma quarter-sample vectors
+};
+static_assert(sizeof(McStrip) == 16, "one 16-byte load per strip");
+// Groups: luma heights 16, 8, 4 then chroma heights 8, 4, 2, each bi then
+// uni (group = kind * 6 + height class * 2 + (uni ? 1 : 0)).
+constexpr int kStripGroups = 12;
+
 // One transform block of an RV_INTRA_DEVICE CU: the TU (coef_off rebased
 // into the job's nonzero list), its intra prediction mode, where its
 // residual goes and its dependency level.
@@ -79,6 +100,14 @@ struct HevcPicDev {
     const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
     const McTile* tiles;     // small blocks
     const McTile* tiles16;   // 16x16 tiles
+    // Strips of the blocks whose windows lie inside the reference planes
+    // (the tiles above hold the others), 64 per wave: wave w takes
+    // strips[64 w ..] of group swg[w]. Waves go band by band of 128 luma
+    // rows, all groups of a band before the next band, so a band's
+    // reference lines are reused from L2 by every group.
+    const McStrip* strips;
+    const uint8_t* swg;
+    int32_t n_swaves;
     const rv_hevc_tu* ctus;  // coded TUs grouped by size, smallest first
     const rv_nz* nz;         // nonzero coefficients of ctus, in ctus order
     const uint16_t* intra_pred;
@@ -214,6 +243,10 @@ struct BatchDev {
     const HevcPicDev* const* hevc;  // per picture (null for H.264)
     const H264PicDev* const* h264;
     const int32_t* prefix;          // kStages x (n + 1)
+    // HEVC motion-compensation strips: prefix over pictures of their waves
+    // (device memory, n + 1) and the total (host: read by the launcher).
+    const int32_t* spre;
+    int32_t stotal;
     // Device intra (HEVC): the most dependency levels of any picture; the
     // prefix over pictures of their TU counts, all levels first, then per
     // level (device memory, (levels + 1) x (n + 1)); the first task of each
