@@ -66,6 +66,11 @@ typedef struct rv_gen_h264_cfg {
                                transforms Intra_16x16 (luma DC group) half
                                the time                                     */
     int32_t scaling;        /* 1: seeded scaling lists (factors 4..64)     */
+    int32_t intra_device;   /* 1: the intra macroblocks are RV_INTRA_DEVICE
+                               with a seeded kind (I4X4 / I8X8 / I16X16)
+                               and seeded modes among those their
+                               neighbours allow                            */
+    int32_t constrained_intra_pred;  /* copied */
 } rv_gen_h264_cfg;
 
 typedef struct rv_gen rv_gen;

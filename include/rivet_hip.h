@@ -225,18 +225,40 @@ typedef struct rv_hevc_pic {
 /* ------------------------------------------------------------------ */
 
 /* One macroblock (raster order, every MB of the picture): the facts the
- * deblocking filter reads (MbInfo, src/h264/mb.rs:322-369). */
+ * deblocking filter reads (MbInfo, src/h264/mb.rs:322-369) and, for
+ * macroblocks predicted on the device, the prediction modes the parser
+ * read (MbLayer's intra16_mode / intra_modes / chroma_mode,
+ * src/h264/recon.rs:611-828). */
 typedef struct rv_h264_mb {
-    uint8_t intra;          /* 1 = intra MB: prediction supplied by host */
+    uint8_t intra;          /* 0 inter, RV_INTRA_HOST (prediction
+                               supplied in intra_pred) or RV_INTRA_DEVICE */
     uint8_t transform_8x8;
     int8_t qp;              /* QP_Y                                   */
     int8_t qpc[2];          /* QP_C for Cb, Cr                        */
     uint8_t pad;
     uint16_t nz_mask;       /* 4x4 luma blocks with coefficients (raster;
                                all four of an 8x8 when any)             */
-    uint32_t pred_off;      /* intra: offset into intra_pred (Y 256,
-                               Cb 64, Cr 64)                            */
+    uint32_t pred_off;      /* RV_INTRA_HOST: offset into intra_pred
+                               (Y 256, Cb 64, Cr 64)                    */
+    uint8_t intra_kind;     /* RV_INTRA_DEVICE: RV_H264_I4X4, RV_H264_I8X8
+                               or RV_H264_I16X16                        */
+    uint8_t chroma_mode;    /* RV_INTRA_DEVICE: intra_chroma_pred_mode
+                               (0 DC, 1 horizontal, 2 vertical, 3 plane) */
+    uint8_t pad2[2];
+    uint8_t modes[16];      /* RV_INTRA_DEVICE: Intra4x4PredMode per 4x4
+                               block in raster order (I4X4); the 8x8
+                               blocks' modes at raster 0, 2, 8, 10 (I8X8);
+                               Intra16x16PredMode in modes[0] (I16X16) */
 } rv_h264_mb;
+
+/* Kinds of RV_INTRA_DEVICE H.264 macroblocks. Their residual blocks are
+ * listed like any other (4x4 for I4X4 and I16X16, the latter with has_dc;
+ * 8x8 for I8X8) and added after the prediction of their block, in decoding
+ * order within the macroblock; macroblocks are predicted in dependency
+ * order after the transform stage. */
+#define RV_H264_I4X4 0
+#define RV_H264_I8X8 1
+#define RV_H264_I16X16 2
 
 /* One inter partition (the arguments of predict_partition,
  * src/h264/inter.rs:172-185). */
@@ -314,6 +336,9 @@ typedef struct rv_h264_pic {
     /* Scaling lists of RV_BLK_LEVELS blocks; NULL = flat 16
      * (Flat_4x4_16 / Flat_8x8_16). */
     const rv_h264_scaling* scaling;
+    /* constrained_intra_pred_flag: RV_INTRA_DEVICE macroblocks predict
+     * only from intra macroblocks. */
+    int32_t constrained_intra_pred;
 } rv_h264_pic;
 
 /* ------------------------------------------------------------------ */

@@ -87,6 +87,9 @@ _sig = {
     "orc_h264_chroma_dc": (None, [i32p, C.c_int, C.c_int]),
     "orc_h264_implicit_pair": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p]),
     "orc_h264_reconstruct": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), C.c_int]),
+    "orc_h264_intra_pred": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int]),
+    "orc_h264_filter8": (None, [i32p, C.c_int, i32p]),
+    "orc_h264_block_avail": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_crc16": (C.c_uint16, [P, C.c_size_t]),
     "orc_checksum": (C.c_uint32, [P, C.c_size_t, C.c_int, C.c_int]),
     "orc_md5": (None, [P, C.c_size_t, u8p]),

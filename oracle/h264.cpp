@@ -721,6 +721,278 @@ void mb_strengths(const rv_h264_pic* pic, const MbSide& sd, int addr, uint32_t b
     }
 }
 
+
+// ---- Intra prediction (clause 8.3), as src/h264/intra.rs -----------------
+// Which neighbours of a block may be read (IntraAvail, src/h264/intra.rs:
+// 10-21).
+struct Av {
+    bool top, left, tl, tr;
+};
+
+// Reference samples of an N x N block (N = 4, 8): t[0..2N) above, the
+// top-right half repeated from t[N - 1] where unavailable; l[0..N) to the
+// left; c the corner; zero where unavailable (Edge4::gather / Edge8::gather,
+// src/h264/intra.rs:39-75, 333-362). d points at the block's first sample.
+struct Nbr {
+    int t[16], l[8], c;
+};
+Nbr gather_nbr(const uint8_t* d, long stride, int n, Av av) {
+    Nbr e;
+    std::memset(&e, 0, sizeof(e));
+    if (av.top) {
+        for (int x = 0; x < n; ++x) e.t[x] = d[x - stride];
+        for (int x = n; x < 2 * n; ++x) e.t[x] = av.tr ? d[x - stride] : e.t[n - 1];
+    }
+    if (av.left)
+        for (int y = 0; y < n; ++y) e.l[y] = d[y * stride - 1];
+    if (av.tl) e.c = d[-stride - 1];
+    return e;
+}
+
+// The Intra_8x8 reference filter (8.3.2.2.1, src/h264/intra.rs:363-404):
+// [1 2 1] / 4 along the edge, the ends mirrored or taken from the corner.
+Nbr filter8(const Nbr& e, Av av) {
+    Nbr f;
+    std::memset(&f, 0, sizeof(f));
+    if (av.top) {
+        f.t[0] = av.tl ? (e.c + 2 * e.t[0] + e.t[1] + 2) >> 2 : (3 * e.t[0] + e.t[1] + 2) >> 2;
+        for (int x = 1; x < 15; ++x) f.t[x] = (e.t[x - 1] + 2 * e.t[x] + e.t[x + 1] + 2) >> 2;
+        f.t[15] = (e.t[14] + 3 * e.t[15] + 2) >> 2;
+    }
+    if (av.tl) {
+        if (av.top && av.left) f.c = (e.t[0] + 2 * e.c + e.l[0] + 2) >> 2;
+        else if (av.top) f.c = (3 * e.c + e.t[0] + 2) >> 2;
+        else if (av.left) f.c = (3 * e.c + e.l[0] + 2) >> 2;
+        else f.c = e.c;
+    }
+    if (av.left) {
+        f.l[0] = av.tl ? (e.c + 2 * e.l[0] + e.l[1] + 2) >> 2 : (3 * e.l[0] + e.l[1] + 2) >> 2;
+        for (int y = 1; y < 7; ++y) f.l[y] = (e.l[y - 1] + 2 * e.l[y] + e.l[y + 1] + 2) >> 2;
+        f.l[7] = (e.l[6] + 3 * e.l[7] + 2) >> 2;
+    }
+    return f;
+}
+
+// Whether an Intra_4x4 / Intra_8x8 mode has the neighbours it reads
+// (mode_nxn_available, src/h264/intra.rs:78-88).
+bool nxn_mode_ok(int mode, Av av) {
+    switch (mode) {
+        case 0: case 3: case 7: return av.top;
+        case 1: case 8: return av.left;
+        case 2: return true;
+        case 4: case 5: case 6: return av.top && av.left && av.tl;
+        default: return false;
+    }
+}
+
+// The nine directional modes of an N x N block (N = 4: 8.3.1.2.1-9, N = 8:
+// 8.3.2.2.2-10; pred_4x4 / pred_8x8, src/h264/intra.rs:109-325, 427-620),
+// written once for both sizes: the 4x4 and 8x8 formulas differ only in N.
+// out is raster with stride N; false when the mode is out of range or lacks
+// its neighbours.
+bool pred_nxn(const Nbr& e, int n, int mode, Av av, int bd, uint8_t* out) {
+    if (!nxn_mode_ok(mode, av)) return false;
+    const int lg = n == 4 ? 2 : 3;
+    // p[x, -1] for x >= -1 on the top row (x = -1 the corner), p[-1, y]
+    // for y >= 0 down the left column.
+    auto T = [&](int x) { return x < 0 ? e.c : e.t[x]; };
+    auto L = [&](int y) { return y < 0 ? e.c : e.l[y]; };
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            int v = 0;
+            switch (mode) {
+                case 0: v = e.t[x]; break;
+                case 1: v = e.l[y]; break;
+                case 2: {
+                    int st = 0, sl = 0;
+                    for (int k = 0; k < n; ++k) st += e.t[k], sl += e.l[k];
+                    v = av.top && av.left ? (st + sl + n) >> (lg + 1)
+                        : av.top          ? (st + n / 2) >> lg
+                        : av.left         ? (sl + n / 2) >> lg
+                                          : 1 << (bd - 1);
+                    break;
+                }
+                case 3:  // diagonal down left
+                    v = (x == n - 1 && y == n - 1) ? (e.t[2 * n - 2] + 3 * e.t[2 * n - 1] + 2) >> 2
+                                                   : (e.t[x + y] + 2 * e.t[x + y + 1] + e.t[x + y + 2] + 2) >> 2;
+                    break;
+                case 4:  // diagonal down right
+                    if (x > y) v = (T(x - y - 2) + 2 * T(x - y - 1) + T(x - y) + 2) >> 2;
+                    else if (x < y) v = (L(y - x - 2) + 2 * L(y - x - 1) + L(y - x) + 2) >> 2;
+                    else v = (T(0) + 2 * e.c + L(0) + 2) >> 2;
+                    break;
+                case 5: {  // vertical right
+                    const int z = 2 * x - y;
+                    if (z >= 0 && !(z & 1)) v = (T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 1) >> 1;
+                    else if (z >= 0) v = (T(x - (y >> 1) - 2) + 2 * T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 2) >> 2;
+                    else if (z == -1) v = (L(0) + 2 * e.c + T(0) + 2) >> 2;
+                    else v = (L(y - 2 * x - 1) + 2 * L(y - 2 * x - 2) + L(y - 2 * x - 3) + 2) >> 2;
+                    break;
+                }
+                case 6: {  // horizontal down
+                    const int z = 2 * y - x;
+                    if (z >= 0 && !(z & 1)) v = (L(y - (x >> 1) - 1) + L(y - (x >> 1)) + 1) >> 1;
+                    else if (z >= 0) v = (L(y - (x >> 1) - 2) + 2 * L(y - (x >> 1) - 1) + L(y - (x >> 1)) + 2) >> 2;
+                    else if (z == -1) v = (L(0) + 2 * e.c + T(0) + 2) >> 2;
+                    else v = (T(x - 2 * y - 1) + 2 * T(x - 2 * y - 2) + T(x - 2 * y - 3) + 2) >> 2;
+                    break;
+                }
+                case 7:  // vertical left
+                    v = (y & 1) ? (e.t[x + (y >> 1)] + 2 * e.t[x + (y >> 1) + 1] + e.t[x + (y >> 1) + 2] + 2) >> 2
+                                : (e.t[x + (y >> 1)] + e.t[x + (y >> 1) + 1] + 1) >> 1;
+                    break;
+                default: {  // 8: horizontal up
+                    const int z = x + 2 * y;
+                    if (z > 2 * n - 3) v = e.l[n - 1];
+                    else if (z == 2 * n - 3) v = (e.l[n - 2] + 3 * e.l[n - 1] + 2) >> 2;
+                    else if (!(z & 1)) v = (e.l[y + (x >> 1)] + e.l[y + (x >> 1) + 1] + 1) >> 1;
+                    else v = (e.l[y + (x >> 1)] + 2 * e.l[y + (x >> 1) + 1] + e.l[y + (x >> 1) + 2] + 2) >> 2;
+                    break;
+                }
+            }
+            out[y * n + x] = (uint8_t)v;
+        }
+    return true;
+}
+
+// Intra_16x16 (8.3.3; predict_16x16 / predict_planar_block,
+// src/h264/intra.rs:622-646, 792-923) at d: 0 vertical, 1 horizontal,
+// 2 DC, 3 plane.
+bool pred16(uint8_t* d, long stride, int mode, Av av, int bd) {
+    const int max = (1 << bd) - 1;
+    auto top = [&](int x) { return (int)d[x - stride]; };   // x = -1: the corner
+    auto left = [&](int y) { return (int)d[y * stride - 1]; };  // y = -1: the corner
+    uint8_t out[256];
+    if (mode == 0) {
+        if (!av.top) return false;
+        for (int i = 0; i < 256; ++i) out[i] = (uint8_t)top(i & 15);
+    } else if (mode == 1) {
+        if (!av.left) return false;
+        for (int i = 0; i < 256; ++i) out[i] = (uint8_t)left(i >> 4);
+    } else if (mode == 2) {
+        int st = 0, sl = 0;
+        for (int k = 0; k < 16; ++k) st += av.top ? top(k) : 0, sl += av.left ? left(k) : 0;
+        const int v = av.top && av.left ? (st + sl + 16) >> 5 : av.top ? (st + 8) >> 4 : av.left ? (sl + 8) >> 4
+                                                                                                  : 1 << (bd - 1);
+        std::memset(out, v, 256);
+    } else if (mode == 3) {
+        if (!(av.top && av.left && av.tl)) return false;
+        int hh = 0, vv = 0;
+        for (int i = 0; i < 8; ++i) {
+            hh += (i + 1) * (top(8 + i) - top(6 - i));
+            vv += (i + 1) * (left(8 + i) - left(6 - i));
+        }
+        const int b = (5 * hh + 32) >> 6, c = (5 * vv + 32) >> 6, a = 16 * (left(15) + top(15));
+        for (int y = 0; y < 16; ++y)
+            for (int x = 0; x < 16; ++x) out[y * 16 + x] = (uint8_t)clampi((a + b * (x - 7) + c * (y - 7) + 16) >> 5, 0, max);
+    } else {
+        return false;
+    }
+    for (int y = 0; y < 16; ++y) std::memcpy(d + y * stride, out + y * 16, 16);
+    return true;
+}
+
+// 4:2:0 chroma prediction (8.3.4; predict_chroma, src/h264/intra.rs:648-790)
+// of the 8 x 8 block at d: 0 DC per 4x4 block (the top-left and bottom-right
+// blocks average both edges, the top-right prefers the top, the bottom-left
+// the left), 1 horizontal, 2 vertical, 3 plane.
+bool pred_chroma(uint8_t* d, long stride, int mode, Av av, int bd) {
+    const int max = (1 << bd) - 1;
+    auto top = [&](int x) { return (int)d[x - stride]; };
+    auto left = [&](int y) { return (int)d[y * stride - 1]; };
+    uint8_t out[64];
+    if (mode == 0) {
+        for (int by = 0; by < 2; ++by)
+            for (int bx = 0; bx < 2; ++bx) {
+                int st = 0, sl = 0;
+                for (int k = 0; k < 4; ++k) {
+                    st += av.top ? top(bx * 4 + k) : 0;
+                    sl += av.left ? left(by * 4 + k) : 0;
+                }
+                int v;
+                const int none = 1 << (bd - 1);
+                if (bx == by) v = av.top && av.left ? (st + sl + 4) >> 3 : av.top ? (st + 2) >> 2 : av.left ? (sl + 2) >> 2 : none;
+                else if (bx) v = av.top ? (st + 2) >> 2 : av.left ? (sl + 2) >> 2 : none;
+                else v = av.left ? (sl + 2) >> 2 : av.top ? (st + 2) >> 2 : none;
+                for (int y = 0; y < 4; ++y)
+                    for (int x = 0; x < 4; ++x) out[(by * 4 + y) * 8 + bx * 4 + x] = (uint8_t)v;
+            }
+    } else if (mode == 1) {
+        if (!av.left) return false;
+        for (int i = 0; i < 64; ++i) out[i] = (uint8_t)left(i >> 3);
+    } else if (mode == 2) {
+        if (!av.top) return false;
+        for (int i = 0; i < 64; ++i) out[i] = (uint8_t)top(i & 7);
+    } else if (mode == 3) {
+        if (!(av.top && av.left && av.tl)) return false;
+        int hh = 0, vv = 0;
+        for (int i = 0; i < 4; ++i) {
+            hh += (i + 1) * (top(4 + i) - top(2 - i));
+            vv += (i + 1) * (left(4 + i) - left(2 - i));
+        }
+        const int b = (34 * hh + 32) >> 6, c = (34 * vv + 32) >> 6, a = 16 * (left(7) + top(7));
+        for (int y = 0; y < 8; ++y)
+            for (int x = 0; x < 8; ++x) out[y * 8 + x] = (uint8_t)clampi((a + b * (x - 3) + c * (y - 3) + 16) >> 5, 0, max);
+    } else {
+        return false;
+    }
+    for (int y = 0; y < 8; ++y) std::memcpy(d + y * stride, out + y * 8, 8);
+    return true;
+}
+
+// Decoding index of the 4x4 block at raster (bx, by) of a macroblock: 8x8
+// quadrants in z order, 4x4 blocks in z order within (BLK4X4_FROM_RASTER).
+inline int blk4_order(int bx, int by) { return ((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1); }
+
+// Availability of a macroblock's neighbours for intra prediction in a
+// one-slice frame: left (A), above (B), above-right (C), above-left (D),
+// each present and, under constrained_intra_pred, intra (intra_ok,
+// src/h264/recon.rs:301-307).
+struct MbAv {
+    bool a, b, c, d;
+};
+MbAv mb_avail(const rv_h264_pic* pic, int addr) {
+    const int mbw = pic->width / 16, mbx = addr % mbw, mby = addr / mbw;
+    auto ok = [&](int n) { return !pic->constrained_intra_pred || pic->mbs[n].intra != 0; };
+    MbAv m;
+    m.a = mbx > 0 && ok(addr - 1);
+    m.b = mby > 0 && ok(addr - mbw);
+    m.c = mby > 0 && mbx < mbw - 1 && ok(addr - mbw + 1);
+    m.d = mbx > 0 && mby > 0 && ok(addr - mbw - 1);
+    return m;
+}
+// intra_avail_4x4 / intra_avail_8x8 (src/h264/recon.rs:936-1011) for the
+// block at 4x4 raster (bx, by) of size 4 or 8.
+Av block_avail(const MbAv& m, int bx, int by, int size) {
+    Av av;
+    av.left = bx > 0 || m.a;
+    av.top = by > 0 || m.b;
+    av.tl = (bx > 0 && by > 0) || (bx == 0 ? (by == 0 ? m.d : m.a) : m.b);
+    const int w4 = size / 4;
+    if (by == 0) av.tr = bx + w4 < 4 ? m.b : m.c;
+    else if (size == 8) av.tr = bx == 0;
+    else av.tr = bx < 3 && blk4_order(bx + 1, by - 1) < blk4_order(bx, by);
+    return av;
+}
+
+// Residual of one listed block into the picture (residual4 / residual8,
+// scaling levels first).
+bool add_block(const rv_h264_pic* pic, const rv_h264_blk& bk, uint8_t* dst, long stride) {
+    const int n = bk.size;
+    if (bk.coef_off + (uint64_t)n * n > pic->n_coefs) return false;
+    if (bk.flags & RV_BLK_LEVELS) {
+        int32_t scaled[64];
+        int dc;
+        if (!scale_block(pic, bk, scaled, &dc)) return false;
+        if (n == 8) residual8(dst, stride, scaled, 255);
+        else residual4(dst, stride, scaled, dc, bk.has_dc != 0, 255);
+    } else if (n == 8) {
+        residual8(dst, stride, pic->coefs + bk.coef_off, 255);
+    } else {
+        residual4(dst, stride, pic->coefs + bk.coef_off, bk.dc, bk.has_dc != 0, 255);
+    }
+    return true;
+}
 }  // namespace
 
 extern "C" {
@@ -788,6 +1060,50 @@ int orc_h264_bs_map(const rv_h264_pic* pic, uint32_t* out) {
     return RV_OK;
 }
 
+// Intra predictors on a caller's plane (d = the block's first sample;
+// every neighbour the availability flags admit must be readable). kind 0:
+// N x N directional (n = 4 or 8, with the 8x8 reference filter), 1:
+// Intra_16x16, 2: 4:2:0 chroma. av: bit 0 top, 1 left, 2 top-left,
+// 3 top-right. Returns 0, or RV_E_INVALID for a mode without its
+// neighbours.
+int orc_h264_intra_pred(int kind, int n, int mode, int av, uint8_t* d, int stride) {
+    const Av a{(av & 1) != 0, (av & 2) != 0, (av & 4) != 0, (av & 8) != 0};
+    if (kind == 0) {
+        if (n != 4 && n != 8) return RV_E_RANGE;
+        Nbr e = gather_nbr(d, stride, n, a);
+        if (n == 8) e = filter8(e, a);
+        uint8_t out[64];
+        if (!pred_nxn(e, n, mode, a, 8, out)) return RV_E_INVALID;
+        for (int y = 0; y < n; ++y) std::memcpy(d + (long)y * stride, out + y * n, n);
+        return RV_OK;
+    }
+    if (kind == 1) return pred16(d, stride, mode, a, 8) ? RV_OK : RV_E_INVALID;
+    if (kind == 2) return pred_chroma(d, stride, mode, a, 8) ? RV_OK : RV_E_INVALID;
+    return RV_E_RANGE;
+}
+
+// The Intra_8x8 reference filter alone: in = t[16], l[8], c; out likewise.
+void orc_h264_filter8(const int32_t in[25], int av, int32_t out[25]) {
+    const Av a{(av & 1) != 0, (av & 2) != 0, (av & 4) != 0, (av & 8) != 0};
+    Nbr e;
+    for (int i = 0; i < 16; ++i) e.t[i] = in[i];
+    for (int i = 0; i < 8; ++i) e.l[i] = in[16 + i];
+    e.c = in[24];
+    const Nbr f = filter8(e, a);
+    for (int i = 0; i < 16; ++i) out[i] = f.t[i];
+    for (int i = 0; i < 8; ++i) out[16 + i] = f.l[i];
+    out[24] = f.c;
+}
+
+// Availability (bit 0 top, 1 left, 2 top-left, 3 top-right) of the block
+// at 4x4 raster (bx, by), size 4 or 8 (16: the whole macroblock), of
+// macroblock addr.
+int orc_h264_block_avail(const rv_h264_pic* pic, int addr, int bx, int by, int size) {
+    const MbAv m = mb_avail(pic, addr);
+    const Av a = size == 16 ? Av{m.b, m.a, m.d, false} : block_avail(m, bx, by, size);
+    return (a.top ? 1 : 0) | (a.left ? 2 : 0) | (a.tl ? 4 : 0) | (a.tr ? 8 : 0);
+}
+
 int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes, int n_slots, void* const out[3],
                          int stop_after) {
     if (!pic || !slot_planes || !out) return RV_E_INVALID;
@@ -819,7 +1135,7 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
     // Intra macroblocks: host-supplied prediction.
     for (int a = 0; a < pic->n_mbs; ++a) {
         const rv_h264_mb& m = pic->mbs[a];
-        if (!m.intra) continue;
+        if (m.intra != RV_INTRA_HOST) continue;
         int mx = (a % mbw) * 16, my = (a / mbw) * 16;
         const uint16_t* pr = pic->intra_pred + m.pred_off;
         for (int c = 0; c < 3; ++c) {
@@ -898,24 +1214,82 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
             }
         }
     }
-    // Residual (src/h264/recon.rs:870-926, 1200-1226).
+    // Residual (src/h264/recon.rs:870-926, 1200-1226); the blocks of
+    // macroblocks predicted here are kept for their macroblock.
+    std::vector<std::vector<int>> own((size_t)pic->n_mbs);
     if (stop_after != 1) {
         for (int i = 0; i < pic->n_blks; ++i) {
             const rv_h264_blk& bk = pic->blks[i];
-            int n = bk.size;
-            if (bk.coef_off + (uint64_t)n * n > pic->n_coefs) return RV_E_RANGE;
+            if (bk.comp > 2 || (bk.size != 4 && bk.size != 8)) return RV_E_RANGE;
+            const int mb = bk.comp ? (bk.y / 8) * mbw + bk.x / 8 : (bk.y / 16) * mbw + bk.x / 16;
+            if (mb < 0 || mb >= pic->n_mbs) return RV_E_RANGE;
+            if (pic->mbs[mb].intra == RV_INTRA_DEVICE) {
+                own[(size_t)mb].push_back(i);
+                continue;
+            }
             Plane& pl = cur.p[bk.comp];
-            uint8_t* dst = &pl.data[pl.offset(bk.x, bk.y)];
-            if (bk.flags & RV_BLK_LEVELS) {
-                int32_t scaled[64];
-                int dc;
-                if (!scale_block(pic, bk, scaled, &dc)) return RV_E_RANGE;
-                if (n == 8) residual8(dst, pl.stride, scaled, max);
-                else residual4(dst, pl.stride, scaled, dc, bk.has_dc != 0, max);
-            } else if (n == 8) {
-                residual8(dst, pl.stride, pic->coefs + bk.coef_off, max);
+            if (!add_block(pic, bk, &pl.data[pl.offset(bk.x, bk.y)], pl.stride)) return RV_E_RANGE;
+        }
+        // Macroblocks predicted from their neighbours, in decoding order
+        // (src/h264/recon.rs:611-828, 1021-1128): luma block by block in
+        // decoding order, each block's residual added before the next is
+        // predicted (Intra_4x4, Intra_8x8) or the whole prediction first
+        // (Intra_16x16); then each chroma component and its residual.
+        for (int a = 0; a < pic->n_mbs; ++a) {
+            const rv_h264_mb& m = pic->mbs[a];
+            if (m.intra != RV_INTRA_DEVICE) continue;
+            const int mx = (a % mbw) * 16, my = (a / mbw) * 16;
+            const MbAv mav = mb_avail(pic, a);
+            // This macroblock's block at (comp, x, y), or -1.
+            auto find = [&](int comp, int x, int y) {
+                for (int i : own[(size_t)a])
+                    if (pic->blks[i].comp == comp && pic->blks[i].x == x && pic->blks[i].y == y) return i;
+                return -1;
+            };
+            Plane& Y = cur.p[0];
+            auto residual_at = [&](int comp, int x, int y) {
+                const int i = find(comp, x, y);
+                if (i < 0) return true;
+                Plane& pl = cur.p[comp];
+                return add_block(pic, pic->blks[i], &pl.data[pl.offset(x, y)], pl.stride);
+            };
+            if (m.intra_kind == RV_H264_I16X16) {
+                uint8_t* d = &Y.data[Y.offset(mx, my)];
+                if (!pred16(d, Y.stride, m.modes[0], Av{mav.b, mav.a, mav.d, false}, 8)) return RV_E_INVALID;
+                for (int r = 0; r < 16; ++r)
+                    if (!residual_at(0, mx + (r & 3) * 4, my + (r >> 2) * 4)) return RV_E_RANGE;
+            } else if (m.intra_kind == RV_H264_I4X4 || m.intra_kind == RV_H264_I8X8) {
+                const int n = m.intra_kind == RV_H264_I8X8 ? 8 : 4, nb = 16 / (n * n / 16);
+                for (int k = 0; k < nb; ++k) {
+                    // Decoding order: 8x8 quadrants in z order, and 4x4
+                    // blocks in z order within each.
+                    int bx, by;
+                    if (n == 8) {
+                        bx = (k & 1) * 2;
+                        by = (k >> 1) * 2;
+                    } else {
+                        bx = ((k >> 2) & 1) * 2 + (k & 1);
+                        by = (k >> 3) * 2 + ((k >> 1) & 1);
+                    }
+                    const Av av = block_avail(mav, bx, by, n);
+                    uint8_t* d = &Y.data[Y.offset(mx + bx * 4, my + by * 4)];
+                    Nbr e = gather_nbr(d, Y.stride, n, av);
+                    if (n == 8) e = filter8(e, av);
+                    uint8_t out[64];
+                    if (!pred_nxn(e, n, m.modes[by * 4 + bx], av, 8, out)) return RV_E_INVALID;
+                    for (int y = 0; y < n; ++y) std::memcpy(d + y * Y.stride, out + y * n, n);
+                    if (!residual_at(0, mx + bx * 4, my + by * 4)) return RV_E_RANGE;
+                }
             } else {
-                residual4(dst, pl.stride, pic->coefs + bk.coef_off, bk.dc, bk.has_dc != 0, max);
+                return RV_E_INVALID;
+            }
+            for (int comp = 1; comp < 3; ++comp) {
+                Plane& pl = cur.p[comp];
+                if (!pred_chroma(&pl.data[pl.offset(mx / 2, my / 2)], pl.stride, m.chroma_mode,
+                                 Av{mav.b, mav.a, mav.d, false}, 8))
+                    return RV_E_INVALID;
+                for (int b = 0; b < 4; ++b)
+                    if (!residual_at(comp, mx / 2 + (b & 1) * 4, my / 2 + (b >> 1) * 4)) return RV_E_RANGE;
             }
         }
     }

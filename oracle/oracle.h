@@ -134,6 +134,10 @@ int orc_h264_reconstruct(const rv_h264_pic* pic, const void* const* slot_planes,
  * the horizontal edges 0..3 (0 = its top edge). Byte k of a word is the bS
  * of the edge's 4-sample segment k (top to bottom / left to right). */
 int orc_h264_bs_map(const rv_h264_pic* pic, uint32_t* out);
+/* H.264 intra prediction (8.3, src/h264/intra.rs) on a caller's plane. */
+int orc_h264_intra_pred(int kind, int n, int mode, int av, uint8_t* d, int stride);
+void orc_h264_filter8(const int32_t in[25], int av, int32_t out[25]);
+int orc_h264_block_avail(const rv_h264_pic* pic, int addr, int bx, int by, int size);
 
 /* ---- Decoded picture hash (src/hevc/hash.rs) -------------------------
  * Over one component's message (n bytes): crc16 (:160-174), checksum

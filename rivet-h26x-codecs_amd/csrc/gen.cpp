@@ -450,6 +450,47 @@ struct H264Builder {
         g.parts.push_back(p);
     }
 
+    // Availability of a device-intra macroblock's neighbours (left, above,
+    // above-right, above-left: present and, under constrained intra
+    // prediction, intra) and of one block's neighbours, as the decoder
+    // derives them for a one-slice frame (src/h264/recon.rs:301-307,
+    // 936-1011); the modes drawn are among those these allow.
+    struct Nb {
+        bool a, b, c, d;
+    };
+    Nb mb_nb(int addr, int mbw) {
+        const int mbx = addr % mbw, mby = addr / mbw;
+        auto ok = [&](int n) { return !c.constrained_intra_pred || g.mbs[(size_t)n].intra != 0; };
+        return Nb{mbx > 0 && ok(addr - 1), mby > 0 && ok(addr - mbw), mby > 0 && mbx < mbw - 1 && ok(addr - mbw + 1),
+                  mbx > 0 && mby > 0 && ok(addr - mbw - 1)};
+    }
+    int pick(const bool (&allowed)[9], int n) {
+        int k = 0, list[9];
+        for (int i = 0; i < n; ++i)
+            if (allowed[i]) list[k++] = i;
+        return list[ri.uniform(0, k - 1)];
+    }
+    void intra_modes(rv_h264_mb& m, int addr, int mbw) {
+        const Nb nb = mb_nb(addr, mbw);
+        m.intra_kind = (uint8_t)ri.uniform(0, 2);
+        if (m.intra_kind == RV_H264_I16X16) {
+            const bool ok[9] = {nb.b, nb.a, true, nb.a && nb.b && nb.d};
+            m.modes[0] = (uint8_t)pick(ok, 4);
+        } else {
+            const int step = m.intra_kind == RV_H264_I8X8 ? 2 : 1;
+            for (int by = 0; by < 4; by += step)
+                for (int bx = 0; bx < 4; bx += step) {
+                    const bool left = bx > 0 || nb.a, top = by > 0 || nb.b;
+                    const bool tl = (bx > 0 && by > 0) || (bx == 0 ? (by == 0 ? nb.d : nb.a) : nb.b);
+                    const bool ok[9] = {top, left, true, top, top && left && tl, top && left && tl, top && left && tl,
+                                        top, left};
+                    m.modes[by * 4 + bx] = (uint8_t)pick(ok, 9);
+                }
+        }
+        const bool okc[9] = {true, nb.a, nb.b, nb.a && nb.b && nb.d};
+        m.chroma_mode = (uint8_t)pick(okc, 4);
+    }
+
     void run() {
         rv_h264_pic& p = g.h264;
         std::memset(&p, 0, sizeof(p));
@@ -466,6 +507,7 @@ struct H264Builder {
         }
         p.weighting = c.weighting;
         p.deblock = c.deblock;
+        p.constrained_intra_pred = c.constrained_intra_pred;
         g.mbs.resize((size_t)mbw * mbh);
         for (int a = 0; a < mbw * mbh; ++a) {
             int mx = (a % mbw) * 16, my = (a / mbw) * 16;
@@ -473,9 +515,11 @@ struct H264Builder {
             std::memset(&m, 0, sizeof(m));
             m.qp = (int8_t)rs.uniform(c.qp_min, c.qp_max);
             m.qpc[0] = m.qpc[1] = (int8_t)CHROMA_QP[m.qp];
-            m.intra = rs.percent(c.intra_percent) ? 1 : 0;
+            m.intra = rs.percent(c.intra_percent) ? (c.intra_device ? RV_INTRA_DEVICE : RV_INTRA_HOST) : 0;
             bool all8 = true;
-            if (m.intra) {
+            if (m.intra == RV_INTRA_DEVICE) {
+                intra_modes(m, a, mbw);
+            } else if (m.intra) {
                 m.pred_off = (uint32_t)g.intra_pred.size();
                 for (int i = 0; i < 256 + 128; ++i) g.intra_pred.push_back((uint16_t)ri.uniform(0, 255));
             } else {
@@ -498,6 +542,9 @@ struct H264Builder {
                 }
             }
             m.transform_8x8 = (all8 && rs.percent(50)) ? 1 : 0;
+            if (m.intra == RV_INTRA_DEVICE) m.transform_8x8 = m.intra_kind == RV_H264_I8X8 ? 1 : 0;
+            const bool i16 = m.intra == RV_INTRA_DEVICE ? m.intra_kind == RV_H264_I16X16
+                                                        : c.levels && m.intra && !m.transform_8x8 && rs.percent(50);
             uint16_t nz = 0;
             if (m.transform_8x8) {
                 for (int b8 = 0; b8 < 4; ++b8) {
@@ -506,12 +553,16 @@ struct H264Builder {
                         nz |= (uint16_t)((1u << (by * 4 + bx)) | (1u << (by * 4 + bx + 1)) |
                                          (1u << ((by + 1) * 4 + bx)) | (1u << ((by + 1) * 4 + bx + 1)));
                 }
-            } else if (c.levels && m.intra && rs.percent(50)) {
+            } else if (i16 && c.levels) {
                 // Intra_16x16: a luma DC group and sixteen blocks taking it.
                 bool gany;
                 const uint32_t go = dc_group(16, &gany);
                 for (int r = 0; r < 16; ++r)
                     if (level_block(0, mx + (r % 4) * 4, my + (r / 4) * 4, 4, true, go, gany)) nz |= (uint16_t)(1u << r);
+            } else if (i16) {
+                // Intra_16x16 with scaled values: each block's DC given.
+                for (int r = 0; r < 16; ++r)
+                    if (block(0, mx + (r % 4) * 4, my + (r / 4) * 4, 4, true)) nz |= (uint16_t)(1u << r);
             } else {
                 for (int r = 0; r < 16; ++r)
                     if (block(0, mx + (r % 4) * 4, my + (r / 4) * 4, 4, false)) nz |= (uint16_t)(1u << r);

@@ -94,7 +94,7 @@ class HevcPic(C.Structure):
 
 class H264Mb(C.Structure):
     _fields_ = [("intra", u8), ("transform_8x8", u8), ("qp", i8), ("qpc", i8 * 2), ("pad", u8), ("nz_mask", u16),
-                ("pred_off", u32)]
+                ("pred_off", u32), ("intra_kind", u8), ("chroma_mode", u8), ("pad2", u8 * 2), ("modes", u8 * 16)]
 
 
 class H264Part(C.Structure):
@@ -121,7 +121,7 @@ class H264Pic(C.Structure):
         ("n_mbs", i32), ("n_parts", i32), ("n_blks", i32),
         ("mbs", C.POINTER(H264Mb)), ("parts", C.POINTER(H264Part)), ("blks", C.POINTER(H264Blk)),
         ("coefs", C.POINTER(i32)), ("n_coefs", u32), ("n_intra_pred", u32), ("intra_pred", C.POINTER(u16)),
-        ("scaling", C.POINTER(H264Scaling)),
+        ("scaling", C.POINTER(H264Scaling)), ("constrained_intra_pred", i32),
     ]
 
 
@@ -138,7 +138,8 @@ class GenHevcCfg(C.Structure):
 class GenH264Cfg(C.Structure):
     _fields_ = [(n, i32) for n in ("width", "height", "b_picture", "bi_percent", "weighting", "intra_percent",
                                    "nz_permille", "lap_b", "qp_min", "qp_max", "mv_range", "deblock", "poc")] + [
-        ("ref_poc", i32 * 2), ("ref_slot", i32 * 2), ("levels", i32), ("scaling", i32)]
+        ("ref_poc", i32 * 2), ("ref_slot", i32 * 2), ("levels", i32), ("scaling", i32), ("intra_device", i32),
+        ("constrained_intra_pred", i32)]
 
 
 ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz, HevcScaling,

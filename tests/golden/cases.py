@@ -34,6 +34,8 @@ def _h264_small(kind: str):
     cfg = dict(cf.H264_C4, width=352, height=288)
     if kind == "p":
         cfg.update(b_picture=0, weighting=0, poc=1, ref_poc=[0, 0])
+    elif kind == "intra_device":
+        cfg.update(bi_percent=50, intra_percent=40, intra_device=1, constrained_intra_pred=1, levels=1)
     else:
         cfg.update(bi_percent=50, intra_percent=30)
     return cfg
@@ -48,6 +50,7 @@ CASES = {
     "h264_c4": cf.single(4),
     "h264_p_352x288": (rh.RV_CODEC_H264, _h264_small("p"), 0x264),
     "h264_b_mixed_352x288": (rh.RV_CODEC_H264, _h264_small("b_mixed"), 0x265),
+    "h264_b_intra_device_352x288": (rh.RV_CODEC_H264, _h264_small("intra_device"), 0x266),
 }
 
 
