@@ -395,10 +395,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h2
     int i = it - pre[p];
     const int tid = threadIdx.x;
     if (i < P.n_intra) {
-        // Intra macroblock: host-supplied prediction; no partition.
+        // Intra macroblock: no partition; host-supplied prediction copied
+        // (device-intra macroblocks are predicted after the transform stage).
         const int a = (int)P.part_order[P.n_parts + i];
         const rv_h264_mb m = P.mbs[a];
         const int mx = (a % P.mbw) * 16, my = (a / P.mbw) * 16;
+        if (tid < 16) P.partmap[(size_t)((my >> 2) + (tid >> 2)) * P.w4 + (mx >> 2) + (tid & 3)] = 0xFFFFFFFFu;
+        if (m.intra != RV_INTRA_HOST) return;
         const uint16_t* pr = P.intra_pred + m.pred_off;
         for (int e = tid; e < 256; e += 64) {
             int yy = e >> 4, xx = e & 15;
@@ -409,7 +412,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) h2
             int yy = tid >> 3, xx = tid & 7;
             ((uint8_t*)P.cur[c])[(size_t)((my >> 1) + yy) * P.pitch[c] + (mx >> 1) + xx] = (uint8_t)pc[tid];
         }
-        if (tid < 16) P.partmap[(size_t)((my >> 2) + (tid >> 2)) * P.w4 + (mx >> 2) + (tid & 3)] = 0xFFFFFFFFu;
         return;
     }
     i -= P.n_intra;
@@ -771,6 +773,38 @@ __device__ __forceinline__ int chroma_dc(const int32_t* g, int idx, int scale00,
     return (int)(((uint32_t)f * (uint32_t)scale00) << (qp / 6)) >> 5;
 }
 
+// The coefficients of block bk as its transform takes them, and its DC:
+// scaled values as listed, or levels scaled with the macroblock's QP and
+// intra / inter lists (MbDequant::for_mb, src/h264/mb.rs:1384-1414), the
+// DC entry of the block's group through its DC transform.
+template <int N>
+__device__ __forceinline__ void block_input(const H264PicDev& P, const rv_h264_blk& bk, int (&c)[N * N], int& dc) {
+    const int32_t* co = P.coefs + bk.coef_off;
+    load_coefs<N * N>(co, c);
+    dc = bk.dc;
+    if (!(bk.flags & RV_BLK_LEVELS)) return;
+    const int mb = bk.comp ? (bk.y >> 3) * P.mbw + (bk.x >> 3) : (bk.y >> 4) * P.mbw + (bk.x >> 4);
+    const rv_h264_mb m = P.mbs[mb];
+    const int qp = bk.comp == 0 ? m.qp : bk.comp == 1 ? m.qpc[0] : m.qpc[1];
+    const int list = (m.intra ? 0 : 3) + bk.comp;
+    const rv_h264_scaling* sl = P.scaling;
+    if constexpr (N == 8) {
+        dequant8(c, sl ? sl->list8[list == 0 ? 0 : 1] : nullptr, qp);
+    } else {
+        dequant4(c, sl ? sl->list4[list] : nullptr, qp);
+        if (bk.has_dc) {
+            // The DC of this block's position in its group, scaled with
+            // the group's list: Intra_16x16 luma uses list 0 (recon.rs:
+            // 701-706), chroma the macroblock's (recon.rs:1201-1211).
+            const int32_t* g = P.coefs + bk.dc;
+            const int dl = bk.comp ? list : 0;
+            const int scale00 = (sl ? (int)sl->list4[dl][0] : 16) * norm_of(kNorm4[0], qp % 6);
+            dc = bk.comp ? chroma_dc(g, ((bk.y >> 2) & 1) * 2 + ((bk.x >> 2) & 1), scale00, qp)
+                         : luma_dc(g, ((bk.y >> 2) & 3) * 4 + ((bk.x >> 2) & 3), scale00, qp);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     const int32_t* pre = b.prefix + 1 * (b.n + 1);
     const int it = blockIdx.x;
@@ -789,41 +823,406 @@ __global__ void __launch_bounds__(64) h264_stage1(BatchDev b) {
     }
     const rv_h264_blk bk = P.blks[k];
     uint8_t* dst = (uint8_t*)P.cur[bk.comp] + (size_t)bk.y * P.pitch[bk.comp] + bk.x;
-    const int32_t* co = P.coefs + bk.coef_off;
-    const bool levels = bk.flags & RV_BLK_LEVELS;
-    // Levels: the macroblock's QP and intra / inter lists (MbDequant::for_mb,
-    // src/h264/mb.rs:1384-1414).
-    int qp = 0, list = 0;
-    const rv_h264_scaling* sl = P.scaling;
-    if (levels) {
-        const int mb = bk.comp ? (bk.y >> 3) * P.mbw + (bk.x >> 3) : (bk.y >> 4) * P.mbw + (bk.x >> 4);
-        const rv_h264_mb m = P.mbs[mb];
-        qp = bk.comp == 0 ? m.qp : bk.comp == 1 ? m.qpc[0] : m.qpc[1];
-        list = (m.intra ? 0 : 3) + bk.comp;
-    }
+    int dc;
     if (bk.size == 8) {
         int c[64];
-        load_coefs<64>(co, c);
-        if (levels) dequant8(c, sl ? sl->list8[list == 0 ? 0 : 1] : nullptr, qp);
+        block_input<8>(P, bk, c, dc);
         residual8_rows(dst, P.pitch[bk.comp], c);
     } else {
         int c[16];
-        load_coefs<16>(co, c);
-        int dc = bk.dc;
-        if (levels) {
-            dequant4(c, sl ? sl->list4[list] : nullptr, qp);
-            if (bk.has_dc) {
-                // The DC of this block's position in its group, scaled with
-                // the group's list: Intra_16x16 luma uses list 0 (recon.rs:
-                // 701-706), chroma the macroblock's (recon.rs:1201-1211).
-                const int32_t* g = P.coefs + bk.dc;
-                const int dl = bk.comp ? list : 0;
-                const int scale00 = (sl ? (int)sl->list4[dl][0] : 16) * norm_of(kNorm4[0], qp % 6);
-                dc = bk.comp ? chroma_dc(g, ((bk.y >> 2) & 1) * 2 + ((bk.x >> 2) & 1), scale00, qp)
-                             : luma_dc(g, ((bk.y >> 2) & 3) * 4 + ((bk.x >> 2) & 3), scale00, qp);
-            }
-        }
+        block_input<4>(P, bk, c, dc);
         residual4_rows(dst, P.pitch[bk.comp], c, bk.has_dc, dc);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Intra prediction of RV_INTRA_DEVICE macroblocks (clause 8.3; the
+// predictors of src/h264/intra.rs, the decoding order and availability of
+// src/h264/recon.rs:611-828, 936-1128), after stage 1, one launch per
+// dependency level, one wave per macroblock. The wave first computes the
+// residual of every block of the macroblock into LDS, then predicts luma
+// block by block in decoding order in an LDS tile that also holds the
+// neighbouring samples (each block's residual added before the next block
+// reads it), then chroma; the macroblock is written out at the end.
+// ---------------------------------------------------------------------
+
+// The values a residual adds to the prediction (then clipped): the
+// transform output r as add_rows_scalar rounds it, (r + 32) >> 6 in
+// wrapping i16 (src/dsp/h264.rs:803-816), or the DC-only value
+// (:984-996) on every sample.
+__device__ __forceinline__ void residual4_vals(const int (&c)[16], int has_dc, int dc, int16_t (&r)[16]) {
+    int16_t d[16];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        d[i] = (int16_t)clampi(c[i], -32768, 32767);
+        if (i) ac |= c[i];
+    }
+    if (has_dc) d[0] = (int16_t)dc;
+    if (ac == 0) {
+        const int16_t v = sh(wa(d[0], 32), 6);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) r[i] = v;
+        return;
+    }
+    int16_t t[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int16_t d0 = d[i * 4], d1 = d[i * 4 + 1], d2 = d[i * 4 + 2], d3 = d[i * 4 + 3];
+        int16_t e0 = wa(d0, d2), e1 = ws(d0, d2), e2 = ws(sh(d1, 1), d3), e3 = wa(d1, sh(d3, 1));
+        t[i * 4] = wa(e0, e3);
+        t[i * 4 + 1] = wa(e1, e2);
+        t[i * 4 + 2] = ws(e1, e2);
+        t[i * 4 + 3] = ws(e0, e3);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int16_t f0 = t[j], f1 = t[4 + j], f2 = t[8 + j], f3 = t[12 + j];
+        int16_t g0 = wa(f0, f2), g1 = ws(f0, f2), g2 = ws(sh(f1, 1), f3), g3 = wa(f1, sh(f3, 1));
+        r[j] = sh(wa(wa(g0, g3), 32), 6);
+        r[4 + j] = sh(wa(wa(g1, g2), 32), 6);
+        r[8 + j] = sh(wa(ws(g1, g2), 32), 6);
+        r[12 + j] = sh(wa(ws(g0, g3), 32), 6);
+    }
+}
+__device__ __forceinline__ void residual8_vals(const int (&c)[64], int16_t (&r)[64]) {
+    int16_t d[64];
+    int ac = 0;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        d[i] = (int16_t)clampi(c[i], -32768, 32767);
+        if (i) ac |= c[i];
+    }
+    if (ac == 0) {
+        const int16_t v = sh(wa(d[0], 32), 6);
+#pragma unroll
+        for (int i = 0; i < 64; ++i) r[i] = v;
+        return;
+    }
+    int16_t t[64];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) idct8_1d(d + i * 8, t + i * 8);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int16_t col[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) col[k] = t[k * 8 + j];
+        idct8_1d(col, o);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i * 8 + j] = sh(wa(o[i], 32), 6);
+    }
+}
+
+// LDS of one macroblock: residual values (luma 16x16, Cb, Cr 8x8, raster);
+// the luma tile y[1 + r][1 + c] = sample (c, r) of the macroblock for
+// r, c in -1..15, row 0 extending to c = 23 (the samples above-right);
+// the chroma tiles likewise (c in -1..7); the filtered edge of the 8x8
+// block being predicted (t[0..15], l[0..7], corner).
+struct alignas(16) H264IntraLds {
+    int16_t res[384];
+    uint8_t y[17][32];
+    uint8_t c[2][9][16];
+    int32_t edge[25];
+};
+
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+struct IAv {
+    bool top, left, tl, tr;
+};
+
+// Sample (x, y) of an N x N directional mode (pred_4x4 / pred_8x8,
+// src/h264/intra.rs:109-325, 427-620; the two sizes share one form) from
+// the reference samples t[0..2N), l[0..N), c (unavailable ones zero, the
+// top-right substituted).
+__device__ int nxn_sample(const int* t, const int* l, int c, int n, int mode, int x, int y, IAv av) {
+    auto T = [&](int i) { return i < 0 ? c : t[i]; };
+    auto Lf = [&](int i) { return i < 0 ? c : l[i]; };
+    switch (mode) {
+        case 0: return t[x];
+        case 1: return l[y];
+        case 2: {
+            int st = 0, sl = 0;
+            for (int k = 0; k < n; ++k) st += t[k], sl += l[k];
+            const int lg = n == 4 ? 2 : 3;
+            return av.top && av.left ? (st + sl + n) >> (lg + 1)
+                   : av.top          ? (st + n / 2) >> lg
+                   : av.left         ? (sl + n / 2) >> lg
+                                     : 128;
+        }
+        case 3:
+            return (x == n - 1 && y == n - 1) ? (t[2 * n - 2] + 3 * t[2 * n - 1] + 2) >> 2
+                                              : (t[x + y] + 2 * t[x + y + 1] + t[x + y + 2] + 2) >> 2;
+        case 4:
+            if (x > y) return (T(x - y - 2) + 2 * T(x - y - 1) + T(x - y) + 2) >> 2;
+            if (x < y) return (Lf(y - x - 2) + 2 * Lf(y - x - 1) + Lf(y - x) + 2) >> 2;
+            return (T(0) + 2 * c + Lf(0) + 2) >> 2;
+        case 5: {
+            const int z = 2 * x - y;
+            if (z >= 0 && !(z & 1)) return (T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 1) >> 1;
+            if (z >= 0) return (T(x - (y >> 1) - 2) + 2 * T(x - (y >> 1) - 1) + T(x - (y >> 1)) + 2) >> 2;
+            if (z == -1) return (Lf(0) + 2 * c + T(0) + 2) >> 2;
+            return (Lf(y - 2 * x - 1) + 2 * Lf(y - 2 * x - 2) + Lf(y - 2 * x - 3) + 2) >> 2;
+        }
+        case 6: {
+            const int z = 2 * y - x;
+            if (z >= 0 && !(z & 1)) return (Lf(y - (x >> 1) - 1) + Lf(y - (x >> 1)) + 1) >> 1;
+            if (z >= 0) return (Lf(y - (x >> 1) - 2) + 2 * Lf(y - (x >> 1) - 1) + Lf(y - (x >> 1)) + 2) >> 2;
+            if (z == -1) return (Lf(0) + 2 * c + T(0) + 2) >> 2;
+            return (T(x - 2 * y - 1) + 2 * T(x - 2 * y - 2) + T(x - 2 * y - 3) + 2) >> 2;
+        }
+        case 7:
+            return (y & 1) ? (t[x + (y >> 1)] + 2 * t[x + (y >> 1) + 1] + t[x + (y >> 1) + 2] + 2) >> 2
+                           : (t[x + (y >> 1)] + t[x + (y >> 1) + 1] + 1) >> 1;
+        default: {
+            const int z = x + 2 * y;
+            if (z > 2 * n - 3) return l[n - 1];
+            if (z == 2 * n - 3) return (l[n - 2] + 3 * l[n - 1] + 2) >> 2;
+            if (!(z & 1)) return (l[y + (x >> 1)] + l[y + (x >> 1) + 1] + 1) >> 1;
+            return (l[y + (x >> 1)] + 2 * l[y + (x >> 1) + 1] + l[y + (x >> 1) + 2] + 2) >> 2;
+        }
+    }
+}
+
+// Availability of the block at 4x4 raster (bx, by), size 4 or 8, from the
+// macroblock's neighbour bits (intra_avail_4x4 / _8x8, src/h264/recon.rs:
+// 936-1011): top-right inside the macroblock only for blocks decoded
+// after the block above-right.
+__device__ __forceinline__ IAv block_av(uint32_t nb, int bx, int by, int size) {
+    const bool A = nb & 1, B = nb & 2, Cc = nb & 4, D = nb & 8;
+    IAv av;
+    av.left = bx > 0 || A;
+    av.top = by > 0 || B;
+    av.tl = (bx > 0 && by > 0) || (bx == 0 ? (by == 0 ? D : A) : B);
+    if (by == 0) av.tr = bx + size / 4 < 4 ? B : Cc;
+    else if (size == 8) av.tr = bx == 0;
+    else {
+        // 4x4: the block above-right is decoded earlier unless it lies in
+        // the 8x8 quadrant to the right (raster (1, 1), (3, 1), (1, 3),
+        // (3, 2), (3, 3): luma4x4BlkIdx 3, 7, 11, 13, 15).
+        const int o = ((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1);
+        const int bx2 = bx + 1, by2 = by - 1;
+        const int o2 = ((by2 >> 1) * 2 + (bx2 >> 1)) * 4 + (by2 & 1) * 2 + (bx2 & 1);
+        av.tr = bx < 3 && o2 < o;
+    }
+    return av;
+}
+
+__global__ void __launch_bounds__(64) h264_intra_level(BatchDev b, int level) {
+    __shared__ H264IntraLds L;
+    const int32_t* pre = b.ipre + (size_t)(level + 1) * (b.n + 1);
+    const int it = blockIdx.x;
+    const int p = find_pic(pre, b.n, it);
+    const H264PicDev& P = *b.h264[p];
+    const H264IntraMb im = P.imbs[P.ilevel[level] + (it - pre[p])];
+    const rv_h264_mb m = P.mbs[im.addr];
+    const int tid = threadIdx.x;
+    const int mx = (im.addr % P.mbw) * 16, my = (im.addr / P.mbw) * 16;
+    const uint32_t nb = im.avail;
+    const bool A = nb & 1, B = nb & 2, Cc = nb & 4, D = nb & 8;
+    uint8_t* Yp = (uint8_t*)P.cur[0];
+    const int py = P.pitch[0], pc = P.pitch[1];
+    // 1. Residual values cleared; the neighbouring samples read.
+    for (int i = tid; i < 192; i += 64) ((uint32_t*)L.res)[i] = 0u;
+    if (tid < 25) {
+        const int x = tid - 1;
+        const bool ok = x < 0 ? D : x < 16 ? B : Cc;
+        L.y[0][tid] = ok ? Yp[(size_t)(my - 1) * py + mx + x] : 0;
+    } else if (tid >= 32 && tid < 48) {
+        const int y = tid - 32;
+        L.y[1 + y][0] = A ? Yp[(size_t)(my + y) * py + mx - 1] : 0;
+    }
+    {
+        const int comp = tid >> 5, k = tid & 31;
+        const uint8_t* Cp = (const uint8_t*)P.cur[1 + comp];
+        const int cx = mx >> 1, cy = my >> 1;
+        if (k < 9) {
+            const int x = k - 1;
+            const bool ok = x < 0 ? D : B;
+            L.c[comp][0][k] = ok ? Cp[(size_t)(cy - 1) * pc + cx + x] : 0;
+        } else if (k < 17) {
+            const int y = k - 9;
+            L.c[comp][1 + y][0] = A ? Cp[(size_t)(cy + y) * pc + cx - 1] : 0;
+        }
+    }
+    lds_sync();
+    // 2. The residual of every listed block of the macroblock.
+    if (tid < im.nblk) {
+        const rv_h264_blk bk = P.iblks[im.blk0 + tid];
+        int16_t* out;
+        int ostride;
+        if (bk.comp == 0) {
+            out = L.res + (bk.y - my) * 16 + (bk.x - mx);
+            ostride = 16;
+        } else {
+            out = L.res + 256 + (bk.comp - 1) * 64 + (bk.y - (my >> 1)) * 8 + (bk.x - (mx >> 1));
+            ostride = 8;
+        }
+        int dc;
+        if (bk.size == 8) {
+            int c[64];
+            int16_t r[64];
+            block_input<8>(P, bk, c, dc);
+            residual8_vals(c, r);
+            for (int y = 0; y < 8; ++y)
+                for (int x = 0; x < 8; ++x) out[y * ostride + x] = r[y * 8 + x];
+        } else {
+            int c[16];
+            int16_t r[16];
+            block_input<4>(P, bk, c, dc);
+            residual4_vals(c, bk.has_dc, dc, r);
+            for (int y = 0; y < 4; ++y)
+                for (int x = 0; x < 4; ++x) out[y * ostride + x] = r[y * 4 + x];
+        }
+    }
+    lds_sync();
+    // 3. Luma.
+    const int kind = m.intra_kind;
+    if (kind == RV_H264_I16X16) {
+        // predict_16x16 (src/h264/intra.rs:622-646, 792-923); av top B,
+        // left A, top-left D.
+        const int mode = m.modes[0];
+        const int y = tid >> 2, x0 = (tid & 3) * 4;
+        int v[4];
+        if (mode == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = L.y[0][1 + x0 + k];
+        } else if (mode == 1) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = L.y[1 + y][0];
+        } else if (mode == 2) {
+            int st = 0, sl = 0;
+            for (int k = 0; k < 16; ++k) st += L.y[0][1 + k], sl += L.y[1 + k][0];
+            const int dcv = A && B ? (st + sl + 16) >> 5 : B ? (st + 8) >> 4 : A ? (sl + 8) >> 4 : 128;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = dcv;
+        } else {
+            int hh = 0, vv = 0;
+            for (int i = 0; i < 8; ++i) {
+                hh += (i + 1) * ((int)L.y[0][1 + 8 + i] - (int)L.y[0][1 + 6 - i]);
+                vv += (i + 1) * ((int)L.y[1 + 8 + i][0] - (int)L.y[1 + 6 - i][0]);
+            }
+            const int bb = (5 * hh + 32) >> 6, cc = (5 * vv + 32) >> 6, a = 16 * (L.y[16][0] + L.y[0][16]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = clampi((a + bb * (x0 + k - 7) + cc * (y - 7) + 16) >> 5, 0, 255);
+        }
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w |= (uint32_t)clampi(v[k] + L.res[y * 16 + x0 + k], 0, 255) << (8 * k);
+        *(uint32_t*)(Yp + (size_t)(my + y) * py + mx + x0) = w;
+    } else {
+        const int n = kind == RV_H264_I8X8 ? 8 : 4, nb_blocks = kind == RV_H264_I8X8 ? 4 : 16;
+        for (int k = 0; k < nb_blocks; ++k) {
+            int bx, by;
+            if (n == 8) {
+                bx = (k & 1) * 2;
+                by = (k >> 1) * 2;
+            } else {
+                bx = ((k >> 2) & 1) * 2 + (k & 1);
+                by = (k >> 3) * 2 + ((k >> 1) & 1);
+            }
+            const IAv av = block_av(nb, bx, by, n);
+            const int r0 = by * 4, c0 = bx * 4;  // tile row / column of the row above / column left
+            // Reference samples: unfiltered for 4x4, filtered (8.3.2.2.1)
+            // into LDS for 8x8 by lanes 0..24.
+            int t[16], l[8], c;
+            if (n == 8) {
+                if (tid < 25) {
+                    auto top = [&](int i) {  // p[i, -1] with the substitution
+                        return (int)L.y[r0][1 + c0 + (i < 8 || av.tr ? i : 7)];
+                    };
+                    const int cr = L.y[r0][c0];
+                    int o = 0;
+                    if (tid < 16) {
+                        const int i = tid;
+                        if (av.top) {
+                            if (i == 0) o = av.tl ? (cr + 2 * top(0) + top(1) + 2) >> 2 : (3 * top(0) + top(1) + 2) >> 2;
+                            else if (i == 15) o = (top(14) + 3 * top(15) + 2) >> 2;
+                            else o = (top(i - 1) + 2 * top(i) + top(i + 1) + 2) >> 2;
+                        }
+                    } else if (tid < 24) {
+                        const int j = tid - 16;
+                        auto lf = [&](int q) { return (int)L.y[r0 + 1 + q][c0]; };
+                        if (av.left) {
+                            if (j == 0) o = av.tl ? (cr + 2 * lf(0) + lf(1) + 2) >> 2 : (3 * lf(0) + lf(1) + 2) >> 2;
+                            else if (j == 7) o = (lf(6) + 3 * lf(7) + 2) >> 2;
+                            else o = (lf(j - 1) + 2 * lf(j) + lf(j + 1) + 2) >> 2;
+                        }
+                    } else if (av.tl) {
+                        const int t0 = top(0), l0 = L.y[r0 + 1][c0];
+                        o = av.top && av.left ? (t0 + 2 * cr + l0 + 2) >> 2
+                            : av.top          ? (3 * cr + t0 + 2) >> 2
+                            : av.left         ? (3 * cr + l0 + 2) >> 2
+                                              : cr;
+                    }
+                    L.edge[tid] = o;
+                }
+                lds_sync();
+#pragma unroll
+                for (int i = 0; i < 16; ++i) t[i] = L.edge[i];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) l[j] = L.edge[16 + j];
+                c = L.edge[24];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) t[i] = av.top ? (int)L.y[r0][1 + c0 + (i < 4 || av.tr ? i : 3)] : 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) l[j] = av.left ? (int)L.y[r0 + 1 + j][c0] : 0;
+                c = av.tl ? (int)L.y[r0][c0] : 0;
+            }
+            const int mode = m.modes[by * 4 + bx];
+            if (tid < n * n) {
+                const int x = tid % n, y = tid / n;
+                const int v = nxn_sample(t, l, c, n, mode, x, y, av);
+                L.y[r0 + 1 + y][c0 + 1 + x] = (uint8_t)clampi(v + L.res[(r0 + y) * 16 + c0 + x], 0, 255);
+            }
+            lds_sync();
+        }
+        // The finished macroblock.
+        const int y = tid >> 2, x0 = (tid & 3) * 4;
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w |= (uint32_t)L.y[1 + y][1 + x0 + k] << (8 * k);
+        *(uint32_t*)(Yp + (size_t)(my + y) * py + mx + x0) = w;
+    }
+    // 4. Chroma (predict_chroma, src/h264/intra.rs:648-790, 4:2:0): lanes
+    // 0..31 Cb, 32..63 Cr, two samples each.
+    {
+        const int comp = tid >> 5, k = tid & 31, y = k >> 2, x0 = (k & 3) * 2;
+        const uint8_t(&T)[9][16] = L.c[comp];
+        const int mode = m.chroma_mode;
+        int v[2];
+        if (mode == 0) {
+            const int bx = x0 >> 2, by = y >> 2;
+            int st = 0, sl = 0;
+            for (int q = 0; q < 4; ++q) st += T[0][1 + bx * 4 + q], sl += T[1 + by * 4 + q][0];
+            int dcv;
+            if (bx == by) dcv = A && B ? (st + sl + 4) >> 3 : B ? (st + 2) >> 2 : A ? (sl + 2) >> 2 : 128;
+            else if (bx) dcv = B ? (st + 2) >> 2 : A ? (sl + 2) >> 2 : 128;
+            else dcv = A ? (sl + 2) >> 2 : B ? (st + 2) >> 2 : 128;
+            v[0] = v[1] = dcv;
+        } else if (mode == 1) {
+            v[0] = v[1] = T[1 + y][0];
+        } else if (mode == 2) {
+            v[0] = T[0][1 + x0];
+            v[1] = T[0][2 + x0];
+        } else {
+            int hh = 0, vv = 0;
+            for (int i = 0; i < 4; ++i) {
+                hh += (i + 1) * ((int)T[0][1 + 4 + i] - (int)T[0][1 + 2 - i]);
+                vv += (i + 1) * ((int)T[1 + 4 + i][0] - (int)T[1 + 2 - i][0]);
+            }
+            const int bb = (34 * hh + 32) >> 6, cc = (34 * vv + 32) >> 6, a = 16 * (T[8][0] + T[0][8]);
+            v[0] = clampi((a + bb * (x0 - 3) + cc * (y - 3) + 16) >> 5, 0, 255);
+            v[1] = clampi((a + bb * (x0 + 1 - 3) + cc * (y - 3) + 16) >> 5, 0, 255);
+        }
+        const int16_t* rs = L.res + 256 + comp * 64 + y * 8 + x0;
+        const uint32_t w = (uint32_t)clampi(v[0] + rs[0], 0, 255) | ((uint32_t)clampi(v[1] + rs[1], 0, 255) << 8);
+        *(uint16_t*)((uint8_t*)P.cur[1 + comp] + (size_t)((my >> 1) + y) * pc + (mx >> 1) + x0) = (uint16_t)w;
     }
 }
 
@@ -1530,10 +1929,16 @@ size_t h264_deblock_xch_words(int mbw, int mbh) {
 }
 
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
+    if (stage == 1) {
+        // The transforms, then the device-intra macroblocks level by level.
+        if (total > 0) hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b);
+        for (int l = 0; l < b.ilevels; ++l)
+            if (b.itotal[l] > 0) hipLaunchKernelGGL(h264_intra_level, dim3(b.itotal[l]), dim3(64), 0, st, b, l);
+        return hipGetLastError();
+    }
     if (total <= 0) return hipSuccess;
     switch (stage) {
         case 0: hipLaunchKernelGGL(h264_stage0, dim3(total), dim3(64), 0, st, b); break;
-        case 1: hipLaunchKernelGGL(h264_stage1, dim3(total), dim3(64), 0, st, b); break;
         case 2: hipLaunchKernelGGL(h264_stage2, dim3(total), dim3(64), 0, st, b); break;
         case 3: hipLaunchKernelGGL(h264_stage3, dim3(total), dim3(kDbkThreads), 0, st, b); break;
         default: return hipSuccess;

@@ -299,8 +299,10 @@ struct rv_job {
     void* desc_dev = nullptr;  // device copy of the descriptor
     // Single-picture tables for the plain path.
     rv::BatchDev single{};
-    std::vector<int32_t> itotal;  // device-intra TUs per level (host copy)
-    int32_t n_itus = 0;           // device-intra TUs
+    // Device intra: HEVC TUs or H.264 macroblocks per level (host copy),
+    // and their total.
+    std::vector<int32_t> itotal;
+    int32_t n_itus = 0;
     int64_t bytes[4] = {0, 0, 0, 0};
 };
 
@@ -312,7 +314,8 @@ struct rv_batch {
     bool var_set = false;            // chosen by rv_batch_set_variant / rv_batch_tune
     rv::BatchDev bh{}, b4{};
     std::vector<int32_t> pre_h, pre_4;  // host copies (totals)
-    std::vector<int32_t> itotal;        // device-intra TUs per level
+    std::vector<int32_t> itotal;        // device-intra TUs per level (HEVC)
+    std::vector<int32_t> itotal4;       // device-intra macroblocks per level (H.264)
     DevBuf tables;
     DevBuf itask;                       // device-intra task table
     hipGraphExec_t exec = nullptr;
@@ -552,6 +555,18 @@ void implicit_weights(int cur, int poc0, int poc1, bool lt0, bool lt1, int* w0, 
     *w1 = v1;
 }
 
+// Neighbouring macroblocks of H.264 macroblock addr that may supply intra
+// prediction samples in a one-slice frame: bit 0 left, 1 above, 2
+// above-right, 3 above-left; each present and, under constrained intra
+// prediction, intra (intra_ok, src/h264/recon.rs:301-307).
+uint32_t h264_neighbours(const rv_h264_pic* p, int addr) {
+    const int mbw = p->width / 16, mbx = addr % mbw, mby = addr / mbw;
+    auto ok = [&](int n) { return !p->constrained_intra_pred || p->mbs[n].intra != 0; };
+    return (mbx > 0 && ok(addr - 1) ? 1u : 0u) | (mby > 0 && ok(addr - mbw) ? 2u : 0u) |
+           (mby > 0 && mbx < mbw - 1 && ok(addr - mbw + 1) ? 4u : 0u) |
+           (mbx > 0 && mby > 0 && ok(addr - mbw - 1) ? 8u : 0u);
+}
+
 int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
     if (p->width != s->w || p->height != s->h || p->bit_depth != 8)
         return fail(RV_E_RANGE, "picture shape differs from the session");
@@ -576,9 +591,39 @@ int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
         const rv_h264_mb& m = p->mbs[i];
         if (m.qp < 0 || m.qp > 51 || m.qpc[0] < 0 || m.qpc[0] > 51 || m.qpc[1] < 0 || m.qpc[1] > 51)
             return fail(RV_E_RANGE, "QP outside 0..51");
-        if (m.intra) {
+        if (m.intra == RV_INTRA_HOST) {
             if ((uint64_t)m.pred_off + 384 > p->n_intra_pred) return fail(RV_E_RANGE, "intra prediction outside intra_pred");
             area += 256;
+        } else if (m.intra == RV_INTRA_DEVICE) {
+            // The modes must have the neighbours they read, as the reference's
+            // predictors require (src/h264/intra.rs:78-88, 109-325, 622-790).
+            const uint32_t nb = h264_neighbours(p, i);
+            const bool A = nb & 1, B = nb & 2, D = nb & 8;
+            if (m.intra_kind > RV_H264_I16X16) return fail(RV_E_RANGE, "intra_kind");
+            if ((m.intra_kind == RV_H264_I8X8) != (m.transform_8x8 != 0))
+                return fail(RV_E_RANGE, "transform_8x8 differs from the intra kind");
+            if (m.intra_kind == RV_H264_I16X16) {
+                const int md = m.modes[0];
+                if (md > 3 || (md == 0 && !B) || (md == 1 && !A) || (md == 3 && !(A && B && D)))
+                    return fail(RV_E_INVALID, "Intra_16x16 mode without its neighbours");
+            } else {
+                const int step = m.intra_kind == RV_H264_I8X8 ? 2 : 1;
+                for (int by = 0; by < 4; by += step)
+                    for (int bx = 0; bx < 4; bx += step) {
+                        const bool left = bx > 0 || A, top = by > 0 || B;
+                        const bool tl = (bx > 0 && by > 0) || (bx == 0 ? (by == 0 ? D : A) : B);
+                        const int md = m.modes[by * 4 + bx];
+                        const bool ok = md == 2 || ((md == 0 || md == 3 || md == 7) && top) ||
+                                        ((md == 1 || md == 8) && left) || (md >= 4 && md <= 6 && top && left && tl);
+                        if (md > 8 || !ok) return fail(RV_E_INVALID, "Intra_NxN mode without its neighbours");
+                    }
+            }
+            const int cm = m.chroma_mode;
+            if (cm > 3 || (cm == 1 && !A) || (cm == 2 && !B) || (cm == 3 && !(A && B && D)))
+                return fail(RV_E_INVALID, "chroma intra mode without its neighbours");
+            area += 256;
+        } else if (m.intra) {
+            return fail(RV_E_RANGE, "intra outside 0..2");
         }
     }
     for (int i = 0; i < p->n_parts; ++i) {
@@ -614,6 +659,30 @@ int validate_h264(const rv_session* s, const rv_h264_pic* p, int out_slot) {
             if (b.dc < 0 || (uint64_t)b.dc + group > p->n_coefs) return fail(RV_E_RANGE, "DC levels outside coefs");
         }
     }
+    // The blocks of device-intra macroblocks: luma of the macroblock's
+    // transform size, a luma DC only under Intra_16x16, each position
+    // listed once (the device adds them in LDS by position).
+    std::vector<uint32_t> seen;
+    for (int i = 0; i < p->n_blks; ++i) {
+        const rv_h264_blk& b = p->blks[i];
+        const int mb = b.comp ? (b.y >> 3) * mbw + (b.x >> 3) : (b.y >> 4) * mbw + (b.x >> 4);
+        const rv_h264_mb& m = p->mbs[mb];
+        if (m.intra != RV_INTRA_DEVICE) continue;
+        if (seen.empty()) seen.assign((size_t)p->n_mbs, 0u);
+        uint32_t bits;
+        if (b.comp == 0) {
+            if (b.size != (m.intra_kind == RV_H264_I8X8 ? 8 : 4)) return fail(RV_E_RANGE, "block size differs from the intra kind");
+            if (b.has_dc && m.intra_kind != RV_H264_I16X16) return fail(RV_E_RANGE, "luma DC outside Intra_16x16");
+            const int bx = (b.x & 15) >> 2, by = (b.y & 15) >> 2, w4 = b.size / 4;
+            bits = 0;
+            for (int y = 0; y < w4; ++y)
+                for (int x = 0; x < w4; ++x) bits |= 1u << ((by + y) * 4 + bx + x);
+        } else {
+            bits = 1u << (16 + (b.comp - 1) * 4 + ((b.y & 7) >> 2) * 2 + ((b.x & 7) >> 2));
+        }
+        if (seen[(size_t)mb] & bits) return fail(RV_E_RANGE, "block listed twice in a device-intra macroblock");
+        seen[(size_t)mb] |= bits;
+    }
     return RV_OK;
 }
 
@@ -625,12 +694,18 @@ void h264_bytes(const rv_h264_pic* p, int64_t out[4]) {
             if ((t.dir >> l) & 1) mc += (int64_t)(t.w + 5) * (t.h + 5) + 2 * (int64_t)(t.w / 2 + 1) * (t.h / 2 + 1);
     }
     int64_t planes = (int64_t)p->width * p->height * 3 / 2;
-    for (int i = 0; i < p->n_mbs; ++i)
-        if (p->mbs[i].intra) mc += 384 * 2;
+    // Intra macroblocks: a host prediction is read and written in stage 0;
+    // a device one reads its neighbours (25 + 16 luma, 2 x 17 chroma) and
+    // writes its 384 samples after the transform stage.
+    int64_t intra_dev = 0;
+    for (int i = 0; i < p->n_mbs; ++i) {
+        if (p->mbs[i].intra == RV_INTRA_HOST) mc += 384 * 2;
+        else if (p->mbs[i].intra == RV_INTRA_DEVICE) intra_dev += 384 + 75;
+    }
     out[0] = mc + planes;
     int64_t coef = 0;
     for (int i = 0; i < p->n_blks; ++i) coef += (int64_t)p->blks[i].size * p->blks[i].size;
-    out[1] = 4 * coef + 2 * coef;
+    out[1] = 4 * coef + 2 * coef + intra_dev;
     out[2] = p->deblock ? 2 * planes : 0;
     out[3] = 0;
 }
@@ -1393,10 +1468,45 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         int rc = validate_h264(s, p, out_slot);
         if (rc) return rc;
         const int W = p->width, H = p->height, mbw = W / 16, mbh = H / 16;
+        // Blocks: those of device-intra macroblocks grouped by macroblock
+        // (iblks); the others 4x4 first, then 8x8.
         std::vector<rv_h264_blk> b4, b8;
-        for (int i = 0; i < p->n_blks; ++i) (p->blks[i].size == 8 ? b8 : b4).push_back(p->blks[i]);
+        std::vector<std::vector<int>> own;
+        for (int i = 0; i < p->n_blks; ++i) {
+            const rv_h264_blk& bk = p->blks[i];
+            const int mb = bk.comp ? (bk.y >> 3) * mbw + (bk.x >> 3) : (bk.y >> 4) * mbw + (bk.x >> 4);
+            if (p->mbs[mb].intra == RV_INTRA_DEVICE) {
+                if (own.empty()) own.resize((size_t)p->n_mbs);
+                own[(size_t)mb].push_back(i);
+                continue;
+            }
+            (bk.size == 8 ? b8 : b4).push_back(bk);
+        }
         std::vector<rv_h264_blk> blks(b4);
         blks.insert(blks.end(), b8.begin(), b8.end());
+        // Device-intra macroblocks by dependency level: one more than the
+        // highest level among the device-intra neighbours it may read (left,
+        // above, above-right, above-left), raster order within a level.
+        std::vector<int32_t> lvl((size_t)p->n_mbs, -1), lvl_cnt;
+        int n_imbs = 0, n_iblks = 0;
+        for (int a = 0; a < p->n_mbs; ++a) {
+            if (p->mbs[a].intra != RV_INTRA_DEVICE) continue;
+            const int mbx = a % mbw, mby = a / mbw;
+            int l = 0;
+            auto dep = [&](bool in, int n) {
+                if (in && lvl[(size_t)n] >= 0) l = std::max(l, lvl[(size_t)n] + 1);
+            };
+            dep(mbx > 0, a - 1);
+            dep(mby > 0, a - mbw);
+            dep(mby > 0 && mbx < mbw - 1, a - mbw + 1);
+            dep(mbx > 0 && mby > 0, a - mbw - 1);
+            lvl[(size_t)a] = l;
+            if ((int)lvl_cnt.size() <= l) lvl_cnt.resize((size_t)l + 1, 0);
+            ++lvl_cnt[(size_t)l];
+            ++n_imbs;
+            if (!own.empty()) n_iblks += (int)own[(size_t)a].size();
+        }
+        const int n_ilevels = (int)lvl_cnt.size();
         rv_job* j = new rv_job();
         j->s = s;
         j->codec = RV_CODEC_H264;
@@ -1406,7 +1516,9 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         for (int a = 0; a < p->n_mbs; ++a) n_intra += p->mbs[a].intra ? 1 : 0;
         size_t sz = carve_size<rv::H264PicDev>(1) + carve_size<rv_h264_mb>(p->n_mbs) +
                     carve_size<rv_h264_part>(p->n_parts) + carve_size<uint32_t>(p->n_parts + n_intra) +
-                    carve_size<rv_h264_blk>(blks.size()) +
+                    carve_size<rv_h264_blk>(blks.size()) + carve_size<rv::H264IntraMb>(n_imbs) +
+                    carve_size<rv_h264_blk>(n_iblks) + carve_size<int32_t>(n_ilevels + 1) +
+                    carve_size<int32_t>(2 * (size_t)(n_ilevels + 1)) +
                     carve_size<int32_t>(p->n_coefs) + carve_size<uint16_t>(p->n_intra_pred) +
                     carve_size<rv_h264_scaling>(p->scaling ? 1 : 0) +
                     carve_size<const rv::H264PicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
@@ -1428,6 +1540,33 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         rv_h264_part* h_parts = carve<rv_h264_part>(hp, p->n_parts);
         uint32_t* h_order = carve<uint32_t>(hp, p->n_parts + n_intra);
         rv_h264_blk* h_blks = carve<rv_h264_blk>(hp, blks.size());
+        rv::H264IntraMb* h_imbs = carve<rv::H264IntraMb>(hp, n_imbs);
+        rv_h264_blk* h_iblks = carve<rv_h264_blk>(hp, n_iblks);
+        int32_t* h_ilevel = carve<int32_t>(hp, n_ilevels + 1);
+        // {0, all macroblocks}, then per level {0, count}: the prefix rows
+        // of a one-picture batch.
+        int32_t* h_ipre = carve<int32_t>(hp, 2 * (size_t)(n_ilevels + 1));
+        {
+            h_ilevel[0] = 0;
+            for (int l = 0; l < n_ilevels; ++l) h_ilevel[l + 1] = h_ilevel[l] + lvl_cnt[(size_t)l];
+            std::vector<int32_t> at(h_ilevel, h_ilevel + n_ilevels);
+            int32_t nb = 0;
+            for (int a = 0; a < p->n_mbs; ++a) {
+                if (lvl[(size_t)a] < 0) continue;
+                rv::H264IntraMb& r = h_imbs[at[(size_t)lvl[(size_t)a]]++];
+                r.addr = a;
+                r.avail = h264_neighbours(p, a);
+                r.blk0 = nb;
+                r.nblk = own.empty() ? 0 : (int32_t)own[(size_t)a].size();
+                for (int i = 0; i < r.nblk; ++i) h_iblks[nb++] = p->blks[own[(size_t)a][(size_t)i]];
+            }
+            h_ipre[0] = 0;
+            h_ipre[1] = n_imbs;
+            for (int l = 0; l < n_ilevels; ++l) {
+                h_ipre[2 * (l + 1)] = 0;
+                h_ipre[2 * (l + 1) + 1] = lvl_cnt[(size_t)l];
+            }
+        }
         int32_t* h_coefs = carve<int32_t>(hp, p->n_coefs);
         uint16_t* h_ip = carve<uint16_t>(hp, p->n_intra_pred);
         rv_h264_scaling* h_scaling = carve<rv_h264_scaling>(hp, p->scaling ? 1 : 0);
@@ -1506,6 +1645,10 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         d.n_big = n_big;
         d.n_intra = n_intra;
         d.blks = (const rv_h264_blk*)dev((char*)h_blks);
+        d.n_ilevels = n_ilevels;
+        d.ilevel = (const int32_t*)dev((char*)h_ilevel);
+        d.imbs = (const rv::H264IntraMb*)dev((char*)h_imbs);
+        d.iblks = (const rv_h264_blk*)dev((char*)h_iblks);
         d.coefs = (const int32_t*)dev((char*)h_coefs);
         d.intra_pred = (const uint16_t*)dev((char*)h_ip);
         d.scaling = p->scaling ? (const rv_h264_scaling*)dev((char*)h_scaling) : nullptr;
@@ -1537,6 +1680,11 @@ int rv_h264_prepare(rv_session* s, const rv_h264_pic* p, int out_slot, rv_job** 
         j->single.hevc = nullptr;
         j->single.h264 = (const rv::H264PicDev* const*)dev((char*)h_ptr);
         j->single.prefix = (const int32_t*)dev((char*)h_pre);
+        j->itotal = lvl_cnt;
+        j->n_itus = n_imbs;
+        j->single.ilevels = n_ilevels;
+        j->single.ipre = (const int32_t*)dev((char*)h_ipre);
+        j->single.itotal = j->itotal.data();
         // No clearing of the scratch (see rv_hevc_prepare): stage 0 writes
         // the partition map, stage 2 the edge words and, every launch, the
         // exchange; partition indices are bounds-checked where read.
@@ -1614,6 +1762,22 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
             b->itotal[l] = row[nh];
             ibase[(size_t)l + 1] = ibase[(size_t)l] + row[nh];
         }
+        // H.264 device-intra macroblocks: the same prefix rows over the
+        // H.264 pictures (no task table: the level kernel locates its
+        // picture by the prefix).
+        int ilevels4 = 0;
+        for (int i = 0; i < n4; ++i) ilevels4 = std::max(ilevels4, b->h264[i]->h4.n_ilevels);
+        std::vector<int32_t> ipre4((size_t)(ilevels4 + 1) * (n4 + 1), 0);
+        b->itotal4.assign(ilevels4, 0);
+        for (int i = 0; i < n4; ++i) ipre4[(size_t)i + 1] = ipre4[(size_t)i] + b->h264[i]->n_itus;
+        for (int l = 0; l < ilevels4; ++l) {
+            int32_t* row = ipre4.data() + (size_t)(l + 1) * (n4 + 1);
+            for (int i = 0; i < n4; ++i) {
+                const std::vector<int32_t>& t = b->h264[i]->itotal;
+                row[i + 1] = row[i] + (l < (int)t.size() ? t[l] : 0);
+            }
+            b->itotal4[l] = row[n4];
+        }
         // Motion-compensation strips: the prefix over the HEVC pictures of
         // their waves.
         std::vector<int32_t> spre((size_t)nh + 1, 0);
@@ -1625,7 +1789,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         }
         size_t sz = carve_size<void*>(nh) + carve_size<void*>(n4) + carve_size<int32_t>(b->pre_h.size()) +
                     carve_size<int32_t>(b->pre_4.size()) + carve_size<int32_t>(ipre.size()) +
-                    carve_size<int32_t>(ibase.size()) + carve_size<int32_t>(spre.size());
+                    carve_size<int32_t>(ibase.size()) + carve_size<int32_t>(spre.size()) +
+                    carve_size<int32_t>(ipre4.size());
         std::vector<uint32_t> host(sz / 4 + 1, 0u);
         char* hbase = (char*)host.data();
         if (!pool_get(sz, &b->tables)) {
@@ -1646,6 +1811,8 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         std::memcpy(hibase, ibase.data(), ibase.size() * 4);
         int32_t* hspre = carve<int32_t>(hp, spre.size());
         std::memcpy(hspre, spre.data(), spre.size() * 4);
+        int32_t* hipre4 = carve<int32_t>(hp, ipre4.size());
+        std::memcpy(hipre4, ipre4.data(), ipre4.size() * 4);
         for (int i = 0; i < nh; ++i) ph[i] = b->hevc[i]->desc_dev;
         for (int i = 0; i < n4; ++i) p4[i] = b->h264[i]->desc_dev;
         std::memcpy(preh, b->pre_h.data(), b->pre_h.size() * 4);
@@ -1664,6 +1831,9 @@ int rv_batch_create(rv_job* const* jobs, int n, rv_batch** out) {
         b->b4.n = n4;
         b->b4.h264 = (const rv::H264PicDev* const*)dev((char*)p4);
         b->b4.prefix = (const int32_t*)dev((char*)pre4);
+        b->b4.ilevels = ilevels4;
+        b->b4.ipre = (const int32_t*)dev((char*)hipre4);
+        b->b4.itotal = b->itotal4.data();
         // Written on the library stream, ahead of the batch's launches.
         hipError_t e = hipSuccess;
         {

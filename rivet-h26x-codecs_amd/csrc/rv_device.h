@@ -143,6 +143,16 @@ constexpr int kH264DbkWords = 64;
 constexpr int kH264DbkRows = 16;
 __host__ __device__ inline int h264_deblock_bands(int mbh) { return (mbh + kH264DbkRows - 1) / kH264DbkRows; }
 
+// One RV_INTRA_DEVICE H.264 macroblock: its address, which neighbouring
+// macroblocks may supply samples (bit 0 left A, 1 above B, 2 above-right
+// C, 3 above-left D: present and, under constrained intra prediction,
+// intra) and its residual blocks iblks[blk0 .. blk0 + nblk).
+struct alignas(16) H264IntraMb {
+    int32_t addr;
+    uint32_t avail;
+    int32_t blk0, nblk;
+};
+
 // Per-picture descriptor of an H.264 job.
 struct H264PicDev {
     int32_t w, h, mbw, mbh, w4, h4;
@@ -164,8 +174,14 @@ struct H264PicDev {
     // the indices of the n_intra intra macroblocks.
     const uint32_t* part_order;
     int32_t n_big, n_intra;
-    const rv_h264_blk* blks;  // 4x4 first, then 8x8
+    const rv_h264_blk* blks;  // 4x4 first, then 8x8 (not of RV_INTRA_DEVICE macroblocks)
     const int32_t* coefs;
+    // Macroblocks predicted on the device, by dependency level: level k
+    // holds imbs[ilevel[k] .. ilevel[k + 1]); their blocks in iblks.
+    int32_t n_ilevels;
+    const int32_t* ilevel;
+    const H264IntraMb* imbs;
+    const rv_h264_blk* iblks;
     const uint16_t* intra_pred;
     const rv_h264_scaling* scaling;  // levels' scaling lists; null = flat 16
     uint32_t* partmap;   // per 4x4: partition index (0xFFFFFFFF = intra)

@@ -34,6 +34,9 @@ RV_CODEC_H264 = 1
 RV_H264_DEFAULT = 0
 RV_H264_EXPLICIT = 1
 RV_H264_IMPLICIT = 2
+RV_H264_I4X4 = 0
+RV_H264_I8X8 = 1
+RV_H264_I16X16 = 2
 RV_LAUNCH_PLAIN = 0
 RV_LAUNCH_BATCHED = 1
 RV_LAUNCH_GRAPH = 2
