@@ -48,25 +48,43 @@ def log(*a):
 TRAFFIC_FILE = ROOT / "profiles" / "hevc_c3_8x4k_stage0_traffic.json"
 
 
-def pmc_traffic(kernel: str, streams: int, variant: str = ""):
-    """(HBM bytes per launch, kernel symbol) of `kernel` on this workload,
+def pmc_traffic(stage: str, streams: int, variant: str = ""):
+    """(HBM bytes per launch of the stage, its kernels) on this workload,
     from the committed rocprofv3 --pmc summary: FETCH_SIZE and WRITE_SIZE
-    per dispatch, each scaled by what the counter reads for a copy kernel
-    of known size, for the kernel variant `variant` when the summary holds
-    its counters. (None, None) when it does not cover this kernel and
-    workload."""
+    per dispatch of each kernel the stage variant `variant` launches, each
+    scaled by what the counter reads for a copy kernel of known size, summed
+    over the stage's kernels. (None, None) when the summary does not cover
+    this stage, variant and workload."""
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None, None
-    if d.get("kernel") != kernel or streams != 8:
+    if d.get("stage") != stage or streams != 8:
         return None, None
-    sym = d.get("variant_symbols", {}).get(variant)
-    kib = d.get("all_stage0_variants_per_dispatch_kib", {}).get(sym) if sym else None
-    if kib:
-        cal = d["calibration"]
-        return int(round((kib["fetch"] * cal["fetch_factor"] + kib["write"] * cal["write_factor"]) * 1024)), sym
-    return int(d["hbm_bytes_per_launch"]), d.get("variant", "").split(":")[0]
+    v = d.get("variants", {}).get(variant)
+    if not v:
+        return None, None
+    return int(v["hbm_bytes_per_launch"]), " + ".join(v["kernels"])
+
+
+def dbk_segment_bytes(oracle, pics):
+    """Algorithmic bytes of the vertical and horizontal deblocking passes
+    of HEVC pictures, from the boundary strengths of their work lists (the
+    oracle's derivation, outside any timed region): per 4-line luma segment
+    with bS > 0, 8 samples read and 6 written per line; per 2-line chroma
+    segment of each component (bS 2, edges on the 16-sample luma grid), 4
+    read and 2 written per line."""
+    vb = hb = 0
+    for g in pics:
+        if not g.pic.contents.deblock:
+            continue
+        s = 2 if g.bit_depth > 8 else 1
+        ver, hor = oracle.hevc_bs_map(g.pic)
+        on16_x = (np.arange(ver.shape[1]) * 4) % 16 == 0
+        on16_y = (np.arange(ver.shape[0]) * 4) % 16 == 0
+        vb += s * (int(np.count_nonzero(ver)) * 4 * 14 + int(np.count_nonzero((ver == 2) & on16_x[None, :])) * 2 * 2 * 6)
+        hb += s * (int(np.count_nonzero(hor)) * 4 * 14 + int(np.count_nonzero((hor == 2) & on16_y[:, None])) * 2 * 2 * 6)
+    return vb, hb
 
 
 def cpu_model() -> str:
@@ -165,7 +183,8 @@ def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int,
             nb[key] += v
     if codec == rh.RV_CODEC_HEVC:
         names = STAGES
-        stage_bytes = [nb["mc"], nb["tx"], nb["dbk"] / 2, nb["dbk"] / 2, nb["sao"]]
+        vb, hb = dbk_segment_bytes(oracle, pics)
+        stage_bytes = [nb["mc"], nb["tx"], vb, hb, nb["sao"]]
     else:
         # H.264: stage 2 derives the strengths and edge parameters (side data
         # only); stage 3, the macroblock wavefront, moves every deblocking
@@ -275,22 +294,28 @@ def main() -> int:
     for j in jobs:
         for k, v in j.bytes().items():
             nb[k] += v
-    stage_bytes = [nb["mc"], nb["tx"], nb["dbk"] / 2, nb["dbk"] / 2, nb["sao"]]
+    vb, hb = dbk_segment_bytes(oracle, pics)
+    stage_bytes = [nb["mc"], nb["tx"], vb, hb, nb["sao"]]
     stage_gbs = [b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0 for b, ms in zip(stage_bytes, stage_ms)]
     dom = int(np.argmax(stage_ms))
     value = luma / t_graph / 1e6 if bit_exact else None
     chosen = {k: names[v] for k, names, v in zip(STAGES, rh.variant_names(), batch.variants())}
-    traffic, traffic_kernel = pmc_traffic(f"hevc_stage{dom}", args.streams, chosen[STAGES[dom]])
+    traffic, traffic_kernel = pmc_traffic(STAGES[dom], args.streams, chosen[STAGES[dom]])
+    # Stage 0 (motion compensation) is two kernels launched back to back:
+    # the tile kernel (side data, intra copies, blocks whose windows cross a
+    # plane edge) and the strip kernel (the other blocks).
+    stage_kernels = {0: "hevc_stage0 + hevc_mc_strips"}.get(dom, f"hevc_stage{dom}")
     roofline = {
         "bound": "hbm",
-        "kernel": f"hevc_stage{dom} ({STAGES[dom]})",
+        "kernel": f"{stage_kernels} ({STAGES[dom]}, one stage)",
         "achieved": round(stage_gbs[dom], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "traffic_source": TRAFFIC_FILE.relative_to(ROOT).as_posix() + f" ({traffic_kernel}: rocprofv3 --pmc "
-                          "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from the copy kernel)",
+        "traffic_source": (TRAFFIC_FILE.relative_to(ROOT).as_posix() + f" ({traffic_kernel}: rocprofv3 --pmc "
+                           "FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch, summed over the stage's kernels, "
+                           "factors from the copy kernel)") if traffic else None,
         "algorithmic_bytes": int(stage_bytes[dom]),
         "avg_launch_ms": round(stage_ms[dom], 4),
         "achievable_copy_gbs": round(copy_gbs, 1),
