@@ -418,6 +418,20 @@ int rv_picture_hash(rv_session* s, int slot, int type, void* out);
 #define RV_HASH_NOALLOC_MAX_ROWS 4352
 int rv_picture_hash_noalloc(rv_session* s, int slot, int type, void* out);
 
+/* The MD5 of each of n pictures packed planar: its Y rows, then Cb, then
+ * Cr, cropped to the output size widths[i] x heights[i] (chroma
+ * ceil(w / 2) x ceil(h / 2)), samples above 8 bits as two bytes low byte
+ * first — the digest of the per-frame line examples/h26xdec.rs:120-131
+ * prints for Picture::into_packed (src/picture.rs:155-163). Picture i is
+ * DPB slot slots[i] of sessions[i]; the hashes run after the latest launch
+ * that covered each session, one workgroup per picture in one launch.
+ * out receives 16 bytes per picture. MD5 is serial within a picture
+ * (about 0.1 GB/s per picture), so many pictures per call is the way to
+ * throughput. RV_E_RANGE for a slot outside its DPB or a size above the
+ * session's. */
+int rv_frame_md5(int n, rv_session* const* sessions, const int* slots, const int* widths, const int* heights,
+                 uint8_t* out);
+
 /* A prepared picture: the work lists copied into device memory (they are
  * not referenced after the call returns). The job writes its picture into
  * DPB slot out_slot of its session when launched; it may be launched many

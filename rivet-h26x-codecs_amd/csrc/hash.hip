@@ -203,7 +203,72 @@ __global__ void __launch_bounds__(256) hash_md5(HashPlanes H, uint8_t* out) {
         for (int i = 0; i < 16; ++i) out[16 * c + i] = (uint8_t)(st[i >> 2] >> (8 * (i & 3)));
 }
 
+// One workgroup per picture: the MD5 of the picture packed planar (the
+// Y rows, then Cb, then Cr, each cropped to the output size, samples as
+// bytes low byte first), the per-frame line of examples/h26xdec.rs:
+// 120-131. The message runs across the three planes; otherwise as
+// hash_md5.
+__global__ void __launch_bounds__(256) frame_md5(const FrameMd5* F, uint8_t* out) {
+    __shared__ uint32_t msg[kMd5Chunk / 4];
+    const FrameMd5 f = F[blockIdx.x];
+    const int t = threadIdx.x;
+    uint64_t len[3], n = 0;
+    for (int c = 0; c < 3; ++c) n += len[c] = (uint64_t)f.w[c] * f.ss * (uint64_t)f.h[c];
+    const uint64_t N = (n + 8) / 64 * 64 + 64, bits = 8 * n;
+    uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+    for (uint64_t base = 0; base < N; base += kMd5Chunk) {
+        // Lane t forms bytes base + 16 t .. + 15: plane c, row y, byte x.
+        uint64_t j = base + 16 * (uint64_t)t;
+        int c = 0;
+        uint64_t r = j;
+        while (c < 2 && r >= len[c]) r -= len[c++];
+        uint64_t L = (uint64_t)f.w[c] * f.ss;
+        uint64_t y = j < n ? r / L : 0, x = j < n ? r - y * L : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k, ++j) {
+                uint32_t v;
+                if (j < n) {
+                    v = f.p[c][y * (uint64_t)f.pitch[c] + x];
+                    if (++x == L) {
+                        x = 0;
+                        if (++y == (uint64_t)f.h[c] && c < 2) {
+                            y = 0;
+                            ++c;
+                            L = (uint64_t)f.w[c] * f.ss;
+                        }
+                    }
+                } else if (j == n) {
+                    v = 0x80u;
+                } else if (j >= N - 8 && j < N) {
+                    v = (uint32_t)(bits >> (8 * (j - (N - 8)))) & 0xFFu;
+                } else {
+                    v = 0u;
+                }
+                w |= v << (8 * k);
+            }
+            msg[4 * t + q] = w;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const uint64_t left = (N - base) / 64;
+            const int nb = left < kMd5Chunk / 64 ? (int)left : kMd5Chunk / 64;
+            for (int b = 0; b < nb; ++b) md5_block(msg + 16 * b, st);
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int i = 0; i < 16; ++i) out[16 * blockIdx.x + i] = (uint8_t)(st[i >> 2] >> (8 * (i & 3)));
+}
+
 }  // namespace
+
+hipError_t frame_md5_launch(const FrameMd5* frames, int n, uint8_t* out, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(frame_md5, dim3(n), dim3(256), 0, st, frames, out);
+    return hipGetLastError();
+}
 
 size_t hash_scratch_bytes(int type, const HashPlanes& H) {
     const int hmax = max(H.h[0], max(H.h[1], H.h[2]));

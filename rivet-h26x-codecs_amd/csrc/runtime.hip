@@ -947,6 +947,62 @@ int rv_picture_hash(rv_session* s, int slot, int type, void* out) {
     }
 }
 
+int rv_frame_md5(int n, rv_session* const* sessions, const int* slots, const int* widths, const int* heights,
+                 uint8_t* out) {
+    try {
+        if (n < 0) return fail(RV_E_INVALID, "negative picture count");
+        if (n == 0) return RV_OK;
+        if (!sessions || !slots || !widths || !heights || !out) return fail(RV_E_INVALID, "null argument");
+        if (!ready()) return fail(RV_E_NODEVICE, "rv_init not called");
+        std::vector<rv::FrameMd5> frames((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const rv_session* s = sessions[i];
+            if (!s) return fail(RV_E_INVALID, "null session");
+            if (slots[i] < 0 || slots[i] >= s->slots) return fail(RV_E_RANGE, "slot outside the DPB");
+            if (widths[i] <= 0 || heights[i] <= 0 || widths[i] > s->w || heights[i] > s->h)
+                return fail(RV_E_RANGE, "output size outside the session's pictures");
+            rv::FrameMd5& f = frames[(size_t)i];
+            for (int c = 0; c < 3; ++c) {
+                f.p[c] = (const uint8_t*)s->planes[(size_t)slots[i] * 3 + c];
+                f.w[c] = c ? (widths[i] + 1) / 2 : widths[i];
+                f.h[c] = c ? (heights[i] + 1) / 2 : heights[i];
+                f.pitch[c] = s->g.pitch[c] * s->g.ss;
+            }
+            f.ss = s->g.ss;
+        }
+        const size_t tbl = align_up((size_t)n * sizeof(rv::FrameMd5), 256);
+        DevBuf buf;
+        if (!pool_get(tbl + 16 * (size_t)n, &buf)) return fail(RV_E_NOMEM, "frame hash buffer allocation failed");
+        hipEvent_t ev = nullptr;
+        int rc = RV_OK;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "event create");
+        } else {
+            // On the kernel stream, after the launches that wrote the slots.
+            std::lock_guard<std::mutex> lk(g_mu);
+            e = hipMemcpyAsync(buf.p, frames.data(), (size_t)n * sizeof(rv::FrameMd5), hipMemcpyHostToDevice, g_stream);
+            if (e == hipSuccess)
+                e = rv::frame_md5_launch((const rv::FrameMd5*)buf.p, n, (uint8_t*)buf.p + tbl, g_stream);
+            if (e == hipSuccess) e = hipEventRecord(ev, g_stream);
+            if (e != hipSuccess) rc = hip_fail(e, "frame hash launch");
+        }
+        if (rc == RV_OK) {
+            e = hipEventSynchronize(ev);
+            if (e == hipSuccess) e = hipMemcpy(out, (uint8_t*)buf.p + tbl, 16 * (size_t)n, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "frame hash result");
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            pool_put(buf);
+        }
+        if (ev) (void)hipEventDestroy(ev);
+        return rc;
+    } catch (...) {
+        return fail(RV_E_NOMEM, "exception in rv_frame_md5");
+    }
+}
+
 int rv_picture_hash_noalloc(rv_session* s, int slot, int type, void* out) {
     try {
         if (!s || !out) return fail(RV_E_INVALID, "null argument");

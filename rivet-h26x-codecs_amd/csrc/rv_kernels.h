@@ -37,6 +37,15 @@ struct HashPlanes {
 size_t hash_scratch_bytes(int type, const HashPlanes& H);
 hipError_t hash_launch(int type, const HashPlanes& H, void* scratch, hipStream_t st);
 
+// The MD5 of each picture packed planar (Y, Cb, Cr rows cropped to w x h,
+// samples of ss bytes, low byte first): 16 bytes per picture into out.
+struct FrameMd5 {
+    const uint8_t* p[3];
+    int w[3], h[3], pitch[3];  // pitch in bytes
+    int ss;
+};
+hipError_t frame_md5_launch(const FrameMd5* frames, int n, uint8_t* out, hipStream_t st);
+
 // Plain copy of `bytes` (a multiple of 4096) with 16-byte vectors.
 hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStream_t st);
 

@@ -177,6 +177,31 @@ class Session:
             pass
 
 
+def frame_md5(pictures) -> list:
+    """MD5 digests (16 bytes each) of pictures packed planar, computed on
+    the device in one launch (rv_frame_md5): `pictures` is a sequence of
+    (session, slot, width, height), width x height the output (cropped)
+    luma size."""
+    n = len(pictures)
+    if n == 0:
+        return []
+    sess = (C.c_void_p * n)(*[p[0]._h.value for p in pictures])
+    slots = (C.c_int * n)(*[p[1] for p in pictures])
+    ws = (C.c_int * n)(*[p[2] for p in pictures])
+    hs = (C.c_int * n)(*[p[3] for p in pictures])
+    out = (C.c_uint8 * (16 * n))()
+    check(lib.rv_frame_md5(n, sess, slots, ws, hs, out))
+    raw = bytes(out)
+    return [raw[16 * i:16 * i + 16] for i in range(n)]
+
+
+def frame_line(index: int, poc: int, decode_index: int, width: int, height: int, digest: bytes) -> str:
+    """One output frame as examples/h26xdec.rs:124-132 prints it:
+    frame index, POC, decode index, WxH and the MD5 of the packed picture
+    in lowercase hex."""
+    return f"{index},{poc},{decode_index},{width}x{height},{digest.hex()}"
+
+
 class Job:
     """A prepared picture (work lists on the device)."""
 

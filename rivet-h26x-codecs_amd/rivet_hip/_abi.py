@@ -162,6 +162,7 @@ PROTOTYPES = {
     "rv_dpb_upload": (i32, [P, i32, P, P, P]),
     "rv_plane_download": (i32, [P, i32, P, P, P]),
     "rv_picture_hash": (i32, [P, i32, i32, P]),
+    "rv_frame_md5": (i32, [i32, P, P, P, P, P]),
     "rv_picture_hash_noalloc": (i32, [P, i32, i32, P]),
     "rv_hevc_prepare": (i32, [P, C.POINTER(HevcPic), i32, C.POINTER(P)]),
     "rv_h264_prepare": (i32, [P, C.POINTER(H264Pic), i32, C.POINTER(P)]),

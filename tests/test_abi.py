@@ -142,3 +142,10 @@ def test_picture_hash_refuses_bad_arguments_without_a_device():
     buf = (C.c_uint8 * 48)()
     assert lib.rv_picture_hash(None, 0, 0, buf) == -1  # null session
     assert lib.rv_picture_hash(None, 0, 9, buf) == -1
+
+
+def test_frame_line_matches_h26xdec():
+    # examples/h26xdec.rs:124-132: "{n},{poc},{decode_index},{w}x{h},{md5 hex}".
+    import rivet_hip as rh
+    d = bytes(range(16))
+    assert rh.frame_line(7, -2, 5, 854, 480, d) == "7,-2,5,854x480,000102030405060708090a0b0c0d0e0f"
