@@ -39,6 +39,11 @@ METRIC = "bit-exact luma Msample/s 1080p/4K batched; p99 picture latency, serve 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["mc", "tx", "dbk_v", "dbk_h", "sao"]
 LEVELS_NOTE = "; coefficients as levels with scaling lists, scaled on the device"
+INTRA_NOTE = "; intra blocks predicted on the device"
+
+
+def workload_note(extra_cfg: dict) -> str:
+    return LEVELS_NOTE * bool(extra_cfg.get("levels")) + INTRA_NOTE * bool(extra_cfg.get("intra_device"))
 
 
 def log(*a):
@@ -194,7 +199,7 @@ def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int,
     luma = n * cf.visible_luma(w, h)
     out = {
         "workload": f"{n} x config {config} ({'HEVC' if codec == rh.RV_CODEC_HEVC else 'H.264'} {w}x{h}, "
-                    f"{bd}-bit), batched, hipGraph replay" + LEVELS_NOTE * bool(extra_cfg),
+                    f"{bd}-bit), batched, hipGraph replay" + workload_note(extra_cfg),
         "bit_exact": ok,
         "ms_per_step": round(t * 1e3, 4),
         "luma_msamples_per_s": round(luma / t / 1e6, 1) if ok else None,
@@ -238,6 +243,10 @@ def main() -> int:
     ap.add_argument("--levels", action="store_true",
                     help="generate coefficients as levels with seeded scaling lists, scaled on the device in "
                          "stage 1 (same pictures' structure; the coefficient values differ)")
+    ap.add_argument("--device-intra", action="store_true",
+                    help="predict the intra blocks of configurations 2 and 4 on the device (RV_INTRA_DEVICE: HEVC "
+                         "intra CUs of every size, H.264 intra macroblocks of every kind) instead of taking "
+                         "host-supplied prediction")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -256,6 +265,8 @@ def main() -> int:
         log(f"[bench] copy probe: 256 MiB read + 256 MiB written, {gbs:.1f} GB/s")
         return 0
     extra_cfg = dict(levels=1, scaling=1) if args.levels else {}
+    if args.device_intra:
+        extra_cfg["intra_device"] = 1
     if args.only_config:
         r = measure_config(rh, cf, oracle, args.only_config, 8, args.steps, args.warmup, extra_cfg)
         log(f"[bench] config {args.only_config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms")
@@ -336,7 +347,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{args.streams} x HEVC Main 8-bit 4:2:0 3840x2160 P/B pictures (one per stream), "
-                        "batched launch per stage, hipGraph replay" + LEVELS_NOTE * args.levels,
+                        "batched launch per stage, hipGraph replay" + workload_note(extra_cfg),
             "streams": args.streams, "width": 3840, "height": 2160, "bit_depth": 8,
             "pictures_per_step": len(pics),
         },
