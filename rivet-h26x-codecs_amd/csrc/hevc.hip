@@ -911,7 +911,7 @@ __device__ __forceinline__ uint32_t pair_rows(int lo, int hi) {
 
 // One strip (LUMA: H rows of 8-tap; else H rows of 4-tap chroma), both
 // lists when BI, the one in st.dir otherwise.
-template <bool LUMA, int H, bool BI>
+template <bool LUMA, int H, bool BI, int PF>
 __device__ __forceinline__ void mc_strip(PicRef P, const McStrip& st, const uint32_t* tab) {
     constexpr int T = LUMA ? 8 : 4;       // taps
     constexpr int R = H + T - 1;          // window rows (odd)
@@ -986,7 +986,6 @@ __device__ __forceinline__ void mc_strip(PicRef P, const McStrip& st, const uint
     // compiler from sinking them to their use), so each wave keeps PF pairs
     // of every list in flight.
     constexpr int NPAIR = (R + 1) / 2;
-    constexpr int PF = 1;
     StripRow<LUMA> buf[PF + 1][NL][2];
     auto load_pair = [&](int kp, StripRow<LUMA> (&into)[NL][2]) {
 #pragma unroll
@@ -1019,26 +1018,29 @@ __device__ __forceinline__ void mc_strip(PicRef P, const McStrip& st, const uint
     }
 }
 
+template <int PF>
 __device__ __forceinline__ void strip_group(PicRef P, int g, const McStrip& st, const uint32_t* tab) {
     switch (g) {
-        case 0: mc_strip<true, 16, true>(P, st, tab); break;
-        case 1: mc_strip<true, 16, false>(P, st, tab); break;
-        case 2: mc_strip<true, 8, true>(P, st, tab); break;
-        case 3: mc_strip<true, 8, false>(P, st, tab); break;
-        case 4: mc_strip<true, 4, true>(P, st, tab); break;
-        case 5: mc_strip<true, 4, false>(P, st, tab); break;
-        case 6: mc_strip<false, 8, true>(P, st, tab); break;
-        case 7: mc_strip<false, 8, false>(P, st, tab); break;
-        case 8: mc_strip<false, 4, true>(P, st, tab); break;
-        case 9: mc_strip<false, 4, false>(P, st, tab); break;
-        case 10: mc_strip<false, 2, true>(P, st, tab); break;
-        default: mc_strip<false, 2, false>(P, st, tab); break;
+        case 0: mc_strip<true, 16, true, PF>(P, st, tab); break;
+        case 1: mc_strip<true, 16, false, PF>(P, st, tab); break;
+        case 2: mc_strip<true, 8, true, PF>(P, st, tab); break;
+        case 3: mc_strip<true, 8, false, PF>(P, st, tab); break;
+        case 4: mc_strip<true, 4, true, PF>(P, st, tab); break;
+        case 5: mc_strip<true, 4, false, PF>(P, st, tab); break;
+        case 6: mc_strip<false, 8, true, PF>(P, st, tab); break;
+        case 7: mc_strip<false, 8, false, PF>(P, st, tab); break;
+        case 8: mc_strip<false, 4, true, PF>(P, st, tab); break;
+        case 9: mc_strip<false, 4, false, PF>(P, st, tab); break;
+        case 10: mc_strip<false, 2, true, PF>(P, st, tab); break;
+        default: mc_strip<false, 2, false, PF>(P, st, tab); break;
     }
 }
 
 // One wave per 64 strips of one group of one 8-bit picture (b.spre: prefix
-// over pictures of their strip waves).
-template <int W>
+// over pictures of their strip waves). W waves per SIMD, PF row pairs of
+// each list in flight per lane (PF 2 at W 3 fits in 155 VGPRs; PF 3 needs
+// W 2, which leaves too few waves to cover the loads).
+template <int W, int PF = 2>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) hevc_mc_strips(BatchDev b) {
     __shared__ uint32_t taps[kTapWords];
     ConstI32* spre = (ConstI32*)b.spre;
@@ -1054,7 +1056,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
     const int g = __builtin_amdgcn_readfirstlane((int)((const uint8_t __attribute__((address_space(4)))*)P.swg)[w]);
     const McStrip st = __builtin_bit_cast(McStrip, *(const uint4*)(P.strips + (size_t)w * 64 + tid));
     __syncthreads();
-    strip_group(P, g, st, taps);
+    strip_group<PF>(P, g, st, taps);
 }
 
 // ---------------------------------------------------------------------
@@ -1135,10 +1137,6 @@ __device__ __forceinline__ void idst1d(const int (&s)[4], int (&o)[4]) {
     o[2] = 74 * s[0] + 0 * s[1] - 74 * s[2] + 74 * s[3];
     o[3] = 84 * s[0] - 74 * s[1] + 55 * s[2] - 29 * s[3];
 }
-
-// Coded TUs per work item (one wave) of each size class (4, 8, 16, 32):
-// 64 / TPW lanes per TU.
-__host__ __device__ constexpr int tus_per_item(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : cls == 2 ? 16 : 4; }
 
 // Add one row of N residuals to N samples at d (16 * k or 8 bytes,
 // aligned), clipping to [0, max] (add_residual_scalar, src/dsp/hevc.rs:558-566).
@@ -1430,12 +1428,8 @@ __device__ __forceinline__ void tu4_lane(PicRef P, int first, int count, int tid
 
 template <typename S>
 __device__ __forceinline__ void tx_item(PicRef P, int i, int tid, int16_t* lds) {
-    int cls = 0;
-    while (cls < 3 && i >= P.tx_items[cls + 1]) ++cls;
-    const int tpw = tus_per_item(cls);
-    const int k = i - P.tx_items[cls];
-    const int first = P.tx_first[cls] + k * tpw;
-    const int count = min(tpw, P.n_tx[cls] - k * tpw);
+    const uint32_t w = ((const uint32_t __attribute__((address_space(4)))*)P.txw)[i];
+    const int cls = (int)(w >> 30), first = (int)(w & 0xFFFFFF), count = (int)((w >> 24) & 63) + 1;
     switch (cls) {
         case 0: tu4_lane<S>(P, first, count, tid, lds); break;
         case 1: tu_lanes<8, 64 / tus_per_item(1), S>(P, first, count, tid, lds); break;
@@ -1607,12 +1601,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) he
     PicRef P = picture(b, p);
     int i = __builtin_amdgcn_readfirstlane(it - pre[p]);
     const int tid = threadIdx.x;
-    if (i < P.tx_items[4]) {
+    if (i < P.n_txw) {
         if (eight_bit<D>(P.bd)) tx_item<uint8_t>(P, i, tid, lds);
         else tx_item<uint16_t>(P, i, tid, lds);
         return;
     }
-    i -= P.tx_items[4];
+    i -= P.n_txw;
     const int a = i * 64 + tid;
     if (a < (P.w4 >> 1) * (P.h4 >> 1)) area_bs(P, a);
 }
@@ -2663,12 +2657,7 @@ hipError_t launch_copy_probe(void* dst, const void* src, int64_t bytes, hipStrea
 // Work items per stage for one picture (the grid unit of each stage).
 void hevc_stage_items(HevcPicDev& P) {
     P.items[0] = P.n_side + P.n_intra + P.n_tiles16 + (P.n_tiles + 3) / 4;
-    P.tx_items[0] = 0;
-    for (int c = 0; c < 4; ++c) {
-        const int tpw = tus_per_item(c);
-        P.tx_items[c + 1] = P.tx_items[c] + (P.n_tx[c] + tpw - 1) / tpw;
-    }
-    P.items[1] = P.tx_items[4] + ((P.w4 >> 1) * (P.h4 >> 1) + 63) / 64;
+    P.items[1] = P.n_txw + ((P.w4 >> 1) * (P.h4 >> 1) + 63) / 64;
     P.items[2] = ((P.w >> 3) * P.h4 + 255) / 256;
     P.items[3] = (P.h >= 16) ? ((((P.h >> 3) - 1) * P.w4) + 255) / 256 : 0;
     P.items[4] = P.wc * P.hc;
@@ -2695,7 +2684,7 @@ struct StageVariant {
 const StageVariant kStage0[] = {
     RV_VARIANT0((hevc_stage0<0, 5>), (hevc_mc_strips<3>), 0, "any/5, strips/3"),
     RV_VARIANT0((hevc_stage0<1, 7>), (hevc_mc_strips<3>), 1, "8-bit/7, strips/3"),
-    RV_VARIANT0((hevc_stage0<1, 8, 1>), (hevc_mc_strips<4>), 1, "8-bit/8 per-list, strips/4"),
+    RV_VARIANT0((hevc_stage0<1, 8, 1>), (hevc_mc_strips<4, 1>), 1, "8-bit/8 per-list, strips/4"),
     RV_VARIANT0((hevc_stage0<1, 8, 1>), (hevc_mc_strips<3>), 1, "8-bit/8 per-list, strips/3"),
     RV_VARIANT0((hevc_stage0<2, 5>), (hevc_mc_strips<3>), 2, "deep/5"),
     RV_VARIANT0((hevc_stage0<2, 6, 1>), (hevc_mc_strips<3>), 2, "deep/6 per-list"),
@@ -2715,7 +2704,9 @@ struct StageTable {
     const StageVariant* v;
     int n;
 };
-const StageTable kStageTables[kStages] = {{kStage0, 9}, {kStage1, 3}, {kStage2, 1}, {kStage3, 1}, {kStage4, 3}};
+template <size_t N> constexpr StageTable table(const StageVariant (&v)[N]) { return StageTable{v, (int)N}; }
+const StageTable kStageTables[kStages] = {table(kStage0), table(kStage1), table(kStage2), table(kStage3),
+                                          table(kStage4)};
 
 }  // namespace
 

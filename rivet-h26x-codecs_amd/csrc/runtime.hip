@@ -1164,9 +1164,16 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                 intra_area += (int64_t)1 << (2 * c.log2_size);
             }
         }
-        int ltu_cnt[5] = {0, 0, 0, 0, 0}, cls_cnt[4] = {0, 0, 0, 0};
-        size_t cls_nz[4] = {0, 0, 0, 0}, n_ltus = 0, n_ctus = 0, n_nzs = 0;
+        int ltu_cnt[5] = {0, 0, 0, 0, 0};
+        size_t n_ltus = 0, n_ctus = 0, n_nzs = 0;
         int64_t coded_area = 0;
+        // Stage-1 segments: (band of 1 << kTxBandShift luma rows, size
+        // class), band-major.
+        const int n_tbands = (H + (1 << rv::kTxBandShift) - 1) >> rv::kTxBandShift;
+        std::vector<int32_t> seg_cnt((size_t)n_tbands * 4, 0);
+        auto tx_seg = [&](const rv_hevc_tu& t) {
+            return (size_t)(((t.comp ? 2 * t.y : t.y) >> rv::kTxBandShift) * 4 + (t.log2 - 2));
+        };
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
             if (t.comp == 0) {
@@ -1174,12 +1181,18 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                 ++ltu_cnt[(5 - t.log2) + 1];
             }
             if ((t.flags & RV_TU_CODED) && !is_itu(i)) {
-                ++cls_cnt[t.log2 - 2];
-                cls_nz[t.log2 - 2] += t.n_nz;
+                ++seg_cnt[tx_seg(t)];
                 ++n_ctus;
                 n_nzs += t.n_nz;
                 coded_area += (int64_t)1 << (2 * t.log2);
             }
+        }
+        // At most 8192 x 8192: 6.3 M TUs, inside txw's 24-bit index.
+        if (n_ctus >= (1u << 24)) return fail(RV_E_RANGE, "more coded TUs than a stage-1 item can index");
+        size_t n_txw = 0;
+        for (size_t k = 0; k < seg_cnt.size(); ++k) {
+            const int tpw = rv::tus_per_item((int)(k & 3));
+            n_txw += (size_t)((seg_cnt[k] + tpw - 1) / tpw);
         }
         for (int k = 0; k < 256; ++k) pu_cnt[(size_t)k + 1] += pu_cnt[(size_t)k];
         for (int k = 0; k < 4; ++k) cu_cnt[k + 1] += cu_cnt[k];
@@ -1216,7 +1229,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         }
         // Device arena.
         size_t sz = carve_size<rv::HevcPicDev>(1) + carve_size<rv_hevc_pu>(p->n_pus) + carve_size<rv_hevc_cu>(p->n_cus) +
-                    carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) +
+                    carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) + carve_size<uint32_t>(n_txw) +
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
                     carve_size<rv::McStrip>(n_strips) + carve_size<uint8_t>(n_swaves) + carve_size<int32_t>(2) +
                     carve_size<rv::SideClass>(n_sclass) +
@@ -1243,6 +1256,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv_hevc_cu* h_cus = carve<rv_hevc_cu>(hp, p->n_cus);
         rv_hevc_tu* h_ltus = carve<rv_hevc_tu>(hp, n_ltus);
         rv_hevc_tu* h_ctus = carve<rv_hevc_tu>(hp, n_ctus);
+        uint32_t* h_txw = carve<uint32_t>(hp, n_txw);
         rv::McTile* h_tiles = carve<rv::McTile>(hp, n_tiles_pad);
         rv::McTile* h_tiles16 = carve<rv::McTile>(hp, n_tiles16);
         rv::McStrip* h_strips = carve<rv::McStrip>(hp, n_strips);
@@ -1325,9 +1339,15 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             h_cus[at] = c;
             if (c.intra == RV_INTRA_HOST) h_intra[ni++] = at;
         }
-        size_t cls_at[4], nz_at[4];
-        for (int k = 0, a = 0; k < 4; a += cls_cnt[k], ++k) cls_at[k] = (size_t)a;
-        for (int k = 0; k < 4; ++k) nz_at[k] = k ? nz_at[k - 1] + cls_nz[k - 1] : 0;
+        // Segment starts in ctus, and the items of each segment.
+        std::vector<size_t> seg_at(seg_cnt.size());
+        for (size_t k = 0, a = 0, w = 0; k < seg_cnt.size(); a += (size_t)seg_cnt[k], ++k) {
+            seg_at[k] = a;
+            const int tpw = rv::tus_per_item((int)(k & 3));
+            for (int f = 0; f < seg_cnt[k]; f += tpw)
+                h_txw[w++] = (uint32_t)(a + (size_t)f) | (uint32_t)(std::min(tpw, seg_cnt[k] - f) - 1) << 24 |
+                             (uint32_t)(k & 3) << 30;
+        }
         // Device-intra TUs by level (list order within a level), their
         // coefficients after the stage-1 ones.
         if (n_ilevels) {
@@ -1372,14 +1392,14 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
             if (t.comp == 0) h_ltus[ltu_cnt[5 - t.log2]++] = t;
-            if ((t.flags & RV_TU_CODED) && !is_itu(i)) {
-                const int k = t.log2 - 2;
-                rv_hevc_tu& c = h_ctus[cls_at[k]++];
-                c = t;
-                c.coef_off = (uint32_t)nz_at[k];
-                std::memcpy(h_coefs + nz_at[k], p->nz + t.coef_off, t.n_nz * sizeof(rv_nz));
-                nz_at[k] += t.n_nz;
-            }
+            if ((t.flags & RV_TU_CODED) && !is_itu(i)) h_ctus[seg_at[tx_seg(t)]++] = t;
+        }
+        // Coefficients in ctus order, so a wave's TUs read one run.
+        for (size_t k = 0, at = 0; k < n_ctus; ++k) {
+            rv_hevc_tu& c = h_ctus[k];
+            std::memcpy(h_coefs + at, p->nz + c.coef_off, c.n_nz * sizeof(rv_nz));
+            c.coef_off = (uint32_t)at;
+            at += c.n_nz;
         }
         if (p->n_intra_pred) std::memcpy(h_ip, p->intra_pred, p->n_intra_pred * sizeof(uint16_t));
         if (p->sao) std::memcpy(h_sao, p->sao, n_sao * sizeof(rv_sao));
@@ -1434,11 +1454,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.sclass = (const rv::SideClass*)dev((char*)h_sclass);
         d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
-        for (int c = 0, acc = 0; c < 4; ++c) {
-            d.n_tx[c] = cls_cnt[c];
-            d.tx_first[c] = acc;
-            acc += d.n_tx[c];
-        }
+        d.n_txw = (int32_t)n_txw;
+        d.txw = (const uint32_t*)dev((char*)h_txw);
         d.cus = (const rv_hevc_cu*)dev((char*)h_cus);
         d.pus = (const rv_hevc_pu*)dev((char*)h_pus);
         d.ltus = (const rv_hevc_tu*)dev((char*)h_ltus);

@@ -43,6 +43,11 @@ static_assert(sizeof(McStrip) == 16, "one 16-byte load per strip");
 // Groups: luma heights 16, 8, 4 then chroma heights 8, 4, 2, each bi then
 // uni (group = kind * 6 + height class * 2 + (uni ? 1 : 0)).
 constexpr int kStripGroups = 12;
+// Coded TUs per stage-1 work item (one wave) of each size class (4, 8,
+// 16, 32): 64 / TPW lanes per TU.
+__host__ __device__ constexpr int tus_per_item(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : cls == 2 ? 16 : 4; }
+// Bands of the stage-1 item order: 1 << kTxBandShift luma rows.
+constexpr int kTxBandShift = 7;
 
 // One transform block of an RV_INTRA_DEVICE CU: the TU (coef_off rebased
 // into the job's nonzero list), its intra prediction mode, where its
@@ -92,9 +97,11 @@ struct HevcPicDev {
     int32_t n_sclass;         // side-data classes
     const SideClass* sclass;
     const int32_t* intra_idx; // indices of the intra CUs in cus
-    int32_t n_tx[4];         // coded TUs per size (4, 8, 16, 32), in that order
-    int32_t tx_first[4];     // index in ctus of each size's first TU
-    int32_t tx_items[5];     // stage-1 item offsets per size (64 / N TUs per item)
+    // Stage-1 transform items: item i takes ctus[first .. first + count)
+    // of one size class cls (4, 8, 16, 32; 64 / N TUs per item), packed as
+    // txw[i] = first | (count - 1) << 24 | cls << 30.
+    int32_t n_txw;
+    const uint32_t* txw;
     const rv_hevc_cu* cus;
     const rv_hevc_pu* pus;
     const rv_hevc_tu* ltus;  // luma TUs (coded or not): side data
@@ -108,7 +115,12 @@ struct HevcPicDev {
     const McStrip* strips;
     const uint8_t* swg;
     int32_t n_swaves;
-    const rv_hevc_tu* ctus;  // coded TUs grouped by size, smallest first
+    // Coded TUs band by band (128 luma rows; chroma TUs by the luma rows
+    // they cover), by size class within a band, list order within a
+    // class: the items of one band run together, so the partial lines of
+    // neighbouring TUs of different sizes meet in L2 rather than going to
+    // memory once per TU.
+    const rv_hevc_tu* ctus;
     const rv_nz* nz;         // nonzero coefficients of ctus, in ctus order
     const uint16_t* intra_pred;
     const rv_sao* sao;
