@@ -1601,6 +1601,10 @@ struct DbkPic {
     size_t pc;
     gptr<const uint32_t> prm;
     int mbw, mbh;
+    // The exchange's poll bound, tag mask and error counter.
+    int spin;
+    uint32_t tag_xor;
+    uint32_t* err;
 };
 
 __device__ __forceinline__ void dbk_fetch(const DbkPic& D, bool lu, int x, int y, int r, DbkIn& in) {
@@ -1633,15 +1637,16 @@ __device__ __forceinline__ uint64_t xch_load(gptr<const uint64_t> p) {
 __device__ __forceinline__ void xch_store(gptr<uint64_t> p, uint64_t v) {
     __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Poll an exchange word until it carries `tag`, at most P.xch_spin times;
-// a wait that gives up is counted in *P.err.
-__device__ __forceinline__ uint64_t xch_wait(const H264PicDev& P, gptr<const uint64_t> p, uint64_t v, uint32_t tag) {
-    tag ^= P.xch_tag_xor;
-    for (int n = 0; (uint32_t)(v >> 32) != tag && n < P.xch_spin; ++n) {
+// Poll an exchange word until it carries `tag`, at most D.spin times; a
+// wait that gives up is counted in *D.err. The fields come from registers:
+// re-reading the descriptor inside the step would wait on LDS as well.
+__device__ __forceinline__ uint64_t xch_wait(const DbkPic& D, gptr<const uint64_t> p, uint64_t v, uint32_t tag) {
+    tag ^= D.tag_xor;
+    for (int n = 0; (uint32_t)(v >> 32) != tag && n < D.spin; ++n) {
         __builtin_amdgcn_s_sleep(1);
         v = xch_load(p);
     }
-    if ((uint32_t)(v >> 32) != tag) __hip_atomic_fetch_add(P.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(v >> 32) != tag) __hip_atomic_fetch_add(D.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return v;
 }
 
@@ -1663,7 +1668,8 @@ __global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
     const int slot = t >> 4, r = t & 15;
     const int pl = r >> 3, rr = r & 7;  // chroma lanes: plane, row / column
     const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
-                   (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh};
+                   (size_t)P.pitch[1 + pl], to_global((const uint32_t*)P.dbk), P.mbw, P.mbh,
+                   P.xch_spin, P.xch_tag_xor, P.err};
     const int mbw = D.mbw, mbh = D.mbh;
     const int y0 = band * kDbkSlots, rows = min(kDbkSlots, mbh - y0);
     const bool first = slot == 0 && band > 0;  // imports its strips
@@ -1771,7 +1777,7 @@ __global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
                 const uint32_t* ue = up + cex * kDbkRing;
                 u = lu ? ue[r] : ((const uint16_t*)(ue + 16))[pl * 8 + rr];
             } else if (first) {
-                u = (uint32_t)xch_wait(P, imp + (size_t)cx * kDbkRing, in.ix, (uint32_t)cx + 1);
+                u = (uint32_t)xch_wait(D, imp + (size_t)cx * kDbkRing, in.ix, (uint32_t)cx + 1);
                 if (!lu) u = (u >> (16 * (r & 1))) & 0xFFFF;
             }
             if (lu) {
