@@ -917,12 +917,13 @@ __device__ __forceinline__ void residual8_vals(const int (&c)[64], int16_t (&r)[
 // the luma tile y[1 + r][1 + c] = sample (c, r) of the macroblock for
 // r, c in -1..15, row 0 extending to c = 23 (the samples above-right);
 // the chroma tiles likewise (c in -1..7); the filtered edge of the 8x8
-// block being predicted (t[0..15], l[0..7], corner).
+// block being predicted (t[0..15], l[0..7], corner); the luma modes.
 struct alignas(16) H264IntraLds {
     int16_t res[384];
     uint8_t y[17][32];
     uint8_t c[2][9][16];
     int32_t edge[25];
+    uint8_t modes[16];
 };
 
 __device__ __forceinline__ void lds_sync() {
@@ -939,7 +940,7 @@ struct IAv {
 // src/h264/intra.rs:109-325, 427-620; the two sizes share one form) from
 // the reference samples t[0..2N), l[0..N), c (unavailable ones zero, the
 // top-right substituted).
-__device__ int nxn_sample(const int* t, const int* l, int c, int n, int mode, int x, int y, IAv av) {
+__device__ __forceinline__ int nxn_sample(const int* t, const int* l, int c, int n, int mode, int x, int y, IAv av) {
     auto T = [&](int i) { return i < 0 ? c : t[i]; };
     auto Lf = [&](int i) { return i < 0 ? c : l[i]; };
     switch (mode) {
@@ -1026,8 +1027,10 @@ __global__ void __launch_bounds__(64) h264_intra_level(BatchDev b, int level) {
     const bool A = nb & 1, B = nb & 2, Cc = nb & 4, D = nb & 8;
     uint8_t* Yp = (uint8_t*)P.cur[0];
     const int py = P.pitch[0], pc = P.pitch[1];
-    // 1. Residual values cleared; the neighbouring samples read.
+    // 1. Residual values cleared; the neighbouring samples read; the modes
+    // (read per block by a position that is not a constant) into LDS.
     for (int i = tid; i < 192; i += 64) ((uint32_t*)L.res)[i] = 0u;
+    if (tid >= 48) L.modes[tid - 48] = m.modes[tid - 48];
     if (tid < 25) {
         const int x = tid - 1;
         const bool ok = x < 0 ? D : x < 16 ? B : Cc;
@@ -1127,9 +1130,9 @@ __global__ void __launch_bounds__(64) h264_intra_level(BatchDev b, int level) {
             }
             const IAv av = block_av(nb, bx, by, n);
             const int r0 = by * 4, c0 = bx * 4;  // tile row / column of the row above / column left
-            // Reference samples: unfiltered for 4x4, filtered (8.3.2.2.1)
-            // into LDS for 8x8 by lanes 0..24.
-            int t[16], l[8], c;
+            // Reference samples into LDS (edge: t[0..2n), l[0..n) from 16,
+            // c at 24): unfiltered for 4x4, filtered (8.3.2.2.1) for 8x8, by
+            // lanes 0..24.
             if (n == 8) {
                 if (tid < 25) {
                     auto top = [&](int i) {  // p[i, -1] with the substitution
@@ -1161,23 +1164,21 @@ __global__ void __launch_bounds__(64) h264_intra_level(BatchDev b, int level) {
                     }
                     L.edge[tid] = o;
                 }
-                lds_sync();
-#pragma unroll
-                for (int i = 0; i < 16; ++i) t[i] = L.edge[i];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) l[j] = L.edge[16 + j];
-                c = L.edge[24];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) t[i] = av.top ? (int)L.y[r0][1 + c0 + (i < 4 || av.tr ? i : 3)] : 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) l[j] = av.left ? (int)L.y[r0 + 1 + j][c0] : 0;
-                c = av.tl ? (int)L.y[r0][c0] : 0;
+            } else if (tid < 25) {
+                // 4x4: t[0..8) (the top-right substituted), l[0..4), c.
+                int o = 0;
+                if (tid < 8) o = av.top ? (int)L.y[r0][1 + c0 + (tid < 4 || av.tr ? tid : 3)] : 0;
+                else if (tid >= 16 && tid < 20) o = av.left ? (int)L.y[r0 + 1 + tid - 16][c0] : 0;
+                else if (tid == 24) o = av.tl ? (int)L.y[r0][c0] : 0;
+                L.edge[tid] = o;
             }
-            const int mode = m.modes[by * 4 + bx];
+            lds_sync();
+            const int mode = L.modes[by * 4 + bx];
             if (tid < n * n) {
                 const int x = tid % n, y = tid / n;
-                const int v = nxn_sample(t, l, c, n, mode, x, y, av);
+                // The lanes read the block's references from LDS (indexed by
+                // position: registers would go to scratch).
+                const int v = nxn_sample(L.edge, L.edge + 16, L.edge[24], n, mode, x, y, av);
                 L.y[r0 + 1 + y][c0 + 1 + x] = (uint8_t)clampi(v + L.res[(r0 + y) * 16 + c0 + x], 0, 255);
             }
             lds_sync();
@@ -1257,12 +1258,16 @@ __device__ __forceinline__ bool far4(const int* a, const int* b) {
 }
 
 // motion_bs (:97-146), frame macroblocks.
-__device__ int motion_bs(const Mot& p, const Mot& q) {
+__device__ __forceinline__ int motion_bs(const Mot& p, const Mot& q) {
     if (p.n != q.n) return 1;
     if (p.n == 1) {
-        int lp = p.ref[0] >= 0 ? 0 : 1, lq = q.ref[0] >= 0 ? 0 : 1;
-        if (p.ref[lp] != q.ref[lq]) return 1;
-        return far4(p.mv[lp], q.mv[lq]);
+        // The one list each uses, by selects (an index that is not a
+        // constant would put both records in scratch memory).
+        const bool lp = p.ref[0] < 0, lq = q.ref[0] < 0;
+        if ((lp ? p.ref[1] : p.ref[0]) != (lq ? q.ref[1] : q.ref[0])) return 1;
+        const int a[2] = {lp ? p.mv[1][0] : p.mv[0][0], lp ? p.mv[1][1] : p.mv[0][1]};
+        const int b[2] = {lq ? q.mv[1][0] : q.mv[0][0], lq ? q.mv[1][1] : q.mv[0][1]};
+        return far4(a, b);
     }
     bool straight = p.ref[0] == q.ref[0] && p.ref[1] == q.ref[1];
     bool crossed = p.ref[0] == q.ref[1] && p.ref[1] == q.ref[0];
