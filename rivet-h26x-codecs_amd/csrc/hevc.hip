@@ -734,12 +734,7 @@ typedef const SideClass __attribute__((address_space(4))) ConstSide;
 
 __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     ConstSide* cl = (ConstSide*)P.sclass;
-    int lo = 0, hi = P.n_sclass - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cl[mid].item0 <= item) lo = mid;
-        else hi = mid - 1;
-    }
+    const int lo = ((const uint16_t __attribute__((address_space(4)))*)P.sidx)[item];
     const int kind = cl[lo].kind, first = cl[lo].first, count = cl[lo].count, cw = cl[lo].cw, ch = cl[lo].ch;
     const int cells = cw * ch, base = (item - cl[lo].item0) * kSideCellsPerItem;
     constexpr int R = kSideCellsPerItem / 64;

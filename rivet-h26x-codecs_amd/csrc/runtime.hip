@@ -1088,8 +1088,16 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         int64_t b_mc = 0;
         const bool weighted = p->weighted != 0;
         // PUs by shape (w, h descending), the side-data classes.
-        auto pu_bucket = [](const rv_hevc_pu& u) { return (size_t)((16 - u.w / 4) * 16 + (16 - u.h / 4)); };
-        std::vector<int> pu_cnt(257, 0);
+        // Side-data keys, band-major (bands of 1 << kTxBandShift luma rows):
+        // CUs by size, PUs by shape (w, h descending), luma TUs by size.
+        const int n_sbands = (H + (1 << rv::kTxBandShift) - 1) >> rv::kTxBandShift;
+        auto sband = [](int y) { return (size_t)(y >> rv::kTxBandShift); };
+        auto pu_bucket = [&](const rv_hevc_pu& u) {
+            return sband(u.y) * 256 + (size_t)((16 - u.w / 4) * 16 + (16 - u.h / 4));
+        };
+        auto cu_bucket = [&](const rv_hevc_cu& c) { return sband(c.y) * 4 + (size_t)(6 - c.log2_size); };
+        auto ltu_bucket = [&](const rv_hevc_tu& t) { return sband(t.y) * 4 + (size_t)(5 - t.log2); };
+        std::vector<int> pu_cnt((size_t)n_sbands * 256 + 1, 0);
         // MC blocks of a PU: the 16-aligned core in 16x16 tiles, the right
         // and bottom strips in blocks of at most 8x8. A block of an 8-bit
         // picture whose windows (luma 8-tap, chroma 4-tap, every list) lie
@@ -1154,17 +1162,18 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         for (size_t k = 0; k < s_cnt.size(); ++k) s_wave0[k + 1] = s_wave0[k] + (int32_t)((s_cnt[k] + 63) / 64);
         const int32_t n_swaves = s_wave0.back();
         const size_t n_strips = (size_t)n_swaves * 64;
-        int cu_cnt[5] = {0, 0, 0, 0, 0}, n_intra = 0;
+        std::vector<int> cu_cnt((size_t)n_sbands * 4 + 1, 0);
+        int n_intra = 0;
         int64_t intra_area = 0;
         for (int i = 0; i < p->n_cus; ++i) {
             const rv_hevc_cu& c = p->cus[i];
-            ++cu_cnt[(6 - c.log2_size) + 1];
+            ++cu_cnt[cu_bucket(c) + 1];
             if (c.intra == RV_INTRA_HOST) {
                 ++n_intra;
                 intra_area += (int64_t)1 << (2 * c.log2_size);
             }
         }
-        int ltu_cnt[5] = {0, 0, 0, 0, 0};
+        std::vector<int> ltu_cnt((size_t)n_sbands * 4 + 1, 0);
         size_t n_ltus = 0, n_ctus = 0, n_nzs = 0;
         int64_t coded_area = 0;
         // Stage-1 segments: (band of 1 << kTxBandShift luma rows, size
@@ -1178,7 +1187,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             const rv_hevc_tu& t = p->tus[i];
             if (t.comp == 0) {
                 ++n_ltus;
-                ++ltu_cnt[(5 - t.log2) + 1];
+                ++ltu_cnt[ltu_bucket(t) + 1];
             }
             if ((t.flags & RV_TU_CODED) && !is_itu(i)) {
                 ++seg_cnt[tx_seg(t)];
@@ -1194,24 +1203,29 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
             const int tpw = rv::tus_per_item((int)(k & 3));
             n_txw += (size_t)((seg_cnt[k] + tpw - 1) / tpw);
         }
-        for (int k = 0; k < 256; ++k) pu_cnt[(size_t)k + 1] += pu_cnt[(size_t)k];
-        for (int k = 0; k < 4; ++k) cu_cnt[k + 1] += cu_cnt[k];
-        for (int k = 0; k < 4; ++k) ltu_cnt[k + 1] += ltu_cnt[k];
-        // Side-data classes over the sorted lists: CUs (64 .. 8), PUs by
-        // shape, luma TUs (32 .. 4); empty classes are left out.
-        rv::SideClass sclass[rv::kMaxSideClasses];
-        int n_sclass = 0, n_side = 0;
+        for (size_t k = 0; k + 1 < pu_cnt.size(); ++k) pu_cnt[k + 1] += pu_cnt[k];
+        for (size_t k = 0; k + 1 < cu_cnt.size(); ++k) cu_cnt[k + 1] += cu_cnt[k];
+        for (size_t k = 0; k + 1 < ltu_cnt.size(); ++k) ltu_cnt[k + 1] += ltu_cnt[k];
+        // Side-data classes over the sorted lists, band by band: CUs (64 ..
+        // 8), PUs by shape, luma TUs (32 .. 4); empty classes are left out.
+        std::vector<rv::SideClass> sclass;
+        int n_side = 0;
         auto add_class = [&](int kind, int first, int count, int cw, int ch) {
             if (count <= 0) return;
-            sclass[n_sclass++] = rv::SideClass{kind, first, count, cw, ch, n_side};
+            sclass.push_back(rv::SideClass{kind, first, count, cw, ch, n_side});
             n_side += (int)(((int64_t)count * cw * ch + rv::kSideCellsPerItem - 1) / rv::kSideCellsPerItem);
         };
-        for (int k = 0; k < 4; ++k)
-            add_class(rv::SIDE_CU, cu_cnt[k], cu_cnt[k + 1] - cu_cnt[k], 16 >> k, 16 >> k);
-        for (int b = 0; b < 256; ++b)
-            add_class(rv::SIDE_PU, pu_cnt[(size_t)b], pu_cnt[(size_t)b + 1] - pu_cnt[(size_t)b], 16 - b / 16, 16 - b % 16);
-        for (int k = 0; k < 4; ++k)
-            add_class(rv::SIDE_TU, ltu_cnt[k], ltu_cnt[k + 1] - ltu_cnt[k], 8 >> k, 8 >> k);
+        for (int b = 0; b < n_sbands; ++b) {
+            const size_t c0 = (size_t)b * 4, p0 = (size_t)b * 256;
+            for (int k = 0; k < 4; ++k)
+                add_class(rv::SIDE_CU, cu_cnt[c0 + k], cu_cnt[c0 + k + 1] - cu_cnt[c0 + k], 16 >> k, 16 >> k);
+            for (int u = 0; u < 256; ++u)
+                add_class(rv::SIDE_PU, pu_cnt[p0 + u], pu_cnt[p0 + u + 1] - pu_cnt[p0 + u], 16 - u / 16, 16 - u % 16);
+            for (int k = 0; k < 4; ++k)
+                add_class(rv::SIDE_TU, ltu_cnt[c0 + k], ltu_cnt[c0 + k + 1] - ltu_cnt[c0 + k], 8 >> k, 8 >> k);
+        }
+        const int n_sclass = (int)sclass.size();
+        if (n_sclass > 65535) return fail(RV_E_RANGE, "too many side-data classes");
         const int ctb = 1 << p->log2_ctb, wc = (W + ctb - 1) / ctb, hc = (H + ctb - 1) / ctb;
         const size_t n_sao = (size_t)wc * hc * 3;
 
@@ -1232,7 +1246,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<rv_hevc_tu>(n_ltus) + carve_size<rv_hevc_tu>(n_ctus) + carve_size<uint32_t>(n_txw) +
                     carve_size<rv::McTile>(n_tiles_pad) + carve_size<rv::McTile>(n_tiles16) +
                     carve_size<rv::McStrip>(n_strips) + carve_size<uint8_t>(n_swaves) + carve_size<int32_t>(2) +
-                    carve_size<rv::SideClass>(n_sclass) +
+                    carve_size<rv::SideClass>(n_sclass) + carve_size<uint16_t>(n_side) +
                     carve_size<rv_nz>(n_nzs + n_inz) + carve_size<rv::IntraTu>(n_itus) +
                     carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)(n_ilevels + 1)) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
@@ -1263,7 +1277,12 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         uint8_t* h_swg = carve<uint8_t>(hp, n_swaves);
         int32_t* h_spre = carve<int32_t>(hp, 2);
         rv::SideClass* h_sclass = carve<rv::SideClass>(hp, n_sclass);
-        if (n_sclass) std::memcpy(h_sclass, sclass, n_sclass * sizeof(rv::SideClass));
+        if (n_sclass) std::memcpy(h_sclass, sclass.data(), n_sclass * sizeof(rv::SideClass));
+        uint16_t* h_sidx = carve<uint16_t>(hp, n_side);
+        for (int k = 0; k < n_sclass; ++k) {
+            const int end = k + 1 < n_sclass ? sclass[(size_t)k + 1].item0 : n_side;
+            for (int i = sclass[(size_t)k].item0; i < end; ++i) h_sidx[i] = (uint16_t)k;
+        }
         rv_nz* h_coefs = carve<rv_nz>(hp, n_nzs + n_inz);
         rv::IntraTu* h_itus = carve<rv::IntraTu>(hp, n_itus);
         int32_t* h_ilevel = carve<int32_t>(hp, n_ilevels + 1);
@@ -1335,7 +1354,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         }
         for (int i = 0, ni = 0; i < p->n_cus; ++i) {
             const rv_hevc_cu& c = p->cus[i];
-            const int at = cu_cnt[6 - c.log2_size]++;
+            const int at = cu_cnt[cu_bucket(c)]++;
             h_cus[at] = c;
             if (c.intra == RV_INTRA_HOST) h_intra[ni++] = at;
         }
@@ -1391,7 +1410,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         }
         for (int i = 0; i < p->n_tus; ++i) {
             const rv_hevc_tu& t = p->tus[i];
-            if (t.comp == 0) h_ltus[ltu_cnt[5 - t.log2]++] = t;
+            if (t.comp == 0) h_ltus[ltu_cnt[ltu_bucket(t)]++] = t;
             if ((t.flags & RV_TU_CODED) && !is_itu(i)) h_ctus[seg_at[tx_seg(t)]++] = t;
         }
         // Coefficients in ctus order, so a wave's TUs read one run.
@@ -1452,6 +1471,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.n_side = n_side;
         d.n_sclass = n_sclass;
         d.sclass = (const rv::SideClass*)dev((char*)h_sclass);
+        d.sidx = (const uint16_t*)dev((char*)h_sidx);
         d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
         d.n_txw = (int32_t)n_txw;

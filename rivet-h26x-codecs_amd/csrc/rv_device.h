@@ -46,7 +46,8 @@ constexpr int kStripGroups = 12;
 // Coded TUs per stage-1 work item (one wave) of each size class (4, 8,
 // 16, 32): 64 / TPW lanes per TU.
 __host__ __device__ constexpr int tus_per_item(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : cls == 2 ? 16 : 4; }
-// Bands of the stage-1 item order: 1 << kTxBandShift luma rows.
+// Bands of the stage-1 item order and of the side-data classes: 1 <<
+// kTxBandShift luma rows.
 constexpr int kTxBandShift = 7;
 
 // One transform block of an RV_INTRA_DEVICE CU: the TU (coef_off rebased
@@ -74,8 +75,8 @@ struct SideClass {
     int32_t item0;   // the class's first work item
 };
 constexpr int kSideCellsPerItem = 256;
-// Classes: CU sizes 64..8, PU shapes (w, h in 4..64), luma TU sizes 32..4.
-constexpr int kMaxSideClasses = 4 + 256 + 4;
+// Classes, per band of 1 << kTxBandShift luma rows: CU sizes 64..8, PU
+// shapes (w, h in 4..64), luma TU sizes 32..4 (the empty ones left out).
 
 // Per-picture descriptor of an HEVC job, resident in device memory.
 struct HevcPicDev {
@@ -94,8 +95,12 @@ struct HevcPicDev {
     int32_t n_cus, n_pus, n_ltus, n_tiles;  // n_tiles: small blocks, a multiple of 4
     int32_t n_tiles16;                      // 16x16 tiles
     int32_t n_side, n_intra;  // stage-0 side-data items, intra CUs
-    int32_t n_sclass;         // side-data classes
+    // Side-data classes band by band (the cells of one band are written
+    // close in time, so their partial lines meet in L2); sidx[i] is the
+    // class of side item i.
+    int32_t n_sclass;
     const SideClass* sclass;
+    const uint16_t* sidx;
     const int32_t* intra_idx; // indices of the intra CUs in cus
     // Stage-1 transform items: item i takes ctus[first .. first + count)
     // of one size class cls (4, 8, 16, 32; 64 / N TUs per item), packed as
