@@ -916,13 +916,13 @@ __device__ __forceinline__ void residual8_vals(const int (&c)[64], int16_t (&r)[
 // LDS of one macroblock: residual values (luma 16x16, Cb, Cr 8x8, raster);
 // the luma tile y[1 + r][1 + c] = sample (c, r) of the macroblock for
 // r, c in -1..15, row 0 extending to c = 23 (the samples above-right);
-// the chroma tiles likewise (c in -1..7); the filtered edge of the 8x8
-// block being predicted (t[0..15], l[0..7], corner); the luma modes.
+// the chroma tiles likewise (c in -1..7); the references of the (one or
+// two) blocks being predicted (t[0..15], l[0..7], corner); the luma modes.
 struct alignas(16) H264IntraLds {
     int16_t res[384];
     uint8_t y[17][32];
     uint8_t c[2][9][16];
-    int32_t edge[25];
+    int32_t edge[2][32];
     uint8_t modes[16];
 };
 
@@ -1118,68 +1118,84 @@ __global__ void __launch_bounds__(64) h264_intra_level(BatchDev b, int level) {
         for (int k = 0; k < 4; ++k) w |= (uint32_t)clampi(v[k] + L.res[y * 16 + x0 + k], 0, 255) << (8 * k);
         *(uint32_t*)(Yp + (size_t)(my + y) * py + mx + x0) = w;
     } else {
-        const int n = kind == RV_H264_I8X8 ? 8 : 4, nb_blocks = kind == RV_H264_I8X8 ? 4 : 16;
-        for (int k = 0; k < nb_blocks; ++k) {
-            int bx, by;
-            if (n == 8) {
-                bx = (k & 1) * 2;
-                by = (k >> 1) * 2;
-            } else {
-                bx = ((k >> 2) & 1) * 2 + (k & 1);
-                by = (k >> 3) * 2 + ((k >> 1) & 1);
-            }
-            const IAv av = block_av(nb, bx, by, n);
-            const int r0 = by * 4, c0 = bx * 4;  // tile row / column of the row above / column left
-            // Reference samples into LDS (edge: t[0..2n), l[0..n) from 16,
-            // c at 24): unfiltered for 4x4, filtered (8.3.2.2.1) for 8x8, by
-            // lanes 0..24.
-            if (n == 8) {
-                if (tid < 25) {
-                    auto top = [&](int i) {  // p[i, -1] with the substitution
-                        return (int)L.y[r0][1 + c0 + (i < 8 || av.tr ? i : 7)];
-                    };
-                    const int cr = L.y[r0][c0];
-                    int o = 0;
-                    if (tid < 16) {
-                        const int i = tid;
-                        if (av.top) {
-                            if (i == 0) o = av.tl ? (cr + 2 * top(0) + top(1) + 2) >> 2 : (3 * top(0) + top(1) + 2) >> 2;
-                            else if (i == 15) o = (top(14) + 3 * top(15) + 2) >> 2;
-                            else o = (top(i - 1) + 2 * top(i) + top(i + 1) + 2) >> 2;
-                        }
-                    } else if (tid < 24) {
-                        const int j = tid - 16;
-                        auto lf = [&](int q) { return (int)L.y[r0 + 1 + q][c0]; };
-                        if (av.left) {
-                            if (j == 0) o = av.tl ? (cr + 2 * lf(0) + lf(1) + 2) >> 2 : (3 * lf(0) + lf(1) + 2) >> 2;
-                            else if (j == 7) o = (lf(6) + 3 * lf(7) + 2) >> 2;
-                            else o = (lf(j - 1) + 2 * lf(j) + lf(j + 1) + 2) >> 2;
-                        }
-                    } else if (av.tl) {
-                        const int t0 = top(0), l0 = L.y[r0 + 1][c0];
-                        o = av.top && av.left ? (t0 + 2 * cr + l0 + 2) >> 2
-                            : av.top          ? (3 * cr + t0 + 2) >> 2
-                            : av.left         ? (3 * cr + l0 + 2) >> 2
-                                              : cr;
-                    }
-                    L.edge[tid] = o;
+        // Blocks by step: Intra_8x8 one block per step in decoding order;
+        // Intra_4x4 block (bx, by) at step bx + 2 by, which follows every
+        // block it reads (left, above, above-left, above-right), so the one
+        // or two blocks of a step are predicted together (lane groups g).
+        const bool n8 = kind == RV_H264_I8X8;
+        const int n = n8 ? 8 : 4, steps = n8 ? 4 : 10;
+        for (int st = 0; st < steps; ++st) {
+            auto block_of = [&](int g, int& bx, int& by) {
+                if (n8) {
+                    bx = (st & 1) * 2;
+                    by = (st >> 1) * 2;
+                    return g == 0;
                 }
-            } else if (tid < 25) {
-                // 4x4: t[0..8) (the top-right substituted), l[0..4), c.
-                int o = 0;
-                if (tid < 8) o = av.top ? (int)L.y[r0][1 + c0 + (tid < 4 || av.tr ? tid : 3)] : 0;
-                else if (tid >= 16 && tid < 20) o = av.left ? (int)L.y[r0 + 1 + tid - 16][c0] : 0;
-                else if (tid == 24) o = av.tl ? (int)L.y[r0][c0] : 0;
-                L.edge[tid] = o;
+                by = (st >= 2 ? (st - 2) / 2 : 0) + g;
+                bx = st - 2 * by;
+                return by <= min(3, st >> 1);
+            };
+            // Reference samples of group g's block into L.edge[g] (t[0..2n),
+            // l[0..n) from 16, c at 24) by lanes 32 g .. 32 g + 24:
+            // unfiltered for 4x4, filtered (8.3.2.2.1) for 8x8.
+            {
+                const int g = tid >> 5, e = tid & 31;
+                int bx, by;
+                if (e < 25 && block_of(g, bx, by)) {
+                    const IAv av = block_av(nb, bx, by, n);
+                    const int r0 = by * 4, c0 = bx * 4;  // tile row above / column left
+                    int o = 0;
+                    if (n8) {
+                        auto top = [&](int i) {  // p[i, -1] with the substitution
+                            return (int)L.y[r0][1 + c0 + (i < 8 || av.tr ? i : 7)];
+                        };
+                        const int cr = L.y[r0][c0];
+                        if (e < 16) {
+                            const int i = e;
+                            if (av.top) {
+                                if (i == 0) o = av.tl ? (cr + 2 * top(0) + top(1) + 2) >> 2 : (3 * top(0) + top(1) + 2) >> 2;
+                                else if (i == 15) o = (top(14) + 3 * top(15) + 2) >> 2;
+                                else o = (top(i - 1) + 2 * top(i) + top(i + 1) + 2) >> 2;
+                            }
+                        } else if (e < 24) {
+                            const int j = e - 16;
+                            auto lf = [&](int q) { return (int)L.y[r0 + 1 + q][c0]; };
+                            if (av.left) {
+                                if (j == 0) o = av.tl ? (cr + 2 * lf(0) + lf(1) + 2) >> 2 : (3 * lf(0) + lf(1) + 2) >> 2;
+                                else if (j == 7) o = (lf(6) + 3 * lf(7) + 2) >> 2;
+                                else o = (lf(j - 1) + 2 * lf(j) + lf(j + 1) + 2) >> 2;
+                            }
+                        } else if (av.tl) {
+                            const int t0 = top(0), l0 = L.y[r0 + 1][c0];
+                            o = av.top && av.left ? (t0 + 2 * cr + l0 + 2) >> 2
+                                : av.top          ? (3 * cr + t0 + 2) >> 2
+                                : av.left         ? (3 * cr + l0 + 2) >> 2
+                                                  : cr;
+                        }
+                    } else {
+                        // 4x4: t[0..8) (the top-right substituted), l[0..4), c.
+                        if (e < 8) o = av.top ? (int)L.y[r0][1 + c0 + (e < 4 || av.tr ? e : 3)] : 0;
+                        else if (e >= 16 && e < 20) o = av.left ? (int)L.y[r0 + 1 + e - 16][c0] : 0;
+                        else if (e == 24) o = av.tl ? (int)L.y[r0][c0] : 0;
+                    }
+                    L.edge[g][e] = o;
+                }
             }
             lds_sync();
-            const int mode = L.modes[by * 4 + bx];
-            if (tid < n * n) {
-                const int x = tid % n, y = tid / n;
-                // The lanes read the block's references from LDS (indexed by
-                // position: registers would go to scratch).
-                const int v = nxn_sample(L.edge, L.edge + 16, L.edge[24], n, mode, x, y, av);
-                L.y[r0 + 1 + y][c0 + 1 + x] = (uint8_t)clampi(v + L.res[(r0 + y) * 16 + c0 + x], 0, 255);
+            // Samples: lanes 16 g .. 16 g + 15 (4x4), all 64 (8x8). The lanes
+            // read the block's references from LDS (indexed by position:
+            // registers would go to scratch).
+            {
+                const int g = n8 ? 0 : tid >> 4, e = n8 ? tid : tid & 15;
+                int bx, by;
+                if ((n8 || tid < 32) && block_of(g, bx, by)) {
+                    const IAv av = block_av(nb, bx, by, n);
+                    const int r0 = by * 4, c0 = bx * 4;
+                    const int mode = L.modes[by * 4 + bx];
+                    const int x = e % n, y = e / n;
+                    const int v = nxn_sample(L.edge[g], L.edge[g] + 16, L.edge[g][24], n, mode, x, y, av);
+                    L.y[r0 + 1 + y][c0 + 1 + x] = (uint8_t)clampi(v + L.res[(r0 + y) * 16 + c0 + x], 0, 255);
+                }
             }
             lds_sync();
         }
