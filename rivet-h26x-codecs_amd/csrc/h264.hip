@@ -1443,11 +1443,10 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 }
 
 // Stage 3 cuts each picture into bands of kH264DbkRows (16) macroblock
-// rows, one workgroup per band: a slot per row, slot s filtering row
-// kH264DbkRows * band + s with 16 luma lanes (the first half of the
-// workgroup's waves) and 16 chroma lanes (the second half); luma and
-// chroma filtering are independent, so each lane's chain of edges per
-// step is one plane's. Each row takes one macroblock per step, with one
+// rows, two workgroups per band, one for luma and one for chroma (the two
+// are independent, and apart their waves each have a SIMD to themselves):
+// a slot per row, slot s filtering row kH264DbkRows * band + s with 16
+// lanes, so each lane's chain of edges per step is one plane's. Each row takes one macroblock per step, with one
 // barrier per step, and each step filters the vertical edges before the
 // horizontal ones. MB (x, y) needs MB (x - 1, y) complete and, for its top
 // edge only, MB (x + 1, y - 1)'s vertical edges (the left edge of that MB
@@ -1455,7 +1454,8 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 // step after row s - 1 when both are in one wave (the wave's program order
 // puts the row above's vertical edges first) and two steps after it across
 // waves (behind the barrier): dbk_lag(). Bands take their index from a
-// per-picture ticket, so a band's predecessor is always running. The first
+// per-picture ticket of their plane kind, so a band's predecessor of the
+// same kind is always running. The first
 // row of a band takes the strips of the row above from the band before
 // through the exchange below.
 //
@@ -1490,7 +1490,7 @@ __device__ void line_filter(uint8_t* d, long across, bool bs4, int tc0, int alph
 // line in registers.
 
 constexpr int kDbkSlots = kH264DbkRows;  // rows per band
-constexpr int kDbkThreads = 32 * kDbkSlots;  // 16 luma and 16 chroma lanes per row
+constexpr int kDbkThreads = 16 * kDbkSlots;  // 16 lanes per row (luma, or Cb and Cr)
 constexpr int kDbkTile = 96;   // words: luma 16 x 16 bytes, then Cb and Cr 8 x 8
 constexpr int kDbkRing = 24;   // words per entry: 16 luma columns, 16 chroma (16-bit)
 constexpr int kDbkRingN = 3;   // entries per slot
@@ -1678,14 +1678,17 @@ __global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
     const int p = find_pic(pre, b.n, (int)blockIdx.x);
     const H264PicDev& P = *b.h264[p];
     if (!P.deblock) return;
-    const gptr<uint64_t> xch = to_global(P.xch);  // [0] ticket, then the boundaries
+    // [0] luma ticket, [1] chroma ticket, then the boundaries. A picture's
+    // workgroups alternate luma, chroma.
+    const gptr<uint64_t> xch = to_global(P.xch);
+    const bool lu = ((blockIdx.x - pre[p]) & 1) == 0;
     if (threadIdx.x == 0)
-        band_s = (int)__hip_atomic_fetch_add((uint64_t*)xch, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        band_s = (int)__hip_atomic_fetch_add((uint64_t*)xch + (lu ? 0 : 1), (uint64_t)1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int band = band_s;
     if (band >= h264_deblock_bands(P.mbh)) return;  // only with a stale ticket
-    const bool lu = (int)threadIdx.x < kDbkThreads / 2;  // luma waves first, then chroma
-    const int t = (int)threadIdx.x & (kDbkThreads / 2 - 1);
+    const int t = (int)threadIdx.x;
     const int slot = t >> 4, r = t & 15;
     const int pl = r >> 3, rr = r & 7;  // chroma lanes: plane, row / column
     const DbkPic D{to_global((uint8_t*)P.cur[0]), to_global((uint8_t*)P.cur[1 + pl]), (uint32_t)P.pitch[0],
@@ -1699,9 +1702,9 @@ __global__ void __launch_bounds__(kDbkThreads) h264_stage3(BatchDev b) {
     const bool last = slot == rows - 1 && y0 + rows < mbh && (lu || r < 8);
     // Exchange words of this lane: imported (boundary band - 1) and
     // published (boundary band), macroblock x at + 24 x.
-    const gptr<const uint64_t> imp = xch + 1 + (size_t)max(band - 1, 0) * mbw * kDbkRing + (lu ? r : 16 + (r >> 1));
+    const gptr<const uint64_t> imp = xch + 2 + (size_t)max(band - 1, 0) * mbw * kDbkRing + (lu ? r : 16 + (r >> 1));
     const int pw = lu ? r : 16 + r;  // ring word this lane publishes
-    const gptr<uint64_t> pub = xch + 1 + (size_t)band * mbw * kDbkRing + pw;
+    const gptr<uint64_t> pub = xch + 2 + (size_t)band * mbw * kDbkRing + pw;
     uint32_t* const tile = dl + slot * kDbkPer;
     uint8_t* const tb = (uint8_t*)tile;
     uint32_t* const ring = tile + kDbkTile;
@@ -1947,12 +1950,12 @@ void h264_stage_items(H264PicDev& P) {
     P.items[0] = P.n_intra + P.n_big + (P.n_parts - P.n_big + 3) / 4;
     P.items[1] = (P.n_blks4 + 63) / 64 + (P.n_blks8 + 63) / 64;
     P.items[2] = (P.n_mbs * 8 + 63) / 64;
-    P.items[3] = P.deblock ? h264_deblock_bands(P.mbh) : 0;  // one workgroup per band
+    P.items[3] = P.deblock ? 2 * h264_deblock_bands(P.mbh) : 0;  // a luma and a chroma workgroup per band
     P.items[4] = 0;
 }
 
 size_t h264_deblock_xch_words(int mbw, int mbh) {
-    return 1 + (size_t)(h264_deblock_bands(mbh) - 1) * mbw * kDbkRing;
+    return 2 + (size_t)(h264_deblock_bands(mbh) - 1) * mbw * kDbkRing;
 }
 
 hipError_t h264_launch_stage(int stage, const BatchDev& b, int total, hipStream_t st) {
