@@ -45,6 +45,17 @@ typedef struct rv_gen_hevc_cfg {
                                with seeded modes; 8x8 CUs split into 4x4
                                luma TUs are PART_NxN                       */
     int32_t strong_intra_smoothing, constrained_intra_pred;  /* copied */
+    /* Slices and tiles: tile_cols x tile_rows uniformly spaced tiles (0 or
+     * 1: one), n_slices slices of about equal CTB counts in tile-scan
+     * order (0 or 1: one slice, no per-CTB tables). lf_across_slices /
+     * lf_across_tiles: the flags (-1: each slice's flag seeded). slice_vary:
+     * 1 seeds each slice's deblocking_disabled (1 in 5) and offsets
+     * (-6..6, even). exempt_percent: % of CUs with loop-filter exemptions
+     * (RV_CU_LF_EXEMPT, with RV_CU_BYPASS half the time, or
+     * RV_CU_DBK_EXEMPT). None of them changes the other seeded streams. */
+    int32_t tile_cols, tile_rows, n_slices;
+    int32_t lf_across_slices, lf_across_tiles, slice_vary;
+    int32_t exempt_percent;
 } rv_gen_hevc_cfg;
 
 typedef struct rv_gen_h264_cfg {

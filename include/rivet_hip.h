@@ -101,8 +101,20 @@ typedef struct rv_hevc_cu {
     uint8_t chroma_mode;  /* RV_INTRA_DEVICE: IntraPredModeC (0..34), the
                            derived mode (intra_chroma_pred_mode 4 and the
                            mode-34 substitution resolved)              */
-    uint8_t pad[3];
+    uint8_t exempt;     /* loop-filter exemptions of the CU's samples, as
+                           PicInfo::filter_exempt (src/hevc/pic.rs:77-80,
+                           set at src/hevc/ctu.rs:763-766): RV_CU_LF_EXEMPT
+                           | RV_CU_BYPASS | RV_CU_DBK_EXEMPT, or 0      */
+    uint8_t pad[2];
 } rv_hevc_cu;
+
+/* rv_hevc_cu.exempt: deblocking and SAO leave the samples alone (PCM with
+ * pcm_loop_filter_disabled_flag, or cu_transquant_bypass_flag). */
+#define RV_CU_LF_EXEMPT 1u
+/* cu_transquant_bypass_flag (recorded with RV_CU_LF_EXEMPT). */
+#define RV_CU_BYPASS 2u
+/* Deblocking only leaves the samples alone (palette_mode_flag). */
+#define RV_CU_DBK_EXEMPT 4u
 
 /* Intra CU whose prediction the host supplies in intra_pred (copied in
  * stage 0). */
@@ -180,6 +192,15 @@ typedef struct rv_hevc_scaling {
     uint8_t dc[2][6];         /* sizeId 2..3: scaling_list_dc_coef     */
 } rv_hevc_scaling;
 
+/* Loop-filter parameters of one slice (SliceFilterParams,
+ * src/hevc/pic.rs:19-35; the chroma QP offsets are the PPS's, in
+ * rv_hevc_pic). */
+typedef struct rv_hevc_slice {
+    int32_t deblocking_disabled;        /* slice_deblocking_filter_disabled_flag */
+    int32_t beta_offset, tc_offset;     /* slice_*_offset_div2 * 2                */
+    int32_t loop_filter_across_slices;  /* slice_loop_filter_across_slices_enabled_flag */
+} rv_hevc_slice;
+
 /* One HEVC P/B picture to reconstruct. Planes are the session's DPB
  * slots; ref_slot names them per list and reference index. */
 typedef struct rv_hevc_pic {
@@ -218,6 +239,24 @@ typedef struct rv_hevc_pic {
     /* Intra prediction on the device (RV_INTRA_DEVICE CUs). */
     int32_t strong_intra_smoothing;  /* strong_intra_smoothing_enabled_flag */
     int32_t constrained_intra_pred;  /* constrained_intra_pred_flag         */
+    /* Slices and tiles (PicInfo::ctb_slice / ctb_slice_addr / slices and
+     * Geometry::ctb_tile / ctb_rs_to_ts, src/hevc/pic.rs:41-96). NULL
+     * ctb_slice: the picture is one slice and one tile, filtered with
+     * deblock, beta_offset and tc_offset above. Otherwise every CTB
+     * (raster order) names its slice in slices[], its SliceAddrRs, its
+     * tile and its tile-scan address; a slice's deblocking_disabled and
+     * offsets apply to the edges whose q block lies in its CTBs, and
+     * deblocking and SAO cross slice and tile boundaries as
+     * src/hevc/deblock.rs:120-165 and src/hevc/sao.rs:193-238 allow
+     * (deblock = 0 still turns deblocking off for the whole picture; a
+     * slice's SAO flags are its CTBs' rv_sao.type_idx). */
+    int32_t n_slices;
+    const rv_hevc_slice* slices;
+    const uint16_t* ctb_slice;
+    const uint32_t* ctb_slice_addr;
+    const uint16_t* ctb_tile;
+    const uint32_t* ctb_ts;
+    int32_t loop_filter_across_tiles;  /* loop_filter_across_tiles_enabled_flag */
 } rv_hevc_pic;
 
 /* ------------------------------------------------------------------ */

@@ -468,7 +468,18 @@ struct PicInfo {
     std::vector<uint8_t> pred_mode, cbf_luma, edges, filter_exempt;
     std::vector<int8_t> qp_y;
     std::vector<MotionInfo> motion;
+    // Slices and tiles (PicInfo::ctb_slice / ctb_slice_addr / slices,
+    // Geometry::ctb_tile / ctb_rs_to_ts, src/hevc/pic.rs:41-96); empty:
+    // one slice (`one`) and one tile.
+    std::vector<uint16_t> ctb_slice, ctb_tile;
+    std::vector<uint32_t> ctb_slice_addr, ctb_ts;
+    std::vector<rv_hevc_slice> slices;
+    rv_hevc_slice one{};
+    bool lf_across_tiles = true;
     int idx4(int x, int y) const { return (y >> 2) * w4 + (x >> 2); }
+    int ctb_of(int x, int y) const { return (y >> log2_ctb) * wc + (x >> log2_ctb); }
+    bool multi() const { return !ctb_slice.empty(); }
+    const rv_hevc_slice& slice_of_ctb(int c) const { return multi() ? slices[ctb_slice[c]] : one; }
 };
 
 // PicInfo::fill4 (src/hevc/pic.rs:374-406).
@@ -648,13 +659,26 @@ int motion_bs(const MotionInfo& a, const MotionInfo& b) {
     return c1 && c2;
 }
 
-// boundary_strengths (src/hevc/deblock.rs:62-180) over the whole picture,
-// with every CTB in one slice and one tile (the neighbour checks of
-// :125-137 and :153-165 always pass).
+// Whether the edge between CTB cq (holding the q block) and CTB cp may be
+// filtered (src/hevc/deblock.rs:125-137, 153-165): across a tile boundary
+// only with loop_filter_across_tiles, across a slice boundary only with
+// the q block's slice's loop_filter_across_slices.
+bool edge_crosses_ok(const PicInfo& info, int cq, int cp) {
+    if (!info.multi() || cq == cp) return true;
+    if (info.ctb_tile[cq] != info.ctb_tile[cp] && !info.lf_across_tiles) return false;
+    if (info.ctb_slice_addr[cq] != info.ctb_slice_addr[cp] && !info.slice_of_ctb(cq).loop_filter_across_slices)
+        return false;
+    return true;
+}
+
+// boundary_strengths (src/hevc/deblock.rs:62-180) over the whole picture:
+// edges whose q block lies in a slice with deblocking disabled carry none,
+// and edges on CTB boundaries cross slices and tiles as edge_crosses_ok.
 void boundary_strengths(const PicInfo& info, std::vector<uint8_t>& ver, std::vector<uint8_t>& hor) {
     int w4 = info.w4;
     ver.assign((size_t)w4 * info.h4, 0);
     hor.assign((size_t)w4 * info.h4, 0);
+    const int ctb_mask = (1 << info.log2_ctb) - 1;
     for (int by = 0; by < info.h4; ++by) {
         int y = by * 4;
         bool hor_row = y > 0 && y % 8 == 0;
@@ -665,6 +689,10 @@ void boundary_strengths(const PicInfo& info, std::vector<uint8_t>& ver, std::vec
             bool want_v = x > 0 && x % 8 == 0 && (edges & 3) != 0;
             bool want_h = hor_row && (edges & 12) != 0;
             if (!want_v && !want_h) continue;
+            const int cq = info.ctb_of(x, y);
+            if (info.slice_of_ctb(cq).deblocking_disabled) continue;
+            if (want_v && (x & ctb_mask) == 0 && !edge_crosses_ok(info, cq, cq - 1)) want_v = false;
+            if (want_h && (y & ctb_mask) == 0 && !edge_crosses_ok(info, cq, cq - info.wc)) want_h = false;
             bool intra_q = info.pred_mode[q] == 1;
             if (want_v) {
                 int p = q - 1;
@@ -690,25 +718,39 @@ struct SliceParams {
 
 // deblock_rows (src/hevc/deblock.rs:210-515) for rows 0..h4 (the whole
 // picture: the standard's order, which row-by-row application equals).
+// The beta / tc offsets are those of the q block's slice (SliceCache,
+// :186-201); the chroma QP offsets the PPS's (sl).
 template <typename S>
 void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, int bd, int threads) {
     std::vector<uint8_t> ver, hor;
     boundary_strengths(info, ver, hor);
     int w4 = info.w4, h4 = info.h4;
     int max = (1 << bd) - 1, sh_l = bd - 8, sh_c = bd - 8;
+    auto offsets = [&](int q, int* beta_offset, int* tc_offset) {
+        if (!info.multi()) {
+            *beta_offset = sl.beta_offset;
+            *tc_offset = sl.tc_offset;
+            return;
+        }
+        const rv_hevc_slice& s = info.slice_of_ctb(info.ctb_of((q % w4) * 4, (q / w4) * 4));
+        *beta_offset = s.beta_offset;
+        *tc_offset = s.tc_offset;
+    };
     auto luma_params = [&](int b, int p, int q, int* beta, int* tc, bool* np, bool* nq) {
-        int qp = (info.qp_y[p] + info.qp_y[q] + 1) >> 1;
-        *beta = BETA_TABLE[clampi(qp + sl.beta_offset, 0, 51)] * (1 << sh_l);
-        *tc = TC_TABLE[clampi(qp + 2 * (b - 1) + sl.tc_offset, 0, 53)] * (1 << sh_l);
+        int qp = (info.qp_y[p] + info.qp_y[q] + 1) >> 1, bo, to;
+        offsets(q, &bo, &to);
+        *beta = BETA_TABLE[clampi(qp + bo, 0, 51)] * (1 << sh_l);
+        *tc = TC_TABLE[clampi(qp + 2 * (b - 1) + to, 0, 53)] * (1 << sh_l);
         *np = (info.filter_exempt[p] & 5) != 0;
         *nq = (info.filter_exempt[q] & 5) != 0;
     };
     auto chroma_tc = [&](int p, int q, int* out) {
-        int qp_avg = (info.qp_y[p] + info.qp_y[q] + 1) >> 1;
+        int qp_avg = (info.qp_y[p] + info.qp_y[q] + 1) >> 1, bo, to;
+        offsets(q, &bo, &to);
         for (int c = 0; c < 2; ++c) {
             int qpi = qp_avg + (c == 0 ? sl.cb_qp_offset : sl.cr_qp_offset);
             int qpc = qpi < 0 ? qpi : chroma_qp420(qpi);
-            out[c] = TC_TABLE[clampi(qpc + 2 + sl.tc_offset, 0, 53)] * (1 << sh_c);
+            out[c] = TC_TABLE[clampi(qpc + 2 + to, 0, 53)] * (1 << sh_c);
         }
     };
     // Each pass over bands of four 4x4 rows (16 luma rows): a vertical edge
@@ -812,23 +854,38 @@ void deblock_picture(Frame<S>& f, const PicInfo& info, const SliceParams& sl, in
 }
 
 // ---------------------------------------------------------------------
-// SAO driver (src/hevc/sao.rs:110-431), one slice and one tile: a
-// neighbouring CTB is usable exactly when it lies inside the picture.
+// SAO driver (src/hevc/sao.rs:110-431): a neighbouring CTB is usable as
+// Neighbours::of (:193-238) says, and a CTB holding a block exempt from the
+// loop filters (filter_exempt bit 0) takes the per-sample path, which
+// leaves the exempt samples alone (:140-151, :280-283).
 // ---------------------------------------------------------------------
 template <typename S>
 void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, int y0, int w, int h, int sw,
-             int sh, int bd, const rv_sao& p, const bool nb[8]) {
+             int sh, int bd, const rv_sao& p, const bool nb[8], bool exempt_any) {
     // nb: l, r, t, b, tl, tr, bl, br.
     int max = (1 << bd) - 1;
     long stride = src.stride;
     int pic_w = dst.width, pic_h = dst.height;
     auto at = [&](int x, int y) { return src.offset(x, y); };
+    auto exempt = [&](int x, int y) {
+        return exempt_any && (info.filter_exempt[info.idx4(x * sw, y * sh)] & 1) != 0;
+    };
     if (p.type_idx == 1) {
         int shift = bd - 5;
         int16_t table[32] = {0};
         for (int k = 0; k < 4; ++k) table[(k + p.band_or_class) & 31] = p.offsets[k];
         long off = at(x0, y0);
-        sao_band_k(&dst.data[off], stride, &src.data[off], stride, w, h, table, shift, max);
+        if (!exempt_any) {
+            sao_band_k(&dst.data[off], stride, &src.data[off], stride, w, h, table, shift, max);
+            return;
+        }
+        for (int y = y0; y < y0 + h; ++y)
+            for (int x = x0; x < x0 + w; ++x) {
+                if (exempt(x, y)) continue;
+                long i = at(x, y);
+                int v = src.data[i];
+                dst.data[i] = (S)clampi(v + table[v >> shift], 0, max);
+            }
         return;
     }
     if (p.type_idx != 2) return;
@@ -845,7 +902,7 @@ void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, in
     if ((uses_y && !Tt) || (diag && (!TL || !TR))) ys += 1;
     if ((uses_y && !B) || (diag && (!BL || !BR))) ye -= 1;
     long na = vp[0] * stride + hp[0], nbb = vp[1] * stride + hp[1];
-    bool interior_ok = xs < xe && ys < ye;
+    bool interior_ok = !exempt_any && xs < xe && ys < ye;
     if (interior_ok) sao_edge_k(dst.data.data(), src.data.data(), at(xs, ys), stride, xe - xs, ye - ys, na, nbb, off_tab, max);
     auto usable = [&](int x, int y, int xn, int yn) -> bool {
         if (xn < 0 || yn < 0 || xn >= pic_w || yn >= pic_h) return false;
@@ -863,6 +920,7 @@ void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, in
         return BR;
     };
     auto sample = [&](int x, int y) {
+        if (exempt(x, y)) return;
         int xa = x + hp[0], ya = y + vp[0], xb = x + hp[1], yb = y + vp[1];
         if (!usable(x, y, xa, ya) || !usable(x, y, xb, yb)) return;
         long i = at(x, y);
@@ -885,6 +943,35 @@ void sao_ctb(const Plane<S>& src, Plane<S>& dst, const PicInfo& info, int x0, in
     }
 }
 
+// Neighbours::of (src/hevc/sao.rs:193-238): which of the eight CTBs around
+// (rx, ry) -- l, r, t, b, tl, tr, bl, br -- this CTB's SAO may read: inside
+// the picture; across a slice boundary only if the later slice in tile-scan
+// order allows filtering across its slice boundaries; across a tile
+// boundary only with loop_filter_across_tiles.
+void sao_neighbours(const PicInfo& info, int rx, int ry, bool nb[8]) {
+    static const int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    const int cur = ry * info.wc + rx;
+    for (int k = 0; k < 8; ++k) {
+        const int nx = rx + DX[k], ny = ry + DY[k];
+        if (nx < 0 || ny < 0 || nx >= info.wc || ny >= info.hc) {
+            nb[k] = false;
+            continue;
+        }
+        if (!info.multi()) {
+            nb[k] = true;
+            continue;
+        }
+        const int n = ny * info.wc + nx;
+        bool ok = true;
+        if (info.ctb_slice_addr[n] != info.ctb_slice_addr[cur]) {
+            if (info.ctb_ts[n] < info.ctb_ts[cur] && !info.slice_of_ctb(cur).loop_filter_across_slices) ok = false;
+            if (info.ctb_ts[cur] < info.ctb_ts[n] && !info.slice_of_ctb(n).loop_filter_across_slices) ok = false;
+        }
+        if (!info.lf_across_tiles && info.ctb_tile[n] != info.ctb_tile[cur]) ok = false;
+        nb[k] = ok;
+    }
+}
+
 template <typename S>
 void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv_hevc_pic* pic, int threads) {
     int ctb = 1 << info.log2_ctb;
@@ -902,17 +989,25 @@ void sao_picture(Frame<S>& f, const Frame<S>& deb, const PicInfo& info, const rv
                 if (!on) eff[c].type_idx = 0;
             }
             if (eff[0].type_idx == 0 && eff[1].type_idx == 0 && eff[2].type_idx == 0) continue;
-            auto in = [&](int dx, int dy) {
-                int nx = rx + dx, ny = ry + dy;
-                return nx >= 0 && ny >= 0 && nx < info.wc && ny < info.hc;
-            };
-            bool nb[8] = {in(-1, 0), in(1, 0), in(0, -1), in(0, 1), in(-1, -1), in(1, -1), in(-1, 1), in(1, 1)};
+            bool nb[8];
+            sao_neighbours(info, rx, ry, nb);
+            bool exempt_any = false;
+            {
+                const int x0 = rx * ctb, y0 = ry * ctb;
+                const int w = std::min(ctb, pw - x0), h = std::min(ctb, ph - y0);
+                for (int by = y0 >> 2; by < (y0 + h) >> 2 && !exempt_any; ++by)
+                    for (int bx = x0 >> 2; bx < (x0 + w) >> 2; ++bx)
+                        if (info.filter_exempt[by * info.w4 + bx] & 1) {
+                            exempt_any = true;
+                            break;
+                        }
+            }
             for (int c = 0; c < 3; ++c) {
                 if (eff[c].type_idx == 0) continue;
                 int sw = c ? 2 : 1, sh = c ? 2 : 1;
                 int x0 = rx * ctb / sw, y0 = ry * ctb / sh;
                 int w = std::min(ctb / sw, pw / sw - x0), h = std::min(ctb / sh, ph / sh - y0);
-                sao_ctb(deb.p[c], f.p[c], info, x0, y0, w, h, sw, sh, pic->bit_depth, eff[c], nb);
+                sao_ctb(deb.p[c], f.p[c], info, x0, y0, w, h, sw, sh, pic->bit_depth, eff[c], nb, exempt_any);
             }
         }
     });
@@ -1120,10 +1215,28 @@ int build_info(const rv_hevc_pic* pic, PicInfo& info) {
     MotionInfo none{};
     none.ref_idx[0] = none.ref_idx[1] = -1;
     info.motion.assign(n4, none);
+    info.one.beta_offset = pic->beta_offset;
+    info.one.tc_offset = pic->tc_offset;
+    info.one.deblocking_disabled = 0;
+    info.one.loop_filter_across_slices = 1;
+    if (pic->ctb_slice) {
+        const size_t nc = (size_t)info.wc * info.hc;
+        if (pic->n_slices <= 0 || !pic->slices || !pic->ctb_slice_addr || !pic->ctb_tile || !pic->ctb_ts)
+            return RV_E_INVALID;
+        info.slices.assign(pic->slices, pic->slices + pic->n_slices);
+        info.ctb_slice.assign(pic->ctb_slice, pic->ctb_slice + nc);
+        info.ctb_slice_addr.assign(pic->ctb_slice_addr, pic->ctb_slice_addr + nc);
+        info.ctb_tile.assign(pic->ctb_tile, pic->ctb_tile + nc);
+        info.ctb_ts.assign(pic->ctb_ts, pic->ctb_ts + nc);
+        for (size_t c = 0; c < nc; ++c)
+            if (info.ctb_slice[c] >= pic->n_slices || info.ctb_ts[c] >= nc) return RV_E_RANGE;
+        info.lf_across_tiles = pic->loop_filter_across_tiles != 0;
+    }
     for (int i = 0; i < pic->n_cus; ++i) {
         const rv_hevc_cu& cu = pic->cus[i];
         int s = 1 << cu.log2_size;
         fill4(info.qp_y, info.w4, cu.x, cu.y, s, s, (int8_t)cu.qp_y);
+        if (cu.exempt) fill4(info.filter_exempt, info.w4, cu.x, cu.y, s, s, (uint8_t)cu.exempt);
         for (int yy = cu.y; yy < cu.y + s; yy += 4) info.edges[info.idx4(cu.x, yy)] |= 1;
         for (int xx = cu.x; xx < cu.x + s; xx += 4) info.edges[info.idx4(xx, cu.y)] |= 4;
         if (cu.intra) {

@@ -7,6 +7,7 @@
 // 8x8 split into four 4x4 TUs keeps one 4x4 chroma TU per component (the
 // 4:2:0 rule). H.264: 8x8 transforms only when every partition is 8x8 or
 // larger.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -50,7 +51,7 @@ struct Lcg {
     }
 };
 
-enum Purpose : uint64_t { STRUCT = 1, COEF = 2, SAO = 3, INTRA = 4, WEIGHTS = 5, SCALING = 6 };
+enum Purpose : uint64_t { STRUCT = 1, COEF = 2, SAO = 3, INTRA = 4, WEIGHTS = 5, SCALING = 6, SLICES = 8 };
 
 // Seeded scaling factors, 4..64 each.
 void fill_factors(Lcg& r, uint8_t* p, size_t n) {
@@ -83,6 +84,9 @@ struct rv_gen {
     std::vector<int32_t> coefs32;
     rv_hevc_scaling hevc_scaling;
     rv_h264_scaling h264_scaling;
+    std::vector<rv_hevc_slice> slices;
+    std::vector<uint16_t> ctb_slice, ctb_tile;
+    std::vector<uint32_t> ctb_slice_addr, ctb_ts;
 };
 
 namespace {
@@ -161,7 +165,7 @@ struct HevcBuilder {
             for (int k = 0; k < 4; ++k) cu(x + (k & 1) * h, y + (k >> 1) * h, log2 - 1);
             return;
         }
-        rv_hevc_cu cu;
+        rv_hevc_cu cu{};
         std::memset(&cu, 0, sizeof(cu));
         cu.x = (uint16_t)x;
         cu.y = (uint16_t)y;
@@ -329,6 +333,73 @@ struct HevcBuilder {
             fill_factors(rsl, &g.hevc_scaling.dc[0][0], sizeof(g.hevc_scaling.dc));
             p.scaling = &g.hevc_scaling;
         }
+        slices_and_tiles(wc, hc, seed);
+    }
+
+    // Uniformly spaced tiles and the CTB raster-to-tile-scan map (6.5.1),
+    // slices as runs of tile-scan order, per-slice filter settings, and
+    // exempt CUs: all from their own seeded stream.
+    void slices_and_tiles(int wc, int hc, uint64_t seed) {
+        rv_hevc_pic& p = g.hevc;
+        Lcg r(rv_gen_subseed(seed, 0, 0, SLICES));
+        if (c.exempt_percent > 0)
+            for (rv_hevc_cu& cu : g.cus) {
+                if (!r.percent(c.exempt_percent)) continue;
+                const int k = r.uniform(0, 2);
+                cu.exempt = (uint8_t)(k == 0 ? RV_CU_LF_EXEMPT : k == 1 ? (RV_CU_LF_EXEMPT | RV_CU_BYPASS) : RV_CU_DBK_EXEMPT);
+            }
+        const int cols = std::max(1, c.tile_cols), rows = std::max(1, c.tile_rows);
+        const int ns = std::max(1, c.n_slices);
+        if (cols == 1 && rows == 1 && ns == 1) return;
+        std::vector<int> cb(cols + 1), rb(rows + 1);
+        for (int i = 0; i <= cols; ++i) cb[i] = i * wc / cols;
+        for (int i = 0; i <= rows; ++i) rb[i] = i * hc / rows;
+        const int n = wc * hc;
+        g.ctb_ts.assign(n, 0);
+        g.ctb_tile.assign(n, 0);
+        std::vector<uint32_t> ts_to_rs(n);
+        for (int rs = 0; rs < n; ++rs) {
+            const int tx = rs % wc, ty = rs / wc;
+            int i = 0, j = 0;
+            while (tx >= cb[i + 1]) ++i;
+            while (ty >= rb[j + 1]) ++j;
+            int ts = 0;
+            for (int jj = 0; jj < j; ++jj) ts += wc * (rb[jj + 1] - rb[jj]);
+            for (int ii = 0; ii < i; ++ii) ts += (rb[j + 1] - rb[j]) * (cb[ii + 1] - cb[ii]);
+            ts += (ty - rb[j]) * (cb[i + 1] - cb[i]) + tx - cb[i];
+            g.ctb_ts[rs] = (uint32_t)ts;
+            g.ctb_tile[rs] = (uint16_t)(j * cols + i);
+            ts_to_rs[ts] = (uint32_t)rs;
+        }
+        g.slices.assign(ns, rv_hevc_slice{0, 0, 0, 1});
+        for (int k = 0; k < ns; ++k) {
+            rv_hevc_slice& sl = g.slices[k];
+            sl.loop_filter_across_slices = c.lf_across_slices < 0 ? (int)(r.next() & 1) : c.lf_across_slices;
+            if (c.slice_vary) {
+                sl.deblocking_disabled = r.percent(20) ? 1 : 0;
+                sl.beta_offset = 2 * r.uniform(-3, 3);
+                sl.tc_offset = 2 * r.uniform(-3, 3);
+            } else {
+                sl.beta_offset = p.beta_offset;
+                sl.tc_offset = p.tc_offset;
+            }
+        }
+        g.ctb_slice.assign(n, 0);
+        g.ctb_slice_addr.assign(n, 0);
+        for (int k = 0; k < ns; ++k) {
+            const int t0 = k * n / ns, t1 = (k + 1) * n / ns;
+            for (int t = t0; t < t1; ++t) {
+                g.ctb_slice[ts_to_rs[t]] = (uint16_t)k;
+                g.ctb_slice_addr[ts_to_rs[t]] = ts_to_rs[t0];
+            }
+        }
+        p.n_slices = ns;
+        p.slices = g.slices.data();
+        p.ctb_slice = g.ctb_slice.data();
+        p.ctb_slice_addr = g.ctb_slice_addr.data();
+        p.ctb_tile = g.ctb_tile.data();
+        p.ctb_ts = g.ctb_ts.data();
+        p.loop_filter_across_tiles = c.lf_across_tiles < 0 ? (int)(r.next() & 1) : c.lf_across_tiles;
     }
 };
 

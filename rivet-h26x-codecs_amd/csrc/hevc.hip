@@ -754,13 +754,16 @@ __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
     const uint32_t* recs = kind == SIDE_CU ? (const uint32_t*)P.cus
                            : kind == SIDE_PU ? (const uint32_t*)P.pus : (const uint32_t*)P.ltus;
     const int stride = kind == SIDE_CU ? 5 : kind == SIDE_PU ? 5 : 4;  // dwords per record
-    uint32_t w0[R], w1[R];
+    uint32_t w0[R], w1[R], wx[R];
+    const bool lf = kind == SIDE_CU && P.ctbw != nullptr;  // exemptions in rv_hevc_cu dword 4
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+        wx[r] = 0;
         if (on[r]) {
             const uint32_t* rp = recs + (size_t)(first + e[r]) * stride;
             w0[r] = rp[0];
             w1[r] = kind == SIDE_PU ? 0u : rp[1];
+            if (lf) wx[r] = rp[4];
         }
     }
     const int w4 = P.w4;
@@ -773,7 +776,10 @@ __device__ __forceinline__ void side_item(PicRef P, int item, int tid) {
         if (kind == SIDE_CU) {
             const uint32_t intra = ((w1[r] >> 16) & 0xFF) ? 1u : 0u;
             P.qp[q] = (int8_t)((w1[r] >> 8) & 0xFF);
-            P.cuinfo[q] = (uint8_t)(intra | (left << 1) | (top << 2));
+            const uint32_t ex = (wx[r] >> 8) & 0xFF;  // rv_hevc_cu.exempt
+            const uint32_t exb = ((ex & RV_CU_LF_EXEMPT) ? kCuSaoExempt : 0u) |
+                                 ((ex & (RV_CU_LF_EXEMPT | RV_CU_DBK_EXEMPT)) ? kCuDbkExempt : 0u);
+            P.cuinfo[q] = (uint8_t)(intra | (left << 1) | (top << 2) | exb);
             if (intra) P.pumap[q] = 0x3FFFFFFFu | (left << 30) | (top << 31);
         } else if (kind == SIDE_PU) {
             P.pumap[q] = (uint32_t)(first + e[r]) | (left << 30) | (top << 31);
@@ -1563,6 +1569,18 @@ __device__ __forceinline__ void area_bs(PicRef P, int a) {
             bs[e] = edge_rule(vert ? has_l : has_t, tu_edge, pu_edge, ci_p[e], ci_q[e], ti_p[e], ti_q[e], pm_p[e],
                               pm_q[e]);
         }
+        if (P.ctbw) {
+            // The four q blocks lie in one CTB: its slice's deblocking
+            // switch, and the CTB boundaries it may not filter across
+            // (src/hevc/deblock.rs:105-165).
+            const uint32_t cw = P.ctbw[(size_t)((by * 4) >> P.log2_ctb) * P.wc + ((bx * 4) >> P.log2_ctb)];
+            const int m = (1 << (P.log2_ctb - 2)) - 1;
+            const bool v_off = (cw & kCtbDbkOff) || ((bx & m) == 0 && (cw & kCtbLeftOff));
+            const bool h_off = (cw & kCtbDbkOff) || ((by & m) == 0 && (cw & kCtbTopOff));
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (e < 2 ? v_off : h_off) bs[e] = 0;
+        }
         uint4 wq[4], wp[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -2044,10 +2062,30 @@ __device__ void luma_segment(int s[4][8], int beta, int tc, int max, bool& chang
 }
 
 // Luma beta / tc of one segment (src/hevc/deblock.rs:264-279).
-__device__ __forceinline__ void luma_thresholds(PicRef P, int b, int qp_p, int qp_q, int* beta, int* tc) {
+__device__ __forceinline__ void luma_thresholds(PicRef P, int b, int qp_p, int qp_q, int bo, int to, int* beta,
+                                                int* tc) {
     int qp = (qp_p + qp_q + 1) >> 1, sh = P.bd - 8;
-    *beta = (int)kBeta[clampi(qp + P.beta_offset, 0, 51)] << sh;
-    *tc = (int)kTc[clampi(qp + 2 * (b - 1) + P.tc_offset, 0, 53)] << sh;
+    *beta = (int)kBeta[clampi(qp + bo, 0, 51)] << sh;
+    *tc = (int)kTc[clampi(qp + 2 * (b - 1) + to, 0, 53)] << sh;
+}
+
+// The slice offsets of the q block at luma (x, y) and the exemptions of
+// blocks p and q (src/hevc/deblock.rs:186-201, 264-279): the picture's
+// offsets and none without per-CTB words.
+struct SegRules {
+    int bo, to;
+    bool np, nq;
+};
+__device__ __forceinline__ SegRules seg_rules(PicRef P, int x, int y, size_t p, size_t q) {
+    SegRules r{P.beta_offset, P.tc_offset, false, false};
+    if (P.ctbw) {
+        const uint32_t cw = P.ctbw[(size_t)(y >> P.log2_ctb) * P.wc + (x >> P.log2_ctb)];
+        r.bo = (int8_t)(cw >> 16);
+        r.to = (int8_t)(cw >> 24);
+        r.np = (P.cuinfo[p] & kCuDbkExempt) != 0;
+        r.nq = (P.cuinfo[q] & kCuDbkExempt) != 0;
+    }
+    return r;
 }
 
 // chroma_qp (src/hevc/ctu.rs:3072-3084), 4:2:0.
@@ -2059,10 +2097,10 @@ __device__ __forceinline__ int chroma_qp420(int qpi) {
 }
 
 // Chroma tc for Cb (c = 0) / Cr (c = 1) (src/hevc/deblock.rs:281-294).
-__device__ __forceinline__ int chroma_tc(PicRef P, int qp_p, int qp_q, int c) {
+__device__ __forceinline__ int chroma_tc(PicRef P, int qp_p, int qp_q, int c, int to) {
     int qpi = ((qp_p + qp_q + 1) >> 1) + (c == 0 ? P.cb_qp_offset : P.cr_qp_offset);
     int qpc = qpi < 0 ? qpi : chroma_qp420(qpi);
-    return (int)kTc[clampi(qpc + 2 + P.tc_offset, 0, 53)] << (P.bd - 8);
+    return (int)kTc[clampi(qpc + 2 + to, 0, 53)] << (P.bd - 8);
 }
 
 // chroma_edge_scalar (src/dsp/hevc.rs:1326-1353), two lines. d points at
@@ -2091,8 +2129,9 @@ __device__ void deblock_v_seg(PicRef P, int t) {
     if (!b) return;
     const int x = kx * 8, y = j * 4, max = (1 << P.bd) - 1;
     const int qpp = P.qp[q - 1], qpq = P.qp[q];
+    const SegRules sr = seg_rules(P, x, y, q - 1, q);
     int beta, tc;
-    luma_thresholds(P, b, qpp, qpq, &beta, &tc);
+    luma_thresholds(P, b, qpp, qpq, sr.bo, sr.to, &beta, &tc);
     if (beta != 0 || tc != 0) {
         S* Y = (S*)P.cur[0];
         const int pitch = P.pitch[0];
@@ -2110,16 +2149,17 @@ __device__ void deblock_v_seg(PicRef P, int t) {
             for (int line = 0; line < 4; ++line) {
                 S* r = Y + (size_t)(y + line) * pitch + x - 4;
 #pragma unroll
-                for (int k = 1; k < 7; ++k) r[k] = (S)s[line][k];
+                for (int k = 1; k < 7; ++k)
+                    if (k < 4 ? !sr.np : !sr.nq) r[k] = (S)s[line][k];
             }
         }
     }
     // Chroma: bS 2 on the 16-sample luma grid, two chroma lines.
     if (b == 2 && (x & 15) == 0) {
         for (int c = 1; c < 3; ++c) {
-            int tcc = chroma_tc(P, qpp, qpq, c - 1);
+            int tcc = chroma_tc(P, qpp, qpq, c - 1, sr.to);
             S* pl = (S*)P.cur[c];
-            chroma_lines(pl + (size_t)(y >> 1) * P.pitch[c] + (x >> 1), 1, P.pitch[c], tcc, max);
+            chroma_lines(pl + (size_t)(y >> 1) * P.pitch[c] + (x >> 1), 1, P.pitch[c], tcc, max, sr.np, sr.nq);
         }
     }
 }
@@ -2135,8 +2175,9 @@ __device__ void deblock_h_seg(PicRef P, int t) {
     if (!b) return;
     const int x = i * 4, max = (1 << P.bd) - 1;
     const int qpp = P.qp[q - w4], qpq = P.qp[q];
+    const SegRules sr = seg_rules(P, x, y, q - w4, q);
     int beta, tc;
-    luma_thresholds(P, b, qpp, qpq, &beta, &tc);
+    luma_thresholds(P, b, qpp, qpq, sr.bo, sr.to, &beta, &tc);
     if (beta != 0 || tc != 0) {
         S* Y = (S*)P.cur[0];
         const int pitch = P.pitch[0];
@@ -2152,6 +2193,7 @@ __device__ void deblock_h_seg(PicRef P, int t) {
         if (changed) {
 #pragma unroll
             for (int k = 1; k < 7; ++k) {
+                if (k < 4 ? sr.np : sr.nq) continue;
                 S* r = Y + (size_t)(y - 4 + k) * pitch + x;
 #pragma unroll
                 for (int line = 0; line < 4; ++line) r[line] = (S)s[line][k];
@@ -2160,9 +2202,9 @@ __device__ void deblock_h_seg(PicRef P, int t) {
     }
     if (b == 2 && (y & 15) == 0) {
         for (int c = 1; c < 3; ++c) {
-            int tcc = chroma_tc(P, qpp, qpq, c - 1);
+            int tcc = chroma_tc(P, qpp, qpq, c - 1, sr.to);
             S* pl = (S*)P.cur[c];
-            chroma_lines(pl + (size_t)(y >> 1) * P.pitch[c] + (x >> 1), P.pitch[c], 1, tcc, max);
+            chroma_lines(pl + (size_t)(y >> 1) * P.pitch[c] + (x >> 1), P.pitch[c], 1, tcc, max, sr.np, sr.nq);
         }
     }
 }
@@ -2360,11 +2402,62 @@ __device__ __forceinline__ void sao_sample(PicRef P, int c, int x, int y, const 
     ((S*)P.out[c])[(size_t)y * pitch + x] = (S)r;
 }
 
+// One sample of a CTB with per-CTB rules (sao_ctb, src/hevc/sao.rs:
+// 252-431): exempt samples keep their deblocked value, and an edge
+// neighbour in another CTB counts only if that CTB's bit of nb (l, r, t,
+// b, tl, tr, bl, br) is set.
+template <typename S>
+__device__ void sao_sample_rules(PicRef P, int c, int x, int y, const rv_sao& sp, bool on, uint32_t nb) {
+    const int pw = P.pw[c], ph = P.ph[c], pitch = P.pitch[c], max = (1 << P.bd) - 1, sh = c ? 1 : 0;
+    const S* src = (const S*)P.cur[c];
+    const int vv = src[(size_t)y * pitch + x];
+    int r = vv;
+    const bool exempt = (P.cuinfo[(size_t)((y << sh) >> 2) * P.w4 + ((x << sh) >> 2)] & kCuSaoExempt) != 0;
+    if (on && !exempt && sp.type_idx == 1) {
+        int band = ((vv >> (P.bd - 5)) - sp.band_or_class) & 31;
+        if (band < 4) r = clampi(vv + sp.offsets[band], 0, max);
+    } else if (on && !exempt && sp.type_idx == 2) {
+        const int cls = sp.band_or_class & 3;
+        const int hx = cls == 1 ? 0 : (cls == 3 ? 1 : -1), vy = cls == 0 ? 0 : -1;
+        const int cx0 = (x << sh) >> P.log2_ctb, cy0 = (y << sh) >> P.log2_ctb;
+        auto usable = [&](int xn, int yn) {
+            if (xn < 0 || yn < 0 || xn >= pw || yn >= ph) return false;
+            const int dx = ((xn << sh) >> P.log2_ctb) - cx0, dy = ((yn << sh) >> P.log2_ctb) - cy0;
+            if (dx == 0 && dy == 0) return true;
+            const int k = dy == 0 ? (dx < 0 ? 0 : 1) : dx == 0 ? (dy < 0 ? 2 : 3) : 4 + (dy > 0) * 2 + (dx > 0);
+            return ((nb >> k) & 1) != 0;
+        };
+        const int xa = x + hx, ya = y + vy, xb = x - hx, yb = y - vy;
+        if (usable(xa, ya) && usable(xb, yb)) {
+            const int a = src[(size_t)ya * pitch + xa], bb = src[(size_t)yb * pitch + xb];
+            const int e = sgn(vv - a) + sgn(vv - bb);
+            const int off = e == -2 ? sp.offsets[0] : e == -1 ? sp.offsets[1] : e == 0 ? 0 : e == 1 ? sp.offsets[2] : sp.offsets[3];
+            r = clampi(vv + off, 0, max);
+        }
+    }
+    ((S*)P.out[c])[(size_t)y * pitch + x] = (S)r;
+}
+
 // One CTB per workgroup (256 lanes), all three components: the parameters
 // are uniform across each wave, so the type and class branches are scalar.
+// CTBs whose SAO may not read a neighbour inside the picture, or that hold
+// an exempt block, go sample by sample.
 template <typename S>
 __device__ __forceinline__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
     constexpr int G = 16 / sizeof(S);
+    const uint32_t cw = P.ctbw ? P.ctbw[(size_t)ry * P.wc + rx] : 0u;
+    if (cw & kCtbSaoPerSample) {
+        for (int c = 0; c < 3; ++c) {
+            const int cs = (1 << P.log2_ctb) >> (c ? 1 : 0);
+            const int x0 = rx * cs, y0 = ry * cs;
+            const int w = min(cs, P.pw[c] - x0), h = min(cs, P.ph[c] - y0);
+            const rv_sao sp = P.sao[3 * (ry * P.wc + rx) + c];
+            const bool on = (c == 0 ? P.sao_luma : P.sao_chroma) && sp.type_idx != 0;
+            for (int t = tid; t < w * h; t += 256)
+                sao_sample_rules<S>(P, c, x0 + t % w, y0 + t / w, sp, on, cw & 0xFF);
+        }
+        return;
+    }
     for (int c = 0; c < 3; ++c) {
         const int sh = c ? 1 : 0;
         const int cs = (1 << P.log2_ctb) >> sh;

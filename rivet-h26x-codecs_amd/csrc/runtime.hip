@@ -728,6 +728,7 @@ int rv_abi_size(int which) {
         case 9: return (int)sizeof(rv_nz);
         case 10: return (int)sizeof(rv_hevc_scaling);
         case 11: return (int)sizeof(rv_h264_scaling);
+        case 12: return (int)sizeof(rv_hevc_slice);
         default: return -1;
     }
 }
@@ -1039,6 +1040,84 @@ int rv_picture_hash_noalloc(rv_session* s, int slot, int type, void* out) {
     }
 }
 
+// Per-CTB loop-filter words (rv::kCtb*) of a picture with several slices
+// or tiles, or with CUs exempt from the loop filters: which CTB boundaries
+// deblocking may cross and the slice's offsets (src/hevc/deblock.rs:
+// 105-165, 186-201), which neighbouring CTBs SAO may read (Neighbours::of,
+// src/hevc/sao.rs:193-238) and whether the CTB's SAO takes the per-sample
+// path (a neighbour inside the picture it may not read, or an exempt
+// block, :140-151). Empty: one slice, one tile, no exempt CU.
+static int hevc_ctb_words(const rv_hevc_pic* p, std::vector<uint32_t>& out) {
+    out.clear();
+    bool any_exempt = false;
+    for (int i = 0; i < p->n_cus; ++i) any_exempt |= p->cus[i].exempt != 0;
+    if (!p->ctb_slice && !any_exempt) return RV_OK;
+    const int ctb = 1 << p->log2_ctb;
+    const int wc = (p->width + ctb - 1) / ctb, hc = (p->height + ctb - 1) / ctb;
+    const size_t nc = (size_t)wc * hc;
+    const bool multi = p->ctb_slice != nullptr;
+    if (multi) {
+        if (p->n_slices <= 0 || p->n_slices > 65535 || !p->slices || !p->ctb_slice_addr || !p->ctb_tile || !p->ctb_ts)
+            return fail(RV_E_INVALID, "ctb_slice without slices, ctb_slice_addr, ctb_tile or ctb_ts");
+        for (size_t c = 0; c < nc; ++c)
+            if (p->ctb_slice[c] >= p->n_slices || p->ctb_ts[c] >= nc)
+                return fail(RV_E_RANGE, "ctb_slice or ctb_ts outside the slices / CTBs");
+        for (int k = 0; k < p->n_slices; ++k) {
+            const rv_hevc_slice& sl = p->slices[k];
+            if (sl.beta_offset < -128 || sl.beta_offset > 127 || sl.tc_offset < -128 || sl.tc_offset > 127)
+                return fail(RV_E_RANGE, "slice beta_offset / tc_offset outside -128..127");
+        }
+    } else if (p->beta_offset < -128 || p->beta_offset > 127 || p->tc_offset < -128 || p->tc_offset > 127) {
+        return fail(RV_E_RANGE, "beta_offset / tc_offset outside -128..127");
+    }
+    std::vector<uint8_t> exempt(nc, 0);
+    for (int i = 0; i < p->n_cus; ++i) {
+        const rv_hevc_cu& cu = p->cus[i];
+        if (!(cu.exempt & RV_CU_LF_EXEMPT)) continue;
+        const int sz = 1 << cu.log2_size;
+        for (int cy = cu.y >> p->log2_ctb; cy <= (cu.y + sz - 1) >> p->log2_ctb; ++cy)
+            for (int cx = cu.x >> p->log2_ctb; cx <= (cu.x + sz - 1) >> p->log2_ctb; ++cx)
+                exempt[(size_t)cy * wc + cx] = 1;
+    }
+    auto slice = [&](size_t c) -> const rv_hevc_slice* { return multi ? &p->slices[p->ctb_slice[c]] : nullptr; };
+    // deblock.rs:125-137: the edge between CTB cq (q side) and cp.
+    auto crosses = [&](size_t cq, size_t cp) {
+        if (!multi) return true;
+        if (p->ctb_tile[cq] != p->ctb_tile[cp] && !p->loop_filter_across_tiles) return false;
+        return p->ctb_slice_addr[cq] == p->ctb_slice_addr[cp] || slice(cq)->loop_filter_across_slices != 0;
+    };
+    static const int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    out.assign(nc, 0);
+    for (int ry = 0; ry < hc; ++ry)
+        for (int rx = 0; rx < wc; ++rx) {
+            const size_t c = (size_t)ry * wc + rx;
+            uint32_t nb = 0, inside = 0;
+            for (int k = 0; k < 8; ++k) {
+                const int nx = rx + DX[k], ny = ry + DY[k];
+                if (nx < 0 || ny < 0 || nx >= wc || ny >= hc) continue;
+                inside |= 1u << k;
+                const size_t n = (size_t)ny * wc + nx;
+                bool ok = true;
+                if (multi && p->ctb_slice_addr[n] != p->ctb_slice_addr[c]) {
+                    if (p->ctb_ts[n] < p->ctb_ts[c] && !slice(c)->loop_filter_across_slices) ok = false;
+                    if (p->ctb_ts[c] < p->ctb_ts[n] && !slice(n)->loop_filter_across_slices) ok = false;
+                }
+                if (multi && !p->loop_filter_across_tiles && p->ctb_tile[n] != p->ctb_tile[c]) ok = false;
+                if (ok) nb |= 1u << k;
+            }
+            uint32_t w = nb;
+            if (nb != inside || exempt[c]) w |= rv::kCtbSaoPerSample;
+            if (multi && slice(c)->deblocking_disabled) w |= rv::kCtbDbkOff;
+            if (rx > 0 && !crosses(c, c - 1)) w |= rv::kCtbLeftOff;
+            if (ry > 0 && !crosses(c, c - wc)) w |= rv::kCtbTopOff;
+            const int bo = multi ? slice(c)->beta_offset : p->beta_offset;
+            const int to = multi ? slice(c)->tc_offset : p->tc_offset;
+            w |= (uint32_t)(uint8_t)(int8_t)bo << 16 | (uint32_t)(uint8_t)(int8_t)to << 24;
+            out[c] = w;
+        }
+    return RV_OK;
+}
+
 int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** out) {
     try {
         if (!s || !p || !out) return fail(RV_E_INVALID, "null argument");
@@ -1048,6 +1127,9 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         if (out_slot < 0 || out_slot >= s->slots) return fail(RV_E_RANGE, "out_slot outside the DPB");
         std::vector<int32_t> cu_at;
         int rc = validate_hevc(s, p, cu_at);
+        if (rc) return rc;
+        std::vector<uint32_t> ctbw;
+        rc = hevc_ctb_words(p, ctbw);
         if (rc) return rc;
         const int W = p->width, H = p->height;
         // Device intra: the TUs of RV_INTRA_DEVICE CUs leave the stage-1
@@ -1251,7 +1333,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
                     carve_size<int32_t>(n_ilevels + 1) + carve_size<int32_t>(2 * (size_t)(n_ilevels + 1)) +
                     carve_size<uint16_t>(p->n_intra_pred) + carve_size<rv_sao>(n_sao) +
                     carve_size<int32_t>(n_intra) + carve_size<rv_hevc_scaling>(p->scaling ? 1 : 0) +
-                    carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
+                    carve_size<uint32_t>(ctbw.size()) + carve_size<const rv::HevcPicDev*>(1) + carve_size<int32_t>(rv::kStages * 2) + 256;
         size_t n4 = (size_t)(W / 4) * (H / 4);
         size_t scr = s->g.bytes[0] + s->g.bytes[1] + s->g.bytes[2] + carve_size<uint8_t>(n4) * 3 +
                      carve_size<int8_t>(n4) + carve_size<uint32_t>(n4) + carve_size<int16_t>(n_ires) +
@@ -1279,6 +1361,8 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         rv::SideClass* h_sclass = carve<rv::SideClass>(hp, n_sclass);
         if (n_sclass) std::memcpy(h_sclass, sclass.data(), n_sclass * sizeof(rv::SideClass));
         uint16_t* h_sidx = carve<uint16_t>(hp, n_side);
+        uint32_t* h_ctbw = carve<uint32_t>(hp, ctbw.size());
+        if (!ctbw.empty()) std::memcpy(h_ctbw, ctbw.data(), ctbw.size() * sizeof(uint32_t));
         for (int k = 0; k < n_sclass; ++k) {
             const int end = k + 1 < n_sclass ? sclass[(size_t)k + 1].item0 : n_side;
             for (int i = sclass[(size_t)k].item0; i < end; ++i) h_sidx[i] = (uint16_t)k;
@@ -1472,6 +1556,7 @@ int rv_hevc_prepare(rv_session* s, const rv_hevc_pic* p, int out_slot, rv_job** 
         d.n_sclass = n_sclass;
         d.sclass = (const rv::SideClass*)dev((char*)h_sclass);
         d.sidx = (const uint16_t*)dev((char*)h_sidx);
+        d.ctbw = ctbw.empty() ? nullptr : (const uint32_t*)dev((char*)h_ctbw);
         d.n_intra = n_intra;
         d.intra_idx = (const int32_t*)dev((char*)h_intra);
         d.n_txw = (int32_t)n_txw;

@@ -79,6 +79,15 @@ constexpr int kSideCellsPerItem = 256;
 // shapes (w, h in 4..64), luma TU sizes 32..4 (the empty ones left out).
 
 // Per-picture descriptor of an HEVC job, resident in device memory.
+// Flags of the per-CTB loop-filter words (HevcPicDev::ctbw).
+constexpr uint32_t kCtbDbkOff = 1u << 8;         // the CTB's slice has deblocking disabled
+constexpr uint32_t kCtbLeftOff = 1u << 9;        // no filtering across the left CTB boundary
+constexpr uint32_t kCtbTopOff = 1u << 10;        // nor across the top one
+constexpr uint32_t kCtbSaoPerSample = 1u << 11;  // SAO per sample: unusable neighbour or exempt block
+// Exemption bits of HevcPicDev::cuinfo (set only with ctbw).
+constexpr uint8_t kCuSaoExempt = 8;   // filter_exempt bit 0: SAO and deblocking leave it alone
+constexpr uint8_t kCuDbkExempt = 16;  // filter_exempt & 5: deblocking leaves it alone
+
 struct HevcPicDev {
     int32_t w, h, bd, log2_ctb, wc, hc, w4, h4;
     int32_t pitch[3];  // samples
@@ -101,6 +110,13 @@ struct HevcPicDev {
     int32_t n_sclass;
     const SideClass* sclass;
     const uint16_t* sidx;
+    // Per-CTB loop-filter words (raster CTBs) of pictures with several
+    // slices or tiles or with exempt CUs, null otherwise: bits 0..7 the
+    // neighbouring CTBs SAO may read (l, r, t, b, tl, tr, bl, br), then
+    // the kCtb* flags, the slice's beta_offset (bits 16..23) and tc_offset
+    // (bits 24..31) as signed bytes. With them, cuinfo carries the
+    // exemption bits kCuSaoExempt / kCuDbkExempt.
+    const uint32_t* ctbw;
     const int32_t* intra_idx; // indices of the intra CUs in cus
     // Stage-1 transform items: item i takes ctus[first .. first + count)
     // of one size class cls (4, 8, 16, 32; 64 / N TUs per item), packed as

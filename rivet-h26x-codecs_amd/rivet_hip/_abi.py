@@ -29,6 +29,9 @@ RV_TU_INTRA = 8
 RV_BLK_LEVELS = 1
 RV_INTRA_HOST = 1
 RV_INTRA_DEVICE = 2
+RV_CU_LF_EXEMPT = 1
+RV_CU_BYPASS = 2
+RV_CU_DBK_EXEMPT = 4
 RV_CODEC_HEVC = 0
 RV_CODEC_H264 = 1
 RV_H264_DEFAULT = 0
@@ -58,7 +61,7 @@ class Sao(C.Structure):
 
 class HevcCu(C.Structure):
     _fields_ = [("x", u16), ("y", u16), ("log2_size", u8), ("qp_y", i8), ("intra", u8), ("part_mode", u8),
-                ("pred_off", u32), ("luma_mode", u8 * 4), ("chroma_mode", u8), ("pad", u8 * 3)]
+                ("pred_off", u32), ("luma_mode", u8 * 4), ("chroma_mode", u8), ("exempt", u8), ("pad", u8 * 2)]
 
 
 class HevcPu(C.Structure):
@@ -79,6 +82,11 @@ class HevcScaling(C.Structure):
     _fields_ = [("list4", (u8 * 16) * 6), ("list8", ((u8 * 64) * 6) * 3), ("dc", (u8 * 6) * 2)]
 
 
+class HevcSlice(C.Structure):
+    _fields_ = [("deblocking_disabled", i32), ("beta_offset", i32), ("tc_offset", i32),
+                ("loop_filter_across_slices", i32)]
+
+
 class HevcPic(C.Structure):
     _fields_ = [
         ("width", i32), ("height", i32), ("bit_depth", i32), ("log2_ctb", i32), ("poc", i32),
@@ -92,6 +100,9 @@ class HevcPic(C.Structure):
         ("nz", C.POINTER(Nz)), ("n_nz", u32), ("n_intra_pred", u32),
         ("intra_pred", C.POINTER(u16)), ("sao", C.POINTER(Sao)), ("scaling", C.POINTER(HevcScaling)),
         ("strong_intra_smoothing", i32), ("constrained_intra_pred", i32),
+        ("n_slices", i32), ("slices", C.POINTER(HevcSlice)), ("ctb_slice", C.POINTER(u16)),
+        ("ctb_slice_addr", C.POINTER(u32)), ("ctb_tile", C.POINTER(u16)), ("ctb_ts", C.POINTER(u32)),
+        ("loop_filter_across_tiles", i32),
     ]
 
 
@@ -135,7 +146,9 @@ class GenHevcCfg(C.Structure):
                                                                 ("levels", i32), ("scaling", i32),
                                                                 ("intra_device", i32),
                                                                 ("strong_intra_smoothing", i32),
-                                                                ("constrained_intra_pred", i32)]
+                                                                ("constrained_intra_pred", i32)] + [
+        (n, i32) for n in ("tile_cols", "tile_rows", "n_slices", "lf_across_slices", "lf_across_tiles",
+                           "slice_vary", "exempt_percent")]
 
 
 class GenH264Cfg(C.Structure):
@@ -146,7 +159,7 @@ class GenH264Cfg(C.Structure):
 
 
 ABI_STRUCTS = [HevcCu, HevcPu, HevcTu, Sao, HevcPic, H264Mb, H264Part, H264Blk, H264Pic, Nz, HevcScaling,
-               H264Scaling]
+               H264Scaling, HevcSlice]
 
 # Every extern "C" entry of include/rivet_hip.h (librivet_hip.so):
 # name -> (restype, argtypes).

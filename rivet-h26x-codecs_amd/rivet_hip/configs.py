@@ -16,6 +16,12 @@ BASE_SEED = 0x5256_4854_3236_3478  # "RVHT26x4"
 HEVC_C1 = dict(width=416, height=240, bit_depth=8, b_picture=0, bi_percent=0, explicit_wp=0, intra_percent=0,
                nz_permille=100, lap_b=8, qp_min=22, qp_max=37, mv_range=256, deblock=1, sao=1,
                poc=1, ref_poc=[0, 0], ref_slot=[0, 0])
+# Slices and tiles with loop-filter rules seeded per slice and exempt CUs:
+# a 416x240 Main10 B-picture (7 x 4 CTBs of 64x64) in 3 x 2 tiles and 5 slices.
+SLICED = dict(width=416, height=240, bit_depth=10, b_picture=1, bi_percent=50, explicit_wp=0, intra_percent=30,
+              nz_permille=150, lap_b=16, qp_min=22, qp_max=40, mv_range=128, deblock=1, sao=1, poc=2,
+              ref_poc=[0, 4], ref_slot=[0, 1], tile_cols=3, tile_rows=2, n_slices=5, lf_across_slices=-1,
+              lf_across_tiles=0, slice_vary=1, exempt_percent=15)
 # 2: HEVC Main10 1920x1080 B-picture, explicit WP (denominators 6), intra
 # 8x8 CUs with DST 4x4 TUs, ~15 % nonzero coefficients at Laplacian b=16.
 HEVC_C2 = dict(width=1920, height=1080, bit_depth=10, b_picture=1, bi_percent=60, explicit_wp=1, intra_percent=50,

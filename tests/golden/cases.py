@@ -47,6 +47,7 @@ CASES = {
     "hevc_c2": cf.single(2),
     **{f"hevc_c3_stream{k}": (rh.RV_CODEC_HEVC, *cf.c3_stream(k)) for k in range(2)},
     "hevc_b_wp_intra_8bit_416x240": (rh.RV_CODEC_HEVC, dict(cf.HEVC_C2, width=416, height=240, bit_depth=8), 12345),
+    "hevc_slices_tiles_exempt_10bit_416x240": (rh.RV_CODEC_HEVC, cf.SLICED, 0x5117),
     "h264_c4": cf.single(4),
     "h264_p_352x288": (rh.RV_CODEC_H264, _h264_small("p"), 0x264),
     "h264_b_mixed_352x288": (rh.RV_CODEC_H264, _h264_small("b_mixed"), 0x265),
@@ -74,9 +75,12 @@ def worklist_digest(codec: int, pic) -> str:
     if codec == rh.RV_CODEC_HEVC:
         ctb = 1 << c.log2_ctb
         n_sao = ((c.width + ctb - 1) // ctb) * ((c.height + ctb - 1) // ctb) * 3
+        n_ctb = n_sao // 3 if c.ctb_slice else 0
         parts = [(c.cus, c.n_cus, _abi.HevcCu), (c.pus, c.n_pus, _abi.HevcPu), (c.tus, c.n_tus, _abi.HevcTu),
                  (c.nz, c.n_nz, _abi.Nz), (c.intra_pred, c.n_intra_pred, C.c_uint16),
-                 (c.sao, n_sao if c.sao else 0, _abi.Sao)]
+                 (c.sao, n_sao if c.sao else 0, _abi.Sao), (c.slices, c.n_slices if n_ctb else 0, _abi.HevcSlice),
+                 (c.ctb_slice, n_ctb, C.c_uint16), (c.ctb_slice_addr, n_ctb, C.c_uint32),
+                 (c.ctb_tile, n_ctb, C.c_uint16), (c.ctb_ts, n_ctb, C.c_uint32)]
     else:
         parts = [(c.mbs, c.n_mbs, _abi.H264Mb), (c.parts, c.n_parts, _abi.H264Part), (c.blks, c.n_blks, _abi.H264Blk),
                  (c.coefs, c.n_coefs, C.c_int32), (c.intra_pred, c.n_intra_pred, C.c_uint16)]
