@@ -42,8 +42,16 @@ LEVELS_NOTE = "; coefficients as levels with scaling lists, scaled on the device
 INTRA_NOTE = "; intra blocks predicted on the device"
 
 
+# Slices and tiles that refuse every crossing, seeded per-slice deblocking
+# switches and offsets, and a few exempt CUs.
+SLICED = dict(n_slices=16, tile_cols=4, tile_rows=3, lf_across_slices=0, lf_across_tiles=0, slice_vary=1,
+              exempt_percent=2)
+SLICED_NOTE = "; 16 slices in 4 x 3 tiles per picture, no loop filtering across them, 2% exempt CUs"
+
+
 def workload_note(extra_cfg: dict) -> str:
-    return LEVELS_NOTE * bool(extra_cfg.get("levels")) + INTRA_NOTE * bool(extra_cfg.get("intra_device"))
+    return (LEVELS_NOTE * bool(extra_cfg.get("levels")) + INTRA_NOTE * bool(extra_cfg.get("intra_device")) +
+            SLICED_NOTE * bool(extra_cfg.get("n_slices")))
 
 
 def log(*a):
@@ -159,6 +167,8 @@ def measure_config(rh, cf, oracle, config: int, n: int, steps: int, warmup: int,
     of the configuration (the first is the configuration's own seed), one
     graph launch per step, every plane checked against the oracle."""
     codec, cfg, seed0 = cf.single(config)
+    if codec != rh.RV_CODEC_HEVC:
+        extra_cfg = {k: v for k, v in extra_cfg.items() if k not in SLICED}  # HEVC slices and tiles only
     cfg = dict(cfg, **extra_cfg)
     w, h, bd = cfg["width"], cfg["height"], cfg.get("bit_depth", 8)
     pics, refs, sessions, jobs = [], [], [], []
@@ -247,6 +257,9 @@ def main() -> int:
                     help="predict the intra blocks of configurations 2 and 4 on the device (RV_INTRA_DEVICE: HEVC "
                          "intra CUs of every size, H.264 intra macroblocks of every kind) instead of taking "
                          "host-supplied prediction")
+    ap.add_argument("--sliced", action="store_true",
+                    help="cut every picture into 16 slices in 4 x 3 tiles with no loop filtering across them, "
+                         "seeded per-slice deblocking switches and offsets and 2%% exempt CUs (HEVC pictures)")
     ap.add_argument("--graph-only", action="store_true",
                     help="time only the graph replay (for kernel profiling: every stage launch is a batched one)")
     args = ap.parse_args()
@@ -267,6 +280,8 @@ def main() -> int:
     extra_cfg = dict(levels=1, scaling=1) if args.levels else {}
     if args.device_intra:
         extra_cfg["intra_device"] = 1
+    if args.sliced:
+        extra_cfg.update(SLICED)
     if args.only_config:
         r = measure_config(rh, cf, oracle, args.only_config, 8, args.steps, args.warmup, extra_cfg)
         log(f"[bench] config {args.only_config}: {r['workload']}: bit-exact {r['bit_exact']}, {r['ms_per_step']} ms")

@@ -2440,22 +2440,57 @@ __device__ void sao_sample_rules(PicRef P, int c, int x, int y, const rv_sao& sp
 
 // One CTB per workgroup (256 lanes), all three components: the parameters
 // are uniform across each wave, so the type and class branches are scalar.
-// CTBs whose SAO may not read a neighbour inside the picture, or that hold
-// an exempt block, go sample by sample.
+// In CTBs whose SAO may not read a neighbour inside the picture, or that
+// hold an exempt block, the top and bottom rows go sample by sample (one
+// lane each), as do vector groups with an exempt block; the other groups
+// take the vector path and then redo their first or last column when it is
+// the CTB's (the other samples with a neighbour in another CTB) by the
+// rules.
+template <typename S>
+__device__ __forceinline__ void sao_ctb_rules(PicRef P, int rx, int ry, int tid, uint32_t nb) {
+    constexpr int G = 16 / sizeof(S);
+    for (int c = 0; c < 3; ++c) {
+        const int sh = c ? 1 : 0;
+        const int cs = (1 << P.log2_ctb) >> sh;
+        const int x0 = rx * cs, y0 = ry * cs;
+        const int w = min(cs, P.pw[c] - x0), h = min(cs, P.ph[c] - y0);
+        const rv_sao sp = P.sao[3 * (ry * P.wc + rx) + c];
+        const bool on = (c == 0 ? P.sao_luma : P.sao_chroma) && sp.type_idx != 0;
+        if (cs % G != 0) {
+            for (int t = tid; t < w * h; t += 256) sao_sample_rules<S>(P, c, x0 + t % w, y0 + t / w, sp, on, nb);
+            continue;
+        }
+        // The top and bottom rows: one lane per sample.
+        const int ring = h > 1 ? 2 * w : w;
+        for (int t = tid; t < ring; t += 256)
+            sao_sample_rules<S>(P, c, x0 + t % w, t < w ? y0 : y0 + h - 1, sp, on, nb);
+        const int gpr = cs / G;
+        for (int t = tid; t < cs * gpr; t += 256) {
+            const int yy = t / gpr, gx = t - yy * gpr;
+            if (yy == 0 || yy >= h - 1 || gx * G >= w) continue;
+            const int x = x0 + gx * G, y = y0 + yy, n = min(G, w - gx * G);
+            // The 4x4 luma blocks under the group.
+            const uint8_t* ci = P.cuinfo + (size_t)((y << sh) >> 2) * P.w4 + ((x << sh) >> 2);
+            bool ex = false;
+            for (int k = 0; k < ((n << sh) + 3) >> 2; ++k) ex |= (ci[k] & kCuSaoExempt) != 0;
+            if (ex) {
+                for (int k = 0; k < n; ++k) sao_sample_rules<S>(P, c, x + k, y, sp, on, nb);
+                continue;
+            }
+            sao_group<S>(P, c, x, y, sp, on);
+            if (!on || sp.type_idx != 2) continue;
+            if (gx == 0) sao_sample_rules<S>(P, c, x, y, sp, on, nb);
+            if (gx * G + n == w) sao_sample_rules<S>(P, c, x + n - 1, y, sp, on, nb);
+        }
+    }
+}
+
 template <typename S>
 __device__ __forceinline__ void sao_ctb(PicRef P, int rx, int ry, int tid) {
     constexpr int G = 16 / sizeof(S);
     const uint32_t cw = P.ctbw ? P.ctbw[(size_t)ry * P.wc + rx] : 0u;
     if (cw & kCtbSaoPerSample) {
-        for (int c = 0; c < 3; ++c) {
-            const int cs = (1 << P.log2_ctb) >> (c ? 1 : 0);
-            const int x0 = rx * cs, y0 = ry * cs;
-            const int w = min(cs, P.pw[c] - x0), h = min(cs, P.ph[c] - y0);
-            const rv_sao sp = P.sao[3 * (ry * P.wc + rx) + c];
-            const bool on = (c == 0 ? P.sao_luma : P.sao_chroma) && sp.type_idx != 0;
-            for (int t = tid; t < w * h; t += 256)
-                sao_sample_rules<S>(P, c, x0 + t % w, y0 + t / w, sp, on, cw & 0xFF);
-        }
+        sao_ctb_rules<S>(P, rx, ry, tid, cw & 0xFF);
         return;
     }
     for (int c = 0; c < 3; ++c) {
